@@ -1,0 +1,149 @@
+"""Benchmark of the MI355X lane engine on its sweep workloads.
+
+One step = one full lane solve of the workload on the GPU: reset every
+lane's solver, upload the start points, run the static Newton kernel to
+convergence, read back the per-lane outcomes. Prints one JSON line.
+
+Workload (the configuration this engine runs end to end today): the
+64-point DC sweep of a 5-stage mos1 CMOS inverter chain, each sweep point an
+independent cold operating point on its own lane.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged sweep lanes/sec at equal Newton tol: learned+batched vs plain GPU lanes"
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+
+# Static FP64 operation counts of one evaluation per device kind (value +
+# 4-wide forward-mode derivatives, the limiter, the stamps), counted from
+# egspice_amd/csrc/devices.h and engine.h; transcendentals count 20 each.
+EVAL_FLOPS = {0: 8, 1: 2, 2: 10, 3: 0, 4: 90, 5: 330}
+LIMIT_FLOPS = {4: 12, 5: 40}
+
+
+def lu_flops(col_ptr, row_idx, vals, n):
+    """Refactor + solve flops of one lane tick on the lead factor's
+    structure: 2 sum over U entries of |L column| + |L| divides (the
+    tape's count, sparse_lu.zig:1186), solve 2(|L| + |U|) + n."""
+    import egspice_amd as E
+    tab, idx = E.tab_build(n, col_ptr, row_idx, vals)
+    lp = idx[tab.lp:tab.lp + n + 1]
+    ui = idx[tab.ui:tab.ui + tab.nnz_u]
+    refactor = 2 * int(sum(int(lp[i + 1] - lp[i]) for i in ui)) + int(tab.nnz_l)
+    solve = 2 * (int(tab.nnz_l) + int(tab.nnz_u)) + n
+    return refactor + solve, tab
+
+
+def cpu_baseline(n_lanes, budget_s=10.0):
+    """The oracle's CPU path (converger.newton with direct.Solver per lane),
+    one thread, on the same workload, repeated for about budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from egspice_amd import workloads as W
+    lib = O.load()
+    c = O.Circuit(lib)
+    W.inverter_chain(c.add)
+    c.freeze()
+    vin = W.dc_sweep_points(n_lanes)
+    L = O.Lanes(c, n_lanes)
+    L.set_param(O.VSRC, 1, O.VSRC_DC, vin)
+    L.prepare()
+    reps, t0 = 0, time.perf_counter()
+    converged = 0
+    while True:
+        _, st, _ = L.newton_op(mode=1)
+        converged += int((st & 1).sum())
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": converged / el, "unit": "lanes/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} repetitions of the {n_lanes}-lane sweep ({reps * n_lanes} lane solves, {el:.1f} s), "
+                      f"oracle CPU path, 1 thread, {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--lanes", type=int, default=64)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(lanes shard across GPUs by contiguous lane ranges, a design not built here)", file=sys.stderr)
+        sys.exit(2)
+
+    import egspice_amd as E
+    from egspice_amd import workloads as W
+
+    t = E.Template()
+    W.inverter_chain(t.add)
+    t.freeze()
+    N = args.lanes
+    vin = W.dc_sweep_points(N)
+    lanes = E.Lanes(t, N)
+    lanes.set_param(E.VSOURCE, 1, E.VSOURCE_DC, vin)
+    lanes.prepare()
+    x0 = np.zeros(t.n * N)
+
+    def step():
+        lanes.reset()
+        return lanes.newton_op(x0)
+
+    for _ in range(args.warmup):
+        step()
+    kernel_ms, iters_all, conv_all = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, st, it, ms = step()
+        kernel_ms.append(ms)
+        iters_all.append(it.copy())
+        conv_all.append(int((st & 1).sum()))
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    converged = conv_all[-1]
+    value = sum(conv_all) / wall
+
+    # roofline of the Newton kernel: algorithmic FP64 flops / kernel time
+    g_eval = sum(EVAL_FLOPS[k] * t_count for k, t_count in [(0, 2), (5, 2 * 5)])
+    g_lim = LIMIT_FLOPS[5] * 10
+    # the lead structure at lane 0's seeded first matrix is the typical lane's
+    vals = np.ones(t.nnz)
+    lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
+    per_tick = g_eval + g_lim + lu + 12 * t.n
+    ticks = float(np.mean([it.sum() for it in iters_all]))
+    flops = per_tick * ticks
+    kms = float(np.mean(kernel_ms))
+    achieved = flops / (kms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "lanes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "mos1 5-stage inverter chain, 64-point DC sweep as cold operating points",
+                   "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz, "arm": "plain FP64 GPU lanes",
+                   "newton": "reltol 1e-3 abstol 1e-12 vntol 1e-6 residual_tol 1e-9 itl1 100",
+                   "converged_lanes": converged, "newton_ticks_per_solve": ticks,
+                   "kernel_ms_per_solve": kms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "newton_op_kernel", "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(N, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+"""MI355X lane engine for ESPice sweeps: Python binding of the C ABI.
+
+The engine is the native library ``libespice_lanes.so`` (HIP kernels for
+gfx950 plus a C++ host), declared in ``include/espice_lanes.h``. This module
+only marshals arguments through ctypes; every solve runs on the GPU, and
+importing fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libespice_lanes.so")
+
+VSOURCE, ISOURCE, RESISTOR, CAPACITOR, DIODE, MOS1 = range(6)
+# user parameter counts per kind (include/espice_lanes.h kinds)
+PARAM_COUNT = {VSOURCE: 9, ISOURCE: 1, RESISTOR: 1, CAPACITOR: 1, DIODE: 14, MOS1: 36}
+VSOURCE_DC = 0
+DIODE_PARAMS = dict(IS=0, N=1, CJO=2, VJ=3, M=4, FC=5, TT=6, GMIN=7, AREA=8, TNOM=9, TEMP=10, DTEMP=11, EG=12, XTI=13)
+MOS1_PARAMS = {name: i for i, name in enumerate(
+    "TYPE VTO KP GAMMA PHI LAMBDA TOX LD U0 NSUB NSS TPG IS JS CBD CBS CJ CJSW PB MJ MJSW FC CGSO CGDO CGBO "
+    "TNOM GMIN TEMP DTEMP L W AD AS PD PS M".split())}
+
+OK = 0
+EXPORTS = (
+    "espice_lanes_error_message", "espice_lanes_template_create", "espice_lanes_template_destroy",
+    "espice_lanes_template_node", "espice_lanes_template_add", "espice_lanes_template_freeze",
+    "espice_lanes_template_info", "espice_lanes_template_pattern", "espice_lanes_create", "espice_lanes_destroy",
+    "espice_lanes_reset", "espice_lanes_set_param", "espice_lanes_prepare", "espice_lanes_upload_x", "espice_lanes_download_x",
+    "espice_lanes_newton_op", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
+)
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double),
+                ("residual_tol", C.c_double), ("gmin", C.c_double),
+                ("max_iter", C.c_uint32), ("init_fix", C.c_uint32)]
+
+
+class LaneResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("iterations", C.c_uint32)]
+
+
+class LaneTab(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("n", "nnz", "nnz_l", "nnz_u")] + \
+               [("growth", C.c_double), ("wide", C.c_double)] + \
+               [(f, C.c_uint32) for f in ("col_ptr", "prow", "q", "pinv", "lp", "li", "up", "ui", "flags",
+                                          "void_slots", "n_void_slots", "pad")]
+
+
+_lib = None
+
+
+def library():
+    """The loaded engine; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineError(f"lane engine not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P, u32p, f64p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)
+    sig = {
+        "espice_lanes_error_message": [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
+        "espice_lanes_template_create": [C.POINTER(P)],
+        "espice_lanes_template_destroy": [P],
+        "espice_lanes_template_node": [P, C.c_char_p, u32p],
+        "espice_lanes_template_add": [P, C.c_uint32, u32p, f64p, C.c_size_t, u32p],
+        "espice_lanes_template_freeze": [P],
+        "espice_lanes_template_info": [P, u32p, u32p, u32p],
+        "espice_lanes_template_pattern": [P, u32p, u32p, u32p, C.POINTER(C.c_uint8), u32p],
+        "espice_lanes_create": [P, C.c_uint32, C.POINTER(P)],
+        "espice_lanes_destroy": [P],
+        "espice_lanes_reset": [P],
+        "espice_lanes_set_param": [P, C.c_uint32, C.c_uint32, C.c_uint32, f64p],
+        "espice_lanes_prepare": [P],
+        "espice_lanes_upload_x": [P, f64p],
+        "espice_lanes_download_x": [P, f64p],
+        "espice_lanes_newton_op": [P, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p],
+        "espice_lanes_tab_build": [C.c_uint32, u32p, u32p, f64p, C.POINTER(LaneTab), u32p, C.c_size_t,
+                                   C.POINTER(C.c_size_t)],
+        "espice_lanes_tab_refactor_solve": [C.POINTER(LaneTab), u32p, C.c_size_t, C.c_uint32, f64p, f64p, f64p,
+                                            C.POINTER(C.c_uint8), f64p],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.restype = C.c_uint32
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != OK:
+        buf = C.create_string_buffer(256)
+        req = C.c_size_t()
+        library().espice_lanes_error_message(buf, 256, C.byref(req))
+        raise EngineError(f"espice status {status}: {buf.value.decode()}")
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e-6, residual_tol=1e-9, gmin=0.0):
+    """converger.Options defaults (src/core/numerics.zig:378-399)."""
+    return NewtonOpts(reltol, abstol, vntol, residual_tol, gmin, max_iter, 1 if init_fix else 0)
+
+
+class Template:
+    """A circuit frozen into one MNA pattern, shared by every lane."""
+
+    def __init__(self):
+        self.lib = library()
+        self.h = C.c_void_p()
+        _check(self.lib.espice_lanes_template_create(C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.espice_lanes_template_destroy(self.h)
+            self.h = None
+
+    def node(self, name):
+        row = C.c_uint32()
+        _check(self.lib.espice_lanes_template_node(self.h, name.encode(), C.byref(row)))
+        return row.value
+
+    def add(self, kind, ports, params=None):
+        rows = [self.node(p) if isinstance(p, str) else p for p in ports]
+        pr = np.full(PARAM_COUNT[kind], np.nan)
+        for k, v in (params or {}).items():
+            pr[k] = v
+        _, rp = _u32(rows)
+        a, ap = _f64(pr)
+        inst = C.c_uint32()
+        _check(self.lib.espice_lanes_template_add(self.h, kind, rp, ap, len(pr), C.byref(inst)))
+        return inst.value
+
+    def freeze(self):
+        _check(self.lib.espice_lanes_template_freeze(self.h))
+        n, nnz, tri = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(self.lib.espice_lanes_template_info(self.h, C.byref(n), C.byref(nnz), C.byref(tri)))
+        self.n, self.nnz, self.tridiagonal = n.value, nnz.value, bool(tri.value)
+        cp, cpp = _u32(np.zeros(self.n + 1))
+        ri, rip = _u32(np.zeros(self.nnz))
+        dg, dgp = _u32(np.zeros(self.n))
+        cr = np.zeros(self.n, dtype=np.uint8)
+        q, qp = _u32(np.zeros(self.n))
+        _check(self.lib.espice_lanes_template_pattern(self.h, cpp, rip, dgp, cr.ctypes.data_as(C.POINTER(C.c_uint8)), qp))
+        self.col_ptr, self.row_idx, self.diag, self.cur_row, self.q = cp, ri, dg, cr, q
+        return self
+
+
+class Lanes:
+    """N lanes of one template, resident on the GPU."""
+
+    def __init__(self, template, n_lanes):
+        self.t = template
+        self.lib = template.lib
+        self.N = n_lanes
+        self.h = C.c_void_p()
+        _check(self.lib.espice_lanes_create(template.h, n_lanes, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.espice_lanes_destroy(self.h)
+            self.h = None
+
+    def set_param(self, kind, inst, k, values):
+        v, vp = _f64(np.broadcast_to(np.asarray(values, dtype=np.float64), (self.N,)))
+        _check(self.lib.espice_lanes_set_param(self.h, kind, inst, k, vp))
+
+    def prepare(self):
+        _check(self.lib.espice_lanes_prepare(self.h))
+
+    def reset(self):
+        _check(self.lib.espice_lanes_reset(self.h))
+
+    def upload_x(self, x):
+        a, ap = _f64(np.asarray(x, dtype=np.float64).reshape(-1))
+        assert a.size == self.t.n * self.N
+        _check(self.lib.espice_lanes_upload_x(self.h, ap))
+
+    def download_x(self):
+        a, ap = _f64(np.zeros(self.t.n * self.N))
+        _check(self.lib.espice_lanes_download_x(self.h, ap))
+        return a.reshape(self.t.n, self.N)
+
+    def newton_op(self, x0=None, **kw):
+        """Static Newton on every lane; returns (x, status, iterations, kernel_ms)."""
+        self.upload_x(np.zeros(self.t.n * self.N) if x0 is None else x0)
+        o = newton_opts(**kw)
+        out = (LaneResult * self.N)()
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_newton_op(self.h, C.byref(o), out, C.byref(ms)))
+        status = np.array([out[i].status for i in range(self.N)], dtype=np.uint32)
+        iters = np.array([out[i].iterations for i in range(self.N)], dtype=np.uint32)
+        return self.download_x(), status, iters, ms.value
+
+
+def tab_build(n, col_ptr, row_idx, vals):
+    """LaneTab and index words of one host factor (ownTape/buildTab)."""
+    lib = library()
+    cp, cpp = _u32(col_ptr)
+    ri, rip = _u32(row_idx)
+    v, vp = _f64(vals)
+    tab = LaneTab()
+    ln = C.c_size_t()
+    lib.espice_lanes_tab_build(n, cpp, rip, vp, C.byref(tab), None, 0, C.byref(ln))
+    idx, ip = _u32(np.zeros(ln.value))
+    _check(lib.espice_lanes_tab_build(n, cpp, rip, vp, C.byref(tab), ip, ln.value, C.byref(ln)))
+    return tab, idx
+
+
+def tab_refactor_solve(tab, idx, a, rhs):
+    """Lead-tape replay on the GPU: a (nnz, N) and rhs (n, N) lane-major."""
+    lib = library()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    N = a.shape[1]
+    ii, ip = _u32(idx)
+    aa, ap = _f64(a)
+    rr, rp = _f64(rhs)
+    dx, dxp = _f64(np.zeros_like(rhs))
+    fail = np.zeros(N, dtype=np.uint8)
+    ms = C.c_double()
+    _check(lib.espice_lanes_tab_refactor_solve(C.byref(tab), ip, len(ii), N, ap, rp, dxp,
+                                               fail.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(ms)))
+    return dx.reshape(rhs.shape), fail, ms.value

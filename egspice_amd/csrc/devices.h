@@ -1,0 +1,563 @@
+// Device models of the lane engine: parameter tables, temperature and
+// geometry preparation, static evaluation with forward-mode AD, and SPICE
+// limiting. One body serves the host tools and the gfx950 kernels.
+//
+// The reference compiles each model from Verilog-A with VerA (absent from
+// its tree); these bodies restate the .va sources by hand:
+//   models/vsource.va  (DC; branch unknown p, n, br)      vsource.va:230
+//   models/isource.va  (DC)
+//   models/resistor.va, models/capacitor.va
+//   models/diode.va    (rs = 0, the ai node collapsed)     diode.va:190-236
+//   models/mos1.va     (rd = rs = 0, di/si collapsed)      mos1.va:151-449
+// Limiting is ngspice's DEVfetlim / DEVlimvds / DEVpnjlim (devsup.c) in
+// node space: the limited image of an instance's unknowns, the residual
+// corrected by J (x - x_lim) (src/device/eval.zig:825-872) and snapped
+// back within 16 ulps (eval.zig:1111-1125).
+#pragma once
+
+#include "dual.h"
+
+namespace esp {
+
+enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_NKIND = 6 };
+
+// User parameter slots (NaN = not given).
+enum { VSRC_DC = 0, VSRC_K = 9 };
+enum { ISRC_DC = 0, ISRC_K = 1 };
+enum { RES_R = 0, RES_K = 1 };
+enum { CAP_C = 0, CAP_K = 1 };
+enum { DIO_IS = 0, DIO_N, DIO_CJO, DIO_VJ, DIO_M, DIO_FC, DIO_TT, DIO_GMIN, DIO_AREA, DIO_TNOM, DIO_TEMP, DIO_DTEMP,
+       DIO_EG, DIO_XTI, DIO_K };
+enum { M1_TYPE = 0, M1_VTO, M1_KP, M1_GAMMA, M1_PHI, M1_LAMBDA, M1_TOX, M1_LD, M1_U0, M1_NSUB, M1_NSS, M1_TPG, M1_IS,
+       M1_JS, M1_CBD, M1_CBS, M1_CJ, M1_CJSW, M1_PB, M1_MJ, M1_MJSW, M1_FC, M1_CGSO, M1_CGDO, M1_CGBO, M1_TNOM, M1_GMIN,
+       M1_TEMP, M1_DTEMP, M1_L, M1_W, M1_AD, M1_AS, M1_PD, M1_PS, M1_M, M1_K };
+
+// Prepared constants per instance and lane.
+enum { M1P_TYPE = 0, M1P_VT, M1P_VBIT, M1P_PHIT, M1P_GAMMA, M1P_BETA, M1P_LAMBDA, M1P_ISATS, M1P_ISATD, M1P_GMIN,
+       M1P_VCRITS, M1P_VCRITD, M1P_COX, M1P_CZBD, M1P_CZBS, M1P_CZBDSW, M1P_CZBSSW, M1P_PBT, M1P_MJ, M1P_MJSW, M1P_FC,
+       M1P_F2D, M1P_F3D, M1P_F4D, M1P_F2S, M1P_F3S, M1P_F4S, M1P_CGSO, M1P_CGDO, M1P_CGBO, M1P_SEED, M1P_K };
+enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT, DIOP_FC, DIOP_TT, DIOP_K };
+
+// Local unknowns, ports, user and prepared parameter counts per kind.
+constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 2, 4};
+constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
+constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
+constexpr int KIND_PK[K_NKIND] = {1, 1, 1, 1, DIOP_K, M1P_K};
+// Residual rows a kind writes in a static evaluation.
+constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x3u, 0xdu};
+// Unknowns a kind's limiter writes (its limited image).
+constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x1u, 0xbu};
+
+// Structural G-plane and C-plane entries per local row (bit masks over
+// local columns). The pattern and slot tapes use their union.
+ESP_HD unsigned kind_gmask(int kind, int ru)
+{
+    switch (kind) {
+    case K_VSRC: return ru == 2 ? 0x3u : 0x4u;
+    case K_RES:
+    case K_DIODE: return 0x3u;
+    case K_MOS1: return ru == 1 ? 0x0u : 0xfu;
+    default: return 0u;
+    }
+}
+ESP_HD unsigned kind_cmask(int kind, int ru)
+{
+    switch (kind) {
+    case K_CAP: return 0x3u;
+    case K_MOS1: return 0xfu;
+    default: (void)ru; return 0u;
+    }
+}
+ESP_HD unsigned kind_pmask(int kind, int ru) { return kind_gmask(kind, ru) | kind_cmask(kind, ru); }
+
+// $temperature of every run: 27 degC.
+constexpr double TEMPERATURE_K = 300.15;
+constexpr double KB = 1.38064852e-23, QE = 1.6021766208e-19;
+
+struct DevOut {
+    double res[4];
+    double jac[4][4];
+};
+
+ESP_HD double given(double v, double dflt) { return v == v ? v : dflt; }
+
+// mos1.va:151-284: bias-independent defaulting, temperature, geometry.
+inline void prep_mos1(const double *u, double *p)
+{
+    const double type = given(u[M1_TYPE], 1.0), tnom = given(u[M1_TNOM], 27.0);
+    const double kp = given(u[M1_KP], 2.0e-5), phi = given(u[M1_PHI], 0.6), gamma = given(u[M1_GAMMA], 0.0);
+    const double vto = given(u[M1_VTO], 0.0), tox = given(u[M1_TOX], 1.0e-7), u0 = given(u[M1_U0], 600.0);
+    const double nsub = given(u[M1_NSUB], 0.0), nss = given(u[M1_NSS], 0.0), tpg = given(u[M1_TPG], 1.0);
+    const double lambda = given(u[M1_LAMBDA], 0.0), ld = given(u[M1_LD], 0.0);
+    const double is = given(u[M1_IS], 1.0e-14), js = given(u[M1_JS], 0.0);
+    const double cbd = given(u[M1_CBD], 0.0), cbs = given(u[M1_CBS], 0.0), cj = given(u[M1_CJ], 0.0);
+    const double cjsw = given(u[M1_CJSW], 0.0), pb = given(u[M1_PB], 0.8), mj = given(u[M1_MJ], 0.5);
+    const double mjsw = given(u[M1_MJSW], 0.5), fc = given(u[M1_FC], 0.5);
+    const double cgso = given(u[M1_CGSO], 0.0), cgdo = given(u[M1_CGDO], 0.0), cgbo = given(u[M1_CGBO], 0.0);
+    const double gmin = given(u[M1_GMIN], 1.0e-12), dtemp = given(u[M1_DTEMP], 0.0);
+    const double l = given(u[M1_L], 1.0e-4), w = given(u[M1_W], 1.0e-4);
+    const double ad = given(u[M1_AD], 0.0), as = given(u[M1_AS], 0.0), pd = given(u[M1_PD], 0.0), ps = given(u[M1_PS], 0.0);
+    const double m = given(u[M1_M], 1.0);
+    const bool tox_given = u[M1_TOX] == u[M1_TOX], nsub_given = u[M1_NSUB] == u[M1_NSUB];
+    const bool phi_given = u[M1_PHI] == u[M1_PHI], gamma_given = u[M1_GAMMA] == u[M1_GAMMA];
+    const bool vto_given = u[M1_VTO] == u[M1_VTO], kp_given = u[M1_KP] == u[M1_KP];
+    const bool temp_given = u[M1_TEMP] == u[M1_TEMP];
+
+    double tnomk = tnom + 273.15;
+    double vtnom = tnomk * (KB / QE);
+    double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    double coxp = (tox_given && tox > 0.0) ? 3.9 * 8.854214871e-12 / tox : 0.0;
+    double phix = phi, gammax = gamma, vtox = vto;
+    if (nsub_given && nsub * 1.0e6 > 1.45e16) {
+        if (!phi_given) {
+            phix = 2.0 * vtnom * e_log(nsub * 1.0e6 / 1.45e16);
+            if (phix < 0.1) phix = 0.1;
+        }
+        if (coxp > 0.0) {
+            if (!gamma_given) gammax = __builtin_sqrt(2.0 * 11.7 * 8.854214871e-12 * 1.6021766208e-19 * nsub * 1.0e6) / coxp;
+            if (!vto_given) {
+                double fermis = type * 0.5 * phix, wkfng;
+                if (tpg != 0.0) {
+                    double fermig = tpg * type * 0.5 * egfet1;
+                    wkfng = 3.25 + 0.5 * egfet1 - fermig;
+                } else {
+                    wkfng = 3.2;
+                }
+                double wkfngs = wkfng - (3.25 + 0.5 * egfet1 + fermis);
+                double vfb = wkfngs - nss * 1.0e4 * 1.6021766208e-19 / coxp;
+                vtox = vfb + type * (gammax * __builtin_sqrt(phix) + phix);
+            }
+        }
+    }
+    double kpx = (kp_given || coxp <= 0.0) ? kp : u0 * coxp * 1.0e-4;
+    double tk = temp_given ? u[M1_TEMP] + 273.15 : TEMPERATURE_K + dtemp;
+    double vt = tk * (KB / QE);
+    double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
+    double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    double pbfact1 = -2.0 * vtnom * (1.5 * e_log(fact1) + QE * arg1);
+    double pbfact = -2.0 * vt * (1.5 * e_log(fact2) + QE * argt);
+    double ratio4 = (tk / tnomk) * __builtin_sqrt(tk / tnomk);
+    double kpt = kpx / ratio4;
+    double phio = (phix - pbfact1) / fact1;
+    double phit = fact2 * phio + pbfact;
+    double vbit = vtox - type * (gammax * __builtin_sqrt(phix)) + 0.5 * (egfet1 - egfet) + type * 0.5 * (phit - phix);
+    double ist = is * e_exp(-egfet / vt + egfet1 / vtnom);
+    double jst = js * e_exp(-egfet / vt + egfet1 / vtnom);
+    double pbo = (pb - pbfact1) / fact1;
+    double pbt = fact2 * pbo + pbfact;
+    double gmaold = (pb - pbo) / pbo, gmanew = (pbt - pbo) / pbo;
+    double cfb = (1.0 + mj * (4.0e-4 * (tk - 300.15) - gmanew)) / (1.0 + mj * (4.0e-4 * (tnomk - 300.15) - gmaold));
+    double cfsw = (1.0 + mjsw * (4.0e-4 * (tk - 300.15) - gmanew)) / (1.0 + mjsw * (4.0e-4 * (tnomk - 300.15) - gmaold));
+    double leff = l - 2.0 * ld;
+    if (leff < 1.0e-10) leff = 1.0e-10;
+    double beta = kpt * m * w / leff;
+    double cox = coxp * w * leff * m;
+    double isatd, isats;
+    if (js > 0.0 && ad > 0.0 && as > 0.0) {
+        isatd = jst * ad * m;
+        isats = jst * as * m;
+    } else {
+        isatd = ist * m;
+        isats = ist * m;
+    }
+    double czbd = cbd > 0.0 ? cbd * cfb * m : cj * cfb * ad * m;
+    double czbs = cbs > 0.0 ? cbs * cfb * m : cj * cfb * as * m;
+    double czbdsw = cjsw * cfsw * pd * m, czbssw = cjsw * cfsw * ps * m;
+    double depcap = fc * pbt, argf = 1.0 - fc;
+    double sargb = e_pow(argf, -mj), sargs = e_pow(argf, -mjsw);
+    double f2d = czbd * (1.0 - fc * (1.0 + mj)) * sargb / argf + czbdsw * (1.0 - fc * (1.0 + mjsw)) * sargs / argf;
+    double f3d = czbd * mj * sargb / (argf * pbt) + czbdsw * mjsw * sargs / (argf * pbt);
+    double f4d = czbd * pbt * (1.0 - argf * sargb) / (1.0 - mj) + czbdsw * pbt * (1.0 - argf * sargs) / (1.0 - mjsw)
+                 - 0.5 * f3d * depcap * depcap - depcap * f2d;
+    double f2s = czbs * (1.0 - fc * (1.0 + mj)) * sargb / argf + czbssw * (1.0 - fc * (1.0 + mjsw)) * sargs / argf;
+    double f3s = czbs * mj * sargb / (argf * pbt) + czbssw * mjsw * sargs / (argf * pbt);
+    double f4s = czbs * pbt * (1.0 - argf * sargb) / (1.0 - mj) + czbssw * pbt * (1.0 - argf * sargs) / (1.0 - mjsw)
+                 - 0.5 * f3s * depcap * depcap - depcap * f2s;
+    p[M1P_TYPE] = type;
+    p[M1P_VT] = vt;
+    p[M1P_VBIT] = vbit;
+    p[M1P_PHIT] = phit;
+    p[M1P_GAMMA] = gammax;
+    p[M1P_BETA] = beta;
+    p[M1P_LAMBDA] = lambda;
+    p[M1P_ISATS] = isats;
+    p[M1P_ISATD] = isatd;
+    p[M1P_GMIN] = gmin;
+    p[M1P_VCRITS] = vt * e_log(vt / (1.4142135623730951 * isats));
+    p[M1P_VCRITD] = vt * e_log(vt / (1.4142135623730951 * isatd));
+    p[M1P_COX] = cox;
+    p[M1P_CZBD] = czbd;
+    p[M1P_CZBS] = czbs;
+    p[M1P_CZBDSW] = czbdsw;
+    p[M1P_CZBSSW] = czbssw;
+    p[M1P_PBT] = pbt;
+    p[M1P_MJ] = mj;
+    p[M1P_MJSW] = mjsw;
+    p[M1P_FC] = fc;
+    p[M1P_F2D] = f2d;
+    p[M1P_F3D] = f3d;
+    p[M1P_F4D] = f4d;
+    p[M1P_F2S] = f2s;
+    p[M1P_F3S] = f3s;
+    p[M1P_F4S] = f4s;
+    p[M1P_CGSO] = cgso * w * m;
+    p[M1P_CGDO] = cgdo * w * m;
+    p[M1P_CGBO] = cgbo * leff * m;
+    p[M1P_SEED] = type * vbit + gammax * __builtin_sqrt(phit);
+}
+
+// diode.va:66-172 (tlev = tlevc = 0, level 1, no sidewall).
+inline void prep_diode(const double *u, double *p)
+{
+    const double is = given(u[DIO_IS], 1e-14), n = given(u[DIO_N], 1.0), cjo = given(u[DIO_CJO], 0.0);
+    const double vj = given(u[DIO_VJ], 1.0), m = given(u[DIO_M], 0.5), fc = given(u[DIO_FC], 0.5);
+    const double tt = given(u[DIO_TT], 0.0), gmin = given(u[DIO_GMIN], 1e-12), area = given(u[DIO_AREA], 1.0);
+    const double tnom = given(u[DIO_TNOM], 27.0), dtemp = given(u[DIO_DTEMP], 0.0), xti = given(u[DIO_XTI], 3.0);
+    const bool temp_given = u[DIO_TEMP] == u[DIO_TEMP];
+    double tk = temp_given ? u[DIO_TEMP] + 273.15 : TEMPERATURE_K + dtemp;
+    double tnomk = tnom + 273.15;
+    double ratio = tk / tnomk, dtk = tk - tnomk;
+    double vt = KB / QE * tk, vtnom = KB / QE * tnomk;
+    double vte = n * vt;
+    double ae = area;
+    double egp = given(u[DIO_EG], 1.11);
+    double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
+    double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
+    double mt = m * (1.0 + 0.0 * dtk + 0.0 * dtk * dtk);
+    double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
+    double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
+    double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
+    double pbfact1 = -2.0 * vtnom * (1.5 * e_log(fact1) + QE * arg1);
+    double pbfact = -2.0 * vt * (1.5 * e_log(fact2) + QE * argt);
+    double pbo = (vj - pbfact1) / fact1;
+    double gmaold = (vj - pbo) / pbo;
+    double vj_t = pbfact + fact2 * pbo;
+    double gmanew = (vj_t - pbo) / pbo;
+    double cjo_t = cjo * (1.0 + mt * (400e-6 * (tk - 300.15) - gmanew)) / (1.0 + mt * (400e-6 * (tnomk - 300.15) - gmaold));
+    cjo_t = cjo_t * ae;
+    double is_t = is * ae * e_exp((ratio - 1.0) * egp / vte + (xti / n) * e_log(ratio));
+    double issw_t = 0.0 * 0.0 * e_exp((ratio - 1.0) * egp / vt + (xti / 1.0) * e_log(ratio));
+    is_t = is_t + issw_t;
+    p[DIOP_IS] = is_t;
+    p[DIOP_VTE] = vte;
+    p[DIOP_GMIN] = gmin;
+    p[DIOP_VCRIT] = vte * e_log(vte / (1.4142135623730951 * (is_t + 0.0)));
+    p[DIOP_CJO] = cjo_t;
+    p[DIOP_VJ] = vj_t;
+    p[DIOP_MT] = mt;
+    p[DIOP_FC] = fc;
+    p[DIOP_TT] = tt;
+    (void)dtemp;
+}
+
+inline void prep_kind(int kind, const double *u, double *p)
+{
+    switch (kind) {
+    case K_MOS1: prep_mos1(u, p); break;
+    case K_DIODE: prep_diode(u, p); break;
+    case K_VSRC: p[0] = u[VSRC_DC] == u[VSRC_DC] ? u[VSRC_DC] : 0.0; break;
+    case K_ISRC: p[0] = u[ISRC_DC] == u[ISRC_DC] ? u[ISRC_DC] : 0.0; break;
+    case K_RES: p[0] = 1.0 / u[RES_R]; break;
+    case K_CAP: p[0] = u[CAP_C]; break;
+    default: break;
+    }
+}
+
+// ------------------------------------------------------------- limiting
+
+ESP_HD double fetlim(double vnew, double vold, double vto)
+{
+    double vtsthi = fabs_(2 * (vold - vto)) + 2;
+    double vtstlo = vtsthi / 2 + 2;
+    double vtox = vto + 3.5;
+    double delv = vnew - vold;
+    if (vold >= vto) {
+        if (vold >= vtox) {
+            if (delv <= 0) {
+                if (vnew >= vtox) {
+                    if (-delv > vtstlo) vnew = vold - vtstlo;
+                } else {
+                    vnew = va_max(vnew, vto + 2);
+                }
+            } else {
+                if (delv >= vtsthi) vnew = vold + vtsthi;
+            }
+        } else {
+            if (delv <= 0) vnew = va_max(vnew, vto - 0.5);
+            else vnew = va_min(vnew, vto + 4);
+        }
+    } else {
+        if (delv <= 0) {
+            if (-delv > vtsthi) vnew = vold - vtsthi;
+        } else {
+            double vtemp = vto + 0.5;
+            if (vnew <= vtemp) {
+                if (delv > vtstlo) vnew = vold + vtstlo;
+            } else {
+                vnew = vtemp;
+            }
+        }
+    }
+    return vnew;
+}
+
+ESP_HD double limvds(double vnew, double vold)
+{
+    if (vold >= 3.5) {
+        if (vnew > vold) vnew = va_min(vnew, 3 * vold + 2);
+        else if (vnew < 3.5) vnew = va_max(vnew, 2);
+    } else {
+        if (vnew > vold) vnew = va_min(vnew, 4);
+        else vnew = va_max(vnew, -0.5);
+    }
+    return vnew;
+}
+
+ESP_HD double pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck)
+{
+    if (vnew > vcrit && fabs_(vnew - vold) > vt + vt) {
+        if (vold > 0) {
+            double arg = 1 + (vnew - vold) / vt;
+            if (arg > 0) vnew = vold + vt * e_log(arg);
+            else vnew = vcrit;
+        } else {
+            vnew = vt * e_log(vnew / vt);
+        }
+        *icheck = 1;
+    } else {
+        if (vnew < 0) {
+            double arg = vold > 0 ? -1 * vold - 1 : 2 * vold - 1;
+            if (vnew < arg) {
+                vnew = arg;
+                *icheck = 1;
+            } else {
+                *icheck = 0;
+            }
+        } else {
+            *icheck = 0;
+        }
+    }
+    return vnew;
+}
+
+ESP_HD void snap_limited(int nu, unsigned reads, unsigned writes, double *lx, const double *cur)
+{
+    double scale = 0;
+    for (int u = 0; u < nu; u++)
+        if (reads >> u & 1) scale = fmax_(scale, fabs_(cur[u]));
+    double tol = 16 * 2.220446049250313e-16 * scale;
+    for (int u = 0; u < nu; u++)
+        if ((writes & reads) >> u & 1)
+            if (fabs_(lx[u] - cur[u]) <= tol) lx[u] = cur[u];
+}
+
+// ---------------------------------------------------------------- eval
+
+ESP_HD Dual jcur(Dual v, double isat, double vtx, double gm)
+{
+    if (v.v > -3.0 * vtx) {
+        Dual w = ddivs(v, vtx);
+        Dual u = dminc(w, 80.0);
+        Dual e = dexp(u);
+        Dual t = dmul(e, dadd(dc(1.0), dsub(w, u)));
+        return dadd(dscale(isat, daddc(t, -1.0)), dscale(gm, v));
+    }
+    return daddc(dscale(gm, v), -isat);
+}
+
+ESP_HD Dual sbody(Dual vbx, double phit)
+{
+    if (vbx.v <= 0.0) return dsqrt(dcsub(phit, vbx));
+    double sp = __builtin_sqrt(phit);
+    return dmaxc(dcsub(sp, ddivs(vbx, 2.0 * sp)), 0.0);
+}
+
+template <class P>
+ESP_HD void eval_mos1(const P &p, const double *lx, DevOut &o)
+{
+    const double type = p[M1P_TYPE];
+    Dual d = dvar(lx[0], 0), g = dvar(lx[1], 1), s = dvar(lx[2], 2), b = dvar(lx[3], 3);
+    Dual vgs = dscale(type, dsub(g, s));
+    Dual vds = dscale(type, dsub(d, s));
+    Dual vbs = dscale(type, dsub(b, s));
+    Dual vbd = dscale(type, dsub(b, d));
+    Dual vgd = dsub(vgs, vds);
+    Dual ibs = jcur(vbs, p[M1P_ISATS], p[M1P_VT], p[M1P_GMIN]);
+    Dual ibd = jcur(vbd, p[M1P_ISATD], p[M1P_VT], p[M1P_GMIN]);
+    double mode;
+    Dual vbx, vgsx;
+    if (vds.v >= 0.0) {
+        mode = 1.0;
+        vbx = vbs;
+        vgsx = vgs;
+    } else {
+        mode = -1.0;
+        vbx = vbd;
+        vgsx = vgd;
+    }
+    Dual vdsx = dscale(mode, vds);
+    Dual sarg = sbody(vbx, p[M1P_PHIT]);
+    if (vbx.v > 0.0 && sarg.v > 0.0) {
+        double sarg0 = sarg.v, vbx0 = vbx.v;
+        sarg = dcsub(sarg0, ddivs(daddc(vbx, -vbx0), 2.0 * sarg0));
+    }
+    Dual von = dadd(dc(p[M1P_VBIT] * type), dscale(p[M1P_GAMMA], sarg));
+    Dual vgst = dsub(vgsx, von);
+    Dual bp = dscale(p[M1P_BETA], dadd(dc(1.0), dscale(p[M1P_LAMBDA], vdsx)));
+    Dual id;
+    if (vgst.v <= 0.0) id = dc(0.0);
+    else if (vgst.v <= vdsx.v) id = dmul(dmul(dscale(0.5, bp), vgst), vgst);
+    else id = dmul(dmul(bp, vdsx), dsub(vgst, dscale(0.5, vdsx)));
+    Dual cd = dscale(type * mode, id);
+    Dual tibd = dscale(type, ibd), tibs = dscale(type, ibs);
+    Dual rd = dsub(cd, tibd);
+    Dual rs = dsub(dneg(cd), tibs);
+    Dual rb = dadd(tibd, tibs);
+    o.res[0] = rd.v;
+    o.res[1] = 0.0;
+    o.res[2] = rs.v;
+    o.res[3] = rb.v;
+    for (int c = 0; c < 4; c++) {
+        o.jac[0][c] = rd.d[c];
+        o.jac[1][c] = 0.0;
+        o.jac[2][c] = rs.d[c];
+        o.jac[3][c] = rb.d[c];
+    }
+}
+
+template <class P>
+ESP_HD int limit_mos1(const P &p, const double *cur, const double *old, double *lim)
+{
+    const double type = p[M1P_TYPE];
+    double vgs_o = type * (old[1] - old[2]), vds_o = type * (old[0] - old[2]);
+    double vbs_o = type * (old[3] - old[2]), vbd_o = type * (old[3] - old[0]);
+    double vgd_o = vgs_o - vds_o;
+    double vgs = type * (cur[1] - cur[2]), vds = type * (cur[0] - cur[2]);
+    double vbs = type * (cur[3] - cur[2]), vbd = type * (cur[3] - cur[0]);
+    double vgd = vgs - vds;
+    double vbx_o = vds_o >= 0.0 ? vbs_o : vbd_o;
+    double phit = p[M1P_PHIT];
+    double sb;
+    if (vbx_o <= 0.0) sb = __builtin_sqrt(phit - vbx_o);
+    else sb = va_max(__builtin_sqrt(phit) - vbx_o / (2.0 * __builtin_sqrt(phit)), 0.0);
+    double von = p[M1P_VBIT] * type + p[M1P_GAMMA] * sb;
+    if (vds_o >= 0.0) {
+        vgs = fetlim(vgs, vgs_o, von);
+        vds = vgs - vgd;
+        vds = limvds(vds, vds_o);
+    } else {
+        vgd = fetlim(vgd, vgd_o, von);
+        vds = vgs - vgd;
+        vds = -limvds(-vds, -vds_o);
+        vgs = vgd + vds;
+    }
+    int check = 0;
+    if (vds >= 0.0) {
+        vbs = pnjlim(vbs, vbs_o, p[M1P_VT], p[M1P_VCRITS], &check);
+    } else {
+        vbd = pnjlim(vbd, vbd_o, p[M1P_VT], p[M1P_VCRITD], &check);
+        vbs = vbd + vds;
+    }
+    double s = cur[2];
+    lim[0] = s + type * vds;
+    lim[1] = s + type * vgs;
+    lim[2] = s;
+    lim[3] = s + type * vbs;
+    snap_limited(4, 0xfu, 0xbu, lim, cur);
+    return check;
+}
+
+template <class P>
+ESP_HD void seed_mos1(const P &p, const double *cur, double *lim)
+{
+    const double type = p[M1P_TYPE];
+    double s = cur[2];
+    lim[0] = s;
+    lim[1] = s + type * p[M1P_SEED];
+    lim[2] = s;
+    lim[3] = s + type * -1.0;
+}
+
+template <class P>
+ESP_HD void eval_diode(const P &p, const double *lx, DevOut &o)
+{
+    Dual a = dvar(lx[0], 0), c = dvar(lx[1], 1);
+    Dual vd = dsub(a, c);
+    const double vte = p[DIOP_VTE], is_t = p[DIOP_IS];
+    Dual id;
+    if (vd.v >= -3.0 * vte) {
+        Dual u = ddivs(vd, vte);
+        Dual ulim = dminc(u, 80.0);
+        id = dscale(is_t, daddc(dmul(dexp(ulim), dadd(dc(1.0), dsub(u, ulim))), -1.0));
+    } else {
+        Dual t = dpowc(ddiv(dc(3.0 * vte), dscale(2.718281828459045, vd)), 3.0);
+        id = dscale(-is_t, dadd(dc(1.0), t));
+    }
+    Dual i = dadd(id, dscale(p[DIOP_GMIN], vd));
+    o.res[0] = i.v;
+    o.res[1] = -i.v;
+    for (int k = 0; k < 2; k++) {
+        o.jac[0][k] = i.d[k];
+        o.jac[1][k] = -i.d[k];
+    }
+}
+
+template <class P>
+ESP_HD int limit_diode(const P &p, const double *cur, const double *old, double *lim)
+{
+    int check = 0;
+    double vd = pnjlim(cur[0] - cur[1], old[0] - old[1], p[DIOP_VTE], p[DIOP_VCRIT], &check);
+    lim[0] = cur[1] + vd;
+    lim[1] = cur[1];
+    snap_limited(2, 0x3u, 0x1u, lim, cur);
+    return check;
+}
+
+template <class P>
+ESP_HD void seed_diode(const P &p, const double *cur, double *lim)
+{
+    lim[0] = cur[1] + p[DIOP_VCRIT];
+    lim[1] = cur[1];
+}
+
+// Static evaluation of one instance of `kind` at local voltages lx.
+template <class P>
+ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o)
+{
+    for (int r = 0; r < 4; r++) {
+        o.res[r] = 0.0;
+        for (int c = 0; c < 4; c++) o.jac[r][c] = 0.0;
+    }
+    switch (kind) {
+    case K_VSRC:
+        o.res[0] = lx[2];
+        o.res[1] = -lx[2];
+        o.res[2] = lx[0] - lx[1] - p[0];
+        o.jac[0][2] = 1.0;
+        o.jac[1][2] = -1.0;
+        o.jac[2][0] = 1.0;
+        o.jac[2][1] = -1.0;
+        break;
+    case K_ISRC:
+        o.res[0] = p[0];
+        o.res[1] = -p[0];
+        break;
+    case K_RES: {
+        double i = p[0] * (lx[0] - lx[1]);
+        o.res[0] = i;
+        o.res[1] = -i;
+        o.jac[0][0] = p[0];
+        o.jac[0][1] = -p[0];
+        o.jac[1][0] = -p[0];
+        o.jac[1][1] = p[0];
+        break;
+    }
+    case K_DIODE: eval_diode(p, lx, o); break;
+    case K_MOS1: eval_mos1(p, lx, o); break;
+    default: break;
+    }
+}
+
+}  // namespace esp

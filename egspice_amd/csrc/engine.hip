@@ -1,0 +1,552 @@
+// The lane engine: circuit templates, lane batches resident on one MI355X,
+// the gfx950 kernels, and the C ABI of include/espice_lanes.h.
+//
+// Kernels (one thread per lane, 64-lane wavefronts, lane-major planes):
+//   newton_op_kernel      the whole static Newton of a lane in one launch:
+//                         device evaluation and MNA stamping, ground pin,
+//                         the lane's own sparse direct factor and solve,
+//                         update, limiting and the gates, per tick, until the
+//                         lane converges, goes singular or runs out of ticks.
+//                         The reference runs the same tick as two captured
+//                         graphs of ~10 kernels with a host sync per tick
+//                         (src/analysis/gpu/lanes.zig:1431-1601); here the
+//                         tick loop lives on the device, so a solve is one
+//                         launch and one sync.
+//   tab_refactor_kernel   the reference's `arp_lane_refactor` + `arp_lane_solve`
+//                         over a shared lead tape (lane_device.zig:101,170).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/espice_lanes.h"
+#include "engine.h"
+
+using namespace esp;
+
+namespace {
+
+thread_local std::string g_error;
+
+espice_status fail(espice_status s, const char *name)
+{
+    g_error = name;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) return fail(ESPICE_FAILED, hipGetErrorName(e_)); \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        n = count ? count : 1;
+        return hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T));
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+struct espice_lanes_template {
+    Template t;
+};
+
+// ---------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= B.N) return;
+    lane_newton(B, l, o);
+}
+
+__global__ void init_lane_state_kernel(uint8_t *state, uint32_t N, uint8_t tri)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    state[(size_t)ST_FACTORED * N + l] = 0;
+    state[(size_t)ST_TRI * N + l] = tri;
+    state[(size_t)ST_LU_FACTORED * N + l] = 0;
+}
+
+__global__ __launch_bounds__(64) void tab_refactor_kernel(LaneTab t, const uint32_t *idx, uint32_t N, const double *a,
+                                                          const double *rhs, double *dx, uint8_t *fail, double *lx,
+                                                          double *ux, double *udiag, double *w, double *y)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    LV<const double> av{a + l, N}, rv{rhs + l, N};
+    LV<double> dv{dx + l, N};
+    bool bad = tab_refactor(t, idx, av, LV<double>{lx + l, N}, LV<double>{ux + l, N}, LV<double>{udiag + l, N},
+                            LV<double>{w + l, N});
+    fail[l] = bad ? 1 : 0;
+    if (!bad) tab_solve_neg(t, idx, LV<double>{lx + l, N}, LV<double>{ux + l, N}, LV<double>{udiag + l, N}, rv,
+                            LV<double>{y + l, N}, dv);
+}
+
+// ------------------------------------------------------------------ batch
+
+struct espice_lanes {
+    const Template *t = nullptr;
+    uint32_t N = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> uprm[K_NKIND];  // host, lane-major
+    bool prepared = false;
+    // device pattern
+    DevBuf<uint32_t> col_ptr, row_idx, q, tri_pos, diag, nodes[K_NKIND], slots[K_NKIND];
+    DevBuf<uint8_t> cur_row;
+    // device lane planes
+    DevBuf<double> prm[K_NKIND], lim[K_NKIND], x, x_old, dx, g, rhs;
+    DevBuf<uint32_t> status, iters;
+    // lane solver
+    DevBuf<uint8_t> st, void_col, scaled_pivot;
+    DevBuf<uint32_t> cnt, pinv, lp, up, li, ui, prow, void_slots, flag, topo, stack, pstack, want, want_of, prev;
+    DevBuf<double> lx, ux, udiag, w, y, rscale, vcopy, tri, probe;
+    uint32_t cap = 0;
+
+    ~espice_lanes()
+    {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    Batch batch() const
+    {
+        Batch B;
+        B.N = N;
+        B.P = Pattern{t->n, t->nnz, col_ptr.p, row_idx.p, q.p, t->tridiagonal ? tri_pos.p : nullptr, t->diag_fill, cap};
+        B.diag = diag.p;
+        B.cur_row = cur_row.p;
+        B.n_order = t->n_order;
+        for (int k = 0; k < K_NKIND; k++) {
+            B.order[k] = t->order[k];
+            B.count[k] = t->count[k];
+            B.kslots[k] = t->kslots[k];
+            B.nodes[k] = nodes[k].p;
+            B.slots[k] = slots[k].p;
+            B.prm[k] = prm[k].p;
+            B.lim[k] = lim[k].p;
+        }
+        B.x = x.p;
+        B.x_old = x_old.p;
+        B.dx = dx.p;
+        B.g = g.p;
+        B.rhs = rhs.p;
+        B.status = status.p;
+        B.iters = iters.p;
+        B.lu = LaneLUBuf{st.p,    cnt.p,    pinv.p, lp.p,    up.p,      li.p,         ui.p,   prow.p,
+                         void_slots.p, flag.p, topo.p, stack.p, pstack.p, want.p, want_of.p, prev.p,
+                         void_col.p, scaled_pivot.p, lx.p, ux.p, udiag.p, w.p, y.p, rscale.p,
+                         vcopy.p, tri.p, probe.p};
+        return B;
+    }
+};
+
+namespace {
+
+template <class T>
+hipError_t upload(DevBuf<T> &d, const std::vector<T> &h)
+{
+    hipError_t e = d.alloc(h.size());
+    if (e != hipSuccess) return e;
+    if (h.empty()) return hipSuccess;
+    return hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+template <class T>
+hipError_t zeroed(DevBuf<T> &d, size_t count)
+{
+    hipError_t e = d.alloc(count);
+    if (e != hipSuccess) return e;
+    return hipMemset(d.p, 0, d.n * sizeof(T));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+espice_status espice_lanes_error_message(char *buffer, size_t capacity, size_t *required)
+{
+    if (required) *required = g_error.size() + 1;
+    if (capacity < g_error.size() + 1) return ESPICE_BUFFER_TOO_SMALL;
+    if (!buffer) return ESPICE_INVALID_ARGUMENT;
+    std::memcpy(buffer, g_error.c_str(), g_error.size() + 1);
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_create(espice_lanes_template **out)
+{
+    if (!out) return fail(ESPICE_INVALID_ARGUMENT, "NullOutput");
+    *out = new (std::nothrow) espice_lanes_template();
+    return *out ? ESPICE_OK : fail(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
+}
+
+espice_status espice_lanes_template_destroy(espice_lanes_template *t)
+{
+    delete t;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_node(espice_lanes_template *t, const char *name, uint32_t *row)
+{
+    if (!t || !name || !row) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (t->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "TemplateFrozen");
+    *row = t->t.node(name);
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_add(espice_lanes_template *t, uint32_t kind, const uint32_t *ports,
+                                        const double *params, size_t param_count, uint32_t *instance)
+{
+    if (!t || !ports || !instance || kind >= K_NKIND) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    if (param_count != (size_t)KIND_UK[kind] || (param_count && !params)) return fail(ESPICE_INVALID_ARGUMENT, "ParamCount");
+    if (t->t.add((int)kind, ports, params, instance) != 0) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_freeze(espice_lanes_template *t)
+{
+    if (!t) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (t->t.freeze() != 0) return fail(ESPICE_FAILED, "OutOfWorkspace");
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_info(const espice_lanes_template *t, uint32_t *n, uint32_t *nnz, uint32_t *tridiagonal)
+{
+    if (!t || !t->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "NotFrozen");
+    if (n) *n = t->t.n;
+    if (nnz) *nnz = t->t.nnz;
+    if (tridiagonal) *tridiagonal = t->t.tridiagonal ? 1 : 0;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_pattern(const espice_lanes_template *t, uint32_t *col_ptr, uint32_t *row_idx,
+                                            uint32_t *diag, uint8_t *current_row, uint32_t *q)
+{
+    if (!t || !t->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "NotFrozen");
+    const Template &T = t->t;
+    if (col_ptr) std::memcpy(col_ptr, T.col_ptr.data(), 4 * (T.n + 1));
+    if (row_idx) std::memcpy(row_idx, T.row_idx.data(), 4 * (size_t)T.nnz);
+    if (diag) std::memcpy(diag, T.diag.data(), 4 * (size_t)T.n);
+    if (current_row) std::memcpy(current_row, T.cur_row.data(), T.n);
+    if (q) std::memcpy(q, T.q.data(), 4 * (size_t)T.n);
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_lanes, espice_lanes **out)
+{
+    if (!tt || !out || n_lanes == 0) return fail(ESPICE_INVALID_ARGUMENT, "InvalidLanes");
+    if (!tt->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "NotFrozen");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(ESPICE_FAILED, "NoGpu");
+    std::unique_ptr<espice_lanes> L(new (std::nothrow) espice_lanes());
+    if (!L) return fail(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
+    const Template &T = tt->t;
+    L->t = &T;
+    L->N = n_lanes;
+    const size_t N = n_lanes, n = T.n, nnz = T.nnz;
+    HIP_TRY(hipStreamCreate(&L->stream));
+    HIP_TRY(hipEventCreate(&L->ev0));
+    HIP_TRY(hipEventCreate(&L->ev1));
+    for (int k = 0; k < K_NKIND; k++) {
+        L->uprm[k].resize((size_t)T.count[k] * KIND_UK[k] * N);
+        for (size_t i = 0; i < T.count[k]; i++)
+            for (int p = 0; p < KIND_UK[k]; p++)
+                std::fill_n(&L->uprm[k][(i * KIND_UK[k] + p) * N], N, T.uprm[k][i * KIND_UK[k] + p]);
+    }
+    HIP_TRY(upload(L->col_ptr, T.col_ptr));
+    HIP_TRY(upload(L->row_idx, T.row_idx));
+    HIP_TRY(upload(L->q, T.q));
+    HIP_TRY(upload(L->tri_pos, T.tri_pos));
+    HIP_TRY(upload(L->diag, T.diag));
+    HIP_TRY(upload(L->cur_row, T.cur_row));
+    for (int k = 0; k < K_NKIND; k++) {
+        HIP_TRY(upload(L->nodes[k], T.nodes[k]));
+        HIP_TRY(upload(L->slots[k], T.slots[k]));
+        HIP_TRY(zeroed(L->prm[k], (size_t)T.count[k] * KIND_PK[k] * N));
+        HIP_TRY(zeroed(L->lim[k], (size_t)T.count[k] * KIND_NU[k] * N));
+    }
+    HIP_TRY(zeroed(L->x, n * N));
+    HIP_TRY(zeroed(L->x_old, n * N));
+    HIP_TRY(zeroed(L->dx, n * N));
+    HIP_TRY(zeroed(L->g, (nnz + 1) * N));
+    HIP_TRY(zeroed(L->rhs, (n + 1) * N));
+    HIP_TRY(zeroed(L->status, N));
+    HIP_TRY(zeroed(L->iters, N));
+    // L + U of any factor with n pivots fit n (n - 1) entries.
+    L->cap = (uint32_t)std::max<size_t>(n * (n - 1), 1);
+    const size_t cap = L->cap;
+    HIP_TRY(zeroed(L->st, 3 * N));
+    HIP_TRY(zeroed(L->cnt, 4 * N));
+    HIP_TRY(zeroed(L->pinv, n * N));
+    HIP_TRY(zeroed(L->lp, (n + 1) * N));
+    HIP_TRY(zeroed(L->up, (n + 1) * N));
+    HIP_TRY(zeroed(L->li, cap * N));
+    HIP_TRY(zeroed(L->ui, cap * N));
+    HIP_TRY(zeroed(L->prow, nnz * N));
+    HIP_TRY(zeroed(L->void_slots, nnz * N));
+    for (DevBuf<uint32_t> *b : {&L->flag, &L->topo, &L->stack, &L->pstack, &L->want, &L->want_of, &L->prev})
+        HIP_TRY(zeroed(*b, n * N));
+    HIP_TRY(zeroed(L->void_col, n * N));
+    HIP_TRY(zeroed(L->scaled_pivot, n * N));
+    HIP_TRY(zeroed(L->lx, cap * N));
+    HIP_TRY(zeroed(L->ux, cap * N));
+    for (DevBuf<double> *b : {&L->udiag, &L->w, &L->y, &L->rscale}) HIP_TRY(zeroed(*b, n * N));
+    HIP_TRY(zeroed(L->vcopy, nnz * N));
+    HIP_TRY(zeroed(L->tri, 3 * n * N));
+    HIP_TRY(zeroed(L->probe, 3 * n * N));
+    const uint32_t blocks = (n_lanes + 63) / 64;
+    hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, n_lanes,
+                       (uint8_t)(T.tridiagonal ? 1 : 0));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    *out = L.release();
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_destroy(espice_lanes *l)
+{
+    delete l;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_reset(espice_lanes *L)
+{
+    if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    const uint32_t blocks = (L->N + 63) / 64;
+    hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, L->N,
+                       (uint8_t)(L->t->tridiagonal ? 1 : 0));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(L->cnt.p, 0, L->cnt.n * sizeof(uint32_t), L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_set_param(espice_lanes *L, uint32_t kind, uint32_t instance, uint32_t param, const double *values)
+{
+    if (!L || !values || kind >= K_NKIND) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    if (instance >= L->t->count[kind] || param >= (uint32_t)KIND_UK[kind]) return fail(ESPICE_INVALID_ARGUMENT, "InvalidParam");
+    std::memcpy(&L->uprm[kind][((size_t)instance * KIND_UK[kind] + param) * L->N], values, sizeof(double) * L->N);
+    L->prepared = false;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_prepare(espice_lanes *L)
+{
+    if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    const size_t N = L->N;
+    for (int kind = 0; kind < K_NKIND; kind++) {
+        const uint32_t cnt = L->t->count[kind];
+        const int uk = KIND_UK[kind], pk = KIND_PK[kind];
+        std::vector<double> host((size_t)cnt * pk * N);
+        double u[64], p[64];
+        for (uint32_t i = 0; i < cnt; i++)
+            for (size_t l = 0; l < N; l++) {
+                for (int k = 0; k < uk; k++) u[k] = L->uprm[kind][((size_t)i * uk + k) * N + l];
+                prep_kind(kind, u, p);
+                for (int k = 0; k < pk; k++) host[((size_t)i * pk + k) * N + l] = p[k];
+            }
+        if (!host.empty())
+            HIP_TRY(hipMemcpyAsync(L->prm[kind].p, host.data(), host.size() * 8, hipMemcpyHostToDevice, L->stream));
+        HIP_TRY(hipStreamSynchronize(L->stream));
+    }
+    L->prepared = true;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_upload_x(espice_lanes *L, const double *x)
+{
+    if (!L || !x) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    HIP_TRY(hipMemcpy(L->x.p, x, sizeof(double) * L->t->n * L->N, hipMemcpyHostToDevice));
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_download_x(espice_lanes *L, double *x)
+{
+    if (!L || !x) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    HIP_TRY(hipMemcpy(x, L->x.p, sizeof(double) * L->t->n * L->N, hipMemcpyDeviceToHost));
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *opts, espice_lane_result *out, double *kernel_ms)
+{
+    if (!L || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
+    Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
+    const Batch B = L->batch();
+    const uint32_t blocks = (L->N + 63) / 64;
+    HIP_TRY(hipEventRecord(L->ev0, L->stream));
+    hipLaunchKernelGGL(newton_op_kernel, dim3(blocks), dim3(64), 0, L->stream, B, o);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(L->ev1, L->stream));
+    std::vector<uint32_t> st(L->N), it(L->N);
+    HIP_TRY(hipMemcpyAsync(st.data(), L->status.p, 4 * (size_t)L->N, hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipMemcpyAsync(it.data(), L->iters.p, 4 * (size_t)L->N, hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    if (kernel_ms) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+        *kernel_ms = ms;
+    }
+    for (uint32_t l = 0; l < L->N; l++) out[l] = espice_lane_result{st[l], it[l]};
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_tab_build(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, const double *vals,
+                                     espice_lane_tab *tab, uint32_t *idx, size_t idx_capacity, size_t *len)
+{
+    if (!col_ptr || !row_idx || !vals || !tab || !len || n == 0) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    const uint32_t nnz = col_ptr[n];
+    std::vector<uint32_t> q(n);
+    if (column_order(n, col_ptr, row_idx, q.data()) != 0) return fail(ESPICE_FAILED, "OutOfWorkspace");
+    if (is_tridiag(n, col_ptr, row_idx)) {
+        // ownTape: the Thomas engine keeps no LU, so there is no tape unless
+        // its factor fails and demotes (direct.zig:359-362).
+        std::vector<uint32_t> pos(3 * (size_t)n, ESP_NONE);
+        for (uint32_t j = 0; j < n; j++)
+            for (uint32_t p = col_ptr[j]; p < col_ptr[j + 1]; p++) {
+                uint32_t r = row_idx[p];
+                if (r == j) pos[n + j] = p;
+                else if (r == j + 1) pos[j + 1] = p;
+                else if (j > 0 && r + 1 == j) pos[2 * n + r] = p;
+            }
+        std::vector<double> tri(3 * (size_t)n);
+        Pattern P{n, nnz, col_ptr, row_idx, q.data(), pos.data(), 0, 0};
+        LaneLU s{};
+        s.bp = LV<double>{tri.data(), 1};
+        s.mul = s.bp.at(n);
+        s.cv = s.bp.at(2 * (size_t)n);
+        if (tri_factor(P, s, vals) == 0) return fail(ESPICE_FAILED, "NoLaneTape");
+    }
+    const uint64_t df = std::max<uint64_t>(2 * (chol_count(n, col_ptr, row_idx, q.data(), UINT64_MAX) - n), nnz);
+    const size_t cap = std::max<size_t>((size_t)n * (n - 1), 1);
+    std::vector<uint8_t> st(3, 0), vcol(n), spiv(n);
+    std::vector<uint32_t> cnt(4, 0), pinv(n), lp(n + 1), up(n + 1), li(cap), ui(cap), prow(nnz + 1), vslots(nnz + 1);
+    std::vector<uint32_t> flag(n), topo(n), stack(n), pstack(n), want(n), want_of(n), prev(n);
+    std::vector<double> lx(cap), ux(cap), udiag(n), w(n, 0.0), y(n), rscale(n), vcopy(nnz + 1), tri(3 * (size_t)n),
+        probe(3 * (size_t)n);
+    LaneLUBuf buf{st.data(),    cnt.data(),    pinv.data(), lp.data(),   up.data(),      li.data(),      ui.data(),
+                  prow.data(),  vslots.data(), flag.data(), topo.data(), stack.data(),   pstack.data(),  want.data(),
+                  want_of.data(), prev.data(), vcol.data(), spiv.data(), lx.data(),      ux.data(),      udiag.data(),
+                  w.data(),     y.data(),      rscale.data(), vcopy.data(), tri.data(), probe.data()};
+    Pattern P{n, nnz, col_ptr, row_idx, q.data(), nullptr, df, (uint32_t)cap};
+    LaneLU s = lane_lu(buf, 1, 0, n);
+    if (direct_factor(P, s, vals) != 0) return fail(ESPICE_FAILED, "SingularMatrix");
+    const uint32_t nl = cnt[C_NL], nu = cnt[C_NU], nv = cnt[C_NVOID];
+    espice_lane_tab T{};
+    T.n = n;
+    T.nnz = nnz;
+    T.nnz_l = nl;
+    T.nnz_u = nu;
+    T.growth = GROWTH_LIMIT;
+    T.wide = 1 / PIVOT_TOL;
+    T.n_void_slots = nv;
+    const size_t total = (size_t)(n + 1) + nnz + n + n + (n + 1) + nl + (n + 1) + nu + nv + n;
+    *len = total;
+    if (idx_capacity < total || !idx) return fail(ESPICE_BUFFER_TOO_SMALL, "BufferTooSmall");
+    uint32_t at = 0;
+    auto part = [&](uint32_t &field, const uint32_t *src, size_t cntw) {
+        field = at;
+        if (cntw) std::memcpy(idx + at, src, 4 * cntw);
+        at += (uint32_t)cntw;
+    };
+    part(T.col_ptr, col_ptr, n + 1);
+    part(T.prow, prow.data(), nnz);
+    part(T.q, q.data(), n);
+    part(T.pinv, pinv.data(), n);
+    part(T.lp, lp.data(), n + 1);
+    part(T.li, li.data(), nl);
+    part(T.up, up.data(), n + 1);
+    part(T.ui, ui.data(), nu);
+    part(T.void_slots, vslots.data(), nv);
+    T.flags = at;
+    for (uint32_t k = 0; k < n; k++) idx[at + k] = (vcol[k] ? 1u : 0u) | (spiv[k] ? 2u : 0u);
+    *tab = T;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_tab_refactor_solve(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
+                                              uint32_t n_lanes, const double *a, const double *rhs, double *dx,
+                                              uint8_t *fail_out, double *kernel_ms)
+{
+    if (!tab || !idx || !a || !rhs || !dx || !fail_out || n_lanes == 0) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    const espice_lane_tab &T = *tab;
+    // Every word the bodies index must lie inside idx: checked here, once,
+    // before anything runs on the device.
+    const size_t need = (size_t)T.flags + T.n;
+    if (need > idx_len || T.col_ptr + T.n + 1 > idx_len || T.prow + T.nnz > idx_len || T.li + T.nnz_l > idx_len ||
+        T.ui + T.nnz_u > idx_len || T.void_slots + T.n_void_slots > idx_len)
+        return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t k = 0; k <= T.n; k++)
+        if (idx[T.lp + k] > T.nnz_l || idx[T.up + k] > T.nnz_u || idx[T.col_ptr + k] > T.nnz)
+            return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t k = 0; k < T.n; k++)
+        if (idx[T.q + k] >= T.n || idx[T.pinv + k] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz; p++)
+        if (idx[T.prow + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz_l; p++)
+        if (idx[T.li + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz_u; p++)
+        if (idx[T.ui + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t v = 0; v < T.n_void_slots; v++)
+        if (idx[T.void_slots + v] >= T.nnz) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(ESPICE_FAILED, "NoGpu");
+    const size_t N = n_lanes, n = T.n;
+    DevBuf<uint32_t> d_idx;
+    DevBuf<double> d_a, d_rhs, d_dx, d_lx, d_ux, d_udiag, d_w, d_y;
+    DevBuf<uint8_t> d_fail;
+    HIP_TRY(d_idx.alloc(idx_len));
+    HIP_TRY(hipMemcpy(d_idx.p, idx, 4 * idx_len, hipMemcpyHostToDevice));
+    HIP_TRY(d_a.alloc((size_t)T.nnz * N));
+    HIP_TRY(hipMemcpy(d_a.p, a, 8 * (size_t)T.nnz * N, hipMemcpyHostToDevice));
+    HIP_TRY(d_rhs.alloc(n * N));
+    HIP_TRY(hipMemcpy(d_rhs.p, rhs, 8 * n * N, hipMemcpyHostToDevice));
+    HIP_TRY(zeroed(d_dx, n * N));
+    HIP_TRY(zeroed(d_lx, std::max<size_t>(T.nnz_l, 1) * N));
+    HIP_TRY(zeroed(d_ux, std::max<size_t>(T.nnz_u, 1) * N));
+    HIP_TRY(zeroed(d_udiag, n * N));
+    HIP_TRY(zeroed(d_w, n * N));
+    HIP_TRY(zeroed(d_y, n * N));
+    HIP_TRY(zeroed(d_fail, N));
+    LaneTab t{T.n, T.nnz, T.nnz_l, T.nnz_u, T.growth, T.wide, T.col_ptr, T.prow, T.q, T.pinv, T.lp, T.li, T.up, T.ui,
+              T.flags, T.void_slots, T.n_void_slots, 0};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(tab_refactor_kernel, dim3((n_lanes + 63) / 64), dim3(64), 0, 0, t, d_idx.p, n_lanes, d_a.p, d_rhs.p,
+                       d_dx.p, d_fail.p, d_lx.p, d_ux.p, d_udiag.p, d_w.p, d_y.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipDeviceSynchronize());
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    HIP_TRY(hipMemcpy(dx, d_dx.p, 8 * n * N, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(fail_out, d_fail.p, N, hipMemcpyDeviceToHost));
+    return ESPICE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+/* espice lane engine C ABI: many instances (lanes) of one circuit template
+ * solved together on an MI355X (gfx950), one lane per GPU thread.
+ *
+ * This is the seam the reference's lane driver calls in place of its Zig
+ * GPU lane backend: `GpuLanes` (src/analysis/gpu/lanes.zig) and its
+ * exported `arp_lane_*` kernels (src/solver/lane_device.zig:623-668).
+ * Plain pointers and sizes only; every call returns an espice_status
+ * (include/espice.h's codes); a lane's numerical failure is a per-lane
+ * result, never an error. Calls on one handle must be serialized; distinct
+ * handles are independent. No call keeps a caller pointer after it
+ * returns. The engine owns its device memory and one HIP stream per lane
+ * batch.
+ *
+ * Layout: lane-major, element i of lane l of N at i * N + l, as the
+ * reference's lane planes (lane_device.zig:1-6).
+ */
+#ifndef ESPICE_LANES_H
+#define ESPICE_LANES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef ESPICE_H
+typedef uint32_t espice_status;
+enum {
+    ESPICE_OK = 0, ESPICE_INVALID_ARGUMENT = 1, ESPICE_BUFFER_TOO_SMALL = 2,
+    ESPICE_OUT_OF_MEMORY = 3, ESPICE_INVALID_QUERY = 4,
+    ESPICE_RESULT_UNAVAILABLE = 5, ESPICE_FAILED = 6, ESPICE_ABI_MISMATCH = 7,
+    ESPICE_INVALID_HANDLE = 8
+};
+#endif
+
+/* Device kinds of a template (models/vsource.va, isource.va, resistor.va,
+ * capacitor.va, diode.va, mos1.va). */
+enum {
+    ESPICE_LANES_VSOURCE = 0, ESPICE_LANES_ISOURCE = 1, ESPICE_LANES_RESISTOR = 2,
+    ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5
+};
+
+typedef struct espice_lanes_template espice_lanes_template;
+typedef struct espice_lanes espice_lanes;
+
+/* converger.Options as the lane Newton reads it
+ * (src/solver/converger.zig:100-122); defaults reltol 1e-3, abstol 1e-12,
+ * vntol 1e-6, residual_tol 1e-9, gmin 0, max_iter 100
+ * (src/core/numerics.zig:378-399). init_fix 1: a cold operating point
+ * (MODEINITJCT seeds, then one INITFIX pass). */
+typedef struct {
+    double reltol, abstol, vntol, residual_tol, gmin;
+    uint32_t max_iter, init_fix;
+} espice_newton_opts;
+
+/* Per-lane outcome, as converger.LaneResult: status bit 0 converged,
+ * bit 1 singular (the lane's factor failed); iterations as
+ * converger.Result.iterations. */
+typedef struct {
+    uint32_t status, iterations;
+} espice_lane_result;
+
+/* `LaneTab` (src/solver/kernels/lane.zig:17-37): word offsets of a frozen
+ * LU tape inside one index buffer, plus its scalars. */
+typedef struct {
+    uint32_t n, nnz, nnz_l, nnz_u;
+    double growth, wide;
+    uint32_t col_ptr, prow, q, pinv, lp, li, up, ui, flags, void_slots, n_void_slots, pad;
+} espice_lane_tab;
+
+/* The thread's last error name (NUL-terminated); empty when none. */
+espice_status espice_lanes_error_message(char *buffer, size_t capacity, size_t *required);
+
+/* ---- template: a circuit frozen into one MNA pattern -------------------
+ * Replaces the frontend's builder for lane batches (frontend/builder.zig
+ * rowOf :559-565, addDevice :238-266; device/Circuit.zig freeze :112-201). */
+espice_status espice_lanes_template_create(espice_lanes_template **out);
+espice_status espice_lanes_template_destroy(espice_lanes_template *t);
+/* Row of a named net, allocated on first touch; "0" and "gnd" are row 0. */
+espice_status espice_lanes_template_node(espice_lanes_template *t, const char *name, uint32_t *row);
+/* One instance of kind on `ports` (rows); params holds the kind's user
+ * parameters (NaN = not given). Internal unknowns take fresh rows now. */
+espice_status espice_lanes_template_add(espice_lanes_template *t, uint32_t kind, const uint32_t *ports,
+    const double *params, size_t param_count, uint32_t *instance);
+/* Freezes the pattern: diagonals plus each device's structural entries,
+ * the diagonal slots, the BTF+AMD column order (src/solver/order.zig). */
+espice_status espice_lanes_template_freeze(espice_lanes_template *t);
+espice_status espice_lanes_template_info(const espice_lanes_template *t, uint32_t *n, uint32_t *nnz,
+    uint32_t *tridiagonal);
+/* Copies the frozen pattern; any pointer may be NULL. */
+espice_status espice_lanes_template_pattern(const espice_lanes_template *t, uint32_t *col_ptr, uint32_t *row_idx,
+    uint32_t *diag, uint8_t *current_row, uint32_t *q);
+
+/* ---- a batch of lanes on the GPU (GpuLanes.initStatic, lanes.zig:1080) - */
+espice_status espice_lanes_create(const espice_lanes_template *t, uint32_t n_lanes, espice_lanes **out);
+espice_status espice_lanes_destroy(espice_lanes *l);
+/* Returns every lane's solver to its fresh state, as direct.Solver.reset
+ * (src/solver/direct.zig:183-207): the next solve factors from scratch. */
+espice_status espice_lanes_reset(espice_lanes *l);
+/* values[n_lanes]: user parameter `param` of instance `instance` per lane. */
+espice_status espice_lanes_set_param(espice_lanes *l, uint32_t kind, uint32_t instance, uint32_t param,
+    const double *values);
+/* Temperature and geometry preparation of every instance on every lane,
+ * then the upload. Required after set_param and before a solve. */
+espice_status espice_lanes_prepare(espice_lanes *l);
+/* x[n * n_lanes] lane-major: each lane's start point. */
+espice_status espice_lanes_upload_x(espice_lanes *l, const double *x);
+espice_status espice_lanes_download_x(espice_lanes *l, double *x);
+/* The static (operating-point) Newton of every lane on the GPU, one
+ * kernel: the plain rung of op.solveGpuLanes (src/analysis/dc/op.zig:626).
+ * out[n_lanes]; kernel_ms (optional) receives the kernel's GPU time. */
+espice_status espice_lanes_newton_op(espice_lanes *l, const espice_newton_opts *opts, espice_lane_result *out,
+    double *kernel_ms);
+
+/* ---- the lead-tape lane LU (lane_device.zig refactor :101, solveNeg :170)
+ * Builds a LaneTab from one factor of (col_ptr, row_idx, vals) on the host
+ * with the template solver's threshold pivoting (lanes.zig:590-607 ownTape,
+ * kernels/lane.zig:778-802 buildTab). idx receives *len words (capacity
+ * idx_capacity; BUFFER_TOO_SMALL sets *len and copies nothing). FAILED when
+ * the matrix is singular. */
+espice_status espice_lanes_tab_build(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, const double *vals,
+    espice_lane_tab *tab, uint32_t *idx, size_t idx_capacity, size_t *len);
+/* Replays the tape on every lane on the GPU: a[nnz * n_lanes] and
+ * rhs[n * n_lanes] lane-major in; dx[n * n_lanes] = -A_l^-1 rhs_l and
+ * fail[n_lanes] (1 = the lane must peel: dead or non-finite pivot, growth,
+ * non-finite factor entry, multiplier past 1/pivot_tol, void slot) out. */
+espice_status espice_lanes_tab_refactor_solve(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
+    uint32_t n_lanes, const double *a, const double *rhs, double *dx, uint8_t *fail, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,695 @@
+/* Circuit template, planes and Newton drivers of the oracle. TEST
+ * INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ *   or_ckt_node/or_ckt_add  frontend/builder.zig:559-565 rowOf (first touch)
+ *                           and :238-266 addDevice (internal rows at add)
+ *   or_ckt_freeze           device/Circuit.zig:112-201 (diagonals, then each
+ *                           batch's structural entries; PatternBuilder.toCsc
+ *                           abi.zig:599-643 sorts and dedups per column;
+ *                           diag slots; slot tapes eval.zig:514-532, ground
+ *                           entries to the trash slot nnz)
+ *   eval_planes             analysis/Circuit.zig evalNewton: clear g and
+ *                           rhs, every batch in proto order (evalRange,
+ *                           eval.zig:780-925), then groundStamp :861-864
+ *   limit_pass              eval.zig:1137-1168 limitRange + applyLimits
+ *                           :2431-2436 (lim_active after the first call)
+ *   newton_lane (mode 0)    the GPU lane Newton: lanes.zig:1516-1601 ticks,
+ *                           lane_device.zig refactor :101, peelDense :120,
+ *                           solveNeg :170, assemble :282, update :316, gates
+ *                           :327, finalize :351 (lane 0's first-step tape,
+ *                           lanes.zig:590-607 ownTape)
+ *   newton_scalar (mode 1)  converger.zig:155-319 newton, finalizeStep
+ *                           :458-526, with direct.zig's Solver per lane
+ *   or_dc_sweep             analysis/dc/dc.zig:23-44 run, Sweep.beginPoint
+ *                           :453-490 (predictor), prepareWarm :493-497,
+ *                           warmOptions :441-443
+ * Parity pinned by: tests/test_oracle_newton.py (the diode_iv golden DC
+ * rows; mode 0 == mode 1 bitwise where the lead tape's pivots hold; KCL
+ * of every converged lane; FD Jacobian).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+#include "oracle_internal.h"
+
+#define NONE OR_NONE
+
+int or_solver_is_tri(const or_solver *s);
+
+static const int NU[OR_NKIND] = {3, 2, 2, 2, 2, 4};
+static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
+static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
+static const int PK[OR_NKIND] = {1, 1, 1, 1, DIOP_K, M1P_K};
+
+/* Structural pattern of each kind (row-major bit masks over local
+ * unknowns): G-plane entries and C-plane entries. */
+static unsigned gmask(int kind, int ru)
+{
+    switch (kind) {
+    case OR_VSRC: return ru == 2 ? 0x3u : 0x4u;
+    case OR_RES: case OR_DIODE: return 0x3u;
+    case OR_MOS1: return ru == 1 ? 0x0u : 0xfu;
+    default: return 0;
+    }
+}
+static unsigned cmask(int kind, int ru)
+{
+    switch (kind) {
+    case OR_CAP: return 0x3u;
+    case OR_MOS1: return 0xfu;
+    default: (void)ru; return 0;
+    }
+}
+static unsigned pmask(int kind, int ru) { return gmask(kind, ru) | cmask(kind, ru); }
+/* Residual rows a kind writes in a static evaluation. */
+static const unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x3u, 0xdu};
+
+typedef struct { uint32_t *p; size_t len, cap; } vu32;
+typedef struct { double *p; size_t len, cap; } vf64;
+static void vpush(vu32 *v, uint32_t x)
+{
+    if (v->len == v->cap) { v->cap = v->cap ? 2 * v->cap : 64; v->p = realloc(v->p, 4 * v->cap); }
+    v->p[v->len++] = x;
+}
+static void fpush(vf64 *v, double x)
+{
+    if (v->len == v->cap) { v->cap = v->cap ? 2 * v->cap : 64; v->p = realloc(v->p, 8 * v->cap); }
+    v->p[v->len++] = x;
+}
+
+struct or_circuit {
+    uint32_t n;
+    /* net name table (open addressing) */
+    char **names;
+    uint32_t *name_row;
+    size_t name_cap, name_len;
+    uint8_t *cur_flag;
+    size_t cur_cap;
+    vu32 nodes[OR_NKIND];
+    vf64 uprm[OR_NKIND];
+    uint32_t count[OR_NKIND];
+    int order[OR_NKIND];
+    int n_order;
+    /* frozen */
+    int frozen;
+    uint32_t nnz;
+    uint32_t *col_ptr, *row_idx, *diag;
+    uint8_t *cur_row;
+    uint32_t *slots[OR_NKIND];
+    uint32_t kslots[OR_NKIND];
+};
+
+or_circuit *or_ckt_new(void)
+{
+    or_circuit *c = calloc(1, sizeof(or_circuit));
+    c->n = 1; /* ground */
+    c->name_cap = 64;
+    c->names = calloc(c->name_cap, sizeof(char *));
+    c->name_row = calloc(c->name_cap, sizeof(uint32_t));
+    c->cur_cap = 64;
+    c->cur_flag = calloc(c->cur_cap, 1);
+    return c;
+}
+
+void or_ckt_free(or_circuit *c)
+{
+    if (!c) return;
+    for (size_t i = 0; i < c->name_cap; i++) free(c->names[i]);
+    free(c->names); free(c->name_row); free(c->cur_flag);
+    for (int k = 0; k < OR_NKIND; k++) { free(c->nodes[k].p); free(c->uprm[k].p); free(c->slots[k]); }
+    free(c->col_ptr); free(c->row_idx); free(c->diag); free(c->cur_row);
+    free(c);
+}
+
+static uint64_t hash_str(const char *s)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (; *s; s++) { h ^= (unsigned char)*s; h *= 1099511628211ull; }
+    return h;
+}
+
+static uint32_t add_row(or_circuit *c, int current)
+{
+    uint32_t r = c->n++;
+    if (r >= c->cur_cap) {
+        size_t nc = c->cur_cap * 2;
+        c->cur_flag = realloc(c->cur_flag, nc);
+        memset(c->cur_flag + c->cur_cap, 0, nc - c->cur_cap);
+        c->cur_cap = nc;
+    }
+    c->cur_flag[r] = (uint8_t)current;
+    return r;
+}
+
+uint32_t or_ckt_node(or_circuit *c, const char *name)
+{
+    if (strcmp(name, "0") == 0 || strcmp(name, "gnd") == 0) return 0;
+    if (2 * (c->name_len + 1) > c->name_cap) {
+        size_t oc = c->name_cap, ncap = oc * 2;
+        char **on = c->names;
+        uint32_t *orow = c->name_row;
+        c->names = calloc(ncap, sizeof(char *));
+        c->name_row = calloc(ncap, sizeof(uint32_t));
+        c->name_cap = ncap;
+        for (size_t i = 0; i < oc; i++)
+            if (on[i]) {
+                size_t h = hash_str(on[i]) & (ncap - 1);
+                while (c->names[h]) h = (h + 1) & (ncap - 1);
+                c->names[h] = on[i];
+                c->name_row[h] = orow[i];
+            }
+        free(on);
+        free(orow);
+    }
+    size_t h = hash_str(name) & (c->name_cap - 1);
+    while (c->names[h]) {
+        if (strcmp(c->names[h], name) == 0) return c->name_row[h];
+        h = (h + 1) & (c->name_cap - 1);
+    }
+    c->names[h] = strdup(name);
+    c->name_row[h] = add_row(c, 0);
+    c->name_len++;
+    return c->name_row[h];
+}
+
+int32_t or_ckt_add(or_circuit *c, int kind, const uint32_t *ports, const double *params)
+{
+    if (kind < 0 || kind >= OR_NKIND || c->frozen) return -1;
+    int seen = 0;
+    for (int i = 0; i < c->n_order; i++) seen |= c->order[i] == kind;
+    if (!seen) c->order[c->n_order++] = kind;
+    for (int p = 0; p < NPORT[kind]; p++) vpush(&c->nodes[kind], ports[p]);
+    for (int u = NPORT[kind]; u < NU[kind]; u++) vpush(&c->nodes[kind], add_row(c, kind == OR_VSRC));
+    for (int k = 0; k < UK[kind]; k++) fpush(&c->uprm[kind], params ? params[k] : NAN);
+    return (int32_t)c->count[kind]++;
+}
+
+static int cmp_u64(const void *a, const void *b)
+{
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+static uint32_t find_slot(const or_circuit *c, uint32_t row, uint32_t col)
+{
+    uint32_t lo = c->col_ptr[col], hi = c->col_ptr[col + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (c->row_idx[mid] < row) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < c->col_ptr[col + 1] && c->row_idx[lo] == row) ? lo : NONE;
+}
+
+int or_ckt_freeze(or_circuit *c)
+{
+    const uint32_t n = c->n;
+    size_t nk = n;
+    for (int t = 0; t < c->n_order; t++) {
+        int kind = c->order[t];
+        nk += (size_t)c->count[kind] * NU[kind] * NU[kind];
+    }
+    uint64_t *keys = malloc(8 * nk);
+    size_t m = 0;
+    for (uint32_t i = 0; i < n; i++) keys[m++] = ((uint64_t)i << 32) | i;
+    for (int t = 0; t < c->n_order; t++) {
+        int kind = c->order[t];
+        for (uint32_t id = 0; id < c->count[kind]; id++) {
+            const uint32_t *nd = c->nodes[kind].p + (size_t)id * NU[kind];
+            for (int ru = 0; ru < NU[kind]; ru++)
+                for (int cu = 0; cu < NU[kind]; cu++)
+                    if ((pmask(kind, ru) >> cu & 1) && nd[ru] != 0 && nd[cu] != 0)
+                        keys[m++] = ((uint64_t)nd[cu] << 32) | nd[ru];
+        }
+    }
+    qsort(keys, m, 8, cmp_u64);
+    c->col_ptr = calloc(n + 1, 4);
+    c->row_idx = malloc(4 * m);
+    uint32_t nnz = 0;
+    for (size_t i = 0; i < m; i++) {
+        if (i > 0 && keys[i] == keys[i - 1]) continue;
+        uint32_t col = (uint32_t)(keys[i] >> 32);
+        c->row_idx[nnz++] = (uint32_t)keys[i];
+        c->col_ptr[col + 1]++;
+    }
+    for (uint32_t j = 0; j < n; j++) c->col_ptr[j + 1] += c->col_ptr[j];
+    free(keys);
+    c->nnz = nnz;
+    c->diag = malloc(4 * n);
+    for (uint32_t i = 0; i < n; i++) c->diag[i] = find_slot(c, i, i);
+    c->cur_row = malloc(n);
+    memcpy(c->cur_row, c->cur_flag, n);
+    for (int kind = 0; kind < OR_NKIND; kind++) {
+        uint32_t ks = 0;
+        for (int ru = 0; ru < NU[kind]; ru++) ks += (uint32_t)__builtin_popcount(pmask(kind, ru));
+        c->kslots[kind] = ks;
+        c->slots[kind] = malloc(4 * ((size_t)c->count[kind] * ks + 1));
+        size_t k = 0;
+        for (uint32_t id = 0; id < c->count[kind]; id++) {
+            const uint32_t *nd = c->nodes[kind].p + (size_t)id * NU[kind];
+            for (int ru = 0; ru < NU[kind]; ru++)
+                for (int cu = 0; cu < NU[kind]; cu++)
+                    if (pmask(kind, ru) >> cu & 1)
+                        c->slots[kind][k++] = (nd[ru] == 0 || nd[cu] == 0) ? nnz : find_slot(c, nd[ru], nd[cu]);
+        }
+    }
+    c->frozen = 1;
+    return 0;
+}
+
+uint32_t or_ckt_n(const or_circuit *c) { return c->n; }
+uint32_t or_ckt_nnz(const or_circuit *c) { return c->nnz; }
+const uint32_t *or_ckt_col_ptr(const or_circuit *c) { return c->col_ptr; }
+const uint32_t *or_ckt_row_idx(const or_circuit *c) { return c->row_idx; }
+const uint32_t *or_ckt_diag(const or_circuit *c) { return c->diag; }
+const uint8_t *or_ckt_cur_row(const or_circuit *c) { return c->cur_row; }
+uint32_t or_ckt_count(const or_circuit *c, int kind) { return c->count[kind]; }
+
+/* ------------------------------------------------------------------ lanes */
+
+struct or_lanes {
+    const or_circuit *c;
+    uint32_t N;
+    double *uprm[OR_NKIND]; /* [(inst*UK + k)*N + l] */
+    double *pprm[OR_NKIND]; /* [(inst*PK + k)*N + l] */
+    /* limiter state per lane: [inst*NU + u] per kind, lane-minor arrays */
+    double *lim[OR_NKIND];
+};
+
+or_lanes *or_lanes_new(const or_circuit *c, uint32_t N)
+{
+    or_lanes *L = calloc(1, sizeof(or_lanes));
+    L->c = c;
+    L->N = N;
+    for (int kind = 0; kind < OR_NKIND; kind++) {
+        size_t cnt = c->count[kind];
+        L->uprm[kind] = malloc(8 * (cnt * UK[kind] * N + 1));
+        L->pprm[kind] = calloc(cnt * PK[kind] * N + 1, 8);
+        L->lim[kind] = calloc(cnt * NU[kind] * N + 1, 8);
+        for (size_t i = 0; i < cnt; i++)
+            for (int k = 0; k < UK[kind]; k++)
+                for (uint32_t l = 0; l < N; l++)
+                    L->uprm[kind][(i * UK[kind] + k) * N + l] = c->uprm[kind].p[i * UK[kind] + k];
+    }
+    return L;
+}
+
+void or_lanes_free(or_lanes *L)
+{
+    if (!L) return;
+    for (int k = 0; k < OR_NKIND; k++) { free(L->uprm[k]); free(L->pprm[k]); free(L->lim[k]); }
+    free(L);
+}
+
+void or_lanes_set_param(or_lanes *L, int kind, uint32_t inst, uint32_t k, const double *values)
+{
+    for (uint32_t l = 0; l < L->N; l++) L->uprm[kind][((size_t)inst * UK[kind] + k) * L->N + l] = values[l];
+}
+
+void or_lanes_prepare(or_lanes *L)
+{
+    const uint32_t N = L->N;
+    for (int kind = 0; kind < OR_NKIND; kind++) {
+        uint32_t cnt = L->c->count[kind];
+        double u[64], p[64];
+        for (uint32_t i = 0; i < cnt; i++)
+            for (uint32_t l = 0; l < N; l++) {
+                for (int k = 0; k < UK[kind]; k++) u[k] = L->uprm[kind][((size_t)i * UK[kind] + k) * N + l];
+                switch (kind) {
+                case OR_MOS1: or_prep_mos1(u, p); break;
+                case OR_DIODE: or_prep_diode(u, p); break;
+                case OR_VSRC: p[0] = u[OR_VSRC_DC] == u[OR_VSRC_DC] ? u[OR_VSRC_DC] : 0.0; break;
+                case OR_ISRC: p[0] = u[OR_ISRC_DC] == u[OR_ISRC_DC] ? u[OR_ISRC_DC] : 0.0; break;
+                case OR_RES: p[0] = 1.0 / u[OR_RES_R]; break;
+                case OR_CAP: p[0] = u[OR_CAP_C]; break;
+                }
+                for (int k = 0; k < PK[kind]; k++) L->pprm[kind][((size_t)i * PK[kind] + k) * N + l] = p[k];
+            }
+    }
+}
+
+/* ------------------------------------------------------------ evaluation */
+
+static void load_p(const or_lanes *L, int kind, uint32_t id, uint32_t l, double *p)
+{
+    for (int k = 0; k < PK[kind]; k++) p[k] = L->pprm[kind][((size_t)id * PK[kind] + k) * L->N + l];
+}
+
+static double *lim_at(const or_lanes *L, int kind, uint32_t id, uint32_t l)
+{
+    return L->lim[kind] + ((size_t)l * L->c->count[kind] + id) * NU[kind];
+}
+
+/* g (nnz+1 with trash), rhs (n+1 with trash row n) at x. */
+static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs)
+{
+    const or_circuit *c = L->c;
+    memset(g, 0, 8 * (c->nnz + 1));
+    memset(rhs, 0, 8 * (c->n + 1));
+    for (int t = 0; t < c->n_order; t++) {
+        int kind = c->order[t];
+        const int nu = NU[kind];
+        for (uint32_t id = 0; id < c->count[kind]; id++) {
+            const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
+            const uint32_t *sl = c->slots[kind] + (size_t)id * c->kslots[kind];
+            double p[64], lx[4], xl[4];
+            or_devout o;
+            memset(&o, 0, sizeof o);
+            load_p(L, kind, id, l, p);
+            for (int u = 0; u < nu; u++) xl[u] = lx[u] = x[nd[u]];
+            int has_lim = kind == OR_MOS1 || kind == OR_DIODE;
+            if (has_lim && lim_active) {
+                const double *lm = lim_at(L, kind, id, l);
+                unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+                for (int u = 0; u < nu; u++)
+                    if (writes >> u & 1) lx[u] = lm[u];
+            }
+            switch (kind) {
+            case OR_VSRC:
+                o.res[0] = lx[2];
+                o.res[1] = -lx[2];
+                o.res[2] = lx[0] - lx[1] - p[0];
+                o.jac[0][2] = 1.0;
+                o.jac[1][2] = -1.0;
+                o.jac[2][0] = 1.0;
+                o.jac[2][1] = -1.0;
+                break;
+            case OR_ISRC:
+                o.res[0] = p[0];
+                o.res[1] = -p[0];
+                break;
+            case OR_RES: {
+                double i = p[0] * (lx[0] - lx[1]);
+                o.res[0] = i;
+                o.res[1] = -i;
+                o.jac[0][0] = p[0];
+                o.jac[0][1] = -p[0];
+                o.jac[1][0] = -p[0];
+                o.jac[1][1] = p[0];
+                break;
+            }
+            case OR_CAP: break;
+            case OR_DIODE: or_eval_diode(p, lx, &o); break;
+            case OR_MOS1: or_eval_mos1(p, lx, &o); break;
+            }
+            /* limiting correction: F += J (x - x_lim) on every row */
+            if (has_lim && lim_active) {
+                double corr[4];
+                int live = 0;
+                for (int u = 0; u < nu; u++) {
+                    corr[u] = xl[u] - lx[u];
+                    live |= corr[u] != 0;
+                }
+                if (live)
+                    for (int ru = 0; ru < nu; ru++) {
+                        double acc = 0;
+                        for (int u = 0; u < nu; u++) acc += o.jac[ru][u] * corr[u];
+                        o.res[ru] += acc;
+                    }
+            }
+            size_t k = 0;
+            for (int ru = 0; ru < nu; ru++) {
+                uint32_t row = nd[ru] == 0 ? c->n : nd[ru];
+                if (RMASK[kind] >> ru & 1) rhs[row] += o.res[ru];
+                for (int cu = 0; cu < nu; cu++) {
+                    if (!(pmask(kind, ru) >> cu & 1)) continue;
+                    if (gmask(kind, ru) >> cu & 1) g[sl[k]] += o.jac[ru][cu];
+                    k++;
+                }
+            }
+        }
+    }
+    /* groundStamp */
+    g[c->diag[0]] += 1.0;
+    rhs[0] += x[0];
+}
+
+/* Limit pass on lane l: cur = x, old = previous image or x_old. Returns 1
+ * when any instance clamped (not converged). */
+static int limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old, int lim_active)
+{
+    const or_circuit *c = L->c;
+    int flag = 0;
+    for (int t = 0; t < c->n_order; t++) {
+        int kind = c->order[t];
+        if (kind != OR_MOS1 && kind != OR_DIODE) continue;
+        const int nu = NU[kind];
+        unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+        for (uint32_t id = 0; id < c->count[kind]; id++) {
+            const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
+            double p[64], cur[4], old[4], lim[4];
+            double *lm = lim_at(L, kind, id, l);
+            load_p(L, kind, id, l, p);
+            for (int u = 0; u < nu; u++) {
+                cur[u] = x[nd[u]];
+                old[u] = ((writes >> u & 1) && lim_active) ? lm[u] : x_old[nd[u]];
+            }
+            int ch = kind == OR_MOS1 ? or_limit_mos1(p, cur, old, lim) : or_limit_diode(p, cur, old, lim);
+            if (ch) flag = 1;
+            for (int u = 0; u < nu; u++)
+                if (writes >> u & 1) lm[u] = lim[u];
+        }
+    }
+    return flag;
+}
+
+static void seed_pass(or_lanes *L, uint32_t l, const double *x)
+{
+    const or_circuit *c = L->c;
+    for (int t = 0; t < c->n_order; t++) {
+        int kind = c->order[t];
+        if (kind != OR_MOS1 && kind != OR_DIODE) continue;
+        const int nu = NU[kind];
+        unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+        for (uint32_t id = 0; id < c->count[kind]; id++) {
+            const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
+            double p[64], cur[4], lim[4];
+            double *lm = lim_at(L, kind, id, l);
+            load_p(L, kind, id, l, p);
+            for (int u = 0; u < nu; u++) cur[u] = x[nd[u]];
+            if (kind == OR_MOS1) or_seed_mos1(p, cur, lim);
+            else or_seed_diode(p, cur, lim);
+            for (int u = 0; u < nu; u++)
+                if (writes >> u & 1) lm[u] = lim[u];
+        }
+    }
+}
+
+void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double *vals)
+{
+    const or_circuit *c = L->c;
+    double *g = malloc(8 * (c->nnz + 1)), *r = malloc(8 * (c->n + 1));
+    eval_planes(L, l, x, 0, g, r);
+    memcpy(rhs, r, 8 * c->n);
+    memcpy(vals, g, 8 * c->nnz);
+    free(g);
+    free(r);
+}
+
+/* ---------------------------------------------------------------- newton */
+
+/* Gate bits, kernels/lane.zig:703-716. */
+#define G_INITFIX 2u
+#define G_CONVERGED 4u
+#define G_SINGULAR 8u
+#define G_PASSED 16u
+
+/* Shared per-lane Newton over an abstract linear step. `lin` returns 0 and
+ * dx on success, nonzero for a singular lane. */
+typedef int (*lin_fn)(void *ctx, uint32_t l, const double *a, const double *rhs, double *dx);
+
+static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double *x, or_outcome *out, lin_fn lin, void *lctx, int lim_active)
+{
+    const or_circuit *c = L->c;
+    const uint32_t n = c->n, nnz = c->nnz;
+    double *g = malloc(8 * (nnz + 1)), *rhs = malloc(8 * (n + 1));
+    double *dx = malloc(8 * n), *x_old = malloc(8 * n);
+    uint32_t bits = o->init_fix ? G_INITFIX : 0;
+    double limit = 1e9;
+    memcpy(x_old, x, 8 * n);
+    out->status = 0;
+    out->iterations = o->max_iter;
+    for (uint32_t tick = 0; tick < o->max_iter; tick++) {
+        eval_planes(L, l, x, lim_active, g, rhs);
+        if (o->gmin > 0) {
+            for (uint32_t i = 0; i < n; i++) {
+                g[c->diag[i]] += o->gmin;
+            }
+            for (uint32_t i = 0; i < n; i++) rhs[i] += o->gmin * x[i];
+        }
+        if (tick == 0) limit = or_lane_plausible_limit(n, x, rhs, g, c->diag, 1, 0);
+        if (lin(lctx, l, g, rhs, dx) != 0) {
+            if (getenv("OR_DEBUG")) {
+                fprintf(stderr, "lane %u tick %u singular\n", l, tick);
+                for (uint32_t p = 0; p < nnz; p++) fprintf(stderr, " a[%u]=%g", p, g[p]);
+                fprintf(stderr, "\n");
+                for (uint32_t i = 0; i < n; i++) fprintf(stderr, " x[%u]=%g rhs=%g", i, x[i], rhs[i]);
+                fprintf(stderr, "\n");
+            }
+            out->status = 2;
+            out->iterations = 0;
+            break;
+        }
+        double scaled = or_lane_update_norm(n, x, dx, x_old, c->cur_row, o->reltol, o->abstol, o->vntol, 1, 0);
+        int limited = limit_pass(L, l, x, x_old, lim_active);
+        lim_active = 1;
+        int ok = !limited && tick != 0 && !(scaled >= 1.0);
+        if (getenv("OR_DEBUG")) fprintf(stderr, "lane %u tick %u scaled %g limited %d x1 %g\n", l, tick, scaled, limited, x[1]);
+        if (ok) ok = or_lane_plausible(n, x, limit, 1, 0);
+        if (ok) ok = or_lane_residual(n, o->reltol, o->vntol, o->residual_tol, x, rhs, g, c->diag, 1, 0);
+        if (ok && (bits & G_INITFIX)) {
+            bits &= ~G_INITFIX;
+        } else if (ok) {
+            memcpy(x, x_old, 8 * n);
+            out->status = 1;
+            out->iterations = tick + 1;
+            break;
+        }
+    }
+    free(g); free(rhs); free(dx); free(x_old);
+}
+
+/* mode 0: replay of the lead tape; a lane whose replay fails (dead or
+ * non-finite pivot, growth, wide multiplier, void slot) factors on its own
+ * direct.Solver, the host peel of lanes.zig:1603-1653, which the engine
+ * runs on the device for every n. */
+typedef struct {
+    const or_lane_tab *tab;
+    const uint32_t *idx;
+    double *lx, *ux, *udiag, *w, *y;
+    const or_circuit *c;
+    or_solver *own;
+    uint32_t peels;
+} tape_ctx;
+
+static int lin_tape(void *vctx, uint32_t l, const double *a, const double *rhs, double *dx)
+{
+    (void)l;
+    tape_ctx *t = vctx;
+    if (or_lane_refactor(t->tab, t->idx, a, t->lx, t->ux, t->udiag, t->w, 1, 0)) {
+        if (!t->own) t->own = or_solver_new(t->c->n, t->c->col_ptr, t->c->row_idx);
+        t->peels++;
+        if (or_solver_factor(t->own, a) != 0) return 1;
+        or_solver_solve_neg(t->own, rhs, dx);
+        return 0;
+    }
+    or_lane_solve_neg(t->tab, t->idx, t->lx, t->ux, t->udiag, rhs, t->y, dx, 1, 0);
+    return 0;
+}
+
+static int lin_direct(void *vctx, uint32_t l, const double *a, const double *rhs, double *dx)
+{
+    (void)l;
+    or_solver *s = vctx;
+    if (or_solver_factor(s, a) != 0) return 1;
+    or_solver_solve_neg(s, rhs, dx);
+    return 0;
+}
+
+int or_lead_tab(or_lanes *L, const double *x, int init_fix, or_lane_tab *tab, uint32_t **idx, size_t *len)
+{
+    const or_circuit *c = L->c;
+    const uint32_t N = L->N, n = c->n;
+    double *x0 = malloc(8 * n);
+    for (uint32_t i = 0; i < n; i++) x0[i] = x[(size_t)i * N];
+    if (init_fix) seed_pass(L, 0, x0);
+    double *g = malloc(8 * (c->nnz + 1)), *rhs = malloc(8 * (n + 1));
+    eval_planes(L, 0, x0, init_fix, g, rhs);
+    or_solver *s = or_solver_new(n, c->col_ptr, c->row_idx);
+    int rc = -1;
+    if (s && or_solver_factor(s, g) == 0 && !or_solver_is_tri(s)) {
+        *idx = or_build_tab(or_solver_lu(s), c->col_ptr, or_solver_q(s), OR_GROWTH_LIMIT, OR_PIVOT_TOL, tab, len);
+        rc = 0;
+    }
+    or_solver_free(s);
+    free(x0); free(g); free(rhs);
+    return rc;
+}
+
+int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out)
+{
+    const or_circuit *c = L->c;
+    const uint32_t N = L->N, n = c->n;
+    double *xl = malloc(8 * n);
+    or_lane_tab tab;
+    uint32_t *idx = NULL;
+    size_t len = 0;
+    if (mode == 0 && or_lead_tab(L, x, o->init_fix, &tab, &idx, &len) != 0) {
+        free(xl);
+        return -1;
+    }
+    for (uint32_t l = 0; l < N; l++) {
+        for (uint32_t i = 0; i < n; i++) xl[i] = x[(size_t)i * N + l];
+        if (o->init_fix) seed_pass(L, l, xl);
+        if (mode == 0) {
+            tape_ctx t = {&tab, idx, NULL, NULL, NULL, NULL, NULL, c, NULL, 0};
+            t.lx = calloc(tab.nnz_l + 1, 8);
+            t.ux = calloc(tab.nnz_u + 1, 8);
+            t.udiag = calloc(n, 8);
+            t.w = calloc(n, 8);
+            t.y = calloc(n, 8);
+            newton_one(L, l, o, xl, &out[l], lin_tape, &t, o->init_fix);
+            if (t.peels) out[l].status |= 4u;
+            or_solver_free(t.own);
+            free(t.lx); free(t.ux); free(t.udiag); free(t.w); free(t.y);
+        } else {
+            or_solver *s = or_solver_new(n, c->col_ptr, c->row_idx);
+            if (!s) {
+                free(xl);
+                return -1;
+            }
+            newton_one(L, l, o, xl, &out[l], lin_direct, s, o->init_fix);
+            or_solver_free(s);
+        }
+        for (uint32_t i = 0; i < n; i++) x[(size_t)i * N + l] = xl[i];
+    }
+    free(idx);
+    free(xl);
+    return 0;
+}
+
+/* ------------------------------------------------------------- dc sweep */
+
+int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t npts, const or_newton_opts *o, double *x_out, uint32_t *converged)
+{
+    const or_circuit *c = L->c;
+    const uint32_t n = c->n, N = L->N;
+    double *sol = calloc(n, 8), *prev = calloc(n, 8);
+    or_solver *s = or_solver_new(n, c->col_ptr, c->row_idx);
+    if (!s) return -1;
+    double v = start;
+    or_newton_opts warm = *o;
+    warm.init_fix = 0;
+    for (uint32_t pt = 0; pt < npts; pt++) {
+        if (pt != 0) v += step;
+        for (uint32_t l = 0; l < N; l++) L->pprm[OR_VSRC][(size_t)vsrc * N + l] = v;
+        for (uint32_t i = 0; i < n; i++) {
+            double last = sol[i];
+            sol[i] = 2.0 * last - prev[i];
+            prev[i] = last;
+        }
+        or_outcome oc = {0, 0};
+        if (pt != 0) {
+            /* prepareWarm: limit the predictor against the last point */
+            limit_pass(L, 0, sol, prev, 0);
+            newton_one(L, 0, &warm, sol, &oc, lin_direct, s, 1);
+        }
+        if (!(oc.status & 1)) {
+            /* cold point: the plain operating-point rung from zero */
+            memset(sol, 0, 8 * n);
+            or_newton_opts cold = *o;
+            cold.init_fix = 1;
+            seed_pass(L, 0, sol);
+            newton_one(L, 0, &cold, sol, &oc, lin_direct, s, 1);
+        }
+        converged[pt] = oc.status & 1;
+        memcpy(x_out + (size_t)pt * n, sol, 8 * n);
+    }
+    or_solver_free(s);
+    free(sol);
+    free(prev);
+    return 0;
+}

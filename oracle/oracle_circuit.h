@@ -1,0 +1,86 @@
+/* Circuit template, devices and Newton drivers of the oracle. TEST
+ * INFRASTRUCTURE ONLY (see oracle.h). Included by oracle.h. */
+#ifndef ESPICE_ORACLE_CIRCUIT_H
+#define ESPICE_ORACLE_CIRCUIT_H
+
+#include <stdint.h>
+
+/* Device kinds, in the fixed order their parameter tables are declared.
+ * Evaluation order is the order kinds first appear in the netlist
+ * (src/device/Circuit.zig: one batch per proto, protos in first-use
+ * order), instances in card order within a kind. */
+enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_NKIND = 6 };
+
+/* User parameter slots per kind (NaN = not given, the default applies). */
+enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC_TR, OR_VSRC_TF, OR_VSRC_PW, OR_VSRC_PER, OR_VSRC_K };
+enum { OR_ISRC_DC = 0, OR_ISRC_K };
+enum { OR_RES_R = 0, OR_RES_K };
+enum { OR_CAP_C = 0, OR_CAP_K };
+enum { OR_DIO_IS = 0, OR_DIO_N, OR_DIO_CJO, OR_DIO_VJ, OR_DIO_M, OR_DIO_FC, OR_DIO_TT, OR_DIO_GMIN, OR_DIO_AREA,
+       OR_DIO_TNOM, OR_DIO_TEMP, OR_DIO_DTEMP, OR_DIO_EG, OR_DIO_XTI, OR_DIO_K };
+enum { OR_M1_TYPE = 0, OR_M1_VTO, OR_M1_KP, OR_M1_GAMMA, OR_M1_PHI, OR_M1_LAMBDA, OR_M1_TOX, OR_M1_LD, OR_M1_U0,
+       OR_M1_NSUB, OR_M1_NSS, OR_M1_TPG, OR_M1_IS, OR_M1_JS, OR_M1_CBD, OR_M1_CBS, OR_M1_CJ, OR_M1_CJSW, OR_M1_PB,
+       OR_M1_MJ, OR_M1_MJSW, OR_M1_FC, OR_M1_CGSO, OR_M1_CGDO, OR_M1_CGBO, OR_M1_TNOM, OR_M1_GMIN, OR_M1_TEMP,
+       OR_M1_DTEMP, OR_M1_L, OR_M1_W, OR_M1_AD, OR_M1_AS, OR_M1_PD, OR_M1_PS, OR_M1_M, OR_M1_K };
+
+typedef struct or_circuit or_circuit;
+typedef struct or_lanes or_lanes;
+
+or_circuit *or_ckt_new(void);
+void or_ckt_free(or_circuit *c);
+/* Row of a named net, allocated on first touch ("0" and "gnd" are row 0),
+ * as frontend/builder.zig:559-565 rowOf does. */
+uint32_t or_ckt_node(or_circuit *c, const char *name);
+/* Adds one instance; `ports` are rows from or_ckt_node; internal unknowns
+ * (the vsource branch current) get fresh rows now, builder.zig:238-266.
+ * Returns the instance index within its kind, or -1. */
+int32_t or_ckt_add(or_circuit *c, int kind, const uint32_t *ports, const double *params);
+/* Builds the CSC pattern (every diagonal, then each device's structural
+ * entries, device/Circuit.zig:112-201), diag slots and slot tapes. */
+int or_ckt_freeze(or_circuit *c);
+uint32_t or_ckt_n(const or_circuit *c);
+uint32_t or_ckt_nnz(const or_circuit *c);
+const uint32_t *or_ckt_col_ptr(const or_circuit *c);
+const uint32_t *or_ckt_row_idx(const or_circuit *c);
+const uint32_t *or_ckt_diag(const or_circuit *c);
+const uint8_t *or_ckt_cur_row(const or_circuit *c);
+uint32_t or_ckt_count(const or_circuit *c, int kind);
+
+or_lanes *or_lanes_new(const or_circuit *c, uint32_t n_lanes);
+void or_lanes_free(or_lanes *L);
+/* Per-lane user parameter values (n_lanes entries). */
+void or_lanes_set_param(or_lanes *L, int kind, uint32_t inst, uint32_t k, const double *values);
+/* Temperature/geometry preparation of every instance on every lane. */
+void or_lanes_prepare(or_lanes *L);
+
+/* Newton options (converger.Options subset). */
+typedef struct {
+    double reltol, abstol, vntol, residual_tol, gmin;
+    uint32_t max_iter;
+    uint32_t init_fix;
+} or_newton_opts;
+
+/* Lane outcome: bit 0 converged, bit 1 singular; iterations as
+ * converger.Result.iterations. */
+typedef struct { uint32_t status; uint32_t iterations; } or_outcome;
+
+/* Static (OP) Newton on every lane from x (lane-major n*N, in/out).
+ * mode 0: the GPU lane algorithm (lane 0's tape, lane bodies, dense peel),
+ *         src/analysis/gpu/lanes.zig newton/ticks + solver/lane_device.zig.
+ * mode 1: the reference CPU path per lane, converger.newton with its own
+ *         direct.Solver (full factor, refactor, replay probe).
+ * Returns 0, or -1 when the template has no lane tape (tridiagonal
+ * pattern or a singular lead factor). */
+int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out);
+/* The lead tape of lane 0 at x (mode 0's tape), for tests. */
+int or_lead_tab(or_lanes *L, const double *x, int init_fix, or_lane_tab *tab, uint32_t **idx, size_t *len);
+/* One residual+Jacobian evaluation of lane l at x (n entries), unlimited,
+ * ground pin included: rhs (n), vals (nnz). For the FD Jacobian tests. */
+void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double *vals);
+
+/* `.dc` sweep of one vsource's dc value on lane 0 (src/analysis/dc/dc.zig
+ * run/Sweep: cold first point, predictor 2 x_k - x_{k-1} limited against
+ * the last point, warm Newton per point). x_out: npts*n, point-major. */
+int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t npts, const or_newton_opts *o, double *x_out, uint32_t *converged);
+
+#endif

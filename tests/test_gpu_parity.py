@@ -1,0 +1,159 @@
+"""GPU parity of the lane engine against the CPU oracle, through the C ABI.
+
+Integer and index work must match bit for bit; the floating-point lane
+results must too, since the engine runs the oracle's operation order with
+contraction off (tolerance: 0 ulp). Each case prints its name and sizes
+before touching the GPU.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import egspice_amd as E
+from egspice_amd import workloads as W
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def build_both(oracle, builder, *args):
+    t = E.Template()
+    builder(t.add, *args)
+    t.freeze()
+    c = O.Circuit(oracle)
+    builder(c.add, *args)
+    c.freeze()
+    assert (t.col_ptr == c.col_ptr).all() and (t.row_idx == c.row_idx).all()
+    return t, c
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a, dtype=np.float64).view(np.uint64), np.asarray(b, dtype=np.float64).view(np.uint64))
+
+
+def test_config1_dc_sweep_lanes_bitwise(oracle, engine):
+    say("case: inverter chain 5 stages, 64 lanes, cold OP, engine vs oracle CPU path")
+    t, c = build_both(oracle, W.inverter_chain)
+    vin = W.dc_sweep_points(64)
+    N = len(vin)
+    gl = E.Lanes(t, N)
+    gl.set_param(E.VSOURCE, 1, E.VSOURCE_DC, vin)
+    gl.prepare()
+    xg, sg, ig, ms = gl.newton_op()
+    ol = O.Lanes(c, N)
+    ol.set_param(O.VSRC, 1, O.VSRC_DC, vin)
+    ol.prepare()
+    xo, so, io = ol.newton_op(mode=1)
+    say(f"  converged gpu {int((sg & 1).sum())} oracle {int((so & 1).sum())} kernel {ms:.3f} ms")
+    assert (so & 1).all()
+    assert (sg == so).all()
+    assert (ig == io).all()
+    assert bits_equal(xg, xo)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mc_lanes_bitwise(oracle, engine, seed):
+    N = 1000
+    say(f"case: inverter chain Monte Carlo, {N} lanes, seed {seed}")
+    t, c = build_both(oracle, W.inverter_chain)
+    rng = np.random.default_rng(seed)
+    vin = rng.uniform(0.0, 1.8, N)
+    vto_n = 0.7 + 0.015 * rng.standard_normal(N)
+    tox = 1e-7 * (1 + 0.02 * rng.standard_normal(N))
+    gl = E.Lanes(t, N)
+    ol = O.Lanes(c, N)
+    for L, kv, km in ((gl, E.VSOURCE, E.MOS1), (ol, O.VSRC, O.MOS1)):
+        L.set_param(kv, 1, 0, vin)
+        L.set_param(km, 0, O.M1["VTO"], vto_n)
+        L.set_param(km, 2, O.M1["TOX"], tox)
+        L.prepare()
+    xg, sg, ig, ms = gl.newton_op()
+    xo, so, io = ol.newton_op(mode=1)
+    say(f"  converged gpu {int((sg & 1).sum())} oracle {int((so & 1).sum())} kernel {ms:.3f} ms")
+    assert (sg == so).all()
+    assert (ig == io).all()
+    assert bits_equal(xg, xo)
+
+
+def random_system(rng, n, density):
+    """sparse_lu.zig:1380-1410 TestSystem: full diagonal, off-diagonals at
+    `density`, diagonal pushed past its column sum except a zero (branch
+    row) diagonal 15% of the time."""
+    col_ptr, row_idx, vals = [0], [], []
+    for j in range(n):
+        rows = [i for i in range(n) if i == j or rng.random() < density]
+        col = [2 * rng.random() - 1 for _ in rows]
+        d = rows.index(j)
+        s = sum(abs(v) for i, v in enumerate(col) if i != d)
+        col[d] = 0.0 if (s > 0 and rng.random() < 0.15) else col[d] + np.copysign(s, col[d])
+        row_idx += rows
+        vals += col
+        col_ptr.append(len(row_idx))
+    return np.array(col_ptr, np.uint32), np.array(row_idx, np.uint32), np.array(vals)
+
+
+def test_lead_tape_replay_bitwise(oracle, engine):
+    """lane.zig:811-908: every lane's replay of the lead tape is bitwise the
+    oracle's body, failing exactly where it fails."""
+    rng = np.random.default_rng(0x1A7E51)
+    W_ = 64
+    checked = fails = 0
+    for trial in range(40):
+        n = 2 + trial % 20
+        cp, ri, v = random_system(rng, n, 0.3)
+        try:
+            tab, idx = E.tab_build(n, cp, ri, v)
+        except E.EngineError:
+            continue
+        # oracle tape from the same factor
+        s = oracle.or_solver_new(n, O.u32(cp)[1], O.u32(ri)[1])
+        if oracle.or_solver_factor(s, O.f64(v)[1]) != 0:
+            oracle.or_solver_free(s)
+            continue
+        import ctypes as C
+        otab = O.LaneTab()
+        ln = C.c_size_t()
+        oidx_p = oracle.or_build_tab(oracle.or_solver_lu(s), O.u32(cp)[1], oracle.or_solver_q(s), 1e-12, 1e-3,
+                                     C.byref(otab), C.byref(ln))
+        oidx = np.ctypeslib.as_array(oidx_p, (ln.value,)).copy()
+        oracle.free_(C.cast(oidx_p, C.c_void_p))
+        oracle.or_solver_free(s)
+        assert np.array_equal(oidx, idx), f"tape words differ at n={n}"
+        nnz = len(v)
+        a = np.repeat(v[:, None], W_, axis=1)
+        for l in range(W_):
+            kind = rng.integers(6)
+            if kind == 0:
+                a[rng.integers(nnz), l] = 4 * rng.random() - 2
+            elif kind == 1:
+                col = rng.integers(n)
+                a[cp[col]:cp[col + 1], l] = 0
+            else:
+                a[:, l] = v * (0.95 + 0.1 * rng.random(nnz))
+        rhs = 2 * rng.random((n, W_)) - 1
+        say(f"case: lead tape n={n} nnz={nnz} lanes={W_}")
+        dx, fail, _ = E.tab_refactor_solve(tab, idx, a, rhs)
+        lx = np.zeros((max(otab.nnz_l, 1), W_))
+        ux = np.zeros((max(otab.nnz_u, 1), W_))
+        ud, w, y, dxo = (np.zeros((n, W_)) for _ in range(4))
+        for l in range(W_):
+            bad = oracle.or_lane_refactor(C.byref(otab), O.u32(oidx)[1], O.f64(a)[1], lx.ctypes.data_as(C.POINTER(C.c_double)),
+                                          ux.ctypes.data_as(C.POINTER(C.c_double)), ud.ctypes.data_as(C.POINTER(C.c_double)),
+                                          w.ctypes.data_as(C.POINTER(C.c_double)), W_, l)
+            assert int(bad) == int(fail[l])
+            if bad:
+                fails += 1
+                continue
+            oracle.or_lane_solve_neg(C.byref(otab), O.u32(oidx)[1], lx.ctypes.data_as(C.POINTER(C.c_double)),
+                                     ux.ctypes.data_as(C.POINTER(C.c_double)), ud.ctypes.data_as(C.POINTER(C.c_double)),
+                                     O.f64(rhs)[1], y.ctypes.data_as(C.POINTER(C.c_double)),
+                                     dxo.ctypes.data_as(C.POINTER(C.c_double)), W_, l)
+            assert bits_equal(dx[:, l], dxo[:, l])
+            checked += 1
+    assert checked > 0 and fails > 0
