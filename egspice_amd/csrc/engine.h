@@ -51,6 +51,18 @@ struct Template {
 
 bool is_tridiag(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
 
+// Host copy of a template's workgroup-per-lane tables (wave.h): instances
+// in serial evaluation order with their staging, constant and limiter
+// offsets, and the stamp tape that sums each residual row and G slot over
+// staging cells in that order.
+struct InstEntry;
+struct WaveHost {
+    std::vector<uint8_t> inst_bytes;  // InstEntry array
+    std::vector<uint32_t> nodes_flat, slots_flat, node_off, slot_off, rptr, ridx, gptr, gidx;
+    uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0;
+};
+void build_wave(const Template &T, WaveHost &H);
+
 // ------------------------------------------------- a batch's lane views
 
 constexpr uint32_t G_INITFIX = 2u;

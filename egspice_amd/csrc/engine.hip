@@ -25,6 +25,7 @@
 
 #include "../../include/espice_lanes.h"
 #include "engine.h"
+#include "wave.h"
 
 using namespace esp;
 
@@ -74,6 +75,75 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
     lane_newton(B, l, o);
 }
 
+// Workgroup-per-lane static Newton (wave.h): block l owns lane l, its
+// state in dynamic LDS laid out by `lay`.
+__global__ __launch_bounds__(64) void newton_wave_kernel(WaveTemplate W, WaveLayout lay, const double *prm_lanes,
+                                                         double *x_glob, uint32_t N, Newton o, uint32_t *status,
+                                                         uint32_t *iters)
+{
+    extern __shared__ double lds[];
+    const uint32_t l = blockIdx.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t n = W.P.n, nnz = W.P.nnz;
+    WaveLane s = wave_lane(W, lay, lds);
+    double *prm_lds = lds + lay.prm;
+    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    for (uint32_t i = tid; i < n; i += nt) {
+        double v = x_glob[(size_t)i * N + l];
+        s.x[i] = v;
+        s.x_old[i] = v;
+    }
+    if (tid == 0) wave_fresh_solver(W, s);
+    __syncthreads();
+    bool lim_active = false;
+    if (o.init_fix) {
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+        lim_active = true;
+    }
+    uint32_t bits = o.init_fix ? G_INITFIX : 0;
+    double limit = 1e9, scaled = 0;
+    uint32_t st = 0, it = o.max_iter;
+    __syncthreads();
+    for (uint32_t tick = 0; tick < o.max_iter; tick++) {
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
+        __syncthreads();
+        for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_entry(W, s, e);
+        __syncthreads();
+        if (tid == 0) s.flags[0] = wave_solve(W, s, o, tick, limit, scaled) ? 0u : 1u;
+        __syncthreads();
+        if (s.flags[0]) {
+            st = 2;
+            it = 0;
+            break;
+        }
+        int lim_any = 0;
+        for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
+        lim_any = __syncthreads_or(lim_any);
+        lim_active = true;
+        if (tid == 0) {
+            bool ok = wave_gates(W, s, o, tick, lim_any != 0, limit, scaled);
+            uint32_t verdict = 0;
+            if (ok && (bits & G_INITFIX)) bits &= ~G_INITFIX;
+            else if (ok) verdict = 1;
+            s.flags[1] = verdict;
+        }
+        __syncthreads();
+        if (s.flags[1]) {
+            for (uint32_t i = tid; i < n; i += nt) s.x[i] = s.x_old[i];
+            st = 1;
+            it = tick + 1;
+            break;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += nt) x_glob[(size_t)i * N + l] = s.x[i];
+    if (tid == 0) {
+        status[l] = st;
+        iters[l] = it;
+    }
+}
+
 __global__ void init_lane_state_kernel(uint8_t *state, uint32_t N, uint8_t tri)
 {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,12 +188,44 @@ struct espice_lanes {
     DevBuf<uint32_t> cnt, pinv, lp, up, li, ui, prow, void_slots, flag, topo, stack, pstack, want, want_of, prev;
     DevBuf<double> lx, ux, udiag, w, y, rscale, vcopy, tri, probe;
     uint32_t cap = 0;
+    // workgroup-per-lane tables
+    WaveHost wh;
+    DevBuf<uint8_t> w_inst;
+    DevBuf<uint32_t> w_nodes, w_slots, w_node_off, w_slot_off, w_rptr, w_ridx, w_gptr, w_gidx;
+    DevBuf<double> prm_lanes;
+    WaveLayout lay{};
+    size_t lds_bytes = 0;
+    bool use_wave = false;
+    std::vector<double> prm_host;
 
     ~espice_lanes()
     {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    WaveTemplate wave() const
+    {
+        WaveTemplate W;
+        W.P = Pattern{t->n, t->nnz, col_ptr.p, row_idx.p, q.p, t->tridiagonal ? tri_pos.p : nullptr, t->diag_fill, cap};
+        W.diag = diag.p;
+        W.cur_row = cur_row.p;
+        W.n_inst = wh.n_inst;
+        W.inst = reinterpret_cast<const InstEntry *>(w_inst.p);
+        W.nodes_flat = w_nodes.p;
+        W.slots_flat = w_slots.p;
+        W.node_off = w_node_off.p;
+        W.slot_off = w_slot_off.p;
+        W.rptr = w_rptr.p;
+        W.ridx = w_ridx.p;
+        W.gptr = w_gptr.p;
+        W.gidx = w_gidx.p;
+        W.n_stage = wh.n_stage;
+        W.n_prm = wh.n_prm;
+        W.n_lim = wh.n_lim;
+        W.cap = cap;
+        return W;
     }
 
     Batch batch() const
@@ -313,6 +415,22 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     HIP_TRY(zeroed(L->vcopy, nnz * N));
     HIP_TRY(zeroed(L->tri, 3 * n * N));
     HIP_TRY(zeroed(L->probe, 3 * n * N));
+    build_wave(T, L->wh);
+    HIP_TRY(upload(L->w_inst, L->wh.inst_bytes));
+    HIP_TRY(upload(L->w_nodes, L->wh.nodes_flat));
+    HIP_TRY(upload(L->w_slots, L->wh.slots_flat));
+    HIP_TRY(upload(L->w_node_off, L->wh.node_off));
+    HIP_TRY(upload(L->w_slot_off, L->wh.slot_off));
+    HIP_TRY(upload(L->w_rptr, L->wh.rptr));
+    HIP_TRY(upload(L->w_ridx, L->wh.ridx));
+    HIP_TRY(upload(L->w_gptr, L->wh.gptr));
+    HIP_TRY(upload(L->w_gidx, L->wh.gidx));
+    HIP_TRY(zeroed(L->prm_lanes, (size_t)L->wh.n_prm * N));
+    {
+        WaveTemplate W = L->wave();
+        L->lds_bytes = 8 * wave_words(W, L->lay);
+        L->use_wave = L->lds_bytes <= 64 * 1024;
+    }
     const uint32_t blocks = (n_lanes + 63) / 64;
     hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, n_lanes,
                        (uint8_t)(T.tridiagonal ? 1 : 0));
@@ -367,7 +485,18 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         if (!host.empty())
             HIP_TRY(hipMemcpyAsync(L->prm[kind].p, host.data(), host.size() * 8, hipMemcpyHostToDevice, L->stream));
         HIP_TRY(hipStreamSynchronize(L->stream));
+        // the same constants per lane contiguous, in serial instance order
+        if (kind == 0) L->prm_host.assign((size_t)L->wh.n_prm * N, 0.0);
+        const InstEntry *ie = reinterpret_cast<const InstEntry *>(L->wh.inst_bytes.data());
+        for (uint32_t e = 0; e < L->wh.n_inst; e++) {
+            if (ie[e].kind != kind) continue;
+            for (size_t l = 0; l < N; l++)
+                for (int k = 0; k < pk; k++)
+                    L->prm_host[l * L->wh.n_prm + ie[e].prm + k] = host[((size_t)ie[e].id * pk + k) * N + l];
+        }
     }
+    if (!L->prm_host.empty())
+        HIP_TRY(hipMemcpy(L->prm_lanes.p, L->prm_host.data(), L->prm_host.size() * 8, hipMemcpyHostToDevice));
     L->prepared = true;
     return ESPICE_OK;
 }
@@ -392,10 +521,15 @@ espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
-    const Batch B = L->batch();
-    const uint32_t blocks = (L->N + 63) / 64;
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
-    hipLaunchKernelGGL(newton_op_kernel, dim3(blocks), dim3(64), 0, L->stream, B, o);
+    if (L->use_wave) {
+        hipLaunchKernelGGL(newton_wave_kernel, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wave(), L->lay,
+                           L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
+    } else {
+        const Batch B = L->batch();
+        const uint32_t blocks = (L->N + 63) / 64;
+        hipLaunchKernelGGL(newton_op_kernel, dim3(blocks), dim3(64), 0, L->stream, B, o);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     std::vector<uint32_t> st(L->N), it(L->N);

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "wave.h"
 
 namespace esp {
 
@@ -124,3 +125,72 @@ int Template::freeze()
 
 }  // namespace esp
 
+
+namespace esp {
+
+void build_wave(const Template &T, WaveHost &H)
+{
+    std::vector<InstEntry> inst;
+    uint32_t stage = 0, prm = 0, lim = 0;
+    std::vector<std::vector<uint32_t>> rows(T.n), slots(T.nnz);
+    H.nodes_flat.clear();
+    H.slots_flat.clear();
+    H.node_off.clear();
+    H.slot_off.clear();
+    for (int t = 0; t < T.n_order; t++) {
+        const int kind = T.order[t], nu = KIND_NU[kind];
+        for (uint32_t id = 0; id < T.count[kind]; id++) {
+            InstEntry e{};
+            e.kind = (uint8_t)kind;
+            e.id = id;
+            e.stage_res = stage;
+            stage += nu;
+            e.stage_jac = stage;
+            e.prm = prm;
+            prm += KIND_PK[kind];
+            e.lim = lim;
+            if (KIND_LIMW[kind]) lim += nu;
+            const uint32_t *nd = &T.nodes[kind][(size_t)id * nu];
+            const uint32_t *sl = &T.slots[kind][(size_t)id * T.kslots[kind]];
+            H.node_off.push_back((uint32_t)H.nodes_flat.size());
+            H.slot_off.push_back((uint32_t)H.slots_flat.size());
+            H.nodes_flat.insert(H.nodes_flat.end(), nd, nd + nu);
+            H.slots_flat.insert(H.slots_flat.end(), sl, sl + T.kslots[kind]);
+            uint32_t kp = 0, kg = 0;
+            for (int ru = 0; ru < nu; ru++) {
+                if ((KIND_RMASK[kind] >> ru & 1) && nd[ru] != 0) rows[nd[ru]].push_back(e.stage_res + ru);
+                const unsigned gm = kind_gmask(kind, ru), pm = kind_pmask(kind, ru);
+                for (int cu = 0; cu < nu; cu++) {
+                    if (!(pm >> cu & 1)) continue;
+                    const uint32_t slot = sl[kp++];
+                    if (gm >> cu & 1) {
+                        if (slot != T.nnz) slots[slot].push_back(e.stage_jac + kg);
+                        kg++;
+                    }
+                }
+            }
+            stage += kg;
+            inst.push_back(e);
+        }
+    }
+    H.n_inst = (uint32_t)inst.size();
+    H.n_stage = stage;
+    H.n_prm = prm;
+    H.n_lim = lim;
+    H.inst_bytes.resize(inst.size() * sizeof(InstEntry));
+    if (!inst.empty()) std::memcpy(H.inst_bytes.data(), inst.data(), H.inst_bytes.size());
+    H.rptr.assign(1, 0);
+    H.ridx.clear();
+    for (uint32_t r = 0; r < T.n; r++) {
+        H.ridx.insert(H.ridx.end(), rows[r].begin(), rows[r].end());
+        H.rptr.push_back((uint32_t)H.ridx.size());
+    }
+    H.gptr.assign(1, 0);
+    H.gidx.clear();
+    for (uint32_t s = 0; s < T.nnz; s++) {
+        H.gidx.insert(H.gidx.end(), slots[s].begin(), slots[s].end());
+        H.gptr.push_back((uint32_t)H.gidx.size());
+    }
+}
+
+}  // namespace esp

@@ -1,0 +1,285 @@
+// The workgroup-per-lane Newton of the lane engine.
+//
+// One 64-thread workgroup owns one lane and keeps all of its state in LDS:
+// node voltages, the G plane and residual, the limiter images, the lane's
+// prepared device constants and its whole sparse direct solver. A Newton
+// tick splits into phases separated by workgroup barriers:
+//   eval     thread t evaluates instances t, t+64, ... (forward-mode AD and
+//            the limiting correction) into per-instance staging cells;
+//   reduce   thread t sums planes entries t, t+64, ... over their staging
+//            cells in the serial evaluation order (the host-built stamp
+//            tape), so every sum is bitwise the serial one
+//            (src/analysis/gpu/lane_color.zig's ordered hub reduction plays
+//            the same role in the reference);
+//   solve    thread 0 runs the lane's direct factor and solve in LDS;
+//   limit    threads over instances; then the gates on thread 0.
+// Every phase is an ESP_HD function of (tid, nthreads), so the host test
+// harness replays the same phases thread by thread.
+#pragma once
+
+#include "engine.h"
+
+namespace esp {
+
+// One entry of the serial evaluation order: kind, instance, and where its
+// stamps, prepared constants and limiter image live in the lane's block.
+struct InstEntry {
+    uint8_t kind;
+    uint8_t pad[3];
+    uint32_t id;
+    uint32_t stage_res;  // first residual cell (NU cells)
+    uint32_t stage_jac;  // first Jacobian cell (one per G entry)
+    uint32_t prm;        // first prepared constant in the lane's block
+    uint32_t lim;        // first limiter image word
+};
+
+// Template-wide data every workgroup reads (device or host pointers).
+struct WaveTemplate {
+    Pattern P;
+    const uint32_t *diag;
+    const uint8_t *cur_row;
+    uint32_t n_inst;
+    const InstEntry *inst;
+    const uint32_t *nodes_flat;  // NU words per instance, serial order
+    const uint32_t *slots_flat;  // kslots words per instance, serial order
+    const uint32_t *node_off;    // per instance: first word in nodes_flat
+    const uint32_t *slot_off;    // per instance: first word in slots_flat
+    const uint32_t *rptr, *ridx;  // residual row r sums stage cells ridx[rptr[r]..rptr[r+1])
+    const uint32_t *gptr, *gidx;  // G slot s sums stage cells gidx[gptr[s]..gptr[s+1])
+    uint32_t n_stage;             // staging cells per lane
+    uint32_t n_prm;               // prepared constants per lane
+    uint32_t n_lim;               // limiter words per lane
+    uint32_t cap;                 // L+U capacity per lane
+};
+
+// LDS offsets (in 8-byte words) of one lane's block, computed on the host.
+struct WaveLayout {
+    uint32_t x, x_old, dx, g, rhs, lim, stage, prm, lu_f64, lu_u32, lu_u8, flags, words;
+};
+
+// The lane's block carved into typed views (stride 1).
+struct WaveLane {
+    double *x, *x_old, *dx, *g, *rhs, *lim, *stage;
+    const double *prm;
+    LaneLU lu;
+    uint32_t *flags;  // 0 singular, 1 limited, 2 done, 3 bits, 4 ok, 5 scaled bits lo, 6 hi
+};
+
+ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L)
+{
+    const size_t n = W.P.n, nnz = W.P.nnz, cap = W.cap;
+    size_t at = 0;
+    L.x = (uint32_t)at; at += n;
+    L.x_old = (uint32_t)at; at += n;
+    L.dx = (uint32_t)at; at += n;
+    L.g = (uint32_t)at; at += nnz + 1;
+    L.rhs = (uint32_t)at; at += n + 1;
+    L.lim = (uint32_t)at; at += W.n_lim;
+    L.stage = (uint32_t)at; at += W.n_stage;
+    L.prm = (uint32_t)at; at += W.n_prm;
+    // doubles of the solver: lx, ux (cap), udiag, w, y, rscale (n), vcopy
+    // (nnz), tri (3n), probe (3n)
+    L.lu_f64 = (uint32_t)at; at += 2 * cap + 4 * n + nnz + 6 * n;
+    // u32 words of the solver: cnt 4, pinv n, lp n+1, up n+1, li, ui (cap),
+    // prow, void_slots (nnz), flag..prev 7n
+    size_t u32w = 4 + n + 2 * (n + 1) + 2 * cap + 2 * nnz + 7 * n;
+    L.lu_u32 = (uint32_t)at; at += (u32w + 1) / 2;
+    // bytes: state 3, void_col n, scaled_pivot n
+    size_t u8w = 3 + 2 * n;
+    L.lu_u8 = (uint32_t)at; at += (u8w + 7) / 8;
+    L.flags = (uint32_t)at; at += 4;  // 8 u32 flags
+    L.words = (uint32_t)at;
+    return at;
+}
+
+ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *block)
+{
+    const uint32_t n = W.P.n, nnz = W.P.nnz, cap = W.cap;
+    WaveLane s;
+    s.x = block + L.x;
+    s.x_old = block + L.x_old;
+    s.dx = block + L.dx;
+    s.g = block + L.g;
+    s.rhs = block + L.rhs;
+    s.lim = block + L.lim;
+    s.stage = block + L.stage;
+    s.prm = block + L.prm;
+    double *f = block + L.lu_f64;
+    LaneLU &u = s.lu;
+    u.lx = LV<double>{f, 1}; f += cap;
+    u.ux = LV<double>{f, 1}; f += cap;
+    u.udiag = LV<double>{f, 1}; f += n;
+    u.w = LV<double>{f, 1}; f += n;
+    u.y = LV<double>{f, 1}; f += n;
+    u.rscale = LV<double>{f, 1}; f += n;
+    u.vcopy = LV<double>{f, 1}; f += nnz;
+    u.bp = LV<double>{f, 1}; f += n;
+    u.mul = LV<double>{f, 1}; f += n;
+    u.cv = LV<double>{f, 1}; f += n;
+    u.px = LV<double>{f, 1}; f += n;
+    u.pr = LV<double>{f, 1}; f += n;
+    u.prows = LV<double>{f, 1};
+    uint32_t *w = reinterpret_cast<uint32_t *>(block + L.lu_u32);
+    u.cnt = LV<uint32_t>{w, 1}; w += 4;
+    u.pinv = LV<uint32_t>{w, 1}; w += n;
+    u.lp = LV<uint32_t>{w, 1}; w += n + 1;
+    u.up = LV<uint32_t>{w, 1}; w += n + 1;
+    u.li = LV<uint32_t>{w, 1}; w += cap;
+    u.ui = LV<uint32_t>{w, 1}; w += cap;
+    u.prow = LV<uint32_t>{w, 1}; w += nnz;
+    u.void_slots = LV<uint32_t>{w, 1}; w += nnz;
+    u.flag = LV<uint32_t>{w, 1}; w += n;
+    u.topo = LV<uint32_t>{w, 1}; w += n;
+    u.stack = LV<uint32_t>{w, 1}; w += n;
+    u.pstack = LV<uint32_t>{w, 1}; w += n;
+    u.want = LV<uint32_t>{w, 1}; w += n;
+    u.want_of = LV<uint32_t>{w, 1}; w += n;
+    u.prev = LV<uint32_t>{w, 1};
+    uint8_t *b = reinterpret_cast<uint8_t *>(block + L.lu_u8);
+    u.state = LV<uint8_t>{b, 1}; b += 3;
+    u.void_col = LV<uint8_t>{b, 1}; b += n;
+    u.scaled_pivot = LV<uint8_t>{b, 1};
+    s.flags = reinterpret_cast<uint32_t *>(block + L.flags);
+    return s;
+}
+
+struct LP {
+    const double *p;
+    ESP_HD double operator[](int k) const { return p[k]; }
+};
+
+// Phase: evaluate instance e into its staging cells.
+ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+{
+    const InstEntry ie = W.inst[e];
+    const int kind = ie.kind, nu = KIND_NU[kind];
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    LP p{s.prm + ie.prm};
+    double lx[4], xl[4];
+    for (int u = 0; u < nu; u++) xl[u] = lx[u] = s.x[nd[u]];
+    const unsigned limw = KIND_LIMW[kind];
+    const bool use_lim = limw != 0 && lim_active;
+    if (use_lim)
+        for (int u = 0; u < nu; u++)
+            if (limw >> u & 1) lx[u] = s.lim[ie.lim + u];
+    DevOut o;
+    eval_kind(kind, p, lx, o);
+    if (use_lim) {
+        double corr[4];
+        bool live = false;
+        for (int u = 0; u < nu; u++) {
+            corr[u] = xl[u] - lx[u];
+            live = live || corr[u] != 0;
+        }
+        if (live)
+            for (int ru = 0; ru < nu; ru++) {
+                double acc = 0;
+                for (int u = 0; u < nu; u++) acc += o.jac[ru][u] * corr[u];
+                o.res[ru] += acc;
+            }
+    }
+    uint32_t k = 0;
+    for (int ru = 0; ru < nu; ru++) {
+        s.stage[ie.stage_res + ru] = o.res[ru];
+        const unsigned gm = kind_gmask(kind, ru);
+        for (int cu = 0; cu < nu; cu++)
+            if (gm >> cu & 1) s.stage[ie.stage_jac + k++] = o.jac[ru][cu];
+    }
+}
+
+// Phase: residual row or G slot `e` of the planes (e < n rows, then slots).
+ESP_HD void wave_reduce_entry(const WaveTemplate &W, WaveLane &s, uint32_t e)
+{
+    const uint32_t n = W.P.n;
+    if (e < n) {
+        double acc = 0.0;
+        for (uint32_t c = W.rptr[e]; c < W.rptr[e + 1]; c++) acc += s.stage[W.ridx[c]];
+        s.rhs[e] = acc;
+    } else {
+        const uint32_t slot = e - n;
+        double acc = 0.0;
+        for (uint32_t c = W.gptr[slot]; c < W.gptr[slot + 1]; c++) acc += s.stage[W.gidx[c]];
+        s.g[slot] = acc;
+    }
+}
+
+// Phase: limit instance e (cur = x, old = image or x_old); true if clamped.
+ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+{
+    const InstEntry ie = W.inst[e];
+    const int kind = ie.kind, nu = KIND_NU[kind];
+    const unsigned limw = KIND_LIMW[kind];
+    if (limw == 0) return false;
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    LP p{s.prm + ie.prm};
+    double cur[4], old[4], lim[4];
+    for (int u = 0; u < nu; u++) {
+        cur[u] = s.x[nd[u]];
+        old[u] = ((limw >> u & 1) && lim_active) ? s.lim[ie.lim + u] : s.x_old[nd[u]];
+    }
+    int ch = kind == K_MOS1 ? limit_mos1(p, cur, old, lim) : limit_diode(p, cur, old, lim);
+    for (int u = 0; u < nu; u++)
+        if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
+    return ch != 0;
+}
+
+ESP_HD void wave_seed_inst(const WaveTemplate &W, WaveLane &s, uint32_t e)
+{
+    const InstEntry ie = W.inst[e];
+    const int kind = ie.kind, nu = KIND_NU[kind];
+    const unsigned limw = KIND_LIMW[kind];
+    if (limw == 0) return;
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    LP p{s.prm + ie.prm};
+    double cur[4], lim[4];
+    for (int u = 0; u < nu; u++) cur[u] = s.x[nd[u]];
+    if (kind == K_MOS1) seed_mos1(p, cur, lim);
+    else seed_diode(p, cur, lim);
+    for (int u = 0; u < nu; u++)
+        if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
+}
+
+// Fresh solver state (direct.Solver right after init).
+ESP_HD void wave_fresh_solver(const WaveTemplate &W, WaveLane &s)
+{
+    LaneLU &u = s.lu;
+    u.state[ST_FACTORED] = 0;
+    u.state[ST_TRI] = W.P.tri_pos != nullptr ? 1 : 0;
+    u.state[ST_LU_FACTORED] = 0;
+    for (int i = 0; i < 4; i++) u.cnt[i] = 0;
+    for (uint32_t i = 0; i < W.P.n; i++) u.w[i] = 0.0;
+}
+
+// Thread 0's part of a tick after the planes: ground stamp, gmin, the
+// runaway bound, factor and solve, update and score. Returns false when
+// the lane is singular.
+ESP_HD bool wave_solve(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, double &limit, double &scaled)
+{
+    const uint32_t n = W.P.n;
+    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1}, x_old{s.x_old, 1}, dx{s.dx, 1};
+    g[W.diag[0]] += 1.0;
+    rhs[0] += x[0];
+    if (o.gmin > 0) {
+        for (uint32_t i = 0; i < n; i++) g[W.diag[i]] += o.gmin;
+        for (uint32_t i = 0; i < n; i++) rhs[i] += o.gmin * x[i];
+    }
+    if (tick == 0) limit = plausible_limit(n, x, rhs, g, W.diag);
+    if (direct_factor(W.P, s.lu, g) != 0) return false;
+    direct_solve_neg(W.P, s.lu, rhs, dx);
+    scaled = update_norm(n, x, dx, x_old, W.cur_row, o.reltol, o.abstol, o.vntol);
+    return true;
+}
+
+// The gates after the limit pass (finalizeStep's order), thread 0.
+ESP_HD bool wave_gates(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, bool limited, double limit,
+                       double scaled)
+{
+    const uint32_t n = W.P.n;
+    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1};
+    bool ok = !limited && tick != 0 && !(scaled >= 1.0);
+    if (ok) ok = plausible(n, x, limit);
+    if (ok) ok = residual_ok(n, o, x, rhs, g, W.diag);
+    return ok;
+}
+
+}  // namespace esp

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../egspice_amd/csrc/engine.h"
+#include "../egspice_amd/csrc/wave.h"
 
 using namespace esp;
 
@@ -160,6 +161,98 @@ void hl_newton_op(void *hv, const Newton *o, double *x, uint32_t *status, uint32
     std::memcpy(status, h->status.data(), 4 * N);
     std::memcpy(iters, h->iters.data(), 4 * N);
 }
+
+// The workgroup-per-lane kernel's phase sequence (engine.hip
+// newton_wave_kernel) replayed with the 64 threads of each phase run one
+// after another; the barriers fall between phases.
+void hl_newton_wave(void *hv, const Newton *o, double *x, uint32_t *status, uint32_t *iters)
+{
+    HL *h = (HL *)hv;
+    Template *t = h->t;
+    const uint32_t N = h->N, n = t->n, nnz = t->nnz, nt = 64;
+    WaveHost wh;
+    build_wave(*t, wh);
+    WaveTemplate W;
+    W.P = Pattern{n, nnz, t->col_ptr.data(), t->row_idx.data(), t->q.data(), t->tridiagonal ? t->tri_pos.data() : nullptr,
+                  t->diag_fill, h->cap};
+    W.diag = t->diag.data();
+    W.cur_row = h->cur_row.data();
+    W.n_inst = wh.n_inst;
+    W.inst = reinterpret_cast<const InstEntry *>(wh.inst_bytes.data());
+    W.nodes_flat = wh.nodes_flat.data();
+    W.slots_flat = wh.slots_flat.data();
+    W.node_off = wh.node_off.data();
+    W.slot_off = wh.slot_off.data();
+    W.rptr = wh.rptr.data();
+    W.ridx = wh.ridx.data();
+    W.gptr = wh.gptr.data();
+    W.gidx = wh.gidx.data();
+    W.n_stage = wh.n_stage;
+    W.n_prm = wh.n_prm;
+    W.n_lim = wh.n_lim;
+    W.cap = h->cap;
+    WaveLayout lay;
+    size_t words = wave_words(W, lay);
+    std::vector<double> block(words);
+    const InstEntry *ie = W.inst;
+    for (uint32_t l = 0; l < N; l++) {
+        std::fill(block.begin(), block.end(), 0.0);
+        WaveLane s = wave_lane(W, lay, block.data());
+        double *prm = block.data() + lay.prm;
+        for (uint32_t e = 0; e < wh.n_inst; e++) {
+            int kind = ie[e].kind, pk = KIND_PK[kind];
+            for (int k = 0; k < pk; k++) prm[ie[e].prm + k] = h->prm[kind][((size_t)ie[e].id * pk + k) * N + l];
+        }
+        for (uint32_t i = 0; i < n; i++) s.x[i] = s.x_old[i] = x[(size_t)i * N + l];
+        wave_fresh_solver(W, s);
+        bool lim_active = false;
+        if (o->init_fix) {
+            for (uint32_t e = 0; e < wh.n_inst; e++) wave_seed_inst(W, s, e);
+            lim_active = true;
+        }
+        uint32_t bits = o->init_fix ? G_INITFIX : 0, st = 0, it = o->max_iter;
+        double limit = 1e9, scaled = 0;
+        for (uint32_t tick = 0; tick < o->max_iter; tick++) {
+            for (uint32_t tid = 0; tid < nt; tid++)
+                for (uint32_t e = tid; e < wh.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
+            for (uint32_t tid = 0; tid < nt; tid++)
+                for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_entry(W, s, e);
+            if (!wave_solve(W, s, *o, tick, limit, scaled)) {
+                st = 2;
+                it = 0;
+                break;
+            }
+            int lim_any = 0;
+            for (uint32_t tid = 0; tid < nt; tid++)
+                for (uint32_t e = tid; e < wh.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
+            lim_active = true;
+            bool ok = wave_gates(W, s, *o, tick, lim_any != 0, limit, scaled);
+            if (ok && (bits & G_INITFIX)) {
+                bits &= ~G_INITFIX;
+            } else if (ok) {
+                for (uint32_t i = 0; i < n; i++) s.x[i] = s.x_old[i];
+                st = 1;
+                it = tick + 1;
+                break;
+            }
+        }
+        for (uint32_t i = 0; i < n; i++) x[(size_t)i * N + l] = s.x[i];
+        status[l] = st;
+        iters[l] = it;
+    }
+}
+
+int hl_column_order(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, uint32_t *q)
+{
+    return column_order(n, col_ptr, row_idx, q);
+}
+
+int hl_amd(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, uint32_t *q)
+{
+    return amd_order(n, col_ptr, row_idx, q);
+}
+
+void hl_prep(int kind, const double *u, double *p) { prep_kind(kind, u, p); }
 
 double hl_exp(double x) { return e_exp(x); }
 double hl_log(double x) { return e_log(x); }

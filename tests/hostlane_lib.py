@@ -18,6 +18,10 @@ def load(path):
         ("hl_set_param", None, [P, C.c_int, C.c_uint32, C.c_uint32, f64p]), ("hl_prepare", None, [P]),
         ("hl_prm", None, [P, C.c_int, f64p]),
         ("hl_newton_op", None, [P, C.POINTER(NewtonOpts), f64p, u32p, u32p]),
+        ("hl_newton_wave", None, [P, C.POINTER(NewtonOpts), f64p, u32p, u32p]),
+        ("hl_column_order", C.c_int, [C.c_uint32, u32p, u32p, u32p]),
+        ("hl_amd", C.c_int, [C.c_uint32, u32p, u32p, u32p]),
+        ("hl_prep", None, [C.c_int, f64p, f64p]),
         ("hl_exp", C.c_double, [C.c_double]), ("hl_log", C.c_double, [C.c_double]),
         ("hl_pow", C.c_double, [C.c_double, C.c_double]),
     ]:
@@ -81,7 +85,7 @@ class HostLanes:
     def prepare(self):
         self.hl.hl_prepare(self.h)
 
-    def newton_op(self, x0=None, **kw):
+    def newton_op(self, x0=None, wave=False, **kw):
         import oracle_lib
         n, N = self.t.n, self.N
         x = np.zeros(n * N) if x0 is None else np.array(x0, dtype=np.float64).reshape(-1)
@@ -89,5 +93,5 @@ class HostLanes:
         st, sp = u32(np.zeros(N))
         it, ip = u32(np.zeros(N))
         o = oracle_lib.newton_opts(**kw)
-        self.hl.hl_newton_op(self.h, C.byref(o), xp, sp, ip)
+        (self.hl.hl_newton_wave if wave else self.hl.hl_newton_op)(self.h, C.byref(o), xp, sp, ip)
         return xa.reshape(n, N), st, it
