@@ -138,8 +138,12 @@ class Template:
 
     def add(self, kind, ports, params=None):
         rows = [self.node(p) if isinstance(p, str) else p for p in ports]
+        if kind not in PARAM_COUNT:
+            raise EngineError(f"unknown device kind {kind}")
         pr = np.full(PARAM_COUNT[kind], np.nan)
         for k, v in (params or {}).items():
+            if not 0 <= k < len(pr):
+                raise EngineError(f"parameter {k} out of range for kind {kind} ({len(pr)} parameters)")
             pr[k] = v
         _, rp = _u32(rows)
         a, ap = _f64(pr)

@@ -522,13 +522,6 @@ static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double 
         }
         if (tick == 0) limit = or_lane_plausible_limit(n, x, rhs, g, c->diag, 1, 0);
         if (lin(lctx, l, g, rhs, dx) != 0) {
-            if (getenv("OR_DEBUG")) {
-                fprintf(stderr, "lane %u tick %u singular\n", l, tick);
-                for (uint32_t p = 0; p < nnz; p++) fprintf(stderr, " a[%u]=%g", p, g[p]);
-                fprintf(stderr, "\n");
-                for (uint32_t i = 0; i < n; i++) fprintf(stderr, " x[%u]=%g rhs=%g", i, x[i], rhs[i]);
-                fprintf(stderr, "\n");
-            }
             out->status = 2;
             out->iterations = 0;
             break;
@@ -537,7 +530,6 @@ static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double 
         int limited = limit_pass(L, l, x, x_old, lim_active);
         lim_active = 1;
         int ok = !limited && tick != 0 && !(scaled >= 1.0);
-        if (getenv("OR_DEBUG")) fprintf(stderr, "lane %u tick %u scaled %g limited %d x1 %g\n", l, tick, scaled, limited, x[1]);
         if (ok) ok = or_lane_plausible(n, x, limit, 1, 0);
         if (ok) ok = or_lane_residual(n, o->reltol, o->vntol, o->residual_tol, x, rhs, g, c->diag, 1, 0);
         if (ok && (bits & G_INITFIX)) {
