@@ -55,13 +55,24 @@ bool is_tridiag(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
 // in serial evaluation order with their staging, constant and limiter
 // offsets, and the stamp tape that sums each residual row and G slot over
 // staging cells in that order.
+//
+// Every table is also packed into one u32 blob (`blob`, word offsets in
+// `at`) that a workgroup copies into LDS before its first tick.
 struct InstEntry;
+struct WaveBlob {
+    uint32_t n, nnz, cap, n_inst, n_stage, n_prm, n_lim, tri;
+    uint64_t diag_fill;
+    uint32_t col_ptr, row_idx, q, tri_pos, diag, cur_row, inst, nodes, slots, node_off, slot_off, rptr, ridx, gptr,
+        gidx, words;
+};
 struct WaveHost {
     std::vector<uint8_t> inst_bytes;  // InstEntry array
     std::vector<uint32_t> nodes_flat, slots_flat, node_off, slot_off, rptr, ridx, gptr, gidx;
     uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0;
+    WaveBlob at{};
+    std::vector<uint32_t> blob;
 };
-void build_wave(const Template &T, WaveHost &H);
+void build_wave(const Template &T, uint32_t cap, WaveHost &H);
 
 // ------------------------------------------------- a batch's lane views
 

@@ -26,6 +26,7 @@
 #include "../../include/espice_lanes.h"
 #include "engine.h"
 #include "wave.h"
+#include "wave_lu.h"
 
 using namespace esp;
 
@@ -76,23 +77,33 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 }
 
 // Workgroup-per-lane static Newton (wave.h): block l owns lane l, its
-// state in dynamic LDS laid out by `lay`.
-__global__ __launch_bounds__(64) void newton_wave_kernel(WaveTemplate W, WaveLayout lay, const double *prm_lanes,
-                                                         double *x_glob, uint32_t N, Newton o, uint32_t *status,
-                                                         uint32_t *iters)
+// state in dynamic LDS laid out by `lay` after the template's tables. With
+// DENSE the lane's direct solver replays on the whole wavefront
+// (wave_lu.h, patterns up to 64 unknowns); otherwise thread 0 runs it.
+template <bool DENSE>
+__global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
+                                                         const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
+                                                         uint32_t *status, uint32_t *iters)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const uint32_t n = W.P.n, nnz = W.P.nnz;
-    WaveLane s = wave_lane(W, lay, lds);
-    double *prm_lds = lds + lay.prm;
+    const uint32_t n = B.n, nnz = B.nnz;
+    // the template's tables first (B.words is even), then the lane block
+    uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
+    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + B.words / 2;
+    const WaveTemplate W = wave_template(B, tl);
+    WaveLane s = wave_lane(W, lay, block);
+    const DenseLU D = dense_lu(block, lay, n);
+    double *prm_lds = block + lay.prm;
     for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
     for (uint32_t i = tid; i < n; i += nt) {
         double v = x_glob[(size_t)i * N + l];
         s.x[i] = v;
         s.x_old[i] = v;
     }
+    __syncthreads();
     if (tid == 0) wave_fresh_solver(W, s);
     __syncthreads();
     bool lim_active = false;
@@ -109,7 +120,25 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveTemplate W, WaveLay
         __syncthreads();
         for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_entry(W, s, e);
         __syncthreads();
-        if (tid == 0) s.flags[0] = wave_solve(W, s, o, tick, limit, scaled) ? 0u : 1u;
+        if (DENSE) {
+            if (tid == 0) wave_assemble(W, s, o, tick, limit);
+            const int fl = wave_direct_factor(W.P, s.lu, D, s.g, &s.flags[2]);
+            if (fl == 0) {
+                if (s.lu.state[ST_TRI]) {
+                    if (tid == 0) {
+                        LV<double> rhs{s.rhs, 1}, dx{s.dx, 1};
+                        direct_solve_neg(W.P, s.lu, rhs, dx);
+                    }
+                } else {
+                    dense_solve_neg(W.P, D, s.rhs, s.dx);
+                }
+                __syncthreads();
+                scaled = wave_update_norm(n, s.x, s.dx, s.x_old, W.cur_row, o.reltol, o.abstol, o.vntol);
+            }
+            if (tid == 0) s.flags[0] = fl != 0 ? 1u : 0u;
+        } else {
+            if (tid == 0) s.flags[0] = wave_solve(W, s, o, tick, limit, scaled) ? 0u : 1u;
+        }
         __syncthreads();
         if (s.flags[0]) {
             st = 2;
@@ -190,8 +219,7 @@ struct espice_lanes {
     uint32_t cap = 0;
     // workgroup-per-lane tables
     WaveHost wh;
-    DevBuf<uint8_t> w_inst;
-    DevBuf<uint32_t> w_nodes, w_slots, w_node_off, w_slot_off, w_rptr, w_ridx, w_gptr, w_gidx;
+    DevBuf<uint32_t> w_blob;
     DevBuf<double> prm_lanes;
     WaveLayout lay{};
     size_t lds_bytes = 0;
@@ -203,29 +231,6 @@ struct espice_lanes {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    WaveTemplate wave() const
-    {
-        WaveTemplate W;
-        W.P = Pattern{t->n, t->nnz, col_ptr.p, row_idx.p, q.p, t->tridiagonal ? tri_pos.p : nullptr, t->diag_fill, cap};
-        W.diag = diag.p;
-        W.cur_row = cur_row.p;
-        W.n_inst = wh.n_inst;
-        W.inst = reinterpret_cast<const InstEntry *>(w_inst.p);
-        W.nodes_flat = w_nodes.p;
-        W.slots_flat = w_slots.p;
-        W.node_off = w_node_off.p;
-        W.slot_off = w_slot_off.p;
-        W.rptr = w_rptr.p;
-        W.ridx = w_ridx.p;
-        W.gptr = w_gptr.p;
-        W.gidx = w_gidx.p;
-        W.n_stage = wh.n_stage;
-        W.n_prm = wh.n_prm;
-        W.n_lim = wh.n_lim;
-        W.cap = cap;
-        return W;
     }
 
     Batch batch() const
@@ -415,20 +420,14 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     HIP_TRY(zeroed(L->vcopy, nnz * N));
     HIP_TRY(zeroed(L->tri, 3 * n * N));
     HIP_TRY(zeroed(L->probe, 3 * n * N));
-    build_wave(T, L->wh);
-    HIP_TRY(upload(L->w_inst, L->wh.inst_bytes));
-    HIP_TRY(upload(L->w_nodes, L->wh.nodes_flat));
-    HIP_TRY(upload(L->w_slots, L->wh.slots_flat));
-    HIP_TRY(upload(L->w_node_off, L->wh.node_off));
-    HIP_TRY(upload(L->w_slot_off, L->wh.slot_off));
-    HIP_TRY(upload(L->w_rptr, L->wh.rptr));
-    HIP_TRY(upload(L->w_ridx, L->wh.ridx));
-    HIP_TRY(upload(L->w_gptr, L->wh.gptr));
-    HIP_TRY(upload(L->w_gidx, L->wh.gidx));
+    build_wave(T, cap, L->wh);
+    HIP_TRY(upload(L->w_blob, L->wh.blob));
     HIP_TRY(zeroed(L->prm_lanes, (size_t)L->wh.n_prm * N));
     {
-        WaveTemplate W = L->wave();
-        L->lds_bytes = 8 * wave_words(W, L->lay);
+        // sizes only: the lane block's layout does not depend on where the
+        // template tables live
+        const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
+        L->lds_bytes = 4 * (size_t)L->wh.at.words + 8 * wave_words(W, L->lay);
         L->use_wave = L->lds_bytes <= 64 * 1024;
     }
     const uint32_t blocks = (n_lanes + 63) / 64;
@@ -523,8 +522,12 @@ espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
-        hipLaunchKernelGGL(newton_wave_kernel, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wave(), L->lay,
-                           L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
+        if (L->lay.dense != ESP_NONE)
+            hipLaunchKernelGGL(newton_wave_kernel<true>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
+        else
+            hipLaunchKernelGGL(newton_wave_kernel<false>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;

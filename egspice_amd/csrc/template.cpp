@@ -128,7 +128,7 @@ int Template::freeze()
 
 namespace esp {
 
-void build_wave(const Template &T, WaveHost &H)
+void build_wave(const Template &T, uint32_t cap, WaveHost &H)
 {
     std::vector<InstEntry> inst;
     uint32_t stage = 0, prm = 0, lim = 0;
@@ -191,6 +191,49 @@ void build_wave(const Template &T, WaveHost &H)
         H.gidx.insert(H.gidx.end(), slots[s].begin(), slots[s].end());
         H.gptr.push_back((uint32_t)H.gidx.size());
     }
+    // the LDS blob: every table at a word offset
+    WaveBlob &B = H.at;
+    std::vector<uint32_t> &w = H.blob;
+    w.clear();
+    auto put = [&w](const uint32_t *p, size_t k) {
+        const uint32_t at = (uint32_t)w.size();
+        w.insert(w.end(), p, p + k);
+        return at;
+    };
+    B.n = T.n;
+    B.nnz = T.nnz;
+    B.cap = cap;
+    B.n_inst = H.n_inst;
+    B.n_stage = H.n_stage;
+    B.n_prm = H.n_prm;
+    B.n_lim = H.n_lim;
+    B.tri = T.tridiagonal ? 1u : 0u;
+    B.diag_fill = T.diag_fill;
+    B.col_ptr = put(T.col_ptr.data(), T.col_ptr.size());
+    B.row_idx = put(T.row_idx.data(), T.row_idx.size());
+    B.q = put(T.q.data(), T.q.size());
+    B.tri_pos = put(T.tri_pos.data(), T.tridiagonal ? T.tri_pos.size() : 0);
+    B.diag = put(T.diag.data(), T.diag.size());
+    {
+        std::vector<uint32_t> packed((T.n + 3) / 4, 0);
+        if (T.n) std::memcpy(packed.data(), T.cur_row.data(), T.n);
+        B.cur_row = put(packed.data(), packed.size());
+    }
+    {
+        std::vector<uint32_t> packed(H.inst_bytes.size() / 4);
+        if (!packed.empty()) std::memcpy(packed.data(), H.inst_bytes.data(), H.inst_bytes.size());
+        B.inst = put(packed.data(), packed.size());
+    }
+    B.nodes = put(H.nodes_flat.data(), H.nodes_flat.size());
+    B.slots = put(H.slots_flat.data(), H.slots_flat.size());
+    B.node_off = put(H.node_off.data(), H.node_off.size());
+    B.slot_off = put(H.slot_off.data(), H.slot_off.size());
+    B.rptr = put(H.rptr.data(), H.rptr.size());
+    B.ridx = put(H.ridx.data(), H.ridx.size());
+    B.gptr = put(H.gptr.data(), H.gptr.size());
+    B.gidx = put(H.gidx.data(), H.gidx.size());
+    if (w.size() & 1) w.push_back(0);  // the lane block after it is 8-byte aligned
+    B.words = (uint32_t)w.size();
 }
 
 }  // namespace esp

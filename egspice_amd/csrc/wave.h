@@ -2,7 +2,9 @@
 //
 // One 64-thread workgroup owns one lane and keeps all of its state in LDS:
 // node voltages, the G plane and residual, the limiter images, the lane's
-// prepared device constants and its whole sparse direct solver. A Newton
+// prepared device constants and its whole sparse direct solver, after a
+// copy of the template's own tables (pattern, order, instance list, stamp
+// tape), so the serial solver's dependent loads never leave the CU. A Newton
 // tick splits into phases separated by workgroup barriers:
 //   eval     thread t evaluates instances t, t+64, ... (forward-mode AD and
 //            the limiting correction) into per-instance staging cells;
@@ -52,10 +54,50 @@ struct WaveTemplate {
     uint32_t cap;                 // L+U capacity per lane
 };
 
+static_assert(sizeof(InstEntry) == 24, "InstEntry packs into six u32 words");
+
+// The template's tables read from a copy of its blob (at `base`, u32 words).
+ESP_HD WaveTemplate wave_template(const WaveBlob &B, const uint32_t *base)
+{
+    WaveTemplate W;
+    W.P = Pattern{B.n, B.nnz, base + B.col_ptr, base + B.row_idx, base + B.q, B.tri ? base + B.tri_pos : nullptr,
+                  B.diag_fill, B.cap};
+    W.diag = base + B.diag;
+    W.cur_row = reinterpret_cast<const uint8_t *>(base + B.cur_row);
+    W.n_inst = B.n_inst;
+    W.inst = reinterpret_cast<const InstEntry *>(base + B.inst);
+    W.nodes_flat = base + B.nodes;
+    W.slots_flat = base + B.slots;
+    W.node_off = base + B.node_off;
+    W.slot_off = base + B.slot_off;
+    W.rptr = base + B.rptr;
+    W.ridx = base + B.ridx;
+    W.gptr = base + B.gptr;
+    W.gidx = base + B.gidx;
+    W.n_stage = B.n_stage;
+    W.n_prm = B.n_prm;
+    W.n_lim = B.n_lim;
+    W.cap = B.cap;
+    return W;
+}
+
 // LDS offsets (in 8-byte words) of one lane's block, computed on the host.
 struct WaveLayout {
-    uint32_t x, x_old, dx, g, rhs, lim, stage, prm, lu_f64, lu_u32, lu_u8, flags, words;
+    uint32_t x, x_old, dx, g, rhs, lim, stage, prm, lu_f64, lu_u32, lu_u8, flags;
+    uint32_t dense;  // wave_lu.h's dense replay arrays, or ESP_NONE when n > 64
+    uint32_t words;
 };
+
+// Largest pattern the dense wave replay takes: one pivot row per thread of
+// the lane's 64-wide wavefront.
+constexpr uint32_t DENSE_MAX_N = 64;
+
+// 8-byte words of the dense replay arrays: F (n*n f64), the two row masks
+// (2n u64), amap (n*n u32) and porig (n u32).
+ESP_HD size_t dense_words(uint32_t n)
+{
+    return (size_t)n * n + 2 * (size_t)n + ((size_t)n * n + 1) / 2 + ((size_t)n + 1) / 2;
+}
 
 // The lane's block carved into typed views (stride 1).
 struct WaveLane {
@@ -88,6 +130,12 @@ ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L)
     size_t u8w = 3 + 2 * n;
     L.lu_u8 = (uint32_t)at; at += (u8w + 7) / 8;
     L.flags = (uint32_t)at; at += 4;  // 8 u32 flags
+    if (n <= DENSE_MAX_N) {
+        L.dense = (uint32_t)at;
+        at += dense_words((uint32_t)n);
+    } else {
+        L.dense = ESP_NONE;
+    }
     L.words = (uint32_t)at;
     return at;
 }
@@ -250,13 +298,12 @@ ESP_HD void wave_fresh_solver(const WaveTemplate &W, WaveLane &s)
     for (uint32_t i = 0; i < W.P.n; i++) u.w[i] = 0.0;
 }
 
-// Thread 0's part of a tick after the planes: ground stamp, gmin, the
-// runaway bound, factor and solve, update and score. Returns false when
-// the lane is singular.
-ESP_HD bool wave_solve(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, double &limit, double &scaled)
+// Thread 0's finish of the planes: ground stamp, gmin, and at tick 0 the
+// runaway bound (converger.zig loadExtras/plausibleLimit).
+ESP_HD void wave_assemble(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, double &limit)
 {
     const uint32_t n = W.P.n;
-    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1}, x_old{s.x_old, 1}, dx{s.dx, 1};
+    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1};
     g[W.diag[0]] += 1.0;
     rhs[0] += x[0];
     if (o.gmin > 0) {
@@ -264,6 +311,16 @@ ESP_HD bool wave_solve(const WaveTemplate &W, WaveLane &s, const Newton &o, uint
         for (uint32_t i = 0; i < n; i++) rhs[i] += o.gmin * x[i];
     }
     if (tick == 0) limit = plausible_limit(n, x, rhs, g, W.diag);
+}
+
+// Thread 0's part of a tick after the planes: ground stamp, gmin, the
+// runaway bound, factor and solve, update and score. Returns false when
+// the lane is singular.
+ESP_HD bool wave_solve(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, double &limit, double &scaled)
+{
+    const uint32_t n = W.P.n;
+    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1}, x_old{s.x_old, 1}, dx{s.dx, 1};
+    wave_assemble(W, s, o, tick, limit);
     if (direct_factor(W.P, s.lu, g) != 0) return false;
     direct_solve_neg(W.P, s.lu, rhs, dx);
     scaled = update_norm(n, x, dx, x_old, W.cur_row, o.reltol, o.abstol, o.vntol);

@@ -1,0 +1,358 @@
+// Wave-parallel replay of a lane's direct solver, for patterns of at most
+// 64 unknowns: the lane's 64-wide wavefront runs the refactor and the two
+// triangular solves with thread r owning pivot row r.
+//
+// The serial bodies in lane_solver.h (the reference's SparseLu replay,
+// sparse_lu.zig:1066-1159, and luSolve) walk a column's U entries in the
+// tape's topological order and, for each, subtract its L column from the
+// scattered work vector. Here the work vector lives in registers (one entry
+// per thread), each U value is broadcast with a lane read, and every L row
+// of it updates its own entry at once. Each row still receives the same
+// operations in the same order, and the mask of the row's structural
+// entries keeps structural zeros out of it (no `w - 0*u`, whose sign of
+// zero could differ), so the factor and the solution are bitwise the
+// serial ones. Failure checks that the serial code makes column by column
+// are deferred to one vote at the end: a failed replay's partial factor is
+// never read (the caller re-pivots), so only the verdict has to agree.
+//
+// Values are kept dense in pivot coordinates, F[k*n + r] = L(r,k) for
+// r > k, U(r,k) for r < k and the pivot for r == k; a thread reads and
+// writes only its own row of F except for the pivots, so no barrier is
+// needed inside the replay. The full factor (DFS reach, threshold pivot
+// search) stays serial on thread 0 in lane_solver.h; after it the dense
+// arrays are rebuilt from its tape.
+#pragma once
+
+#include "wave.h"
+
+namespace esp {
+
+struct DenseLU {
+    double *F;        // n*n, column k at F + k*n, rows in pivot order
+    uint64_t *lmask;  // n: bit k of lmask[r] when L(r,k) is structural
+    uint64_t *umask;  // n: bit k of umask[r] when U(r,k) is structural
+    uint32_t *amap;   // n*n: pattern slot of A at (pivot row r, step k), or ESP_NONE
+    uint32_t *porig;  // n: original row of pivot row t
+};
+
+ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n)
+{
+    DenseLU d;
+    double *f = block + L.dense;
+    d.F = f;
+    f += (size_t)n * n;
+    d.lmask = reinterpret_cast<uint64_t *>(f);
+    f += n;
+    d.umask = reinterpret_cast<uint64_t *>(f);
+    f += n;
+    d.amap = reinterpret_cast<uint32_t *>(f);
+    f += ((size_t)n * n + 1) / 2;
+    d.porig = reinterpret_cast<uint32_t *>(f);
+    return d;
+}
+
+// ----------------------------------------------------------- wave helpers
+
+// Lane `src` (wave-uniform) of a per-thread double.
+__device__ inline double lane_read(double v, uint32_t src)
+{
+    const uint64_t b = bits(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)src);
+    return from_bits((uint64_t)hi << 32 | lo);
+}
+
+__device__ inline uint64_t wave_ballot(bool p) { return __ballot(p ? 1 : 0); }
+
+__device__ inline int64_t wave_max_i64(int64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------- the tape
+
+// Dense arrays from the serial full factor's tape (sparse lx/li/lp,
+// ux/ui/up, udiag, prow, pinv). All threads; ends with a barrier.
+__device__ inline void dense_build(const Pattern &P, const LaneLU &s, const DenseLU &D)
+{
+    const uint32_t n = P.n, tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = tid; i < n; i += nt) {
+        D.lmask[i] = 0;
+        D.umask[i] = 0;
+        D.porig[s.pinv[i]] = i;
+    }
+    for (uint32_t i = tid; i < n * n; i += nt) D.amap[i] = ESP_NONE;
+    __syncthreads();
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t c = P.q[k];
+        for (uint32_t p = s.lp[k] + tid; p < s.lp[k + 1]; p += nt) {
+            const uint32_t r = s.li[p];
+            D.F[k * n + r] = s.lx[p];
+            atomicOr(reinterpret_cast<unsigned long long *>(&D.lmask[r]), 1ull << k);
+        }
+        for (uint32_t p = s.up[k] + tid; p < s.up[k + 1]; p += nt) {
+            const uint32_t r = s.ui[p];
+            D.F[k * n + r] = s.ux[p];
+            atomicOr(reinterpret_cast<unsigned long long *>(&D.umask[r]), 1ull << k);
+        }
+        if (tid == 0) D.F[k * n + k] = s.udiag[k];
+        for (uint32_t p = P.col_ptr[c] + tid; p < P.col_ptr[c + 1]; p += nt) D.amap[k * n + s.prow[p]] = p;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ the replay
+
+// Replay.column on the lane's own pivot sequence (lane_solver.h
+// lu_refactor) with the multiplier scan of max_multiplier folded in.
+// Returns true when the replay fails; `mm` is the max multiplier of a
+// successful one. All threads; uniform result.
+template <class A>
+__device__ bool dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals, double &mm)
+{
+    const uint32_t n = P.n, r = threadIdx.x, nt = blockDim.x;
+    const bool own = r < n;
+    const uint32_t nvoid = s.cnt[C_NVOID];
+    bool live_void = false;
+    for (uint32_t t = r; t < nvoid; t += nt) live_void = live_void || vals[s.void_slots[t]] != 0;
+    if (wave_ballot(live_void)) return true;
+    const uint64_t lm = own ? D.lmask[r] : 0;
+    const uint64_t scaled = wave_ballot(own && s.scaled_pivot[r] != 0);
+    const uint64_t voided = wave_ballot(own && s.void_col[r] != 0);
+    uint64_t nonfinite = 0, u_nonfinite = 0;
+    bool growth_fail = false, pivot_fail = false;
+    int64_t mm_all = 0, mm_kept = 0;
+    double w = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t a = own ? D.amap[k * n + r] : ESP_NONE;
+        w = a != ESP_NONE ? vals[a] : 0.0;
+        const uint32_t u1 = s.up[k + 1];
+        for (uint32_t p = s.up[k]; p < u1; p++) {
+            const uint32_t i = s.ui[p];
+            const double uki = lane_read(w, i);
+            if (r == i) {
+                w = 0;
+                D.F[k * n + r] = uki;
+            }
+            u_nonfinite |= bits(uki - uki);
+            if (lm >> i & 1) w -= D.F[i * n + r] * uki;
+        }
+        if (voided >> k & 1) {
+            if (r == k) D.F[k * n + k] = 1;
+            if (lm >> k & 1) D.F[k * n + r] = 0;
+            w = 0;
+            continue;
+        }
+        const double d = lane_read(w, k);
+        if (r == k) {
+            w = 0;
+            D.F[k * n + k] = d;
+        }
+        pivot_fail = pivot_fail || d == 0 || !isfinite_(d);
+        if (lm >> k & 1) {
+            const double v = w;
+            w = 0;
+            const double l = v / d;
+            D.F[k * n + r] = l;
+            const int64_t lb = (int64_t)bits(fabs_(l));
+            if (!(scaled >> k & 1)) {
+                // growth: |d| against every unscaled step's column entry
+                growth_fail = growth_fail || !(fabs_(d) >= GROWTH_LIMIT * fabs_(v));
+                mm_kept = lb > mm_kept ? lb : mm_kept;
+            } else {
+                nonfinite |= bits(l - l);
+            }
+            mm_all = lb > mm_all ? lb : mm_all;
+        }
+    }
+    const bool bad = pivot_fail || u_nonfinite != 0 || wave_ballot(growth_fail || nonfinite != 0) != 0;
+    if (bad) return true;
+    // max_multiplier (lane_solver.h): the whole of L unless some pivot was
+    // taken by the scaled test, then only the other columns
+    const double all = from_bits((uint64_t)wave_max_i64(mm_all));
+    mm = (all != all || scaled == 0) ? all : from_bits((uint64_t)wave_max_i64(mm_kept));
+    return false;
+}
+
+// luSolve of -rhs into dx (lane_solver.h direct_solve_neg / lu_solve).
+// All threads; reads the pivots other threads wrote, so it must follow a
+// barrier; writes dx[q[r]] (needs a barrier before others read it).
+template <class R, class X>
+__device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs, X &dx)
+{
+    const uint32_t n = P.n, r = threadIdx.x;
+    const bool own = r < n;
+    const uint64_t lm = own ? D.lmask[r] : 0, um = own ? D.umask[r] : 0;
+    double y = own ? -1.0 * rhs[D.porig[r]] : 0.0;
+    for (uint32_t k = 0; k < n; k++) {
+        const double yk = lane_read(y, k);
+        if (yk == 0) continue;
+        if (lm >> k & 1) y -= D.F[k * n + r] * yk;
+    }
+    for (uint32_t k = n; k > 0;) {
+        k--;
+        const double zk = lane_read(y, k) / D.F[k * n + k];
+        if (r == k) y = zk;
+        if (zk == 0) continue;
+        if (um >> k & 1) y -= D.F[k * n + r] * zk;
+    }
+    if (own) dx[P.q[r]] = y;
+}
+
+// ------------------------------------------------------- replay's probe
+
+// lane_solver.h lu_solve / lu_solve_t on the dense values, serial.
+template <class X, class Y>
+__device__ void dense_lu_solve(const Pattern &P, const LaneLU &s, const DenseLU &D, const X &b, Y &x)
+{
+    const uint32_t n = P.n;
+    for (uint32_t r = 0; r < n; r++) s.y[s.pinv[r]] = b[r];
+    for (uint32_t k = 0; k < n; k++) {
+        double yk = s.y[k];
+        if (yk == 0) continue;
+        for (uint32_t p = s.lp[k]; p < s.lp[k + 1]; p++) s.y[s.li[p]] -= D.F[k * n + s.li[p]] * yk;
+    }
+    for (uint32_t k = n; k > 0;) {
+        k--;
+        double zk = s.y[k] / D.F[k * n + k];
+        s.y[k] = zk;
+        if (zk == 0) continue;
+        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) s.y[s.ui[p]] -= D.F[k * n + s.ui[p]] * zk;
+    }
+    for (uint32_t j = 0; j < n; j++) x[P.q[j]] = s.y[j];
+}
+
+template <class X, class Y>
+__device__ void dense_lu_solve_t(const Pattern &P, const LaneLU &s, const DenseLU &D, const X &b, Y &x)
+{
+    const uint32_t n = P.n;
+    for (uint32_t j = 0; j < n; j++) s.y[j] = b[P.q[j]];
+    for (uint32_t k = 0; k < n; k++) {
+        double v = s.y[k];
+        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) v -= D.F[k * n + s.ui[p]] * s.y[s.ui[p]];
+        s.y[k] = v / D.F[k * n + k];
+    }
+    for (uint32_t k = n; k > 0;) {
+        k--;
+        double v = s.y[k];
+        for (uint32_t p = s.lp[k]; p < s.lp[k + 1]; p++) v -= D.F[k * n + s.li[p]] * s.y[s.li[p]];
+        s.y[k] = v;
+    }
+    for (uint32_t r = 0; r < n; r++) x[r] = s.y[s.pinv[r]];
+}
+
+// The probe half of replay_holds (lane_solver.h), on thread 0.
+template <class A>
+__device__ bool dense_probe_holds(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals)
+{
+    const uint32_t n = P.n;
+    for (uint32_t i = 0; i < n; i++) s.prows[i] = 0;
+    double norm_1 = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        double col = 0;
+        for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++) {
+            double a = fabs_(vals[p]);
+            s.prows[P.row_idx[p]] += a;
+            col += a;
+        }
+        if (col > norm_1) norm_1 = col;
+    }
+    double norm_inf_a = norm_inf_lv(s.prows, 0, n);
+    double nf = (double)n;
+    for (int transpose = 0; transpose < 2; transpose++) {
+        probe_rhs(s.pr, n);
+        if (transpose) dense_lu_solve_t(P, s, D, s.pr, s.px);
+        else dense_lu_solve(P, s, D, s.pr, s.px);
+        probe_rhs(s.pr, n);
+        for (uint32_t j = 0; j < n; j++)
+            for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++) {
+                uint32_t i = P.row_idx[p];
+                if (transpose) s.pr[j] -= vals[p] * s.px[i];
+                else s.pr[i] -= vals[p] * s.px[j];
+            }
+        double a_norm = transpose ? norm_1 : norm_inf_a;
+        double ratio = norm_inf_lv(s.pr, 0, n) / (a_norm * norm_inf_lv(s.px, 0, n) * nf * EPS);
+        if (!(ratio <= 30.0 / PIVOT_TOL)) return false;
+    }
+    return true;
+}
+
+// --------------------------------------------------- direct.Solver.factor
+
+// lane_solver.h direct_factor for one lane on its wavefront: 0 factored,
+// 1 singular. `vals` is the lane's G plane in LDS; `flag` one LDS word for
+// thread 0's verdicts. All threads; uniform result.
+template <class A>
+__device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D, const A &vals, uint32_t *flag)
+{
+    const uint32_t nnz = P.nnz, tid = threadIdx.x, nt = blockDim.x;
+    __syncthreads();
+    if (nnz != 0 && s.state[ST_FACTORED]) {
+        bool differs = false;
+        for (uint32_t p = tid; p < nnz; p += nt) differs = differs || !(vals[p] == s.vcopy[p]);
+        if (!wave_ballot(differs)) return 0;
+    }
+    if (s.state[ST_TRI]) {
+        // the Thomas engine, or its one-time demotion to the LU, serially
+        if (tid == 0) *flag = (uint32_t)direct_factor(P, s, vals);
+        __syncthreads();
+        const uint32_t r = *flag;
+        if (r == 0 && !s.state[ST_TRI]) dense_build(P, s, D);
+        __syncthreads();
+        return (int)r;
+    }
+    if (s.state[ST_FACTORED]) {
+        double mm = 0;
+        const bool failed = dense_refactor(P, s, D, vals, mm);
+        bool holds = !failed && mm <= 1 / PIVOT_TOL;
+        if (!failed && !holds) {
+            __syncthreads();
+            if (tid == 0) *flag = dense_probe_holds(P, s, D, vals) ? 1u : 0u;
+            __syncthreads();
+            holds = *flag != 0;
+        }
+        if (holds) {
+            for (uint32_t p = tid; p < nnz; p += nt) s.vcopy[p] = vals[p];
+            __syncthreads();
+            return 0;
+        }
+        __syncthreads();
+        if (tid == 0) s.state[ST_FACTORED] = 0;
+    }
+    if (tid == 0) {
+        uint32_t r = (uint32_t)lu_factor(P, s, vals);
+        if (r == 0) s.state[ST_FACTORED] = 1;
+        *flag = r;
+    }
+    __syncthreads();
+    if (*flag) return 1;
+    for (uint32_t p = tid; p < nnz; p += nt) s.vcopy[p] = vals[p];
+    dense_build(P, s, D);
+    return 0;
+}
+
+// update_norm (engine.h) across the wavefront: x_old = x, x += dx and the
+// worst scaled step, as the max of the scores' bit patterns.
+__device__ inline double wave_update_norm(uint32_t n, double *x, const double *dx, double *x_old, const uint8_t *cur_row,
+                                          double reltol, double abstol, double vntol)
+{
+    int64_t worst = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double xo = x[i], d = dx[i], xn = xo + d;
+        x_old[i] = xo;
+        x[i] = xn;
+        double tol = reltol * fmax_(fabs_(xn), fabs_(xo)) + (cur_row[i] ? abstol : vntol);
+        bool finite = fabs_(xn) < inf_() && fabs_(d) < inf_();
+        double q = fabs_(d) / tol;
+        double score = !finite ? inf_() : (q == q ? q : 0);
+        int64_t b = (int64_t)bits(score);
+        if (b > worst) worst = b;
+    }
+    return from_bits((uint64_t)wave_max_i64(worst));
+}
+
+}  // namespace esp

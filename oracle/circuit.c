@@ -28,6 +28,7 @@
  * of every converged lane; FD Jacobian).
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -601,46 +602,107 @@ int or_lead_tab(or_lanes *L, const double *x, int init_fix, or_lane_tab *tab, ui
     return rc;
 }
 
-int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out)
+/* Shared, read-only state of one or_newton_op call; workers take lanes
+ * first + k*stride. The reference's CPU path solves each point on its own
+ * (op.zig:588,671) with the template's one column order. */
+typedef struct {
+    or_lanes *L;
+    int mode;
+    const or_newton_opts *o;
+    double *x;
+    or_outcome *out;
+    or_lane_tab tab;
+    uint32_t *idx;
+    const uint32_t *q;
+    uint64_t diag_fill;
+    uint32_t first, stride;
+    int rc;
+} op_job;
+
+static void *op_worker(void *arg)
 {
+    op_job *j = arg;
+    or_lanes *L = j->L;
     const or_circuit *c = L->c;
     const uint32_t N = L->N, n = c->n;
-    double *xl = malloc(8 * n);
-    or_lane_tab tab;
-    uint32_t *idx = NULL;
-    size_t len = 0;
-    if (mode == 0 && or_lead_tab(L, x, o->init_fix, &tab, &idx, &len) != 0) {
-        free(xl);
-        return -1;
-    }
-    for (uint32_t l = 0; l < N; l++) {
-        for (uint32_t i = 0; i < n; i++) xl[i] = x[(size_t)i * N + l];
+    const or_newton_opts *o = j->o;
+    double *xl = malloc(8 * (n ? n : 1));
+    for (uint32_t l = j->first; l < N; l += j->stride) {
+        for (uint32_t i = 0; i < n; i++) xl[i] = j->x[(size_t)i * N + l];
         if (o->init_fix) seed_pass(L, l, xl);
-        if (mode == 0) {
-            tape_ctx t = {&tab, idx, NULL, NULL, NULL, NULL, NULL, c, NULL, 0};
-            t.lx = calloc(tab.nnz_l + 1, 8);
-            t.ux = calloc(tab.nnz_u + 1, 8);
+        if (j->mode == 0) {
+            tape_ctx t = {&j->tab, j->idx, NULL, NULL, NULL, NULL, NULL, c, NULL, 0};
+            t.lx = calloc(j->tab.nnz_l + 1, 8);
+            t.ux = calloc(j->tab.nnz_u + 1, 8);
             t.udiag = calloc(n, 8);
             t.w = calloc(n, 8);
             t.y = calloc(n, 8);
-            newton_one(L, l, o, xl, &out[l], lin_tape, &t, o->init_fix);
-            if (t.peels) out[l].status |= 4u;
+            newton_one(L, l, o, xl, &j->out[l], lin_tape, &t, o->init_fix);
+            if (t.peels) j->out[l].status |= 4u;
             or_solver_free(t.own);
             free(t.lx); free(t.ux); free(t.udiag); free(t.w); free(t.y);
         } else {
-            or_solver *s = or_solver_new(n, c->col_ptr, c->row_idx);
+            or_solver *s = or_solver_new_ordered(n, c->col_ptr, c->row_idx, j->q, j->diag_fill);
             if (!s) {
-                free(xl);
-                return -1;
+                j->rc = -1;
+                break;
             }
-            newton_one(L, l, o, xl, &out[l], lin_direct, s, o->init_fix);
+            newton_one(L, l, o, xl, &j->out[l], lin_direct, s, o->init_fix);
             or_solver_free(s);
         }
-        for (uint32_t i = 0; i < n; i++) x[(size_t)i * N + l] = xl[i];
+        for (uint32_t i = 0; i < n; i++) j->x[(size_t)i * N + l] = xl[i];
     }
-    free(idx);
     free(xl);
-    return 0;
+    return NULL;
+}
+
+int or_newton_op_mt(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out, uint32_t threads)
+{
+    const or_circuit *c = L->c;
+    const uint32_t n = c->n;
+    op_job base;
+    memset(&base, 0, sizeof base);
+    base.L = L;
+    base.mode = mode;
+    base.o = o;
+    base.x = x;
+    base.out = out;
+    size_t len = 0;
+    if (mode == 0 && or_lead_tab(L, x, o->init_fix, &base.tab, &base.idx, &len) != 0) return -1;
+    uint32_t *q = malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (or_order(n, c->col_ptr, c->row_idx, q) != 0) {
+        free(q);
+        free(base.idx);
+        return -1;
+    }
+    base.q = q;
+    base.diag_fill = or_diag_fill(n, c->col_ptr, c->row_idx, q);
+    if (threads < 1) threads = 1;
+    if (threads > L->N) threads = L->N ? L->N : 1;
+    op_job *jobs = malloc(sizeof(op_job) * threads);
+    pthread_t *tid = malloc(sizeof(pthread_t) * threads);
+    for (uint32_t t = 0; t < threads; t++) {
+        jobs[t] = base;
+        jobs[t].first = t;
+        jobs[t].stride = threads;
+    }
+    for (uint32_t t = 1; t < threads; t++) pthread_create(&tid[t], NULL, op_worker, &jobs[t]);
+    op_worker(&jobs[0]);
+    int rc = jobs[0].rc;
+    for (uint32_t t = 1; t < threads; t++) {
+        pthread_join(tid[t], NULL);
+        if (jobs[t].rc) rc = jobs[t].rc;
+    }
+    free(jobs);
+    free(tid);
+    free(q);
+    free(base.idx);
+    return rc;
+}
+
+int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out)
+{
+    return or_newton_op_mt(L, mode, o, x, out, 1);
 }
 
 /* ------------------------------------------------------------- dc sweep */

@@ -5,7 +5,8 @@
  *                       Thomas engine, which has no lane tape; GPU lanes
  *                       refuse such a pattern, lanes.zig:595-605)
  *   or_solver_new       direct.zig:147-172, initLu :218-240 (BTF+AMD order,
- *                       diag_fill = max(diagFill, nnz))
+ *                       diag_fill = max(diagFill, nnz)); _ordered shares
+ *                       one order across the instances of a pattern
  *   or_solver_factor    direct.zig:310-320, factorInner :348-383 (unchanged
  *                       bypass, refactor + replayHolds, else full factor;
  *                       fill_cap 3, keep_min = growth limit 1e-12)
@@ -88,7 +89,11 @@ static void tri_solve(const or_solver *s, double *x)
 
 int or_solver_is_tri(const or_solver *s) { return s->tri; }
 
-or_solver *or_solver_new(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx)
+/* A solver on a column order computed once for the pattern and shared by
+ * every instance of it (device/Circuit.zig:55-60 ColumnOrder, direct.zig:
+ * 143-152 initOrdered): `q` is copied, `diag_fill` is or_diag_fill's count. */
+or_solver *or_solver_new_ordered(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, const uint32_t *q,
+                                 uint64_t diag_fill)
 {
     or_solver *s = calloc(1, sizeof(or_solver));
     if (or_is_tridiag(n, col_ptr, row_idx)) {
@@ -112,19 +117,25 @@ or_solver *or_solver_new(uint32_t n, const uint32_t *col_ptr, const uint32_t *ro
     s->col_ptr = col_ptr;
     s->row_idx = row_idx;
     s->q = malloc(sizeof(uint32_t) * (n ? n : 1));
-    if (or_order(n, col_ptr, row_idx, s->q) != 0) {
-        free(s->q);
-        free(s->a_pos);
-        free(s->bp);
-        free(s);
-        return NULL;
-    }
-    uint64_t df = or_diag_fill(n, col_ptr, row_idx, s->q);
+    if (n) memcpy(s->q, q, sizeof(uint32_t) * n);
+    uint64_t df = diag_fill;
     if (df < col_ptr[n]) df = col_ptr[n];
     s->diag_fill = df;
     s->lu = or_lu_new(n, col_ptr, s->q);
     or_lu_set_caps(s->lu, 0, 0, df);
     s->vcopy = malloc(sizeof(double) * (col_ptr[n] ? col_ptr[n] : 1));
+    return s;
+}
+
+or_solver *or_solver_new(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx)
+{
+    uint32_t *q = malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (or_order(n, col_ptr, row_idx, q) != 0) {
+        free(q);
+        return NULL;
+    }
+    or_solver *s = or_solver_new_ordered(n, col_ptr, row_idx, q, or_diag_fill(n, col_ptr, row_idx, q));
+    free(q);
     return s;
 }
 

@@ -86,6 +86,8 @@ typedef struct or_solver or_solver;
 /* NULL when the pattern is tridiagonal (that engine has no lane tape and
  * is not restated) or ordering ran out of workspace. */
 or_solver *or_solver_new(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
+or_solver *or_solver_new_ordered(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx, const uint32_t *q,
+                                 uint64_t diag_fill);
 void or_solver_free(or_solver *s);
 int or_solver_factor(or_solver *s, const double *vals);
 void or_solver_solve_neg(or_solver *s, const double *rhs, double *dx);

@@ -72,6 +72,8 @@ typedef struct { uint32_t status; uint32_t iterations; } or_outcome;
  * Returns 0, or -1 when the template has no lane tape (tridiagonal
  * pattern or a singular lead factor). */
 int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out);
+/* The same on `threads` POSIX threads, lanes dealt round robin. */
+int or_newton_op_mt(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out, uint32_t threads);
 /* The lead tape of lane 0 at x (mode 0's tape), for tests. */
 int or_lead_tab(or_lanes *L, const double *x, int init_fix, or_lane_tab *tab, uint32_t **idx, size_t *len);
 /* One residual+Jacobian evaluation of lane l at x (n entries), unlimited,

@@ -171,26 +171,9 @@ void hl_newton_wave(void *hv, const Newton *o, double *x, uint32_t *status, uint
     Template *t = h->t;
     const uint32_t N = h->N, n = t->n, nnz = t->nnz, nt = 64;
     WaveHost wh;
-    build_wave(*t, wh);
-    WaveTemplate W;
-    W.P = Pattern{n, nnz, t->col_ptr.data(), t->row_idx.data(), t->q.data(), t->tridiagonal ? t->tri_pos.data() : nullptr,
-                  t->diag_fill, h->cap};
-    W.diag = t->diag.data();
-    W.cur_row = h->cur_row.data();
-    W.n_inst = wh.n_inst;
-    W.inst = reinterpret_cast<const InstEntry *>(wh.inst_bytes.data());
-    W.nodes_flat = wh.nodes_flat.data();
-    W.slots_flat = wh.slots_flat.data();
-    W.node_off = wh.node_off.data();
-    W.slot_off = wh.slot_off.data();
-    W.rptr = wh.rptr.data();
-    W.ridx = wh.ridx.data();
-    W.gptr = wh.gptr.data();
-    W.gidx = wh.gidx.data();
-    W.n_stage = wh.n_stage;
-    W.n_prm = wh.n_prm;
-    W.n_lim = wh.n_lim;
-    W.cap = h->cap;
+    build_wave(*t, h->cap, wh);
+    // the kernel's view: every table read from the packed blob
+    const WaveTemplate W = wave_template(wh.at, wh.blob.data());
     WaveLayout lay;
     size_t words = wave_words(W, lay);
     std::vector<double> block(words);

@@ -84,6 +84,7 @@ def load():
         "or_lanes_set_param": (None, [P, C.c_int, C.c_uint32, C.c_uint32, f64p]),
         "or_lanes_prepare": (None, [P]),
         "or_newton_op": (C.c_int, [P, C.c_int, C.POINTER(NewtonOpts), f64p, C.POINTER(Outcome)]),
+        "or_newton_op_mt": (C.c_int, [P, C.c_int, C.POINTER(NewtonOpts), f64p, C.POINTER(Outcome), C.c_uint32]),
         "or_lead_tab": (C.c_int, [P, f64p, C.c_int, C.POINTER(LaneTab), C.POINTER(u32p), C.POINTER(C.c_size_t)]),
         "or_eval_lane": (None, [P, C.c_uint32, f64p, f64p, f64p]),
         "or_dc_sweep": (C.c_int, [P, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.POINTER(NewtonOpts), f64p, u32p]),
@@ -175,13 +176,13 @@ class Lanes:
     def prepare(self):
         self.lib.or_lanes_prepare(self.h)
 
-    def newton_op(self, mode=0, x0=None, **kw):
+    def newton_op(self, mode=0, x0=None, threads=1, **kw):
         n, N = self.ckt.n, self.N
         x = np.zeros(n * N) if x0 is None else np.array(x0, dtype=np.float64).reshape(-1)
         xa, xp = f64(x)
         out = (Outcome * N)()
         o = newton_opts(**kw)
-        rc = self.lib.or_newton_op(self.h, mode, C.byref(o), xp, out)
+        rc = self.lib.or_newton_op_mt(self.h, mode, C.byref(o), xp, out, threads)
         if rc != 0:
             raise RuntimeError("no lane tape")
         status = np.array([out[i].status for i in range(N)], dtype=np.uint32)

@@ -142,3 +142,21 @@ def test_lane_tape_driver_agrees_with_cpu_path(oracle):
     assert (s0 & 1).all() and (s1 & 1).all()
     # both publish x_k inside the Newton tolerance of the same root
     assert np.abs(x0 - x1).max() <= 1e-5
+
+
+def test_threaded_driver_is_the_serial_one(oracle):
+    """Lanes are independent: dealing them over threads changes no bit."""
+    c = O.Circuit(oracle)
+    W.inverter_chain(c.add)
+    c.freeze()
+    rng = np.random.default_rng(4)
+    N = 96
+    L = O.Lanes(c, N)
+    L.set_param(O.VSRC, 1, O.VSRC_DC, rng.uniform(0, 1.8, N))
+    L.set_param(O.MOS1, 1, O.M1["VTO"], -0.7 + 0.02 * rng.standard_normal(N))
+    L.prepare()
+    ref = L.newton_op(mode=1)
+    for threads in (2, 7):
+        got = L.newton_op(mode=1, threads=threads)
+        assert np.array_equal(ref[0].view(np.uint64), got[0].view(np.uint64))
+        assert np.array_equal(ref[1], got[1]) and np.array_equal(ref[2], got[2])
