@@ -1,17 +1,23 @@
 // The lane engine: circuit templates, lane batches resident on one MI355X,
 // the gfx950 kernels, and the C ABI of include/espice_lanes.h.
 //
-// Kernels (one thread per lane, 64-lane wavefronts, lane-major planes):
-//   newton_op_kernel      the whole static Newton of a lane in one launch:
-//                         device evaluation and MNA stamping, ground pin,
-//                         the lane's own sparse direct factor and solve,
-//                         update, limiting and the gates, per tick, until the
-//                         lane converges, goes singular or runs out of ticks.
-//                         The reference runs the same tick as two captured
-//                         graphs of ~10 kernels with a host sync per tick
-//                         (src/analysis/gpu/lanes.zig:1431-1601); here the
-//                         tick loop lives on the device, so a solve is one
-//                         launch and one sync.
+// Kernels:
+//   newton_wave_kernel    the whole static Newton of a lane in one launch,
+//                         one 64-thread workgroup per lane with the lane's
+//                         state and the template's tables in LDS (wave.h):
+//                         device evaluation into staging cells, the ordered
+//                         stamp reduction, the lane's own direct factor and
+//                         solve (wave-parallel for up to 64 unknowns,
+//                         wave_lu.h), update, limiting and the gates, per
+//                         tick, until the lane converges, goes singular or
+//                         runs out of ticks. The reference runs the same tick
+//                         as two captured graphs of ~10 kernels with a host
+//                         sync per tick (src/analysis/gpu/lanes.zig:
+//                         1431-1601); here the tick loop lives on the device,
+//                         so a solve is one launch and one sync.
+//   newton_op_kernel      the same Newton one thread per lane over lane-major
+//                         global planes, for templates whose lane block does
+//                         not fit in LDS.
 //   tab_refactor_kernel   the reference's `arp_lane_refactor` + `arp_lane_solve`
 //                         over a shared lead tape (lane_device.zig:101,170).
 #include <hip/hip_runtime.h>

@@ -1,5 +1,6 @@
 /* espice lane engine C ABI: many instances (lanes) of one circuit template
- * solved together on an MI355X (gfx950), one lane per GPU thread.
+ * solved together on an MI355X (gfx950), one 64-wide workgroup per lane
+ * with the lane's whole state in LDS.
  *
  * This is the seam the reference's lane driver calls in place of its Zig
  * GPU lane backend: `GpuLanes` (src/analysis/gpu/lanes.zig) and its

@@ -22,11 +22,11 @@
  *   diode.va    rs = 0 (collapsed), no breakdown/ikf/ikr/recombination
  *   mos1.va     rd = rs = 0 (di/si collapsed onto d/s), static currents,
  *               bulk junctions; Meyer gate charge is not restated yet
- * Parity pinned by: the docs golden DC vector of examples/diode_iv.sp
+ * Parity pinned by: the docs golden DC vector of docs/examples/diode_iv.sp
  * (docs/learn/analyses.md:88-93, i(v1) at 10 and 20 mV to full precision),
- * the FD Jacobian of every body (tests/test_oracle_devices.py, the
+ * the FD Jacobian of every body (tests/test_oracle_newton.py, the
  * reference's device/tests/jacobian_fd.zig property) and KCL at every
- * converged point. Agreement with VerA's generated mos1 bits: parity
+ * converged point (an independent Shichman-Hodges evaluation, same file). Agreement with VerA's generated mos1 bits: parity
  * unpinned (compiler absent).
  */
 #include <math.h>

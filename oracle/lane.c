@@ -15,9 +15,11 @@
  * Parity pinned by: tests/test_oracle_lane.py, which replays
  * lane.zig:811-908 (bodies bitwise the scalar SparseLu refactor and
  * solve, failing exactly where the replay fails or a multiplier runs wide)
- * and lane.zig:910-968 (update/plausible bitwise the W=1 converger forms)
- * against this file's own sparse_lu.c and the converger restatement in
- * newton.c, plus the dense solve against numpy.linalg.solve.
+ * against this file's own sparse_lu.c; the dense solve against
+ * numpy.linalg.solve (tests/test_oracle_lu.py). The update, plausible,
+ * residual and plausible-limit forms are the Newton driver's own
+ * (circuit.c), pinned with it by tests/test_oracle_newton.py (the
+ * diode_iv golden rows, KCL at every converged lane).
  */
 #include <math.h>
 #include <stdlib.h>

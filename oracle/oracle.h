@@ -12,9 +12,9 @@
  *   direct.c     src/solver/direct.zig (engine choice, bypass, replay probe)
  *   lane.c       src/solver/kernels/lane.zig (LaneTab, per-lane bodies, dense peel)
  *   devices.c    models/{vsource,resistor,capacitor,diode,mos1}.va bodies
- *   circuit.c    src/device/Circuit.zig freeze + src/analysis/Circuit.zig planes
- *   newton.c     src/solver/converger.zig newton/finalizeStep and the lane
- *                Newton of src/solver/lane_device.zig
+ *   circuit.c    src/device/Circuit.zig freeze + src/analysis/Circuit.zig planes,
+ *                src/solver/converger.zig newton/finalizeStep and the lane
+ *                Newton of src/solver/lane_device.zig, the DC sweep
  *
  * Parity status of each function is recorded in the header comment of its
  * file ("parity pinned by ..." or "parity unpinned").
