@@ -84,6 +84,24 @@ def cpu_baseline(n_lanes, budget_s=10.0):
                       f"oracle CPU path on {threads} threads, {platform.processor() or platform.machine()}"}
 
 
+def copy_calibration(mib):
+    """Five device-to-device copies of `mib` MiB (hipMemcpy D2D, the
+    runtime's copy kernel): a kernel whose HBM traffic is known exactly
+    (mib MiB read and mib MiB written per dispatch), against which the
+    profiler's FETCH_SIZE and WRITE_SIZE counters are scaled."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == src[-4:]).all())
+    print(json.dumps({"copy_calibration_mib": mib, "dispatches": 5, "bytes_read_per_dispatch": mib << 20,
+                      "bytes_written_per_dispatch": mib << 20}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +110,13 @@ def main():
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (the reference kernel for "
+                         "scaling profiler byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(lanes shard across GPUs by contiguous lane ranges, a design not built here)", file=sys.stderr)

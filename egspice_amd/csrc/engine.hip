@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const uint32_t n = B.n, nnz = B.nnz;
+    const uint32_t n = B.n;
     // the template's tables first (B.words is even), then the lane block
     uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
     for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
@@ -117,6 +117,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
         for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
         lim_active = true;
     }
+    const ReduceCache rc = wave_reduce_cache(W, tid);
     uint32_t bits = o.init_fix ? G_INITFIX : 0;
     double limit = 1e9, scaled = 0;
     uint32_t st = 0, it = o.max_iter;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     for (uint32_t tick = 0; tick < o.max_iter; tick++) {
         for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
         __syncthreads();
-        for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_entry(W, s, e);
+        wave_reduce(W, s, rc, tid, nt);
         __syncthreads();
         if (DENSE) {
             if (tid == 0) wave_assemble(W, s, o, tick, limit);

@@ -196,95 +196,203 @@ struct LP {
     ESP_HD double operator[](int k) const { return p[k]; }
 };
 
+// The per-instance phases are instantiated per device kind so that the
+// port count and the stamp masks are compile-time constants: every local
+// array then lives in registers (a runtime port count would index them
+// dynamically and push them to scratch memory).
+
 // Phase: evaluate instance e into its staging cells.
-ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+template <int KIND>
+ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active)
 {
-    const InstEntry ie = W.inst[e];
-    const int kind = ie.kind, nu = KIND_NU[kind];
+    constexpr int nu = KIND_NU[KIND];
+    constexpr unsigned limw = KIND_LIMW[KIND];
     const uint32_t *nd = W.nodes_flat + W.node_off[e];
     LP p{s.prm + ie.prm};
-    double lx[4], xl[4];
+    double lx[nu], xl[nu];
+#pragma unroll
     for (int u = 0; u < nu; u++) xl[u] = lx[u] = s.x[nd[u]];
-    const unsigned limw = KIND_LIMW[kind];
     const bool use_lim = limw != 0 && lim_active;
-    if (use_lim)
+    if (use_lim) {
+#pragma unroll
         for (int u = 0; u < nu; u++)
             if (limw >> u & 1) lx[u] = s.lim[ie.lim + u];
+    }
     DevOut o;
-    eval_kind(kind, p, lx, o);
+    eval_kind(KIND, p, lx, o);
     if (use_lim) {
-        double corr[4];
+        double corr[nu];
         bool live = false;
+#pragma unroll
         for (int u = 0; u < nu; u++) {
             corr[u] = xl[u] - lx[u];
             live = live || corr[u] != 0;
         }
-        if (live)
+        if (live) {
+#pragma unroll
             for (int ru = 0; ru < nu; ru++) {
                 double acc = 0;
+#pragma unroll
                 for (int u = 0; u < nu; u++) acc += o.jac[ru][u] * corr[u];
                 o.res[ru] += acc;
             }
+        }
     }
     uint32_t k = 0;
+#pragma unroll
     for (int ru = 0; ru < nu; ru++) {
         s.stage[ie.stage_res + ru] = o.res[ru];
-        const unsigned gm = kind_gmask(kind, ru);
+        const unsigned gm = kind_gmask(KIND, ru);
+#pragma unroll
         for (int cu = 0; cu < nu; cu++)
             if (gm >> cu & 1) s.stage[ie.stage_jac + k++] = o.jac[ru][cu];
     }
 }
 
-// Phase: residual row or G slot `e` of the planes (e < n rows, then slots).
-ESP_HD void wave_reduce_entry(const WaveTemplate &W, WaveLane &s, uint32_t e)
+ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
 {
-    const uint32_t n = W.P.n;
-    if (e < n) {
-        double acc = 0.0;
-        for (uint32_t c = W.rptr[e]; c < W.rptr[e + 1]; c++) acc += s.stage[W.ridx[c]];
-        s.rhs[e] = acc;
-    } else {
-        const uint32_t slot = e - n;
-        double acc = 0.0;
-        for (uint32_t c = W.gptr[slot]; c < W.gptr[slot + 1]; c++) acc += s.stage[W.gidx[c]];
-        s.g[slot] = acc;
+    const InstEntry ie = W.inst[e];
+    switch (ie.kind) {
+    case K_VSRC: wave_eval_k<K_VSRC>(W, s, ie, e, lim_active); break;
+    case K_ISRC: wave_eval_k<K_ISRC>(W, s, ie, e, lim_active); break;
+    case K_RES: wave_eval_k<K_RES>(W, s, ie, e, lim_active); break;
+    case K_CAP: wave_eval_k<K_CAP>(W, s, ie, e, lim_active); break;
+    case K_DIODE: wave_eval_k<K_DIODE>(W, s, ie, e, lim_active); break;
+    case K_MOS1: wave_eval_k<K_MOS1>(W, s, ie, e, lim_active); break;
+    default: break;
     }
 }
 
-// Phase: limit instance e (cur = x, old = image or x_old); true if clamped.
-ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+// Phase: residual row or G slot `e` of the planes (e < n rows, then slots):
+// the sum over its staging cells in tape order. The cell indices, then the
+// cells, are loaded eight at a time before the adds.
+ESP_HD void wave_reduce_entry(const WaveTemplate &W, WaveLane &s, uint32_t e)
 {
-    const InstEntry ie = W.inst[e];
-    const int kind = ie.kind, nu = KIND_NU[kind];
-    const unsigned limw = KIND_LIMW[kind];
-    if (limw == 0) return false;
+    const uint32_t n = W.P.n;
+    const uint32_t *ptr = e < n ? W.rptr + e : W.gptr + (e - n);
+    const uint32_t *idx = e < n ? W.ridx : W.gidx;
+    const uint32_t c0 = ptr[0], c1 = ptr[1];
+    double acc = 0.0;
+    for (uint32_t c = c0; c < c1; c += 8) {
+        uint32_t ix[8];
+        double v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) ix[k] = idx[c + k < c1 ? c + k : c1 - 1];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) v[k] = s.stage[ix[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++)
+            if (c + k < c1) acc += v[k];
+    }
+    if (e < n) s.rhs[e] = acc;
+    else s.g[e - n] = acc;
+}
+
+// The reduce phase with the stamp tape held in registers: thread t's first
+// entry (t < n + nnz) and up to RCACHE of its staging cells are read from
+// the tape once per launch; each tick then issues those cell loads back to
+// back and adds them in tape order. Entries past the first, or with more
+// cells, take wave_reduce_entry.
+constexpr uint32_t RCACHE = 16;
+
+struct ReduceCache {
+    uint32_t cnt;  // cells of the thread's first entry, or ESP_NONE when uncached
+    uint32_t idx[RCACHE];
+};
+
+ESP_HD ReduceCache wave_reduce_cache(const WaveTemplate &W, uint32_t tid)
+{
+    ReduceCache c;
+    c.cnt = ESP_NONE;
+    const uint32_t n = W.P.n, e = tid;
+    for (uint32_t k = 0; k < RCACHE; k++) c.idx[k] = 0;
+    if (e >= n + W.P.nnz) return c;
+    const uint32_t *ptr = e < n ? W.rptr + e : W.gptr + (e - n);
+    const uint32_t *idx = e < n ? W.ridx : W.gidx;
+    const uint32_t c0 = ptr[0], c1 = ptr[1];
+    if (c1 - c0 > RCACHE) return c;
+    c.cnt = c1 - c0;
+    for (uint32_t k = 0; k < RCACHE; k++) c.idx[k] = k < c.cnt ? idx[c0 + k] : (c.cnt ? idx[c0] : 0);
+    return c;
+}
+
+ESP_HD void wave_reduce(const WaveTemplate &W, WaveLane &s, const ReduceCache &c, uint32_t tid, uint32_t nt)
+{
+    const uint32_t n = W.P.n, total = n + W.P.nnz;
+    uint32_t e = tid;
+    if (c.cnt != ESP_NONE) {
+        double v[RCACHE];
+#pragma unroll
+        for (uint32_t k = 0; k < RCACHE; k++) v[k] = s.stage[c.idx[k]];
+        double acc = 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < RCACHE; k++)
+            if (k < c.cnt) acc += v[k];
+        if (e < n) s.rhs[e] = acc;
+        else s.g[e - n] = acc;
+        e += nt;
+    }
+    for (; e < total; e += nt) wave_reduce_entry(W, s, e);
+}
+
+// Phase: limit instance e (cur = x, old = image or x_old); true if clamped.
+template <int KIND>
+ESP_HD bool wave_limit_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active)
+{
+    constexpr int nu = KIND_NU[KIND];
+    constexpr unsigned limw = KIND_LIMW[KIND];
     const uint32_t *nd = W.nodes_flat + W.node_off[e];
     LP p{s.prm + ie.prm};
-    double cur[4], old[4], lim[4];
+    double cur[nu], old[nu], lim[nu];
+#pragma unroll
     for (int u = 0; u < nu; u++) {
         cur[u] = s.x[nd[u]];
         old[u] = ((limw >> u & 1) && lim_active) ? s.lim[ie.lim + u] : s.x_old[nd[u]];
     }
-    int ch = kind == K_MOS1 ? limit_mos1(p, cur, old, lim) : limit_diode(p, cur, old, lim);
+    int ch;
+    if constexpr (KIND == K_MOS1) ch = limit_mos1(p, cur, old, lim);
+    else ch = limit_diode(p, cur, old, lim);
+#pragma unroll
     for (int u = 0; u < nu; u++)
         if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
     return ch != 0;
 }
 
+ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+{
+    const InstEntry ie = W.inst[e];
+    switch (ie.kind) {
+    case K_DIODE: return wave_limit_k<K_DIODE>(W, s, ie, e, lim_active);
+    case K_MOS1: return wave_limit_k<K_MOS1>(W, s, ie, e, lim_active);
+    default: return false;
+    }
+}
+
+template <int KIND>
+ESP_HD void wave_seed_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e)
+{
+    constexpr int nu = KIND_NU[KIND];
+    constexpr unsigned limw = KIND_LIMW[KIND];
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    LP p{s.prm + ie.prm};
+    double cur[nu], lim[nu];
+#pragma unroll
+    for (int u = 0; u < nu; u++) cur[u] = s.x[nd[u]];
+    if constexpr (KIND == K_MOS1) seed_mos1(p, cur, lim);
+    else seed_diode(p, cur, lim);
+#pragma unroll
+    for (int u = 0; u < nu; u++)
+        if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
+}
+
 ESP_HD void wave_seed_inst(const WaveTemplate &W, WaveLane &s, uint32_t e)
 {
     const InstEntry ie = W.inst[e];
-    const int kind = ie.kind, nu = KIND_NU[kind];
-    const unsigned limw = KIND_LIMW[kind];
-    if (limw == 0) return;
-    const uint32_t *nd = W.nodes_flat + W.node_off[e];
-    LP p{s.prm + ie.prm};
-    double cur[4], lim[4];
-    for (int u = 0; u < nu; u++) cur[u] = s.x[nd[u]];
-    if (kind == K_MOS1) seed_mos1(p, cur, lim);
-    else seed_diode(p, cur, lim);
-    for (int u = 0; u < nu; u++)
-        if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
+    switch (ie.kind) {
+    case K_DIODE: wave_seed_k<K_DIODE>(W, s, ie, e); break;
+    case K_MOS1: wave_seed_k<K_MOS1>(W, s, ie, e); break;
+    default: break;
+    }
 }
 
 // Fresh solver state (direct.Solver right after init).

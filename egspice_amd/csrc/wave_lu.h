@@ -64,15 +64,6 @@ __device__ inline double lane_read(double v, uint32_t src)
 
 __device__ inline uint64_t wave_ballot(bool p) { return __ballot(p ? 1 : 0); }
 
-__device__ inline int64_t wave_max_i64(int64_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const int64_t t = __shfl_xor(v, o, 64);
-        v = t > v ? t : v;
-    }
-    return v;
-}
-
 // ------------------------------------------------------------- the tape
 
 // Dense arrays from the serial full factor's tape (sparse lx/li/lp,
@@ -107,75 +98,115 @@ __device__ inline void dense_build(const Pattern &P, const LaneLU &s, const Dens
 
 // ------------------------------------------------------------ the replay
 
+// Lane `src` (wave-uniform) of a per-thread u32.
+__device__ inline uint32_t lane_read_u32(uint32_t v, uint32_t src)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+}
+
+enum { REPLAY_HOLDS = 0, REPLAY_PROBE = 1, REPLAY_FAILED = 2 };
+
+// Loads of CHUNK consecutive U entries (or solve columns) are issued back
+// to back, unconditionally and in bounds, before their dependent updates,
+// so the LDS round trip is paid once per chunk.
+constexpr uint32_t CHUNK = 8;
+
 // Replay.column on the lane's own pivot sequence (lane_solver.h
-// lu_refactor) with the multiplier scan of max_multiplier folded in.
-// Returns true when the replay fails; `mm` is the max multiplier of a
-// successful one. All threads; uniform result.
+// lu_refactor), then replay_holds' multiplier bound: REPLAY_FAILED when
+// the replay fails, REPLAY_HOLDS when every multiplier is within
+// 1/pivot_tol (max_multiplier's columns), REPLAY_PROBE when the probe must
+// decide. All threads; uniform result.
+//
+// A U row is final once consumed (every later entry of the column only
+// updates rows below it in pivot order), so each thread keeps its row's
+// value in `w` and stores it, with the L multipliers and the pivot, once
+// at the end of the column. The next column's index list, scattered value
+// and amap entry are loaded while this column runs.
 template <class A>
-__device__ bool dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals, double &mm)
+__device__ int dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals)
 {
     const uint32_t n = P.n, r = threadIdx.x, nt = blockDim.x;
     const bool own = r < n;
+    const uint32_t rr = own ? r : n - 1;  // in-bounds row for unconditional loads
     const uint32_t nvoid = s.cnt[C_NVOID];
     bool live_void = false;
     for (uint32_t t = r; t < nvoid; t += nt) live_void = live_void || vals[s.void_slots[t]] != 0;
-    if (wave_ballot(live_void)) return true;
-    const uint64_t lm = own ? D.lmask[r] : 0;
+    if (wave_ballot(live_void)) return REPLAY_FAILED;
+    const uint64_t lm = own ? D.lmask[r] : 0, um = own ? D.umask[r] : 0;
     const uint64_t scaled = wave_ballot(own && s.scaled_pivot[r] != 0);
     const uint64_t voided = wave_ballot(own && s.void_col[r] != 0);
-    uint64_t nonfinite = 0, u_nonfinite = 0;
-    bool growth_fail = false, pivot_fail = false;
-    int64_t mm_all = 0, mm_kept = 0;
-    double w = 0;
+    // thread k holds column k's U range
+    const uint32_t ub = own ? s.up[r] : 0, ue = own ? s.up[r + 1] : 0;
+    uint64_t nonfinite = 0;
+    bool growth_fail = false, pivot_fail = false, wide_all = false, wide_kept = false, nan_l = false;
+    const uint32_t a0 = own ? D.amap[r] : ESP_NONE;
+    uint32_t a_next = own && n > 1 ? D.amap[n + r] : ESP_NONE;
+    double wv = a0 != ESP_NONE ? vals[a0] : 0.0;
+    uint32_t u0 = lane_read_u32(ub, 0), u1 = lane_read_u32(ue, 0);
+    uint32_t uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t a = own ? D.amap[k * n + r] : ESP_NONE;
-        w = a != ESP_NONE ? vals[a] : 0.0;
-        const uint32_t u1 = s.up[k + 1];
-        for (uint32_t p = s.up[k]; p < u1; p++) {
-            const uint32_t i = s.ui[p];
-            const double uki = lane_read(w, i);
-            if (r == i) {
-                w = 0;
-                D.F[k * n + r] = uki;
+        double w = wv;
+        const uint32_t cnt = u1 - u0, col_idx = uidx;
+        if (k + 1 < n) {
+            wv = a_next != ESP_NONE ? vals[a_next] : 0.0;
+            a_next = own && k + 2 < n ? D.amap[(k + 2) * n + r] : ESP_NONE;
+            u0 = lane_read_u32(ub, k + 1);
+            u1 = lane_read_u32(ue, k + 1);
+            uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+        }
+        for (uint32_t j0 = 0; j0 < cnt; j0 += CHUNK) {
+            uint32_t iv[CHUNK];
+            double fv[CHUNK];
+#pragma unroll
+            for (uint32_t c = 0; c < CHUNK; c++) {
+                const uint32_t j = j0 + c < cnt ? j0 + c : cnt - 1;
+                iv[c] = lane_read_u32(col_idx, j);
+                fv[c] = D.F[iv[c] * n + rr];
             }
-            u_nonfinite |= bits(uki - uki);
-            if (lm >> i & 1) w -= D.F[i * n + r] * uki;
+#pragma unroll
+            for (uint32_t c = 0; c < CHUNK; c++) {
+                if (j0 + c < cnt) {
+                    const double uki = lane_read(w, iv[c]);
+                    const double upd = w - fv[c] * uki;
+                    w = (lm >> iv[c] & 1) ? upd : w;
+                }
+            }
+        }
+        const bool is_u = um >> k & 1, is_l = lm >> k & 1;
+        if (is_u) {
+            D.F[k * n + r] = w;
+            nonfinite |= bits(w - w);
         }
         if (voided >> k & 1) {
             if (r == k) D.F[k * n + k] = 1;
-            if (lm >> k & 1) D.F[k * n + r] = 0;
-            w = 0;
+            if (is_l) D.F[k * n + r] = 0;
             continue;
         }
         const double d = lane_read(w, k);
-        if (r == k) {
-            w = 0;
-            D.F[k * n + k] = d;
-        }
+        if (r == k) D.F[k * n + k] = d;
         pivot_fail = pivot_fail || d == 0 || !isfinite_(d);
-        if (lm >> k & 1) {
-            const double v = w;
-            w = 0;
-            const double l = v / d;
+        if (is_l) {
+            const double l = w / d;
             D.F[k * n + r] = l;
-            const int64_t lb = (int64_t)bits(fabs_(l));
+            const double al = fabs_(l);
+            const bool wide = !(al <= 1 / PIVOT_TOL);
+            nan_l = nan_l || al != al;
+            wide_all = wide_all || wide;
             if (!(scaled >> k & 1)) {
                 // growth: |d| against every unscaled step's column entry
-                growth_fail = growth_fail || !(fabs_(d) >= GROWTH_LIMIT * fabs_(v));
-                mm_kept = lb > mm_kept ? lb : mm_kept;
+                growth_fail = growth_fail || !(fabs_(d) >= GROWTH_LIMIT * fabs_(w));
+                wide_kept = wide_kept || wide;
             } else {
                 nonfinite |= bits(l - l);
             }
-            mm_all = lb > mm_all ? lb : mm_all;
         }
     }
-    const bool bad = pivot_fail || u_nonfinite != 0 || wave_ballot(growth_fail || nonfinite != 0) != 0;
-    if (bad) return true;
-    // max_multiplier (lane_solver.h): the whole of L unless some pivot was
-    // taken by the scaled test, then only the other columns
-    const double all = from_bits((uint64_t)wave_max_i64(mm_all));
-    mm = (all != all || scaled == 0) ? all : from_bits((uint64_t)wave_max_i64(mm_kept));
-    return false;
+    if (pivot_fail || wave_ballot(growth_fail || nonfinite != 0) != 0) return REPLAY_FAILED;
+    // max_multiplier <= 1/pivot_tol (lane_solver.h): every multiplier of L
+    // within the bound, or, when some pivot was taken by the scaled test and
+    // no multiplier is NaN, every one of the other columns
+    const bool wide = (scaled == 0 || wave_ballot(nan_l)) ? wave_ballot(wide_all) != 0 : wave_ballot(wide_kept) != 0;
+    return wide ? REPLAY_PROBE : REPLAY_HOLDS;
 }
 
 // luSolve of -rhs into dx (lane_solver.h direct_solve_neg / lu_solve).
@@ -186,19 +217,44 @@ __device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs
 {
     const uint32_t n = P.n, r = threadIdx.x;
     const bool own = r < n;
+    const uint32_t rr = own ? r : n - 1;
     const uint64_t lm = own ? D.lmask[r] : 0, um = own ? D.umask[r] : 0;
     double y = own ? -1.0 * rhs[D.porig[r]] : 0.0;
-    for (uint32_t k = 0; k < n; k++) {
-        const double yk = lane_read(y, k);
-        if (yk == 0) continue;
-        if (lm >> k & 1) y -= D.F[k * n + r] * yk;
+    for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
+        double fv[CHUNK];
+#pragma unroll
+        for (uint32_t c = 0; c < CHUNK; c++) {
+            const uint32_t k = k0 + c < n ? k0 + c : n - 1;
+            fv[c] = D.F[k * n + rr];
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CHUNK; c++) {
+            const uint32_t k = k0 + c;
+            if (k < n) {
+                const double yk = lane_read(y, k);
+                const double upd = y - fv[c] * yk;
+                y = (yk != 0 && (lm >> k & 1)) ? upd : y;
+            }
+        }
     }
-    for (uint32_t k = n; k > 0;) {
-        k--;
-        const double zk = lane_read(y, k) / D.F[k * n + k];
-        if (r == k) y = zk;
-        if (zk == 0) continue;
-        if (um >> k & 1) y -= D.F[k * n + r] * zk;
+    for (uint32_t k1 = n; k1 > 0;) {
+        double fv[CHUNK], dv[CHUNK];
+#pragma unroll
+        for (uint32_t c = 0; c < CHUNK; c++) {
+            const uint32_t k = k1 > c ? k1 - 1 - c : 0;
+            fv[c] = D.F[k * n + rr];
+            dv[c] = D.F[k * n + k];
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CHUNK; c++) {
+            if (k1 > c) {
+                const uint32_t k = k1 - 1 - c;
+                const double zk = lane_read(y, k) / dv[c];
+                const double upd = y - fv[c] * zk;
+                y = r == k ? zk : ((zk != 0 && (um >> k & 1)) ? upd : y);
+            }
+        }
+        k1 = k1 > CHUNK ? k1 - CHUNK : 0;
     }
     if (own) dx[P.q[r]] = y;
 }
@@ -306,10 +362,9 @@ __device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D,
         return (int)r;
     }
     if (s.state[ST_FACTORED]) {
-        double mm = 0;
-        const bool failed = dense_refactor(P, s, D, vals, mm);
-        bool holds = !failed && mm <= 1 / PIVOT_TOL;
-        if (!failed && !holds) {
+        const int replay = dense_refactor(P, s, D, vals);
+        bool holds = replay == REPLAY_HOLDS;
+        if (replay == REPLAY_PROBE) {
             __syncthreads();
             if (tid == 0) *flag = dense_probe_holds(P, s, D, vals) ? 1u : 0u;
             __syncthreads();
@@ -335,12 +390,15 @@ __device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D,
     return 0;
 }
 
-// update_norm (engine.h) across the wavefront: x_old = x, x += dx and the
-// worst scaled step, as the max of the scores' bit patterns.
+// update_norm (engine.h) across the wavefront: x_old = x, x += dx. The
+// caller only compares the worst scaled step with 1 (finalizeStep's delta
+// gate), so this returns 1 when some score reaches 1 (max >= 1 iff some
+// element >= 1; the scores are non-negative, NaN scores count as 0) and 0
+// otherwise, which the gate reads the same way as the max itself.
 __device__ inline double wave_update_norm(uint32_t n, double *x, const double *dx, double *x_old, const uint8_t *cur_row,
                                           double reltol, double abstol, double vntol)
 {
-    int64_t worst = 0;
+    bool big = false;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         double xo = x[i], d = dx[i], xn = xo + d;
         x_old[i] = xo;
@@ -349,10 +407,9 @@ __device__ inline double wave_update_norm(uint32_t n, double *x, const double *d
         bool finite = fabs_(xn) < inf_() && fabs_(d) < inf_();
         double q = fabs_(d) / tol;
         double score = !finite ? inf_() : (q == q ? q : 0);
-        int64_t b = (int64_t)bits(score);
-        if (b > worst) worst = b;
+        big = big || score >= 1.0;
     }
-    return from_bits((uint64_t)wave_max_i64(worst));
+    return wave_ballot(big) ? 1.0 : 0.0;
 }
 
 }  // namespace esp

@@ -198,8 +198,7 @@ void hl_newton_wave(void *hv, const Newton *o, double *x, uint32_t *status, uint
         for (uint32_t tick = 0; tick < o->max_iter; tick++) {
             for (uint32_t tid = 0; tid < nt; tid++)
                 for (uint32_t e = tid; e < wh.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
-            for (uint32_t tid = 0; tid < nt; tid++)
-                for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_entry(W, s, e);
+            for (uint32_t tid = 0; tid < nt; tid++) wave_reduce(W, s, wave_reduce_cache(W, tid), tid, nt);
             if (!wave_solve(W, s, *o, tick, limit, scaled)) {
                 st = 2;
                 it = 0;
