@@ -43,7 +43,7 @@ int or_solver_is_tri(const or_solver *s);
 static const int NU[OR_NKIND] = {3, 2, 2, 2, 2, 4};
 static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
 static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
-static const int PK[OR_NKIND] = {1, 1, 1, 1, DIOP_K, M1P_K};
+static const int PK[OR_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
 
 /* Structural pattern of each kind (row-major bit masks over local
  * unknowns): G-plane entries and C-plane entries. */
@@ -268,17 +268,12 @@ const uint32_t *or_ckt_row_idx(const or_circuit *c) { return c->row_idx; }
 const uint32_t *or_ckt_diag(const or_circuit *c) { return c->diag; }
 const uint8_t *or_ckt_cur_row(const or_circuit *c) { return c->cur_row; }
 uint32_t or_ckt_count(const or_circuit *c, int kind) { return c->count[kind]; }
+int or_prepared_count(int kind) { return PK[kind]; }
+uint32_t or_tape_len(const or_circuit *c) { return c->count[OR_CAP]; }
 
 /* ------------------------------------------------------------------ lanes */
 
-struct or_lanes {
-    const or_circuit *c;
-    uint32_t N;
-    double *uprm[OR_NKIND]; /* [(inst*UK + k)*N + l] */
-    double *pprm[OR_NKIND]; /* [(inst*PK + k)*N + l] */
-    /* limiter state per lane: [inst*NU + u] per kind, lane-minor arrays */
-    double *lim[OR_NKIND];
-};
+/* struct or_lanes lives in oracle_internal.h (shared with tran.c). */
 
 or_lanes *or_lanes_new(const or_circuit *c, uint32_t N)
 {
@@ -322,7 +317,12 @@ void or_lanes_prepare(or_lanes *L)
                 switch (kind) {
                 case OR_MOS1: or_prep_mos1(u, p); break;
                 case OR_DIODE: or_prep_diode(u, p); break;
-                case OR_VSRC: p[0] = u[OR_VSRC_DC] == u[OR_VSRC_DC] ? u[OR_VSRC_DC] : 0.0; break;
+                case OR_VSRC: {
+                    /* vsource.va parameter defaults; NaN = not given */
+                    const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0};
+                    for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+                    break;
+                }
                 case OR_ISRC: p[0] = u[OR_ISRC_DC] == u[OR_ISRC_DC] ? u[OR_ISRC_DC] : 0.0; break;
                 case OR_RES: p[0] = 1.0 / u[OR_RES_R]; break;
                 case OR_CAP: p[0] = u[OR_CAP_C]; break;
@@ -334,7 +334,7 @@ void or_lanes_prepare(or_lanes *L)
 
 /* ------------------------------------------------------------ evaluation */
 
-static void load_p(const or_lanes *L, int kind, uint32_t id, uint32_t l, double *p)
+void or_load_p(const or_lanes *L, int kind, uint32_t id, uint32_t l, double *p)
 {
     for (int k = 0; k < PK[kind]; k++) p[k] = L->pprm[kind][((size_t)id * PK[kind] + k) * L->N + l];
 }
@@ -345,11 +345,15 @@ static double *lim_at(const or_lanes *L, int kind, uint32_t id, uint32_t l)
 }
 
 /* g (nnz+1 with trash), rhs (n+1 with trash row n) at x. */
-static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs)
+void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs,
+                    const or_evctx *ec)
 {
     const or_circuit *c = L->c;
     memset(g, 0, 8 * (c->nnz + 1));
     memset(rhs, 0, 8 * (c->n + 1));
+    if (ec && ec->q) memset(ec->q, 0, 8 * (c->n + 1));
+    if (ec && ec->cc) memset(ec->cc, 0, 8 * (c->nnz + 1));
+    uint32_t tape_at = 0;
     for (int t = 0; t < c->n_order; t++) {
         int kind = c->order[t];
         const int nu = NU[kind];
@@ -359,7 +363,7 @@ static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_
             double p[64], lx[4], xl[4];
             or_devout o;
             memset(&o, 0, sizeof o);
-            load_p(L, kind, id, l, p);
+            or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) xl[u] = lx[u] = x[nd[u]];
             int has_lim = kind == OR_MOS1 || kind == OR_DIODE;
             if (has_lim && lim_active) {
@@ -372,7 +376,7 @@ static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_
             case OR_VSRC:
                 o.res[0] = lx[2];
                 o.res[1] = -lx[2];
-                o.res[2] = lx[0] - lx[1] - p[0];
+                o.res[2] = lx[0] - lx[1] - (ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC]);
                 o.jac[0][2] = 1.0;
                 o.jac[1][2] = -1.0;
                 o.jac[2][0] = 1.0;
@@ -392,7 +396,18 @@ static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_
                 o.jac[1][1] = p[0];
                 break;
             }
-            case OR_CAP: break;
+            case OR_CAP: {
+                /* capacitor.va: I(p,n) <+ c*ddt(V(p,n)), charge c*V(p,n) */
+                double qv = p[0] * (lx[0] - lx[1]);
+                o.q[0] = qv;
+                o.q[1] = -qv;
+                o.cjac[0][0] = p[0];
+                o.cjac[0][1] = -p[0];
+                o.cjac[1][0] = -p[0];
+                o.cjac[1][1] = p[0];
+                if (ec && ec->tape) ec->tape[tape_at++] = qv;
+                break;
+            }
             case OR_DIODE: or_eval_diode(p, lx, &o); break;
             case OR_MOS1: or_eval_mos1(p, lx, &o); break;
             }
@@ -415,9 +430,11 @@ static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_
             for (int ru = 0; ru < nu; ru++) {
                 uint32_t row = nd[ru] == 0 ? c->n : nd[ru];
                 if (RMASK[kind] >> ru & 1) rhs[row] += o.res[ru];
+                if (ec && ec->q && (cmask(kind, ru) != 0)) ec->q[row] += o.q[ru];
                 for (int cu = 0; cu < nu; cu++) {
                     if (!(pmask(kind, ru) >> cu & 1)) continue;
                     if (gmask(kind, ru) >> cu & 1) g[sl[k]] += o.jac[ru][cu];
+                    if (ec && ec->cc && (cmask(kind, ru) >> cu & 1)) ec->cc[sl[k]] += o.cjac[ru][cu];
                     k++;
                 }
             }
@@ -430,7 +447,7 @@ static void eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_
 
 /* Limit pass on lane l: cur = x, old = previous image or x_old. Returns 1
  * when any instance clamped (not converged). */
-static int limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old, int lim_active)
+int or_limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old, int lim_active)
 {
     const or_circuit *c = L->c;
     int flag = 0;
@@ -443,7 +460,7 @@ static int limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             double p[64], cur[4], old[4], lim[4];
             double *lm = lim_at(L, kind, id, l);
-            load_p(L, kind, id, l, p);
+            or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) {
                 cur[u] = x[nd[u]];
                 old[u] = ((writes >> u & 1) && lim_active) ? lm[u] : x_old[nd[u]];
@@ -457,7 +474,7 @@ static int limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_
     return flag;
 }
 
-static void seed_pass(or_lanes *L, uint32_t l, const double *x)
+void or_seed_pass(or_lanes *L, uint32_t l, const double *x)
 {
     const or_circuit *c = L->c;
     for (int t = 0; t < c->n_order; t++) {
@@ -469,7 +486,7 @@ static void seed_pass(or_lanes *L, uint32_t l, const double *x)
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             double p[64], cur[4], lim[4];
             double *lm = lim_at(L, kind, id, l);
-            load_p(L, kind, id, l, p);
+            or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) cur[u] = x[nd[u]];
             if (kind == OR_MOS1) or_seed_mos1(p, cur, lim);
             else or_seed_diode(p, cur, lim);
@@ -483,7 +500,7 @@ void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double 
 {
     const or_circuit *c = L->c;
     double *g = malloc(8 * (c->nnz + 1)), *r = malloc(8 * (c->n + 1));
-    eval_planes(L, l, x, 0, g, r);
+    or_eval_planes(L, l, x, 0, g, r, NULL);
     memcpy(rhs, r, 8 * c->n);
     memcpy(vals, g, 8 * c->nnz);
     free(g);
@@ -514,7 +531,7 @@ static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double 
     out->status = 0;
     out->iterations = o->max_iter;
     for (uint32_t tick = 0; tick < o->max_iter; tick++) {
-        eval_planes(L, l, x, lim_active, g, rhs);
+        or_eval_planes(L, l, x, lim_active, g, rhs, NULL);
         if (o->gmin > 0) {
             for (uint32_t i = 0; i < n; i++) {
                 g[c->diag[i]] += o->gmin;
@@ -528,7 +545,7 @@ static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double 
             break;
         }
         double scaled = or_lane_update_norm(n, x, dx, x_old, c->cur_row, o->reltol, o->abstol, o->vntol, 1, 0);
-        int limited = limit_pass(L, l, x, x_old, lim_active);
+        int limited = or_limit_pass(L, l, x, x_old, lim_active);
         lim_active = 1;
         int ok = !limited && tick != 0 && !(scaled >= 1.0);
         if (ok) ok = or_lane_plausible(n, x, limit, 1, 0);
@@ -588,9 +605,9 @@ int or_lead_tab(or_lanes *L, const double *x, int init_fix, or_lane_tab *tab, ui
     const uint32_t N = L->N, n = c->n;
     double *x0 = malloc(8 * n);
     for (uint32_t i = 0; i < n; i++) x0[i] = x[(size_t)i * N];
-    if (init_fix) seed_pass(L, 0, x0);
+    if (init_fix) or_seed_pass(L, 0, x0);
     double *g = malloc(8 * (c->nnz + 1)), *rhs = malloc(8 * (n + 1));
-    eval_planes(L, 0, x0, init_fix, g, rhs);
+    or_eval_planes(L, 0, x0, init_fix, g, rhs, NULL);
     or_solver *s = or_solver_new(n, c->col_ptr, c->row_idx);
     int rc = -1;
     if (s && or_solver_factor(s, g) == 0 && !or_solver_is_tri(s)) {
@@ -629,7 +646,7 @@ static void *op_worker(void *arg)
     double *xl = malloc(8 * (n ? n : 1));
     for (uint32_t l = j->first; l < N; l += j->stride) {
         for (uint32_t i = 0; i < n; i++) xl[i] = j->x[(size_t)i * N + l];
-        if (o->init_fix) seed_pass(L, l, xl);
+        if (o->init_fix) or_seed_pass(L, l, xl);
         if (j->mode == 0) {
             tape_ctx t = {&j->tab, j->idx, NULL, NULL, NULL, NULL, NULL, c, NULL, 0};
             t.lx = calloc(j->tab.nnz_l + 1, 8);
@@ -728,7 +745,7 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
         or_outcome oc = {0, 0};
         if (pt != 0) {
             /* prepareWarm: limit the predictor against the last point */
-            limit_pass(L, 0, sol, prev, 0);
+            or_limit_pass(L, 0, sol, prev, 0);
             newton_one(L, 0, &warm, sol, &oc, lin_direct, s, 1);
         }
         if (!(oc.status & 1)) {
@@ -736,7 +753,7 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
             memset(sol, 0, 8 * n);
             or_newton_opts cold = *o;
             cold.init_fix = 1;
-            seed_pass(L, 0, sol);
+            or_seed_pass(L, 0, sol);
             newton_one(L, 0, &cold, sol, &oc, lin_direct, s, 1);
         }
         converged[pt] = oc.status & 1;

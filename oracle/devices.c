@@ -526,3 +526,50 @@ void or_seed_diode(const double *p, const double *cur, double *lim)
     lim[0] = cur[1] + p[DIOP_VCRIT];
     lim[1] = cur[1];
 }
+
+/* ---------------------------------------------------------------- sources */
+
+static double vmax(double a, double b) { return a > b ? a : b; }
+
+/* The PULSE widths of models/vsource.va:44-50 (P_TR, P_TF, P_PW, P_PER). */
+static void pulse_widths(const double *p, double *tr, double *tf, double *pw, double *per)
+{
+    *tr = vmax(p[SRCP_TR], 1.0e-12);
+    *tf = vmax(p[SRCP_TF], 1.0e-12);
+    *pw = vmax(p[SRCP_PW], 0.0) + vmax(-p[SRCP_PW], 0.0) * 1.0e30;
+    *per = vmax(p[SRCP_PER], *tr + *pw + *tf + 1.0e-12);
+}
+
+/* models/vsource.va:226-247: DC, or PULSE at t (PHASE 0). */
+double or_source_value(const double *p, double t)
+{
+    if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
+    double tr, tf, pw, per;
+    pulse_widths(p, &tr, &tf, &pw, &per);
+    const double v1 = p[SRCP_V1], v2 = p[SRCP_V2];
+    const double t_rel = t - p[SRCP_TD];
+    double t_mod = t_rel + (0.0 / 360.0) * per;
+    t_mod = t_mod - floor(t_mod / per) * per;
+    t_mod = vmax(t_mod, 0.0);
+    if (t_rel < 0.0) return v1;
+    if (t_mod < tr) return v1 + (v2 - v1) * (t_mod / tr);
+    if (t_mod < tr + pw) return v2;
+    if (t_mod < tr + pw + tf) return v2 - (v2 - v1) * ((t_mod - (tr + pw)) / tf);
+    return v1;
+}
+
+/* src/device/eval.zig:1636-1650 spicePulseBreak: the corner after tq,
+ * rounded from the accepted time t_acc; inf when none (not PULSE, or the
+ * rounded time is not after tq). */
+double or_pulse_break(const double *p, double t_acc, double tq)
+{
+    if (p[SRCP_WAVE] != 1.0) return INFINITY;
+    double tr, tf, pw, per;
+    pulse_widths(p, &tr, &tf, &pw, &per);
+    double time = t_acc - p[SRCP_TD];
+    if (time >= per) time -= per * floor(time / per);
+    const double atime = time + (tq - t_acc);
+    const double corner = atime < 0.0 ? 0.0 : atime < tr ? tr : atime < tr + pw ? tr + pw : atime < tr + pw + tf ? tr + pw + tf : per;
+    const double bp = t_acc + (corner - time);
+    return bp > tq ? bp : INFINITY;
+}

@@ -72,6 +72,42 @@ typedef struct { uint32_t status; uint32_t iterations; } or_outcome;
  * Returns 0, or -1 when the template has no lane tape (tridiagonal
  * pattern or a singular lead factor). */
 int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out);
+
+/* Transient of one lane (tran.c): `.tran dt_init t_stop` with the step
+ * ceiling dt_max already resolved (the deck rule is min(dt_init,
+ * t_stop/50)); method 0 BE, 1 trapezoidal, 2 gear-2. */
+typedef struct {
+    double t_stop, dt_init, dt_max, dt_min, xmu;
+    double reltol, abstol, vntol, chgtol, trtol, residual_tol_op;
+    uint32_t method, max_iter, max_steps, _pad;
+} or_tran_opts;
+
+typedef struct {
+    uint32_t points, steps, attempts, nr_iters, rej_newton, rej_lte, bp_landings;
+    int32_t status; /* 0 reached t_stop, 1 dt underflow, 2 no operating point, 3 out of memory */
+} or_tran_stats;
+
+/* Records every accepted row: times[k] and rows[k*n + i] (malloc'd, the
+ * caller frees them). Returns the row count. */
+int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *o, double **times, double **rows, or_tran_stats *st);
+
+/* .meas passes over a recorded waveform (meas.c). A clause's counts use
+ * -1 for not given and -2 for LAST. */
+typedef struct {
+    double from, to, val, td;
+    int32_t rise, fall, cross, _pad;
+} or_meas_clause;
+
+/* The event a WHEN/TRIG/TARG clause names; NaN when it never happens. */
+double or_meas_cross(const double *x, const double *y, uint32_t len, const or_meas_clause *c);
+/* y at `at` (slope 0) or the slope of the segment holding it (slope 1). */
+double or_meas_at(const double *x, const double *y, uint32_t len, double at, int slope);
+/* op 0 MIN, 1 MAX (value, where), 2 AVG (value, last scale read). */
+void or_meas_extremum(const double *x, const double *y, uint32_t len, double from, double to, int op, double *value,
+                      double *at);
+/* INTEG (rms 0) or RMS (rms 1) over [from, to]. */
+void or_meas_integ(const double *x, const double *y, uint32_t len, double from, double to, int rms, double *value,
+                   double *rfrom, double *rto);
 /* The same on `threads` POSIX threads, lanes dealt round robin. */
 int or_newton_op_mt(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out, uint32_t threads);
 /* The lead tape of lane 0 at x (mode 0's tape), for tests. */

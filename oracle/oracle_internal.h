@@ -34,6 +34,44 @@ int or_limit_mos1(const double *p, const double *cur, const double *old, double 
 int or_limit_diode(const double *p, const double *cur, const double *old, double *lim);
 void or_seed_mos1(const double *p, const double *cur, double *lim);
 void or_seed_diode(const double *p, const double *cur, double *lim);
+/* Prepared V/I source constants: the DC value, then the waveform (0 DC,
+ * 1 PULSE) and PULSE(V1 V2 TD TR TF PW PER) with the -1 sentinels of
+ * models/vsource.va:58-66 for TR, TF, PW and PER not given. */
+enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_K };
+/* The source's value at transient time t (models/vsource.va:233-247). */
+double or_source_value(const double *p, double t);
+/* ngspice's PULSE breakpoint after tq, requested at the accepted time
+ * t_acc (src/device/eval.zig:1636-1650 spicePulseBreak); inf when none. */
+double or_pulse_break(const double *p, double t_acc, double tq);
+
+/* What one evaluation adds beyond the static planes: the transient's time
+ * (sources read their waveform), and the charge outputs. */
+typedef struct {
+    int tran;
+    double t;
+    double *q;    /* n + 1 row charges (ground and trash row included), or NULL */
+    double *cc;   /* nnz + 1 C-plane values, or NULL */
+    double *tape; /* one charge per LTE site, or NULL */
+} or_evctx;
+
+struct or_lanes {
+    const struct or_circuit *c;
+    uint32_t N;
+    double *uprm[6]; /* [(inst*UK + k)*N + l] */
+    double *pprm[6]; /* [(inst*PK + k)*N + l] */
+    double *lim[6];  /* [(l*count + inst)*NU + u] */
+};
+
+void or_eval_planes(const struct or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs,
+                    const or_evctx *ec);
+int or_limit_pass(struct or_lanes *L, uint32_t l, const double *x, const double *x_old, int lim_active);
+void or_seed_pass(struct or_lanes *L, uint32_t l, const double *x);
+/* Prepared constants of instance id of kind on lane l (copied into p). */
+void or_load_p(const struct or_lanes *L, int kind, uint32_t id, uint32_t l, double *p);
+int or_prepared_count(int kind);
+/* Length of the LTE charge tape: one site per capacitor. */
+uint32_t or_tape_len(const struct or_circuit *c);
+
 double or_fetlim(double vnew, double vold, double vto);
 double or_limvds(double vnew, double vold);
 double or_pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck);

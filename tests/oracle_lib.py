@@ -36,6 +36,36 @@ class NewtonOpts(C.Structure):
                 ("max_iter", C.c_uint32), ("init_fix", C.c_uint32)]
 
 
+class TranOpts(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("t_stop", "dt_init", "dt_max", "dt_min", "xmu", "reltol", "abstol", "vntol",
+                                          "chgtol", "trtol", "residual_tol_op")] + \
+               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "pad")]
+
+
+class TranStats(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("points", "steps", "attempts", "nr_iters", "rej_newton", "rej_lte",
+                                          "bp_landings")] + [("status", C.c_int32)]
+
+
+class MeasClause(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("frm", "to", "val", "td")] + \
+               [(f, C.c_int32) for f in ("rise", "fall", "cross", "pad")]
+
+
+def clause(val=1e99, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):
+    return MeasClause(frm, to, val, td, rise, fall, cross, 0)
+
+
+def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
+              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000):
+    """`.tran dt_init t_stop` with the deck's ceiling min(dt_init, t_stop/50)
+    (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680)."""
+    if dt_max is None:
+        dt_max = min(dt_init, t_stop / 50)
+    return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, 1e-9, method, max_iter,
+                    max_steps, 0)
+
+
 class Outcome(C.Structure):
     _fields_ = [("status", C.c_uint32), ("iterations", C.c_uint32)]
 
@@ -89,6 +119,12 @@ def load():
         "or_eval_lane": (None, [P, C.c_uint32, f64p, f64p, f64p]),
         "or_dc_sweep": (C.c_int, [P, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.POINTER(NewtonOpts), f64p, u32p]),
         "or_fetlim": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "or_tran_run": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), C.POINTER(f64p), C.POINTER(f64p),
+                                  C.POINTER(TranStats)]),
+        "or_meas_cross": (C.c_double, [f64p, f64p, C.c_uint32, C.POINTER(MeasClause)]),
+        "or_meas_at": (C.c_double, [f64p, f64p, C.c_uint32, C.c_double, C.c_int]),
+        "or_meas_extremum": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p]),
+        "or_meas_integ": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p, f64p]),
         "or_limvds": (C.c_double, [C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
@@ -210,6 +246,18 @@ class Lanes:
         vals, vp = f64(np.zeros(nnz))
         self.lib.or_eval_lane(self.h, l, xp, rp, vp)
         return rhs, vals
+
+    def tran(self, l, opts):
+        """Lane l's transient: (times, rows (points, n), stats)."""
+        tp, rp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+        st = TranStats()
+        k = self.lib.or_tran_run(self.h, l, C.byref(opts), C.byref(tp), C.byref(rp), C.byref(st))
+        n = self.ckt.n
+        t = np.ctypeslib.as_array(tp, (k,)).copy() if k else np.zeros(0)
+        x = np.ctypeslib.as_array(rp, (k * n,)).copy().reshape(k, n) if k else np.zeros((0, n))
+        self.lib.free_(C.cast(tp, C.c_void_p))
+        self.lib.free_(C.cast(rp, C.c_void_p))
+        return t, x, st
 
     def dc_sweep(self, vsrc, start, step, npts, **kw):
         n = self.ckt.n

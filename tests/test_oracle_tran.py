@@ -1,0 +1,134 @@
+"""The oracle's transient step machine and .meas passes, pinned by the
+reference's printed outputs:
+  - docs/examples/measure.sp (docs/learn/measure.md:24-36): 1029 points
+    and ten .meas results to their printed digits;
+  - docs/examples/rc_lowpass.sp (docs/learn/analyses.md:25-30): 519 points;
+and by mathematics: the RC step response 1 - exp(-t/RC) within the LTE
+tolerance, and the PULSE waveform and its breakpoints at known corners."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def P(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def rc_deck(lib, pulse, r, cap, dc=None):
+    """V1 in 0 PULSE(...); R1 in out r; C1 out 0 cap, added in the
+    builder's element-letter order (src/frontend/builder.zig:886: v, c, r)."""
+    v1, v2, td, tr, tf, pw, per = pulse
+    c = O.Circuit(lib)
+    c.add(O.VSRC, ["in", 0], {0: np.nan if dc is None else dc, 1: 1, 2: v1, 3: v2, 4: td, 5: tr, 6: tf, 7: pw, 8: per})
+    c.add(O.CAP, ["out", 0], {0: cap})
+    c.add(O.RES, ["in", "out"], {0: r})
+    return c.freeze()
+
+
+def printed(value, golden, digits):
+    """Equal at the printed precision: `digits` after the point in %e."""
+    assert value == pytest.approx(golden, rel=0, abs=0.5 * 10 ** (math.floor(math.log10(abs(golden))) - digits) * 1.0001)
+
+
+@pytest.fixture(scope="module")
+def measure_run(oracle):
+    c = rc_deck(oracle, (0, 1, 1e-6, 1e-9, 1e-9, 5e-6, 20e-6), 1e3, 1e-9)
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(10e-6, 10e-9))
+    return c, t, x, st
+
+
+def test_measure_sp_points(measure_run):
+    _, t, _, st = measure_run
+    assert st.status == 0
+    assert st.points == 1029 == len(t)
+    assert t[0] == 0 and t[-1] == pytest.approx(10e-6, rel=1e-12)
+    assert np.all(np.diff(t) > 0)
+
+
+def test_measure_sp_values(oracle, measure_run):
+    c, t, x, _ = measure_run
+    tt = np.ascontiguousarray(t)
+    N = len(tt)
+    vout = np.ascontiguousarray(x[:, c.node("out")])
+    vin = np.ascontiguousarray(x[:, c.node("in")])
+    iv1 = np.ascontiguousarray(x[:, 2])  # V1's branch row
+    cross = lambda y, **k: oracle.or_meas_cross(P(tt), P(y), N, C.byref(O.clause(**k)))
+    trig, targ = cross(vout, val=0.1, rise=1), cross(vout, val=0.9, rise=1)
+    printed(targ - trig, 2.197201e-06, 6)
+    printed(targ, 3.303064e-06, 6)
+    printed(trig, 1.105863e-06, 6)
+    trise = targ - trig
+    trig, targ = cross(vin, val=0.5, rise=1), cross(vout, val=0.5, rise=1)
+    printed(targ - trig, 6.931367e-07, 6)
+    printed(targ, 1.693637e-06, 6)
+    printed(trig, 1.000500e-06, 6)
+    printed(trise / (targ - trig), 3.169938e+00, 6)
+    printed(oracle.or_meas_at(P(tt), P(vout), N, 3e-6, 0), 8.645984e-01, 6)
+    printed(cross(vout, val=0.5, fall=1), 6.68790e-06, 5)
+    v, a, b = C.c_double(), C.c_double(), C.c_double()
+    oracle.or_meas_extremum(P(tt), P(vout), N, 0.0, 0.0, 1, C.byref(v), C.byref(a))
+    printed(v.value, 9.932657e-01, 6)
+    printed(a.value, 6.001000e-06, 6)
+    oracle.or_meas_extremum(P(tt), P(vout), N, 0.0, 10e-6, 2, C.byref(v), C.byref(a))
+    printed(v.value, 4.982781e-01, 6)
+    oracle.or_meas_integ(P(tt), P(vout), N, 0.0, 10e-6, 1, C.byref(v), C.byref(a), C.byref(b))
+    printed(v.value, 6.33054e-01, 5)
+    printed(oracle.or_meas_at(P(tt), P(vout), N, 2e-6, 1), 3.685512e+05, 6)
+    oracle.or_meas_integ(P(tt), P(iv1), N, 1e-6, 6e-6, 0, C.byref(v), C.byref(a), C.byref(b))
+    printed(v.value, -9.93246e-10, 5)
+    assert (a.value, b.value) == (1e-6, 6e-6)
+
+
+def test_rc_lowpass_points(oracle):
+    c = rc_deck(oracle, (0, 1, 0, 1e-6, 1e-6, 5e-3, 10e-3), 1e3, 159e-9, dc=0.0)
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, _, st = L.tran(0, O.tran_opts(5e-3, 10e-6))
+    assert st.status == 0 and st.points == 519
+    assert st.rej_lte >= 1  # the run goes through an LTE rejection
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_rc_step_closed_form(oracle, method):
+    """After the edge the output follows 1 - exp(-(t - t0)/RC); BE and the
+    trapezoid stay within a few reltol of it."""
+    c = rc_deck(oracle, (0, 1, 1e-6, 1e-9, 1e-9, 5e-6, 20e-6), 1e3, 1e-9)
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(10e-6, 10e-9, method=method))
+    assert st.status == 0
+    vout = x[:, c.node("out")]
+    sel = (t > 1.2e-6) & (t < 5.9e-6)
+    exact = 1 - np.exp(-(t[sel] - 1.0005e-6) / 1e-6)
+    assert np.max(np.abs(vout[sel] - exact)) < (2e-3 if method == 0 else 5e-4)
+    assert np.all(x[:, c.node("in")][t <= 1e-6] == 0)
+
+
+def test_pulse_waveform_and_breakpoints(oracle):
+    """models/vsource.va PULSE at known times, and ngspice's corner rounding
+    (the corner after the request, from the accepted time)."""
+    c = rc_deck(oracle, (0.2, 1.7, 1e-6, 1e-9, 2e-9, 5e-6, 20e-6), 1e3, 1e-9)
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, _ = L.tran(0, O.tran_opts(30e-6, 10e-9))
+    vin = x[:, c.node("in")]
+    for tk, vk in zip(t, vin):
+        m = (tk - 1e-6) % 20e-6 if tk >= 1e-6 else -1
+        if m < 0 or m >= 1e-9 + 5e-6 + 2e-9:
+            expect = 0.2
+        elif m < 1e-9:
+            expect = 0.2 + 1.5 * (m / 1e-9)
+        elif m < 1e-9 + 5e-6:
+            expect = 1.7
+        else:
+            expect = 1.7 - 1.5 * ((m - 1e-9 - 5e-6) / 2e-9)
+        assert vk == pytest.approx(expect, abs=1e-9)
+    # every corner of both periods is a timepoint
+    for corner in (1e-6, 1.001e-6, 6.001e-6, 6.003e-6, 21e-6, 21.001e-6, 26.001e-6, 26.003e-6):
+        assert np.min(np.abs(t - corner)) <= 1e-18 * 10 + 1e-21
