@@ -28,7 +28,7 @@ EXPORTS = (
     "espice_lanes_template_node", "espice_lanes_template_add", "espice_lanes_template_freeze",
     "espice_lanes_template_info", "espice_lanes_template_pattern", "espice_lanes_create", "espice_lanes_destroy",
     "espice_lanes_reset", "espice_lanes_set_param", "espice_lanes_prepare", "espice_lanes_upload_x", "espice_lanes_download_x",
-    "espice_lanes_newton_op", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
+    "espice_lanes_newton_op", "espice_lanes_tran", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
 )
 
 
@@ -44,6 +44,31 @@ class NewtonOpts(C.Structure):
 
 class LaneResult(C.Structure):
     _fields_ = [("status", C.c_uint32), ("iterations", C.c_uint32)]
+
+
+class TranOpts(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("t_stop", "dt_init", "dt_max", "dt_min", "xmu", "reltol", "abstol", "vntol",
+                                          "chgtol", "trtol")] + \
+               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity")]
+
+
+class MeasPass(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("row", C.c_uint32), ("rise", C.c_int32), ("fall", C.c_int32),
+                ("cross", C.c_int32), ("op", C.c_uint32)] + \
+               [(f, C.c_double) for f in ("val", "td", "frm", "to", "at")]
+
+
+class MeasResult(C.Structure):
+    _fields_ = [("value", C.c_double), ("aux0", C.c_double), ("aux1", C.c_double)]
+
+
+class TranResult(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("status", "points", "steps", "attempts", "rej_newton", "rej_lte",
+                                          "bp_landings", "nr_iters")]
+
+
+MEAS_CROSS, MEAS_AT, MEAS_EXTREMUM, MEAS_INTEG = range(4)
+LAST = -2
 
 
 class LaneTab(C.Structure):
@@ -82,6 +107,8 @@ def library():
         "espice_lanes_upload_x": [P, f64p],
         "espice_lanes_download_x": [P, f64p],
         "espice_lanes_newton_op": [P, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p],
+        "espice_lanes_tran": [P, C.POINTER(TranOpts), C.POINTER(MeasPass), C.c_uint32, C.POINTER(TranResult),
+                              C.POINTER(MeasResult), f64p, f64p, f64p],
         "espice_lanes_tab_build": [C.c_uint32, u32p, u32p, f64p, C.POINTER(LaneTab), u32p, C.c_size_t,
                                    C.POINTER(C.c_size_t)],
         "espice_lanes_tab_refactor_solve": [C.POINTER(LaneTab), u32p, C.c_size_t, C.c_uint32, f64p, f64p, f64p,
@@ -118,6 +145,37 @@ def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e
     return NewtonOpts(reltol, abstol, vntol, residual_tol, gmin, max_iter, 1 if init_fix else 0)
 
 
+def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
+              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0):
+    """`.tran dt_init t_stop`; a deck's step ceiling is min(dt_init, t_stop/50)
+    (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680), the
+    tolerances are converger.Options' (src/core/numerics.zig:378-399)."""
+    if dt_max is None:
+        dt_max = min(dt_init, t_stop / 50)
+    return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, method, max_iter,
+                    max_steps, record_capacity)
+
+
+def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):
+    """WHEN / TRIG / TARG on unknown `row` (src/output/scan.zig Cross)."""
+    return MeasPass(MEAS_CROSS, row, rise, fall, cross, 0, val, td, frm, to, 0.0)
+
+
+def meas_at(row, at, deriv=False):
+    """FIND / DERIV ... AT= (src/output/scan.zig At)."""
+    return MeasPass(MEAS_AT, row, -1, -1, -1, 1 if deriv else 0, 0.0, 0.0, 0.0, 0.0, at)
+
+
+def meas_extremum(row, op, frm=0.0, to=0.0):
+    """MIN (op 0), MAX (1), AVG (2) (src/output/scan.zig Extremum)."""
+    return MeasPass(MEAS_EXTREMUM, row, -1, -1, -1, op, 0.0, 0.0, frm, to, 0.0)
+
+
+def meas_integ(row, rms=False, frm=0.0, to=0.0):
+    """INTEG / RMS (src/output/scan.zig Integ)."""
+    return MeasPass(MEAS_INTEG, row, -1, -1, -1, 1 if rms else 0, 0.0, 0.0, frm, to, 0.0)
+
+
 class Template:
     """A circuit frozen into one MNA pattern, shared by every lane."""
 
@@ -125,6 +183,7 @@ class Template:
         self.lib = library()
         self.h = C.c_void_p()
         _check(self.lib.espice_lanes_template_create(C.byref(self.h)))
+        self.rows = {}
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
@@ -132,8 +191,13 @@ class Template:
             self.h = None
 
     def node(self, name):
+        """The row of node `name`, numbered on first touch; a frozen
+        template answers only for names it already holds."""
+        if name in self.rows:
+            return self.rows[name]
         row = C.c_uint32()
         _check(self.lib.espice_lanes_template_node(self.h, name.encode(), C.byref(row)))
+        self.rows[name] = row.value
         return row.value
 
     def add(self, kind, ports, params=None):
@@ -211,6 +275,30 @@ class Lanes:
         status = np.array([out[i].status for i in range(self.N)], dtype=np.uint32)
         iters = np.array([out[i].iterations for i in range(self.N)], dtype=np.uint32)
         return self.download_x(), status, iters, ms.value
+
+
+    def tran(self, opts, passes=(), record_capacity=None):
+        """Every lane's transient from a cold operating point.
+
+        Returns (results (N,) TranResult fields as a dict of arrays, meas
+        (N, P, 3), times (N, cap), rows (N, cap, n), kernel_ms); times/rows
+        are None when nothing is recorded."""
+        if record_capacity is not None:
+            opts.record_capacity = record_capacity
+        cap, n, N, P = opts.record_capacity, self.t.n, self.N, len(passes)
+        pa = (MeasPass * max(P, 1))(*passes)
+        out = (TranResult * N)()
+        meas, mp = _f64(np.zeros(max(N * P * 3, 1)))
+        times, tp = _f64(np.zeros(max(N * cap, 1)))
+        rows, rp = _f64(np.zeros(max(N * cap * n, 1)))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_tran(self.h, C.byref(opts), pa, P, out,
+                                          C.cast(mp, C.POINTER(MeasResult)), tp, rp, C.byref(ms)))
+        res = {f: np.array([getattr(out[i], f) for i in range(N)], dtype=np.uint32) for f, _ in TranResult._fields_}
+        m = meas[:N * P * 3].reshape(N, P, 3)
+        if not cap:
+            return res, m, None, None, ms.value
+        return res, m, times.reshape(N, cap), rows.reshape(N, cap, n), ms.value
 
 
 def tab_build(n, col_ptr, row_idx, vals):

@@ -22,7 +22,10 @@ namespace esp {
 enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_NKIND = 6 };
 
 // User parameter slots (NaN = not given).
-enum { VSRC_DC = 0, VSRC_K = 9 };
+enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_K };
+// Prepared source constants: the user slots with vsource.va's defaults
+// (models/vsource.va:58-66: 0, and -1 sentinels for TR, TF, PW, PER).
+enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_K };
 enum { ISRC_DC = 0, ISRC_K = 1 };
 enum { RES_R = 0, RES_K = 1 };
 enum { CAP_C = 0, CAP_K = 1 };
@@ -42,7 +45,7 @@ enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT,
 constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 2, 4};
 constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
 constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
-constexpr int KIND_PK[K_NKIND] = {1, 1, 1, 1, DIOP_K, M1P_K};
+constexpr int KIND_PK[K_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
 // Residual rows a kind writes in a static evaluation.
 constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x3u, 0xdu};
 // Unknowns a kind's limiter writes (its limited image).
@@ -77,6 +80,8 @@ constexpr double KB = 1.38064852e-23, QE = 1.6021766208e-19;
 struct DevOut {
     double res[4];
     double jac[4][4];
+    double q[4];        // charges (transient only)
+    double cjac[4][4];  // dQ/du (transient only)
 };
 
 ESP_HD double given(double v, double dflt) { return v == v ? v : dflt; }
@@ -257,7 +262,11 @@ inline void prep_kind(int kind, const double *u, double *p)
     switch (kind) {
     case K_MOS1: prep_mos1(u, p); break;
     case K_DIODE: prep_diode(u, p); break;
-    case K_VSRC: p[0] = u[VSRC_DC] == u[VSRC_DC] ? u[VSRC_DC] : 0.0; break;
+    case K_VSRC: {
+        const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0};
+        for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+        break;
+    }
     case K_ISRC: p[0] = u[ISRC_DC] == u[ISRC_DC] ? u[ISRC_DC] : 0.0; break;
     case K_RES: p[0] = 1.0 / u[RES_R]; break;
     case K_CAP: p[0] = u[CAP_C]; break;
@@ -351,6 +360,59 @@ ESP_HD void snap_limited(int nu, unsigned reads, unsigned writes, double *lx, co
     for (int u = 0; u < nu; u++)
         if ((writes & reads) >> u & 1)
             if (fabs_(lx[u] - cur[u]) <= tol) lx[u] = cur[u];
+}
+
+// -------------------------------------------------------------- sources
+
+ESP_HD double src_max(double a, double b) { return a > b ? a : b; }
+
+// The PULSE widths of models/vsource.va:44-50 (P_TR, P_TF, P_PW, P_PER).
+template <class P>
+ESP_HD void pulse_widths(const P &p, double &tr, double &tf, double &pw, double &per)
+{
+    tr = src_max(p[SRCP_TR], 1.0e-12);
+    tf = src_max(p[SRCP_TF], 1.0e-12);
+    pw = src_max(p[SRCP_PW], 0.0) + src_max(-p[SRCP_PW], 0.0) * 1.0e30;
+    per = src_max(p[SRCP_PER], tr + pw + tf + 1.0e-12);
+}
+
+// models/vsource.va:226-247: the DC value, or PULSE at t (PHASE 0).
+template <class P>
+ESP_HD double src_value(const P &p, double t)
+{
+    if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
+    double tr, tf, pw, per;
+    pulse_widths(p, tr, tf, pw, per);
+    const double v1 = p[SRCP_V1], v2 = p[SRCP_V2];
+    const double t_rel = t - p[SRCP_TD];
+    double t_mod = t_rel + (0.0 / 360.0) * per;
+    t_mod = t_mod - __builtin_floor(t_mod / per) * per;
+    t_mod = src_max(t_mod, 0.0);
+    if (t_rel < 0.0) return v1;
+    if (t_mod < tr) return v1 + (v2 - v1) * (t_mod / tr);
+    if (t_mod < tr + pw) return v2;
+    if (t_mod < tr + pw + tf) return v2 - (v2 - v1) * ((t_mod - (tr + pw)) / tf);
+    return v1;
+}
+
+// ngspice's PULSE breakpoint after tq rounded from the accepted time t_acc
+// (src/device/eval.zig:1636-1650 spicePulseBreak); inf when none.
+template <class P>
+ESP_HD double pulse_break(const P &p, double t_acc, double tq)
+{
+    if (p[SRCP_WAVE] != 1.0) return inf_();
+    double tr, tf, pw, per;
+    pulse_widths(p, tr, tf, pw, per);
+    double time = t_acc - p[SRCP_TD];
+    if (time >= per) time -= per * __builtin_floor(time / per);
+    const double atime = time + (tq - t_acc);
+    const double corner = atime < 0.0 ? 0.0
+                          : atime < tr ? tr
+                          : atime < tr + pw ? tr + pw
+                          : atime < tr + pw + tf ? tr + pw + tf
+                          : per;
+    const double bp = t_acc + (corner - time);
+    return bp > tq ? bp : inf_();
 }
 
 // ---------------------------------------------------------------- eval
@@ -522,19 +584,25 @@ ESP_HD void seed_diode(const P &p, const double *cur, double *lim)
     lim[1] = cur[1];
 }
 
-// Static evaluation of one instance of `kind` at local voltages lx.
+// Evaluation of one instance of `kind` at local voltages lx. In a
+// transient (`tran`) sources read their waveform at t and the charges and
+// dQ/du of capacitors are filled in (capacitor.va: charge c*V(p,n)).
 template <class P>
-ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o)
+ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tran = false, double t = 0.0)
 {
     for (int r = 0; r < 4; r++) {
         o.res[r] = 0.0;
-        for (int c = 0; c < 4; c++) o.jac[r][c] = 0.0;
+        o.q[r] = 0.0;
+        for (int c = 0; c < 4; c++) {
+            o.jac[r][c] = 0.0;
+            o.cjac[r][c] = 0.0;
+        }
     }
     switch (kind) {
     case K_VSRC:
         o.res[0] = lx[2];
         o.res[1] = -lx[2];
-        o.res[2] = lx[0] - lx[1] - p[0];
+        o.res[2] = lx[0] - lx[1] - (tran ? src_value(p, t) : p[SRCP_DC]);
         o.jac[0][2] = 1.0;
         o.jac[1][2] = -1.0;
         o.jac[2][0] = 1.0;
@@ -554,6 +622,17 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o)
         o.jac[1][1] = p[0];
         break;
     }
+    case K_CAP:
+        if (tran) {
+            const double qv = p[0] * (lx[0] - lx[1]);
+            o.q[0] = qv;
+            o.q[1] = -qv;
+            o.cjac[0][0] = p[0];
+            o.cjac[0][1] = -p[0];
+            o.cjac[1][0] = -p[0];
+            o.cjac[1][1] = p[0];
+        }
+        break;
     case K_DIODE: eval_diode(p, lx, o); break;
     case K_MOS1: eval_mos1(p, lx, o); break;
     default: break;

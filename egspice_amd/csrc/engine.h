@@ -60,15 +60,15 @@ bool is_tridiag(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
 // `at`) that a workgroup copies into LDS before its first tick.
 struct InstEntry;
 struct WaveBlob {
-    uint32_t n, nnz, cap, n_inst, n_stage, n_prm, n_lim, tri;
+    uint32_t n, nnz, cap, n_inst, n_stage, n_prm, n_lim, tri, n_tape, _pad;
     uint64_t diag_fill;
     uint32_t col_ptr, row_idx, q, tri_pos, diag, cur_row, inst, nodes, slots, node_off, slot_off, rptr, ridx, gptr,
-        gidx, words;
+        gidx, qptr, qidx, cptr, cidx, words;
 };
 struct WaveHost {
     std::vector<uint8_t> inst_bytes;  // InstEntry array
-    std::vector<uint32_t> nodes_flat, slots_flat, node_off, slot_off, rptr, ridx, gptr, gidx;
-    uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0;
+    std::vector<uint32_t> nodes_flat, slots_flat, node_off, slot_off, rptr, ridx, gptr, gidx, qptr, qidx, cptr, cidx;
+    uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0, n_tape = 0;
     WaveBlob at{};
     std::vector<uint32_t> blob;
 };

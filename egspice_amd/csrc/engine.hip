@@ -33,6 +33,8 @@
 #include "engine.h"
 #include "wave.h"
 #include "wave_lu.h"
+#include "wave_newton.h"
+#include "wave_tran.h"
 
 using namespace esp;
 
@@ -118,65 +120,335 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
         lim_active = true;
     }
     const ReduceCache rc = wave_reduce_cache(W, tid);
-    uint32_t bits = o.init_fix ? G_INITFIX : 0;
-    double limit = 1e9, scaled = 0;
     uint32_t st = 0, it = o.max_iter;
     __syncthreads();
-    for (uint32_t tick = 0; tick < o.max_iter; tick++) {
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
-        __syncthreads();
-        wave_reduce(W, s, rc, tid, nt);
-        __syncthreads();
-        if (DENSE) {
-            if (tid == 0) wave_assemble(W, s, o, tick, limit);
-            const int fl = wave_direct_factor(W.P, s.lu, D, s.g, &s.flags[2]);
-            if (fl == 0) {
-                if (s.lu.state[ST_TRI]) {
-                    if (tid == 0) {
-                        LV<double> rhs{s.rhs, 1}, dx{s.dx, 1};
-                        direct_solve_neg(W.P, s.lu, rhs, dx);
-                    }
-                } else {
-                    dense_solve_neg(W.P, D, s.rhs, s.dx);
-                }
-                __syncthreads();
-                scaled = wave_update_norm(n, s.x, s.dx, s.x_old, W.cur_row, o.reltol, o.abstol, o.vntol);
-            }
-            if (tid == 0) s.flags[0] = fl != 0 ? 1u : 0u;
-        } else {
+    if (DENSE) {
+        const WaveHook off{};
+        const int r = wave_newton(W, s, D, rc, o, off, o.init_fix != 0, lim_active, false, 0.0, it);
+        st = r == NEWTON_CONVERGED ? 1u : r == NEWTON_SINGULAR ? 2u : 0u;
+    } else {
+        uint32_t bits = o.init_fix ? G_INITFIX : 0;
+        double limit = 1e9, scaled = 0;
+        for (uint32_t tick = 0; tick < o.max_iter; tick++) {
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
+            __syncthreads();
+            wave_reduce(W, s, rc, tid, nt);
+            __syncthreads();
             if (tid == 0) s.flags[0] = wave_solve(W, s, o, tick, limit, scaled) ? 0u : 1u;
+            __syncthreads();
+            if (s.flags[0]) {
+                st = 2;
+                it = 0;
+                break;
+            }
+            int lim_any = 0;
+            for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
+            lim_any = __syncthreads_or(lim_any);
+            lim_active = true;
+            if (tid == 0) {
+                bool ok = wave_gates(W, s, o, tick, lim_any != 0, limit, scaled);
+                uint32_t verdict = 0;
+                if (ok && (bits & G_INITFIX)) bits &= ~G_INITFIX;
+                else if (ok) verdict = 1;
+                s.flags[1] = verdict;
+            }
+            __syncthreads();
+            if (s.flags[1]) {
+                for (uint32_t i = tid; i < n; i += nt) s.x[i] = s.x_old[i];
+                st = 1;
+                it = tick + 1;
+                break;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (s.flags[0]) {
-            st = 2;
-            it = 0;
-            break;
-        }
-        int lim_any = 0;
-        for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
-        lim_any = __syncthreads_or(lim_any);
-        lim_active = true;
-        if (tid == 0) {
-            bool ok = wave_gates(W, s, o, tick, lim_any != 0, limit, scaled);
-            uint32_t verdict = 0;
-            if (ok && (bits & G_INITFIX)) bits &= ~G_INITFIX;
-            else if (ok) verdict = 1;
-            s.flags[1] = verdict;
-        }
-        __syncthreads();
-        if (s.flags[1]) {
-            for (uint32_t i = tid; i < n; i += nt) s.x[i] = s.x_old[i];
-            st = 1;
-            it = tick + 1;
-            break;
-        }
-        __syncthreads();
     }
     __syncthreads();
     for (uint32_t i = tid; i < n; i += nt) x_glob[(size_t)i * N + l] = s.x[i];
     if (tid == 0) {
         status[l] = st;
         iters[l] = it;
+    }
+}
+
+// One lane's transient per workgroup (wave_tran.h): the operating point at
+// t = 0, then steps until t_stop, each accepted row fed to the .meas
+// passes and, up to `to.record_cap` rows, recorded. The lane's final
+// state goes back to x_glob.
+__global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
+                                                       const double *prm_lanes, double *x_glob, uint32_t N,
+                                                       Newton op_opts, Newton step_opts, TranOpts to,
+                                                       const MeasPass *passes, uint32_t n_pass, TranOut *outs,
+                                                       MeasOut *meas_out, double *rec_t, double *rec_x)
+{
+    extern __shared__ double lds[];
+    const uint32_t l = blockIdx.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t n = B.n, nnz = B.nnz;
+    uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
+    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + B.words / 2;
+    const WaveTemplate W = wave_template(B, tl);
+    WaveLane s = wave_lane(W, lay, block);
+    const DenseLU D = dense_lu(block, lay, n);
+    const uint32_t n_tape = W.n_tape;
+    double *const a = s.cc + (nnz + 1);
+    double *const ring = s.tape + n_tape;
+    double *const i_prev = ring + 4 * (size_t)n;
+    double *const tring = i_prev + n;
+    double *const ti_prev = tring + 4 * (size_t)n_tape;
+    double *const cur = ti_prev + n_tape;
+    double *const prev = cur + n;
+    double *const scal = prev + n;
+    double *const bpc = scal + TRAN_SCALARS;
+    MeasState *const ms = reinterpret_cast<MeasState *>(bpc + 2 * (size_t)W.n_inst);
+    double *prm_lds = block + lay.prm;
+    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    for (uint32_t i = tid; i < n; i += nt) {
+        s.x[i] = 0.0;
+        i_prev[i] = 0.0;
+    }
+    for (uint32_t j = tid; j < n_tape; j += nt) ti_prev[j] = 0.0;
+    __syncthreads();
+    // the .tran card's PULSE defaults (frontend/builder.zig:845-877)
+    for (uint32_t e = tid; e < W.n_inst; e += nt) {
+        const InstEntry ie = W.inst[e];
+        bpc[2 * e] = inf_();
+        bpc[2 * e + 1] = inf_();
+        if (ie.kind != K_VSRC) continue;
+        double *p = prm_lds + ie.prm;
+        if (p[SRCP_WAVE] != 1.0) {
+            p[SRCP_TD] = 1e30;
+        } else {
+            if (p[SRCP_TR] <= 0) p[SRCP_TR] = to.dt_init;
+            if (p[SRCP_TF] <= 0) p[SRCP_TF] = to.dt_init;
+            if (p[SRCP_PW] <= 0) p[SRCP_PW] = to.t_stop;
+            if (p[SRCP_PER] <= 0) p[SRCP_PER] = to.t_stop;
+        }
+    }
+    if (tid == 0) {
+        wave_fresh_solver(W, s);
+        for (uint32_t k = 0; k < n_pass; k++) meas_init(passes[k], ms[k]);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+    const ReduceCache rc = wave_reduce_cache(W, tid);
+    __syncthreads();
+
+    TranStep st{};
+    uint32_t it = 0;
+    const WaveHook off{};
+    const int op = wave_newton(W, s, D, rc, op_opts, off, true, true, true, 0.0, it);
+    if (op != NEWTON_CONVERGED) {
+        if (tid == 0) {
+            st.out.status = op == NEWTON_SINGULAR ? 3u : 2u;
+            outs[l] = st.out;
+            const double nan = __builtin_nan("");
+            for (uint32_t k = 0; k < n_pass; k++) meas_out[(size_t)l * n_pass + k] = MeasOut{nan, nan, nan};
+        }
+        return;
+    }
+    for (uint32_t i = tid; i < n; i += nt) {
+        cur[i] = s.x[i];
+        prev[i] = s.x[i];
+    }
+    const bool has_charge = n_tape > 0;
+    if (has_charge) {
+        // q(op) seeds a flat history
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true, true, 0.0);
+        __syncthreads();
+        for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
+        __syncthreads();
+        for (uint32_t k = 1; k < 4; k++) {
+            for (uint32_t i = tid; i < n; i += nt) ring[k * (size_t)n + i] = s.q[i];
+            for (uint32_t j = tid; j < n_tape; j += nt) tring[k * (size_t)n_tape + j] = s.tape[j];
+        }
+    }
+    const size_t cap = to.record_cap;
+    uint32_t points = 1;
+    if (cap > 0)
+        for (uint32_t i = tid; i < n; i += nt) rec_x[((size_t)l * cap) * n + i] = cur[i];
+    __syncthreads();
+    if (tid == 0) {
+        if (cap > 0) rec_t[(size_t)l * cap] = 0.0;
+        for (uint32_t k = 0; k < n_pass; k++) meas_feed(passes[k], ms[k], 0.0, cur[passes[k].row]);
+        st.delmin = 1e-11 * to.dt_max;
+        st.min_break = 10.0 * st.delmin;
+        st.bp_lo = inf_();
+        st.bp_hi = -inf_();
+        double dt = zmin(zmin(to.dt_init, to.t_stop / 100.0) / 10.0, to.dt_max);
+        const double bp0 = tran_next_bp(W, prm_lds, bpc, st, 0.0, st.min_break);
+        if (bp0 != inf_()) dt = zmin(dt, 0.1 * bp0);
+        st.dt = dt / 10.0;
+        for (int k = 0; k < 6; k++) st.dt_old[k] = to.dt_max;
+        st.order = st.run_order = 1;
+        st.bp_next = inf_();
+        st.bp_save_dt = to.t_stop / 50.0;
+        for (uint32_t k = 0; k < 4; k++) st.perm[k] = k;
+        st.out.points = 1;
+    }
+    for (;;) {
+        // Sim.begin
+        if (tid == 0) {
+            const bool go = !st.stopped && st.t < to.t_stop && !ulps_close(st.t, to.t_stop, 100) && st.steps < to.max_steps;
+            scal[0] = go ? 1.0 : 0.0;
+            if (go) {
+                st.dt_att = st.dt;
+                const int method = st.order == 1 ? 0 : (int)to.method;
+                const TranCoeffs c = tran_coeffs(method, st.dt, st.dt_old[0], to.xmu);
+                scal[1] = st.t + st.dt;
+                scal[2] = st.dt / st.dt_old[0];
+                scal[3] = (double)method;
+                scal[4] = c.ag0;
+                scal[5] = c.ag1;
+                scal[6] = c.ag2;
+                for (uint32_t k = 0; k < 4; k++) scal[8 + k] = (double)st.perm[k];
+            }
+        }
+        __syncthreads();
+        if (scal[0] == 0.0) break;
+        const double t_dev = scal[1], xfact = scal[2];
+        const int method = (int)scal[3];
+        const uint32_t p0 = (uint32_t)scal[8], p1 = (uint32_t)scal[9], p2 = (uint32_t)scal[10];
+        for (uint32_t i = tid; i < n; i += nt) s.x[i] = cur[i] + xfact * (cur[i] - prev[i]);
+        WaveHook h{true, method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
+        const int r = wave_newton(W, s, D, rc, step_opts, h, false, true, true, t_dev, it);
+        if (r == NEWTON_CONVERGED && has_charge) {
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true, true, t_dev);
+            __syncthreads();
+            for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += nt) ring[p0 * (size_t)n + i] = s.q[i];
+            for (uint32_t j = tid; j < n_tape; j += nt) tring[p0 * (size_t)n_tape + j] = s.tape[j];
+        }
+        __syncthreads();
+        // Sim.end up to the commit
+        if (tid == 0) {
+            const TranCoeffs c{method, scal[4], scal[5], scal[6]};
+            const double dt = st.dt;
+            bool accept = false;
+            st.out.attempts++;
+            st.out.nr_iters += r == NEWTON_SINGULAR ? 0u : it;
+            if (r != NEWTON_CONVERGED) {
+                st.out.rej_newton++;
+                st.order = 1;
+                st.dt = dt / 8.0;
+                tran_floor_retry(to, st);
+            } else {
+                double dt_next = st.steps == 0 ? dt : zmin(dt * 2.0, to.dt_max);
+                const LteIn e{dt, st.dt_old[0], st.dt_old[1], to.reltol, to.abstol, to.chgtol, to.trtol};
+                const uint32_t q3 = st.perm[3];
+                const double *t0p = tring + p0 * (size_t)n_tape, *t1p = tring + p1 * (size_t)n_tape;
+                const double *t2p = tring + p2 * (size_t)n_tape, *t3p = tring + q3 * (size_t)n_tape;
+                accept = true;
+                if (has_charge && st.steps > 0) {
+                    const double del = step_bound(method, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+                    if (!(del >= 0.9 * dt)) {
+                        st.out.rej_lte++;
+                        if (del != del) {
+                            st.order = 1;
+                            st.dt = dt / 8.0;
+                        } else {
+                            st.dt = del;
+                        }
+                        tran_floor_retry(to, st);
+                        accept = false;
+                    } else {
+                        dt_next = zmin(zmin(sel_max(del, to.dt_min), 2.0 * dt), to.dt_max);
+                    }
+                }
+                if (accept && has_charge) {
+                    st.order_run = st.order == st.run_order ? st.order_run + 1 : 1;
+                    st.run_order = st.order;
+                    if (st.steps > 0 && st.order == 1) {
+                        const int promote = (int)to.method;
+                        const double trial_del = step_bound(promote, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+                        if (trial_del == trial_del) {
+                            const double nd2 = zmin(2.0 * dt, trial_del);
+                            if (nd2 > 1.05 * dt) st.order = 2;
+                            dt_next = zmin(sel_max(nd2, to.dt_min), to.dt_max);
+                        }
+                    }
+                }
+                st.dt_next = dt_next;
+            }
+            scal[7] = accept ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (scal[7] == 0.0) continue;
+        // advanceCurrent on the pre-rotation rings, then cur <- trial
+        if (has_charge) {
+            const double ag0 = scal[4], ag1 = scal[5], ag2 = scal[6];
+            for (uint32_t i = tid; i < n; i += nt) {
+                const double q0v = ring[p0 * (size_t)n + i], q1v = ring[p1 * (size_t)n + i];
+                const double d = ag0 * (q0v - q1v);
+                i_prev[i] = method == 1 ? d - ag1 * i_prev[i] : method == 2 ? d - ag2 * (q1v - ring[p2 * (size_t)n + i]) : d;
+            }
+            for (uint32_t j = tid; j < n_tape; j += nt) {
+                const double q0v = tring[p0 * (size_t)n_tape + j], q1v = tring[p1 * (size_t)n_tape + j];
+                const double d = ag0 * (q0v - q1v);
+                ti_prev[j] = method == 1 ? d - ag1 * ti_prev[j]
+                             : method == 2 ? d - ag2 * (q1v - tring[p2 * (size_t)n_tape + j])
+                                           : d;
+            }
+        }
+        for (uint32_t i = tid; i < n; i += nt) {
+            prev[i] = cur[i];
+            cur[i] = s.x[i];
+        }
+        if (points < cap)
+            for (uint32_t i = tid; i < n; i += nt) rec_x[((size_t)l * cap + points) * n + i] = s.x[i];
+        points++;
+        __syncthreads();
+        // Sim.end after the commit: rotation, landing, the row, next dt
+        if (tid == 0) {
+            if (has_charge) {
+                const uint32_t last = st.perm[3];
+                for (int k = 3; k > 0; k--) st.perm[k] = st.perm[k - 1];
+                st.perm[0] = last;
+            }
+            for (int k = 5; k > 0; k--) st.dt_old[k] = st.dt_old[k - 1];
+            st.dt_old[0] = st.dt;
+            st.t = st.t + st.dt;
+            st.steps++;
+            const double t_new = st.t;
+            double dt_next = st.dt_next;
+            if (st.bp_next != inf_()) {
+                const double bp = st.bp_next;
+                const bool landed = st.bp_clamped ? fabs_(t_new - bp) <= st.min_break
+                                                  : (bp - t_new <= st.delmin || ulps_close(t_new, bp, 100));
+                if (landed) {
+                    st.out.bp_landings++;
+                    st.order = 1;
+                    double shrink = st.bp_save_dt;
+                    const double nb = tran_next_bp(W, prm_lds, bpc, st, t_new, t_new + st.min_break);
+                    if (nb != inf_()) shrink = zmin(shrink, nb - t_new);
+                    dt_next = zmin(dt_next, 0.1 * shrink);
+                }
+                st.bp_next = inf_();
+            }
+            if (st.out.points < cap) rec_t[(size_t)l * cap + st.out.points] = t_new;
+            st.out.points++;
+            for (uint32_t k = 0; k < n_pass; k++) meas_feed(passes[k], ms[k], t_new, cur[passes[k].row]);
+            st.bp_next = tran_next_bp(W, prm_lds, bpc, st, t_new, t_new + st.min_break);
+            st.bp_clamped = false;
+            if (st.bp_next != inf_()) {
+                const double dt_to_bp = st.bp_next - t_new;
+                if (dt_to_bp < dt_next) {
+                    st.bp_save_dt = dt_next;
+                    dt_next = dt_to_bp;
+                    st.bp_clamped = true;
+                }
+            }
+            st.dt = dt_next;
+            if (t_new + st.dt > to.t_stop) st.dt = to.t_stop - t_new;
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < n; i += nt) x_glob[(size_t)i * N + l] = cur[i];
+    if (tid == 0) {
+        st.out.steps = st.steps;
+        outs[l] = st.out;
+        for (uint32_t k = 0; k < n_pass; k++) meas_out[(size_t)l * n_pass + k] = meas_result(passes[k], ms[k]);
     }
 }
 
@@ -552,6 +824,77 @@ espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *
         *kernel_ms = ms;
     }
     for (uint32_t l = 0; l < L->N; l++) out[l] = espice_lane_result{st[l], it[l]};
+    return ESPICE_OK;
+}
+
+static_assert(sizeof(espice_tran_opts) == sizeof(TranOpts), "espice_tran_opts mirrors TranOpts");
+static_assert(sizeof(espice_meas_pass) == sizeof(MeasPass), "espice_meas_pass mirrors MeasPass");
+static_assert(sizeof(espice_meas_result) == sizeof(MeasOut), "espice_meas_result mirrors MeasOut");
+static_assert(sizeof(espice_tran_result) == sizeof(TranOut), "espice_tran_result mirrors TranOut");
+
+espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, const espice_meas_pass *passes,
+                                uint32_t n_passes, espice_tran_result *out, espice_meas_result *meas, double *times,
+                                double *rows, double *kernel_ms)
+{
+    if (!L || !opts || !out || (n_passes && (!passes || !meas))) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    const uint32_t n = L->t->n, N = L->N;
+    const size_t cap = opts->record_capacity;
+    if (cap && (!times || !rows)) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!(opts->t_stop > 0) || !(opts->dt_init > 0) || !(opts->dt_max > 0) || opts->method > 2)
+        return fail(ESPICE_INVALID_ARGUMENT, "InvalidTran");
+    for (uint32_t k = 0; k < n_passes; k++)
+        if (passes[k].kind > 3 || passes[k].row >= n) return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
+    // The charge models of the junction and Meyer capacitances (diode.va,
+    // mos1.va) are not staged yet: refuse rather than run them as static.
+    if (L->t->count[K_DIODE] || L->t->count[K_MOS1]) return fail(ESPICE_FAILED, "TranChargesUnsupported");
+    WaveLayout lay{};
+    const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
+    const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));
+    const size_t lds = 4 * (size_t)L->wh.at.words + 8 * words;
+    if (lay.dense == ESP_NONE || lds > 64 * 1024) return fail(ESPICE_FAILED, "LaneTooLarge");
+    DevBuf<MeasPass> d_pass;
+    DevBuf<TranOut> d_out;
+    DevBuf<MeasOut> d_meas;
+    DevBuf<double> d_t, d_x;
+    HIP_TRY(d_pass.alloc(n_passes));
+    HIP_TRY(d_out.alloc(N));
+    HIP_TRY(d_meas.alloc((size_t)N * n_passes));
+    HIP_TRY(d_t.alloc(cap ? (size_t)N * cap : 1));
+    HIP_TRY(d_x.alloc(cap ? (size_t)N * cap * n : 1));
+    if (n_passes)
+        HIP_TRY(hipMemcpyAsync(d_pass.p, passes, sizeof(MeasPass) * n_passes, hipMemcpyHostToDevice, L->stream));
+    TranOpts to;
+    std::memcpy(&to, opts, sizeof to);
+    Newton op{};
+    op.reltol = opts->reltol;
+    op.abstol = opts->abstol;
+    op.vntol = opts->vntol;
+    op.residual_tol = 1e-9;
+    op.gmin = 0.0;
+    op.max_iter = opts->max_iter;
+    op.init_fix = 1;
+    Newton step = op;
+    step.residual_tol = inf_();
+    step.init_fix = 0;
+    HIP_TRY(hipEventRecord(L->ev0, L->stream));
+    hipLaunchKernelGGL(tran_wave_kernel, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay, L->prm_lanes.p,
+                       L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(L->ev1, L->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));
+    if (n_passes)
+        HIP_TRY(hipMemcpyAsync(meas, d_meas.p, sizeof(MeasOut) * N * n_passes, hipMemcpyDeviceToHost, L->stream));
+    if (cap) {
+        HIP_TRY(hipMemcpyAsync(times, d_t.p, sizeof(double) * N * cap, hipMemcpyDeviceToHost, L->stream));
+        HIP_TRY(hipMemcpyAsync(rows, d_x.p, sizeof(double) * N * cap * n, hipMemcpyDeviceToHost, L->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    if (kernel_ms) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+        *kernel_ms = ms;
+    }
     return ESPICE_OK;
 }
 

@@ -131,8 +131,8 @@ namespace esp {
 void build_wave(const Template &T, uint32_t cap, WaveHost &H)
 {
     std::vector<InstEntry> inst;
-    uint32_t stage = 0, prm = 0, lim = 0;
-    std::vector<std::vector<uint32_t>> rows(T.n), slots(T.nnz);
+    uint32_t stage = 0, prm = 0, lim = 0, tape = 0;
+    std::vector<std::vector<uint32_t>> rows(T.n), slots(T.nnz), qrows(T.n), cslots(T.nnz);
     H.nodes_flat.clear();
     H.slots_flat.clear();
     H.node_off.clear();
@@ -170,9 +170,37 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
                 }
             }
             stage += kg;
+            // charge cells: q per row with a C entry, then one per C entry,
+            // added in the serial order the transient evaluation uses
+            bool charged = false;
+            for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(kind, ru) != 0;
+            e.stage_q = ESP_NONE;
+            e.stage_c = ESP_NONE;
+            e.tape = kind == K_CAP ? tape++ : ESP_NONE;
+            if (charged) {
+                e.stage_q = stage;
+                stage += nu;
+                e.stage_c = stage;
+                uint32_t kc = 0;
+                kp = 0;
+                for (int ru = 0; ru < nu; ru++) {
+                    const unsigned cm = kind_cmask(kind, ru), pm = kind_pmask(kind, ru);
+                    if (cm != 0 && nd[ru] != 0) qrows[nd[ru]].push_back(e.stage_q + ru);
+                    for (int cu = 0; cu < nu; cu++) {
+                        if (!(pm >> cu & 1)) continue;
+                        const uint32_t slot = sl[kp++];
+                        if (cm >> cu & 1) {
+                            if (slot != T.nnz) cslots[slot].push_back(e.stage_c + kc);
+                            kc++;
+                        }
+                    }
+                }
+                stage += kc;
+            }
             inst.push_back(e);
         }
     }
+    H.n_tape = tape;
     H.n_inst = (uint32_t)inst.size();
     H.n_stage = stage;
     H.n_prm = prm;
@@ -190,6 +218,18 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
     for (uint32_t s = 0; s < T.nnz; s++) {
         H.gidx.insert(H.gidx.end(), slots[s].begin(), slots[s].end());
         H.gptr.push_back((uint32_t)H.gidx.size());
+    }
+    H.qptr.assign(1, 0);
+    H.qidx.clear();
+    for (uint32_t r = 0; r < T.n; r++) {
+        H.qidx.insert(H.qidx.end(), qrows[r].begin(), qrows[r].end());
+        H.qptr.push_back((uint32_t)H.qidx.size());
+    }
+    H.cptr.assign(1, 0);
+    H.cidx.clear();
+    for (uint32_t s = 0; s < T.nnz; s++) {
+        H.cidx.insert(H.cidx.end(), cslots[s].begin(), cslots[s].end());
+        H.cptr.push_back((uint32_t)H.cidx.size());
     }
     // the LDS blob: every table at a word offset
     WaveBlob &B = H.at;
@@ -232,6 +272,11 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
     B.ridx = put(H.ridx.data(), H.ridx.size());
     B.gptr = put(H.gptr.data(), H.gptr.size());
     B.gidx = put(H.gidx.data(), H.gidx.size());
+    B.qptr = put(H.qptr.data(), H.qptr.size());
+    B.qidx = put(H.qidx.data(), H.qidx.size());
+    B.cptr = put(H.cptr.data(), H.cptr.size());
+    B.cidx = put(H.cidx.data(), H.cidx.size());
+    B.n_tape = H.n_tape;
     if (w.size() & 1) w.push_back(0);  // the lane block after it is 8-byte aligned
     B.words = (uint32_t)w.size();
 }

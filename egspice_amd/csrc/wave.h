@@ -33,6 +33,9 @@ struct InstEntry {
     uint32_t stage_jac;  // first Jacobian cell (one per G entry)
     uint32_t prm;        // first prepared constant in the lane's block
     uint32_t lim;        // first limiter image word
+    uint32_t stage_q;    // first charge cell (NU cells), transient only
+    uint32_t stage_c;    // first dQ/du cell (one per C entry), transient only
+    uint32_t tape;       // LTE charge site, or ESP_NONE
 };
 
 // Template-wide data every workgroup reads (device or host pointers).
@@ -48,13 +51,16 @@ struct WaveTemplate {
     const uint32_t *slot_off;    // per instance: first word in slots_flat
     const uint32_t *rptr, *ridx;  // residual row r sums stage cells ridx[rptr[r]..rptr[r+1])
     const uint32_t *gptr, *gidx;  // G slot s sums stage cells gidx[gptr[s]..gptr[s+1])
+    const uint32_t *qptr, *qidx;  // charge row r (transient)
+    const uint32_t *cptr, *cidx;  // C slot s (transient)
+    uint32_t n_tape;              // LTE charge sites
     uint32_t n_stage;             // staging cells per lane
     uint32_t n_prm;               // prepared constants per lane
     uint32_t n_lim;               // limiter words per lane
     uint32_t cap;                 // L+U capacity per lane
 };
 
-static_assert(sizeof(InstEntry) == 24, "InstEntry packs into six u32 words");
+static_assert(sizeof(InstEntry) == 36, "InstEntry packs into nine u32 words");
 
 // The template's tables read from a copy of its blob (at `base`, u32 words).
 ESP_HD WaveTemplate wave_template(const WaveBlob &B, const uint32_t *base)
@@ -74,6 +80,11 @@ ESP_HD WaveTemplate wave_template(const WaveBlob &B, const uint32_t *base)
     W.ridx = base + B.ridx;
     W.gptr = base + B.gptr;
     W.gidx = base + B.gidx;
+    W.qptr = base + B.qptr;
+    W.qidx = base + B.qidx;
+    W.cptr = base + B.cptr;
+    W.cidx = base + B.cidx;
+    W.n_tape = B.n_tape;
     W.n_stage = B.n_stage;
     W.n_prm = B.n_prm;
     W.n_lim = B.n_lim;
@@ -85,8 +96,18 @@ ESP_HD WaveTemplate wave_template(const WaveBlob &B, const uint32_t *base)
 struct WaveLayout {
     uint32_t x, x_old, dx, g, rhs, lim, stage, prm, lu_f64, lu_u32, lu_u8, flags;
     uint32_t dense;  // wave_lu.h's dense replay arrays, or ESP_NONE when n > 64
+    uint32_t tran;   // the transient's arrays (wave_tran.h), or ESP_NONE
     uint32_t words;
 };
+
+// 8-byte words of a transient lane's extra arrays (wave_tran.h TranLane):
+// q (n), cc (nnz + 1), a (nnz + 1), the current tape (n_tape), the charge
+// ring (4 n) and its dynamic current (n), the tape ring (4 n_tape) and its
+// current (n_tape), x_prev and x_trial (n each), and 32 step scalars.
+ESP_HD size_t tran_words(uint32_t n, uint32_t nnz, uint32_t n_tape)
+{
+    return (size_t)n + 2 * ((size_t)nnz + 1) + n_tape + 5 * (size_t)n + 5 * (size_t)n_tape + 2 * (size_t)n + 32;
+}
 
 // Largest pattern the dense wave replay takes: one pivot row per thread of
 // the lane's 64-wide wavefront.
@@ -105,9 +126,10 @@ struct WaveLane {
     const double *prm;
     LaneLU lu;
     uint32_t *flags;  // 0 singular, 1 limited, 2 done, 3 bits, 4 ok, 5 scaled bits lo, 6 hi
+    double *q, *cc, *tape;  // transient: row charges, C plane, charge per LTE site
 };
 
-ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L)
+ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L, bool transient = false, size_t tran_extra = 0)
 {
     const size_t n = W.P.n, nnz = W.P.nnz, cap = W.cap;
     size_t at = 0;
@@ -135,6 +157,11 @@ ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L)
         at += dense_words((uint32_t)n);
     } else {
         L.dense = ESP_NONE;
+    }
+    L.tran = ESP_NONE;
+    if (transient) {
+        L.tran = (uint32_t)at;
+        at += tran_words((uint32_t)n, (uint32_t)nnz, W.n_tape) + tran_extra;
     }
     L.words = (uint32_t)at;
     return at;
@@ -188,6 +215,13 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
     u.void_col = LV<uint8_t>{b, 1}; b += n;
     u.scaled_pivot = LV<uint8_t>{b, 1};
     s.flags = reinterpret_cast<uint32_t *>(block + L.flags);
+    s.q = s.cc = s.tape = nullptr;
+    if (L.tran != ESP_NONE) {
+        double *t = block + L.tran;
+        s.q = t;
+        s.cc = t + n;
+        s.tape = t + n + 2 * ((size_t)nnz + 1);
+    }
     return s;
 }
 
@@ -203,7 +237,8 @@ struct LP {
 
 // Phase: evaluate instance e into its staging cells.
 template <int KIND>
-ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active)
+ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active,
+                        bool tran, double t)
 {
     constexpr int nu = KIND_NU[KIND];
     constexpr unsigned limw = KIND_LIMW[KIND];
@@ -219,7 +254,7 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
             if (limw >> u & 1) lx[u] = s.lim[ie.lim + u];
     }
     DevOut o;
-    eval_kind(KIND, p, lx, o);
+    eval_kind(KIND, p, lx, o, tran, t);
     if (use_lim) {
         double corr[nu];
         bool live = false;
@@ -247,20 +282,60 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
         for (int cu = 0; cu < nu; cu++)
             if (gm >> cu & 1) s.stage[ie.stage_jac + k++] = o.jac[ru][cu];
     }
+    bool charged = false;
+#pragma unroll
+    for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(KIND, ru) != 0;
+    if (charged && tran) {
+        uint32_t kc = 0;
+#pragma unroll
+        for (int ru = 0; ru < nu; ru++) {
+            s.stage[ie.stage_q + ru] = o.q[ru];
+            const unsigned cm = kind_cmask(KIND, ru);
+#pragma unroll
+            for (int cu = 0; cu < nu; cu++)
+                if (cm >> cu & 1) s.stage[ie.stage_c + kc++] = o.cjac[ru][cu];
+        }
+        if (ie.tape != ESP_NONE) s.tape[ie.tape] = o.q[0];
+    }
 }
 
-ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
+ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, bool tran = false,
+                           double t = 0.0)
 {
     const InstEntry ie = W.inst[e];
     switch (ie.kind) {
-    case K_VSRC: wave_eval_k<K_VSRC>(W, s, ie, e, lim_active); break;
-    case K_ISRC: wave_eval_k<K_ISRC>(W, s, ie, e, lim_active); break;
-    case K_RES: wave_eval_k<K_RES>(W, s, ie, e, lim_active); break;
-    case K_CAP: wave_eval_k<K_CAP>(W, s, ie, e, lim_active); break;
-    case K_DIODE: wave_eval_k<K_DIODE>(W, s, ie, e, lim_active); break;
-    case K_MOS1: wave_eval_k<K_MOS1>(W, s, ie, e, lim_active); break;
+    case K_VSRC: wave_eval_k<K_VSRC>(W, s, ie, e, lim_active, tran, t); break;
+    case K_ISRC: wave_eval_k<K_ISRC>(W, s, ie, e, lim_active, tran, t); break;
+    case K_RES: wave_eval_k<K_RES>(W, s, ie, e, lim_active, tran, t); break;
+    case K_CAP: wave_eval_k<K_CAP>(W, s, ie, e, lim_active, tran, t); break;
+    case K_DIODE: wave_eval_k<K_DIODE>(W, s, ie, e, lim_active, tran, t); break;
+    case K_MOS1: wave_eval_k<K_MOS1>(W, s, ie, e, lim_active, tran, t); break;
     default: break;
     }
+}
+
+// Phase (transient): charge row r (e < n) or C slot e - n over its staging
+// cells in tape order, loaded eight at a time before the adds.
+ESP_HD void wave_reduce_charge(const WaveTemplate &W, WaveLane &s, uint32_t e)
+{
+    const uint32_t n = W.P.n;
+    const uint32_t *ptr = e < n ? W.qptr + e : W.cptr + (e - n);
+    const uint32_t *idx = e < n ? W.qidx : W.cidx;
+    const uint32_t c0 = ptr[0], c1 = ptr[1];
+    double acc = 0.0;
+    for (uint32_t c = c0; c < c1; c += 8) {
+        uint32_t ix[8];
+        double v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) ix[k] = idx[c + k < c1 ? c + k : c1 - 1];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) v[k] = s.stage[ix[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++)
+            if (c + k < c1) acc += v[k];
+    }
+    if (e < n) s.q[e] = acc;
+    else s.cc[e - n] = acc;
 }
 
 // Phase: residual row or G slot `e` of the planes (e < n rows, then slots):
@@ -436,11 +511,13 @@ ESP_HD bool wave_solve(const WaveTemplate &W, WaveLane &s, const Newton &o, uint
 }
 
 // The gates after the limit pass (finalizeStep's order), thread 0.
+// `vals` is the matrix the tick factored (G, or G + ag0 C under the
+// transient's hook): the residual gate scales rows by its diagonal.
 ESP_HD bool wave_gates(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, bool limited, double limit,
-                       double scaled)
+                       double scaled, double *vals = nullptr)
 {
     const uint32_t n = W.P.n;
-    LV<double> g{s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1};
+    LV<double> g{vals ? vals : s.g, 1}, rhs{s.rhs, 1}, x{s.x, 1};
     bool ok = !limited && tick != 0 && !(scaled >= 1.0);
     if (ok) ok = plausible(n, x, limit);
     if (ok) ok = residual_ok(n, o, x, rhs, g, W.diag);

@@ -5,7 +5,7 @@ give, so the engine's template and any other builder with the same
 """
 import numpy as np
 
-from . import MOS1, MOS1_PARAMS, VSOURCE, VSOURCE_DC
+from . import CAPACITOR, MOS1, MOS1_PARAMS, RESISTOR, VSOURCE, VSOURCE_DC
 
 M = MOS1_PARAMS
 
@@ -37,3 +37,39 @@ def inverter_chain(add, stages=5, vdd=1.8):
 def dc_sweep_points(n_points=64, vmax=1.8):
     """Input values of the chain's sweep: vin_k = vmax * k / (n_points - 1)."""
     return vmax * np.arange(n_points) / (n_points - 1)
+
+
+def pulse_params(v1, v2, td, tr, tf, pw, per, dc=None):
+    """V source parameters of `PULSE(v1 v2 td tr tf pw per)` (models/vsource.va
+    order: dc, wave, v1, v2, td, tr, tf, pw, per); dc None reads as not given."""
+    return {0: np.nan if dc is None else dc, 1: 1.0, 2: v1, 3: v2, 4: td, 5: tr, 6: tf, 7: pw, 8: per}
+
+
+# docs/examples/measure.sp: V1 in 0 PULSE(0 1 1u 1n 1n 5u 20u); R1 in out 1k;
+# C1 out 0 1n; .tran 10n 10u
+MEASURE_PULSE = (0.0, 1.0, 1e-6, 1e-9, 1e-9, 5e-6, 20e-6)
+# docs/examples/rc_lowpass.sp: V1 in 0 DC 0 PULSE(0 1 0 1u 1u 5m 10m);
+# R1 in out 1k; C1 out 0 159n; .tran 10u 5m
+RC_LOWPASS_PULSE = (0.0, 1.0, 0.0, 1e-6, 1e-6, 5e-3, 10e-3)
+
+
+def rc_pulse(add, r=1e3, c=1e-9, pulse=MEASURE_PULSE, dc=None):
+    """V1 in 0 PULSE(...); R1 in out r; C1 out 0 c, in the order a deck's
+    builder adds elements (by letter: v, then c, then r;
+    src/frontend/builder.zig:886). Returns the capacitor's instance index."""
+    add(VSOURCE, ["in", "0"], pulse_params(*pulse, dc=dc))
+    ci = add(CAPACITOR, ["out", "0"], {0: c})
+    add(RESISTOR, ["in", "out"], {0: r})
+    return ci
+
+
+def rc_ladder(add, stages=3, r=1e3, c=1e-9, pulse=MEASURE_PULSE):
+    """V1 in 0 PULSE(...); R<k> n<k-1> n<k> r; C<k> n<k> 0 c (n0 = in),
+    added v, c, r like rc_pulse."""
+    add(VSOURCE, ["in", "0"], pulse_params(*pulse))
+    for k in range(1, stages + 1):
+        add(CAPACITOR, [f"n{k}", "0"], {0: c})
+    prev = "in"
+    for k in range(1, stages + 1):
+        add(RESISTOR, [prev, f"n{k}"], {0: r})
+        prev = f"n{k}"

@@ -129,6 +129,58 @@ espice_status espice_lanes_tab_build(uint32_t n, const uint32_t *col_ptr, const 
 espice_status espice_lanes_tab_refactor_solve(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
     uint32_t n_lanes, const double *a, const double *rhs, double *dx, uint8_t *fail, double *kernel_ms);
 
+/* ---- transient lanes (the device step control of tran.runGpuLanes,
+ * src/analysis/tran/tran.zig:1736-1912, with src/solver/kernels/lane.zig's
+ * step machine :353-700 and the .meas passes of src/output/scan.zig) ----
+ * `.tran dt_init t_stop` with the step ceiling dt_max resolved by the
+ * caller (a deck's is min(dt_init, t_stop/50), src/core/query.zig:673-680);
+ * method 0 BE, 1 trapezoidal (default), 2 gear-2; tolerances as
+ * src/core/numerics.zig:378-399 (reltol 1e-3, abstol 1e-12, vntol 1e-6,
+ * chgtol 1e-14, trtol 7); max_iter as optionsFromTolerances (100). */
+typedef struct {
+    double t_stop, dt_init, dt_max, dt_min, xmu;
+    double reltol, abstol, vntol, chgtol, trtol;
+    uint32_t method, max_iter, max_steps, record_capacity;
+} espice_tran_opts;
+
+/* One .meas pass over an unknown `row` (a node voltage or a branch
+ * current), scan.zig's kinds: 0 Cross (WHEN/TRIG/TARG: val, td, from, to,
+ * rise/fall/cross counts, -1 not given, -2 LAST), 1 At (FIND, op 1 DERIV:
+ * at), 2 Extremum (op 0 MIN, 1 MAX, 2 AVG: from, to), 3 Integ (op 0 INTEG,
+ * 1 RMS: from, to). */
+typedef struct {
+    uint32_t kind, row;
+    int32_t rise, fall, cross;
+    uint32_t op;
+    double val, td, from, to, at;
+} espice_meas_pass;
+
+/* Cross: value = the event time; At: value; Extremum: value, aux0 = where
+ * (AVG: the last time read); Integ: value, aux0 = from, aux1 = to. NaN when
+ * the pass found nothing. */
+typedef struct {
+    double value, aux0, aux1;
+} espice_meas_result;
+
+/* status 0 reached t_stop, 1 dt underflow, 2 no operating point, 3
+ * singular operating point; points = recorded rows including t = 0. */
+typedef struct {
+    uint32_t status, points, steps, attempts, rej_newton, rej_lte, bp_landings, nr_iters;
+} espice_tran_result;
+
+/* Every lane's transient in one launch from a cold operating point:
+ * out[n_lanes]; meas[n_lanes * n_passes] lane-major by pass; when
+ * record_capacity > 0, times[n_lanes * record_capacity] and
+ * rows[n_lanes * record_capacity * n] receive each lane's first
+ * record_capacity accepted rows (t = 0 included). download_x then returns
+ * each lane's state at its last accepted point. FAILED "LaneTooLarge" for a
+ * template whose transient lane does not fit one workgroup's LDS or has
+ * more than 64 unknowns; FAILED "TranChargesUnsupported" for a template
+ * with diodes or mos1 devices, whose charge models are not staged. */
+espice_status espice_lanes_tran(espice_lanes *l, const espice_tran_opts *opts, const espice_meas_pass *passes,
+    uint32_t n_passes, espice_tran_result *out, espice_meas_result *meas, double *times, double *rows,
+    double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

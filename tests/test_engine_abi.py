@@ -48,6 +48,23 @@ def test_template_matches_oracle_pattern(oracle, engine):
         assert not t.tridiagonal
 
 
+def test_rc_template_rows_and_frozen_names(oracle, engine):
+    """The RC decks number nodes alike in both builders, and a frozen
+    template still answers for the names it holds (and only those)."""
+    for builder, kw in ((W.rc_pulse, {}), (W.rc_ladder, {"stages": 4})):
+        t = E.Template()
+        builder(t.add, **kw)
+        t.freeze()
+        c = O.Circuit(oracle)
+        builder(c.add, **kw)
+        c.freeze()
+        assert np.array_equal(t.col_ptr, c.col_ptr) and np.array_equal(t.row_idx, c.row_idx)
+        name = "out" if builder is W.rc_pulse else "n4"
+        assert t.node(name) == c.node(name)
+        with pytest.raises(E.EngineError, match="TemplateFrozen"):
+            t.node("not_there")
+
+
 def status_of(fn, *args):
     return fn(*args)
 

@@ -1,0 +1,226 @@
+"""GPU parity of the transient lanes against the CPU oracle, through the C ABI.
+
+Each lane runs the whole transient on the GPU (operating point, steps,
+breakpoints, LTE control, .meas passes); the oracle runs the same lane on
+the CPU. Every recorded row, every step statistic and every .meas value
+must match bit for bit (tolerance: 0 ulp; a NaN matches any NaN). The
+nominal lane of docs/examples/measure.sp must also reproduce the
+reference's printed results (docs/learn/measure.md:24-36). Each case prints
+its name and sizes before touching the GPU.
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import egspice_amd as E
+from egspice_amd import workloads as W
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+CAP = 4096
+MAX_STEPS = 200_000
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def build_both(oracle, builder, **kw):
+    t = E.Template()
+    builder(t.add, **kw)
+    t.freeze()
+    c = O.Circuit(oracle)
+    builder(c.add, **kw)
+    c.freeze()
+    assert (t.col_ptr == c.col_ptr).all() and (t.row_idx == c.row_idx).all()
+    return t, c
+
+
+def bits_equal(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        return False
+    same = a.view(np.uint64) == b.view(np.uint64)
+    return bool(np.all(same | (np.isnan(a) & np.isnan(b))))
+
+
+def P(a):
+    return np.ascontiguousarray(a).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def printed(value, golden, digits):
+    assert value == pytest.approx(golden, rel=0, abs=0.5 * 10 ** (math.floor(math.log10(abs(golden))) - digits) * 1.0001)
+
+
+def measure_passes(t):
+    """The ten .meas lines of docs/examples/measure.sp as lane passes."""
+    vin, vout, iv1 = t.node("in"), t.node("out"), 2
+    return [
+        E.meas_cross(vout, 0.1, rise=1),            # trise TRIG
+        E.meas_cross(vout, 0.9, rise=1),            # trise TARG
+        E.meas_cross(vin, 0.5, rise=1),             # tpd TRIG
+        E.meas_cross(vout, 0.5, rise=1),            # tpd TARG
+        E.meas_at(vout, 3e-6),                      # vout_3u FIND AT
+        E.meas_cross(vout, 0.5, fall=1),            # t_half WHEN FALL
+        E.meas_extremum(vout, 1),                   # vmax MAX
+        E.meas_extremum(vout, 2, frm=0.0, to=10e-6),  # vavg AVG
+        E.meas_integ(vout, rms=True, frm=0.0, to=10e-6),  # vrms RMS
+        E.meas_at(vout, 2e-6, deriv=True),          # slope DERIV AT
+        E.meas_integ(iv1, frm=1e-6, to=6e-6),       # q_src INTEG I(V1)
+    ]
+
+
+def oracle_meas(oracle, passes, tt, xx):
+    """The oracle's scan passes over one recorded waveform, shaped like
+    the engine's (value, aux0, aux1) rows."""
+    N = len(tt)
+    out = np.full((len(passes), 3), np.nan)
+    for k, p in enumerate(passes):
+        y = np.ascontiguousarray(xx[:, p.row])
+        if p.kind == E.MEAS_CROSS:
+            cl = O.clause(val=p.val, rise=p.rise, fall=p.fall, cross=p.cross, td=p.td, frm=p.frm, to=p.to)
+            out[k, 0] = oracle.or_meas_cross(P(tt), P(y), N, C.byref(cl))
+        elif p.kind == E.MEAS_AT:
+            out[k, 0] = oracle.or_meas_at(P(tt), P(y), N, p.at, p.op)
+        elif p.kind == E.MEAS_EXTREMUM:
+            v, a = C.c_double(), C.c_double()
+            oracle.or_meas_extremum(P(tt), P(y), N, p.frm, p.to, p.op, C.byref(v), C.byref(a))
+            out[k, :2] = v.value, a.value
+        else:
+            v, a, b = C.c_double(), C.c_double(), C.c_double()
+            oracle.or_meas_integ(P(tt), P(y), N, p.frm, p.to, p.op, C.byref(v), C.byref(a), C.byref(b))
+            out[k] = v.value, a.value, b.value
+    return out
+
+
+def run_pair(oracle, t, c, sets, opts_kw, passes=(), check_meas=True):
+    """Engine lanes vs oracle lanes; returns the engine's recording."""
+    N = len(next(iter(sets.values()))) if sets else 1
+    gl = E.Lanes(t, N)
+    ol = O.Lanes(c, N)
+    for (kind, inst, k), vals in sets.items():
+        gl.set_param(kind, inst, k, vals)
+        ol.set_param(kind, inst, k, vals)
+    gl.prepare()
+    ol.prepare()
+    go = E.tran_opts(**opts_kw, max_steps=MAX_STEPS, record_capacity=CAP)
+    res, meas, times, rows, ms = gl.tran(go, passes)
+    oo = O.tran_opts(**opts_kw, max_steps=MAX_STEPS)
+    say(f"  kernel {ms:.3f} ms, points {res['points'].min()}..{res['points'].max()}, "
+        f"attempts {res['attempts'].min()}..{res['attempts'].max()}")
+    for l in range(N):
+        tt, xx, st = ol.tran(l, oo)
+        k = int(res["points"][l])
+        assert st.status == res["status"][l], l
+        assert st.points == k, (l, st.points, k)
+        assert k <= CAP
+        assert (st.steps, st.attempts, st.rej_newton, st.rej_lte, st.bp_landings, st.nr_iters) == \
+            tuple(int(res[f][l]) for f in ("steps", "attempts", "rej_newton", "rej_lte", "bp_landings", "nr_iters")), l
+        assert bits_equal(times[l, :k], tt), l
+        assert bits_equal(rows[l, :k], xx), l
+        if passes and check_meas:
+            assert bits_equal(meas[l], oracle_meas(oracle, passes, tt, xx)), (l, meas[l])
+    return res, meas, times, rows
+
+
+def test_measure_sp_lanes_bitwise(oracle, engine):
+    N = 64
+    say(f"case: measure.sp RC, {N} lanes (lane 0 nominal, R and C varied), tran 10n 10u, 11 .meas passes")
+    t, c = build_both(oracle, W.rc_pulse)
+    rng = np.random.default_rng(11)
+    r = rng.uniform(500.0, 2000.0, N)
+    cap = rng.uniform(0.5e-9, 2e-9, N)
+    r[0], cap[0] = 1e3, 1e-9
+    passes = measure_passes(t)
+    res, meas, times, _ = run_pair(oracle, t, c, {(E.RESISTOR, 0, 0): r, (E.CAPACITOR, 0, 0): cap},
+                                   dict(t_stop=10e-6, dt_init=10e-9), passes)
+    assert (res["status"] == 0).all()
+    # the reference's printed results on the nominal lane
+    m = meas[0, :, 0]
+    assert res["points"][0] == 1029
+    printed(m[1] - m[0], 2.197201e-06, 6)
+    printed(m[1], 3.303064e-06, 6)
+    printed(m[0], 1.105863e-06, 6)
+    printed(m[3] - m[2], 6.931367e-07, 6)
+    printed(m[3], 1.693637e-06, 6)
+    printed(m[2], 1.000500e-06, 6)
+    printed((m[1] - m[0]) / (m[3] - m[2]), 3.169938e+00, 6)
+    printed(m[4], 8.645984e-01, 6)
+    printed(m[5], 6.68790e-06, 5)
+    printed(m[6], 9.932657e-01, 6)
+    printed(meas[0, 6, 1], 6.001000e-06, 6)
+    printed(m[7], 4.982781e-01, 6)
+    printed(m[8], 6.33054e-01, 5)
+    printed(m[9], 3.685512e+05, 6)
+    printed(m[10], -9.93246e-10, 5)
+    assert tuple(meas[0, 10, 1:]) == (1e-6, 6e-6)
+
+
+def test_rc_lowpass_lanes_bitwise(oracle, engine):
+    N = 8
+    say(f"case: rc_lowpass.sp, {N} lanes, tran 10u 5m")
+    t, c = build_both(oracle, W.rc_pulse, c=159e-9, pulse=W.RC_LOWPASS_PULSE, dc=0.0)
+    r = 1e3 * (1 + 0.05 * np.arange(N))
+    res, _, _, _ = run_pair(oracle, t, c, {(E.RESISTOR, 0, 0): r}, dict(t_stop=5e-3, dt_init=10e-6))
+    assert res["points"][0] == 519
+    assert (res["status"] == 0).all()
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_rc_ladder_methods_bitwise(oracle, engine, method):
+    N = 32
+    say(f"case: 4-stage RC ladder, {N} lanes, method {method}, tran 10n 10u")
+    t, c = build_both(oracle, W.rc_ladder, stages=4)
+    rng = np.random.default_rng(100 + method)
+    sets = {(E.RESISTOR, k, 0): rng.uniform(300.0, 3000.0, N) for k in range(4)}
+    sets[(E.CAPACITOR, 3, 0)] = rng.uniform(0.2e-9, 3e-9, N)
+    passes = [E.meas_cross(t.node("n4"), 0.5, rise=1), E.meas_cross(t.node("n4"), 0.5, fall=E.LAST),
+              E.meas_extremum(t.node("n2"), 0), E.meas_integ(t.node("n3"), rms=True)]
+    res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=10e-6, dt_init=10e-9, method=method), passes)
+    assert (res["status"] == 0).all()
+
+
+def test_tran_many_lanes_bitwise_sample(oracle, engine):
+    """4096 lanes in one launch; a sample of them checked against the
+    oracle (the rest only for completion)."""
+    N = 4096
+    say(f"case: measure.sp RC, {N} lanes, sampled parity")
+    t, c = build_both(oracle, W.rc_pulse)
+    rng = np.random.default_rng(7)
+    r = rng.uniform(500.0, 2000.0, N)
+    cap = rng.uniform(0.5e-9, 2e-9, N)
+    gl = E.Lanes(t, N)
+    gl.set_param(E.RESISTOR, 0, 0, r)
+    gl.set_param(E.CAPACITOR, 0, 0, cap)
+    gl.prepare()
+    passes = measure_passes(t)
+    res, meas, times, rows, ms = gl.tran(E.tran_opts(10e-6, 10e-9, max_steps=MAX_STEPS, record_capacity=0), passes)
+    say(f"  kernel {ms:.3f} ms for {N} lanes")
+    assert times is None and (res["status"] == 0).all()
+    sample = [0, 1, 2047, 4095]
+    ol = O.Lanes(c, len(sample))
+    ol.set_param(O.RES, 0, 0, r[sample])
+    ol.set_param(O.CAP, 0, 0, cap[sample])
+    ol.prepare()
+    for j, l in enumerate(sample):
+        tt, xx, st = ol.tran(j, O.tran_opts(10e-6, 10e-9, max_steps=MAX_STEPS))
+        assert st.points == res["points"][l]
+        assert bits_equal(meas[l], oracle_meas(oracle, passes, tt, xx)), l
+
+
+def test_tran_refuses_uncharged_models(engine):
+    say("case: diode template refused by the transient lanes")
+    t = E.Template()
+    t.add(E.VSOURCE, ["a", "0"], {0: 1.0})
+    t.add(E.DIODE, ["a", "0"], {0: 1e-14})
+    t.freeze()
+    gl = E.Lanes(t, 2)
+    gl.prepare()
+    with pytest.raises(E.EngineError, match="TranChargesUnsupported"):
+        gl.tran(E.tran_opts(1e-6, 1e-8))
