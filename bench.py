@@ -102,6 +102,93 @@ def copy_calibration(mib):
                       "bytes_written_per_dispatch": mib << 20}), flush=True)
 
 
+def tran_lanes(N, seed=11):
+    """The measure.sp RC deck on N lanes: R ~ U(500, 2000) ohm and C ~
+    U(0.5, 2) nF per lane (lane 0 nominal), its ten .meas lines as passes."""
+    import egspice_amd as E
+    from egspice_amd import workloads as W
+    t = E.Template()
+    W.rc_pulse(t.add)
+    t.freeze()
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(500.0, 2000.0, N)
+    c = rng.uniform(0.5e-9, 2e-9, N)
+    r[0], c[0] = 1e3, 1e-9
+    vin, vout = t.node("in"), t.node("out")
+    passes = [E.meas_cross(vout, 0.1, rise=1), E.meas_cross(vout, 0.9, rise=1), E.meas_cross(vin, 0.5, rise=1),
+              E.meas_cross(vout, 0.5, rise=1), E.meas_at(vout, 3e-6), E.meas_cross(vout, 0.5, fall=1),
+              E.meas_extremum(vout, 1), E.meas_extremum(vout, 2, frm=0.0, to=10e-6),
+              E.meas_integ(vout, rms=True, frm=0.0, to=10e-6), E.meas_at(vout, 2e-6, deriv=True),
+              E.meas_integ(2, frm=1e-6, to=6e-6)]
+    return t, r, c, passes
+
+
+def bench_tran(args):
+    """Transient lanes/s: one step = every lane's whole `.tran 10n 10u`
+    (operating point, ~1029 accepted points, the .meas passes) in one launch."""
+    import egspice_amd as E
+    N = args.lanes
+    t, r, c, passes = tran_lanes(N)
+    lanes = E.Lanes(t, N)
+    lanes.set_param(E.RESISTOR, 0, 0, r)
+    lanes.set_param(E.CAPACITOR, 0, 0, c)
+    lanes.prepare()
+    opts = E.tran_opts(10e-6, 10e-9, max_steps=200_000)
+
+    def step():
+        return lanes.tran(opts, passes)
+
+    for _ in range(args.warmup):
+        step()
+    kms, done, t0 = [], 0, time.perf_counter()
+    for _ in range(args.steps):
+        res, meas, _, _, ms = step()
+        kms.append(ms)
+        done += int(((res["status"] == 0) & np.all(np.isfinite(meas[:, :, 0]), axis=1)).sum())
+    wall = time.perf_counter() - t0
+    attempts = float(res["attempts"].mean())
+    line = {
+        "metric": "converged transient lanes/sec (every .meas resolved)", "value": done / wall, "unit": "lanes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "docs/examples/measure.sp RC, .tran 10n 10u, 11 .meas passes, R and C per lane",
+                   "lanes": N, "unknowns": t.n, "points_per_lane": float(res["points"].mean()),
+                   "attempts_per_lane": attempts, "newton_ticks_per_lane": float(res["nr_iters"].mean()),
+                   "kernel_ms": float(np.mean(kms))},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_tran(N, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_tran(n_lanes, budget_s=10.0):
+    """The oracle's transient (tran.Sim restated) over the same lanes, one
+    lane per call on one thread, for about budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from egspice_amd import workloads as W
+    lib = O.load()
+    c = O.Circuit(lib)
+    W.rc_pulse(c.add)
+    c.freeze()
+    _, r, cc, _ = tran_lanes(n_lanes)
+    L = O.Lanes(c, n_lanes)
+    L.set_param(O.RES, 0, 0, r)
+    L.set_param(O.CAP, 0, 0, cc)
+    L.prepare()
+    opts = O.tran_opts(10e-6, 10e-9, max_steps=200_000)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        _, _, st = L.tran(done % n_lanes, opts)
+        done += 1 if st.status == 0 else 0
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": done / el, "unit": "lanes/s", "cores": 1, "kind": "port",
+            "sample": f"{done} lane transients in {el:.1f} s (waveform recorded, .meas not included), "
+                      f"oracle CPU path on 1 thread, {platform.processor() or platform.machine()}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +197,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("dc-inverter-chain", "tran-rc-measure"), default="dc-inverter-chain",
+                    help="dc-inverter-chain: the 64-point DC sweep (default); tran-rc-measure: the RC transient of "
+                         "docs/examples/measure.sp with its .meas passes, R and C varied per lane")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling profiler byte counters) and exit")
@@ -121,6 +211,10 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(lanes shard across GPUs by contiguous lane ranges, a design not built here)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "tran-rc-measure":
+        bench_tran(args)
+        return
 
     import egspice_amd as E
     from egspice_amd import workloads as W

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     __syncthreads();
     if (DENSE) {
         const WaveHook off{};
-        const int r = wave_newton(W, s, D, rc, o, off, o.init_fix != 0, lim_active, false, 0.0, it);
+        const int r = wave_newton<false, false>(W, s, D, rc, o, off, o.init_fix != 0, lim_active, 0.0, it);
         st = r == NEWTON_CONVERGED ? 1u : r == NEWTON_SINGULAR ? 2u : 0u;
     } else {
         uint32_t bits = o.init_fix ? G_INITFIX : 0;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     TranStep st{};
     uint32_t it = 0;
     const WaveHook off{};
-    const int op = wave_newton(W, s, D, rc, op_opts, off, true, true, true, 0.0, it);
+    const int op = wave_newton<true, false>(W, s, D, rc, op_opts, off, true, true, 0.0, it);
     if (op != NEWTON_CONVERGED) {
         if (tid == 0) {
             st.out.status = op == NEWTON_SINGULAR ? 3u : 2u;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     const bool has_charge = n_tape > 0;
     if (has_charge) {
         // q(op) seeds a flat history
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true, true, 0.0);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, true, 0.0);
         __syncthreads();
         for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
         __syncthreads();
@@ -310,10 +310,10 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         const int method = (int)scal[3];
         const uint32_t p0 = (uint32_t)scal[8], p1 = (uint32_t)scal[9], p2 = (uint32_t)scal[10];
         for (uint32_t i = tid; i < n; i += nt) s.x[i] = cur[i] + xfact * (cur[i] - prev[i]);
-        WaveHook h{true, method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
-        const int r = wave_newton(W, s, D, rc, step_opts, h, false, true, true, t_dev, it);
+        WaveHook h{method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
+        const int r = wave_newton<true, true>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
         if (r == NEWTON_CONVERGED && has_charge) {
-            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true, true, t_dev);
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, true, t_dev);
             __syncthreads();
             for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
             __syncthreads();

@@ -236,9 +236,8 @@ struct LP {
 // dynamically and push them to scratch memory).
 
 // Phase: evaluate instance e into its staging cells.
-template <int KIND>
-ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active,
-                        bool tran, double t)
+template <int KIND, bool TRAN>
+ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active, double t)
 {
     constexpr int nu = KIND_NU[KIND];
     constexpr unsigned limw = KIND_LIMW[KIND];
@@ -254,7 +253,7 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
             if (limw >> u & 1) lx[u] = s.lim[ie.lim + u];
     }
     DevOut o;
-    eval_kind(KIND, p, lx, o, tran, t);
+    eval_kind(KIND, p, lx, o, TRAN, t);
     if (use_lim) {
         double corr[nu];
         bool live = false;
@@ -285,7 +284,7 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
     bool charged = false;
 #pragma unroll
     for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(KIND, ru) != 0;
-    if (charged && tran) {
+    if (TRAN && charged) {
         uint32_t kc = 0;
 #pragma unroll
         for (int ru = 0; ru < nu; ru++) {
@@ -299,17 +298,18 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
     }
 }
 
-ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, bool tran = false,
-                           double t = 0.0)
+// TRAN: sources read their waveform at t and charges are staged too.
+template <bool TRAN = false>
+ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t = 0.0)
 {
     const InstEntry ie = W.inst[e];
     switch (ie.kind) {
-    case K_VSRC: wave_eval_k<K_VSRC>(W, s, ie, e, lim_active, tran, t); break;
-    case K_ISRC: wave_eval_k<K_ISRC>(W, s, ie, e, lim_active, tran, t); break;
-    case K_RES: wave_eval_k<K_RES>(W, s, ie, e, lim_active, tran, t); break;
-    case K_CAP: wave_eval_k<K_CAP>(W, s, ie, e, lim_active, tran, t); break;
-    case K_DIODE: wave_eval_k<K_DIODE>(W, s, ie, e, lim_active, tran, t); break;
-    case K_MOS1: wave_eval_k<K_MOS1>(W, s, ie, e, lim_active, tran, t); break;
+    case K_VSRC: wave_eval_k<K_VSRC, TRAN>(W, s, ie, e, lim_active, t); break;
+    case K_ISRC: wave_eval_k<K_ISRC, TRAN>(W, s, ie, e, lim_active, t); break;
+    case K_RES: wave_eval_k<K_RES, TRAN>(W, s, ie, e, lim_active, t); break;
+    case K_CAP: wave_eval_k<K_CAP, TRAN>(W, s, ie, e, lim_active, t); break;
+    case K_DIODE: wave_eval_k<K_DIODE, TRAN>(W, s, ie, e, lim_active, t); break;
+    case K_MOS1: wave_eval_k<K_MOS1, TRAN>(W, s, ie, e, lim_active, t); break;
     default: break;
     }
 }

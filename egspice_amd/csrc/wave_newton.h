@@ -12,7 +12,6 @@ namespace esp {
 // residual rhs += ag0 (q - q1) [- ag1 i_prev | - ag2 (q1 - q2)] on every
 // row and the matrix a = G + ag0 C, added after the ground stamp.
 struct WaveHook {
-    bool on;
     int method;  // 0 BE, 1 trapezoidal, 2 gear-2
     double ag0, ag1, ag2;
     const double *q1, *q2, *i_prev;
@@ -22,11 +21,15 @@ struct WaveHook {
 enum { NEWTON_SINGULAR = -1, NEWTON_FAILED = 0, NEWTON_CONVERGED = 1 };
 
 // Solves from s.x; a converged lane returns x_k (x = x_old) and the
-// iteration count. `tran`: sources read their waveform at t (a transient's
-// operating point and its steps). All threads; uniform result.
-__device__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D, const ReduceCache &rc, const Newton &o,
-                           const WaveHook &h, bool init_fix, bool lim_active, bool tran, double t, uint32_t &iters)
+// iteration count. TRAN: sources read their waveform at t (a transient's
+// operating point and its steps); HOOK: the integrator's companion applies
+// (h). Both are compile-time so the static solve carries none of it. All
+// threads; uniform result.
+template <bool TRAN, bool HOOK>
+__device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D, const ReduceCache &rc, const Newton &o,
+                           const WaveHook &h, bool init_fix, bool lim_active, double t, uint32_t &iters)
 {
+    static_assert(TRAN || !HOOK, "the companion only runs in a transient");
     const uint32_t tid = threadIdx.x, nt = blockDim.x, n = W.P.n, nnz = W.P.nnz;
     uint32_t bits = init_fix ? G_INITFIX : 0;
     double limit = 1e9, scaled = 0;
@@ -34,14 +37,14 @@ __device__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D,
     for (uint32_t i = tid; i < n; i += nt) s.x_old[i] = s.x[i];
     __syncthreads();
     for (uint32_t tick = 0; tick < o.max_iter; tick++) {
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active, tran, t);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<TRAN>(W, s, e, lim_active, t);
         __syncthreads();
         wave_reduce(W, s, rc, tid, nt);
-        if (h.on)
+        if constexpr (HOOK)
             for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
         __syncthreads();
-        double *vals = s.g;
-        if (h.on) {
+        double *const vals = HOOK ? h.a : s.g;
+        if constexpr (HOOK) {
             if (tid == 0) {
                 s.g[W.diag[0]] += 1.0;
                 s.rhs[0] += s.x[0];
@@ -56,13 +59,12 @@ __device__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D,
             }
             for (uint32_t p = tid; p <= nnz; p += nt) h.a[p] = s.g[p] + h.ag0 * (p < nnz ? s.cc[p] : 0.0);
             __syncthreads();
-            vals = h.a;
             if (tid == 0 && tick == 0) {
                 LV<double> x{s.x, 1}, rhs{s.rhs, 1}, a{vals, 1};
                 limit = plausible_limit(n, x, rhs, a, W.diag);
             }
-        } else if (tid == 0) {
-            wave_assemble(W, s, o, tick, limit);
+        } else {
+            if (tid == 0) wave_assemble(W, s, o, tick, limit);
         }
         const int fl = wave_direct_factor(W.P, s.lu, D, vals, &s.flags[2]);
         if (fl == 0) {
