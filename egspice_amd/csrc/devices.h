@@ -415,6 +415,33 @@ ESP_HD double pulse_break(const P &p, double t_acc, double tq)
     return bp > tq ? bp : inf_();
 }
 
+// The fire time after t of `@(timer(start, period))` (LRM 5.10.5): start +
+// k period for the first k strictly after t.
+ESP_HD double timer_next(double start, double period, double t)
+{
+    if (start > t) return start;
+    const double k = __builtin_floor((t - start) / period) + 1.0;
+    const double next = start + k * period;
+    return next > t ? next : start + (k + 1.0) * period;
+}
+
+// vsource's own nextBreakpoint, the fallback of srcBreak when ngspice's
+// rounding names no corner (src/device/eval.zig:1592-1598): the earliest of
+// the four PULSE corner timers of models/vsource.va:137-140 after t, for
+// every waveform (a non-PULSE source parks TD at 1e30).
+template <class P>
+ESP_HD double pulse_timer_break(const P &p, double t)
+{
+    double tr, tf, pw, per;
+    pulse_widths(p, tr, tf, pw, per);
+    const double td = p[SRCP_TD];
+    const double c[4] = {timer_next(td, per, t), timer_next(td + tr, per, t), timer_next(td + tr + pw, per, t),
+                         timer_next(td + tr + pw + tf, per, t)};
+    double best = c[0];
+    for (int k = 1; k < 4; k++) best = c[k] < best ? c[k] : best;
+    return best;
+}
+
 // ---------------------------------------------------------------- eval
 
 ESP_HD Dual jcur(Dual v, double isat, double vtx, double gm)

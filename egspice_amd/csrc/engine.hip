@@ -832,6 +832,33 @@ static_assert(sizeof(espice_meas_pass) == sizeof(MeasPass), "espice_meas_pass mi
 static_assert(sizeof(espice_meas_result) == sizeof(MeasOut), "espice_meas_result mirrors MeasOut");
 static_assert(sizeof(espice_tran_result) == sizeof(TranOut), "espice_tran_result mirrors TranOut");
 
+// Whether every diode and MOSFET of every lane has zero charge: diode.va
+// charges only with CJO or TT (:334,363); mos1.va only with an oxide (TOX
+// given and > 0, :164), overlaps CGSO/CGDO/CGBO, or junction capacitance
+// (CBD/CBS, or CJ and CJSW with nonzero AD/AS/PD/PS, :253-255).
+static bool tran_charges_zero(const espice_lanes &L)
+{
+    auto zero = [](double v) { return v != v || v == 0.0; };
+    auto none = [](double v) { return v != v || v <= 0.0; };
+    const size_t N = L.N;
+    for (uint32_t i = 0; i < L.t->count[K_DIODE]; i++)
+        for (size_t l = 0; l < N; l++) {
+            const double *u = &L.uprm[K_DIODE][(size_t)i * KIND_UK[K_DIODE] * N + l];
+            if (!zero(u[2 * N]) || !zero(u[6 * N])) return false;  // CJO, TT
+        }
+    for (uint32_t i = 0; i < L.t->count[K_MOS1]; i++)
+        for (size_t l = 0; l < N; l++) {
+            const double *u = &L.uprm[K_MOS1][(size_t)i * KIND_UK[K_MOS1] * N + l];
+            auto at = [&](int k) { return u[(size_t)k * N]; };
+            if (!none(at(6))) return false;                                      // TOX
+            if (!zero(at(22)) || !zero(at(23)) || !zero(at(24))) return false;  // CGSO CGDO CGBO
+            if (!none(at(14)) || !none(at(15))) return false;                   // CBD CBS
+            if (!zero(at(16)) && !(zero(at(31)) && zero(at(32)))) return false;  // CJ with AD/AS
+            if (!zero(at(17)) && !(zero(at(33)) && zero(at(34)))) return false;  // CJSW with PD/PS
+        }
+    return true;
+}
+
 espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, const espice_meas_pass *passes,
                                 uint32_t n_passes, espice_tran_result *out, espice_meas_result *meas, double *times,
                                 double *rows, double *kernel_ms)
@@ -845,9 +872,11 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
         return fail(ESPICE_INVALID_ARGUMENT, "InvalidTran");
     for (uint32_t k = 0; k < n_passes; k++)
         if (passes[k].kind > 3 || passes[k].row >= n) return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
-    // The charge models of the junction and Meyer capacitances (diode.va,
-    // mos1.va) are not staged yet: refuse rather than run them as static.
-    if (L->t->count[K_DIODE] || L->t->count[K_MOS1]) return fail(ESPICE_FAILED, "TranChargesUnsupported");
+    // The junction and Meyer charges of diode.va and mos1.va are not staged
+    // yet: a lane whose diodes or MOSFETs would carry charge is refused
+    // rather than run as static; with every such parameter zero (or not
+    // given) their charges are identically zero and the lane is exact.
+    if (!tran_charges_zero(*L)) return fail(ESPICE_FAILED, "TranChargesUnsupported");
     WaveLayout lay{};
     const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
     const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));

@@ -73,3 +73,27 @@ def rc_ladder(add, stages=3, r=1e3, c=1e-9, pulse=MEASURE_PULSE):
     for k in range(1, stages + 1):
         add(RESISTOR, [prev, f"n{k}"], {0: r})
         prev = f"n{k}"
+
+
+# docs/examples/cmos_inverter.sp: .model nch NMOS(LEVEL=1 VTO=0.45 KP=200u
+# LAMBDA=0.05 GAMMA=0.4 PHI=0.8), pch the same with VTO=-0.45 KP=80u.
+INV_NCH = {M["TYPE"]: 1.0, M["VTO"]: 0.45, M["KP"]: 200e-6, M["LAMBDA"]: 0.05, M["GAMMA"]: 0.4, M["PHI"]: 0.8}
+INV_PCH = {M["TYPE"]: -1.0, M["VTO"]: -0.45, M["KP"]: 80e-6, M["LAMBDA"]: 0.05, M["GAMMA"]: 0.4, M["PHI"]: 0.8}
+
+
+def cmos_inverter(add, vdd=1.8, wn=1e-6, cl=10e-15):
+    """docs/examples/cmos_inverter.sp in builder order (v, c, m):
+
+        VDD vdd 0 {vdd}
+        VIN in 0 DC 0 PULSE(0 {vdd} 1n 100p 100p 5n 10n)
+        CL out 0 10f
+        M1 out in 0 0 nch W={wn} L=180n
+        M2 out in vdd vdd pch W={2*wn} L=180n
+
+    Returns the instance indices (vin, cl, m1, m2) per kind."""
+    add(VSOURCE, ["vdd", "0"], {VSOURCE_DC: vdd})
+    vin = add(VSOURCE, ["in", "0"], pulse_params(0.0, vdd, 1e-9, 100e-12, 100e-12, 5e-9, 10e-9, dc=0.0))
+    c = add(CAPACITOR, ["out", "0"], {0: cl})
+    m1 = add(MOS1, ["out", "in", "0", "0"], {**INV_NCH, M["W"]: wn, M["L"]: 180e-9})
+    m2 = add(MOS1, ["out", "in", "vdd", "vdd"], {**INV_PCH, M["W"]: 2 * wn, M["L"]: 180e-9})
+    return vin, c, m1, m2

@@ -561,6 +561,35 @@ double or_source_value(const double *p, double t)
 /* src/device/eval.zig:1636-1650 spicePulseBreak: the corner after tq,
  * rounded from the accepted time t_acc; inf when none (not PULSE, or the
  * rounded time is not after tq). */
+/* The fire time after t of `@(timer(start, period))` (LRM 5.10.5): start
+ * + k period for the first k that lands strictly after t. VerA generates
+ * vsource's nextBreakpoint from the four PULSE corner timers of
+ * models/vsource.va:137-140; its rounding of start + k period is not in the
+ * reference tree (parity pinned through the point counts of the decks
+ * whose second period needs it, docs/learn/models.md:95-102). */
+static double timer_next(double start, double period, double t)
+{
+    if (start > t) return start;
+    const double k = floor((t - start) / period) + 1.0;
+    double next = start + k * period;
+    if (!(next > t)) next = start + (k + 1.0) * period;
+    return next;
+}
+
+/* vsource's nextBreakpoint: the earliest of the four PULSE corner timers
+ * after t, whatever the waveform (a non-PULSE source parks TD at 1e30). */
+double or_pulse_timer_break(const double *p, double t)
+{
+    double tr, tf, pw, per;
+    pulse_widths(p, &tr, &tf, &pw, &per);
+    const double td = p[SRCP_TD];
+    double best = timer_next(td, per, t);
+    best = fmin(best, timer_next(td + tr, per, t));
+    best = fmin(best, timer_next(td + tr + pw, per, t));
+    best = fmin(best, timer_next(td + tr + pw + tf, per, t));
+    return best;
+}
+
 double or_pulse_break(const double *p, double t_acc, double tq)
 {
     if (p[SRCP_WAVE] != 1.0) return INFINITY;

@@ -43,6 +43,7 @@ double or_source_value(const double *p, double t);
 /* ngspice's PULSE breakpoint after tq, requested at the accepted time
  * t_acc (src/device/eval.zig:1636-1650 spicePulseBreak); inf when none. */
 double or_pulse_break(const double *p, double t_acc, double tq);
+double or_pulse_timer_break(const double *p, double t);
 
 /* What one evaluation adds beyond the static planes: the transient's time
  * (sources read their waveform), and the charge outputs. */

@@ -214,13 +214,43 @@ def test_tran_many_lanes_bitwise_sample(oracle, engine):
         assert bits_equal(meas[l], oracle_meas(oracle, passes, tt, xx)), l
 
 
-def test_tran_refuses_uncharged_models(engine):
-    say("case: diode template refused by the transient lanes")
+def test_cmos_inverter_lanes_bitwise(oracle, engine):
+    """docs/examples/cmos_inverter.sp (mos1, no oxide or junction charge):
+    lane 0 nominal must give the reference's 2032 points and tphl; the other
+    lanes vary VDD, the NMOS threshold and CL."""
+    N = 48
+    say(f"case: cmos_inverter.sp mos1, {N} lanes, tran 10p 20n")
+    t, c = build_both(oracle, W.cmos_inverter)
+    rng = np.random.default_rng(5)
+    vdd = rng.uniform(1.6, 2.0, N)
+    vto = 0.45 + 0.015 * rng.standard_normal(N)
+    cl = rng.uniform(5e-15, 20e-15, N)
+    vdd[0], vto[0], cl[0] = 1.8, 0.45, 10e-15
+    M = E.MOS1_PARAMS
+    passes = [E.meas_cross(t.node("in"), 0.9, rise=1), E.meas_cross(t.node("out"), 0.9, fall=1)]
+    res, meas, _, _ = run_pair(oracle, t, c, {(E.VSOURCE, 0, 0): vdd, (E.MOS1, 0, M["VTO"]): vto,
+                                              (E.CAPACITOR, 0, 0): cl}, dict(t_stop=20e-9, dt_init=10e-12), passes)
+    assert (res["status"] == 0).all()
+    assert res["points"][0] == 2032
+    trig, targ = meas[0, 0, 0], meas[0, 1, 0]
+    printed(targ - trig, 2.818778e-11, 6)
+    printed(targ, 1.078188e-09, 6)
+    printed(trig, 1.050000e-09, 6)
+
+
+def test_tran_refuses_charged_models(engine):
+    say("case: a diode with junction charge is refused by the transient lanes")
     t = E.Template()
     t.add(E.VSOURCE, ["a", "0"], {0: 1.0})
-    t.add(E.DIODE, ["a", "0"], {0: 1e-14})
+    t.add(E.RESISTOR, ["a", "b"], {0: 1e3})
+    t.add(E.DIODE, ["b", "0"], {0: 1e-14})
     t.freeze()
     gl = E.Lanes(t, 2)
+    gl.set_param(E.DIODE, 0, E.DIODE_PARAMS["CJO"], [0.0, 1e-12])
     gl.prepare()
     with pytest.raises(E.EngineError, match="TranChargesUnsupported"):
         gl.tran(E.tran_opts(1e-6, 1e-8))
+    gl.set_param(E.DIODE, 0, E.DIODE_PARAMS["CJO"], [0.0, 0.0])
+    gl.prepare()
+    res, _, _, _, _ = gl.tran(E.tran_opts(1e-6, 1e-8))
+    assert (res["status"] == 0).all()

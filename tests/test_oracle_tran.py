@@ -3,6 +3,9 @@ reference's printed outputs:
   - docs/examples/measure.sp (docs/learn/measure.md:24-36): 1029 points
     and ten .meas results to their printed digits;
   - docs/examples/rc_lowpass.sp (docs/learn/analyses.md:25-30): 519 points;
+  - docs/examples/cmos_inverter.sp (docs/learn/models.md:95-102, mos1):
+    2032 points and tphl, trig and targ to their printed digits (its second
+    period's corners come from vsource's own timers);
 and by mathematics: the RC step response 1 - exp(-t/RC) within the LTE
 tolerance, and the PULSE waveform and its breakpoints at known corners."""
 import ctypes as C
@@ -132,3 +135,25 @@ def test_pulse_waveform_and_breakpoints(oracle):
     # every corner of both periods is a timepoint
     for corner in (1e-6, 1.001e-6, 6.001e-6, 6.003e-6, 21e-6, 21.001e-6, 26.001e-6, 26.003e-6):
         assert np.min(np.abs(t - corner)) <= 1e-18 * 10 + 1e-21
+
+
+def test_cmos_inverter_golden(oracle):
+    """docs/learn/models.md:95-102: `.tran 10p 20n` of the mos1 inverter,
+    2032 points; tphl = 2.818778e-11, targ 1.078188e-09, trig 1.050000e-09."""
+    from egspice_amd import workloads as W
+    c = O.Circuit(oracle)
+    W.cmos_inverter(c.add)
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(20e-9, 10e-12))
+    assert st.status == 0 and st.points == 2032 == len(t)
+    assert st.bp_landings == 8  # every corner of both periods
+    tt = np.ascontiguousarray(t)
+    vin = np.ascontiguousarray(x[:, c.node("in")])
+    vout = np.ascontiguousarray(x[:, c.node("out")])
+    trig = oracle.or_meas_cross(P(tt), P(vin), len(tt), C.byref(O.clause(val=0.9, rise=1)))
+    targ = oracle.or_meas_cross(P(tt), P(vout), len(tt), C.byref(O.clause(val=0.9, fall=1)))
+    printed(targ - trig, 2.818778e-11, 6)
+    printed(targ, 1.078188e-09, 6)
+    printed(trig, 1.050000e-09, 6)
