@@ -301,6 +301,166 @@ class Lanes:
         return res, m, times.reshape(N, cap), rows.reshape(N, cap, n), ms.value
 
 
+# ---- the points API (include/espice.h)
+
+ESPICE_FILE, ESPICE_BYTES = 0, 1
+ESPICE_NGSPICE, ESPICE_HSPICE, ESPICE_SPECTRE = 0, 1, 2
+ESPICE_CPU, ESPICE_AUTO, ESPICE_CUDA, ESPICE_HIP = 0, 1, 2, 3
+REASONS = {0: "OK", 1: "NO_CONVERGENCE", 2: "TIMESTEP_TOO_SMALL", 3: "SINGULAR_MATRIX", 4: "MEASURE_NOT_FOUND",
+           5: "MEASURE_UNRESOLVED", 6: "ABORTED", 7: "MODEL_BIN_NOT_FOUND", 8: "TOPOLOGY_CHANGED",
+           9: "INVALID_PARAMETER", 10: "DEVICE_REFUSED", 255: "OTHER"}
+
+
+class EspiceBytes(C.Structure):
+    _fields_ = [("data", C.c_char_p), ("len", C.c_size_t)]
+
+
+def _bytes(b):
+    b = b.encode() if isinstance(b, str) else b
+    return EspiceBytes(b, len(b))
+
+
+class CreateOptions(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("abi_version", "struct_size", "source_kind", "dialect", "backend",
+                                          "explicit_gpu", "output_format", "max_parallel")] + \
+               [(f, EspiceBytes) for f in ("source", "origin", "output_path", "zig_path", "share_path")] + \
+               [("live_params", C.POINTER(EspiceBytes)), ("live_param_count", C.c_size_t),
+                ("display", C.c_void_p), ("display_user", C.c_void_p)]
+
+
+class PointsOptions(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t), ("abort_measure", EspiceBytes), ("abort_after", C.c_double),
+                ("measures_only", C.c_uint8), ("early_stop", C.c_uint8), ("warm_start", C.c_uint8),
+                ("tangent_step", C.c_double)]
+
+
+def _points_lib():
+    lib = library()
+    if getattr(lib, "_points_bound", False):
+        return lib
+    P = C.c_void_p
+    sig = {
+        "espice_abi_version": (C.c_uint32, []),
+        "espice_default_options": (None, [C.POINTER(CreateOptions)]),
+        "espice_create": (C.c_uint32, [C.POINTER(CreateOptions), C.POINTER(P), C.c_char_p, C.c_size_t]),
+        "espice_destroy": (C.c_uint32, [P]),
+        "espice_query_count": (C.c_uint32, [P, C.POINTER(C.c_uint32)]),
+        "espice_run_all": (C.c_uint32, [P]),
+        "espice_error_message": (C.c_uint32, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "espice_append_points": (C.c_uint32, [P, C.POINTER(C.c_double), C.c_size_t, C.POINTER(PointsOptions),
+                                              C.POINTER(C.c_uint32)]),
+        "espice_copy_point_measures": (C.c_uint32, [P, C.c_uint32, C.c_size_t, C.POINTER(EspiceBytes), C.c_size_t,
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+        "espice_release_points": (C.c_uint32, [P, C.c_uint32, C.c_size_t]),
+        "espice_lane_width": (C.c_uint32, [P, C.POINTER(C.c_uint32)]),
+        "espice_cli_report": (C.c_uint32, [P, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "espice_spice_number": (C.c_int, [C.c_char_p, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._points_bound = True
+    return lib
+
+
+def spice_number(text):
+    """ngspice's INPevaluate of one number with its scale suffix, or None."""
+    v = C.c_double()
+    return v.value if _points_lib().espice_spice_number(text.encode(), C.byref(v)) else None
+
+
+class PointsError(EngineError):
+    def __init__(self, status, name):
+        super().__init__(f"espice status {status}: {name}")
+        self.status = status
+        self.name = name
+
+
+def create_options(text=None, path=None, live=(), backend=ESPICE_HIP, dialect=ESPICE_NGSPICE):
+    o = CreateOptions()
+    _points_lib().espice_default_options(C.byref(o))
+    o.backend = backend
+    o.explicit_gpu = 1 if backend == ESPICE_HIP else 0
+    o.dialect = dialect
+    if path is not None:
+        o.source_kind, o.source = ESPICE_FILE, _bytes(path)
+    else:
+        o.source_kind, o.source = ESPICE_BYTES, _bytes(text)
+    names = (EspiceBytes * max(len(live), 1))(*[_bytes(n) for n in live])
+    o.live_params = C.cast(names, C.POINTER(EspiceBytes))
+    o.live_param_count = len(live)
+    o._keep = (names, text, path)
+    return o
+
+
+class Problem:
+    """One deck on the lane engine through the points API (include/espice.h)."""
+
+    def __init__(self, text=None, path=None, live=(), backend=ESPICE_HIP, options=None):
+        self.lib = _points_lib()
+        self.h = C.c_void_p()
+        o = options or create_options(text, path, live, backend)
+        diag = C.create_string_buffer(256)
+        st = self.lib.espice_create(C.byref(o), C.byref(self.h), diag, 256)
+        if st != OK:
+            self.h = None
+            raise PointsError(st, diag.value.decode())
+        self.live = list(live)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.espice_destroy(self.h)
+            self.h = None
+
+    def _check(self, st):
+        if st != OK:
+            buf = C.create_string_buffer(256)
+            req = C.c_size_t()
+            self.lib.espice_error_message(self.h, buf, 256, C.byref(req))
+            raise PointsError(st, buf.value.decode())
+
+    def query_count(self):
+        n = C.c_uint32()
+        self._check(self.lib.espice_query_count(self.h, C.byref(n)))
+        return n.value
+
+    def append_points(self, rows, options=None):
+        rows = np.ascontiguousarray(np.atleast_2d(np.asarray(rows, dtype=np.float64)))
+        first = C.c_uint32()
+        self._check(self.lib.espice_append_points(self.h, rows.ctypes.data_as(C.POINTER(C.c_double)), rows.shape[0],
+                                                  C.byref(options) if options is not None else None,
+                                                  C.byref(first)))
+        return first.value
+
+    def run_all(self):
+        self._check(self.lib.espice_run_all(self.h))
+
+    def measures(self, first_id, count, names):
+        out = np.zeros((count, len(names)))
+        reasons = np.zeros(count, dtype=np.uint32)
+        nb = (EspiceBytes * max(len(names), 1))(*[_bytes(n) for n in names])
+        self._check(self.lib.espice_copy_point_measures(self.h, first_id, count, nb, len(names),
+                                                        out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                        reasons.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out, reasons
+
+    def release(self, first_id, count):
+        self._check(self.lib.espice_release_points(self.h, first_id, count))
+
+    def lane_width(self):
+        n = C.c_uint32()
+        self._check(self.lib.espice_lane_width(self.h, C.byref(n)))
+        return n.value
+
+    def report(self, json=False):
+        req = C.c_size_t()
+        self.lib.espice_cli_report(self.h, 1 if json else 0, None, 0, C.byref(req))
+        buf = C.create_string_buffer(req.value)
+        self._check(self.lib.espice_cli_report(self.h, 1 if json else 0, buf, req.value, C.byref(req)))
+        return buf.value.decode()
+
+
 def tab_build(n, col_ptr, row_idx, vals):
     """LaneTab and index words of one host factor (ownTape/buildTab)."""
     lib = library()

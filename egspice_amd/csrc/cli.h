@@ -1,0 +1,20 @@
+// Library entry points for the command line front end and its tests; not
+// part of the reference's API (include/espice.h).
+#pragma once
+
+#include "../../include/espice.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Point 0's .meas lines as the reference prints them (json 0), or its
+ * measures as a JSON object (json 1); buffer protocol as espice_copy_*. */
+espice_status espice_cli_report(espice_problem *problem, int json, char *buf, size_t cap, size_t *required);
+/* ngspice's INPevaluate of one number with its scale suffix; 1 when `text`
+ * is a number, else 0. */
+int espice_spice_number(const char *text, double *out);
+
+#ifdef __cplusplus
+}
+#endif

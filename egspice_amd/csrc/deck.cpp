@@ -1,0 +1,952 @@
+// The deck subset reader and its builder onto a lane template (deck.h).
+#include "deck.h"
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "../../include/espice_lanes.h"
+
+namespace esp {
+
+// ------------------------------------------------------------- numbers
+
+// 10^e as glibc's pow(10.0, e) gives it: correctly rounded for every e but
+// 23 and 210, which it rounds up; 0 below 1e-330, inf past 1e308
+// (src/frontend/lines.zig:250-263).
+static double pow10_glibc(int e)
+{
+    static double table[308 + 330 + 1];
+    static bool init = false;
+    if (!init) {
+        char buf[32];
+        for (int k = -330; k <= 308; k++) {
+            std::snprintf(buf, sizeof buf, "1e%d", k);
+            table[k + 330] = std::strtod(buf, nullptr);
+        }
+        table[23 + 330] = 1.0000000000000001e23;
+        table[210 + 330] = 1.0000000000000001e210;
+        init = true;
+    }
+    if (e < -330) return 0.0;
+    if (e > 308) return HUGE_VAL;
+    return table[e + 330];
+}
+
+// Length of the leading decimal literal (numberEnd, lines.zig:150-178,
+// with `d` as an exponent letter); 0 when there is none.
+static size_t number_end(const std::string &t, size_t at)
+{
+    size_t end = at;
+    bool digit = false, dot = false, exp = false;
+    while (end < t.size()) {
+        const char c = t[end];
+        if (c >= '0' && c <= '9') {
+            digit = true;
+        } else if (c == '.' && !dot && !exp) {
+            dot = true;
+        } else if ((c == 'e' || c == 'd') && !exp && digit && end + 1 < t.size() &&
+                   (std::isdigit((unsigned char)t[end + 1]) || t[end + 1] == '-' || t[end + 1] == '+')) {
+            exp = true;
+            end++;
+            if (!std::isdigit((unsigned char)t[end])) {
+                if (end + 1 >= t.size() || !std::isdigit((unsigned char)t[end + 1])) return 0;
+            }
+        } else if ((c == '+' || c == '-') && end == at) {
+        } else {
+            break;
+        }
+        end++;
+    }
+    return digit ? end - at : 0;
+}
+
+// inpEvaluate (lines.zig:222-248): one f64 mantissa of every digit, then
+// sign * mantissa * 10^exponent.
+static double inp_evaluate(const std::string &lit, int scale_exp, bool mil)
+{
+    size_t i = 0;
+    double sign = 1;
+    if (lit[0] == '+' || lit[0] == '-') {
+        if (lit[0] == '-') sign = -1;
+        i = 1;
+    }
+    double mantis = 0;
+    int e = scale_exp;
+    while (i < lit.size() && std::isdigit((unsigned char)lit[i])) mantis = 10 * mantis + (double)(lit[i++] - '0');
+    if (i < lit.size() && lit[i] == '.') {
+        i++;
+        while (i < lit.size() && std::isdigit((unsigned char)lit[i])) {
+            mantis = 10 * mantis + (double)(lit[i++] - '0');
+            e -= 1;
+        }
+    }
+    if (i < lit.size()) {
+        i++;
+        int esign = 1;
+        if (lit[i] == '+' || lit[i] == '-') {
+            if (lit[i] == '-') esign = -1;
+            i++;
+        }
+        int x = 0;
+        while (i < lit.size()) {
+            x = 10 * x + (lit[i] - '0');
+            if (x > 100000) x = 100000;
+            i++;
+        }
+        e += esign * x;
+    }
+    if (mil) mantis *= 25.4;
+    if (mantis == 0) return sign * 0.0;
+    return sign * mantis * pow10_glibc(e);
+}
+
+// parseSpiceNum (lines.zig:182-213) from position `at`; `len` receives the
+// characters read (the literal and its suffix letters).
+static bool spice_number_at(const std::string &t, size_t at, double &out, size_t &len)
+{
+    const size_t n = number_end(t, at);
+    if (n == 0) return false;
+    size_t s = at + n;
+    int scale = 0;
+    bool mil = false;
+    if (t.compare(s, 3, "meg") == 0) scale = 6;
+    else if (t.compare(s, 3, "mil") == 0) scale = -6, mil = true;
+    else if (s < t.size()) {
+        switch (t[s]) {
+        case 't': scale = 12; break;
+        case 'g': scale = 9; break;
+        case 'k': scale = 3; break;
+        case 'm': scale = -3; break;
+        case 'u': scale = -6; break;
+        case 'n': scale = -9; break;
+        case 'p': scale = -12; break;
+        case 'f': scale = -15; break;
+        case 'a': scale = -18; break;
+        default: break;
+        }
+    }
+    out = inp_evaluate(t.substr(at, n), scale, mil);
+    size_t e = s;
+    while (e < t.size() && (std::isalpha((unsigned char)t[e]) || t[e] == '_')) e++;
+    len = e - at;
+    return true;
+}
+
+bool spice_number(const std::string &text, double &out)
+{
+    size_t len = 0;
+    return spice_number_at(text, 0, out, len) && len == text.size();
+}
+
+// ---------------------------------------------------------- expressions
+
+struct Expr {
+    enum Op { NUM, NAME, NEG, ADD, SUB, MUL, DIV, POW, CALL } op = NUM;
+    double num = 0;
+    std::string name;
+    std::vector<ExprP> args;
+};
+
+namespace {
+
+struct ExprParser {
+    const std::string &s;
+    size_t i = 0;
+    std::string err;
+
+    void ws()
+    {
+        while (i < s.size() && std::isspace((unsigned char)s[i])) i++;
+    }
+    ExprP make(Expr::Op op, std::vector<ExprP> args)
+    {
+        auto e = std::make_shared<Expr>();
+        e->op = op;
+        e->args = std::move(args);
+        return e;
+    }
+    ExprP expr()
+    {
+        ExprP l = term();
+        for (;;) {
+            ws();
+            if (!l || i >= s.size() || (s[i] != '+' && s[i] != '-')) return l;
+            const char c = s[i++];
+            ExprP r = term();
+            if (!r) return nullptr;
+            l = make(c == '+' ? Expr::ADD : Expr::SUB, {l, r});
+        }
+    }
+    ExprP term()
+    {
+        ExprP l = unary();
+        for (;;) {
+            ws();
+            if (!l || i >= s.size()) return l;
+            if (s[i] == '*' && !(i + 1 < s.size() && s[i + 1] == '*')) {
+                i++;
+                ExprP r = unary();
+                if (!r) return nullptr;
+                l = make(Expr::MUL, {l, r});
+            } else if (s[i] == '/') {
+                i++;
+                ExprP r = unary();
+                if (!r) return nullptr;
+                l = make(Expr::DIV, {l, r});
+            } else {
+                return l;
+            }
+        }
+    }
+    ExprP unary()
+    {
+        ws();
+        if (i < s.size() && (s[i] == '-' || s[i] == '+')) {
+            const char c = s[i++];
+            ExprP a = unary();
+            if (!a) return nullptr;
+            return c == '-' ? make(Expr::NEG, {a}) : a;
+        }
+        return power();
+    }
+    ExprP power()
+    {
+        ExprP b = primary();
+        ws();
+        if (!b) return nullptr;
+        if (i < s.size() && (s[i] == '^' || (s[i] == '*' && i + 1 < s.size() && s[i + 1] == '*'))) {
+            i += s[i] == '^' ? 1 : 2;
+            ExprP x = unary();
+            if (!x) return nullptr;
+            return make(Expr::POW, {b, x});
+        }
+        return b;
+    }
+    ExprP primary()
+    {
+        ws();
+        if (i >= s.size()) {
+            err = "ExpressionEnd";
+            return nullptr;
+        }
+        if (s[i] == '(') {
+            i++;
+            ExprP e = expr();
+            ws();
+            if (!e || i >= s.size() || s[i] != ')') {
+                if (err.empty()) err = "UnbalancedParenthesis";
+                return nullptr;
+            }
+            i++;
+            return e;
+        }
+        double v;
+        size_t len;
+        if ((std::isdigit((unsigned char)s[i]) || s[i] == '.') && spice_number_at(s, i, v, len)) {
+            i += len;
+            auto e = std::make_shared<Expr>();
+            e->op = Expr::NUM;
+            e->num = v;
+            return e;
+        }
+        if (std::isalpha((unsigned char)s[i]) || s[i] == '_') {
+            size_t b = i;
+            while (i < s.size() && (std::isalnum((unsigned char)s[i]) || s[i] == '_' || s[i] == '.')) i++;
+            auto e = std::make_shared<Expr>();
+            e->name = s.substr(b, i - b);
+            ws();
+            if (i < s.size() && s[i] == '(') {
+                i++;
+                e->op = Expr::CALL;
+                for (;;) {
+                    ExprP a = expr();
+                    if (!a) return nullptr;
+                    e->args.push_back(a);
+                    ws();
+                    if (i < s.size() && s[i] == ',') {
+                        i++;
+                        continue;
+                    }
+                    if (i < s.size() && s[i] == ')') {
+                        i++;
+                        break;
+                    }
+                    err = "UnbalancedParenthesis";
+                    return nullptr;
+                }
+            } else {
+                e->op = Expr::NAME;
+            }
+            return e;
+        }
+        err = "BadExpression";
+        return nullptr;
+    }
+};
+
+}  // namespace
+
+ExprP parse_expr(const std::string &text, std::string &err)
+{
+    ExprParser p{text};
+    ExprP e = p.expr();
+    p.ws();
+    if (e && p.i != text.size()) {
+        err = "BadExpression";
+        return nullptr;
+    }
+    if (!e) err = p.err.empty() ? "BadExpression" : p.err;
+    return e;
+}
+
+double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err)
+{
+    auto arg = [&](size_t k) { return eval_expr(*e.args[k], env, err); };
+    switch (e.op) {
+    case Expr::NUM: return e.num;
+    case Expr::NAME: {
+        auto it = env.find(e.name);
+        if (it == env.end()) {
+            err = "UnknownParameter " + e.name;
+            return NAN;
+        }
+        return it->second;
+    }
+    case Expr::NEG: return -arg(0);
+    case Expr::ADD: return arg(0) + arg(1);
+    case Expr::SUB: return arg(0) - arg(1);
+    case Expr::MUL: return arg(0) * arg(1);
+    case Expr::DIV: return arg(0) / arg(1);
+    case Expr::POW: return std::pow(arg(0), arg(1));
+    case Expr::CALL: {
+        const std::string &f = e.name;
+        const size_t n = e.args.size();
+        if (n == 1) {
+            const double x = arg(0);
+            if (f == "sqrt") return std::sqrt(x);
+            if (f == "exp") return std::exp(x);
+            if (f == "log" || f == "ln") return std::log(x);
+            if (f == "log10") return std::log10(x);
+            if (f == "abs") return std::fabs(x);
+            if (f == "sin") return std::sin(x);
+            if (f == "cos") return std::cos(x);
+            if (f == "tan") return std::tan(x);
+            if (f == "atan") return std::atan(x);
+            if (f == "sinh") return std::sinh(x);
+            if (f == "cosh") return std::cosh(x);
+            if (f == "tanh") return std::tanh(x);
+            if (f == "floor") return std::floor(x);
+            if (f == "ceil") return std::ceil(x);
+        } else if (n == 2) {
+            const double x = arg(0), y = arg(1);
+            if (f == "min") return x < y ? x : y;
+            if (f == "max") return x > y ? x : y;
+            if (f == "pow" || f == "pwr") return std::pow(x, y);
+        }
+        err = "UnknownFunction " + f;
+        return NAN;
+    }
+    }
+    return NAN;
+}
+
+void expr_names(const Expr &e, std::vector<std::string> &out)
+{
+    if (e.op == Expr::NAME) out.push_back(e.name);
+    for (auto &a : e.args) expr_names(*a, out);
+}
+
+// ------------------------------------------------------------ the cards
+
+namespace {
+
+// A card's tokens: words, and `(`, `)`, `=` on their own; {...} and '...'
+// stay whole (their inner text, marked by the leading brace or quote).
+std::vector<std::string> tokens(const std::string &line)
+{
+    std::vector<std::string> out;
+    size_t i = 0;
+    while (i < line.size()) {
+        const char c = line[i];
+        if (std::isspace((unsigned char)c) || c == ',') {
+            i++;
+        } else if (c == '(' || c == ')' || c == '=') {
+            out.emplace_back(1, c);
+            i++;
+        } else if (c == '{' || c == '\'' || c == '"') {
+            const char close = c == '{' ? '}' : c;
+            size_t j = line.find(close, i + 1);
+            if (j == std::string::npos) j = line.size();
+            out.push_back(std::string(1, '{') + line.substr(i + 1, j - i - 1));
+            i = j + 1;
+        } else {
+            size_t j = i;
+            while (j < line.size() && !std::isspace((unsigned char)line[j]) && line[j] != '(' && line[j] != ')' &&
+                   line[j] != '=' && line[j] != ',' && line[j] != '{' && line[j] != '\'')
+                j++;
+            out.push_back(line.substr(i, j - i));
+            i = j;
+        }
+    }
+    return out;
+}
+
+bool to_value(const std::string &tok, Value &v, std::string &err)
+{
+    v.given = true;
+    if (!tok.empty() && tok[0] == '{') {
+        v.expr = parse_expr(tok.substr(1), err);
+        return v.expr != nullptr;
+    }
+    if (spice_number(tok, v.num)) return true;
+    v.expr = parse_expr(tok, err);
+    return v.expr != nullptr;
+}
+
+struct Cursor {
+    const std::vector<std::string> &t;
+    size_t i;
+    bool more() const { return i < t.size(); }
+    const std::string &peek() const { return t[i]; }
+    bool next_is(const char *s) const { return i < t.size() && t[i] == s; }
+};
+
+// k = v pairs to the end of the card (parentheses skipped).
+bool kv_pairs(Cursor &c, std::map<std::string, Value> &kv, std::string &err)
+{
+    while (c.more()) {
+        if (c.peek() == "(" || c.peek() == ")") {
+            c.i++;
+            continue;
+        }
+        const std::string key = c.peek();
+        if (c.i + 2 >= c.t.size() + 1 || !(c.i + 1 < c.t.size() && c.t[c.i + 1] == "=") || c.i + 2 >= c.t.size()) {
+            err = "ExpectedKeyValue " + key;
+            return false;
+        }
+        Value v;
+        if (!to_value(c.t[c.i + 2], v, err)) return false;
+        kv[key] = v;
+        c.i += 3;
+    }
+    return true;
+}
+
+// v(node) or i(name) at the cursor.
+bool vector_at(Cursor &c, std::string &vec, std::string &err)
+{
+    if (c.i + 3 < c.t.size() + 0 && (c.peek() == "v" || c.peek() == "i") && c.t[c.i + 1] == "(" && c.t[c.i + 3] == ")") {
+        vec = c.peek() + "(" + c.t[c.i + 2] + ")";
+        c.i += 4;
+        return true;
+    }
+    err = "UnsupportedVector";
+    return false;
+}
+
+int count_of(const std::string &tok)
+{
+    if (tok == "last") return -2;
+    double v;
+    if (!spice_number(tok, v) || v < 1 || v != std::floor(v)) return -3;
+    return (int)v;
+}
+
+// TRIG/TARG/WHEN side conditions: VAL=, TD=, RISE=, FALL=, CROSS=, FROM=, TO=.
+bool side_conditions(Cursor &c, Meas::Side &s, std::string &err, const char *stop)
+{
+    while (c.more() && !(stop && c.peek() == stop)) {
+        const std::string key = c.peek();
+        if (key == "when") {
+            err = "UnsupportedMeasure FIND WHEN";
+            return false;
+        }
+        if (!(c.i + 2 < c.t.size() && c.t[c.i + 1] == "=")) {
+            err = "BadMeasure " + key;
+            return false;
+        }
+        const std::string &val = c.t[c.i + 2];
+        c.i += 3;
+        if (key == "val") {
+            if (!to_value(val, s.val, err)) return false;
+        } else if (key == "td") {
+            if (!to_value(val, s.td, err)) return false;
+        } else if (key == "from") {
+            if (!to_value(val, s.from, err)) return false;
+        } else if (key == "to") {
+            if (!to_value(val, s.to, err)) return false;
+        } else if (key == "at") {
+            if (!to_value(val, s.at, err)) return false;
+        } else if (key == "rise" || key == "fall" || key == "cross") {
+            const int k = count_of(val);
+            if (k == -3) {
+                err = "BadMeasureCount";
+                return false;
+            }
+            (key == "rise" ? s.rise : key == "fall" ? s.fall : s.cross) = k;
+        } else {
+            err = "UnsupportedMeasureOption " + key;
+            return false;
+        }
+    }
+    return true;
+}
+
+bool read_meas(const std::vector<std::string> &t, Meas &m, std::string &err)
+{
+    if (t.size() < 4) {
+        err = "BadMeasure";
+        return false;
+    }
+    m.analysis = t[1];
+    m.name = t[2];
+    if (m.analysis != "tran" && m.analysis != "op") {
+        err = "UnsupportedMeasureAnalysis " + m.analysis;
+        return false;
+    }
+    Cursor c{t, 3};
+    const std::string f = c.peek();
+    c.i++;
+    if (f == "trig") {
+        m.kind = "trig_targ";
+        if (!vector_at(c, m.a.vec, err) || !side_conditions(c, m.a, err, "targ")) return false;
+        if (!c.next_is("targ")) {
+            err = "MeasureWithoutTarg";
+            return false;
+        }
+        c.i++;
+        return vector_at(c, m.b.vec, err) && side_conditions(c, m.b, err, nullptr);
+    }
+    if (f == "when") {
+        m.kind = "when";
+        if (!vector_at(c, m.a.vec, err)) return false;
+        if (!(c.next_is("=") && c.i + 1 < t.size())) {
+            err = "BadWhen";
+            return false;
+        }
+        if (!to_value(t[c.i + 1], m.a.val, err)) return false;
+        c.i += 2;
+        return side_conditions(c, m.a, err, nullptr);
+    }
+    if (f == "find" || f == "deriv" || f == "derivative") {
+        m.kind = f == "find" ? "find" : "deriv";
+        if (!vector_at(c, m.a.vec, err) || !side_conditions(c, m.a, err, nullptr)) return false;
+        if (m.analysis == "tran" && !m.a.at.given) {
+            err = "UnsupportedMeasure " + f;
+            return false;
+        }
+        return true;
+    }
+    if (f == "min" || f == "max" || f == "avg" || f == "rms" || f == "integ" || f == "integral") {
+        m.kind = f == "integral" ? "integ" : f;
+        return vector_at(c, m.a.vec, err) && side_conditions(c, m.a, err, nullptr);
+    }
+    if (f == "param") {
+        m.kind = "param";
+        if (!(c.next_is("=") && c.i + 1 < t.size())) {
+            err = "BadParamMeasure";
+            return false;
+        }
+        const std::string &x = t[c.i + 1];
+        m.param = parse_expr(x[0] == '{' ? x.substr(1) : x, err);
+        return m.param != nullptr;
+    }
+    err = "UnsupportedMeasure " + f;
+    return false;
+}
+
+bool read_element(const std::vector<std::string> &t, Element &e, std::string &err)
+{
+    e.name = t[0];
+    e.letter = t[0][0];
+    size_t ports = 0;
+    switch (e.letter) {
+    case 'r':
+    case 'c':
+    case 'v':
+    case 'i':
+    case 'd': ports = 2; break;
+    case 'm': ports = 4; break;
+    default: err = "UnsupportedElement " + e.name; return false;
+    }
+    if (t.size() < 1 + ports) {
+        err = "MissingNodes " + e.name;
+        return false;
+    }
+    for (size_t k = 0; k < ports; k++) e.nodes.push_back(t[1 + k]);
+    Cursor c{t, 1 + ports};
+    if (e.letter == 'r' || e.letter == 'c') {
+        if (!c.more() || !to_value(c.peek(), e.value, err)) {
+            if (err.empty()) err = "MissingValue " + e.name;
+            return false;
+        }
+        c.i++;
+        if (c.more()) {
+            err = "UnsupportedOption " + e.name;
+            return false;
+        }
+        return true;
+    }
+    if (e.letter == 'd' || e.letter == 'm') {
+        if (!c.more()) {
+            err = "MissingModel " + e.name;
+            return false;
+        }
+        e.model = c.peek();
+        c.i++;
+        if (e.letter == 'd' && c.more() && !(c.i + 1 < t.size() && t[c.i + 1] == "=")) {
+            Value area;
+            if (!to_value(c.peek(), area, err)) return false;
+            e.kv["area"] = area;
+            c.i++;
+        }
+        if (!kv_pairs(c, e.kv, err)) return false;
+        return true;
+    }
+    // V and I: [dc] value, ac ..., pulse(...)
+    bool have_value = false;
+    while (c.more()) {
+        const std::string w = c.peek();
+        if (w == "dc") {
+            if (c.i + 1 >= t.size() || !to_value(t[c.i + 1], e.value, err)) {
+                if (err.empty()) err = "MissingValue " + e.name;
+                return false;
+            }
+            have_value = true;
+            c.i += 2;
+        } else if (w == "ac") {
+            c.i++;
+            double x;
+            for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
+        } else if (w == "pulse" && e.letter == 'v') {
+            c.i++;
+            if (!c.next_is("(")) {
+                err = "BadPulse " + e.name;
+                return false;
+            }
+            c.i++;
+            int k = 0;
+            while (c.more() && !c.next_is(")")) {
+                if (k >= 7 || !to_value(c.peek(), e.pulse_args[k], err)) {
+                    if (err.empty()) err = "BadPulse " + e.name;
+                    return false;
+                }
+                k++;
+                c.i++;
+            }
+            if (!c.next_is(")") || k < 2) {
+                err = "BadPulse " + e.name;
+                return false;
+            }
+            c.i++;
+            e.pulse = true;
+        } else if (w == "sin" || w == "pwl" || w == "exp" || w == "sffm" || w == "am" || w == "pulse") {
+            err = "UnsupportedWaveform " + w;
+            return false;
+        } else if (!have_value) {
+            if (!to_value(w, e.value, err)) return false;
+            have_value = true;
+            c.i++;
+        } else {
+            err = "UnsupportedOption " + e.name;
+            return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+bool read_deck(const std::string &text, Deck &d, std::string &err)
+{
+    // fold case, split lines, join '+' continuations, drop comments
+    std::string low(text);
+    for (char &ch : low) ch = (char)std::tolower((unsigned char)ch);
+    std::vector<std::pair<int, std::string>> lines;
+    size_t at = 0;
+    int no = 0;
+    bool first = true;
+    while (at <= low.size()) {
+        size_t e = low.find('\n', at);
+        if (e == std::string::npos) e = low.size();
+        std::string ln = low.substr(at, e - at);
+        at = e + 1;
+        no++;
+        if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+        if (first) {
+            d.title = text.substr(0, text.find('\n') == std::string::npos ? text.size() : text.find('\n'));
+            if (!d.title.empty() && d.title.back() == '\r') d.title.pop_back();
+            first = false;
+            continue;
+        }
+        const size_t semi = ln.find(';');
+        if (semi != std::string::npos) ln.resize(semi);
+        for (size_t k = 1; k < ln.size(); k++)
+            if (ln[k] == '$' && std::isspace((unsigned char)ln[k - 1])) {
+                ln.resize(k);
+                break;
+            }
+        size_t b = ln.find_first_not_of(" \t");
+        if (b == std::string::npos || ln[b] == '*') continue;
+        ln = ln.substr(b);
+        if (ln[0] == '+') {
+            if (lines.empty()) {
+                err = "ContinuationFirst";
+                return false;
+            }
+            lines.back().second += " " + ln.substr(1);
+            continue;
+        }
+        lines.emplace_back(no, ln);
+        if (e == low.size()) break;
+    }
+    for (auto &card : lines) {
+        const int line_no = card.first;
+        const std::vector<std::string> t = tokens(card.second);
+        if (t.empty()) continue;
+        const std::string &w = t[0];
+        auto fail_at = [&](const std::string &e) {
+            err = e + " (line " + std::to_string(line_no) + ")";
+            return false;
+        };
+        if (w[0] != '.') {
+            Element e;
+            e.line = line_no;
+            if (!read_element(t, e, err)) return fail_at(err);
+            d.elements.push_back(std::move(e));
+        } else if (w == ".end") {
+            break;
+        } else if (w == ".param") {
+            Cursor c{t, 1};
+            while (c.more()) {
+                if (!(c.i + 2 < t.size() && t[c.i + 1] == "=")) return fail_at("BadParam");
+                Value v;
+                if (!to_value(t[c.i + 2], v, err)) return fail_at(err);
+                d.params.emplace_back(t[c.i], v);
+                c.i += 3;
+            }
+        } else if (w == ".model") {
+            if (t.size() < 3) return fail_at("BadModel");
+            Model m;
+            m.name = t[1];
+            m.type = t[2];
+            m.line = line_no;
+            Cursor c{t, 3};
+            if (!kv_pairs(c, m.kv, err)) return fail_at(err);
+            if (m.type != "nmos" && m.type != "pmos" && m.type != "d") return fail_at("UnsupportedModel " + m.type);
+            if (m.type != "d") {
+                auto lv = m.kv.find("level");
+                if (lv != m.kv.end() && !(lv->second.expr == nullptr && lv->second.num == 1.0))
+                    return fail_at("UnsupportedModelLevel");
+                m.kv.erase("level");
+            }
+            d.models[m.name] = m;
+        } else if (w == ".op") {
+            d.op = true;
+        } else if (w == ".tran") {
+            Cursor c{t, 1};
+            Value *dst[4] = {&d.tstep, &d.tstop, &d.tstart, &d.tmax};
+            int k = 0;
+            while (c.more()) {
+                if (c.peek() == "uic") return fail_at("UicUnsupported");
+                if (k >= 4 || !to_value(c.peek(), *dst[k], err)) return fail_at(err.empty() ? "BadTran" : err);
+                k++;
+                c.i++;
+            }
+            if (k < 2) return fail_at("BadTran");
+            if (d.tstart.given && (d.tstart.expr || d.tstart.num != 0)) return fail_at("TranStartUnsupported");
+            d.tran = true;
+        } else if (w == ".meas" || w == ".measure") {
+            Meas m;
+            m.line = line_no;
+            if (!read_meas(t, m, err)) return fail_at(err);
+            d.meas.push_back(std::move(m));
+        } else if (w == ".ac" || w == ".dc" || w == ".noise" || w == ".print" || w == ".plot" || w == ".save" ||
+                   w == ".probe" || w == ".width" || w == ".title") {
+            d.ignored.push_back(w);
+        } else {
+            return fail_at("UnsupportedCard " + w);
+        }
+    }
+    return true;
+}
+
+bool param_env(const Deck &d, const std::map<std::string, double> &live, std::map<std::string, double> &env,
+               std::string &err)
+{
+    env.clear();
+    for (auto &[name, v] : d.params) {
+        auto it = live.find(name);
+        if (it != live.end()) {
+            env[name] = it->second;
+            continue;
+        }
+        env[name] = v.expr ? eval_expr(*v.expr, env, err) : v.num;
+        if (!err.empty()) return false;
+    }
+    for (auto &[name, v] : live)
+        if (!env.count(name)) {
+            err = "NotAParameter " + name;
+            return false;
+        }
+    return true;
+}
+
+// ------------------------------------------------------------ the build
+
+namespace {
+
+// Engine parameter indices of the deck's keys (include/espice_lanes.h).
+const char *const MOS1_KEYS[] = {"type", "vto", "kp", "gamma", "phi", "lambda", "tox", "ld", "u0", "nsub", "nss",
+                                 "tpg", "is", "js", "cbd", "cbs", "cj", "cjsw", "pb", "mj", "mjsw", "fc", "cgso",
+                                 "cgdo", "cgbo", "tnom", "gmin", "temp", "dtemp", "l", "w", "ad", "as", "pd", "ps", "m"};
+const char *const DIODE_KEYS[] = {"is", "n", "cjo", "vj", "m", "fc", "tt", "gmin", "area", "tnom", "temp", "dtemp",
+                                  "eg", "xti"};
+
+int key_index(const char *const *keys, size_t n, const std::string &k)
+{
+    for (size_t i = 0; i < n; i++)
+        if (k == keys[i]) return (int)i;
+    return -1;
+}
+
+Value lit(double x)
+{
+    Value v;
+    v.given = true;
+    v.num = x;
+    return v;
+}
+
+}  // namespace
+
+bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
+{
+    Deck &d = bd.deck;
+    // NetBuilder.build's bucket order over the subset's letters
+    const char *order = "vlifhwkabcdegjmnopqrstuxyz";
+    bd.elem_kind.assign(d.elements.size(), -1);
+    bd.elem_inst.assign(d.elements.size(), 0);
+    for (const char *o = order; *o; o++)
+        for (size_t k = 0; k < d.elements.size(); k++)
+            if (d.elements[k].letter == *o) bd.add_order.push_back((uint32_t)k);
+    uint32_t next_row = 1;  // rows are numbered in order of allocation
+    for (uint32_t k : bd.add_order) {
+        Element &e = d.elements[k];
+        uint32_t rows[4];
+        for (size_t p = 0; p < e.nodes.size(); p++) {
+            const std::string &nn = e.nodes[p];
+            if (espice_lanes_template_node(t, nn.c_str(), &rows[p]) != ESPICE_OK) {
+                err = "BadNode " + nn;
+                return false;
+            }
+            if (nn != "0" && nn != "gnd") bd.rows[nn] = rows[p];
+            if (rows[p] + 1 > next_row) next_row = rows[p] + 1;
+        }
+        int kind;
+        std::vector<std::pair<uint32_t, Value>> prm;
+        switch (e.letter) {
+        case 'v': {
+            kind = ESPICE_LANES_VSOURCE;
+            if (e.pulse) {
+                prm.emplace_back(1, lit(1.0));
+                for (int a = 0; a < 7; a++)
+                    if (e.pulse_args[a].given) prm.emplace_back(2 + a, e.pulse_args[a]);
+                // .op biases at dc, which the builder fills from waveform(0)
+                // (models/vsource.va:216-221): v1 before TD and at a zero TD
+                prm.emplace_back(0, e.value.given ? e.value : e.pulse_args[0]);
+            } else {
+                prm.emplace_back(0, e.value.given ? e.value : lit(0.0));
+            }
+            break;
+        }
+        case 'i':
+            kind = ESPICE_LANES_ISOURCE;
+            prm.emplace_back(0, e.value.given ? e.value : lit(0.0));
+            break;
+        case 'r':
+            kind = ESPICE_LANES_RESISTOR;
+            prm.emplace_back(0, e.value);
+            break;
+        case 'c':
+            kind = ESPICE_LANES_CAPACITOR;
+            prm.emplace_back(0, e.value);
+            break;
+        case 'd':
+        case 'm': {
+            auto mi = d.models.find(e.model);
+            if (mi == d.models.end()) {
+                err = "UnknownModel " + e.model;
+                return false;
+            }
+            const Model &m = mi->second;
+            const bool mos = e.letter == 'm';
+            if (mos != (m.type != "d")) {
+                err = "ModelTypeMismatch " + e.name;
+                return false;
+            }
+            kind = mos ? ESPICE_LANES_MOS1 : ESPICE_LANES_DIODE;
+            const char *const *keys = mos ? MOS1_KEYS : DIODE_KEYS;
+            const size_t nk = mos ? 36 : 14;
+            if (mos) prm.emplace_back(0, lit(m.type == "nmos" ? 1.0 : -1.0));
+            for (auto &[k, v] : m.kv) {
+                const int idx = key_index(keys, nk, k);
+                if (idx < 0 || (mos && idx == 0)) {
+                    err = "UnsupportedModelParameter " + k;
+                    return false;
+                }
+                prm.emplace_back((uint32_t)idx, v);
+            }
+            for (auto &[k, v] : e.kv) {
+                const int idx = key_index(keys, nk, k);
+                if (idx < 0 || (mos && idx == 0)) {
+                    err = "UnsupportedInstanceParameter " + k;
+                    return false;
+                }
+                prm.emplace_back((uint32_t)idx, v);
+            }
+            break;
+        }
+        default: err = "UnsupportedElement " + e.name; return false;
+        }
+        // template constants: the literals now; expressions are point values
+        double pv[64];
+        const int npk[] = {9, 1, 1, 1, 14, 36};
+        for (int a = 0; a < npk[kind]; a++) pv[a] = NAN;
+        for (auto &[idx, v] : prm)
+            if (!v.expr) pv[idx] = v.num;
+        uint32_t inst = 0;
+        if (espice_lanes_template_add(t, (uint32_t)kind, rows, pv, (size_t)npk[kind], &inst) != ESPICE_OK) {
+            size_t req;
+            char buf[128];
+            espice_lanes_error_message(buf, sizeof buf, &req);
+            err = std::string(buf) + " " + e.name;
+            return false;
+        }
+        bd.elem_kind[k] = kind;
+        bd.elem_inst[k] = inst;
+        for (auto &[idx, v] : prm) bd.slots.push_back(ParamSlot{(uint32_t)kind, inst, idx, v});
+        // a V source's branch current takes the next row when it is added
+        if (e.letter == 'v') bd.vsrc_branch[e.name] = next_row++;
+    }
+    if (espice_lanes_template_freeze(t) != ESPICE_OK) {
+        size_t req;
+        char buf[128];
+        espice_lanes_error_message(buf, sizeof buf, &req);
+        err = buf;
+        return false;
+    }
+    uint32_t n = 0, nnz = 0, tri = 0;
+    espice_lanes_template_info(t, &n, &nnz, &tri);
+    bd.n = n;
+    if (n != next_row) {
+        err = "RowCount";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace esp

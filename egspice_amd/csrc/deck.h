@@ -1,0 +1,119 @@
+// A SPICE deck subset read into a lane template: the cards the points API
+// and the CLI run on the lane engine (include/espice.h lists them).
+//
+// Reading follows the reference front end where results depend on it:
+// numbers are ngspice's INPevaluate (src/frontend/lines.zig:150-262: one
+// f64 mantissa of every digit, times a correctly rounded power of ten);
+// text is case-folded; elements are added bucket by bucket in the letter
+// order of NetBuilder.build (src/frontend/builder.zig:886), each bucket in
+// card order, and nodes are numbered on first touch (rowOf :559-565), so
+// patterns, column orders and therefore every rounding match a deck built
+// by the reference; PULSE fields not given are resolved from .tran as
+// resolvePulseDefaults does (:845-877).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+struct espice_lanes_template;
+
+namespace esp {
+
+// ngspice's INPevaluate of a number with its scale suffix (`10meg`, `2.5u`,
+// `1mil`); false when `text` does not start with a number.
+bool spice_number(const std::string &text, double &out);
+
+// An expression over numbers, .param names and a few functions, parsed once
+// and evaluated per point.
+struct Expr;
+using ExprP = std::shared_ptr<const Expr>;
+ExprP parse_expr(const std::string &text, std::string &err);
+double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err);
+// Names an expression reads.
+void expr_names(const Expr &e, std::vector<std::string> &out);
+
+// A value as written on a card: a literal, or an expression.
+struct Value {
+    bool given = false;
+    double num = 0;
+    ExprP expr;  // null for a literal
+};
+
+struct Element {
+    char letter = 0;
+    std::string name;                 // full card name, folded
+    std::vector<std::string> nodes;   // port nets
+    std::string model;                // D and M cards
+    Value value;                      // R, C, I value; V DC value
+    bool pulse = false;               // V with PULSE(v1 v2 td tr tf pw per)
+    Value pulse_args[7];
+    std::map<std::string, Value> kv;  // W=, L=, AD=, M=, area
+    int line = 0;
+};
+
+struct Model {
+    std::string name, type;  // type: nmos, pmos, d
+    std::map<std::string, Value> kv;
+    int line = 0;
+};
+
+// One .meas card. kind: trig_targ, when, find, deriv, min, max, avg, rms,
+// integ, param. A vector is v(node) or i(vsource).
+struct Meas {
+    std::string analysis, name, kind;
+    struct Side {
+        std::string vec;
+        Value val, td, at, from, to;
+        int rise = -1, fall = -1, cross = -1;  // -2 LAST
+    } a, b;
+    ExprP param;
+    int line = 0;
+};
+
+struct Deck {
+    std::string title;
+    std::vector<std::pair<std::string, Value>> params;  // .param in card order
+    std::vector<Element> elements;                       // card order
+    std::map<std::string, Model> models;
+    bool op = false, tran = false;
+    Value tstep, tstop, tstart, tmax;
+    std::vector<Meas> meas;
+    std::vector<std::string> ignored;  // cards read and not run (.ac, .dc, .options, ...)
+};
+
+// Reads deck text; false with a named error (and its line) on anything
+// outside the subset.
+bool read_deck(const std::string &text, Deck &d, std::string &err);
+
+// ---- the deck built as a lane template
+
+// Device parameter slots of the lane engine that a point sets.
+struct ParamSlot {
+    uint32_t kind, inst, index;
+    Value value;
+};
+
+struct BuiltDeck {
+    Deck deck;
+    // card order of addition and, per element, its engine kind and instance
+    std::vector<uint32_t> add_order;
+    std::vector<int> elem_kind;
+    std::vector<uint32_t> elem_inst;
+    std::vector<ParamSlot> slots;        // every parameter given by the deck
+    std::map<std::string, uint32_t> rows;  // node name -> row
+    std::map<std::string, uint32_t> vsrc_branch;  // V card name -> branch row
+    uint32_t n = 0;
+};
+
+// Builds `bd.deck` into template `t` (empty, not frozen) and freezes it.
+bool build_deck(BuiltDeck &bd, struct ::espice_lanes_template *t, std::string &err);
+
+// The .param environment of one point: every .param in card order, with
+// `live` values taking the place of their cards' expressions.
+bool param_env(const Deck &d, const std::map<std::string, double> &live, std::map<std::string, double> &env,
+               std::string &err);
+
+}  // namespace esp

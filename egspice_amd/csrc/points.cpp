@@ -1,0 +1,728 @@
+// The points subset of include/espice.h on the lane engine.
+//
+// A Problem is one deck built into one lane template (deck.h). Its own
+// analyses are point 0 (the deck's .param values); espice_append_points
+// adds rows of live .param values. espice_run_all runs every pending point
+// in batches of at most the lane cap, one GPU lane each: the deck's .op as
+// a cold operating point (espice_lanes_newton_op) and its .tran as a whole
+// device transient with the .meas cards as device passes
+// (espice_lanes_tran). Query ids, handles, statuses and reasons follow
+// src/c_api.zig and src/espice.zig.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/espice.h"
+#include "../../include/espice_lanes.h"
+#include "cli.h"
+#include "deck.h"
+
+using esp::BuiltDeck;
+using esp::Meas;
+
+namespace {
+
+constexpr uint32_t LANE_CAP = 16384;  // the reference's gpu_lane_cap (docs/using/gpu.md:84-85)
+
+struct Point {
+    std::vector<double> live;  // live parameter values (point 0: the deck's)
+    uint32_t first_id = 0;
+    bool ran = false, released = false;
+    uint32_t reason = ESPICE_REASON_OK;
+    bool failed = false;       // no measure is read: NaN throughout
+    std::vector<double> meas;  // one per .meas card
+    std::vector<double> aux;   // two per card: the printed side values
+    uint32_t tran_points = 0;
+};
+
+// One .meas card as device passes.
+struct CardPasses {
+    int first = -1, second = -1;  // pass indices; -1 none
+    uint32_t op_row = 0;          // `.meas op find`
+};
+
+struct Problem {
+    BuiltDeck bd;
+    espice_lanes_template *tpl = nullptr;
+    std::vector<std::string> live;
+    std::vector<Point> points;
+    uint32_t per_point = 1;  // query ids per point
+    uint32_t queries = 0;
+    std::vector<espice_meas_pass> passes;
+    std::vector<CardPasses> cards;
+    std::string error;
+    ~Problem()
+    {
+        if (tpl) espice_lanes_template_destroy(tpl);
+    }
+};
+
+// ------------------------------------------------------- handle table
+// src/c_api.zig:185-268: a token is slot | generation << 12; slot 0 is never
+// handed out; destroy bumps the generation, so stale tokens stop matching.
+struct Handles {
+    static constexpr unsigned BITS = 12, CAP = 1u << BITS;
+    std::mutex mu;
+    Problem *live[CAP] = {};
+    uint64_t gen[CAP] = {};
+    uint32_t free_list[CAP];
+    uint32_t free_len = 0, fresh = 1;
+};
+Handles &handles()
+{
+    static Handles h;
+    return h;
+}
+constexpr uint64_t GEN_MAX = (uint64_t(1) << (sizeof(uintptr_t) * 8 - Handles::BITS)) - 1;
+
+espice_problem *insert(Problem *p)
+{
+    Handles &h = handles();
+    std::lock_guard<std::mutex> g(h.mu);
+    uint32_t idx;
+    if (h.free_len) idx = h.free_list[--h.free_len];
+    else if (h.fresh < Handles::CAP) idx = h.fresh++;
+    else return nullptr;
+    h.live[idx] = p;
+    return reinterpret_cast<espice_problem *>((uintptr_t)(h.gen[idx] << Handles::BITS | idx));
+}
+
+int find_slot(Handles &h, espice_problem *tok)
+{
+    const uintptr_t b = reinterpret_cast<uintptr_t>(tok);
+    const uint32_t idx = (uint32_t)(b & (Handles::CAP - 1));
+    if (!h.live[idx] || h.gen[idx] != (uint64_t)(b >> Handles::BITS)) return -1;
+    return (int)idx;
+}
+
+Problem *get(espice_problem *tok)
+{
+    Handles &h = handles();
+    std::lock_guard<std::mutex> g(h.mu);
+    const int i = find_slot(h, tok);
+    return i < 0 ? nullptr : h.live[i];
+}
+
+Problem *remove(espice_problem *tok)
+{
+    Handles &h = handles();
+    std::lock_guard<std::mutex> g(h.mu);
+    const int i = find_slot(h, tok);
+    if (i < 0) return nullptr;
+    Problem *p = h.live[i];
+    h.live[i] = nullptr;
+    if (h.gen[i] == GEN_MAX) return p;  // retired
+    h.gen[i]++;
+    h.free_list[h.free_len++] = (uint32_t)i;
+    return p;
+}
+
+espice_status fail(Problem *p, espice_status s, const std::string &e)
+{
+    if (p) p->error = e;
+    return s;
+}
+
+std::string fold(const char *data, size_t len)
+{
+    std::string s(data ? data : "", data ? len : 0);
+    for (char &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+bool gpu_present()
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+// The row a vector names: v(node) or i(vsource).
+bool vector_row(const BuiltDeck &bd, const std::string &vec, uint32_t &row)
+{
+    const std::string inner = vec.substr(2, vec.size() - 3);
+    if (vec[0] == 'v') {
+        if (inner == "0" || inner == "gnd") return false;
+        auto it = bd.rows.find(inner);
+        if (it == bd.rows.end()) return false;
+        row = it->second;
+        return true;
+    }
+    auto it = bd.vsrc_branch.find(inner);
+    if (it == bd.vsrc_branch.end()) return false;
+    row = it->second;
+    return true;
+}
+
+double val_of(const esp::Value &v, const std::map<std::string, double> &env, double dflt, std::string &err)
+{
+    if (!v.given) return dflt;
+    return v.expr ? esp::eval_expr(*v.expr, env, err) : v.num;
+}
+
+// The deck's .meas cards as device passes (their values are the deck's
+// literals or .param expressions, fixed per problem: a pass's val/at/from
+// may not depend on a live parameter).
+bool plan_passes(Problem &P, std::string &err)
+{
+    std::map<std::string, double> env;
+    if (!esp::param_env(P.bd.deck, {}, env, err)) return false;
+    for (const Meas &m : P.bd.deck.meas) {
+        CardPasses c;
+        auto side_pass = [&](const Meas::Side &s, uint32_t kind, uint32_t op) -> int {
+            espice_meas_pass q{};
+            if (!vector_row(P.bd, s.vec, q.row)) {
+                err = "NoSuchVector " + s.vec;
+                return -1;
+            }
+            q.kind = kind;
+            q.op = op;
+            q.rise = s.rise;
+            q.fall = s.fall;
+            q.cross = s.cross;
+            q.val = val_of(s.val, env, 1e99, err);
+            q.td = val_of(s.td, env, 0.0, err);
+            q.from = val_of(s.from, env, 0.0, err);
+            q.to = val_of(s.to, env, 0.0, err);
+            q.at = val_of(s.at, env, 0.0, err);
+            if (!err.empty()) return -1;
+            P.passes.push_back(q);
+            return (int)P.passes.size() - 1;
+        };
+        if (m.analysis == "op") {
+            if (m.kind != "find" || !vector_row(P.bd, m.a.vec, c.op_row)) {
+                err = "UnsupportedMeasure " + m.name;
+                return false;
+            }
+        } else if (m.kind == "trig_targ") {
+            if ((c.first = side_pass(m.a, 0, 0)) < 0 || (c.second = side_pass(m.b, 0, 0)) < 0) return false;
+        } else if (m.kind == "when") {
+            if ((c.first = side_pass(m.a, 0, 0)) < 0) return false;
+        } else if (m.kind == "find" || m.kind == "deriv") {
+            if ((c.first = side_pass(m.a, 1, m.kind == "deriv" ? 1 : 0)) < 0) return false;
+        } else if (m.kind == "min" || m.kind == "max" || m.kind == "avg") {
+            if ((c.first = side_pass(m.a, 2, m.kind == "min" ? 0 : m.kind == "max" ? 1 : 2)) < 0) return false;
+        } else if (m.kind == "rms" || m.kind == "integ") {
+            if ((c.first = side_pass(m.a, 3, m.kind == "rms" ? 1 : 0)) < 0) return false;
+        } else if (m.kind != "param") {
+            err = "UnsupportedMeasure " + m.name;
+            return false;
+        }
+        P.cards.push_back(c);
+    }
+    return true;
+}
+
+// Runs points [b, e) on one batch of lanes.
+espice_status run_batch(Problem &P, size_t b, size_t e)
+{
+    const esp::Deck &d = P.bd.deck;
+    const uint32_t N = (uint32_t)(e - b), n = P.bd.n;
+    espice_lanes *L = nullptr;
+    if (espice_lanes_create(P.tpl, N, &L) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "GpuUnavailable");
+    struct Guard {
+        espice_lanes *l;
+        ~Guard() { espice_lanes_destroy(l); }
+    } guard{L};
+    // per point .param environments, and every parameter slot per lane
+    std::vector<std::map<std::string, double>> envs(N);
+    std::vector<bool> bad(N, false);
+    for (uint32_t l = 0; l < N; l++) {
+        Point &pt = P.points[b + l];
+        std::map<std::string, double> live;
+        for (size_t k = 0; k < P.live.size(); k++) live[P.live[k]] = pt.live[k];
+        std::string err;
+        if (!esp::param_env(d, live, envs[l], err)) bad[l] = true;
+    }
+    std::vector<double> v(N);
+    for (const esp::ParamSlot &s : P.bd.slots) {
+        if (!s.value.expr) continue;
+        for (uint32_t l = 0; l < N; l++) {
+            std::string err;
+            v[l] = bad[l] ? NAN : esp::eval_expr(*s.value.expr, envs[l], err);
+            if (!err.empty() || !std::isfinite(v[l])) bad[l] = true;
+        }
+        if (espice_lanes_set_param(L, s.kind, s.inst, s.index, v.data()) != ESPICE_OK)
+            return fail(&P, ESPICE_FAILED, "SetParam");
+    }
+    if (espice_lanes_prepare(L) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Prepare");
+    const size_t nm = d.meas.size();
+    for (uint32_t l = 0; l < N; l++) {
+        Point &pt = P.points[b + l];
+        pt.meas.assign(nm, NAN);
+        pt.aux.assign(2 * nm, NAN);
+        pt.reason = bad[l] ? ESPICE_REASON_INVALID_PARAMETER : ESPICE_REASON_OK;
+        pt.failed = bad[l];
+    }
+    std::vector<double> xop;
+    bool have_op = false;
+    bool need_op = d.op;
+    for (const Meas &m : d.meas) need_op = need_op || m.analysis == "op";
+    if (d.op) {
+        std::vector<double> x0((size_t)n * N, 0.0);
+        if (espice_lanes_upload_x(L, x0.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Upload");
+        espice_newton_opts o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
+        std::vector<espice_lane_result> r(N);
+        double ms;
+        if (espice_lanes_newton_op(L, &o, r.data(), &ms) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "NewtonOp");
+        xop.resize((size_t)n * N);
+        if (espice_lanes_download_x(L, xop.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Download");
+        have_op = true;
+        for (uint32_t l = 0; l < N; l++) {
+            Point &pt = P.points[b + l];
+            if (pt.failed) continue;
+            if (r[l].status != 1) {
+                pt.failed = true;
+                pt.reason = r[l].status == 2 ? ESPICE_REASON_SINGULAR_MATRIX : ESPICE_REASON_NO_CONVERGENCE;
+            }
+        }
+    }
+    if (d.tran) {
+        std::map<std::string, double> env;
+        std::string err;
+        esp::param_env(d, {}, env, err);
+        espice_tran_opts o{};
+        o.t_stop = val_of(d.tstop, env, 0, err);
+        o.dt_init = val_of(d.tstep, env, 0, err);
+        o.dt_max = d.tmax.given ? val_of(d.tmax, env, 0, err) : std::fmin(o.dt_init, o.t_stop / 50);
+        o.dt_min = 1e-18;
+        o.xmu = 0.5;
+        o.reltol = 1e-3;
+        o.abstol = 1e-12;
+        o.vntol = 1e-6;
+        o.chgtol = 1e-14;
+        o.trtol = 7.0;
+        o.method = 1;
+        o.max_iter = 100;
+        o.max_steps = 1000000000u;
+        o.record_capacity = need_op && !have_op ? 1 : 0;
+        std::vector<espice_tran_result> tr(N);
+        std::vector<espice_meas_result> mr(P.passes.size() * N + 1);
+        std::vector<double> t0(o.record_capacity ? N : 1), x0(o.record_capacity ? (size_t)N * n : 1);
+        double ms;
+        const espice_status st = espice_lanes_tran(L, &o, P.passes.data(), (uint32_t)P.passes.size(), tr.data(),
+                                                   mr.data(), t0.data(), x0.data(), &ms);
+        if (st != ESPICE_OK) {
+            size_t req;
+            char buf[128];
+            espice_lanes_error_message(buf, sizeof buf, &req);
+            return fail(&P, st, std::string("Tran: ") + buf);
+        }
+        if (o.record_capacity) {
+            // the transient's operating point: row t = 0, laid out like x
+            xop.assign((size_t)n * N, NAN);
+            for (uint32_t l = 0; l < N; l++)
+                for (uint32_t i = 0; i < n; i++) xop[(size_t)i * N + l] = x0[(size_t)l * n + i];
+            have_op = true;
+        }
+        for (uint32_t l = 0; l < N; l++) {
+            Point &pt = P.points[b + l];
+            if (pt.failed) continue;
+            const uint32_t s = tr[l].status;
+            pt.tran_points = tr[l].points;
+            if (s != 0) {
+                pt.failed = true;
+                pt.reason = s == 1 ? ESPICE_REASON_TIMESTEP_TOO_SMALL
+                          : s == 3 ? ESPICE_REASON_SINGULAR_MATRIX
+                                   : ESPICE_REASON_NO_CONVERGENCE;
+                continue;
+            }
+            const espice_meas_result *m = &mr[(size_t)l * P.passes.size()];
+            for (size_t k = 0; k < nm; k++) {
+                const Meas &card = d.meas[k];
+                const CardPasses &c = P.cards[k];
+                if (card.analysis != "tran" || card.kind == "param") continue;
+                const espice_meas_result &a = m[c.first];
+                if (card.kind == "trig_targ") {
+                    const double trig = a.value, targ = m[c.second].value;
+                    pt.meas[k] = targ - trig;
+                    pt.aux[2 * k] = targ;
+                    pt.aux[2 * k + 1] = trig;
+                } else {
+                    pt.meas[k] = a.value;
+                    pt.aux[2 * k] = a.aux0;
+                    pt.aux[2 * k + 1] = a.aux1;
+                }
+            }
+        }
+    }
+    for (uint32_t l = 0; l < N; l++) {
+        Point &pt = P.points[b + l];
+        if (pt.failed) {
+            pt.meas.assign(nm, NAN);
+            pt.ran = true;
+            continue;
+        }
+        // .meas op, then PARAM cards over the other cards' results (measure.c
+        // order: PARAM cards read every other card)
+        std::map<std::string, double> env = envs[l];
+        for (size_t k = 0; k < nm; k++) {
+            const Meas &card = d.meas[k];
+            if (card.analysis == "op" && have_op) pt.meas[k] = xop[(size_t)P.cards[k].op_row * N + l];
+            if (card.kind != "param") env[card.name] = pt.meas[k];
+        }
+        for (size_t k = 0; k < nm; k++) {
+            const Meas &card = d.meas[k];
+            if (card.kind != "param") continue;
+            std::string err;
+            pt.meas[k] = esp::eval_expr(*card.param, env, err);
+            if (!err.empty()) pt.meas[k] = NAN;
+            env[card.name] = pt.meas[k];
+        }
+        for (size_t k = 0; k < nm; k++)
+            if (std::isnan(pt.meas[k]) && pt.reason == ESPICE_REASON_OK) pt.reason = ESPICE_REASON_MEASURE_UNRESOLVED;
+        pt.ran = true;
+    }
+    return ESPICE_OK;
+}
+
+// The point whose first query is `id`.
+int point_at(const Problem &P, uint32_t id)
+{
+    for (size_t i = 0; i < P.points.size(); i++)
+        if (P.points[i].first_id == id) return (int)i;
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t espice_abi_version(void) { return ESPICE_ABI_VERSION; }
+
+void espice_default_options(espice_create_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->abi_version = ESPICE_ABI_VERSION;
+    o->struct_size = sizeof *o;
+    o->max_parallel = 1;
+}
+
+espice_status espice_create(const espice_create_options *o, espice_problem **out, char *diag, size_t cap)
+{
+    auto say = [&](espice_status s, const std::string &e) {
+        if (diag && cap) {
+            const size_t k = e.size() < cap - 1 ? e.size() : cap - 1;
+            std::memcpy(diag, e.data(), k);
+            diag[k] = 0;
+        }
+        return s;
+    };
+    if (out) *out = nullptr;
+    if (!o || !out) return say(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (o->abi_version != ESPICE_ABI_VERSION || o->struct_size < offsetof(espice_create_options, zig_path))
+        return say(ESPICE_ABI_MISMATCH, "AbiMismatch");
+    if (o->source_kind > ESPICE_BYTES || o->dialect > ESPICE_SPECTRE || o->backend > ESPICE_HIP ||
+        o->explicit_gpu > 1 || o->max_parallel == 0 || o->max_parallel > 65535 || !o->source.data || !o->source.len)
+        return say(ESPICE_INVALID_ARGUMENT, "InvalidArgument");
+    if (o->dialect != ESPICE_NGSPICE) return say(ESPICE_INVALID_ARGUMENT, "UnsupportedDialect");
+    if (o->backend == ESPICE_CPU || o->backend == ESPICE_CUDA) return say(ESPICE_FAILED, "BackendUnavailable");
+    std::string text;
+    if (o->source_kind == ESPICE_FILE) {
+        std::ifstream f(std::string(o->source.data, o->source.len), std::ios::binary);
+        if (!f) return say(ESPICE_FAILED, "FileNotFound");
+        std::stringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    } else {
+        text.assign(o->source.data, o->source.len);
+    }
+    auto P = std::make_unique<Problem>();
+    std::string err;
+    if (!esp::read_deck(text, P->bd.deck, err)) return say(ESPICE_INVALID_ARGUMENT, err);
+    const esp::Deck &d = P->bd.deck;
+    if (!d.op && !d.tran) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
+    if (espice_lanes_template_create(&P->tpl) != ESPICE_OK) return say(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
+    if (!esp::build_deck(P->bd, P->tpl, err)) return say(ESPICE_INVALID_ARGUMENT, err);
+    for (size_t k = 0; k < o->live_param_count; k++) {
+        const std::string name = fold(o->live_params[k].data, o->live_params[k].len);
+        bool found = false;
+        for (auto &pr : d.params) found = found || pr.first == name;
+        if (!found) return say(ESPICE_INVALID_ARGUMENT, "NotAParameter " + name);
+        P->live.push_back(name);
+    }
+    if (!plan_passes(*P, err)) return say(ESPICE_INVALID_ARGUMENT, err);
+    if (!gpu_present()) return say(ESPICE_FAILED, "GpuUnavailable");
+    // point 0: the deck as written
+    Point p0;
+    std::map<std::string, double> env;
+    esp::param_env(d, {}, env, err);
+    for (auto &name : P->live) p0.live.push_back(env[name]);
+    P->per_point = (d.op ? 1u : 0u) + (d.tran ? 1u : 0u);
+    p0.first_id = 0;
+    P->queries = P->per_point;
+    P->points.push_back(p0);
+    espice_problem *tok = insert(P.get());
+    if (!tok) return say(ESPICE_OUT_OF_MEMORY, "TooManyProblems");
+    P.release();
+    *out = tok;
+    return ESPICE_OK;
+}
+
+espice_status espice_destroy(espice_problem *problem)
+{
+    if (!problem) return ESPICE_OK;
+    Problem *p = remove(problem);
+    if (!p) return ESPICE_INVALID_HANDLE;
+    delete p;
+    return ESPICE_OK;
+}
+
+espice_status espice_query_count(espice_problem *problem, uint32_t *out)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    if (!out) return fail(P, ESPICE_INVALID_ARGUMENT, "NullArgument");
+    *out = P->queries;
+    return ESPICE_OK;
+}
+
+espice_status espice_error_message(espice_problem *problem, char *buf, size_t cap, size_t *required)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    const std::string &e = P->error;
+    if (required) *required = e.size() + 1;
+    if (cap < e.size() + 1) return ESPICE_BUFFER_TOO_SMALL;
+    std::memcpy(buf, e.c_str(), e.size() + 1);
+    return ESPICE_OK;
+}
+
+espice_status espice_run_all(espice_problem *problem)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    size_t b = 0;
+    const size_t n = P->points.size();
+    while (b < n) {
+        while (b < n && (P->points[b].ran || P->points[b].released)) b++;
+        if (b >= n) break;
+        size_t e = b;
+        while (e < n && !P->points[e].ran && !P->points[e].released && e - b < LANE_CAP) e++;
+        const espice_status s = run_batch(*P, b, e);
+        if (s != ESPICE_OK) return s;
+        b = e;
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_append_points(espice_problem *problem, const double *values, size_t count,
+                                   const espice_points_options *opts, uint32_t *first_id)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    if (opts && opts->struct_size < offsetof(espice_points_options, tangent_step))
+        return fail(P, ESPICE_ABI_MISMATCH, "AbiMismatch");
+    if (count == 0 || !values || !first_id) return fail(P, ESPICE_INVALID_ARGUMENT, "InvalidArgument");
+    if (P->live.empty()) return fail(P, ESPICE_INVALID_ARGUMENT, "NoLiveParameters");
+    if (opts) {
+        if (opts->measures_only > 1 || opts->early_stop > 1 || opts->warm_start > 1)
+            return fail(P, ESPICE_INVALID_ARGUMENT, "InvalidArgument");
+        if (!std::isnan(opts->abort_after) && (opts->abort_after < 0 || std::isinf(opts->abort_after)))
+            return fail(P, ESPICE_INVALID_ARGUMENT, "InvalidArgument");
+        if (opts->abort_measure.len) {
+            const std::string name = fold(opts->abort_measure.data, opts->abort_measure.len);
+            bool found = false;
+            for (auto &m : P->bd.deck.meas) found = found || m.name == name;
+            if (!found) return fail(P, ESPICE_INVALID_ARGUMENT, "UnknownMeasure");
+        }
+        if (opts->measures_only || opts->early_stop || !std::isnan(opts->abort_after) || opts->warm_start)
+            return fail(P, ESPICE_INVALID_ARGUMENT, "Unsupported");
+        if (opts->struct_size >= sizeof(espice_points_options) && opts->tangent_step > 0)
+            return fail(P, ESPICE_INVALID_ARGUMENT, "TangentsUnavailable");
+    }
+    const size_t k = P->live.size();
+    *first_id = P->queries;
+    for (size_t i = 0; i < count; i++) {
+        Point pt;
+        pt.live.assign(values + i * k, values + (i + 1) * k);
+        pt.first_id = P->queries;
+        P->queries += P->per_point;
+        bool finite = true;
+        for (double v : pt.live) finite = finite && std::isfinite(v);
+        if (!finite) {
+            pt.ran = true;
+            pt.failed = true;
+            pt.reason = ESPICE_REASON_INVALID_PARAMETER;
+            pt.meas.assign(P->bd.deck.meas.size(), NAN);
+        }
+        P->points.push_back(std::move(pt));
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_copy_point_measures(espice_problem *problem, uint32_t first_id, size_t count,
+                                         const espice_bytes *names, size_t measure_count, double *out,
+                                         uint32_t *reasons)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    if ((measure_count && (!names || !out)) || (count && !reasons))
+        return fail(P, ESPICE_INVALID_ARGUMENT, "NullArgument");
+    std::vector<size_t> idx;
+    for (size_t j = 0; j < measure_count; j++) {
+        const std::string name = fold(names[j].data, names[j].len);
+        size_t k = 0;
+        while (k < P->bd.deck.meas.size() && P->bd.deck.meas[k].name != name) k++;
+        if (k == P->bd.deck.meas.size()) return fail(P, ESPICE_INVALID_ARGUMENT, "UnknownMeasure");
+        idx.push_back(k);
+    }
+    const int p0 = point_at(*P, first_id);
+    if (p0 < 0 || (size_t)p0 + count > P->points.size()) return fail(P, ESPICE_INVALID_QUERY, "InvalidQuery");
+    for (size_t i = 0; i < count; i++) {
+        const Point &pt = P->points[p0 + i];
+        if (!pt.ran || pt.released) return fail(P, ESPICE_RESULT_UNAVAILABLE, "ResultUnavailable");
+    }
+    for (size_t i = 0; i < count; i++) {
+        const Point &pt = P->points[p0 + i];
+        for (size_t j = 0; j < measure_count; j++) out[i * measure_count + j] = pt.failed ? NAN : pt.meas[idx[j]];
+        reasons[i] = pt.reason;
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_release_points(espice_problem *problem, uint32_t first_id, size_t count)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    const int p0 = point_at(*P, first_id);
+    if (p0 < 0 || (size_t)p0 + count > P->points.size()) return fail(P, ESPICE_INVALID_QUERY, "InvalidQuery");
+    for (size_t i = 0; i < count; i++) {
+        const Point &pt = P->points[p0 + i];
+        if (!pt.ran && !pt.released) return fail(P, ESPICE_RESULT_UNAVAILABLE, "ResultUnavailable");
+    }
+    for (size_t i = 0; i < count; i++) {
+        Point &pt = P->points[p0 + i];
+        pt.released = true;
+        std::vector<double>().swap(pt.meas);
+        std::vector<double>().swap(pt.aux);
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_lane_width(espice_problem *problem, uint32_t *out)
+{
+    if (!problem) return ESPICE_INVALID_ARGUMENT;
+    Problem *P = get(problem);
+    if (!P) return ESPICE_INVALID_HANDLE;
+    if (!out) return fail(P, ESPICE_INVALID_ARGUMENT, "NullArgument");
+    *out = LANE_CAP;
+    return ESPICE_OK;
+}
+
+}  // extern "C"
+
+// ---- the CLI's view of point 0 (not part of espice.h): the printed lines
+// of the deck's .meas cards in the reference's format
+// (src/output/measure.zig:220-237, 948-1051), then its summary.
+namespace esp {
+
+// C's %e with `prec` digits (glibc's rounding of the exact decimal).
+static std::string sci(double x, int prec)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%.*e", prec, x);
+    return b;
+}
+
+std::string problem_report(espice_problem *problem)
+{
+    Problem *P = get(problem);
+    if (!P || P->points.empty() || !P->points[0].ran) return "";
+    const Point &pt = P->points[0];
+    const esp::Deck &d = P->bd.deck;
+    std::string out;
+    auto head = [&](const char *what) { out += std::string("\n  Measurements for ") + what + " Analysis\n\n"; };
+    auto line = [&](const std::string &name, const std::string &body) {
+        char b[32];
+        std::snprintf(b, sizeof b, "%-20s", name.c_str());
+        out += std::string(b) + "=" + body + "\n";
+    };
+    for (const char *an : {"op", "tran"}) {
+        bool headed = false;
+        for (size_t k = 0; k < d.meas.size(); k++) {
+            const Meas &m = d.meas[k];
+            if (m.analysis != an) continue;
+            if (!headed) head(std::string(an) == "op" ? "Operating Point" : "Transient");
+            headed = true;
+            const double v = pt.failed ? NAN : pt.meas[k];
+            const double a0 = pt.aux.size() > 2 * k ? pt.aux[2 * k] : NAN;
+            const double a1 = pt.aux.size() > 2 * k ? pt.aux[2 * k + 1] : NAN;
+            if (std::isnan(v)) {
+                line(m.name, "  NaN");
+                continue;
+            }
+            if (m.kind == "trig_targ") line(m.name, "  " + sci(v, 6) + " targ=  " + sci(a0, 6) + " trig=  " + sci(a1, 6));
+            else if (m.kind == "when") line(m.name, "   " + sci(v, 5));
+            else if (m.kind == "rms" || m.kind == "integ")
+                line(m.name, "   " + sci(v, 5) + " from=  " + sci(a0, 5) + " to=  " + sci(a1, 5));
+            else if (m.kind == "avg") {
+                std::map<std::string, double> env;
+                std::string err;
+                esp::param_env(d, {}, env, err);
+                const double from = val_of(m.a.from, env, 0.0, err);
+                line(m.name, "  " + sci(v, 6) + " from=  " + sci(from, 6) + " to=  " + sci(a0, 6));
+            } else if (m.kind == "min" || m.kind == "max")
+                line(m.name, "  " + sci(v, 6) + " at=  " + sci(a0, 6));
+            else if (m.kind == "param") line(m.name, "  " + sci(v, 5));
+            else line(m.name, "  " + sci(v, 6));
+        }
+    }
+    out += d.title + ": " + std::to_string(d.elements.size()) + " devices\n";
+    if (d.op) out += "  Operating Point: " + std::string(pt.failed ? "failed" : "done") + "\n";
+    if (d.tran)
+        out += "  Transient Analysis: " + std::to_string(pt.tran_points) + " points, " + std::to_string(P->bd.n) +
+               " variables\n";
+    for (const std::string &w : d.ignored) out += "note: " + w + " is not run by the lane engine\n";
+    return out;
+}
+
+// The measures of point 0 as JSON: {"name": value, ...} (null for NaN).
+std::string problem_measures_json(espice_problem *problem)
+{
+    Problem *P = get(problem);
+    if (!P || P->points.empty() || !P->points[0].ran) return "{}";
+    const Point &pt = P->points[0];
+    std::string out = "{";
+    for (size_t k = 0; k < P->bd.deck.meas.size(); k++) {
+        const double v = pt.failed ? NAN : pt.meas[k];
+        char b[64];
+        if (std::isnan(v)) std::snprintf(b, sizeof b, "null");
+        else std::snprintf(b, sizeof b, "%.17g", v);
+        out += std::string(k ? ", " : "") + "\"" + P->bd.deck.meas[k].name + "\": " + b;
+    }
+    return out + "}";
+}
+
+}  // namespace esp
+
+extern "C" espice_status espice_cli_report(espice_problem *problem, int json, char *buf, size_t cap, size_t *required)
+{
+    if (!get(problem)) return ESPICE_INVALID_HANDLE;
+    const std::string s = json ? esp::problem_measures_json(problem) : esp::problem_report(problem);
+    if (required) *required = s.size() + 1;
+    if (cap < s.size() + 1) return ESPICE_BUFFER_TOO_SMALL;
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return ESPICE_OK;
+}
+
+extern "C" int espice_spice_number(const char *text, double *out)
+{
+    if (!text || !out) return 0;
+    std::string t(text);
+    for (char &c : t) c = (char)std::tolower((unsigned char)c);
+    return esp::spice_number(t, *out) ? 1 : 0;
+}

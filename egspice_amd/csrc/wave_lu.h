@@ -209,17 +209,17 @@ __device__ int dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &
     return wide ? REPLAY_PROBE : REPLAY_HOLDS;
 }
 
-// luSolve of -rhs into dx (lane_solver.h direct_solve_neg / lu_solve).
-// All threads; reads the pivots other threads wrote, so it must follow a
-// barrier; writes dx[q[r]] (needs a barrier before others read it).
-template <class R, class X>
-__device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs, X &dx)
+// lu_solve's two sweeps on the wavefront: thread r enters with y_r, the
+// right-hand side at pivot position r, and leaves with the solution's
+// entry there. Forward over L skips a zero y_k, backward over U a zero
+// z_k, as the serial sweeps do. All threads; reads the factor other
+// threads wrote, so it must follow a barrier.
+__device__ inline double dense_sweeps(uint32_t n, const DenseLU &D, double y)
 {
-    const uint32_t n = P.n, r = threadIdx.x;
+    const uint32_t r = threadIdx.x;
     const bool own = r < n;
     const uint32_t rr = own ? r : n - 1;
     const uint64_t lm = own ? D.lmask[r] : 0, um = own ? D.umask[r] : 0;
-    double y = own ? -1.0 * rhs[D.porig[r]] : 0.0;
     for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
         double fv[CHUNK];
 #pragma unroll
@@ -256,7 +256,45 @@ __device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs
         }
         k1 = k1 > CHUNK ? k1 - CHUNK : 0;
     }
+    return y;
+}
+
+// luSolve of -rhs into dx (lane_solver.h direct_solve_neg / lu_solve).
+// All threads; reads the pivots other threads wrote, so it must follow a
+// barrier; writes dx[q[r]] (needs a barrier before others read it).
+template <class R, class X>
+__device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs, X &dx)
+{
+    const uint32_t n = P.n, r = threadIdx.x;
+    const bool own = r < n;
+    double y = own ? -1.0 * rhs[D.porig[r]] : 0.0;
+    y = dense_sweeps(n, D, y);
     if (own) dx[P.q[r]] = y;
+}
+
+// lu_solve_t's sweeps (U^T then L^T) on the wavefront, thread r holding
+// y_r at pivot position r. Each step's sum runs over the column's U (L)
+// rows in the tape's order, one term per row, as the serial dot product
+// subtracts them; the products are formed on their own rows in parallel.
+__device__ inline double dense_sweeps_t(uint32_t n, const LaneLU &s, const DenseLU &D, double y)
+{
+    const uint32_t r = threadIdx.x;
+    const uint32_t rr = r < n ? r : n - 1;
+    for (uint32_t k = 0; k < n; k++) {
+        const double prod = D.F[k * n + rr] * y;
+        double v = lane_read(y, k);
+        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) v = v - lane_read(prod, s.ui[p]);
+        const double yk = v / D.F[k * n + k];
+        y = r == k ? yk : y;
+    }
+    for (uint32_t k = n; k > 0;) {
+        k--;
+        const double prod = D.F[k * n + rr] * y;
+        double v = lane_read(y, k);
+        for (uint32_t p = s.lp[k]; p < s.lp[k + 1]; p++) v = v - lane_read(prod, s.li[p]);
+        y = r == k ? v : y;
+    }
+    return y;
 }
 
 // ------------------------------------------------------- replay's probe
@@ -337,6 +375,75 @@ __device__ bool dense_probe_holds(const Pattern &P, const LaneLU &s, const Dense
     return true;
 }
 
+// The largest bit pattern over the wavefront (norm_inf_lv's bitwise max;
+// callers pass 0 from idle threads).
+__device__ inline uint64_t wave_bitmax(uint64_t b)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const uint64_t o = __shfl_xor(b, m, 64);
+        b = o > b ? o : b;
+    }
+    return b;
+}
+
+// probe_rhs's entry i (lane_solver.h).
+__device__ inline double probe_entry(uint32_t i, uint32_t n)
+{
+    const double den = (double)((n > 2 ? n : 2) - 1);
+    const double mag = 1 + (double)i / den;
+    return i % 2 == 0 ? mag : -mag;
+}
+
+// dense_probe_holds on the wavefront: thread i forms row i's sums and
+// residual in the serial order (columns ascending), thread j column j's;
+// the norms are order-free bit maxima; the two solves are dense_sweeps and
+// dense_sweeps_t. `px` is n doubles of LDS for the probe's solution. All
+// threads; uniform result.
+template <class A>
+__device__ bool dense_probe_holds_wave(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals, double *px)
+{
+    const uint32_t n = P.n, r = threadIdx.x;
+    const bool own = r < n;
+    double row = 0, col = 0;
+    for (uint32_t j = 0; j < n; j++)
+        for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++)
+            if (P.row_idx[p] == r) row += fabs_(vals[p]);
+    if (own)
+        for (uint32_t p = P.col_ptr[r]; p < P.col_ptr[r + 1]; p++) col += fabs_(vals[p]);
+    // `if (col > norm_1) norm_1 = col` from 0: the largest non-NaN column sum
+    const double norm_1 = from_bits(wave_bitmax(own && col == col ? bits(col) : 0));
+    const double norm_inf_a = from_bits(wave_bitmax(own ? bits(fabs_(row)) : 0));
+    const double nf = (double)n;
+    for (int transpose = 0; transpose < 2; transpose++) {
+        if (!transpose) {
+            const double y = dense_sweeps(n, D, own ? probe_entry(D.porig[r], n) : 0.0);
+            if (own) px[P.q[r]] = y;
+        } else {
+            const double y = dense_sweeps_t(n, s, D, own ? probe_entry(P.q[r], n) : 0.0);
+            if (own) px[D.porig[r]] = y;
+        }
+        __syncthreads();
+        double res = own ? probe_entry(r, n) : 0.0;
+        if (!transpose) {
+            for (uint32_t j = 0; j < n; j++) {
+                const double xj = px[j];
+                for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++)
+                    if (P.row_idx[p] == r) res -= vals[p] * xj;
+            }
+        } else if (own) {
+            for (uint32_t p = P.col_ptr[r]; p < P.col_ptr[r + 1]; p++) res -= vals[p] * px[P.row_idx[p]];
+        }
+        const double nr = from_bits(wave_bitmax(own ? bits(fabs_(res)) : 0));
+        const double nx = from_bits(wave_bitmax(own ? bits(fabs_(px[r])) : 0));
+        const double a_norm = transpose ? norm_1 : norm_inf_a;
+        const double ratio = nr / (a_norm * nx * nf * EPS);
+        __syncthreads();
+        if (!(ratio <= 30.0 / PIVOT_TOL)) return false;
+    }
+    return true;
+}
+
 // --------------------------------------------------- direct.Solver.factor
 
 // lane_solver.h direct_factor for one lane on its wavefront: 0 factored,
@@ -366,9 +473,7 @@ __device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D,
         bool holds = replay == REPLAY_HOLDS;
         if (replay == REPLAY_PROBE) {
             __syncthreads();
-            if (tid == 0) *flag = dense_probe_holds(P, s, D, vals) ? 1u : 0u;
-            __syncthreads();
-            holds = *flag != 0;
+            holds = dense_probe_holds_wave(P, s, D, vals, s.px.p);
         }
         if (holds) {
             for (uint32_t p = tid; p < nnz; p += nt) s.vcopy[p] = vals[p];

@@ -1,0 +1,66 @@
+"""Input decks of the reference's documented runs (docs/examples/*.sp), the
+circuits and analysis cards as the docs give them, with the outputs the
+reference printed for them (docs/learn/measure.md:24-36, analyses.md:25-30,
+models.md:95-102)."""
+
+MEASURE_SP = """Measurements on an RC step response
+* first-order RC, tau = 1 us; the input steps up at 1 us and down at 6 us
+V1 in 0 PULSE(0 1 1u 1n 1n 5u 20u)
+R1 in out 1k
+C1 out 0 1n
+.tran 10n 10u
+.meas tran trise TRIG v(out) VAL=0.1 RISE=1 TARG v(out) VAL=0.9 RISE=1
+.meas tran tdelay TRIG v(in) VAL=0.5 RISE=1 TARG v(out) VAL=0.5 RISE=1
+.meas tran vat3u FIND v(out) AT=3u
+.meas tran tcross WHEN v(out)=0.5 FALL=1
+.meas tran vmax MAX v(out)
+.meas tran vavg AVG v(out) FROM=0 TO=10u
+.meas tran vrms RMS v(out) FROM=0 TO=10u
+.meas tran slope DERIV v(out) AT=2u
+.meas tran charge INTEG i(V1) FROM=1u TO=6u
+.meas tran ratio PARAM='trise/tdelay'
+.end
+"""
+
+# name -> (value, printed digits) from docs/learn/measure.md:24-36; the
+# side values (targ/trig, at, from/to) follow as name.side.
+MEASURE_GOLDEN = {
+    "trise": (2.197201e-06, 6), "tdelay": (6.931367e-07, 6), "vat3u": (8.645984e-01, 6),
+    "tcross": (6.68790e-06, 5), "vmax": (9.932657e-01, 6), "vavg": (4.982781e-01, 6),
+    "vrms": (6.33054e-01, 5), "slope": (3.685512e+05, 6), "charge": (-9.93246e-10, 5),
+    "ratio": (3.169938e+00, 6),
+}
+MEASURE_POINTS = 1029
+
+RC_LOWPASS_SP = """RC low-pass filter
+* 1 kOhm with 159 nF: corner near 1 kHz
+V1 in 0 DC 0 AC 1 PULSE(0 1 0 1u 1u 5m 10m)
+R1 in out 1k
+C1 out 0 159n
+.op
+.ac dec 10 10 100k
+.tran 10u 5m
+.end
+"""
+RC_LOWPASS_POINTS = 519
+
+CMOS_INVERTER_SP = """CMOS inverter
+.param vdd=1.8 wn=1u wp={2*wn}
+VDD vdd 0 {vdd}
+VIN in 0 DC 0 PULSE(0 {vdd} 1n 100p 100p 5n 10n)
+M1 out in 0 0 nch W={wn} L=180n
+M2 out in vdd vdd pch W={wp} L=180n
+CL out 0 10f
+.model nch NMOS(LEVEL=1 VTO=0.45 KP=200u LAMBDA=0.05 GAMMA=0.4 PHI=0.8)
+.model pch PMOS(LEVEL=1 VTO=-0.45 KP=80u LAMBDA=0.05 GAMMA=0.4 PHI=0.8)
+.dc VIN 0 1.8 0.01
+.tran 10p 20n
+.meas tran tphl TRIG v(in) VAL=0.9 RISE=1 TARG v(out) VAL=0.9 FALL=1
+.end
+"""
+CMOS_INVERTER_GOLDEN = {"tphl": (2.818778e-11, 6), "tphl.targ": (1.078188e-09, 6), "tphl.trig": (1.050000e-09, 6)}
+CMOS_INVERTER_POINTS = 2032
+
+# measure.sp with its R and C as .param, so points can move them
+MEASURE_PARAM_SP = MEASURE_SP.replace("R1 in out 1k", ".param r=1k c=1n\nR1 in out {r}").replace(
+    "C1 out 0 1n", "C1 out 0 {c}")
