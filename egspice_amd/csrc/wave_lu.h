@@ -299,82 +299,6 @@ __device__ inline double dense_sweeps_t(uint32_t n, const LaneLU &s, const Dense
 
 // ------------------------------------------------------- replay's probe
 
-// lane_solver.h lu_solve / lu_solve_t on the dense values, serial.
-template <class X, class Y>
-__device__ void dense_lu_solve(const Pattern &P, const LaneLU &s, const DenseLU &D, const X &b, Y &x)
-{
-    const uint32_t n = P.n;
-    for (uint32_t r = 0; r < n; r++) s.y[s.pinv[r]] = b[r];
-    for (uint32_t k = 0; k < n; k++) {
-        double yk = s.y[k];
-        if (yk == 0) continue;
-        for (uint32_t p = s.lp[k]; p < s.lp[k + 1]; p++) s.y[s.li[p]] -= D.F[k * n + s.li[p]] * yk;
-    }
-    for (uint32_t k = n; k > 0;) {
-        k--;
-        double zk = s.y[k] / D.F[k * n + k];
-        s.y[k] = zk;
-        if (zk == 0) continue;
-        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) s.y[s.ui[p]] -= D.F[k * n + s.ui[p]] * zk;
-    }
-    for (uint32_t j = 0; j < n; j++) x[P.q[j]] = s.y[j];
-}
-
-template <class X, class Y>
-__device__ void dense_lu_solve_t(const Pattern &P, const LaneLU &s, const DenseLU &D, const X &b, Y &x)
-{
-    const uint32_t n = P.n;
-    for (uint32_t j = 0; j < n; j++) s.y[j] = b[P.q[j]];
-    for (uint32_t k = 0; k < n; k++) {
-        double v = s.y[k];
-        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) v -= D.F[k * n + s.ui[p]] * s.y[s.ui[p]];
-        s.y[k] = v / D.F[k * n + k];
-    }
-    for (uint32_t k = n; k > 0;) {
-        k--;
-        double v = s.y[k];
-        for (uint32_t p = s.lp[k]; p < s.lp[k + 1]; p++) v -= D.F[k * n + s.li[p]] * s.y[s.li[p]];
-        s.y[k] = v;
-    }
-    for (uint32_t r = 0; r < n; r++) x[r] = s.y[s.pinv[r]];
-}
-
-// The probe half of replay_holds (lane_solver.h), on thread 0.
-template <class A>
-__device__ bool dense_probe_holds(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals)
-{
-    const uint32_t n = P.n;
-    for (uint32_t i = 0; i < n; i++) s.prows[i] = 0;
-    double norm_1 = 0;
-    for (uint32_t j = 0; j < n; j++) {
-        double col = 0;
-        for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++) {
-            double a = fabs_(vals[p]);
-            s.prows[P.row_idx[p]] += a;
-            col += a;
-        }
-        if (col > norm_1) norm_1 = col;
-    }
-    double norm_inf_a = norm_inf_lv(s.prows, 0, n);
-    double nf = (double)n;
-    for (int transpose = 0; transpose < 2; transpose++) {
-        probe_rhs(s.pr, n);
-        if (transpose) dense_lu_solve_t(P, s, D, s.pr, s.px);
-        else dense_lu_solve(P, s, D, s.pr, s.px);
-        probe_rhs(s.pr, n);
-        for (uint32_t j = 0; j < n; j++)
-            for (uint32_t p = P.col_ptr[j]; p < P.col_ptr[j + 1]; p++) {
-                uint32_t i = P.row_idx[p];
-                if (transpose) s.pr[j] -= vals[p] * s.px[i];
-                else s.pr[i] -= vals[p] * s.px[j];
-            }
-        double a_norm = transpose ? norm_1 : norm_inf_a;
-        double ratio = norm_inf_lv(s.pr, 0, n) / (a_norm * norm_inf_lv(s.px, 0, n) * nf * EPS);
-        if (!(ratio <= 30.0 / PIVOT_TOL)) return false;
-    }
-    return true;
-}
-
 // The largest bit pattern over the wavefront (norm_inf_lv's bitwise max;
 // callers pass 0 from idle threads).
 __device__ inline uint64_t wave_bitmax(uint64_t b)

@@ -1,0 +1,215 @@
+"""The points API (include/espice.h) and the espice CLI on the GPU.
+
+The documented decks run through espice_create / espice_append_points /
+espice_run_all / espice_copy_point_measures, and every measure must equal,
+bit for bit, the oracle's transient and .meas passes on the same circuit with
+the deck's numbers read the same way (ngspice's INPevaluate, so `10u` is
+10 * 1e-6). The nominal points must also reproduce the reference's printed
+results (docs/learn/measure.md:24-36, models.md:95-102), and the CLI must
+print them as the reference does. Each case prints its name before touching
+the GPU.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import egspice_amd as E
+from egspice_amd import workloads as W
+import oracle_lib as O
+import decks
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sn = E.spice_number
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def P(a):
+    return np.ascontiguousarray(a).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def printed(value, golden, digits):
+    assert value == pytest.approx(golden, rel=0, abs=0.5 * 10 ** (math.floor(math.log10(abs(golden))) - digits) * 1.0001)
+
+
+def bits_equal(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    same = a.view(np.uint64) == b.view(np.uint64)
+    return bool(np.all(same | (np.isnan(a) & np.isnan(b))))
+
+
+MEAS_NAMES = ["trise", "tdelay", "vat3u", "tcross", "vmax", "vavg", "vrms", "slope", "charge", "ratio"]
+
+
+def oracle_measure_sp(oracle, r, c):
+    """measure.sp on the oracle with the deck's numbers as INPevaluate reads
+    them; the ten .meas values."""
+    pulse = tuple(sn(x) for x in ("0", "1", "1u", "1n", "1n", "5u", "20u"))
+    ck = O.Circuit(oracle)
+    W.rc_pulse(ck.add, r=r, c=c, pulse=pulse)
+    ck.freeze()
+    L = O.Lanes(ck, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(sn("10u"), sn("10n"), max_steps=200_000))
+    assert st.status == 0
+    n = len(t)
+    vin, vout, iv = (np.ascontiguousarray(x[:, k]) for k in (ck.node("in"), ck.node("out"), 2))
+    cross = lambda y, **k: oracle.or_meas_cross(P(t), P(y), n, C.byref(O.clause(**k)))
+    trise = cross(vout, val=sn("0.9"), rise=1) - cross(vout, val=sn("0.1"), rise=1)
+    tdelay = cross(vout, val=sn("0.5"), rise=1) - cross(vin, val=sn("0.5"), rise=1)
+    v, a, b = C.c_double(), C.c_double(), C.c_double()
+    oracle.or_meas_extremum(P(t), P(vout), n, 0.0, 0.0, 1, C.byref(v), C.byref(a))
+    vmax = v.value
+    oracle.or_meas_extremum(P(t), P(vout), n, sn("0"), sn("10u"), 2, C.byref(v), C.byref(a))
+    vavg = v.value
+    oracle.or_meas_integ(P(t), P(vout), n, sn("0"), sn("10u"), 1, C.byref(v), C.byref(a), C.byref(b))
+    vrms = v.value
+    oracle.or_meas_integ(P(t), P(iv), n, sn("1u"), sn("6u"), 0, C.byref(v), C.byref(a), C.byref(b))
+    charge = v.value
+    return [trise, tdelay, oracle.or_meas_at(P(t), P(vout), n, sn("3u"), 0), cross(vout, val=sn("0.5"), fall=1),
+            vmax, vavg, vrms, oracle.or_meas_at(P(t), P(vout), n, sn("2u"), 1), charge, trise / tdelay], len(t)
+
+
+def test_measure_sp_nominal_point(oracle, engine):
+    say("case: measure.sp through espice_create/run_all/copy_point_measures, point 0")
+    pb = E.Problem(decks.MEASURE_SP)
+    assert pb.query_count() == 1 and pb.lane_width() == 16384
+    pb.run_all()
+    m, reasons = pb.measures(0, 1, MEAS_NAMES)
+    assert reasons.tolist() == [0]
+    for k, name in enumerate(MEAS_NAMES):
+        printed(m[0, k], *decks.MEASURE_GOLDEN[name])
+    want, points = oracle_measure_sp(oracle, sn("1k"), sn("1n"))
+    assert points == decks.MEASURE_POINTS
+    assert bits_equal(m[0], want)
+    assert "Transient Analysis: 1029 points, 4 variables" in pb.report()
+
+
+def test_measure_sp_live_points(oracle, engine):
+    N = 48
+    say(f"case: measure.sp with live r, c: {N} points in one lane batch, bitwise vs the oracle")
+    pb = E.Problem(decks.MEASURE_PARAM_SP, live=("r", "c"))
+    rng = np.random.default_rng(3)
+    # tau = RC up to 1.44 us, so v(out) passes 0.9 V within the 5 us pulse
+    # and every measure resolves
+    rows = np.stack([rng.uniform(500, 1200, N), rng.uniform(0.5e-9, 1.2e-9, N)], axis=1)
+    first = pb.append_points(rows)
+    assert first == 1 and pb.query_count() == 1 + N
+    pb.run_all()
+    m, reasons = pb.measures(first, N, MEAS_NAMES)
+    assert (reasons == 0).all()
+    for l in range(0, N, 6):
+        want, _ = oracle_measure_sp(oracle, rows[l, 0], rows[l, 1])
+        assert bits_equal(m[l], want), l
+    # the nominal point is still point 0, unchanged by the appended ones
+    m0, _ = pb.measures(0, 1, ["trise"])
+    printed(m0[0, 0], 2.197201e-06, 6)
+    # released points give RESULT_UNAVAILABLE, their ids stay known
+    pb.release(first, N)
+    with pytest.raises(E.PointsError) as e:
+        pb.measures(first, N, ["trise"])
+    assert e.value.status == 5
+
+
+def test_point_reasons(engine):
+    say("case: measure.sp points with a non-finite value and an unresolved measure")
+    pb = E.Problem(decks.MEASURE_PARAM_SP.replace(".end", ".meas tran never WHEN v(out)=5 RISE=1\n.end"),
+                   live=("r", "c"))
+    first = pb.append_points([[1e3, 1e-9], [float("nan"), 1e-9]])
+    pb.run_all()
+    m, reasons = pb.measures(first, 2, ["trise", "never"])
+    assert reasons.tolist() == [5, 9]  # MEASURE_UNRESOLVED, INVALID_PARAMETER
+    assert np.isfinite(m[0, 0]) and np.isnan(m[0, 1]) and np.isnan(m[1]).all()
+    with pytest.raises(E.PointsError, match="UnknownMeasure"):
+        pb.measures(first, 2, ["nope"])
+    with pytest.raises(E.PointsError) as e:
+        pb.measures(first + 1, 2, ["trise"])  # not a point's first query / past the end
+    assert e.value.status == 4
+
+
+def test_cmos_inverter_points(oracle, engine):
+    N = 24
+    say(f"case: cmos_inverter.sp, live vdd and wn, {N} points + nominal, bitwise vs the oracle")
+    pb = E.Problem(decks.CMOS_INVERTER_SP, live=("vdd", "wn"))
+    rng = np.random.default_rng(9)
+    rows = np.stack([rng.uniform(1.6, 2.0, N), rng.uniform(0.8e-6, 1.5e-6, N)], axis=1)
+    first = pb.append_points(rows)
+    pb.run_all()
+    m0, r0 = pb.measures(0, 1, ["tphl"])
+    assert r0.tolist() == [0]
+    printed(m0[0, 0], *decks.CMOS_INVERTER_GOLDEN["tphl"])
+    assert f"Transient Analysis: {decks.CMOS_INVERTER_POINTS} points" in pb.report()
+    m, reasons = pb.measures(first, N, ["tphl"])
+    assert (reasons == 0).all()
+    M = E.MOS1_PARAMS
+    model = lambda typ, vto, kp: {M["TYPE"]: typ, M["VTO"]: sn(vto), M["KP"]: sn(kp), M["LAMBDA"]: sn("0.05"),
+                                  M["GAMMA"]: sn("0.4"), M["PHI"]: sn("0.8"), M["L"]: sn("180n")}
+    for l in range(0, N, 4):
+        vdd, wn = rows[l]
+        ck = O.Circuit(oracle)
+        ck.add(O.VSRC, ["vdd", "0"], {0: vdd})
+        ck.add(O.VSRC, ["in", "0"], W.pulse_params(sn("0"), vdd, sn("1n"), sn("100p"), sn("100p"), sn("5n"),
+                                                   sn("10n"), dc=sn("0")))
+        ck.add(O.CAP, ["out", "0"], {0: sn("10f")})
+        ck.add(O.MOS1, ["out", "in", "0", "0"], {**model(1.0, "0.45", "200u"), M["W"]: wn})
+        ck.add(O.MOS1, ["out", "in", "vdd", "vdd"], {**model(-1.0, "-0.45", "80u"), M["W"]: 2 * wn})
+        ck.freeze()
+        L = O.Lanes(ck, 1)
+        L.prepare()
+        t, x, st = L.tran(0, O.tran_opts(sn("20n"), sn("10p"), max_steps=200_000))
+        assert st.status == 0
+        n = len(t)
+        vin, vout = (np.ascontiguousarray(x[:, ck.node(k)]) for k in ("in", "out"))
+        trig = oracle.or_meas_cross(P(t), P(vin), n, C.byref(O.clause(val=sn("0.9"), rise=1)))
+        targ = oracle.or_meas_cross(P(t), P(vout), n, C.byref(O.clause(val=sn("0.9"), fall=1)))
+        assert bits_equal(m[l, 0], targ - trig), l
+
+
+def test_cli_prints_the_reference_lines(tmp_path, engine):
+    say("case: egspice_amd/espice --backend=hip measure.sp")
+    deck = tmp_path / "measure.sp"
+    deck.write_text(decks.MEASURE_SP)
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    r = subprocess.run([exe, "--backend=hip", "--measures=-", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    for want in (
+        "trise               =  2.197201e-06 targ=  3.303064e-06 trig=  1.105863e-06",
+        "tdelay              =  6.931367e-07 targ=  1.693637e-06 trig=  1.000500e-06",
+        "vat3u               =  8.645984e-01",
+        "tcross              =   6.68790e-06",
+        "vmax                =  9.932657e-01 at=  6.001000e-06",
+        "vavg                =  4.982781e-01 from=  0.000000e+00 to=  1.000000e-05",
+        "vrms                =   6.33054e-01 from=  0.00000e+00 to=  1.00000e-05",
+        "slope               =  3.685512e+05",
+        "charge              =   -9.93246e-10 from=  1.00000e-06 to=  6.00000e-06",
+        "  Transient Analysis: 1029 points, 4 variables",
+    ):
+        assert want in lines, (want, r.stdout)
+    # PARAM cards print %.5e (src/output/measure.zig:1025-1029)
+    assert "ratio               =  3.16994e+00" in lines
+    assert lines[-1].startswith("{") and '"trise": 2.19720' in lines[-1]
+
+
+def test_cli_refuses_what_the_lanes_cannot_run(tmp_path, engine):
+    say("case: espice CLI errors (unsupported card, charged diode)")
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    bad = tmp_path / "bad.sp"
+    bad.write_text(decks.MEASURE_SP.replace(".end", "L1 out 0 1u\n.end"))
+    r = subprocess.run([exe, "--backend=hip", str(bad)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "UnsupportedElement" in r.stderr
+    charged = tmp_path / "charged.sp"
+    charged.write_text("diode with junction charge\nV1 a 0 PULSE(0 1 1n 1n 1n 5n 10n)\nR1 a b 1k\nD1 b 0 dj\n"
+                       ".model dj D(IS=1e-14 CJO=1p)\n.tran 10p 20n\n.end\n")
+    r = subprocess.run([exe, "--backend=hip", str(charged)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "TranChargesUnsupported" in r.stderr
