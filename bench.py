@@ -84,6 +84,20 @@ def cpu_baseline(n_lanes, budget_s=10.0):
                       f"oracle CPU path on {threads} threads, {platform.processor() or platform.machine()}"}
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch measured for this kernel and workload by the
+    FETCH_SIZE/WRITE_SIZE counter passes, copy-calibrated, as recorded in
+    profiles/<name>.md (`traffic_bytes_per_launch:`); None when absent."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_calibration(mib):
     """Five device-to-device copies of `mib` MiB (hipMemcpy D2D, the
     runtime's copy kernel): a kernel whose HBM traffic is known exactly
@@ -147,6 +161,17 @@ def bench_tran(args):
         done += int(((res["status"] == 0) & np.all(np.isfinite(meas[:, :, 0]), axis=1)).sum())
     wall = time.perf_counter() - t0
     attempts = float(res["attempts"].mean())
+    # FP64 model of one lane: per Newton tick the evaluation (V 8, R 10,
+    # C 3 with its charge), the companion rows and a = g + ag0 c (5n + 2 nnz),
+    # refactor + solve on the lane's tape, update and gates (12n); per
+    # accepted step the LTE bound (30 per charge site), the predictor (3n),
+    # advanceCurrent (4n) and the .meas passes (10 each)
+    lu, _ = lu_flops(t.col_ptr, t.row_idx, np.ones(t.nnz), t.n)
+    per_tick = 8 + 10 + 3 + 5 * t.n + 2 * t.nnz + lu + 12 * t.n
+    per_step = 30 + 7 * t.n + 10 * len(passes)
+    flops = float(res["nr_iters"].sum()) * per_tick + float(res["steps"].sum()) * per_step
+    kms = float(np.mean(kms))
+    achieved = flops / (kms * 1e-3) / 1e12
     line = {
         "metric": "converged transient lanes/sec (every .meas resolved)", "value": done / wall, "unit": "lanes/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -154,7 +179,11 @@ def bench_tran(args):
         "config": {"workload": "docs/examples/measure.sp RC, .tran 10n 10u, 11 .meas passes, R and C per lane",
                    "lanes": N, "unknowns": t.n, "points_per_lane": float(res["points"].mean()),
                    "attempts_per_lane": attempts, "newton_ticks_per_lane": float(res["nr_iters"].mean()),
-                   "kernel_ms": float(np.mean(kms))},
+                   "kernel_ms": kms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": profiled_traffic("tran_wave_kernel_rc_measure_4096") if N == 4096 else None,
+                     "kernel": "tran_wave_kernel", "flops_per_launch": flops},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_tran(N, args.cpu_budget)
@@ -268,7 +297,8 @@ def main():
                    "converged_lanes": converged, "newton_ticks_per_solve": ticks,
                    "kernel_ms_per_solve": kms},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": profiled_traffic("newton_wave_kernel_inverter_chain_64") if N == 64 else None,
                      "kernel": "newton_wave_kernel", "flops_per_launch": flops},
     }
     if not args.no_cpu:
