@@ -29,6 +29,7 @@ EXPORTS = (
     "espice_lanes_template_info", "espice_lanes_template_pattern", "espice_lanes_create", "espice_lanes_destroy",
     "espice_lanes_reset", "espice_lanes_set_param", "espice_lanes_prepare", "espice_lanes_upload_x", "espice_lanes_download_x",
     "espice_lanes_newton_op", "espice_lanes_tran", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
+    "espice_lanes_mc_draw",
 )
 
 
@@ -109,6 +110,7 @@ def library():
         "espice_lanes_newton_op": [P, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p],
         "espice_lanes_tran": [P, C.POINTER(TranOpts), C.POINTER(MeasPass), C.c_uint32, C.POINTER(TranResult),
                               C.POINTER(MeasResult), f64p, f64p, f64p],
+        "espice_lanes_mc_draw": [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, f64p, C.c_uint32, f64p],
         "espice_lanes_tab_build": [C.c_uint32, u32p, u32p, f64p, C.POINTER(LaneTab), u32p, C.c_size_t,
                                    C.POINTER(C.c_size_t)],
         "espice_lanes_tab_refactor_solve": [C.POINTER(LaneTab), u32p, C.c_size_t, C.c_uint32, f64p, f64p, f64p,
@@ -118,6 +120,8 @@ def library():
         f = getattr(lib, name)
         f.restype = C.c_uint32
         f.argtypes = args
+    lib.espice_lanes_site_key.restype = C.c_uint64
+    lib.espice_lanes_site_key.argtypes = [C.c_uint64, C.c_uint64]
     _lib = lib
     return lib
 
@@ -459,6 +463,24 @@ class Problem:
         buf = C.create_string_buffer(req.value)
         self._check(self.lib.espice_cli_report(self.h, 1 if json else 0, buf, req.value, C.byref(req)))
         return buf.value.decode()
+
+
+MC_AGAUSS, MC_GAUSS, MC_AUNIF, MC_UNIF, MC_LIMIT = range(5)
+
+
+def site_key(owner, salt):
+    """netlist.siteKey: owner mixed with salt (splitmix64's finalizer)."""
+    return library().espice_lanes_site_key(owner & (2**64 - 1), salt & (2**64 - 1))
+
+
+def mc_draw(seed, site, first_trial, count, fn, *args):
+    """Trials first_trial .. first_trial + count - 1 of distribution fn at
+    site: one value per lane (agauss(nom, abs, sigmas[, mult]) and kin)."""
+    a, ap = _f64(np.asarray(args, dtype=np.float64))
+    out, op = _f64(np.zeros(count))
+    _check(library().espice_lanes_mc_draw(seed & (2**64 - 1), site & (2**64 - 1), first_trial, count, fn, ap,
+                                          len(a), op))
+    return out
 
 
 def tab_build(n, col_ptr, row_idx, vals):

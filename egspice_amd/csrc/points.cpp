@@ -23,6 +23,7 @@
 #include "../../include/espice_lanes.h"
 #include "cli.h"
 #include "deck.h"
+#include "sampler.h"
 
 using esp::BuiltDeck;
 using esp::Meas;
@@ -726,3 +727,13 @@ extern "C" int espice_spice_number(const char *text, double *out)
     for (char &c : t) c = (char)std::tolower((unsigned char)c);
     return esp::spice_number(t, *out) ? 1 : 0;
 }
+
+extern "C" espice_status espice_lanes_mc_draw(uint64_t seed, uint64_t site, uint32_t first_trial, uint32_t count,
+                                              uint32_t fn, const double *args, uint32_t n_args, double *out)
+{
+    if (!args || n_args < 1 || (count && !out) || fn > esp::MC_LIMIT) return ESPICE_INVALID_ARGUMENT;
+    for (uint32_t l = 0; l < count; l++) out[l] = esp::trial_draw(seed, first_trial + l, site, (int)fn, args, n_args);
+    return ESPICE_OK;
+}
+
+extern "C" uint64_t espice_lanes_site_key(uint64_t owner, uint64_t salt) { return esp::site_key(owner, salt); }

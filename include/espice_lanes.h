@@ -181,6 +181,18 @@ espice_status espice_lanes_tran(espice_lanes *l, const espice_tran_opts *opts, c
     uint32_t n_passes, espice_tran_result *out, espice_meas_result *meas, double *times, double *rows,
     double *kernel_ms);
 
+/* ---- Monte Carlo draws (src/frontend/variants.zig:1716-1720, 1862-1895;
+ * src/frontend/netlist.zig:683-688; src/core/numerics.zig:416-432) ----
+ * Fills out[l] with trial first_trial + l's draw of distribution `fn`
+ * (ESPICE_MC_*) at `site`: args = (nom, width[, sigmas][, mult]) as the
+ * reference's agauss/gauss/aunif/unif/limit read them. Draws depend only
+ * on (seed, trial, site), so lanes may be drawn in any order and count. */
+enum { ESPICE_MC_AGAUSS = 0, ESPICE_MC_GAUSS = 1, ESPICE_MC_AUNIF = 2, ESPICE_MC_UNIF = 3, ESPICE_MC_LIMIT = 4 };
+espice_status espice_lanes_mc_draw(uint64_t seed, uint64_t site, uint32_t first_trial, uint32_t count, uint32_t fn,
+    const double *args, uint32_t n_args, double *out);
+/* netlist.siteKey: owner mixed with salt (splitmix64's finalizer). */
+uint64_t espice_lanes_site_key(uint64_t owner, uint64_t salt);
+
 #ifdef __cplusplus
 }
 #endif

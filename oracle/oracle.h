@@ -15,6 +15,11 @@
  *   circuit.c    src/device/Circuit.zig freeze + src/analysis/Circuit.zig planes,
  *                src/solver/converger.zig newton/finalizeStep and the lane
  *                Newton of src/solver/lane_device.zig, the DC sweep
+ *   tran.c       src/analysis/tran/tran.zig Sim (step control, companion,
+ *                LTE, PULSE breakpoints) and the lane step machine
+ *   meas.c       src/output/scan.zig .meas passes
+ *   mc.c         the Monte Carlo sampler (netlist.siteKey, Sampler.uniform,
+ *                normalQuantile, TrialDraw.sample)
  *
  * Parity status of each function is recorded in the header comment of its
  * file ("parity pinned by ..." or "parity unpinned").

@@ -126,6 +126,10 @@ def load():
         "or_meas_extremum": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p]),
         "or_meas_integ": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p, f64p]),
         "or_limvds": (C.c_double, [C.c_double, C.c_double]),
+        "or_site_key": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "or_mc_uniform": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint32]),
+        "or_normal_quantile": (C.c_double, [C.c_double]),
+        "or_trial_draw": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, f64p, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -235,3 +235,23 @@ def test_monte_carlo_wide_bitwise(oracle, engine):
               (E.MOS1, 2, O.M1["TOX"], 1e-7 * (1 + 0.02 * rng.standard_normal(N)))]
     so = run_pair(oracle, W.inverter_chain, N, params)
     assert (so & 1).mean() > 0.95
+
+
+def test_sampler_drawn_lanes_bitwise(oracle, engine):
+    """Per-lane draws from the reference's sampler, as a Monte Carlo sweep
+    feeds them: every MOSFET's VTO ~ agauss(nom, 45 mV, 3) and KP ~
+    gauss(nom, 2 %, 1), keyed (seed, trial, site) with one site per device
+    parameter; the input swept uniformly."""
+    N = 2048
+    say(f"case: inverter chain, {N} trials drawn by the Monte Carlo sampler (seed 7)")
+    seed = 7
+    params = [(E.VSOURCE, 1, 0, E.mc_draw(seed, E.site_key(1, 0), 1, N, E.MC_AUNIF, 0.9, 0.9))]
+    for k in range(10):
+        nom_vto = 0.7 if k % 2 == 0 else -0.7
+        nom_kp = 1e-4 if k % 2 == 0 else 4e-5
+        params.append((E.MOS1, k, O.M1["VTO"], E.mc_draw(seed, E.site_key(100 + k, 1), 1, N, E.MC_AGAUSS,
+                                                          nom_vto, 0.045, 3)))
+        params.append((E.MOS1, k, O.M1["KP"], E.mc_draw(seed, E.site_key(100 + k, 2), 1, N, E.MC_GAUSS,
+                                                         nom_kp, 0.02, 1)))
+    so = run_pair(oracle, W.inverter_chain, N, params)
+    assert (so & 1).mean() > 0.95
