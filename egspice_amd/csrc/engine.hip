@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     const bool has_charge = n_tape > 0;
     if (has_charge) {
         // q(op) seeds a flat history
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, true, 0.0);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, false, 0.0);
         __syncthreads();
         for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
         __syncthreads();
@@ -309,11 +309,23 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         const double t_dev = scal[1], xfact = scal[2];
         const int method = (int)scal[3];
         const uint32_t p0 = (uint32_t)scal[8], p1 = (uint32_t)scal[9], p2 = (uint32_t)scal[10];
-        for (uint32_t i = tid; i < n; i += nt) s.x[i] = cur[i] + xfact * (cur[i] - prev[i]);
+        for (uint32_t i = tid; i < n; i += nt) {
+            s.x[i] = cur[i] + xfact * (cur[i] - prev[i]);
+            s.x_old[i] = cur[i];
+        }
+        __syncthreads();
+        // MODEINITPRED: the predicted iterate limited against the last
+        // accepted point, the limiters cleared by the previous solve
+        // (tran.zig:1185-1189, eval.zig:2430-2434); Newton then starts with
+        // limiting active
+        for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst(W, s, e, false);
+        __syncthreads();
         WaveHook h{method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
         const int r = wave_newton<true, true>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
         if (r == NEWTON_CONVERGED && has_charge) {
-            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, true, t_dev);
+            // the charge re-read at the solution, unlimited (tran.zig:859-864,
+            // eval.zig:1086-1108)
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, false, t_dev);
             __syncthreads();
             for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
             __syncthreads();

@@ -320,7 +320,7 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     if (has_charge) {
         /* q(op) seeds a flat history; i_prev starts at 0 */
         or_evctx ec = {1, 0.0, qh[1], NULL, th[1]};
-        or_eval_planes(L, l, cur, 1, w.g, w.rhs, &ec);
+        or_eval_planes(L, l, cur, 0, w.g, w.rhs, &ec);
         for (int k = 2; k < TR_NH; k++) {
             memcpy(qh[k], qh[1], 8 * nq);
             memcpy(th[k], th[1], 8 * n_qt);
@@ -355,6 +355,10 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
         const tr_coeffs cf = coeffs(method, dt, dt_old[0], opt->xmu);
         const double xfact = dt / dt_old[0];
         for (uint32_t i = 0; i < n; i++) trial[i] = cur[i] + xfact * (cur[i] - prev[i]);
+        /* MODEINITPRED: the predicted iterate limited against the last
+         * accepted point, with the limiters cleared by the previous solve
+         * (HostOps.attempt tran.zig:1185-1189; applyLimits eval.zig:2430) */
+        (void)or_limit_pass(L, l, trial, cur, 0);
         tr_hook hk = {1, cf, qh[1], qh[2], i_prev};
         uint32_t iters = 0;
         const int nr = tr_newton(&w, trial, t_dev, &hk, &nopt, 0, 1, &iters);
@@ -370,8 +374,10 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
         double dt_next = steps == 0 ? dt : fmin(dt * 2.0, dt_max);
         const lte_in e = {dt, dt_old[0], dt_old[1], opt->reltol, opt->abstol, opt->chgtol, opt->trtol};
         if (has_charge) {
+            /* the charge re-read at the solution, unlimited (HostOps.charges
+             * tran.zig:1207-1213, evalQRange eval.zig:1086-1108) */
             or_evctx ec = {1, t_dev, qh[0], NULL, th[0]};
-            or_eval_planes(L, l, trial, 1, w.g, w.rhs, &ec);
+            or_eval_planes(L, l, trial, 0, w.g, w.rhs, &ec);
         }
         if (has_charge && steps > 0) {
             const double del = step_bound(method, method, th[0], th[1], th[2], th[3], ti_prev, n_qt, cf, e);
