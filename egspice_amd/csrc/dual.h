@@ -1,6 +1,10 @@
 // Forward-mode dual numbers over a device's local unknowns (the reference's
-// `Dual` of src/device/eval.zig:107-350, restated with this engine's fixed
-// operation order, shared by host and device).
+// `Dual` of src/device/eval.zig:107-350, shared by host and device). The
+// lanes follow the reference's formulas: the product rule and the
+// quotient's numerator fused into one rounding (its `mulAddV` is a fused
+// multiply-add on every FMA target, eval.zig:56-61, 346-348: x86-64 with
+// FMA and amdgcn), a quotient's lanes scaled by the reciprocal, and the
+// sqrt/pow slopes with their zero and non-finite guards (:260-316).
 #pragma once
 
 #include "emath.h"
@@ -52,14 +56,15 @@ ESP_HD Dual dmul(Dual a, Dual b)
 {
     Dual r;
     r.v = a.v * b.v;
-    for (int i = 0; i < DW; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    for (int i = 0; i < DW; i++) r.d[i] = __builtin_fma(b.d[i], a.v, a.d[i] * b.v);
     return r;
 }
 ESP_HD Dual ddiv(Dual a, Dual b)
 {
     Dual r;
+    const double inv = 1.0 / b.v;
     r.v = a.v / b.v;
-    for (int i = 0; i < DW; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+    for (int i = 0; i < DW; i++) r.d[i] = __builtin_fma(b.d[i], -r.v, a.d[i]) * inv;
     return r;
 }
 ESP_HD Dual dscale(double s, Dual a)
@@ -72,8 +77,9 @@ ESP_HD Dual dscale(double s, Dual a)
 ESP_HD Dual ddivs(Dual a, double s)
 {
     Dual r;
+    const double inv = 1.0 / s;
     r.v = a.v / s;
-    for (int i = 0; i < DW; i++) r.d[i] = a.d[i] / s;
+    for (int i = 0; i < DW; i++) r.d[i] = a.d[i] * inv;
     return r;
 }
 ESP_HD Dual daddc(Dual a, double c)
@@ -99,7 +105,7 @@ ESP_HD Dual dsqrt(Dual a)
 {
     Dual r;
     r.v = __builtin_sqrt(a.v);
-    double h = 0.5 / r.v;
+    const double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
     for (int i = 0; i < DW; i++) r.d[i] = a.d[i] * h;
     return r;
 }
@@ -107,8 +113,9 @@ ESP_HD Dual dpowc(Dual a, double c)
 {
     Dual r;
     r.v = e_pow(a.v, c);
-    double g = c * e_pow(a.v, c - 1.0);
-    for (int i = 0; i < DW; i++) r.d[i] = g * a.d[i];
+    double g = a.v != 0.0 ? c * r.v / a.v : c * e_pow(a.v, c - 1.0);
+    if (!(fabs_(g) < inf_())) g = 0.0;
+    for (int i = 0; i < DW; i++) r.d[i] = a.d[i] * g;
     return r;
 }
 ESP_HD Dual dminc(Dual a, double c) { return c < a.v ? dc(c) : a; }

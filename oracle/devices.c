@@ -35,7 +35,11 @@
 #include "oracle.h"
 #include "oracle_internal.h"
 
-/* ------------------------------------------------------------------ dual */
+/* ------------------------------------------------------------------ dual
+ * The reference's Dual (src/device/eval.zig:107-350): the product rule and
+ * a quotient's numerator fused (mulAddV is a fused multiply-add on the FMA
+ * targets it ships, eval.zig:56-61, 346-348), the quotient's lanes scaled by
+ * the reciprocal, guarded sqrt and pow slopes. */
 
 static dual dc(double v) { dual r; memset(&r, 0, sizeof r); r.v = v; return r; }
 static dual dvar(double v, int i) { dual r = dc(v); r.d[i] = 1.0; return r; }
@@ -46,18 +50,27 @@ static dual dmul(dual a, dual b)
 {
     dual r;
     r.v = a.v * b.v;
-    for (int i = 0; i < DUAL_W; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    for (int i = 0; i < DUAL_W; i++) r.d[i] = fma(b.d[i], a.v, a.d[i] * b.v);
     return r;
 }
+/* the lanes keep the reciprocal (eval.zig:259-268) */
 static dual ddiv(dual a, dual b)
 {
     dual r;
+    const double inv = 1.0 / b.v;
     r.v = a.v / b.v;
-    for (int i = 0; i < DUAL_W; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+    for (int i = 0; i < DUAL_W; i++) r.d[i] = fma(b.d[i], -r.v, a.d[i]) * inv;
     return r;
 }
 static dual dscale(double s, dual a) { dual r; r.v = s * a.v; for (int i = 0; i < DUAL_W; i++) r.d[i] = s * a.d[i]; return r; }
-static dual ddivs(dual a, double s) { dual r; r.v = a.v / s; for (int i = 0; i < DUAL_W; i++) r.d[i] = a.d[i] / s; return r; }
+static dual ddivs(dual a, double s)
+{
+    dual r;
+    const double inv = 1.0 / s;
+    r.v = a.v / s;
+    for (int i = 0; i < DUAL_W; i++) r.d[i] = a.d[i] * inv;
+    return r;
+}
 static dual daddc(dual a, double c) { a.v = a.v + c; return a; }
 static dual dcsub(double c, dual a) { dual r; r.v = c - a.v; for (int i = 0; i < DUAL_W; i++) r.d[i] = -a.d[i]; return r; }
 static dual dexp(dual a)
@@ -71,17 +84,19 @@ static dual dsqrt(dual a)
 {
     dual r;
     r.v = sqrt(a.v);
-    double h = 0.5 / r.v;
+    double h = r.v > 0.0 ? 0.5 / r.v : 0.0; /* eval.zig:289-294 */
     for (int i = 0; i < DUAL_W; i++) r.d[i] = a.d[i] * h;
     return r;
 }
-/* x^c for a constant c: c x^(c-1) dx */
+/* x^c for a constant c, slope c x^c / x (c x^(c-1) at x = 0), a non-finite
+ * slope becoming 0 (eval.zig:306-316) */
 static dual dpowc(dual a, double c)
 {
     dual r;
     r.v = om_pow(a.v, c);
-    double g = c * om_pow(a.v, c - 1.0);
-    for (int i = 0; i < DUAL_W; i++) r.d[i] = g * a.d[i];
+    double g = a.v != 0.0 ? c * r.v / a.v : c * om_pow(a.v, c - 1.0);
+    if (!isfinite(g)) g = 0.0;
+    for (int i = 0; i < DUAL_W; i++) r.d[i] = a.d[i] * g;
     return r;
 }
 /* Verilog-A min/max on values: the second operand wins only when strictly

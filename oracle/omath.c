@@ -15,8 +15,11 @@
  *        log(1+f) = f - (f^2/2 - s (f^2/2 + R)), R = sum 2 z^j/(2j+1),
  *        z = s^2 (the atanh series), as in Sun's fdlibm e_log.c.
  *   pow: exp(y log x) for x > 0 with the IEEE special cases.
+ *   sin: Sun's fdlibm s_sin.c: x = n pi/2 + (y0 + y1) by Cody-Waite with
+ *        pi/2 in three 33-bit pieces (e_rem_pio2.c, |x| < 2^20 pi/2), then
+ *        the k_sin.c / k_cos.c minimax polynomials on [-pi/4, pi/4].
  * Parity pinned by: tests/test_oracle_math.py (against the platform libm
- * within 2 ulp over 10^6 arguments, plus special values). Agreement with
+ * within 2 ulp over 10^6 arguments, plus special values; sin within 1 ulp). Agreement with
  * Gompute's last bit: parity unpinned (library absent).
  * Compile with -ffp-contract=off: every operation here is one IEEE op.
  */
@@ -124,3 +127,80 @@ double om_pow(double x, double y)
  * smaller (larger), so a NaN first operand propagates. */
 double om_min(double a, double b) { return b < a ? b : a; }
 double om_max(double a, double b) { return b > a ? b : a; }
+
+static const double S_[6] = {-1.66666666666666324348e-01, 8.33333333332248946124e-03, -1.98412698298579493134e-04,
+                             2.75573137070700676789e-06, -2.50507602534068634195e-08, 1.58969099521155010221e-10};
+static const double C_[6] = {4.16666666666666019037e-02, -1.38888888888741095749e-03, 2.48015872894767294178e-05,
+                             -2.75573143513906633035e-07, 2.08757232129817482790e-09, -1.13596475577881948265e-11};
+
+static double kernel_sin(double x, double y, int iy)
+{
+    double z = x * x, v = z * x;
+    double r = S_[1] + z * (S_[2] + z * (S_[3] + z * (S_[4] + z * S_[5])));
+    if (iy == 0) return x + v * (S_[0] + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S_[0]);
+}
+
+static double kernel_cos(double x, double y)
+{
+    uint32_t ix = (uint32_t)(to_bits(x) >> 32) & 0x7fffffffu;
+    if (ix < 0x3e400000u && (int)x == 0) return 1.0;
+    double z = x * x;
+    double r = z * (C_[0] + z * (C_[1] + z * (C_[2] + z * (C_[3] + z * (C_[4] + z * C_[5])))));
+    if (ix < 0x3FD33333u) return 1.0 - (0.5 * z - (z * r - x * y));
+    double qx = ix > 0x3fe90000u ? 0.28125 : from_bits((uint64_t)(ix - 0x00200000u) << 32);
+    double hz = 0.5 * z - qx, a = 1.0 - qx;
+    return a - (hz - (z * r - x * y));
+}
+
+static int rem_pio2(double x, double *y0, double *y1)
+{
+    const double invpio2 = 6.36619772367581382433e-01, pio2_1 = 1.57079632673412561417e+00,
+                 pio2_1t = 6.07710050650619224932e-11, pio2_2 = 6.07710050630396597660e-11,
+                 pio2_2t = 2.02226624879595063154e-21, pio2_3 = 2.02226624871116645580e-21,
+                 pio2_3t = 8.47842766036889956997e-32;
+    double ax = fabs(x);
+    int j = (int)((to_bits(ax) >> 52) & 0x7ff);
+    int n = (int)(ax * invpio2 + 0.5);
+    double fn = (double)n;
+    double r = ax - fn * pio2_1, w = fn * pio2_1t;
+    *y0 = r - w;
+    int i = j - (int)((to_bits(*y0) >> 52) & 0x7ff);
+    if (i > 16) {
+        double t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        *y0 = r - w;
+        i = j - (int)((to_bits(*y0) >> 52) & 0x7ff);
+        if (i > 49) {
+            t = r;
+            w = fn * pio2_3;
+            r = t - w;
+            w = fn * pio2_3t - ((t - r) - w);
+            *y0 = r - w;
+        }
+    }
+    *y1 = (r - *y0) - w;
+    if (x < 0) {
+        *y0 = -*y0;
+        *y1 = -*y1;
+        return -n;
+    }
+    return n;
+}
+
+double om_sin(double x)
+{
+    uint32_t ix = (uint32_t)(to_bits(x) >> 32) & 0x7fffffffu;
+    if (ix <= 0x3fe921fbu) return ix < 0x3e400000u ? x : kernel_sin(x, 0.0, 0);
+    if (ix >= 0x7ff00000u) return x - x;
+    double y0, y1;
+    int n = rem_pio2(x, &y0, &y1);
+    switch (n & 3) {
+    case 0: return kernel_sin(y0, y1, 1);
+    case 1: return kernel_cos(y0, y1);
+    case 2: return -kernel_sin(y0, y1, 1);
+    default: return -kernel_cos(y0, y1);
+    }
+}

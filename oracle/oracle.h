@@ -40,6 +40,7 @@ extern "C" {
 double om_exp(double x);
 double om_log(double x);
 double om_pow(double x, double y);
+double om_sin(double x);
 double om_min(double a, double b);
 double om_max(double a, double b);
 

@@ -238,6 +238,7 @@ void hl_prep(int kind, const double *u, double *p) { prep_kind(kind, u, p); }
 
 double hl_exp(double x) { return e_exp(x); }
 double hl_log(double x) { return e_log(x); }
+double hl_sin(double x) { return e_sin(x); }
 double hl_pow(double x, double y) { return e_pow(x, y); }
 
 }  // extern "C"

@@ -22,7 +22,7 @@ def load(path):
         ("hl_column_order", C.c_int, [C.c_uint32, u32p, u32p, u32p]),
         ("hl_amd", C.c_int, [C.c_uint32, u32p, u32p, u32p]),
         ("hl_prep", None, [C.c_int, f64p, f64p]),
-        ("hl_exp", C.c_double, [C.c_double]), ("hl_log", C.c_double, [C.c_double]),
+        ("hl_exp", C.c_double, [C.c_double]), ("hl_log", C.c_double, [C.c_double]), ("hl_sin", C.c_double, [C.c_double]),
         ("hl_pow", C.c_double, [C.c_double, C.c_double]),
     ]:
         f = getattr(hl, name)

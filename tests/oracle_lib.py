@@ -81,7 +81,7 @@ def load():
     P = C.c_void_p
     u32p, f64p = C.POINTER(C.c_uint32), C.POINTER(C.c_double)
     sig = {
-        "om_exp": (C.c_double, [C.c_double]), "om_log": (C.c_double, [C.c_double]),
+        "om_exp": (C.c_double, [C.c_double]), "om_log": (C.c_double, [C.c_double]), "om_sin": (C.c_double, [C.c_double]),
         "om_pow": (C.c_double, [C.c_double, C.c_double]),
         "or_order": (C.c_int, [C.c_uint32, u32p, u32p, u32p]),
         "or_amd": (C.c_int, [C.c_uint32, u32p, u32p, u32p]),

@@ -51,3 +51,18 @@ def test_engine_math_is_the_oracle_bitwise(oracle, hostlane):
         assert ulps(hostlane.hl_log(float(x)), oracle.om_log(float(x))) == 0
     for x, y in zip(rng.uniform(0.01, 3, 2000), rng.uniform(-3, 3, 2000)):
         assert ulps(hostlane.hl_pow(float(x), float(y)), oracle.om_pow(float(x), float(y))) == 0
+
+
+def test_sin_within_1_ulp(oracle, hostlane):
+    """fdlibm's sin against the platform libm (itself within 1 ulp), over the
+    range a source's phase reaches and beyond; the engine's host build is
+    the oracle's bit for bit."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-1, 1, 20000), rng.uniform(-50, 50, 20000), rng.uniform(-1e5, 1e5, 20000),
+                         [0.0, -0.0, 1e-300, 5e-324, math.pi / 4, math.pi / 2, math.pi, 2 * math.pi, 710.0,
+                          2 * math.pi * 50 * 0.06]])
+    worst = max(ulps(oracle.om_sin(float(x)), math.sin(x)) for x in xs)
+    assert worst <= 1
+    assert all(ulps(hostlane.hl_sin(float(x)), oracle.om_sin(float(x))) == 0 for x in xs[::7])
+    assert math.isnan(oracle.om_sin(math.inf)) and math.isnan(oracle.om_sin(math.nan))
+    assert math.copysign(1, oracle.om_sin(-0.0)) == -1
