@@ -621,28 +621,33 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             c.i++;
             double x;
             for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
-        } else if (w == "pulse" && e.letter == 'v') {
+        } else if ((w == "pulse" || w == "sin") && e.letter == 'v' && e.wave == 0) {
+            // positional arguments fill the waveform's slots in order; the
+            // ones not written keep the model defaults
+            // (frontend/builder.zig:2841-2871, 2985-2990)
+            const int most = w == "pulse" ? 7 : 6;
+            const std::string bad = (w == "pulse" ? "BadPulse " : "BadSin ") + e.name;
             c.i++;
             if (!c.next_is("(")) {
-                err = "BadPulse " + e.name;
+                err = bad;
                 return false;
             }
             c.i++;
             int k = 0;
             while (c.more() && !c.next_is(")")) {
-                if (k >= 7 || !to_value(c.peek(), e.pulse_args[k], err)) {
-                    if (err.empty()) err = "BadPulse " + e.name;
+                if (k >= most || !to_value(c.peek(), e.wave_args[k], err)) {
+                    if (err.empty()) err = bad;
                     return false;
                 }
                 k++;
                 c.i++;
             }
             if (!c.next_is(")") || k < 2) {
-                err = "BadPulse " + e.name;
+                err = bad;
                 return false;
             }
             c.i++;
-            e.pulse = true;
+            e.wave = w == "pulse" ? 1 : 2;
         } else if (w == "sin" || w == "pwl" || w == "exp" || w == "sffm" || w == "am" || w == "pulse") {
             err = "UnsupportedWaveform " + w;
             return false;
@@ -804,7 +809,7 @@ const char *const MOS1_KEYS[] = {"type", "vto", "kp", "gamma", "phi", "lambda", 
                                  "tpg", "is", "js", "cbd", "cbs", "cj", "cjsw", "pb", "mj", "mjsw", "fc", "cgso",
                                  "cgdo", "cgbo", "tnom", "gmin", "temp", "dtemp", "l", "w", "ad", "as", "pd", "ps", "m"};
 const char *const DIODE_KEYS[] = {"is", "n", "cjo", "vj", "m", "fc", "tt", "gmin", "area", "tnom", "temp", "dtemp",
-                                  "eg", "xti"};
+                                  "eg", "xti", "rs", "trs", "trs2", "tm1", "tm2"};
 
 int key_index(const char *const *keys, size_t n, const std::string &k)
 {
@@ -834,6 +839,11 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         for (size_t k = 0; k < d.elements.size(); k++)
             if (d.elements[k].letter == *o) bd.add_order.push_back((uint32_t)k);
     uint32_t next_row = 1;  // rows are numbered in order of allocation
+    // the template holds the nominal circuit: expressions at the deck's own
+    // .param values (points overwrite them per lane)
+    std::map<std::string, double> nominal;
+    std::string nerr;
+    param_env(d, {}, nominal, nerr);
     for (uint32_t k : bd.add_order) {
         Element &e = d.elements[k];
         uint32_t rows[4];
@@ -851,13 +861,15 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         switch (e.letter) {
         case 'v': {
             kind = ESPICE_LANES_VSOURCE;
-            if (e.pulse) {
-                prm.emplace_back(1, lit(1.0));
+            if (e.wave) {
+                // PULSE args go to slots 2..8, SIN args to 9..14; a dc left
+                // unwritten is filled from the waveform's t = 0 value when
+                // the lane constants are prepared
+                const uint32_t base = e.wave == 1 ? 2 : 9;
+                prm.emplace_back(1, lit(double(e.wave)));
                 for (int a = 0; a < 7; a++)
-                    if (e.pulse_args[a].given) prm.emplace_back(2 + a, e.pulse_args[a]);
-                // .op biases at dc, which the builder fills from waveform(0)
-                // (models/vsource.va:216-221): v1 before TD and at a zero TD
-                prm.emplace_back(0, e.value.given ? e.value : e.pulse_args[0]);
+                    if (e.wave_args[a].given) prm.emplace_back(base + a, e.wave_args[a]);
+                if (e.value.given) prm.emplace_back(0, e.value);
             } else {
                 prm.emplace_back(0, e.value.given ? e.value : lit(0.0));
             }
@@ -890,7 +902,7 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
             }
             kind = mos ? ESPICE_LANES_MOS1 : ESPICE_LANES_DIODE;
             const char *const *keys = mos ? MOS1_KEYS : DIODE_KEYS;
-            const size_t nk = mos ? 36 : 14;
+            const size_t nk = mos ? 36 : 19;
             if (mos) prm.emplace_back(0, lit(m.type == "nmos" ? 1.0 : -1.0));
             for (auto &[k, v] : m.kv) {
                 const int idx = key_index(keys, nk, k);
@@ -914,10 +926,14 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         }
         // template constants: the literals now; expressions are point values
         double pv[64];
-        const int npk[] = {9, 1, 1, 1, 14, 36};
+        const int npk[] = {ESPICE_LANES_VSOURCE_PARAMS,   ESPICE_LANES_ISOURCE_PARAMS, ESPICE_LANES_RESISTOR_PARAMS,
+                           ESPICE_LANES_CAPACITOR_PARAMS, ESPICE_LANES_DIODE_PARAMS,   ESPICE_LANES_MOS1_PARAMS};
         for (int a = 0; a < npk[kind]; a++) pv[a] = NAN;
-        for (auto &[idx, v] : prm)
-            if (!v.expr) pv[idx] = v.num;
+        for (auto &[idx, v] : prm) {
+            std::string xerr;
+            pv[idx] = v.expr ? eval_expr(*v.expr, nominal, xerr) : v.num;
+            if (!xerr.empty()) pv[idx] = NAN;
+        }
         uint32_t inst = 0;
         if (espice_lanes_template_add(t, (uint32_t)kind, rows, pv, (size_t)npk[kind], &inst) != ESPICE_OK) {
             size_t req;
@@ -929,8 +945,10 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         bd.elem_kind[k] = kind;
         bd.elem_inst[k] = inst;
         for (auto &[idx, v] : prm) bd.slots.push_back(ParamSlot{(uint32_t)kind, inst, idx, v});
-        // a V source's branch current takes the next row when it is added
+        // a V source's branch current takes the next row when it is added,
+        // and so does a diode's internal anode when its RS is positive
         if (e.letter == 'v') bd.vsrc_branch[e.name] = next_row++;
+        if (e.letter == 'd' && pv[ESPICE_LANES_DIODE_RS] > 0.0) next_row++;
     }
     if (espice_lanes_template_freeze(t) != ESPICE_OK) {
         size_t req;

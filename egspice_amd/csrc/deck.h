@@ -48,8 +48,9 @@ struct Element {
     std::vector<std::string> nodes;   // port nets
     std::string model;                // D and M cards
     Value value;                      // R, C, I value; V DC value
-    bool pulse = false;               // V with PULSE(v1 v2 td tr tf pw per)
-    Value pulse_args[7];
+    int wave = 0;                     // V waveform: 0 none, 1 PULSE(v1 v2 td tr tf pw per),
+                                      // 2 SIN(vo va freq td theta phase)
+    Value wave_args[7];
     std::map<std::string, Value> kv;  // W=, L=, AD=, M=, area
     int line = 0;
 };

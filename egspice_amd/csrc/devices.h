@@ -22,15 +22,20 @@ namespace esp {
 enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_NKIND = 6 };
 
 // User parameter slots (NaN = not given).
-enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_K };
+// waveform: 0 DC, 1 PULSE(V1 V2 TD TR TF PW PER), 2 SIN(VO VA FREQ TD THETA
+// PHASE).
+enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_SVO, VSRC_SVA,
+       VSRC_SFREQ, VSRC_STD, VSRC_STHETA, VSRC_SPHASE, VSRC_K };
 // Prepared source constants: the user slots with vsource.va's defaults
 // (models/vsource.va:58-66: 0, and -1 sentinels for TR, TF, PW, PER).
-enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_K };
+enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
+       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_K };
+constexpr double VA_PI = 3.14159265358979323846, VA_TWO_PI = 6.28318530717958647652;  // constants.vams
 enum { ISRC_DC = 0, ISRC_K = 1 };
 enum { RES_R = 0, RES_K = 1 };
 enum { CAP_C = 0, CAP_K = 1 };
 enum { DIO_IS = 0, DIO_N, DIO_CJO, DIO_VJ, DIO_M, DIO_FC, DIO_TT, DIO_GMIN, DIO_AREA, DIO_TNOM, DIO_TEMP, DIO_DTEMP,
-       DIO_EG, DIO_XTI, DIO_K };
+       DIO_EG, DIO_XTI, DIO_RS, DIO_TRS, DIO_TRS2, DIO_TM1, DIO_TM2, DIO_K };
 enum { M1_TYPE = 0, M1_VTO, M1_KP, M1_GAMMA, M1_PHI, M1_LAMBDA, M1_TOX, M1_LD, M1_U0, M1_NSUB, M1_NSS, M1_TPG, M1_IS,
        M1_JS, M1_CBD, M1_CBS, M1_CJ, M1_CJSW, M1_PB, M1_MJ, M1_MJSW, M1_FC, M1_CGSO, M1_CGDO, M1_CGBO, M1_TNOM, M1_GMIN,
        M1_TEMP, M1_DTEMP, M1_L, M1_W, M1_AD, M1_AS, M1_PD, M1_PS, M1_M, M1_K };
@@ -39,17 +44,25 @@ enum { M1_TYPE = 0, M1_VTO, M1_KP, M1_GAMMA, M1_PHI, M1_LAMBDA, M1_TOX, M1_LD, M
 enum { M1P_TYPE = 0, M1P_VT, M1P_VBIT, M1P_PHIT, M1P_GAMMA, M1P_BETA, M1P_LAMBDA, M1P_ISATS, M1P_ISATD, M1P_GMIN,
        M1P_VCRITS, M1P_VCRITD, M1P_COX, M1P_CZBD, M1P_CZBS, M1P_CZBDSW, M1P_CZBSSW, M1P_PBT, M1P_MJ, M1P_MJSW, M1P_FC,
        M1P_F2D, M1P_F3D, M1P_F4D, M1P_F2S, M1P_F3S, M1P_F4S, M1P_CGSO, M1P_CGDO, M1P_CGBO, M1P_SEED, M1P_K };
-enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT, DIOP_FC, DIOP_TT, DIOP_K };
+// QON: the raw CJO is positive (the depletion charge is on); RSON: the raw RS
+// is positive (ai is its own row, behind RST = RS(T)).
+enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT, DIOP_FC, DIOP_TT, DIOP_QON, DIOP_RSON,
+       DIOP_RST, DIOP_K };
 
 // Local unknowns, ports, user and prepared parameter counts per kind.
-constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 2, 4};
+// The diode's unknowns are (a, c, ai): ai, the junction's anode behind RS,
+// is the row of a itself when RS is 0 (diode.va:10-11, the collapse).
+constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4};
 constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
 constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
 constexpr int KIND_PK[K_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
 // Residual rows a kind writes in a static evaluation.
-constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x3u, 0xdu};
+constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
 // Unknowns a kind's limiter writes (its limited image).
-constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x1u, 0xbu};
+constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x4u, 0xbu};
+// The local row whose charge is the kind's LTE site (one ddt per capacitor
+// and per diode), or -1.
+constexpr int KIND_TAPE_ROW[K_NKIND] = {-1, -1, -1, 0, 2, -1};
 
 // Structural G-plane and C-plane entries per local row (bit masks over
 // local columns). The pattern and slot tapes use their union.
@@ -57,8 +70,8 @@ ESP_HD unsigned kind_gmask(int kind, int ru)
 {
     switch (kind) {
     case K_VSRC: return ru == 2 ? 0x3u : 0x4u;
-    case K_RES:
-    case K_DIODE: return 0x3u;
+    case K_RES: return 0x3u;
+    case K_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case K_MOS1: return ru == 1 ? 0x0u : 0xfu;
     default: return 0u;
     }
@@ -67,6 +80,7 @@ ESP_HD unsigned kind_cmask(int kind, int ru)
 {
     switch (kind) {
     case K_CAP: return 0x3u;
+    case K_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case K_MOS1: return 0xfu;
     default: (void)ru; return 0u;
     }
@@ -213,7 +227,7 @@ inline void prep_mos1(const double *u, double *p)
     p[M1P_SEED] = type * vbit + gammax * __builtin_sqrt(phit);
 }
 
-// diode.va:66-172 (tlev = tlevc = 0, level 1, no sidewall).
+// diode.va:66-172, 259-262 (tlev = tlevc = 0, level 1, no sidewall).
 inline void prep_diode(const double *u, double *p)
 {
     const double is = given(u[DIO_IS], 1e-14), n = given(u[DIO_N], 1.0), cjo = given(u[DIO_CJO], 0.0);
@@ -230,7 +244,9 @@ inline void prep_diode(const double *u, double *p)
     double egp = given(u[DIO_EG], 1.11);
     double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
     double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
-    double mt = m * (1.0 + 0.0 * dtk + 0.0 * dtk * dtk);
+    const double tm1 = given(u[DIO_TM1], 0.0), tm2 = given(u[DIO_TM2], 0.0);
+    const double rs = given(u[DIO_RS], 0.0), trs = given(u[DIO_TRS], 0.0), trs2 = given(u[DIO_TRS2], 0.0);
+    double mt = m * (1.0 + tm1 * dtk + tm2 * dtk * dtk);
     double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
     double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
     double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
@@ -254,6 +270,11 @@ inline void prep_diode(const double *u, double *p)
     p[DIOP_MT] = mt;
     p[DIOP_FC] = fc;
     p[DIOP_TT] = tt;
+    p[DIOP_QON] = cjo > 0.0 ? 1.0 : 0.0;
+    p[DIOP_RSON] = rs > 0.0 ? 1.0 : 0.0;
+    double rs_t = rs * (1.0 + trs * dtk + trs2 * dtk * dtk) / ae;
+    if (rs_t < 1.0e-9) rs_t = 1.0e-9;
+    p[DIOP_RST] = rs_t;
     (void)dtemp;
 }
 
@@ -263,8 +284,14 @@ inline void prep_kind(int kind, const double *u, double *p)
     case K_MOS1: prep_mos1(u, p); break;
     case K_DIODE: prep_diode(u, p); break;
     case K_VSRC: {
-        const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0};
+        const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+        // .op biases at dc, which the builder fills from the waveform's t = 0
+        // value when the card gives none (models/vsource.va:216-221)
+        if (u[VSRC_DC] != u[VSRC_DC]) {
+            if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
+            else if (p[SRCP_WAVE] == 2.0) p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * e_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+        }
         break;
     }
     case K_ISRC: p[0] = u[ISRC_DC] == u[ISRC_DC] ? u[ISRC_DC] : 0.0; break;
@@ -380,6 +407,14 @@ ESP_HD void pulse_widths(const P &p, double &tr, double &tf, double &pw, double 
 template <class P>
 ESP_HD double src_value(const P &p, double t)
 {
+    if (p[SRCP_WAVE] == 2.0) {
+        // SIN (models/vsource.va:233-244): before TD the waveform's own t = 0
+        // value; after it the damped sine
+        const double t_sin = t - p[SRCP_STD];
+        if (t_sin < 0.0) return p[SRCP_SVO] + p[SRCP_SVA] * e_sin(p[SRCP_SPHASE] * VA_PI / 180.0);
+        return p[SRCP_SVO] + p[SRCP_SVA] * e_sin(VA_TWO_PI * p[SRCP_SFREQ] * t_sin + p[SRCP_SPHASE] * VA_PI / 180.0) *
+                                 e_exp(va_min(-p[SRCP_STHETA] * t_sin, 80.0));
+    }
     if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
     double tr, tf, pw, per;
     pulse_widths(p, tr, tf, pw, per);
@@ -569,11 +604,15 @@ ESP_HD void seed_mos1(const P &p, const double *cur, double *lim)
     lim[3] = s + type * -1.0;
 }
 
+// diode.va:282-372 (no ikf/ikr/isr/bv, level 1): the junction current and
+// charge across (ai, c) at the limited vd, and the series branch (a, ai),
+// V/RS(T), when RS is on. Unknowns (a, c, ai); rows a: +ir, c: -ij,
+// ai: ij - ir; charge rows c: -qj, ai: +qj.
 template <class P>
-ESP_HD void eval_diode(const P &p, const double *lx, DevOut &o)
+ESP_HD void eval_diode(const P &p, const double *lx, DevOut &o, bool tran)
 {
-    Dual a = dvar(lx[0], 0), c = dvar(lx[1], 1);
-    Dual vd = dsub(a, c);
+    const Dual a = dvar(lx[0], 0), c = dvar(lx[1], 1), ai = dvar(lx[2], 2);
+    const Dual vd = dsub(ai, c);
     const double vte = p[DIOP_VTE], is_t = p[DIOP_IS];
     Dual id;
     if (vd.v >= -3.0 * vte) {
@@ -584,31 +623,63 @@ ESP_HD void eval_diode(const P &p, const double *lx, DevOut &o)
         Dual t = dpowc(ddiv(dc(3.0 * vte), dscale(2.718281828459045, vd)), 3.0);
         id = dscale(-is_t, dadd(dc(1.0), t));
     }
-    Dual i = dadd(id, dscale(p[DIOP_GMIN], vd));
-    o.res[0] = i.v;
-    o.res[1] = -i.v;
-    for (int k = 0; k < 2; k++) {
-        o.jac[0][k] = i.d[k];
-        o.jac[1][k] = -i.d[k];
+    const Dual ij = dadd(id, dscale(p[DIOP_GMIN], vd));
+    const Dual ir = p[DIOP_RSON] != 0.0 ? ddivs(dsub(a, ai), p[DIOP_RST]) : dc(0.0);
+    const Dual rai = dsub(ij, ir);
+    o.res[0] = ir.v;
+    o.res[1] = -ij.v;
+    o.res[2] = rai.v;
+    for (int k = 0; k < 3; k++) {
+        o.jac[0][k] = ir.d[k];
+        o.jac[1][k] = -ij.d[k];
+        o.jac[2][k] = rai.d[k];
+    }
+    if (!tran) return;
+    // one junction charge, depletion plus diffusion (diode.va:328-363)
+    Dual qj = dc(0.0);
+    if (p[DIOP_QON] != 0.0) {
+        const double cjo_t = p[DIOP_CJO], vj_t = p[DIOP_VJ], mt = p[DIOP_MT], fc = p[DIOP_FC];
+        const double vfc = fc * vj_t;
+        if (vd.v < vfc) {
+            qj = ddivs(dscale(cjo_t * vj_t, dcsub(1.0, dpowc(dcsub(1.0, ddivs(vd, vj_t)), 1.0 - mt))), 1.0 - mt);
+        } else {
+            const double f1 = vj_t * (1.0 - e_pow(1.0 - fc, 1.0 - mt)) / (1.0 - mt);
+            const double f2 = e_pow(1.0 - fc, 1.0 + mt);
+            const double f3 = 1.0 - fc * (1.0 + mt);
+            const Dual lin = dscale(f3, daddc(vd, -vfc));
+            const Dual quad = dscale(mt / (2.0 * vj_t), daddc(dmul(vd, vd), -(vfc * vfc)));
+            qj = dscale(cjo_t, dadd(dc(f1), dscale(1.0 / f2, dadd(lin, quad))));
+        }
+    }
+    qj = dadd(qj, dscale(0.0, vd));  // + ccox*vd, no plate caps at level 1
+    if (p[DIOP_TT] > 0.0) qj = dadd(qj, dscale(p[DIOP_TT], ij));
+    o.q[1] = -qj.v;
+    o.q[2] = qj.v;
+    for (int k = 0; k < 3; k++) {
+        o.cjac[1][k] = -qj.d[k];
+        o.cjac[2][k] = qj.d[k];
     }
 }
 
+// pnjlim on V(ai, c), written back as ai's image (reads c and ai).
 template <class P>
 ESP_HD int limit_diode(const P &p, const double *cur, const double *old, double *lim)
 {
     int check = 0;
-    double vd = pnjlim(cur[0] - cur[1], old[0] - old[1], p[DIOP_VTE], p[DIOP_VCRIT], &check);
-    lim[0] = cur[1] + vd;
+    double vd = pnjlim(cur[2] - cur[1], old[2] - old[1], p[DIOP_VTE], p[DIOP_VCRIT], &check);
+    lim[0] = cur[0];
     lim[1] = cur[1];
-    snap_limited(2, 0x3u, 0x1u, lim, cur);
+    lim[2] = cur[1] + vd;
+    snap_limited(3, 0x6u, 0x4u, lim, cur);
     return check;
 }
 
 template <class P>
 ESP_HD void seed_diode(const P &p, const double *cur, double *lim)
 {
-    lim[0] = cur[1] + p[DIOP_VCRIT];
+    lim[0] = cur[0];
     lim[1] = cur[1];
+    lim[2] = cur[1] + p[DIOP_VCRIT];
 }
 
 // Evaluation of one instance of `kind` at local voltages lx. In a
@@ -660,7 +731,7 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tr
             o.cjac[1][1] = p[0];
         }
         break;
-    case K_DIODE: eval_diode(p, lx, o); break;
+    case K_DIODE: eval_diode(p, lx, o, tran); break;
     case K_MOS1: eval_mos1(p, lx, o); break;
     default: break;
     }

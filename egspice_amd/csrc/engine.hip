@@ -209,13 +209,14 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     }
     for (uint32_t j = tid; j < n_tape; j += nt) ti_prev[j] = 0.0;
     __syncthreads();
-    // the .tran card's PULSE defaults (frontend/builder.zig:845-877)
+    // the .tran card's PULSE and SIN defaults (frontend/builder.zig:845-877)
     for (uint32_t e = tid; e < W.n_inst; e += nt) {
         const InstEntry ie = W.inst[e];
         bpc[2 * e] = inf_();
         bpc[2 * e + 1] = inf_();
         if (ie.kind != K_VSRC) continue;
         double *p = prm_lds + ie.prm;
+        if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / to.t_stop;
         if (p[SRCP_WAVE] != 1.0) {
             p[SRCP_TD] = 1e30;
         } else {
@@ -769,6 +770,12 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         for (uint32_t i = 0; i < cnt; i++)
             for (size_t l = 0; l < N; l++) {
                 for (int k = 0; k < uk; k++) u[k] = L->uprm[kind][((size_t)i * uk + k) * N + l];
+                // a diode's ai row is the template's topology: RS may move
+                // per lane but not across zero (diode.va:364-365)
+                if (kind == K_DIODE) {
+                    const uint32_t *nd = &L->t->nodes[K_DIODE][(size_t)i * 3];
+                    if ((u[DIO_RS] > 0.0) != (nd[2] != nd[0])) return fail(ESPICE_INVALID_ARGUMENT, "DiodeRsTopology");
+                }
                 prep_kind(kind, u, p);
                 for (int k = 0; k < pk; k++) host[((size_t)i * pk + k) * N + l] = p[k];
             }
@@ -839,25 +846,25 @@ espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *
     return ESPICE_OK;
 }
 
+static_assert(KIND_UK[K_VSRC] == ESPICE_LANES_VSOURCE_PARAMS && KIND_UK[K_ISRC] == ESPICE_LANES_ISOURCE_PARAMS &&
+                  KIND_UK[K_RES] == ESPICE_LANES_RESISTOR_PARAMS && KIND_UK[K_CAP] == ESPICE_LANES_CAPACITOR_PARAMS &&
+                  KIND_UK[K_DIODE] == ESPICE_LANES_DIODE_PARAMS && KIND_UK[K_MOS1] == ESPICE_LANES_MOS1_PARAMS,
+              "user parameter counts match include/espice_lanes.h");
+static_assert(DIO_RS == ESPICE_LANES_DIODE_RS, "diode RS slot matches include/espice_lanes.h");
 static_assert(sizeof(espice_tran_opts) == sizeof(TranOpts), "espice_tran_opts mirrors TranOpts");
 static_assert(sizeof(espice_meas_pass) == sizeof(MeasPass), "espice_meas_pass mirrors MeasPass");
 static_assert(sizeof(espice_meas_result) == sizeof(MeasOut), "espice_meas_result mirrors MeasOut");
 static_assert(sizeof(espice_tran_result) == sizeof(TranOut), "espice_tran_result mirrors TranOut");
 
-// Whether every diode and MOSFET of every lane has zero charge: diode.va
-// charges only with CJO or TT (:334,363); mos1.va only with an oxide (TOX
-// given and > 0, :164), overlaps CGSO/CGDO/CGBO, or junction capacitance
-// (CBD/CBS, or CJ and CJSW with nonzero AD/AS/PD/PS, :253-255).
+// Whether every MOSFET of every lane has zero charge: mos1.va charges only
+// with an oxide (TOX given and > 0, :164), overlaps CGSO/CGDO/CGBO, or
+// junction capacitance (CBD/CBS, or CJ and CJSW with nonzero AD/AS/PD/PS,
+// :253-255).
 static bool tran_charges_zero(const espice_lanes &L)
 {
     auto zero = [](double v) { return v != v || v == 0.0; };
     auto none = [](double v) { return v != v || v <= 0.0; };
     const size_t N = L.N;
-    for (uint32_t i = 0; i < L.t->count[K_DIODE]; i++)
-        for (size_t l = 0; l < N; l++) {
-            const double *u = &L.uprm[K_DIODE][(size_t)i * KIND_UK[K_DIODE] * N + l];
-            if (!zero(u[2 * N]) || !zero(u[6 * N])) return false;  // CJO, TT
-        }
     for (uint32_t i = 0; i < L.t->count[K_MOS1]; i++)
         for (size_t l = 0; l < N; l++) {
             const double *u = &L.uprm[K_MOS1][(size_t)i * KIND_UK[K_MOS1] * N + l];
@@ -884,8 +891,8 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
         return fail(ESPICE_INVALID_ARGUMENT, "InvalidTran");
     for (uint32_t k = 0; k < n_passes; k++)
         if (passes[k].kind > 3 || passes[k].row >= n) return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
-    // The junction and Meyer charges of diode.va and mos1.va are not staged
-    // yet: a lane whose diodes or MOSFETs would carry charge is refused
+    // The Meyer and junction charges of mos1.va are not staged yet: a lane
+    // whose MOSFETs would carry charge is refused
     // rather than run as static; with every such parameter zero (or not
     // given) their charges are identically zero and the lane is exact.
     if (!tran_charges_zero(*L)) return fail(ESPICE_FAILED, "TranChargesUnsupported");

@@ -685,8 +685,10 @@ std::string problem_report(espice_problem *problem)
     out += d.title + ": " + std::to_string(d.elements.size()) + " devices\n";
     if (d.op) out += "  Operating Point: " + std::string(pt.failed ? "failed" : "done") + "\n";
     if (d.tran)
-        out += "  Transient Analysis: " + std::to_string(pt.tran_points) + " points, " + std::to_string(P->bd.n) +
-               " variables\n";
+        // the vectors are time, every named node and every V source's
+        // branch current; device-internal rows are not output variables
+        out += "  Transient Analysis: " + std::to_string(pt.tran_points) + " points, " +
+               std::to_string(1 + P->bd.rows.size() + P->bd.vsrc_branch.size()) + " variables\n";
     for (const std::string &w : d.ignored) out += "note: " + w + " is not run by the lane engine\n";
     return out;
 }

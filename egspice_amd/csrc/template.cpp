@@ -28,7 +28,14 @@ int Template::add(int kind, const uint32_t *ports, const double *params, uint32_
     for (int i = 0; i < n_order; i++) seen = seen || order[i] == kind;
     if (!seen) order[n_order++] = kind;
     for (int p = 0; p < KIND_NPORT[kind]; p++) nodes[kind].push_back(ports[p]);
+    // Internal unknowns take fresh rows in unknown order unless the kind's
+    // collapse maps them onto a port (frontend/builder.zig:258-263): a
+    // diode's ai is its anode when RS is not positive (diode.va:364-371).
     for (int u = KIND_NPORT[kind]; u < KIND_NU[kind]; u++) {
+        if (kind == K_DIODE && !(params[DIO_RS] > 0.0)) {
+            nodes[kind].push_back(ports[0]);
+            continue;
+        }
         nodes[kind].push_back(n++);
         cur_flag.push_back(kind == K_VSRC ? 1 : 0);
     }
@@ -176,7 +183,7 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
             for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(kind, ru) != 0;
             e.stage_q = ESP_NONE;
             e.stage_c = ESP_NONE;
-            e.tape = kind == K_CAP ? tape++ : ESP_NONE;
+            e.tape = KIND_TAPE_ROW[kind] >= 0 ? tape++ : ESP_NONE;
             if (charged) {
                 e.stage_q = stage;
                 stage += nu;

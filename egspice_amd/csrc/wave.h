@@ -254,7 +254,12 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
     }
     DevOut o;
     eval_kind(KIND, p, lx, o, TRAN, t);
+    bool charged = false;
+#pragma unroll
+    for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(KIND, ru) != 0;
     if (use_lim) {
+        // x minus its limited image, dotted with each row's gradient, on the
+        // residual and (transient) on the charges (eval.zig:825-845, 893-897)
         double corr[nu];
         bool live = false;
 #pragma unroll
@@ -270,6 +275,16 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
                 for (int u = 0; u < nu; u++) acc += o.jac[ru][u] * corr[u];
                 o.res[ru] += acc;
             }
+            if (TRAN && charged) {
+#pragma unroll
+                for (int ru = 0; ru < nu; ru++) {
+                    if (kind_cmask(KIND, ru) == 0) continue;
+                    double acc = 0;
+#pragma unroll
+                    for (int u = 0; u < nu; u++) acc += o.cjac[ru][u] * corr[u];
+                    o.q[ru] += acc;
+                }
+            }
         }
     }
     uint32_t k = 0;
@@ -281,9 +296,6 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
         for (int cu = 0; cu < nu; cu++)
             if (gm >> cu & 1) s.stage[ie.stage_jac + k++] = o.jac[ru][cu];
     }
-    bool charged = false;
-#pragma unroll
-    for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(KIND, ru) != 0;
     if (TRAN && charged) {
         uint32_t kc = 0;
 #pragma unroll
@@ -294,7 +306,7 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
             for (int cu = 0; cu < nu; cu++)
                 if (cm >> cu & 1) s.stage[ie.stage_c + kc++] = o.cjac[ru][cu];
         }
-        if (ie.tape != ESP_NONE) s.tape[ie.tape] = o.q[0];
+        if (ie.tape != ESP_NONE) s.tape[ie.tape] = o.q[KIND_TAPE_ROW[KIND] < 0 ? 0 : KIND_TAPE_ROW[KIND]];
     }
 }
 

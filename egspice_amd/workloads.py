@@ -5,9 +5,10 @@ give, so the engine's template and any other builder with the same
 """
 import numpy as np
 
-from . import CAPACITOR, MOS1, MOS1_PARAMS, RESISTOR, VSOURCE, VSOURCE_DC
+from . import CAPACITOR, DIODE, DIODE_PARAMS, MOS1, MOS1_PARAMS, RESISTOR, VSOURCE, VSOURCE_DC
 
 M = MOS1_PARAMS
+D = DIODE_PARAMS
 
 # 5-stage CMOS inverter chain, mos1 (LEVEL=1): NMOS KP 1e-4, VTO 0.7,
 # W/L 2u/1u; PMOS KP 4e-5, VTO -0.7, W/L 4u/1u; VDD 1.8 V.
@@ -43,6 +44,13 @@ def pulse_params(v1, v2, td, tr, tf, pw, per, dc=None):
     """V source parameters of `PULSE(v1 v2 td tr tf pw per)` (models/vsource.va
     order: dc, wave, v1, v2, td, tr, tf, pw, per); dc None reads as not given."""
     return {0: np.nan if dc is None else dc, 1: 1.0, 2: v1, 3: v2, 4: td, 5: tr, 6: tf, 7: pw, 8: per}
+
+
+def sin_params(vo, va, freq=None, td=None, theta=None, phase=None, dc=None):
+    """V source parameters of `SIN(vo va freq td theta phase)` (models/vsource.va
+    slots 9-14); arguments left None read as not given."""
+    g = lambda v: np.nan if v is None else v
+    return {0: g(dc), 1: 2.0, 9: vo, 10: va, 11: g(freq), 12: g(td), 13: g(theta), 14: g(phase)}
 
 
 # docs/examples/measure.sp: V1 in 0 PULSE(0 1 1u 1n 1n 5u 20u); R1 in out 1k;
@@ -97,3 +105,23 @@ def cmos_inverter(add, vdd=1.8, wn=1e-6, cl=10e-15):
     m1 = add(MOS1, ["out", "in", "0", "0"], {**INV_NCH, M["W"]: wn, M["L"]: 180e-9})
     m2 = add(MOS1, ["out", "in", "vdd", "vdd"], {**INV_PCH, M["W"]: 2 * wn, M["L"]: 180e-9})
     return vin, c, m1, m2
+
+
+DIODE_RECT = {D["IS"]: 1e-14, D["N"]: 1.05, D["RS"]: 0.5, D["CJO"]: 10e-12}
+
+
+def diode_rectifier(add, rload=1e3, cres=100e-6, dmodel=None):
+    """docs/examples/diode_rectifier.sp in builder order (v, c, d, r):
+
+        V1 in 0 SIN(0 10 50)
+        C1 out 0 {cres}
+        D1 in out dmod      .model dmod D(IS=1e-14 N=1.05 RS=0.5 CJO=10p)
+        R1 out 0 {rload}
+
+    The diode's RS puts its junction on an internal row. Returns the
+    instance indices (v1, c1, d1, r1) per kind."""
+    v = add(VSOURCE, ["in", "0"], sin_params(0.0, 10.0, 50.0))
+    c = add(CAPACITOR, ["out", "0"], {0: cres})
+    d = add(DIODE, ["in", "out"], dict(DIODE_RECT if dmodel is None else dmodel))
+    r = add(RESISTOR, ["out", "0"], {0: rload})
+    return v, c, d, r

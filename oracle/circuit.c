@@ -40,7 +40,8 @@
 
 int or_solver_is_tri(const or_solver *s);
 
-static const int NU[OR_NKIND] = {3, 2, 2, 2, 2, 4};
+/* A diode's unknowns are (a, c, ai); ai is a's own row when RS is 0. */
+static const int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4};
 static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
 static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
 static const int PK[OR_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
@@ -51,7 +52,8 @@ static unsigned gmask(int kind, int ru)
 {
     switch (kind) {
     case OR_VSRC: return ru == 2 ? 0x3u : 0x4u;
-    case OR_RES: case OR_DIODE: return 0x3u;
+    case OR_RES: return 0x3u;
+    case OR_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case OR_MOS1: return ru == 1 ? 0x0u : 0xfu;
     default: return 0;
     }
@@ -60,13 +62,16 @@ static unsigned cmask(int kind, int ru)
 {
     switch (kind) {
     case OR_CAP: return 0x3u;
+    case OR_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case OR_MOS1: return 0xfu;
     default: (void)ru; return 0;
     }
 }
 static unsigned pmask(int kind, int ru) { return gmask(kind, ru) | cmask(kind, ru); }
 /* Residual rows a kind writes in a static evaluation. */
-static const unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x3u, 0xdu};
+static const unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
+/* Unknowns each kind's limiter writes. */
+static const unsigned LIMW[OR_NKIND] = {0, 0, 0, 0, 0x4u, 0xbu};
 
 typedef struct { uint32_t *p; size_t len, cap; } vu32;
 typedef struct { double *p; size_t len, cap; } vf64;
@@ -183,7 +188,12 @@ int32_t or_ckt_add(or_circuit *c, int kind, const uint32_t *ports, const double 
     for (int i = 0; i < c->n_order; i++) seen |= c->order[i] == kind;
     if (!seen) c->order[c->n_order++] = kind;
     for (int p = 0; p < NPORT[kind]; p++) vpush(&c->nodes[kind], ports[p]);
-    for (int u = NPORT[kind]; u < NU[kind]; u++) vpush(&c->nodes[kind], add_row(c, kind == OR_VSRC));
+    /* internal rows fresh, in unknown order, except a diode's ai, which is
+     * its anode unless RS > 0 (builder.zig:258-263, diode.va:364-371) */
+    for (int u = NPORT[kind]; u < NU[kind]; u++) {
+        if (kind == OR_DIODE && !(params && params[OR_DIO_RS] > 0.0)) vpush(&c->nodes[kind], ports[0]);
+        else vpush(&c->nodes[kind], add_row(c, kind == OR_VSRC));
+    }
     for (int k = 0; k < UK[kind]; k++) fpush(&c->uprm[kind], params ? params[k] : NAN);
     return (int32_t)c->count[kind]++;
 }
@@ -269,7 +279,7 @@ const uint32_t *or_ckt_diag(const or_circuit *c) { return c->diag; }
 const uint8_t *or_ckt_cur_row(const or_circuit *c) { return c->cur_row; }
 uint32_t or_ckt_count(const or_circuit *c, int kind) { return c->count[kind]; }
 int or_prepared_count(int kind) { return PK[kind]; }
-uint32_t or_tape_len(const or_circuit *c) { return c->count[OR_CAP]; }
+uint32_t or_tape_len(const or_circuit *c) { return c->count[OR_CAP] + c->count[OR_DIODE]; }
 
 /* ------------------------------------------------------------------ lanes */
 
@@ -319,8 +329,16 @@ void or_lanes_prepare(or_lanes *L)
                 case OR_DIODE: or_prep_diode(u, p); break;
                 case OR_VSRC: {
                     /* vsource.va parameter defaults; NaN = not given */
-                    const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0};
+                    const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0,
+                                                 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                     for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+                    /* dc not given: the waveform's t = 0 value, as the builder
+                     * fills it (models/vsource.va:216-221) */
+                    if (u[OR_VSRC_DC] != u[OR_VSRC_DC]) {
+                        if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
+                        else if (p[SRCP_WAVE] == 2.0)
+                            p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * om_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+                    }
                     break;
                 }
                 case OR_ISRC: p[0] = u[OR_ISRC_DC] == u[OR_ISRC_DC] ? u[OR_ISRC_DC] : 0.0; break;
@@ -368,7 +386,7 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
             int has_lim = kind == OR_MOS1 || kind == OR_DIODE;
             if (has_lim && lim_active) {
                 const double *lm = lim_at(L, kind, id, l);
-                unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+                unsigned writes = LIMW[kind];
                 for (int u = 0; u < nu; u++)
                     if (writes >> u & 1) lx[u] = lm[u];
             }
@@ -408,7 +426,7 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
                 if (ec && ec->tape) ec->tape[tape_at++] = qv;
                 break;
             }
-            case OR_DIODE: or_eval_diode(p, lx, &o); break;
+            case OR_DIODE: or_eval_diode(p, lx, &o, ec && ec->tran); break;
             case OR_MOS1: or_eval_mos1(p, lx, &o); break;
             }
             /* limiting correction: F += J (x - x_lim) on every row */
@@ -424,8 +442,16 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
                         double acc = 0;
                         for (int u = 0; u < nu; u++) acc += o.jac[ru][u] * corr[u];
                         o.res[ru] += acc;
+                        /* the charges take the same correction (eval.zig:893-897) */
+                        if (cmask(kind, ru) != 0) {
+                            double qa = 0;
+                            for (int u = 0; u < nu; u++) qa += o.cjac[ru][u] * corr[u];
+                            o.q[ru] += qa;
+                        }
                     }
             }
+            /* a diode's LTE site: its junction charge, row ai */
+            if (kind == OR_DIODE && ec && ec->tape) ec->tape[tape_at++] = o.q[2];
             size_t k = 0;
             for (int ru = 0; ru < nu; ru++) {
                 uint32_t row = nd[ru] == 0 ? c->n : nd[ru];
@@ -455,7 +481,7 @@ int or_limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old,
         int kind = c->order[t];
         if (kind != OR_MOS1 && kind != OR_DIODE) continue;
         const int nu = NU[kind];
-        unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+        unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             double p[64], cur[4], old[4], lim[4];
@@ -481,7 +507,7 @@ void or_seed_pass(or_lanes *L, uint32_t l, const double *x)
         int kind = c->order[t];
         if (kind != OR_MOS1 && kind != OR_DIODE) continue;
         const int nu = NU[kind];
-        unsigned writes = kind == OR_MOS1 ? 0xbu : 0x1u;
+        unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             double p[64], cur[4], lim[4];

@@ -19,7 +19,8 @@
  *   vsource.va  DC and PULSE (waveforms 0 and 1), branch row p,n,br
  *   isource.va  DC
  *   resistor.va, capacitor.va
- *   diode.va    rs = 0 (collapsed), no breakdown/ikf/ikr/recombination
+ *   diode.va    series RS (or collapsed), junction charge; no
+ *               breakdown/ikf/ikr/recombination/sidewall
  *   mos1.va     rd = rs = 0 (di/si collapsed onto d/s), static currents,
  *               bulk junctions; Meyer gate charge is not restated yet
  * Parity pinned by: the docs golden DC vector of docs/examples/diode_iv.sp
@@ -344,7 +345,9 @@ void or_prep_diode(const double *u, double *p)
     double egp = given(u[OR_DIO_EG], 1.11);
     double egfet = 1.16 - (7.02e-4 * tk * tk) / (tk + 1108.0);
     double egfet1 = 1.16 - (7.02e-4 * tnomk * tnomk) / (tnomk + 1108.0);
-    double mt = m * (1.0 + 0.0 * dtk + 0.0 * dtk * dtk);
+    const double tm1 = given(u[OR_DIO_TM1], 0.0), tm2 = given(u[OR_DIO_TM2], 0.0);
+    const double rs = given(u[OR_DIO_RS], 0.0), trs = given(u[OR_DIO_TRS], 0.0), trs2 = given(u[OR_DIO_TRS2], 0.0);
+    double mt = m * (1.0 + tm1 * dtk + tm2 * dtk * dtk);
     double fact1 = tnomk / 300.15, fact2 = tk / 300.15;
     double arg1 = -egfet1 / (2.0 * KB * tnomk) + 1.1150877 / (2.0 * KB * 300.15);
     double argt = -egfet / (2.0 * KB * tk) + 1.1150877 / (2.0 * KB * 300.15);
@@ -368,6 +371,11 @@ void or_prep_diode(const double *u, double *p)
     p[DIOP_MT] = mt;
     p[DIOP_FC] = fc;
     p[DIOP_TT] = tt;
+    p[DIOP_QON] = cjo > 0.0;
+    p[DIOP_RSON] = rs > 0.0;
+    /* diode.va:259-262, RS(T) floored at 1 nOhm */
+    double rs_t = rs * (1.0 + trs * dtk + trs2 * dtk * dtk) / ae;
+    p[DIOP_RST] = rs_t < 1.0e-9 ? 1.0e-9 : rs_t;
     (void)dtemp;
 }
 
@@ -502,11 +510,15 @@ void or_seed_mos1(const double *p, const double *cur, double *lim)
     lim[3] = s + type * -1.0;
 }
 
-/* diode.va static: I(a, c) = id + gmin vd, rs = 0. */
-void or_eval_diode(const double *p, const double *lx, or_devout *o)
+/* diode.va:282-372 without ikf/ikr/isr/bv/sidewall. Unknowns a, c, ai (ai
+ * is a's row when RS is 0). Junction across (ai, c) at the limited vd:
+ * I = ihi + gmin vd into ai, out of c; with RS on, I(a, ai) = V(a, ai) / RS(T).
+ * tran: the junction charge qj (depletion with the fc continuation, plus
+ * tt * I) on rows ai (+) and c (-). */
+void or_eval_diode(const double *p, const double *lx, or_devout *o, int tran)
 {
-    dual a = dvar(lx[0], 0), c = dvar(lx[1], 1);
-    dual vd = dsub(a, c);
+    dual a = dvar(lx[0], 0), c = dvar(lx[1], 1), ai = dvar(lx[2], 2);
+    dual vd = dsub(ai, c);
     const double vte = p[DIOP_VTE], is_t = p[DIOP_IS];
     dual id;
     if (vd.v >= -3.0 * vte) {
@@ -517,29 +529,63 @@ void or_eval_diode(const double *p, const double *lx, or_devout *o)
         dual t = dpowc(ddiv(dc(3.0 * vte), dscale(2.718281828459045, vd)), 3.0);
         id = dscale(-is_t, dadd(dc(1.0), t));
     }
-    dual i = dadd(id, dscale(p[DIOP_GMIN], vd));
-    o->res[0] = i.v;
-    o->res[1] = -i.v;
-    for (int k = 0; k < 2; k++) {
-        o->jac[0][k] = i.d[k];
-        o->jac[1][k] = -i.d[k];
+    dual ij = dadd(id, dscale(p[DIOP_GMIN], vd));
+    dual ir = dc(0.0);
+    if (p[DIOP_RSON] != 0.0) ir = ddivs(dsub(a, ai), p[DIOP_RST]);
+    dual rai = dsub(ij, ir);
+    o->res[0] = ir.v;
+    o->res[1] = -ij.v;
+    o->res[2] = rai.v;
+    for (int k = 0; k < 3; k++) {
+        o->jac[0][k] = ir.d[k];
+        o->jac[1][k] = -ij.d[k];
+        o->jac[2][k] = rai.d[k];
+    }
+    if (!tran) return;
+    dual qj = dc(0.0);
+    if (p[DIOP_QON] != 0.0) {
+        const double cj = p[DIOP_CJO], vj = p[DIOP_VJ], mt = p[DIOP_MT], fc = p[DIOP_FC];
+        const double vfc = fc * vj;
+        if (vd.v < vfc) {
+            /* cjo(T) vj(T) (1 - (1 - vd/vj)^(1-M)) / (1 - M) */
+            dual base = dcsub(1.0, ddivs(vd, vj));
+            qj = ddivs(dscale(cj * vj, dcsub(1.0, dpowc(base, 1.0 - mt))), 1.0 - mt);
+        } else {
+            double f1 = vj * (1.0 - om_pow(1.0 - fc, 1.0 - mt)) / (1.0 - mt);
+            double f2 = om_pow(1.0 - fc, 1.0 + mt);
+            double f3 = 1.0 - fc * (1.0 + mt);
+            dual t1 = dscale(f3, daddc(vd, -vfc));
+            dual t2 = dscale(mt / (2.0 * vj), daddc(dmul(vd, vd), -(vfc * vfc)));
+            qj = dscale(cj, dadd(dc(f1), dscale(1.0 / f2, dadd(t1, t2))));
+        }
+    }
+    qj = dadd(qj, dscale(0.0, vd)); /* ccox vd, ccox = 0 below level 3 */
+    if (p[DIOP_TT] > 0.0) qj = dadd(qj, dscale(p[DIOP_TT], ij));
+    o->q[1] = -qj.v;
+    o->q[2] = qj.v;
+    for (int k = 0; k < 3; k++) {
+        o->cjac[1][k] = -qj.d[k];
+        o->cjac[2][k] = qj.d[k];
     }
 }
 
+/* $limit(V(ai, c), "pnjlim"): reads c and ai, writes ai. */
 int or_limit_diode(const double *p, const double *cur, const double *old, double *lim)
 {
     int check = 0;
-    double vd = or_pnjlim(cur[0] - cur[1], old[0] - old[1], p[DIOP_VTE], p[DIOP_VCRIT], &check);
-    lim[0] = cur[1] + vd;
+    double vd = or_pnjlim(cur[2] - cur[1], old[2] - old[1], p[DIOP_VTE], p[DIOP_VCRIT], &check);
+    lim[0] = cur[0];
     lim[1] = cur[1];
-    snap(2, 0x3, 0x1, lim, cur);
+    lim[2] = cur[1] + vd;
+    snap(3, 0x6, 0x4, lim, cur);
     return check;
 }
 
 void or_seed_diode(const double *p, const double *cur, double *lim)
 {
-    lim[0] = cur[1] + p[DIOP_VCRIT];
+    lim[0] = cur[0];
     lim[1] = cur[1];
+    lim[2] = cur[1] + p[DIOP_VCRIT];
 }
 
 /* ---------------------------------------------------------------- sources */
@@ -558,6 +604,13 @@ static void pulse_widths(const double *p, double *tr, double *tf, double *pw, do
 /* models/vsource.va:226-247: DC, or PULSE at t (PHASE 0). */
 double or_source_value(const double *p, double t)
 {
+    if (p[SRCP_WAVE] == 2.0) {
+        /* SIN, models/vsource.va:233-244 */
+        const double t_sin = t - p[SRCP_STD];
+        if (t_sin < 0.0) return p[SRCP_SVO] + p[SRCP_SVA] * om_sin(p[SRCP_SPHASE] * VA_PI / 180.0);
+        return p[SRCP_SVO] + p[SRCP_SVA] * om_sin(VA_TWO_PI * p[SRCP_SFREQ] * t_sin + p[SRCP_SPHASE] * VA_PI / 180.0) *
+                                 om_exp(om_min(-p[SRCP_STHETA] * t_sin, 80.0));
+    }
     if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
     double tr, tf, pw, per;
     pulse_widths(p, &tr, &tf, &pw, &per);

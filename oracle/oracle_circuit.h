@@ -12,12 +12,14 @@
 enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_NKIND = 6 };
 
 /* User parameter slots per kind (NaN = not given, the default applies). */
-enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC_TR, OR_VSRC_TF, OR_VSRC_PW, OR_VSRC_PER, OR_VSRC_K };
+enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC_TR, OR_VSRC_TF, OR_VSRC_PW, OR_VSRC_PER,
+       OR_VSRC_SVO, OR_VSRC_SVA, OR_VSRC_SFREQ, OR_VSRC_STD, OR_VSRC_STHETA, OR_VSRC_SPHASE, OR_VSRC_K };
 enum { OR_ISRC_DC = 0, OR_ISRC_K };
 enum { OR_RES_R = 0, OR_RES_K };
 enum { OR_CAP_C = 0, OR_CAP_K };
 enum { OR_DIO_IS = 0, OR_DIO_N, OR_DIO_CJO, OR_DIO_VJ, OR_DIO_M, OR_DIO_FC, OR_DIO_TT, OR_DIO_GMIN, OR_DIO_AREA,
-       OR_DIO_TNOM, OR_DIO_TEMP, OR_DIO_DTEMP, OR_DIO_EG, OR_DIO_XTI, OR_DIO_K };
+       OR_DIO_TNOM, OR_DIO_TEMP, OR_DIO_DTEMP, OR_DIO_EG, OR_DIO_XTI, OR_DIO_RS, OR_DIO_TRS, OR_DIO_TRS2, OR_DIO_TM1,
+       OR_DIO_TM2, OR_DIO_K };
 enum { OR_M1_TYPE = 0, OR_M1_VTO, OR_M1_KP, OR_M1_GAMMA, OR_M1_PHI, OR_M1_LAMBDA, OR_M1_TOX, OR_M1_LD, OR_M1_U0,
        OR_M1_NSUB, OR_M1_NSS, OR_M1_TPG, OR_M1_IS, OR_M1_JS, OR_M1_CBD, OR_M1_CBS, OR_M1_CJ, OR_M1_CJSW, OR_M1_PB,
        OR_M1_MJ, OR_M1_MJSW, OR_M1_FC, OR_M1_CGSO, OR_M1_CGDO, OR_M1_CGBO, OR_M1_TNOM, OR_M1_GMIN, OR_M1_TEMP,

@@ -24,22 +24,27 @@ enum { M1P_TYPE = 0, M1P_VT, M1P_VBIT, M1P_PHIT, M1P_GAMMA, M1P_BETA, M1P_LAMBDA
        M1P_VCRITS, M1P_VCRITD, M1P_COX, M1P_CZBD, M1P_CZBS, M1P_CZBDSW, M1P_CZBSSW, M1P_PBT, M1P_MJ, M1P_MJSW,
        M1P_FC, M1P_F2D, M1P_F3D, M1P_F4D, M1P_F2S, M1P_F3S, M1P_F4S, M1P_CGSO, M1P_CGDO, M1P_CGBO, M1P_SEED, M1P_K };
 /* Prepared diode constants. */
-enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT, DIOP_FC, DIOP_TT, DIOP_K };
+enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT, DIOP_FC, DIOP_TT, DIOP_QON, DIOP_RSON,
+       DIOP_RST, DIOP_K };
 
 void or_prep_mos1(const double *u, double *p);
 void or_prep_diode(const double *u, double *p);
 void or_eval_mos1(const double *p, const double *lx, or_devout *o);
-void or_eval_diode(const double *p, const double *lx, or_devout *o);
+void or_eval_diode(const double *p, const double *lx, or_devout *o, int tran);
 int or_limit_mos1(const double *p, const double *cur, const double *old, double *lim);
 int or_limit_diode(const double *p, const double *cur, const double *old, double *lim);
 void or_seed_mos1(const double *p, const double *cur, double *lim);
 void or_seed_diode(const double *p, const double *cur, double *lim);
 /* Prepared V/I source constants: the DC value, then the waveform (0 DC,
- * 1 PULSE) and PULSE(V1 V2 TD TR TF PW PER) with the -1 sentinels of
- * models/vsource.va:58-66 for TR, TF, PW and PER not given. */
-enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_K };
+ * 1 PULSE, 2 SIN), PULSE(V1 V2 TD TR TF PW PER) with the -1 sentinels of
+ * models/vsource.va:58-66 for TR, TF, PW and PER not given, and SIN(VO VA
+ * FREQ TD THETA PHASE). */
+enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
+       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_K };
 /* The source's value at transient time t (models/vsource.va:233-247). */
 double or_source_value(const double *p, double t);
+#define VA_PI 3.14159265358979323846     /* constants.vams M_PI */
+#define VA_TWO_PI 6.28318530717958647652 /* M_TWO_PI */
 /* ngspice's PULSE breakpoint after tq, requested at the accepted time
  * t_acc (src/device/eval.zig:1636-1650 spicePulseBreak); inf when none. */
 double or_pulse_break(const double *p, double t_acc, double tq);
@@ -70,7 +75,7 @@ void or_seed_pass(struct or_lanes *L, uint32_t l, const double *x);
 /* Prepared constants of instance id of kind on lane l (copied into p). */
 void or_load_p(const struct or_lanes *L, int kind, uint32_t id, uint32_t l, double *p);
 int or_prepared_count(int kind);
-/* Length of the LTE charge tape: one site per capacitor. */
+/* Length of the LTE charge tape: one site per capacitor and per diode. */
 uint32_t or_tape_len(const struct or_circuit *c);
 
 double or_fetlim(double vnew, double vold, double vto);

@@ -270,6 +270,7 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     for (uint32_t i = 0; i < n_src; i++) {
         double *p = bps.prm[i];
         or_load_p(L, OR_VSRC, i, l, p);
+        if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / opt->t_stop;
         if (p[SRCP_WAVE] != 1.0) {
             p[SRCP_TD] = 1e30;
         } else {

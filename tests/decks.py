@@ -1,7 +1,7 @@
 """Input decks of the reference's documented runs (docs/examples/*.sp), the
 circuits and analysis cards as the docs give them, with the outputs the
 reference printed for them (docs/learn/measure.md:24-36, analyses.md:25-30,
-models.md:95-102)."""
+models.md:38-45, 95-102)."""
 
 MEASURE_SP = """Measurements on an RC step response
 * first-order RC, tau = 1 us; the input steps up at 1 us and down at 6 us
@@ -62,5 +62,22 @@ CMOS_INVERTER_GOLDEN = {"tphl": (2.818778e-11, 6), "tphl.targ": (1.078188e-09, 6
 CMOS_INVERTER_POINTS = 2032
 
 # measure.sp with its R and C as .param, so points can move them
+DIODE_RECTIFIER_SP = """Half-wave rectifier
+* A 50 Hz, 10 V peak sine through a diode into an RC load.
+V1 in 0 SIN(0 10 50)
+D1 in out dmod
+R1 out 0 1k
+C1 out 0 100u
+.model dmod D(IS=1e-14 N=1.05 RS=0.5 CJO=10p)
+.tran 0.1m 60m
+.meas tran vpeak MAX v(out)
+.meas tran vmin MIN v(out) FROM=40m TO=60m
+.end
+"""
+
+DIODE_RECTIFIER_GOLDEN = {"vpeak": (9.228482e+00, 6), "vpeak.at": (5.173860e-03, 6), "vmin": (7.719858e+00, 6),
+                          "vmin.at": (4.317386e-02, 6)}
+DIODE_RECTIFIER_POINTS = 610
+
 MEASURE_PARAM_SP = MEASURE_SP.replace("R1 in out 1k", ".param r=1k c=1n\nR1 in out {r}").replace(
     "C1 out 0 1n", "C1 out 0 {c}")

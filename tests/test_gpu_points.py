@@ -201,15 +201,66 @@ def test_cli_prints_the_reference_lines(tmp_path, engine):
     assert lines[-1].startswith("{") and '"trise": 2.19720' in lines[-1]
 
 
+def test_cli_diode_rectifier(tmp_path, engine):
+    say("case: egspice_amd/espice --backend=hip diode_rectifier.sp (SIN, diode RS and CJO)")
+    deck = tmp_path / "diode_rectifier.sp"
+    deck.write_text(decks.DIODE_RECTIFIER_SP)
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    r = subprocess.run([exe, "--backend=hip", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    for want in (
+        "vpeak               =  9.228482e+00 at=  5.173860e-03",
+        "vmin                =  7.719858e+00 at=  4.317386e-02",
+        "Half-wave rectifier: 4 devices",
+        "  Transient Analysis: 610 points, 4 variables",
+    ):
+        assert want in lines, (want, r.stdout)
+
+
+def test_diode_rectifier_points(oracle, engine):
+    N = 12
+    say(f"case: diode_rectifier.sp with live rs, cjo: {N} points, bitwise vs the oracle")
+    deck = decks.DIODE_RECTIFIER_SP.replace(".model dmod D(IS=1e-14 N=1.05 RS=0.5 CJO=10p)",
+                                            ".param rs=0.5 cj=10p\n.model dmod D(IS=1e-14 N=1.05 RS={rs} CJO={cj})")
+    pb = E.Problem(deck, live=("rs", "cj"))
+    rng = np.random.default_rng(4)
+    rows = np.stack([rng.uniform(0.2, 2.0, N), 10 ** rng.uniform(-12, -10, N)], axis=1)
+    first = pb.append_points(rows)
+    pb.run_all()
+    m0, r0 = pb.measures(0, 1, ["vpeak", "vmin"])
+    assert r0.tolist() == [0]
+    printed(m0[0, 0], *decks.DIODE_RECTIFIER_GOLDEN["vpeak"])
+    printed(m0[0, 1], *decks.DIODE_RECTIFIER_GOLDEN["vmin"])
+    m, reasons = pb.measures(first, N, ["vpeak", "vmin"])
+    assert (reasons == 0).all()
+    D = E.DIODE_PARAMS
+    for l in range(0, N, 3):
+        ck = O.Circuit(oracle)
+        W.diode_rectifier(ck.add, rload=sn("1k"), cres=sn("100u"),
+                          dmodel={D["IS"]: sn("1e-14"), D["N"]: sn("1.05"), D["RS"]: rows[l, 0], D["CJO"]: rows[l, 1]})
+        ck.freeze()
+        L = O.Lanes(ck, 1)
+        L.prepare()
+        t, x, st = L.tran(0, O.tran_opts(sn("60m"), sn("0.1m"), max_steps=200_000))
+        assert st.status == 0
+        y = np.ascontiguousarray(x[:, ck.node("out")])
+        v, a = C.c_double(), C.c_double()
+        oracle.or_meas_extremum(P(t), P(y), len(t), 0.0, 0.0, 1, C.byref(v), C.byref(a))
+        vpeak = v.value
+        oracle.or_meas_extremum(P(t), P(y), len(t), sn("40m"), sn("60m"), 0, C.byref(v), C.byref(a))
+        assert bits_equal(m[l], [vpeak, v.value]), l
+
+
 def test_cli_refuses_what_the_lanes_cannot_run(tmp_path, engine):
-    say("case: espice CLI errors (unsupported card, charged diode)")
+    say("case: espice CLI errors (unsupported card, charged MOSFET)")
     exe = os.path.join(ROOT, "egspice_amd", "espice")
     bad = tmp_path / "bad.sp"
     bad.write_text(decks.MEASURE_SP.replace(".end", "L1 out 0 1u\n.end"))
     r = subprocess.run([exe, "--backend=hip", str(bad)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "UnsupportedElement" in r.stderr
     charged = tmp_path / "charged.sp"
-    charged.write_text("diode with junction charge\nV1 a 0 PULSE(0 1 1n 1n 1n 5n 10n)\nR1 a b 1k\nD1 b 0 dj\n"
-                       ".model dj D(IS=1e-14 CJO=1p)\n.tran 10p 20n\n.end\n")
+    charged.write_text("mosfet with oxide charge\nV1 d 0 1.8\nV2 g 0 PULSE(0 1.8 1n 1n 1n 5n 10n)\nR1 d o 10k\n"
+                       "M1 o g 0 0 nch W=1u L=180n\n.model nch nmos (VTO=0.45 KP=200u TOX=10n)\n.tran 10p 20n\n.end\n")
     r = subprocess.run([exe, "--backend=hip", str(charged)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "TranChargesUnsupported" in r.stderr

@@ -238,19 +238,87 @@ def test_cmos_inverter_lanes_bitwise(oracle, engine):
     printed(trig, 1.050000e-09, 6)
 
 
-def test_tran_refuses_charged_models(engine):
-    say("case: a diode with junction charge is refused by the transient lanes")
+def test_diode_rectifier_lanes_bitwise(oracle, engine):
+    """docs/examples/diode_rectifier.sp: SIN source, a diode with RS (its
+    junction on an internal row) and depletion charge. Lane 0 nominal must
+    give the reference's 610 points, vpeak and vmin (docs/learn/models.md:
+    38-45); the other lanes vary RS, CJO, TT (diffusion charge), M and the
+    load."""
+    N = 24
+    say(f"case: diode_rectifier.sp, {N} lanes, tran 0.1m 60m, MAX and MIN passes")
+    t, c = build_both(oracle, W.diode_rectifier)
+    rng = np.random.default_rng(21)
+    D = E.DIODE_PARAMS
+    rs = rng.uniform(0.2, 2.0, N)
+    cjo = 10 ** rng.uniform(-12, -10, N)
+    tt = np.where(np.arange(N) % 3 == 0, 0.0, 10 ** rng.uniform(-9, -6, N))
+    mj = rng.uniform(0.3, 0.6, N)
+    rl = rng.uniform(500.0, 2000.0, N)
+    rs[0], cjo[0], tt[0], mj[0], rl[0] = 0.5, 10e-12, 0.0, 0.5, 1e3
+    vout = t.node("out")
+    passes = [E.meas_extremum(vout, 1), E.meas_extremum(vout, 0, frm=40e-3, to=60e-3)]
+    sets = {(E.DIODE, 0, D["RS"]): rs, (E.DIODE, 0, D["CJO"]): cjo, (E.DIODE, 0, D["TT"]): tt,
+            (E.DIODE, 0, D["M"]): mj, (E.RESISTOR, 0, 0): rl}
+    res, meas, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=60e-3, dt_init=0.1e-3), passes)
+    assert (res["status"] == 0).all()
+    assert res["points"][0] == 610
+    printed(meas[0, 0, 0], 9.228482e+00, 6)
+    printed(meas[0, 0, 1], 5.173860e-03, 6)
+    printed(meas[0, 1, 0], 7.719858e+00, 6)
+    printed(meas[0, 1, 1], 4.317386e-02, 6)
+
+
+def test_collapsed_diode_charge_lanes_bitwise(oracle, engine):
+    """A diode without RS (ai collapsed onto the anode) carrying depletion
+    and diffusion charge, driven by PULSE edges through a resistor: reverse
+    recovery stresses the charge's limiting correction and the LTE site."""
+    N = 16
+    say(f"case: collapsed diode with CJO and TT, {N} lanes, tran 1n 200n")
+
+    def deck(add):
+        add(E.VSOURCE, ["in", "0"], W.pulse_params(-2.0, 2.0, 10e-9, 1e-9, 1e-9, 40e-9, 100e-9))
+        add(E.DIODE, ["a", "0"], {E.DIODE_PARAMS["IS"]: 1e-14, E.DIODE_PARAMS["CJO"]: 2e-12,
+                                  E.DIODE_PARAMS["TT"]: 5e-9})
+        add(E.RESISTOR, ["in", "a"], {0: 1e3})
+
+    t, c = build_both(oracle, deck)
+    assert t.n == c.n == 4  # in, V's branch, a: no internal row
+    rng = np.random.default_rng(33)
+    D = E.DIODE_PARAMS
+    sets = {(E.DIODE, 0, D["CJO"]): 10 ** rng.uniform(-13, -11, N), (E.DIODE, 0, D["TT"]): 10 ** rng.uniform(-10, -8, N),
+            (E.DIODE, 0, D["VJ"]): rng.uniform(0.6, 0.9, N), (E.RESISTOR, 0, 0): rng.uniform(200.0, 2000.0, N)}
+    passes = [E.meas_cross(t.node("a"), 0.0, fall=1), E.meas_extremum(t.node("a"), 0)]
+    for method in (0, 1, 2):
+        res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=200e-9, dt_init=1e-9, method=method), passes)
+        assert (res["status"] == 0).all()
+
+
+def test_diode_rs_topology_is_per_template(engine):
+    say("case: a lane may not move a diode's RS across zero")
     t = E.Template()
-    t.add(E.VSOURCE, ["a", "0"], {0: 1.0})
-    t.add(E.RESISTOR, ["a", "b"], {0: 1e3})
-    t.add(E.DIODE, ["b", "0"], {0: 1e-14})
+    W.diode_rectifier(t.add)
     t.freeze()
     gl = E.Lanes(t, 2)
-    gl.set_param(E.DIODE, 0, E.DIODE_PARAMS["CJO"], [0.0, 1e-12])
+    gl.set_param(E.DIODE, 0, E.DIODE_PARAMS["RS"], [0.5, 0.0])
+    with pytest.raises(E.EngineError, match="DiodeRsTopology"):
+        gl.prepare()
+
+
+def test_tran_refuses_charged_models(engine):
+    say("case: a MOSFET with oxide charge is refused by the transient lanes")
+    M = E.MOS1_PARAMS
+    t = E.Template()
+    t.add(E.VSOURCE, ["d", "0"], {0: 1.8})
+    t.add(E.VSOURCE, ["g", "0"], {0: 1.0})
+    t.add(E.RESISTOR, ["d", "o"], {0: 1e4})
+    t.add(E.MOS1, ["o", "g", "0", "0"], {**W.INV_NCH, M["W"]: 1e-6, M["L"]: 180e-9})
+    t.freeze()
+    gl = E.Lanes(t, 2)
+    gl.set_param(E.MOS1, 0, M["TOX"], [float("nan"), 10e-9])
     gl.prepare()
     with pytest.raises(E.EngineError, match="TranChargesUnsupported"):
         gl.tran(E.tran_opts(1e-6, 1e-8))
-    gl.set_param(E.DIODE, 0, E.DIODE_PARAMS["CJO"], [0.0, 0.0])
+    gl.set_param(E.MOS1, 0, M["TOX"], [float("nan"), float("nan")])
     gl.prepare()
     res, _, _, _, _ = gl.tran(E.tran_opts(1e-6, 1e-8))
     assert (res["status"] == 0).all()

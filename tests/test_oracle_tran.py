@@ -6,6 +6,9 @@ reference's printed outputs:
   - docs/examples/cmos_inverter.sp (docs/learn/models.md:95-102, mos1):
     2032 points and tphl, trig and targ to their printed digits (its second
     period's corners come from vsource's own timers);
+  - docs/examples/diode_rectifier.sp (docs/learn/models.md:38-45): a SIN
+    source into a diode with RS (an internal row) and junction charge, 610
+    points, vpeak and vmin with their times to the printed digits;
 and by mathematics: the RC step response 1 - exp(-t/RC) within the LTE
 tolerance, and the PULSE waveform and its breakpoints at known corners."""
 import ctypes as C
@@ -157,3 +160,48 @@ def test_cmos_inverter_golden(oracle):
     printed(targ - trig, 2.818778e-11, 6)
     printed(targ, 1.078188e-09, 6)
     printed(trig, 1.050000e-09, 6)
+
+
+def rectifier_run(oracle, **kw):
+    from egspice_amd import workloads as W
+    c = O.Circuit(oracle)
+    W.diode_rectifier(c.add, **kw)
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(60e-3, 0.1e-3))
+    return c, t, x, st
+
+
+def test_diode_rectifier_golden(oracle):
+    """docs/learn/models.md:38-45: `.tran 0.1m 60m`, 610 points, 4 variables;
+    vpeak = 9.228482e+00 at 5.173860e-03 (MAX v(out)), vmin = 7.719858e+00
+    at 4.317386e-02 (MIN v(out) FROM=40m TO=60m)."""
+    c, t, x, st = rectifier_run(oracle)
+    assert c.n == 5  # ground, in, V1's branch, out, D1's internal anode
+    assert st.status == 0 and st.points == 610 == len(t)
+    tt = np.ascontiguousarray(t)
+    vout = np.ascontiguousarray(x[:, c.node("out")])
+    v, at = C.c_double(), C.c_double()
+    oracle.or_meas_extremum(P(tt), P(vout), len(tt), 0.0, 0.0, 1, C.byref(v), C.byref(at))
+    printed(v.value, 9.228482e+00, 6)
+    printed(at.value, 5.173860e-03, 6)
+    oracle.or_meas_extremum(P(tt), P(vout), len(tt), 40e-3, 60e-3, 0, C.byref(v), C.byref(at))
+    printed(v.value, 7.719858e+00, 6)
+    printed(at.value, 4.317386e-02, 6)
+
+
+def test_diode_rectifier_physics(oracle):
+    """Closed-form checks of the same run: v(in) is the SIN source at every
+    accepted time (models/vsource.va:243-249, theta = td = phase = 0), the
+    series resistor carries the diode current (V(in, ai) / RS equals the
+    source's branch current), and while the diode conducts the output sits
+    one junction drop (0.5-0.8 V at these currents) below the input."""
+    c, t, x, _ = rectifier_run(oracle)
+    vin = x[:, c.node("in")]
+    assert np.allclose(vin, 10.0 * np.sin(2 * np.pi * 50.0 * t), rtol=0, atol=1e-12)
+    ai = 4  # D1's internal anode: the row after out
+    ib = x[:, 2]  # V1's branch current, into the source's + terminal
+    assert np.allclose((vin - x[:, ai]) / 0.5, -ib, rtol=1e-9, atol=1e-12)
+    k = int(np.argmax(x[:, c.node("out")]))
+    assert 0.5 < vin[k] - x[k, c.node("out")] < 0.9
