@@ -30,7 +30,9 @@ EXPORTS = (
     "espice_lanes_template_info", "espice_lanes_template_pattern", "espice_lanes_create", "espice_lanes_destroy",
     "espice_lanes_reset", "espice_lanes_set_param", "espice_lanes_prepare", "espice_lanes_upload_x", "espice_lanes_download_x",
     "espice_lanes_newton_op", "espice_lanes_tran", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
-    "espice_lanes_mc_draw",
+    "espice_lanes_mc_draw", "espice_mlp_create", "espice_mlp_destroy", "espice_mlp_param_count",
+    "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
+    "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start",
 )
 
 
@@ -80,6 +82,15 @@ class LaneTab(C.Structure):
                                           "void_slots", "n_void_slots", "pad")]
 
 
+class MlpShape(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("n_in", "hidden", "depth", "n_out")] + [("seed", C.c_uint64)]
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32)] + \
+               [(f, C.c_float) for f in ("lr", "beta1", "beta2", "eps")] + [("seed", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -116,6 +127,16 @@ def library():
                                    C.POINTER(C.c_size_t)],
         "espice_lanes_tab_refactor_solve": [C.POINTER(LaneTab), u32p, C.c_size_t, C.c_uint32, f64p, f64p, f64p,
                                             C.POINTER(C.c_uint8), f64p],
+        "espice_mlp_create": [C.POINTER(MlpShape), C.POINTER(P)],
+        "espice_mlp_destroy": [P],
+        "espice_mlp_param_count": [P, C.POINTER(C.c_size_t)],
+        "espice_mlp_set_scaling": [P, f64p, f64p, f64p, f64p],
+        "espice_mlp_get_params": [P, C.POINTER(C.c_float)],
+        "espice_mlp_set_params": [P, C.POINTER(C.c_float)],
+        "espice_mlp_train": [P, f64p, f64p, C.c_uint32, C.POINTER(MlpTrainOpts), C.POINTER(C.c_float)],
+        "espice_mlp_gradients": [P, f64p, f64p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "espice_mlp_predict": [P, f64p, C.c_uint32, f64p],
+        "espice_lanes_warm_start": [P, P, f64p, f64p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -273,6 +294,18 @@ class Lanes:
     def newton_op(self, x0=None, **kw):
         """Static Newton on every lane; returns (x, status, iterations, kernel_ms)."""
         self.upload_x(np.zeros(self.t.n * self.N) if x0 is None else x0)
+        return self.newton(**kw)
+
+    def warm_start(self, mlp, params):
+        """The network's start point for params[N][n_in] written into the
+        lanes' x on the GPU; returns the GPU milliseconds it took."""
+        a, ap = _f64(np.asarray(params, dtype=np.float64).reshape(self.N, -1))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_warm_start(self.h, mlp.h, ap, C.byref(ms)))
+        return ms.value
+
+    def newton(self, **kw):
+        """Static Newton from the lanes' current x (no upload)."""
         o = newton_opts(**kw)
         out = (LaneResult * self.N)()
         ms = C.c_double()
@@ -513,3 +546,69 @@ def tab_refactor_solve(tab, idx, a, rhs):
     _check(lib.espice_lanes_tab_refactor_solve(C.byref(tab), ip, len(ii), N, ap, rp, dxp,
                                                fail.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(ms)))
     return dx.reshape(rhs.shape), fail, ms.value
+
+
+class Mlp:
+    """The warm-start network (include/espice_mlp.h): n_in -> depth x hidden
+    tanh -> n_out, BF16 MFMA kernels with FP32 master weights."""
+
+    def __init__(self, n_in, n_out, hidden=256, depth=3, seed=1):
+        self.lib = library()
+        self.n_in, self.n_out, self.hidden, self.depth = n_in, n_out, hidden, depth
+        h = C.c_void_p()
+        _check(self.lib.espice_mlp_create(C.byref(MlpShape(n_in, hidden, depth, n_out, seed)), C.byref(h)))
+        self.h = h
+        n = C.c_size_t()
+        _check(self.lib.espice_mlp_param_count(self.h, C.byref(n)))
+        self.param_count = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.espice_mlp_destroy(self.h)
+            self.h = None
+
+    def set_scaling(self, in_mean, in_std, out_mean, out_std):
+        arrs = [_f64(np.asarray(a, dtype=np.float64).reshape(-1)) for a in (in_mean, in_std, out_mean, out_std)]
+        assert arrs[0][0].size == arrs[1][0].size == self.n_in and arrs[2][0].size == arrs[3][0].size == self.n_out
+        _check(self.lib.espice_mlp_set_scaling(self.h, *[p for _, p in arrs]))
+
+    def fit_scaling(self, x, y):
+        """Mean and standard deviation of the training set's columns."""
+        x, y = np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64)
+        self.set_scaling(x.mean(0), x.std(0), y.mean(0), y.std(0))
+
+    def params(self):
+        out = np.zeros(self.param_count, dtype=np.float32)
+        _check(self.lib.espice_mlp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.param_count
+        _check(self.lib.espice_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def train(self, x, y, epochs=100, batch=256, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=7):
+        xa, xp = _f64(np.asarray(x, dtype=np.float64).reshape(-1, self.n_in))
+        ya, yp = _f64(np.asarray(y, dtype=np.float64).reshape(-1, self.n_out))
+        assert xa.shape[0] == ya.shape[0]
+        loss = np.zeros(epochs, dtype=np.float32)
+        o = MlpTrainOpts(epochs, batch, lr, beta1, beta2, eps, seed)
+        _check(self.lib.espice_mlp_train(self.h, xp, yp, xa.shape[0], C.byref(o),
+                                         loss.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss
+
+    def gradients(self, x, y):
+        """(grads in packed order, loss) of one pass over x, y."""
+        xa, xp = _f64(np.asarray(x, dtype=np.float64).reshape(-1, self.n_in))
+        ya, yp = _f64(np.asarray(y, dtype=np.float64).reshape(-1, self.n_out))
+        g = np.zeros(self.param_count, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.lib.espice_mlp_gradients(self.h, xp, yp, xa.shape[0], g.ctypes.data_as(C.POINTER(C.c_float)),
+                                             C.byref(loss)))
+        return g, loss.value
+
+    def predict(self, x):
+        xa, xp = _f64(np.asarray(x, dtype=np.float64).reshape(-1, self.n_in))
+        y, yp = _f64(np.zeros((xa.shape[0], self.n_out)))
+        _check(self.lib.espice_mlp_predict(self.h, xp, xa.shape[0], yp))
+        return y

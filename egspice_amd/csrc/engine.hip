@@ -35,6 +35,7 @@
 #include "wave_lu.h"
 #include "wave_newton.h"
 #include "wave_tran.h"
+#include "mlp.h"
 
 using namespace esp;
 
@@ -70,6 +71,8 @@ struct DevBuf {
 };
 
 }  // namespace
+
+void esp::set_error(const char *name) { g_error = name; }
 
 struct espice_lanes_template {
     Template t;
@@ -795,6 +798,21 @@ espice_status espice_lanes_prepare(espice_lanes *L)
     if (!L->prm_host.empty())
         HIP_TRY(hipMemcpy(L->prm_lanes.p, L->prm_host.data(), L->prm_host.size() * 8, hipMemcpyHostToDevice));
     L->prepared = true;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_warm_start(espice_lanes *L, espice_mlp *m, const double *params, double *kernel_ms)
+{
+    if (!L || !m || !params) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    HIP_TRY(hipEventRecord(L->ev0, L->stream));
+    if (espice_status e = mlp_warm_start(m, params, L->N, L->x.p, L->t->n, L->N, L->stream)) return e;
+    HIP_TRY(hipEventRecord(L->ev1, L->stream));
+    HIP_TRY(hipEventSynchronize(L->ev1));
+    if (kernel_ms) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+        *kernel_ms = ms;
+    }
     return ESPICE_OK;
 }
 
