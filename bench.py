@@ -1,12 +1,20 @@
 """Benchmark of the MI355X lane engine on its sweep workloads.
 
 One step = one full lane solve of the workload on the GPU: reset every
-lane's solver, upload the start points, run the static Newton kernel to
+lane's solver, place the start points, run the static Newton kernel to
 convergence, read back the per-lane outcomes. Prints one JSON line.
 
-Workload (the configuration this engine runs end to end today): the
-64-point DC sweep of a 5-stage mos1 CMOS inverter chain, each sweep point an
-independent cold operating point on its own lane.
+Workload: the 64-point DC sweep of a 5-stage mos1 CMOS inverter chain, each
+sweep point an independent operating point on its own lane, in two arms at
+the same Newton tolerances:
+  plain    every lane starts cold (MODEINITJCT seeds, INITFIX);
+  learned  every lane starts at the warm-start network's prediction
+           (egspice_amd/warm.py), with the nominal point and then a cold
+           start as fallbacks; the network is trained beforehand, untimed,
+           on 4096 cold operating points of other input voltages that the
+           engine solves itself.
+`value` is the learned arm's converged lanes/s; the plain arm's is in
+`config`.
 """
 import argparse
 import json
@@ -229,6 +237,9 @@ def main():
     ap.add_argument("--workload", choices=("dc-inverter-chain", "tran-rc-measure"), default="dc-inverter-chain",
                     help="dc-inverter-chain: the 64-point DC sweep (default); tran-rc-measure: the RC transient of "
                          "docs/examples/measure.sp with its .meas passes, R and C varied per lane")
+    ap.add_argument("--arm", choices=("both", "learned", "plain"), default="both",
+                    help="dc-inverter-chain: time both arms (default), or only one (for profiling one arm's "
+                         "kernels); with one arm, the other's fields are null")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the reference kernel for "
                          "scaling profiler byte counters) and exit")
@@ -258,23 +269,68 @@ def main():
     lanes.prepare()
     x0 = np.zeros(t.n * N)
 
-    def step():
+    def plain_step():
         lanes.reset()
         return lanes.newton_op(x0)
 
-    for _ in range(args.warmup):
-        step()
-    kernel_ms, iters_all, conv_all = [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        x, st, it, ms = step()
-        kernel_ms.append(ms)
-        iters_all.append(it.copy())
-        conv_all.append(int((st & 1).sum()))
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    converged = conv_all[-1]
-    value = sum(conv_all) / wall
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        kernel_ms, iters_all, conv_all = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            x, st, it, ms = step()
+            kernel_ms.append(ms)
+            iters_all.append(it.copy())
+            conv_all.append(int((st & 1).sum()))
+        wall = time.perf_counter() - t0
+        return x, kernel_ms, iters_all, conv_all, wall
+
+    if args.arm != "learned":
+        x_cold, kernel_ms, iters_all, conv_all, wall = timed(plain_step)
+        plain_value = sum(conv_all) / wall
+        plain_ms = wall * 1e3 / args.steps
+        plain_ticks = float(np.mean([it.sum() for it in iters_all]))
+    else:
+        lanes.reset()
+        x_cold = lanes.newton_op(x0)[0]
+        plain_value = plain_ms = plain_ticks = None
+
+    # learned arm: train (untimed) on cold operating points of other inputs
+    # (also for --arm plain, so both runs share the setup)
+    from egspice_amd import warm
+
+    def set_vin(ln, p):
+        ln.set_param(E.VSOURCE, 1, E.VSOURCE_DC, np.asarray(p, dtype=np.float64)[:, 0])
+
+    train_in = np.random.default_rng(2024).uniform(0.0, 1.8, (4096, 1))
+    mlp, info = warm.train_warm_start(t, set_vin, train_in, batch=256)
+    nominal, _ = warm.cold_ops(t, set_vin, np.zeros((1, 1)))
+    sweep = vin.reshape(-1, 1)
+
+    timings = []
+
+    def learned_step():
+        lanes.reset()
+        tm = {}
+        out = warm.learned_op(lanes, mlp, sweep, nominal_x=nominal[0], timing=tm)
+        timings.append(tm)
+        return out
+
+    if args.arm == "plain":
+        x_l, l_kernel_ms, l_iters, l_conv, l_wall = x_cold, kernel_ms, iters_all, conv_all, wall
+        timings = [{"newton_ms": k, "warm_ms": 0.0, "fallbacks": {}} for k in kernel_ms]
+    else:
+        x_l, l_kernel_ms, l_iters, l_conv, l_wall = timed(learned_step)
+    timings = timings[-args.steps:]
+    newton_ms = float(np.mean([tm["newton_ms"] for tm in timings]))
+    warm_ms = float(np.mean([tm["warm_ms"] for tm in timings]))
+    fallbacks = sum(sum(tm["fallbacks"].values()) for tm in timings)
+    value = sum(l_conv) / l_wall
+    ms_per_step = l_wall * 1e3 / args.steps
+    converged = l_conv[-1]
+    node_rows = [t.node(nm) for nm in ["in", "vdd"] + [f"n{k}" for k in range(1, 6)]]
+    max_dev = float(np.abs(x_l - x_cold)[node_rows].max())
 
     # roofline of the Newton kernel: algorithmic FP64 flops / kernel time
     g_eval = sum(EVAL_FLOPS[k] * t_count for k, t_count in [(0, 2), (5, 2 * 5)])
@@ -283,19 +339,30 @@ def main():
     vals = np.ones(t.nnz)
     lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
     per_tick = g_eval + g_lim + lu + 12 * t.n
-    ticks = float(np.mean([it.sum() for it in iters_all]))
+    ticks = float(np.mean([it.sum() for it in l_iters]))
     flops = per_tick * ticks
-    kms = float(np.mean(kernel_ms))
-    achieved = flops / (kms * 1e-3) / 1e12
+    kms = float(np.mean(l_kernel_ms))
+    achieved = flops / (newton_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "lanes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "mos1 5-stage inverter chain, 64-point DC sweep as cold operating points",
-                   "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz, "arm": "plain FP64 GPU lanes",
+        "config": {"workload": "mos1 5-stage inverter chain, 64-point DC sweep as independent operating points",
+                   "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz,
+                   "arm": ("plain FP64 GPU lanes (cold starts)" if args.arm == "plain" else
+                           "learned warm start (BF16 MFMA MLP 1->3x256->n) + FP64 GPU lanes"),
                    "newton": "reltol 1e-3 abstol 1e-12 vntol 1e-6 residual_tol 1e-9 itl1 100",
                    "converged_lanes": converged, "newton_ticks_per_solve": ticks,
-                   "kernel_ms_per_solve": kms},
+                   "kernel_ms_per_solve": kms, "newton_kernel_ms": newton_ms, "warm_start_ms": warm_ms,
+                   "fallback_lanes": fallbacks,
+                   "plain_lanes_per_s": plain_value, "plain_ms_per_step": plain_ms,
+                   "plain_newton_ticks_per_solve": plain_ticks,
+                   "speedup_vs_plain": value / plain_value if plain_value else None,
+                   "max_node_dev_vs_cold_V": max_dev,
+                   "training": {"samples": info["samples"], "epochs": info["epochs"], "batch": 256,
+                                "schedule": [list(e) for e in warm.SCHEDULE],
+                                "solve_s": round(info["solve_s"], 3), "train_s": round(info["train_s"], 3),
+                                "loss_first": info["loss_first"], "loss_last": info["loss_last"]}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic("newton_wave_kernel_inverter_chain_64") if N == 64 else None,

@@ -207,14 +207,25 @@ __global__ void mlp_tanh_back_kernel(const float *__restrict__ dH, const float *
     dZb[e] = (bf16)g;
 }
 
-// db[n] = sum over the rows of dZ[:, n], rows in order.
-__global__ void mlp_colsum_kernel(const float *__restrict__ dZ, uint32_t rows, uint32_t ld, float *__restrict__ db)
+// db[n] = sum over the rows of dZ[:, n]: 32 columns per block of 256
+// threads, row group g = 0..7 sums rows g, g + 8, ... and the eight partial
+// sums are added in group order (grid: ld / 32).
+__global__ __launch_bounds__(256) void mlp_colsum_kernel(const float *__restrict__ dZ, uint32_t rows, uint32_t ld,
+                                                         float *__restrict__ db)
 {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= ld) return;
+    __shared__ float part[8][32];
+    const uint32_t c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const uint32_t n = blockIdx.x * 32 + c;
     float s = 0.0f;
-    for (uint32_t m = 0; m < rows; m++) s += dZ[(size_t)m * ld + n];
-    db[n] = s;
+#pragma unroll 4
+    for (uint32_t m = g; m < rows; m += 8) s += dZ[(size_t)m * ld + n];
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0) {
+        float t = 0.0f;
+        for (int k = 0; k < 8; k++) t += part[k][c];
+        db[n] = t;
+    }
 }
 
 // Adam on count FP32 parameters (bias-corrected step sizes c1, c2 folded
@@ -375,7 +386,7 @@ espice_status backward(espice_mlp &m, uint32_t b, uint32_t rows_pad, hipStream_t
         // dW(n, k) = sum over rows r of dZ(r, n) in(r, k)
         mlp_gemm_kernel<EPI_STORE, false><<<dim3(pi / TILE, po / TILE), 64, 0, s>>>(
             m.dzb.p, 1, po, in, 1, pi, rows_pad, nullptr, m.g.p + m.woff[l], pi, nullptr);
-        mlp_colsum_kernel<<<blocks(po, 64), 64, 0, s>>>(m.dzf.p, rows_pad, po, m.g.p + m.boff[l]);
+        mlp_colsum_kernel<<<po / TILE, 256, 0, s>>>(m.dzf.p, rows_pad, po, m.g.p + m.boff[l]);
         if (l == 0) break;
         // dH(r, k) = sum_n dZ(r, n) W(n, k) = sum_n dZ(r, n) Wt(k, n)
         mlp_gemm_kernel<EPI_STORE, true><<<dim3(pi / TILE, rows_pad / TILE), 64, 0, s>>>(

@@ -33,8 +33,13 @@ def cold_ops(template, set_lane_params, params):
     return x.T.copy(), st
 
 
-def train_warm_start(template, set_lane_params, params, hidden=256, depth=3, epochs=200, batch=256, lr=1e-3,
-                     seed=1):
+# Adam's step size falls in three stages (epochs, learning rate): the
+# sweep's steep region (an inverter chain's switching point) is learned in
+# the later, finer stages.
+SCHEDULE = ((600, 1e-3), (300, 2e-4), (300, 5e-5))
+
+
+def train_warm_start(template, set_lane_params, params, hidden=256, depth=3, schedule=SCHEDULE, batch=256, seed=1):
     """Network trained on the converged cold operating points of `params`
     (rows of sweep parameters). Returns (mlp, info)."""
     t0 = time.perf_counter()
@@ -44,21 +49,27 @@ def train_warm_start(template, set_lane_params, params, hidden=256, depth=3, epo
     t1 = time.perf_counter()
     mlp = Mlp(params.shape[1], template.n, hidden=hidden, depth=depth, seed=seed)
     mlp.fit_scaling(params[ok], x[ok])
-    loss = mlp.train(params[ok], x[ok], epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    losses = [mlp.train(params[ok], x[ok], epochs=ep, batch=batch, lr=lr, seed=seed + 1 + k)
+              for k, (ep, lr) in enumerate(schedule)]
     t2 = time.perf_counter()
-    return mlp, {"samples": int(ok.sum()), "solve_s": t1 - t0, "train_s": t2 - t1, "loss_first": float(loss[0]),
-                 "loss_last": float(loss[-1])}
+    return mlp, {"samples": int(ok.sum()), "solve_s": t1 - t0, "train_s": t2 - t1,
+                 "epochs": int(sum(ep for ep, _ in schedule)), "loss_first": float(losses[0][0]),
+                 "loss_last": float(losses[-1][-1])}
 
 
-def learned_op(lanes, mlp, params, nominal_x=None, **newton_kw):
+def learned_op(lanes, mlp, params, nominal_x=None, timing=None, **newton_kw):
     """Every lane from the network's start, then the fallbacks for lanes
     that did not converge: the nominal point (when given), then cold.
-    Returns (x[n][N], status, iterations summed over the attempts, GPU ms)."""
+    Returns (x[n][N], status, iterations summed over the attempts, GPU ms);
+    `timing` (a dict) receives the GPU ms of the inference ("warm_ms") and
+    of the Newton kernels ("newton_ms") and the fallback lane counts."""
     kw = dict(newton_kw)
     kw["init_fix"] = False
-    ms = lanes.warm_start(mlp, params)
+    w_ms = lanes.warm_start(mlp, params)
     x, st, it, k_ms = lanes.newton(**kw)
-    ms += k_ms
+    ms = w_ms + k_ms
+    n_ms = k_ms
+    fell = {}
     iters = it.astype(np.int64)
     for fallback in ("nominal", "cold"):
         bad = (st & CONVERGED) == 0
@@ -74,6 +85,10 @@ def learned_op(lanes, mlp, params, nominal_x=None, **newton_kw):
             lanes.upload_x(x0.reshape(-1))
             x2, st2, it2, k_ms = lanes.newton(**dict(newton_kw, init_fix=True))
         ms += k_ms
+        n_ms += k_ms
+        fell[fallback] = int(bad.sum())
         x[:, bad], st[bad] = x2[:, bad], st2[bad]
         iters[bad] += it2[bad]
+    if timing is not None:
+        timing.update(warm_ms=w_ms, newton_ms=n_ms, fallbacks=fell)
     return x, st, iters, ms

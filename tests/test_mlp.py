@@ -154,7 +154,7 @@ def test_learned_warm_start_inverter_chain(engine):
     t = chain(64)
     rng = np.random.default_rng(11)
     train = rng.uniform(0.0, 1.8, (2048, 1))
-    net, info = warm.train_warm_start(t, set_vin, train, epochs=150, batch=256, lr=1e-3)
+    net, info = warm.train_warm_start(t, set_vin, train, batch=256)
     say(f"  training: {info}")
     assert info["loss_last"] < info["loss_first"]
     sweep = W.dc_sweep_points(64).reshape(-1, 1)
@@ -169,7 +169,7 @@ def test_learned_warm_start_inverter_chain(engine):
     x, st, it, ms = warm.learned_op(lanes, net, sweep)
     say(f"  Newton ticks: cold sum {itc.sum()} max {itc.max()}; learned sum {it.sum()} max {it.max()}; {ms:.3f} ms")
     assert (st & 1).all()
-    assert it.sum() < 0.6 * itc.sum()
+    assert it.sum() < 0.6 * itc.sum() and it.max() < itc.max()
     # the same operating points within the Newton tolerance (reltol 1e-3,
     # vntol 1e-6 on node rows)
     diff = np.abs(x - xc)
