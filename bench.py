@@ -365,7 +365,8 @@ def main():
                                 "loss_first": info["loss_first"], "loss_last": info["loss_last"]}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": profiled_traffic("newton_wave_kernel_inverter_chain_64") if N == 64 else None,
+                     "traffic": (profiled_traffic("newton_wave_kernel_inverter_chain_64" if args.arm == "plain" else
+                                                  "newton_wave_kernel_inverter_chain_64_learned") if N == 64 else None),
                      "kernel": "newton_wave_kernel", "flops_per_launch": flops},
     }
     if not args.no_cpu:
