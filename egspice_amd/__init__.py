@@ -491,11 +491,15 @@ class Problem:
         self._check(self.lib.espice_lane_width(self.h, C.byref(n)))
         return n.value
 
-    def report(self, json=False):
+    def report(self, json=False, tables=False):
+        """Point 0's .meas lines as the CLI prints them; json: its measures
+        as JSON; tables: the run's plots in SPICE3 print format (a `.dc
+        monte` deck's Monte Carlo table)."""
+        mode = 2 if tables else 1 if json else 0
         req = C.c_size_t()
-        self.lib.espice_cli_report(self.h, 1 if json else 0, None, 0, C.byref(req))
+        self.lib.espice_cli_report(self.h, mode, None, 0, C.byref(req))
         buf = C.create_string_buffer(req.value)
-        self._check(self.lib.espice_cli_report(self.h, 1 if json else 0, buf, req.value, C.byref(req)))
+        self._check(self.lib.espice_cli_report(self.h, mode, buf, req.value, C.byref(req)))
         return buf.value.decode()
 
 

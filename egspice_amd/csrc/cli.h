@@ -8,9 +8,11 @@
 extern "C" {
 #endif
 
-/* Point 0's .meas lines as the reference prints them (json 0), or its
- * measures as a JSON object (json 1); buffer protocol as espice_copy_*. */
-espice_status espice_cli_report(espice_problem *problem, int json, char *buf, size_t cap, size_t *required);
+/* Point 0's .meas lines as the reference prints them (mode 0), its
+ * measures as a JSON object (mode 1), or the run's plots in SPICE3 print
+ * format (mode 2: a `.dc monte` deck's Monte Carlo table); buffer protocol
+ * as espice_copy_*. */
+espice_status espice_cli_report(espice_problem *problem, int mode, char *buf, size_t cap, size_t *required);
 /* ngspice's INPevaluate of one number with its scale suffix; 1 when `text`
  * is a number, else 0. */
 int espice_spice_number(const char *text, double *out);

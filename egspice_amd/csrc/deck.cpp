@@ -1,5 +1,6 @@
 // The deck subset reader and its builder onto a lane template (deck.h).
 #include "deck.h"
+#include "sampler.h"
 
 #include <cctype>
 #include <cmath>
@@ -302,9 +303,9 @@ ExprP parse_expr(const std::string &text, std::string &err)
     return e;
 }
 
-double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err)
+double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err, Draw *draw)
 {
-    auto arg = [&](size_t k) { return eval_expr(*e.args[k], env, err); };
+    auto arg = [&](size_t k) { return eval_expr(*e.args[k], env, err, draw); };
     switch (e.op) {
     case Expr::NUM: return e.num;
     case Expr::NAME: {
@@ -324,6 +325,19 @@ double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::s
     case Expr::CALL: {
         const std::string &f = e.name;
         const size_t n = e.args.size();
+        const int dist = f == "agauss" ? MC_AGAUSS : f == "gauss" ? MC_GAUSS : f == "aunif" ? MC_AUNIF
+                       : f == "unif"   ? MC_UNIF   : f == "limit" ? MC_LIMIT : -1;
+        if (dist >= 0) {
+            if (n < 1 || n > 4) {
+                err = "BadDistribution " + f;
+                return NAN;
+            }
+            double a[4];
+            for (size_t k = 0; k < n; k++) a[k] = arg(k);
+            if (!draw || draw->trial == 0) return a[0];
+            const uint64_t site = site_key(draw->owner, draw->ordinal++);
+            return trial_draw(draw->seed, draw->trial, site, dist, a, (uint32_t)n);
+        }
         if (n == 1) {
             const double x = arg(0);
             if (f == "sqrt") return std::sqrt(x);
@@ -665,6 +679,74 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
 
 }  // namespace
 
+// `.param name = value ...` from the card's text: a value is a quoted or
+// braced expression, or bare text running to the next top-level `name =`
+// (so a function call keeps its parentheses and commas).
+static bool read_params(const std::string &line, Deck &d, std::string &err)
+{
+    auto word = [](char c) { return std::isalnum((unsigned char)c) || c == '_' || c == '.'; };
+    auto space = [](char c) { return std::isspace((unsigned char)c) || c == ','; };
+    size_t i = line.find_first_of(" \t");
+    if (i == std::string::npos) return true;
+    for (;;) {
+        while (i < line.size() && space(line[i])) i++;
+        if (i >= line.size()) return true;
+        const size_t b = i;
+        while (i < line.size() && word(line[i])) i++;
+        const std::string name = line.substr(b, i - b);
+        while (i < line.size() && std::isspace((unsigned char)line[i])) i++;
+        if (name.empty() || i >= line.size() || line[i] != '=') {
+            err = "BadParam";
+            return false;
+        }
+        i++;
+        while (i < line.size() && std::isspace((unsigned char)line[i])) i++;
+        std::string text;
+        bool braced = false;
+        if (i < line.size() && (line[i] == '{' || line[i] == '\'' || line[i] == '"')) {
+            const char close = line[i] == '{' ? '}' : line[i];
+            const size_t j = line.find(close, i + 1);
+            if (j == std::string::npos) {
+                err = "BadParam";
+                return false;
+            }
+            text = line.substr(i + 1, j - i - 1);
+            braced = true;
+            i = j + 1;
+        } else {
+            const size_t v0 = i;
+            int depth = 0;
+            while (i < line.size()) {
+                const char c = line[i];
+                if (c == '(') depth++;
+                if (c == ')') depth--;
+                if (depth == 0 && space(c)) {
+                    size_t k = i;
+                    while (k < line.size() && space(line[k])) k++;
+                    size_t k2 = k;
+                    while (k2 < line.size() && word(line[k2])) k2++;
+                    size_t k3 = k2;
+                    while (k3 < line.size() && std::isspace((unsigned char)line[k3])) k3++;
+                    if (k >= line.size() || (k2 > k && k3 < line.size() && line[k3] == '=')) break;
+                }
+                i++;
+            }
+            text = line.substr(v0, i - v0);
+        }
+        Value v;
+        v.given = true;
+        double x;
+        size_t len;
+        if (!braced && spice_number_at(text, 0, x, len) && len == text.size()) {
+            v.num = x;
+        } else {
+            v.expr = parse_expr(text, err);
+            if (!v.expr) return false;
+        }
+        d.params.emplace_back(name, v);
+    }
+}
+
 bool read_deck(const std::string &text, Deck &d, std::string &err)
 {
     // fold case, split lines, join '+' continuations, drop comments
@@ -725,14 +807,7 @@ bool read_deck(const std::string &text, Deck &d, std::string &err)
         } else if (w == ".end") {
             break;
         } else if (w == ".param") {
-            Cursor c{t, 1};
-            while (c.more()) {
-                if (!(c.i + 2 < t.size() && t[c.i + 1] == "=")) return fail_at("BadParam");
-                Value v;
-                if (!to_value(t[c.i + 2], v, err)) return fail_at(err);
-                d.params.emplace_back(t[c.i], v);
-                c.i += 3;
-            }
+            if (!read_params(card.second, d, err)) return fail_at(err);
         } else if (w == ".model") {
             if (t.size() < 3) return fail_at("BadModel");
             Model m;
@@ -751,12 +826,40 @@ bool read_deck(const std::string &text, Deck &d, std::string &err)
             d.models[m.name] = m;
         } else if (w == ".op") {
             d.op = true;
+        } else if (w == ".option" || w == ".options") {
+            // the sampler's options (variants.zig:1694-1705); any other
+            // option would change the solve and is refused
+            Cursor c{t, 1};
+            while (c.more()) {
+                if (!(c.i + 2 < t.size() && t[c.i + 1] == "=")) return fail_at("UnsupportedCard " + w);
+                const std::string &k = t[c.i], &v = t[c.i + 2];
+                double x;
+                if (k == "seed" && spice_number(v, x)) {
+                    d.seed = x != x ? 0 : std::fabs(x) >= 18446744073709551616.0 ? UINT64_MAX : (uint64_t)std::fabs(x);
+                    d.has_seed = true;
+                } else if (!(k == "sampling_method" && v == "srs")) {
+                    return fail_at("UnsupportedCard " + w + " " + k);
+                }
+                c.i += 3;
+            }
+        } else if (w == ".dc" && t.size() == 4 && t[1] == "monte" && t[2] == "=") {
+            double x;
+            if (!spice_number(t[3], x) || !(x >= 1) || x > 1e6) return fail_at("BadMonte");
+            d.monte_dc = (uint32_t)x;
         } else if (w == ".tran") {
             Cursor c{t, 1};
             Value *dst[4] = {&d.tstep, &d.tstop, &d.tstart, &d.tmax};
             int k = 0;
             while (c.more()) {
                 if (c.peek() == "uic") return fail_at("UicUnsupported");
+                if (c.peek() == "sweep") {
+                    double x;
+                    if (!(c.i + 4 == t.size() && t[c.i + 1] == "monte" && t[c.i + 2] == "=" &&
+                          spice_number(t[c.i + 3], x) && x >= 1 && x <= 1e6))
+                        return fail_at("UnsupportedSweep");
+                    d.monte_tran = (uint32_t)x;
+                    break;
+                }
                 if (k >= 4 || !to_value(c.peek(), *dst[k], err)) return fail_at(err.empty() ? "BadTran" : err);
                 k++;
                 c.i++;
@@ -780,7 +883,7 @@ bool read_deck(const std::string &text, Deck &d, std::string &err)
 }
 
 bool param_env(const Deck &d, const std::map<std::string, double> &live, std::map<std::string, double> &env,
-               std::string &err)
+               std::string &err, uint32_t trial)
 {
     env.clear();
     for (auto &[name, v] : d.params) {
@@ -789,7 +892,11 @@ bool param_env(const Deck &d, const std::map<std::string, double> &live, std::ma
             env[name] = it->second;
             continue;
         }
-        env[name] = v.expr ? eval_expr(*v.expr, env, err) : v.num;
+        Draw dr;
+        dr.seed = d.seed;
+        dr.trial = trial;
+        dr.owner = top_owner(name.data(), name.size());
+        env[name] = v.expr ? eval_expr(*v.expr, env, err, &dr) : v.num;
         if (!err.empty()) return false;
     }
     for (auto &[name, v] : live)
@@ -944,7 +1051,7 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         }
         bd.elem_kind[k] = kind;
         bd.elem_inst[k] = inst;
-        for (auto &[idx, v] : prm) bd.slots.push_back(ParamSlot{(uint32_t)kind, inst, idx, v});
+        for (auto &[idx, v] : prm) bd.slots.push_back(ParamSlot{(uint32_t)kind, inst, idx, v, e.name});
         // a V source's branch current takes the next row when it is added,
         // and so does a diode's internal anode when its RS is positive
         if (e.letter == 'v') bd.vsrc_branch[e.name] = next_row++;
@@ -965,6 +1072,17 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         return false;
     }
     return true;
+}
+
+double slot_value(const Deck &d, const ParamSlot &sl, const std::map<std::string, double> &env, std::string &err,
+                  uint32_t trial)
+{
+    if (!sl.value.expr) return sl.value.num;
+    Draw dr;
+    dr.seed = d.seed;
+    dr.trial = trial;
+    dr.owner = top_owner(sl.card.data(), sl.card.size());
+    return eval_expr(*sl.value.expr, env, err, &dr);
 }
 
 }  // namespace esp

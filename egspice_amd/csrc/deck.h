@@ -31,7 +31,19 @@ bool spice_number(const std::string &text, double &out);
 struct Expr;
 using ExprP = std::shared_ptr<const Expr>;
 ExprP parse_expr(const std::string &text, std::string &err);
-double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err);
+
+// A Monte Carlo trial's draws (src/frontend/variants.zig:1716-1720,
+// 1862-1895): trial 0 answers every distribution call (agauss, gauss, aunif,
+// unif, limit) with its nominal; trial k >= 1 draws the call at site
+// site_key(owner, ordinal), the ordinal counting the owner's calls in
+// evaluation order (src/frontend/netlist.zig:3595-3601).
+struct Draw {
+    uint64_t seed = 1;
+    uint32_t trial = 0;
+    uint64_t owner = 0;
+    uint32_t ordinal = 0;
+};
+double eval_expr(const Expr &e, const std::map<std::string, double> &env, std::string &err, Draw *draw = nullptr);
 // Names an expression reads.
 void expr_names(const Expr &e, std::vector<std::string> &out);
 
@@ -83,6 +95,12 @@ struct Deck {
     Value tstep, tstop, tstart, tmax;
     std::vector<Meas> meas;
     std::vector<std::string> ignored;  // cards read and not run (.ac, .dc, .options, ...)
+    // Monte Carlo: `.option seed=` (the reference's default 1), `.dc
+    // monte=N` (N operating-point trials, each warm from the nominal
+    // operating point) and `.tran ... sweep monte=N`
+    uint64_t seed = 1;
+    bool has_seed = false;
+    uint32_t monte_dc = 0, monte_tran = 0;
 };
 
 // Reads deck text; false with a named error (and its line) on anything
@@ -95,6 +113,7 @@ bool read_deck(const std::string &text, Deck &d, std::string &err);
 struct ParamSlot {
     uint32_t kind, inst, index;
     Value value;
+    std::string card;  // the element's card name: its distribution calls' site owner
 };
 
 struct BuiltDeck {
@@ -113,8 +132,15 @@ struct BuiltDeck {
 bool build_deck(BuiltDeck &bd, struct ::espice_lanes_template *t, std::string &err);
 
 // The .param environment of one point: every .param in card order, with
-// `live` values taking the place of their cards' expressions.
+// `live` values taking the place of their cards' expressions; Monte Carlo
+// trial `trial` (0: nominal) draws each parameter's distribution calls under
+// the parameter's own site owner.
 bool param_env(const Deck &d, const std::map<std::string, double> &live, std::map<std::string, double> &env,
-               std::string &err);
+               std::string &err, uint32_t trial = 0);
+
+// A card's expression value for trial `trial` (its distribution calls owned
+// by the card).
+double slot_value(const Deck &d, const ParamSlot &s, const std::map<std::string, double> &env, std::string &err,
+                  uint32_t trial);
 
 }  // namespace esp

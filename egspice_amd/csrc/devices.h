@@ -55,7 +55,7 @@ enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT,
 constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4};
 constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
 constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
-constexpr int KIND_PK[K_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
+constexpr int KIND_PK[K_NKIND] = {SRCP_K, 1, 2, 1, DIOP_K, M1P_K};
 // Residual rows a kind writes in a static evaluation.
 constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
 // Unknowns a kind's limiter writes (its limited image).
@@ -295,7 +295,12 @@ inline void prep_kind(int kind, const double *u, double *p)
         break;
     }
     case K_ISRC: p[0] = u[ISRC_DC] == u[ISRC_DC] ? u[ISRC_DC] : 0.0; break;
-    case K_RES: p[0] = 1.0 / u[RES_R]; break;
+    case K_RES:
+        // resistor.va: rt = r at the nominal temperature (no tempcos), and
+        // I(p, n) <+ V(p, n) / rt, whose slope is the reciprocal
+        p[0] = u[RES_R];
+        p[1] = 1.0 / u[RES_R];
+        break;
     case K_CAP: p[0] = u[CAP_C]; break;
     default: break;
     }
@@ -711,13 +716,15 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tr
         o.res[1] = -p[0];
         break;
     case K_RES: {
-        double i = p[0] * (lx[0] - lx[1]);
+        // V(p, n) / rt: a quotient by a constant, its lanes scaled by the
+        // reciprocal (models/resistor.va, eval.zig's Dual division)
+        double i = (lx[0] - lx[1]) / p[0];
         o.res[0] = i;
         o.res[1] = -i;
-        o.jac[0][0] = p[0];
-        o.jac[0][1] = -p[0];
-        o.jac[1][0] = -p[0];
-        o.jac[1][1] = p[0];
+        o.jac[0][0] = p[1];
+        o.jac[0][1] = -p[1];
+        o.jac[1][0] = -p[1];
+        o.jac[1][1] = p[1];
         break;
     }
     case K_CAP:

@@ -1,7 +1,11 @@
 // espice: the command line subset that runs a deck on the MI355X lane
 // engine (src/main.zig's batch mode with --backend).
 //
-//   espice [--backend=hip|auto] [--measures=FILE] DECK
+//   espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] DECK
+//
+// --rawfile with --format=print writes the run's plots in SPICE3 print
+// format (src/output/spice_print.zig): a `.dc monte` deck's Monte Carlo
+// table; no other format is written.
 //
 // The deck runs through the points API (include/espice.h) as one point: its
 // .op and .tran on GPU lanes, its .meas cards printed as the reference
@@ -17,7 +21,7 @@
 
 static int usage()
 {
-    std::fprintf(stderr, "Usage: espice [--backend=hip|auto] [--measures=FILE] FILE\n");
+    std::fprintf(stderr, "Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] FILE\n");
     return 2;
 }
 
@@ -32,7 +36,7 @@ static std::string report(espice_problem *p, int json)
 
 int main(int argc, char **argv)
 {
-    std::string deck, backend = "auto", measures;
+    std::string deck, backend = "auto", measures, rawfile, format;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto value = [&](const char *opt, std::string &dst) {
@@ -48,10 +52,12 @@ int main(int argc, char **argv)
             return false;
         };
         if (a == "-h" || a == "--help") {
-            std::printf("Usage: espice [--backend=hip|auto] [--measures=FILE] FILE\n");
+            std::printf("Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] FILE\n");
             return 0;
         }
-        if (value("--backend", backend) || value("--measures", measures)) continue;
+        if (value("--backend", backend) || value("--measures", measures) || value("--rawfile", rawfile) ||
+            value("--format", format))
+            continue;
         if (!a.empty() && a[0] == '-') {
             std::fprintf(stderr, "Error: unsupported option '%s'\n", a.c_str());
             return usage();
@@ -60,6 +66,10 @@ int main(int argc, char **argv)
         deck = a;
     }
     if (deck.empty()) return usage();
+    if (!rawfile.empty() && format != "print") {
+        std::fprintf(stderr, "Error: --rawfile is written only with --format=print\n");
+        return 2;
+    }
     espice_create_options o;
     espice_default_options(&o);
     if (backend == "hip") {
@@ -91,6 +101,16 @@ int main(int argc, char **argv)
         return 1;
     }
     std::fputs(report(p, 0).c_str(), stdout);
+    if (!rawfile.empty()) {
+        FILE *f = std::fopen(rawfile.c_str(), "w");
+        if (!f) {
+            std::fprintf(stderr, "Error: cannot write %s\n", rawfile.c_str());
+            espice_destroy(p);
+            return 1;
+        }
+        std::fputs(report(p, 2).c_str(), f);
+        std::fclose(f);
+    }
     if (!measures.empty()) {
         const std::string j = report(p, 1) + "\n";
         if (measures == "-") {

@@ -10,6 +10,7 @@
 // src/c_api.zig and src/espice.zig.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -41,6 +42,8 @@ struct Point {
     std::vector<double> meas;  // one per .meas card
     std::vector<double> aux;   // two per card: the printed side values
     uint32_t tran_points = 0;
+    uint32_t trial = 0;        // Monte Carlo trial (HSPICE numbering from 1), 0: nominal draws
+    std::vector<double> x;     // the operating point (n rows), when one was solved
 };
 
 // One .meas card as device passes.
@@ -58,6 +61,8 @@ struct Problem {
     uint32_t queries = 0;
     std::vector<espice_meas_pass> passes;
     std::vector<CardPasses> cards;
+    std::vector<double> x_nominal;  // point 0's operating point: the start of `.dc monte` trials
+    uint32_t seed_trial = 0;        // the trial every non-Monte-Carlo point draws
     std::string error;
     ~Problem()
     {
@@ -239,14 +244,14 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
         std::map<std::string, double> live;
         for (size_t k = 0; k < P.live.size(); k++) live[P.live[k]] = pt.live[k];
         std::string err;
-        if (!esp::param_env(d, live, envs[l], err)) bad[l] = true;
+        if (!esp::param_env(d, live, envs[l], err, pt.trial)) bad[l] = true;
     }
     std::vector<double> v(N);
     for (const esp::ParamSlot &s : P.bd.slots) {
         if (!s.value.expr) continue;
         for (uint32_t l = 0; l < N; l++) {
             std::string err;
-            v[l] = bad[l] ? NAN : esp::eval_expr(*s.value.expr, envs[l], err);
+            v[l] = bad[l] ? NAN : esp::slot_value(d, s, envs[l], err, P.points[b + l].trial);
             if (!err.empty() || !std::isfinite(v[l])) bad[l] = true;
         }
         if (espice_lanes_set_param(L, s.kind, s.inst, s.index, v.data()) != ESPICE_OK)
@@ -265,10 +270,17 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
     bool have_op = false;
     bool need_op = d.op;
     for (const Meas &m : d.meas) need_op = need_op || m.analysis == "op";
-    if (d.op) {
+    if (d.op || d.monte_dc) {
+        // a `.dc monte` trial starts from the nominal operating point with
+        // no junction seeding (src/analysis/sweep/lanes.zig:73-83,
+        // mc.zig:128-131); a batch holds trials only or no trials
+        const bool warm = P.points[b].trial != 0 && d.monte_dc && !P.x_nominal.empty();
         std::vector<double> x0((size_t)n * N, 0.0);
+        if (warm)
+            for (uint32_t l = 0; l < N; l++)
+                for (uint32_t i = 0; i < n; i++) x0[(size_t)i * N + l] = P.x_nominal[i];
         if (espice_lanes_upload_x(L, x0.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Upload");
-        espice_newton_opts o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
+        espice_newton_opts o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, warm ? 0u : 1u};
         std::vector<espice_lane_result> r(N);
         double ms;
         if (espice_lanes_newton_op(L, &o, r.data(), &ms) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "NewtonOp");
@@ -281,7 +293,11 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             if (r[l].status != 1) {
                 pt.failed = true;
                 pt.reason = r[l].status == 2 ? ESPICE_REASON_SINGULAR_MATRIX : ESPICE_REASON_NO_CONVERGENCE;
+                continue;
             }
+            pt.x.resize(n);
+            for (uint32_t i = 0; i < n; i++) pt.x[i] = xop[(size_t)i * N + l];
+            if (b + l == 0) P.x_nominal = pt.x;
         }
     }
     if (d.tran) {
@@ -439,7 +455,8 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     std::string err;
     if (!esp::read_deck(text, P->bd.deck, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     const esp::Deck &d = P->bd.deck;
-    if (!d.op && !d.tran) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
+    if (!d.op && !d.tran && !d.monte_dc) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
+    if (d.monte_tran) return say(ESPICE_INVALID_ARGUMENT, "UnsupportedAnalysis tran sweep monte");
     if (espice_lanes_template_create(&P->tpl) != ESPICE_OK) return say(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
     if (!esp::build_deck(P->bd, P->tpl, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     for (size_t k = 0; k < o->live_param_count; k++) {
@@ -456,10 +473,24 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     std::map<std::string, double> env;
     esp::param_env(d, {}, env, err);
     for (auto &name : P->live) p0.live.push_back(env[name]);
-    P->per_point = (d.op ? 1u : 0u) + (d.tran ? 1u : 0u);
+    // `.option seed` without a Monte Carlo card draws trial 1 for every point
+    // (the deck's mismatch sample, src/frontend/netlist.zig:1401,
+    // variants.zig:1088-1090, 1172-1176)
+    P->seed_trial = d.has_seed && !d.monte_dc && !d.monte_tran ? 1 : 0;
+    p0.trial = P->seed_trial;
+    P->per_point = (d.op || d.monte_dc ? 1u : 0u) + (d.tran ? 1u : 0u);
     p0.first_id = 0;
     P->queries = P->per_point;
     P->points.push_back(p0);
+    // `.dc monte=N`: trials 1..N, each a point of its own after the nominal
+    for (uint32_t k = 1; k <= d.monte_dc; k++) {
+        Point pt;
+        pt.live = p0.live;
+        pt.trial = k;
+        pt.first_id = P->queries;
+        P->queries += P->per_point;
+        P->points.push_back(std::move(pt));
+    }
     espice_problem *tok = insert(P.get());
     if (!tok) return say(ESPICE_OUT_OF_MEMORY, "TooManyProblems");
     P.release();
@@ -505,11 +536,20 @@ espice_status espice_run_all(espice_problem *problem)
     if (!P) return ESPICE_INVALID_HANDLE;
     size_t b = 0;
     const size_t n = P->points.size();
+    // the nominal point first and alone when trials start from its
+    // operating point
+    if (P->bd.deck.monte_dc && !P->points[0].ran) {
+        const espice_status s = run_batch(*P, 0, 1);
+        if (s != ESPICE_OK) return s;
+    }
     while (b < n) {
         while (b < n && (P->points[b].ran || P->points[b].released)) b++;
         if (b >= n) break;
         size_t e = b;
-        while (e < n && !P->points[e].ran && !P->points[e].released && e - b < LANE_CAP) e++;
+        const bool trial = P->points[b].trial != 0 && P->bd.deck.monte_dc;
+        while (e < n && !P->points[e].ran && !P->points[e].released && e - b < LANE_CAP &&
+               (P->points[e].trial != 0 && P->bd.deck.monte_dc) == trial)
+            e++;
         const espice_status s = run_batch(*P, b, e);
         if (s != ESPICE_OK) return s;
         b = e;
@@ -548,6 +588,7 @@ espice_status espice_append_points(espice_problem *problem, const double *values
     for (size_t i = 0; i < count; i++) {
         Point pt;
         pt.live.assign(values + i * k, values + (i + 1) * k);
+        pt.trial = P->seed_trial;
         pt.first_id = P->queries;
         P->queries += P->per_point;
         bool finite = true;
@@ -693,6 +734,55 @@ std::string problem_report(espice_problem *problem)
     return out;
 }
 
+// f64 as the reference's `{e}` prints it: the shortest digits that read
+// back to the same double, then `e` and the exponent without sign padding
+// (`1e0`, `-4.98898735103262e-4`, `5e-1`).
+static std::string shortest_e(double x)
+{
+    if (std::isnan(x)) return "nan";
+    if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+    if (x == 0) return std::signbit(x) ? "-0e0" : "0e0";
+    char b[64];
+    for (int p = 0; p <= 17; p++) {
+        std::snprintf(b, sizeof b, "%.*e", p, x);
+        if (std::strtod(b, nullptr) == x) break;
+    }
+    const std::string t(b);
+    const size_t e = t.find('e');
+    return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
+}
+
+// The run's plots in SPICE3 print format (src/output/spice_print.zig:
+// 10-44): for `.dc monte=N` the "Monte Carlo" plot, one row per converged
+// trial: its ordinal, the trial number as `run`, then every node voltage and
+// V-source current, names in order (src/analysis/sweep/mc.zig:148-171).
+std::string problem_print_tables(espice_problem *problem)
+{
+    Problem *P = get(problem);
+    if (!P) return "";
+    const esp::Deck &d = P->bd.deck;
+    std::string out;
+    if (d.monte_dc) {
+        std::vector<std::pair<std::string, uint32_t>> probes;
+        for (auto &[name, row] : P->bd.rows) probes.emplace_back("v(" + name + ")", row);
+        for (auto &[name, row] : P->bd.vsrc_branch) probes.emplace_back("i(" + name + ")", row);
+        std::sort(probes.begin(), probes.end());
+        out += "Monte Carlo: " + d.title + "\nIndex\trun";
+        for (auto &p : probes) out += "\t" + p.first;
+        out += "\n-----";
+        for (size_t k = 0; k <= probes.size(); k++) out += "\t---------------";
+        out += "\n";
+        size_t k = 0;
+        for (const Point &pt : P->points) {
+            if (pt.trial == 0 || !pt.ran || pt.failed || pt.x.empty()) continue;
+            out += std::to_string(k++) + "\t" + shortest_e((double)pt.trial);
+            for (auto &p : probes) out += "\t" + shortest_e(pt.x[p.second]);
+            out += "\n";
+        }
+    }
+    return out;
+}
+
 // The measures of point 0 as JSON: {"name": value, ...} (null for NaN).
 std::string problem_measures_json(espice_problem *problem)
 {
@@ -712,10 +802,12 @@ std::string problem_measures_json(espice_problem *problem)
 
 }  // namespace esp
 
-extern "C" espice_status espice_cli_report(espice_problem *problem, int json, char *buf, size_t cap, size_t *required)
+extern "C" espice_status espice_cli_report(espice_problem *problem, int mode, char *buf, size_t cap, size_t *required)
 {
     if (!get(problem)) return ESPICE_INVALID_HANDLE;
-    const std::string s = json ? esp::problem_measures_json(problem) : esp::problem_report(problem);
+    const std::string s = mode == 1 ? esp::problem_measures_json(problem)
+                        : mode == 2 ? esp::problem_print_tables(problem)
+                                    : esp::problem_report(problem);
     if (required) *required = s.size() + 1;
     if (cap < s.size() + 1) return ESPICE_BUFFER_TOO_SMALL;
     std::memcpy(buf, s.c_str(), s.size() + 1);

@@ -9,9 +9,14 @@
 //   trial_draw    src/frontend/variants.zig:1862-1895 (agauss, gauss,
 //                 aunif, unif, limit; with a multiplier m the draw of
 //                 largest deviation among m)
+//   wyhash        the sites' name hash (std.hash.Wyhash as the reference
+//                 calls it, src/frontend/netlist.zig:690-692): Wang Yi's
+//                 published wyhash with its 64-bit multiply-fold mixer
+//   name_key      src/frontend/netlist.zig:690-692
 #pragma once
 
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 
 namespace esp {
@@ -25,6 +30,76 @@ inline uint64_t site_key(uint64_t owner, uint64_t salt)
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
 }
+
+// 128-bit product of a and b, folded: low ^ high.
+inline void wy_mum(uint64_t &a, uint64_t &b)
+{
+    const unsigned __int128 r = (unsigned __int128)a * b;
+    a = (uint64_t)r;
+    b = (uint64_t)(r >> 64);
+}
+inline uint64_t wy_mix(uint64_t a, uint64_t b)
+{
+    wy_mum(a, b);
+    return a ^ b;
+}
+inline uint64_t wy_read(const unsigned char *p, int k)
+{
+    uint64_t v = 0;
+    for (int i = k - 1; i >= 0; i--) v = v << 8 | p[i];
+    return v;
+}
+
+inline uint64_t wyhash(uint64_t seed, const void *key, size_t len)
+{
+    static const uint64_t secret[4] = {0xa0761d6478bd642full, 0xe7037ed1a0b428dbull, 0x8ebc6af09c88c6e3ull,
+                                       0x589965cc75374cc3ull};
+    const unsigned char *p = (const unsigned char *)key;
+    seed ^= wy_mix(seed ^ secret[0], secret[1]);
+    uint64_t a, b;
+    if (len <= 16) {
+        if (len >= 4) {
+            const size_t q = (len >> 3) << 2;
+            a = wy_read(p, 4) << 32 | wy_read(p + q, 4);
+            b = wy_read(p + len - 4, 4) << 32 | wy_read(p + len - 4 - q, 4);
+        } else if (len > 0) {
+            a = (uint64_t)p[0] << 16 | (uint64_t)p[len >> 1] << 8 | p[len - 1];
+            b = 0;
+        } else {
+            a = b = 0;
+        }
+    } else {
+        size_t i = len;
+        if (i > 48) {
+            uint64_t s1 = seed, s2 = seed;
+            do {
+                seed = wy_mix(wy_read(p, 8) ^ secret[1], wy_read(p + 8, 8) ^ seed);
+                s1 = wy_mix(wy_read(p + 16, 8) ^ secret[2], wy_read(p + 24, 8) ^ s1);
+                s2 = wy_mix(wy_read(p + 32, 8) ^ secret[3], wy_read(p + 40, 8) ^ s2);
+                p += 48;
+                i -= 48;
+            } while (i > 48);
+            seed ^= s1 ^ s2;
+        }
+        while (i > 16) {
+            seed = wy_mix(wy_read(p, 8) ^ secret[1], wy_read(p + 8, 8) ^ seed);
+            i -= 16;
+            p += 16;
+        }
+        a = wy_read(p + i - 16, 8);
+        b = wy_read(p + i - 8, 8);
+    }
+    a ^= secret[1];
+    b ^= seed;
+    wy_mum(a, b);
+    return wy_mix(a ^ secret[0] ^ len, b ^ secret[1]);
+}
+
+inline uint64_t name_key(uint64_t owner, const char *name, size_t len) { return site_key(owner, wyhash(0, name, len)); }
+
+// The site owner of a distribution call in `.param name` text or on card
+// `name` at the top level (netlist.zig:1832, 3570, 3826).
+inline uint64_t top_owner(const char *name, size_t len) { return name_key(name_key(0, "", 0), name, len); }
 
 inline double mc_uniform(uint64_t seed, uint64_t site, uint32_t trial)
 {

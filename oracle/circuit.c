@@ -44,7 +44,7 @@ int or_solver_is_tri(const or_solver *s);
 static const int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4};
 static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
 static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
-static const int PK[OR_NKIND] = {SRCP_K, 1, 1, 1, DIOP_K, M1P_K};
+static const int PK[OR_NKIND] = {SRCP_K, 1, 2, 1, DIOP_K, M1P_K};
 
 /* Structural pattern of each kind (row-major bit masks over local
  * unknowns): G-plane entries and C-plane entries. */
@@ -342,7 +342,10 @@ void or_lanes_prepare(or_lanes *L)
                     break;
                 }
                 case OR_ISRC: p[0] = u[OR_ISRC_DC] == u[OR_ISRC_DC] ? u[OR_ISRC_DC] : 0.0; break;
-                case OR_RES: p[0] = 1.0 / u[OR_RES_R]; break;
+                case OR_RES: /* resistor.va: rt = r, I = V / rt, slope 1 / rt */
+                    p[0] = u[OR_RES_R];
+                    p[1] = 1.0 / u[OR_RES_R];
+                    break;
                 case OR_CAP: p[0] = u[OR_CAP_C]; break;
                 }
                 for (int k = 0; k < PK[kind]; k++) L->pprm[kind][((size_t)i * PK[kind] + k) * N + l] = p[k];
@@ -405,13 +408,13 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
                 o.res[1] = -p[0];
                 break;
             case OR_RES: {
-                double i = p[0] * (lx[0] - lx[1]);
+                double i = (lx[0] - lx[1]) / p[0];
                 o.res[0] = i;
                 o.res[1] = -i;
-                o.jac[0][0] = p[0];
-                o.jac[0][1] = -p[0];
-                o.jac[1][0] = -p[0];
-                o.jac[1][1] = p[0];
+                o.jac[0][0] = p[1];
+                o.jac[0][1] = -p[1];
+                o.jac[1][0] = -p[1];
+                o.jac[1][1] = p[1];
                 break;
             }
             case OR_CAP: {

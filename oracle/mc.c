@@ -7,14 +7,26 @@
  *                       sampling: the top 53 bits, centred in their cell)
  *   or_normal_quantile  src/core/numerics.zig:416-432 (Acklam)
  *   or_trial_draw       src/frontend/variants.zig:1862-1895 (TrialDraw.sample)
+ *   or_wyhash           the name hash of the sites (std.hash.Wyhash as
+ *                       src/frontend/netlist.zig:690-692 calls it): Wang
+ *                       Yi's published wyhash (64-bit multiply-fold mixer,
+ *                       secret 0xa0761d6478bd642f, 0xe7037ed1a0b428db,
+ *                       0x8ebc6af09c88c6e3, 0x589965cc75374cc3)
+ *   or_param_site       a `.param`'s first distribution call's site,
+ *                       siteKey(nameKey(nameKey(0, ""), name), 0)
+ *                       (netlist.zig:3564-3601, levelPath :3666)
  * Parity pinned by: tests/test_mc.py (splitmix64's published output
  * sequence from state 0; the normal quantile against Python's
  * statistics.NormalDist within Acklam's 1.2e-9 bound and the reference's
  * own quantile tests, numerics.zig:520-545; TrialDraw's tests,
- * variants.zig:1917-1933).
+ * variants.zig:1917-1933); the wyhash keys by the reference's printed
+ * Monte Carlo rows of docs/examples/monte_agauss.sp (docs/learn/
+ * monte-carlo.md:62-69, 17 digits), which only the right hash reproduces.
  */
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "oracle.h"
 
@@ -82,4 +94,76 @@ double or_trial_draw(uint64_t seed, uint32_t trial, uint64_t site, int fn, const
         if (m == 0 || fabs(dv) > fabs(best)) best = dv;
     }
     return nom + best;
+}
+
+static void wy_mum(uint64_t *a, uint64_t *b)
+{
+    unsigned __int128 r = (unsigned __int128)*a * *b;
+    *a = (uint64_t)r;
+    *b = (uint64_t)(r >> 64);
+}
+
+static uint64_t wy_mix(uint64_t a, uint64_t b)
+{
+    wy_mum(&a, &b);
+    return a ^ b;
+}
+
+static uint64_t wy_le(const unsigned char *p, int k)
+{
+    uint64_t v = 0;
+    for (int i = 0; i < k; i++) v |= (uint64_t)p[i] << (8 * i);
+    return v;
+}
+
+uint64_t or_wyhash(uint64_t seed, const char *key, size_t len)
+{
+    static const uint64_t S[4] = {0xa0761d6478bd642full, 0xe7037ed1a0b428dbull, 0x8ebc6af09c88c6e3ull,
+                                  0x589965cc75374cc3ull};
+    const unsigned char *p = (const unsigned char *)key;
+    uint64_t a, b;
+    seed ^= wy_mix(seed ^ S[0], S[1]);
+    if (len <= 16) {
+        if (len >= 4) {
+            size_t q = (len >> 3) << 2;
+            a = (wy_le(p, 4) << 32) | wy_le(p + q, 4);
+            b = (wy_le(p + len - 4, 4) << 32) | wy_le(p + len - 4 - q, 4);
+        } else if (len > 0) {
+            a = ((uint64_t)p[0] << 16) | ((uint64_t)p[len >> 1] << 8) | p[len - 1];
+            b = 0;
+        } else {
+            a = b = 0;
+        }
+    } else {
+        size_t i = len;
+        if (i > 48) {
+            uint64_t s1 = seed, s2 = seed;
+            do {
+                seed = wy_mix(wy_le(p, 8) ^ S[1], wy_le(p + 8, 8) ^ seed);
+                s1 = wy_mix(wy_le(p + 16, 8) ^ S[2], wy_le(p + 24, 8) ^ s1);
+                s2 = wy_mix(wy_le(p + 32, 8) ^ S[3], wy_le(p + 40, 8) ^ s2);
+                p += 48;
+                i -= 48;
+            } while (i > 48);
+            seed ^= s1 ^ s2;
+        }
+        while (i > 16) {
+            seed = wy_mix(wy_le(p, 8) ^ S[1], wy_le(p + 8, 8) ^ seed);
+            i -= 16;
+            p += 16;
+        }
+        a = wy_le(p + i - 16, 8);
+        b = wy_le(p + i - 8, 8);
+    }
+    a ^= S[1];
+    b ^= seed;
+    wy_mum(&a, &b);
+    return wy_mix(a ^ S[0] ^ (uint64_t)len, b ^ S[1]);
+}
+
+uint64_t or_param_site(const char *name, uint32_t ordinal)
+{
+    const uint64_t top = or_site_key(0, or_wyhash(0, "", 0));
+    const uint64_t owner = or_site_key(top, or_wyhash(0, name, strlen(name)));
+    return or_site_key(owner, ordinal);
 }

@@ -116,6 +116,8 @@ uint64_t or_site_key(uint64_t owner, uint64_t salt);
 double or_mc_uniform(uint64_t seed, uint64_t site, uint32_t trial);
 double or_normal_quantile(double u);
 double or_trial_draw(uint64_t seed, uint32_t trial, uint64_t site, int fn, const double *args, uint32_t nargs);
+uint64_t or_wyhash(uint64_t seed, const char *key, size_t len);
+uint64_t or_param_site(const char *name, uint32_t ordinal);
 /* The same on `threads` POSIX threads, lanes dealt round robin. */
 int or_newton_op_mt(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_outcome *out, uint32_t threads);
 /* The lead tape of lane 0 at x (mode 0's tape), for tests. */

@@ -79,5 +79,24 @@ DIODE_RECTIFIER_GOLDEN = {"vpeak": (9.228482e+00, 6), "vpeak.at": (5.173860e-03,
                           "vmin.at": (4.317386e-02, 6)}
 DIODE_RECTIFIER_POINTS = 610
 
+MONTE_AGAUSS_SP = """Divider spread with agauss and a Monte Carlo DC
+* agauss(nominal, variation, sigmas): 30 ohms is the 3-sigma spread.
+.param r1v=agauss(1k, 30, 3) r2v=agauss(1k, 30, 3)
+V1 in 0 DC 1
+R1 in out {r1v}
+R2 out 0 {r2v}
+.option seed=7
+.dc monte=200
+.end
+"""
+
+# docs/learn/monte-carlo.md:62-69 (`--rawfile --format=print`): the first
+# three trials, (run, i(v1), v(in), v(out)), as the reference printed them.
+MONTE_AGAUSS_ROWS = [
+    (1, -4.98898735103262e-4, 1.0, 4.9498315376777796e-1),
+    (2, -4.97205754614453e-4, 1.0, 4.945092576673322e-1),
+    (3, -5.003152460377415e-4, 1.0, 4.957336298692346e-1),
+]
+
 MEASURE_PARAM_SP = MEASURE_SP.replace("R1 in out 1k", ".param r=1k c=1n\nR1 in out {r}").replace(
     "C1 out 0 1n", "C1 out 0 {c}")

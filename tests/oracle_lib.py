@@ -130,6 +130,8 @@ def load():
         "or_mc_uniform": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint32]),
         "or_normal_quantile": (C.c_double, [C.c_double]),
         "or_trial_draw": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, f64p, C.c_uint32]),
+        "or_wyhash": (C.c_uint64, [C.c_uint64, C.c_char_p, C.c_size_t]),
+        "or_param_site": (C.c_uint64, [C.c_char_p, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

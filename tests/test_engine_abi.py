@@ -1,5 +1,5 @@
 """The C ABI of the lane engine without a GPU: the library loads and exports
-every entry point include/espice_lanes.h declares, the template calls (host
+every entry point include/espice_lanes.h and include/espice_mlp.h declare, the template calls (host
 only) give the oracle's pattern and column order, argument errors come back
 as status codes with a message, and the host tape build gives the oracle's
 tape words. No compute call touches a device here."""
@@ -14,12 +14,13 @@ import egspice_amd as E
 import oracle_lib as O
 from egspice_amd import workloads as W
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "espice_lanes.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+HEADERS = [os.path.join(INCLUDE, h) for h in ("espice_lanes.h", "espice_mlp.h")]
 
 
 def declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"\bespice_status\s+(espice_lanes_\w+)\s*\(", text)))
+    text = "".join(open(h).read() for h in HEADERS)
+    return sorted(set(re.findall(r"\bespice_status\s+(espice_(?:lanes|mlp)_\w+)\s*\(", text)))
 
 
 def test_every_declared_symbol_is_exported(engine):

@@ -264,3 +264,49 @@ def test_cli_refuses_what_the_lanes_cannot_run(tmp_path, engine):
                        "M1 o g 0 0 nch W=1u L=180n\n.model nch nmos (VTO=0.45 KP=200u TOX=10n)\n.tran 10p 20n\n.end\n")
     r = subprocess.run([exe, "--backend=hip", str(charged)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "TranChargesUnsupported" in r.stderr
+
+
+def parse_print_table(text):
+    lines = text.splitlines()
+    names = lines[1].split("\t")
+    rows = [[float(v) for v in ln.split("\t")] for ln in lines[3:] if ln]
+    return lines[0], names, np.array(rows)
+
+
+def test_monte_agauss_dc(oracle, engine):
+    from test_mc import agauss_trial_op
+    say("case: monte_agauss.sp, .dc monte=200 with .param agauss draws, seed 7")
+    pb = E.Problem(decks.MONTE_AGAUSS_SP)
+    assert pb.query_count() == 201
+    pb.run_all()
+    head, names, rows = parse_print_table(pb.report(tables=True))
+    assert head == "Monte Carlo: Divider spread with agauss and a Monte Carlo DC"
+    assert names == ["Index", "run", "i(v1)", "v(in)", "v(out)"]
+    assert rows.shape == (200, 5)
+    assert (rows[:, 0] == np.arange(200)).all() and (rows[:, 1] == np.arange(1, 201)).all()
+    for k, (run, i_v1, v_in, v_out) in enumerate(decks.MONTE_AGAUSS_ROWS):
+        assert rows[k, 2] == i_v1 and rows[k, 3] == v_in, k
+        assert abs(rows[k, 4] - v_out) <= np.spacing(v_out), k
+    for run in (1, 2, 3, 50, 117, 200):
+        _, _, x, c = agauss_trial_op(oracle, run)
+        assert bits_equal(rows[run - 1, 2:], [x[2], x[c.node("in")], x[c.node("out")]]), run
+    # the spread the docs state: sigma(v(out)) about 3.8 mV over 200 trials
+    assert 3.0e-3 < rows[:, 4].std() < 4.6e-3
+
+
+def test_cli_monte_rawfile(tmp_path, engine):
+    say("case: espice --rawfile=FILE --format=print monte_agauss.sp")
+    deck = tmp_path / "monte_agauss.sp"
+    deck.write_text(decks.MONTE_AGAUSS_SP)
+    raw = tmp_path / "mc.txt"
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    r = subprocess.run([exe, f"--rawfile={raw}", "--format=print", str(deck)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = raw.read_text().splitlines()
+    assert lines[:3] == ["Monte Carlo: Divider spread with agauss and a Monte Carlo DC",
+                         "Index\trun\ti(v1)\tv(in)\tv(out)",
+                         "-----\t---------------\t---------------\t---------------\t---------------"]
+    assert lines[3].startswith("0\t1e0\t-4.98898735103262e-4\t1e0\t")
+    assert lines[5] == "2\t3e0\t-5.003152460377415e-4\t1e0\t4.957336298692346e-1"
+    assert len(lines) == 203

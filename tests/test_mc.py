@@ -20,8 +20,13 @@ import numpy as np
 import pytest
 
 import egspice_amd as E
+import oracle_lib as O
 
 GAMMA = 0x9e3779b97f4a7c15
+
+
+def P(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
 SPLITMIX64_FROM_0 = [0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f, 0xf88bb8a8724c81ec]
 
 
@@ -92,3 +97,63 @@ def test_engine_draws_equal_oracle(oracle, engine, fn, args):
     a, ap = args_ptr(args)
     want = np.array([oracle.or_trial_draw(seed, 1 + l, site, fn, ap, len(a)) for l in range(n)])
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+# ------------------------------------------------ deck-level Monte Carlo
+
+def agauss_trial_op(oracle, trial, seed=7):
+    """One `.dc monte` trial of docs/examples/monte_agauss.sp on the oracle:
+    r1v and r2v drawn at their `.param` sites (siteKey(nameKey(nameKey(0,
+    ""), name), 0), netlist.zig:3564-3601), then the operating point warm
+    from the nominal one (sweep/lanes.zig:73-83). Returns (r1, r2, x)."""
+    args = np.array([1000.0, 30.0, 3.0])
+    r = [oracle.or_trial_draw(seed, trial, oracle.or_param_site(nm, 0), 0, P(args), 3) for nm in (b"r1v", b"r2v")]
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["in", "0"], {0: 1.0})
+    c.add(O.RES, ["in", "out"], {0: 1000.0})
+    c.add(O.RES, ["out", "0"], {0: 1000.0})
+    c.freeze()
+    nom = O.Lanes(c, 1)
+    nom.prepare()
+    x0 = nom.newton_op(x0=np.zeros(c.n))[0].reshape(-1)
+    L = O.Lanes(c, 1)
+    L.set_param(O.RES, 0, 0, [r[0]])
+    L.set_param(O.RES, 1, 0, [r[1]])
+    L.prepare()
+    x, st = L.newton_op(x0=x0, init_fix=False)[:2]
+    assert st[0] & 1
+    return r[0], r[1], np.asarray(x).reshape(-1), c
+
+
+def test_monte_agauss_rows(oracle):
+    """docs/learn/monte-carlo.md:62-69 at full precision: the draws (the
+    wyhash site keys, the sampler, Acklam's quantile, agauss's scale)
+    reproduce i(v1) and v(in) of the first three trials bit for bit; v(out)
+    agrees to within one ulp (trial 3 bit for bit). The reference's last
+    bit of v(out) in trials 1-2 is not reproduced by this Newton from the
+    nominal point: parity unpinned for that bit."""
+    import decks
+    for run, i_v1, v_in, v_out in decks.MONTE_AGAUSS_ROWS:
+        r1, r2, x, c = agauss_trial_op(oracle, run)
+        assert 970 < r1 < 1030 and 970 < r2 < 1030
+        assert x[2] == i_v1, run
+        assert x[c.node("in")] == v_in, run
+        assert abs(x[c.node("out")] - v_out) <= np.spacing(v_out), run
+    assert agauss_trial_op(oracle, 3)[2][3] == decks.MONTE_AGAUSS_ROWS[2][3]
+
+
+def test_wyhash_structure(oracle):
+    """The hash's own invariants: seeds and every length class (0, 1-3,
+    4-16, 17-48, > 48 bytes) give distinct keys, and a byte change anywhere
+    changes the key; the keys that matter are pinned end to end above."""
+    seen = set()
+    for n in (0, 1, 2, 3, 4, 7, 8, 15, 16, 17, 31, 48, 49, 96, 97, 200):
+        key = bytes(range(1, n + 1))
+        h = oracle.or_wyhash(0, key, n)
+        assert h not in seen
+        seen.add(h)
+        assert oracle.or_wyhash(1, key, n) != h
+        for k in range(n):
+            flipped = bytearray(key)
+            flipped[k] ^= 0x80
+            assert oracle.or_wyhash(0, bytes(flipped), n) != h
