@@ -662,6 +662,39 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             }
             c.i++;
             e.wave = w == "pulse" ? 1 : 2;
+        } else if (w == "pwl" && e.letter == 'v' && e.wave == 0) {
+            // (time, value) pairs (frontend/builder.zig:2848-2859), then
+            // `r=` and `td=` on the card
+            c.i++;
+            if (!c.next_is("(")) {
+                err = "BadPwl " + e.name;
+                return false;
+            }
+            c.i++;
+            while (c.more() && !c.next_is(")")) {
+                Value v;
+                if (!to_value(c.peek(), v, err)) {
+                    if (err.empty()) err = "BadPwl " + e.name;
+                    return false;
+                }
+                e.pwl.push_back(v);
+                c.i++;
+            }
+            if (!c.next_is(")") || e.pwl.empty() || e.pwl.size() % 2) {
+                err = "BadPwl " + e.name;
+                return false;
+            }
+            if (e.pwl.size() / 2 > 16) {
+                err = "PwlTooLong " + e.name;
+                return false;
+            }
+            c.i++;
+            e.wave = 4;
+            while (c.i + 2 < t.size() && (c.peek() == "r" || c.peek() == "td") && t[c.i + 1] == "=") {
+                Value &dst = c.peek() == "r" ? e.pwl_r : e.pwl_td;
+                if (!to_value(t[c.i + 2], dst, err)) return false;
+                c.i += 3;
+            }
         } else if (w == "sin" || w == "pwl" || w == "exp" || w == "sffm" || w == "am" || w == "pulse") {
             err = "UnsupportedWaveform " + w;
             return false;
@@ -968,7 +1001,20 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         switch (e.letter) {
         case 'v': {
             kind = ESPICE_LANES_VSOURCE;
-            if (e.wave) {
+            if (e.wave == 4) {
+                // PWL: point count (slot 15), r= (16), td= (17), times from
+                // slot 18 and values from slot 34
+                prm.emplace_back(1, lit(4.0));
+                const size_t np = e.pwl.size() / 2;
+                prm.emplace_back(15, lit((double)np));
+                if (e.pwl_r.given) prm.emplace_back(16, e.pwl_r);
+                if (e.pwl_td.given) prm.emplace_back(17, e.pwl_td);
+                for (size_t k = 0; k < np; k++) {
+                    prm.emplace_back((uint32_t)(18 + k), e.pwl[2 * k]);
+                    prm.emplace_back((uint32_t)(34 + k), e.pwl[2 * k + 1]);
+                }
+                if (e.value.given) prm.emplace_back(0, e.value);
+            } else if (e.wave) {
                 // PULSE args go to slots 2..8, SIN args to 9..14; a dc left
                 // unwritten is filled from the waveform's t = 0 value when
                 // the lane constants are prepared

@@ -23,13 +23,19 @@ enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_M
 
 // User parameter slots (NaN = not given).
 // waveform: 0 DC, 1 PULSE(V1 V2 TD TR TF PW PER), 2 SIN(VO VA FREQ TD THETA
-// PHASE).
+// PHASE), 4 PWL(T1 V1 ...) with its point count, r= and td=; the table holds
+// PWL_CAP points (vsource.va's holds 64; a longer deck table is refused).
+constexpr int PWL_CAP = 16;
 enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_SVO, VSRC_SVA,
-       VSRC_SFREQ, VSRC_STD, VSRC_STHETA, VSRC_SPHASE, VSRC_K };
+       VSRC_SFREQ, VSRC_STD, VSRC_STHETA, VSRC_SPHASE, VSRC_PWL_LEN, VSRC_PWL_R, VSRC_PWL_TD, VSRC_PWL_T,
+       VSRC_PWL_V = VSRC_PWL_T + PWL_CAP, VSRC_K = VSRC_PWL_V + PWL_CAP };
+static_assert(VSRC_PWL_LEN == 15 && VSRC_PWL_R == 16 && VSRC_PWL_TD == 17 && VSRC_PWL_T == 18 && VSRC_PWL_V == 34,
+              "the deck reader writes PWL tables into these slots");
 // Prepared source constants: the user slots with vsource.va's defaults
 // (models/vsource.va:58-66: 0, and -1 sentinels for TR, TF, PW, PER).
 enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
-       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_K };
+       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_PWL_LEN, SRCP_PWL_R, SRCP_PWL_TD, SRCP_PWL_T,
+       SRCP_PWL_V = SRCP_PWL_T + PWL_CAP, SRCP_K = SRCP_PWL_V + PWL_CAP };
 constexpr double VA_PI = 3.14159265358979323846, VA_TWO_PI = 6.28318530717958647652;  // constants.vams
 enum { ISRC_DC = 0, ISRC_K = 1 };
 enum { RES_R = 0, RES_K = 1 };
@@ -284,13 +290,20 @@ inline void prep_kind(int kind, const double *u, double *p)
     case K_MOS1: prep_mos1(u, p); break;
     case K_DIODE: prep_diode(u, p); break;
     case K_VSRC: {
-        const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+        // 0 everywhere but the -1 sentinels of TR, TF, PW, PER and PWL's r=
+        for (int k = 0; k < SRCP_K; k++) {
+            const bool sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
+            p[k] = u[k] == u[k] ? u[k] : sentinel ? -1.0 : 0.0;
+        }
+        // an integer point count within the table
+        const double len = p[SRCP_PWL_LEN];
+        p[SRCP_PWL_LEN] = len >= 1.0 ? (len < (double)PWL_CAP ? __builtin_floor(len) : (double)PWL_CAP) : 0.0;
         // .op biases at dc, which the builder fills from the waveform's t = 0
         // value when the card gives none (models/vsource.va:216-221)
         if (u[VSRC_DC] != u[VSRC_DC]) {
             if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
             else if (p[SRCP_WAVE] == 2.0) p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * e_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+            else if (p[SRCP_WAVE] == 4.0) p[SRCP_DC] = p[SRCP_PWL_V];
         }
         break;
     }
@@ -408,7 +421,40 @@ ESP_HD void pulse_widths(const P &p, double &tr, double &tf, double &pw, double 
     per = src_max(p[SRCP_PER], tr + pw + tf + 1.0e-12);
 }
 
-// models/vsource.va:226-247: the DC value, or PULSE at t (PHASE 0).
+// PWL (models/vsource.va:260-299): the table with r= repeat and td=
+// delay, bisection for the segment, 1 fs the steepest ramp; an empty table is
+// the DC value.
+template <class P>
+ESP_HD double pwl_value(const P &p, double t)
+{
+    const int len = (int)p[SRCP_PWL_LEN];
+    if (len <= 0) return p[SRCP_DC];
+    double te = t - p[SRCP_PWL_TD];
+    const double t_first = p[SRCP_PWL_T], v_first = p[SRCP_PWL_V];
+    const double t_last = p[SRCP_PWL_T + len - 1], v_last = p[SRCP_PWL_V + len - 1];
+    const double r = p[SRCP_PWL_R];
+    if (r >= 0.0 && len > 1 && te > t_last && t_last > r) {
+        const double span = t_last - r;
+        if (span > 0.0) {
+            const double excess = te - r;
+            te = r + (excess - __builtin_floor(excess / span) * span);
+        }
+    }
+    if (te < 0.0 || te <= t_first) return v_first;
+    if (te >= t_last) return v_last;
+    int klo = 0, khi = len - 1;
+    while (khi - klo > 1) {
+        const int k = (klo + khi) / 2;
+        if (p[SRCP_PWL_T + k] > te) khi = k;
+        else klo = k;
+    }
+    const double tp = p[SRCP_PWL_T + klo], vp = p[SRCP_PWL_V + klo];
+    const double tk = p[SRCP_PWL_T + khi], vk = p[SRCP_PWL_V + khi];
+    return vp + (vk - vp) * (te - tp) / src_max(tk - tp, 1.0e-15);
+}
+
+// models/vsource.va:226-299: the DC value, or PULSE (PHASE 0), SIN or PWL
+// at t.
 template <class P>
 ESP_HD double src_value(const P &p, double t)
 {
@@ -420,6 +466,7 @@ ESP_HD double src_value(const P &p, double t)
         return p[SRCP_SVO] + p[SRCP_SVA] * e_sin(VA_TWO_PI * p[SRCP_SFREQ] * t_sin + p[SRCP_SPHASE] * VA_PI / 180.0) *
                                  e_exp(va_min(-p[SRCP_STHETA] * t_sin, 80.0));
     }
+    if (p[SRCP_WAVE] == 4.0) return pwl_value(p, t);
     if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
     double tr, tf, pw, per;
     pulse_widths(p, tr, tf, pw, per);
@@ -479,6 +526,72 @@ ESP_HD double pulse_timer_break(const P &p, double t)
                          timer_next(td + tr + pw + tf, per, t)};
     double best = c[0];
     for (int k = 1; k < 4; k++) best = c[k] < best ? c[k] : best;
+    return best;
+}
+
+// ngspice's PWL breakpoint of a V source (src/device/eval.zig:1651-1697
+// spicePwlBreak): the first table time after the request, rounded from the
+// accepted time; folded by r=; inf past the end of a table that does not
+// repeat. Returns 0 for null (ask the timers), else 1 with *bp set.
+template <class P>
+ESP_HD int pwl_break(const P &p, double t_acc, double tq, double *bp)
+{
+    const int n = (int)p[SRCP_PWL_LEN];
+    const double t_last = p[SRCP_PWL_T + n - 1], r = p[SRCP_PWL_R];
+    const bool repeats = r >= 0.0 && n > 1 && t_last > r;
+    const double period = t_last - r;
+    double time = t_acc - p[SRCP_PWL_TD];
+    if (time > t_last) {
+        if (!repeats) {
+            *bp = inf_();
+            return 1;
+        }
+        time -= r;
+        time -= period * __builtin_floor(time / period);
+        time += r;
+    }
+    const double atime = time + (tq - t_acc);
+    double corner = inf_();
+    bool found = false;
+    for (int k = 0; k < n && !found; k++)
+        if (p[SRCP_PWL_T + k] > atime) {
+            corner = p[SRCP_PWL_T + k];
+            found = true;
+        }
+    if (!found) {
+        if (!repeats) {
+            *bp = inf_();
+            return 1;
+        }
+        for (int k = 0; k < n && !found; k++)
+            if (p[SRCP_PWL_T + k] + period > atime) {
+                corner = p[SRCP_PWL_T + k] + period;
+                found = true;
+            }
+        if (!found) return 0;
+    }
+    const double b = t_acc + (corner - time);
+    if (!(b > tq)) return 0;
+    *bp = b;
+    return 1;
+}
+
+// The V source's next breakpoint after tq (srcBreak, src/device/eval.zig:
+// 1592-1598): ngspice's PULSE corner, else its PWL corner, else vsource's own
+// timers (the PULSE corners, parked at 1e30 for other waveforms, and one
+// one-shot timer per PWL slot, models/vsource.va:137-204), else inf.
+template <class P>
+ESP_HD double src_break(const P &p, double t_acc, double tq)
+{
+    const double pb = pulse_break(p, t_acc, tq);
+    if (pb != inf_()) return pb;
+    double b;
+    if (p[SRCP_WAVE] == 4.0 && p[SRCP_PWL_LEN] > 0 && pwl_break(p, t_acc, tq, &b)) return b;
+    double best = pulse_timer_break(p, tq);
+    for (int k = 0; k < PWL_CAP; k++) {
+        const double at = p[SRCP_PWL_T + k] + p[SRCP_PWL_TD];
+        if (at > tq && at < best) best = at;
+    }
     return best;
 }
 

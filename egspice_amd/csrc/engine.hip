@@ -211,6 +211,11 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         i_prev[i] = 0.0;
     }
     for (uint32_t j = tid; j < n_tape; j += nt) ti_prev[j] = 0.0;
+    // the charge history starts at zero: a template without charge sites
+    // never seeds or rotates it, and its companion terms must read 0, not
+    // whatever an earlier launch left in LDS
+    for (uint32_t i = tid; i < 4 * n; i += nt) ring[i] = 0.0;
+    for (uint32_t j = tid; j < 4 * n_tape; j += nt) tring[j] = 0.0;
     __syncthreads();
     // the .tran card's PULSE and SIN defaults (frontend/builder.zig:845-877)
     for (uint32_t e = tid; e < W.n_inst; e += nt) {

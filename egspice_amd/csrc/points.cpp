@@ -456,7 +456,6 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     if (!esp::read_deck(text, P->bd.deck, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     const esp::Deck &d = P->bd.deck;
     if (!d.op && !d.tran && !d.monte_dc) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
-    if (d.monte_tran) return say(ESPICE_INVALID_ARGUMENT, "UnsupportedAnalysis tran sweep monte");
     if (espice_lanes_template_create(&P->tpl) != ESPICE_OK) return say(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
     if (!esp::build_deck(P->bd, P->tpl, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     for (size_t k = 0; k < o->live_param_count; k++) {
@@ -482,8 +481,9 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     p0.first_id = 0;
     P->queries = P->per_point;
     P->points.push_back(p0);
-    // `.dc monte=N`: trials 1..N, each a point of its own after the nominal
-    for (uint32_t k = 1; k <= d.monte_dc; k++) {
+    // `.dc monte=N` or `.tran ... sweep monte=N`: trials 1..N, each a point
+    // of its own after the nominal
+    for (uint32_t k = 1; k <= d.monte_dc + d.monte_tran; k++) {
         Point pt;
         pt.live = p0.live;
         pt.trial = k;
@@ -693,34 +693,71 @@ std::string problem_report(espice_problem *problem)
         std::snprintf(b, sizeof b, "%-20s", name.c_str());
         out += std::string(b) + "=" + body + "\n";
     };
+    // a card's printed line for one point
+    auto card_line = [&](const Point &pt, size_t k) {
+        const Meas &m = d.meas[k];
+        const double v = pt.failed ? NAN : pt.meas[k];
+        const double a0 = pt.aux.size() > 2 * k ? pt.aux[2 * k] : NAN;
+        const double a1 = pt.aux.size() > 2 * k ? pt.aux[2 * k + 1] : NAN;
+        if (std::isnan(v)) {
+            line(m.name, "  NaN");
+            return;
+        }
+        if (m.kind == "trig_targ") line(m.name, "  " + sci(v, 6) + " targ=  " + sci(a0, 6) + " trig=  " + sci(a1, 6));
+        else if (m.kind == "when") line(m.name, "   " + sci(v, 5));
+        else if (m.kind == "rms" || m.kind == "integ")
+            line(m.name, "   " + sci(v, 5) + " from=  " + sci(a0, 5) + " to=  " + sci(a1, 5));
+        else if (m.kind == "avg") {
+            std::map<std::string, double> env;
+            std::string err;
+            esp::param_env(d, {}, env, err);
+            const double from = val_of(m.a.from, env, 0.0, err);
+            line(m.name, "  " + sci(v, 6) + " from=  " + sci(from, 6) + " to=  " + sci(a0, 6));
+        } else if (m.kind == "min" || m.kind == "max")
+            line(m.name, "  " + sci(v, 6) + " at=  " + sci(a0, 6));
+        else if (m.kind == "param") line(m.name, "  " + sci(v, 5));
+        else line(m.name, "  " + sci(v, 6));
+    };
     for (const char *an : {"op", "tran"}) {
         bool headed = false;
         for (size_t k = 0; k < d.meas.size(); k++) {
-            const Meas &m = d.meas[k];
-            if (m.analysis != an) continue;
+            if (d.meas[k].analysis != an) continue;
             if (!headed) head(std::string(an) == "op" ? "Operating Point" : "Transient");
             headed = true;
-            const double v = pt.failed ? NAN : pt.meas[k];
-            const double a0 = pt.aux.size() > 2 * k ? pt.aux[2 * k] : NAN;
-            const double a1 = pt.aux.size() > 2 * k ? pt.aux[2 * k + 1] : NAN;
-            if (std::isnan(v)) {
-                line(m.name, "  NaN");
-                continue;
+            card_line(pt, k);
+        }
+    }
+    if (d.monte_tran) {
+        // each trial's lines, then every card's statistics over the trials
+        // where it is finite: Welford's mean and M2, sigma with Bessel's
+        // correction (src/output/measure.zig:816-830, 904-943)
+        std::vector<const Point *> trials;
+        for (const Point &q : P->points)
+            if (q.trial != 0 && q.ran) trials.push_back(&q);
+        for (const Point *q : trials)
+            for (size_t k = 0; k < d.meas.size(); k++)
+                if (d.meas[k].analysis == "tran") card_line(*q, k);
+        bool heading = false;
+        for (size_t k = 0; k < d.meas.size(); k++) {
+            if (d.meas[k].analysis != "tran") continue;
+            double n = 0, mean = 0, m2 = 0, lo = INFINITY, hi = -INFINITY;
+            for (const Point *q : trials) {
+                const double x = q->failed ? NAN : q->meas[k];
+                if (!std::isfinite(x)) continue;
+                n += 1;
+                const double delta = x - mean;
+                mean += delta / n;
+                m2 += delta * (x - mean);
+                lo = std::fmin(lo, x);
+                hi = std::fmax(hi, x);
             }
-            if (m.kind == "trig_targ") line(m.name, "  " + sci(v, 6) + " targ=  " + sci(a0, 6) + " trig=  " + sci(a1, 6));
-            else if (m.kind == "when") line(m.name, "   " + sci(v, 5));
-            else if (m.kind == "rms" || m.kind == "integ")
-                line(m.name, "   " + sci(v, 5) + " from=  " + sci(a0, 5) + " to=  " + sci(a1, 5));
-            else if (m.kind == "avg") {
-                std::map<std::string, double> env;
-                std::string err;
-                esp::param_env(d, {}, env, err);
-                const double from = val_of(m.a.from, env, 0.0, err);
-                line(m.name, "  " + sci(v, 6) + " from=  " + sci(from, 6) + " to=  " + sci(a0, 6));
-            } else if (m.kind == "min" || m.kind == "max")
-                line(m.name, "  " + sci(v, 6) + " at=  " + sci(a0, 6));
-            else if (m.kind == "param") line(m.name, "  " + sci(v, 5));
-            else line(m.name, "  " + sci(v, 6));
+            if (n == 0) continue;
+            if (!heading) out += "\n  Monte Carlo statistics over " + std::to_string(trials.size()) + " trials\n\n";
+            heading = true;
+            const std::string &nm = d.meas[k].name;
+            const double sigma = n > 1 ? std::sqrt(m2 / (n - 1)) : 0;
+            out += "mean(" + nm + ") = " + sci(mean, 6) + "\nsigma(" + nm + ") = " + sci(sigma, 6) + "\nmin(" + nm +
+                   ") = " + sci(lo, 6) + "\nmax(" + nm + ") = " + sci(hi, 6) + "\n";
         }
     }
     out += d.title + ": " + std::to_string(d.elements.size()) + " devices\n";

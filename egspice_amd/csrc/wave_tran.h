@@ -374,8 +374,7 @@ ESP_HD double tran_next_bp(const WaveTemplate &W, const double *prm, double *bpc
         double *c = bpc + 2 * e;
         if (tq < c[0] || tq >= c[1]) {
             c[0] = tq;
-            c[1] = pulse_break(LP{prm + ie.prm}, t_acc, tq);
-            if (c[1] == inf_()) c[1] = pulse_timer_break(LP{prm + ie.prm}, tq);
+            c[1] = src_break(LP{prm + ie.prm}, t_acc, tq);
         }
         best = c[1] < best ? c[1] : best;
     }

@@ -329,15 +329,24 @@ void or_lanes_prepare(or_lanes *L)
                 case OR_DIODE: or_prep_diode(u, p); break;
                 case OR_VSRC: {
                     /* vsource.va parameter defaults; NaN = not given */
-                    const double dflt[SRCP_K] = {0.0, 0.0, 0.0, 0.0, 0.0, -1.0, -1.0, -1.0, -1.0,
-                                                 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    for (int k = 0; k < SRCP_K; k++) p[k] = u[k] == u[k] ? u[k] : dflt[k];
+                    /* vsource.va's defaults: 0, the -1 sentinels of TR, TF, PW,
+                     * PER and PWL's r=; the point count an integer in the table */
+                    for (int k = 0; k < SRCP_K; k++) {
+                        const int sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
+                        p[k] = u[k] == u[k] ? u[k] : sentinel ? -1.0 : 0.0;
+                    }
+                    {
+                        const double len = p[SRCP_PWL_LEN];
+                        p[SRCP_PWL_LEN] = len >= 1.0 ? (len < OR_PWL_CAP ? floor(len) : OR_PWL_CAP) : 0.0;
+                    }
                     /* dc not given: the waveform's t = 0 value, as the builder
                      * fills it (models/vsource.va:216-221) */
                     if (u[OR_VSRC_DC] != u[OR_VSRC_DC]) {
                         if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
                         else if (p[SRCP_WAVE] == 2.0)
                             p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * om_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+                        else if (p[SRCP_WAVE] == 4.0)
+                            p[SRCP_DC] = p[SRCP_PWL_V];
                     }
                     break;
                 }

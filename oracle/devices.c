@@ -601,9 +601,37 @@ static void pulse_widths(const double *p, double *tr, double *tf, double *pw, do
     *per = vmax(p[SRCP_PER], *tr + *pw + *tf + 1.0e-12);
 }
 
-/* models/vsource.va:226-247: DC, or PULSE at t (PHASE 0). */
+/* models/vsource.va:260-299, PWL: te = t - td folded into [r, t_last] by
+ * r=, v_first before the first point, v_last after the last, else the
+ * segment bisection finds, its ramp at most 1 fs steep. */
+static double pwl_at(const double *p, double t)
+{
+    const int n = (int)p[SRCP_PWL_LEN];
+    if (n <= 0) return p[SRCP_DC];
+    const double *T = p + SRCP_PWL_T, *V = p + SRCP_PWL_V;
+    double te = t - p[SRCP_PWL_TD];
+    if (p[SRCP_PWL_R] >= 0.0 && n > 1 && te > T[n - 1] && T[n - 1] > p[SRCP_PWL_R]) {
+        const double span = T[n - 1] - p[SRCP_PWL_R];
+        if (span > 0.0) {
+            const double excess = te - p[SRCP_PWL_R];
+            te = p[SRCP_PWL_R] + (excess - floor(excess / span) * span);
+        }
+    }
+    if (te < 0.0 || te <= T[0]) return V[0];
+    if (te >= T[n - 1]) return V[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (T[mid] > te) hi = mid;
+        else lo = mid;
+    }
+    return V[lo] + (V[hi] - V[lo]) * (te - T[lo]) / vmax(T[hi] - T[lo], 1.0e-15);
+}
+
+/* models/vsource.va:226-299: DC, or PULSE at t (PHASE 0), SIN, PWL. */
 double or_source_value(const double *p, double t)
 {
+    if (p[SRCP_WAVE] == 4.0) return pwl_at(p, t);
     if (p[SRCP_WAVE] == 2.0) {
         /* SIN, models/vsource.va:233-244 */
         const double t_sin = t - p[SRCP_STD];
@@ -669,4 +697,62 @@ double or_pulse_break(const double *p, double t_acc, double tq)
     const double corner = atime < 0.0 ? 0.0 : atime < tr ? tr : atime < tr + pw ? tr + pw : atime < tr + pw + tf ? tr + pw + tf : per;
     const double bp = t_acc + (corner - time);
     return bp > tq ? bp : INFINITY;
+}
+
+/* src/device/eval.zig:1661-1697 spicePwlBreak: 1 with *bp set (a corner, or
+ * INFINITY past a table that does not repeat), 0 for none (the timers then
+ * answer). */
+static int pwl_break(const double *p, double t_acc, double tq, double *bp)
+{
+    const int n = (int)p[SRCP_PWL_LEN];
+    const double *T = p + SRCP_PWL_T, r = p[SRCP_PWL_R];
+    const int repeats = r >= 0.0 && n > 1 && T[n - 1] > r;
+    const double period = T[n - 1] - r;
+    double time = t_acc - p[SRCP_PWL_TD];
+    if (time > T[n - 1]) {
+        if (!repeats) {
+            *bp = INFINITY;
+            return 1;
+        }
+        time = time - r;
+        time = time - period * floor(time / period);
+        time = time + r;
+    }
+    const double atime = time + (tq - t_acc);
+    int k = 0;
+    while (k < n && !(T[k] > atime)) k++;
+    double corner;
+    if (k < n) {
+        corner = T[k];
+    } else {
+        if (!repeats) {
+            *bp = INFINITY;
+            return 1;
+        }
+        k = 0;
+        while (k < n && !(T[k] + period > atime)) k++;
+        if (k == n) return 0;
+        corner = T[k] + period;
+    }
+    const double b = t_acc + (corner - time);
+    if (!(b > tq)) return 0;
+    *bp = b;
+    return 1;
+}
+
+/* srcBreak (src/device/eval.zig:1592-1598): the PULSE corner, else the PWL
+ * corner, else the source's timers: the four PULSE corners and one
+ * one-shot timer per PWL slot at t_k + td (models/vsource.va:137-204). */
+double or_src_break(const double *p, double t_acc, double tq)
+{
+    const double pb = or_pulse_break(p, t_acc, tq);
+    if (pb != INFINITY) return pb;
+    double b;
+    if (p[SRCP_WAVE] == 4.0 && p[SRCP_PWL_LEN] > 0 && pwl_break(p, t_acc, tq, &b)) return b;
+    double best = or_pulse_timer_break(p, tq);
+    for (int k = 0; k < OR_PWL_CAP; k++) {
+        const double at = p[SRCP_PWL_T + k] + p[SRCP_PWL_TD];
+        if (at > tq && at < best) best = at;
+    }
+    return best;
 }

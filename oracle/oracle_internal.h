@@ -39,8 +39,10 @@ void or_seed_diode(const double *p, const double *cur, double *lim);
  * 1 PULSE, 2 SIN), PULSE(V1 V2 TD TR TF PW PER) with the -1 sentinels of
  * models/vsource.va:58-66 for TR, TF, PW and PER not given, and SIN(VO VA
  * FREQ TD THETA PHASE). */
+#define OR_PWL_CAP 16 /* PWL points a table holds here (vsource.va: 64) */
 enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
-       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_K };
+       SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_PWL_LEN, SRCP_PWL_R, SRCP_PWL_TD, SRCP_PWL_T,
+       SRCP_PWL_V = SRCP_PWL_T + OR_PWL_CAP, SRCP_K = SRCP_PWL_V + OR_PWL_CAP };
 /* The source's value at transient time t (models/vsource.va:233-247). */
 double or_source_value(const double *p, double t);
 #define VA_PI 3.14159265358979323846     /* constants.vams M_PI */
@@ -49,6 +51,7 @@ double or_source_value(const double *p, double t);
  * t_acc (src/device/eval.zig:1636-1650 spicePulseBreak); inf when none. */
 double or_pulse_break(const double *p, double t_acc, double tq);
 double or_pulse_timer_break(const double *p, double t);
+double or_src_break(const double *p, double t_acc, double tq);
 
 /* What one evaluation adds beyond the static planes: the transient's time
  * (sources read their waveform), and the charge outputs. */

@@ -156,8 +156,7 @@ static double batch_next(tr_bps *b, double t_acc, double t)
         src_cache *c = &b->cache[i];
         if (t < c->req || t >= c->at) {
             c->req = t;
-            c->at = or_pulse_break(b->prm[i], t_acc, t);
-            if (c->at == INFINITY) c->at = or_pulse_timer_break(b->prm[i], t);
+            c->at = or_src_break(b->prm[i], t_acc, t);
         }
         best = c->at < best ? c->at : best;
     }

@@ -98,5 +98,23 @@ MONTE_AGAUSS_ROWS = [
     (3, -5.003152460377415e-4, 1.0, 4.957336298692346e-1),
 ]
 
+MONTE_TRAN_SP = """RC delay spread over Monte Carlo trials
+* R varies by 3 % (one sigma). The 50 % crossing is at R C ln 2 = 693 ns.
+.param rv=agauss(1k, 30, 1)
+V1 in 0 PWL(0 0 1p 1)
+R1 in out 'rv'
+C1 out 0 1n
+.option seed=3
+.tran 10n 2u sweep monte=50
+.meas tran tc WHEN v(out)=0.5 RISE=1
+.end
+"""
+
+# docs/learn/monte-carlo.md:80-90: the last two trials' tc lines and the
+# statistics over the 50 trials, as the reference printed them.
+MONTE_TRAN_TAIL = ["tc                  =   6.81503e-07", "tc                  =   6.87338e-07"]
+MONTE_TRAN_STATS = ["mean(tc) = 6.894056e-07", "sigma(tc) = 2.106016e-08", "min(tc) = 6.229492e-07",
+                    "max(tc) = 7.412147e-07"]
+
 MEASURE_PARAM_SP = MEASURE_SP.replace("R1 in out 1k", ".param r=1k c=1n\nR1 in out {r}").replace(
     "C1 out 0 1n", "C1 out 0 {c}")

@@ -310,3 +310,17 @@ def test_cli_monte_rawfile(tmp_path, engine):
     assert lines[3].startswith("0\t1e0\t-4.98898735103262e-4\t1e0\t")
     assert lines[5] == "2\t3e0\t-5.003152460377415e-4\t1e0\t4.957336298692346e-1"
     assert len(lines) == 203
+
+
+def test_monte_tran_statistics(engine):
+    say("case: monte_tran.sp, PWL source, .tran sweep monte=50 with its WHEN statistics")
+    pb = E.Problem(decks.MONTE_TRAN_SP)
+    assert pb.query_count() == 51
+    pb.run_all()
+    lines = pb.report().splitlines()
+    stats_at = lines.index("  Monte Carlo statistics over 50 trials")
+    assert lines[stats_at + 2:stats_at + 6] == decks.MONTE_TRAN_STATS
+    assert lines[stats_at - 3:stats_at - 1] == decks.MONTE_TRAN_TAIL
+    # the nominal point: R C ln 2 within the step control's resolution
+    m0, r0 = pb.measures(0, 1, ["tc"])
+    assert r0.tolist() == [0] and abs(m0[0, 0] - 1e3 * 1e-9 * math.log(2)) < 1e-9

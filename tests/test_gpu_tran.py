@@ -293,6 +293,31 @@ def test_collapsed_diode_charge_lanes_bitwise(oracle, engine):
         assert (res["status"] == 0).all()
 
 
+def test_pwl_lanes_bitwise(oracle, engine):
+    """A PWL source (4 points, with r= repeat and td= delay per lane) into
+    the RC of measure.sp: its corners land as ngspice rounds them, its
+    repeats fold; every row bitwise the oracle's."""
+    N = 16
+    say(f"case: PWL RC, {N} lanes (r= and td= varied), tran 10n 4u")
+
+    def deck(add):
+        p = {0: np.nan, 1: 4.0, 15: 4.0}
+        for k, (tk, vk) in enumerate(zip([0.0, 1e-7, 3e-7, 6e-7], [0.0, 1.0, 1.0, -0.5])):
+            p[18 + k], p[34 + k] = tk, vk
+        add(E.VSOURCE, ["in", "0"], p)
+        add(E.CAPACITOR, ["out", "0"], {0: 1e-9})
+        add(E.RESISTOR, ["in", "out"], {0: 1e3})
+
+    t, c = build_both(oracle, deck)
+    rng = np.random.default_rng(17)
+    rep = np.where(np.arange(N) % 2 == 0, -1.0, rng.uniform(0.0, 5e-7, N))
+    td = np.where(np.arange(N) % 3 == 0, 0.0, rng.uniform(0.0, 2e-7, N))
+    sets = {(E.VSOURCE, 0, 16): rep, (E.VSOURCE, 0, 17): td, (E.RESISTOR, 0, 0): rng.uniform(200.0, 2000.0, N)}
+    passes = [E.meas_cross(t.node("out"), 0.25, rise=1), E.meas_extremum(t.node("out"), 0)]
+    res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=4e-6, dt_init=1e-8), passes)
+    assert (res["status"] == 0).all()
+
+
 def test_diode_rs_topology_is_per_template(engine):
     say("case: a lane may not move a diode's RS across zero")
     t = E.Template()

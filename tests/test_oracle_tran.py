@@ -6,6 +6,9 @@ reference's printed outputs:
   - docs/examples/cmos_inverter.sp (docs/learn/models.md:95-102, mos1):
     2032 points and tphl, trig and targ to their printed digits (its second
     period's corners come from vsource's own timers);
+  - docs/examples/monte_tran.sp (docs/learn/monte-carlo.md:80-90): a PWL
+    source, 50 Monte Carlo trials of an agauss'd R; each trial's WHEN and
+    their mean, sigma, min and max to the printed digits;
   - docs/examples/diode_rectifier.sp (docs/learn/models.md:38-45): a SIN
     source into a diode with RS (an internal row) and junction charge, 610
     points, vpeak and vmin with their times to the printed digits;
@@ -205,3 +208,74 @@ def test_diode_rectifier_physics(oracle):
     assert np.allclose((vin - x[:, ai]) / 0.5, -ib, rtol=1e-9, atol=1e-12)
     k = int(np.argmax(x[:, c.node("out")]))
     assert 0.5 < vin[k] - x[k, c.node("out")] < 0.9
+
+
+def pwl_rc(oracle, r, times, values, rep=None, td=None):
+    """V1 in 0 PWL(...) [r=] [td=]; C1 out 0 1n; R1 in out r (builder order v,
+    c, r). PWL slots: count 15, r= 16, td= 17, times 18.., values 34.."""
+    from egspice_amd import spice_number as sn
+    p = {0: np.nan, 1: 4.0, 15: float(len(times))}
+    if rep is not None:
+        p[16] = rep
+    if td is not None:
+        p[17] = td
+    for k, (tk, vk) in enumerate(zip(times, values)):
+        p[18 + k] = tk
+        p[34 + k] = vk
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["in", 0], p)
+    c.add(O.CAP, ["out", 0], {0: sn("1n")})
+    c.add(O.RES, ["in", "out"], {0: r})
+    return c.freeze()
+
+
+def test_monte_tran_statistics(oracle):
+    from egspice_amd import spice_number as sn
+    import decks
+    vals, lines = [], []
+    for trial in range(1, 51):
+        args = np.array([sn("1k"), 30.0, 1.0])
+        r = oracle.or_trial_draw(3, trial, oracle.or_param_site(b"rv", 0), 0, P(args), 3)
+        c = pwl_rc(oracle, r, [0.0, sn("1p")], [0.0, 1.0])
+        L = O.Lanes(c, 1)
+        L.prepare()
+        t, x, st = L.tran(0, O.tran_opts(sn("2u"), sn("10n")))
+        assert st.status == 0
+        tt = np.ascontiguousarray(t)
+        tc = oracle.or_meas_cross(P(tt), P(np.ascontiguousarray(x[:, c.node("out")])), len(tt),
+                                  C.byref(O.clause(val=0.5, rise=1)))
+        vals.append(tc)
+        lines.append("%-20s=   %.5e" % ("tc", tc))
+    assert lines[-2:] == decks.MONTE_TRAN_TAIL
+    n = mean = m2 = 0.0
+    for v in vals:
+        n += 1
+        d = v - mean
+        mean += d / n
+        m2 += d * (v - mean)
+    got = ["mean(tc) = %.6e" % mean, "sigma(tc) = %.6e" % math.sqrt(m2 / (n - 1)), "min(tc) = %.6e" % min(vals),
+           "max(tc) = %.6e" % max(vals)]
+    assert got == decks.MONTE_TRAN_STATS
+
+
+def test_pwl_waveform_and_corners(oracle):
+    """PWL at every accepted time equals the table's linear interpolation
+    (numpy's interp on the same points), the repeat r= folds the waveform,
+    and every table corner (shifted by td=) is an accepted time."""
+    times, values = [0.0, 1e-7, 3e-7, 6e-7], [0.0, 1.0, 1.0, -0.5]
+    for rep, td in ((None, None), (1e-7, None), (None, 2e-8), (0.0, 5e-8)):
+        c = pwl_rc(oracle, 1e3, times, values, rep=rep, td=td)
+        L = O.Lanes(c, 1)
+        L.prepare()
+        t, x, st = L.tran(0, O.tran_opts(2e-6, 1e-8))
+        assert st.status == 0
+        te = t - (td or 0.0)
+        if rep is not None:
+            last = times[-1]
+            span = last - rep
+            over = te > last
+            te = np.where(over, rep + np.mod(te - rep, span), te)
+        want = np.interp(te, times, values, left=values[0], right=values[-1])
+        assert np.allclose(x[:, c.node("in")], want, rtol=0, atol=1e-12), (rep, td)
+        for tk in times[1:]:
+            assert np.min(np.abs(t - (tk + (td or 0.0)))) < 1e-18, (rep, td, tk)
