@@ -635,12 +635,12 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             c.i++;
             double x;
             for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
-        } else if ((w == "pulse" || w == "sin") && e.letter == 'v' && e.wave == 0) {
+        } else if ((w == "pulse" || w == "sin" || w == "exp") && e.letter == 'v' && e.wave == 0) {
             // positional arguments fill the waveform's slots in order; the
             // ones not written keep the model defaults
             // (frontend/builder.zig:2841-2871, 2985-2990)
             const int most = w == "pulse" ? 7 : 6;
-            const std::string bad = (w == "pulse" ? "BadPulse " : "BadSin ") + e.name;
+            const std::string bad = (w == "pulse" ? "BadPulse " : w == "sin" ? "BadSin " : "BadExp ") + e.name;
             c.i++;
             if (!c.next_is("(")) {
                 err = bad;
@@ -661,7 +661,7 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
                 return false;
             }
             c.i++;
-            e.wave = w == "pulse" ? 1 : 2;
+            e.wave = w == "pulse" ? 1 : w == "sin" ? 2 : 3;
         } else if (w == "pwl" && e.letter == 'v' && e.wave == 0) {
             // (time, value) pairs (frontend/builder.zig:2848-2859), then
             // `r=` and `td=` on the card
@@ -1015,11 +1015,18 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
                 }
                 if (e.value.given) prm.emplace_back(0, e.value);
             } else if (e.wave) {
-                // PULSE args go to slots 2..8, SIN args to 9..14; a dc left
-                // unwritten is filled from the waveform's t = 0 value when
-                // the lane constants are prepared
-                const uint32_t base = e.wave == 1 ? 2 : 9;
+                // PULSE args go to slots 2..8, SIN args to 9..14, EXP args
+                // to 50..55; a dc left unwritten is filled from the
+                // waveform's t = 0 value when the lane constants are
+                // prepared. EXP's time constants start at 0 on a card, so
+                // the .tran step fills the omitted ones
+                // (frontend/builder.zig:2860-2866)
+                const uint32_t base = e.wave == 1 ? 2 : e.wave == 2 ? 9 : 50;
                 prm.emplace_back(1, lit(double(e.wave)));
+                if (e.wave == 3) {
+                    if (!e.wave_args[3].given) prm.emplace_back(53, lit(0.0));
+                    if (!e.wave_args[5].given) prm.emplace_back(55, lit(0.0));
+                }
                 for (int a = 0; a < 7; a++)
                     if (e.wave_args[a].given) prm.emplace_back(base + a, e.wave_args[a]);
                 if (e.value.given) prm.emplace_back(0, e.value);

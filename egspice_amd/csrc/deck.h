@@ -61,7 +61,8 @@ struct Element {
     std::string model;                // D and M cards
     Value value;                      // R, C, I value; V DC value
     int wave = 0;                     // V waveform: 0 none, 1 PULSE(v1 v2 td tr tf pw per),
-                                      // 2 SIN(vo va freq td theta phase), 4 PWL(t1 v1 ...)
+                                      // 2 SIN(vo va freq td theta phase),
+                                      // 3 EXP(v1 v2 td1 tau1 td2 tau2), 4 PWL(t1 v1 ...)
     Value wave_args[7];
     std::vector<Value> pwl;           // PWL times and values, interleaved
     Value pwl_r, pwl_td;              // PWL r= and td=

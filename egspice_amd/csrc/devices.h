@@ -23,19 +23,22 @@ enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_M
 
 // User parameter slots (NaN = not given).
 // waveform: 0 DC, 1 PULSE(V1 V2 TD TR TF PW PER), 2 SIN(VO VA FREQ TD THETA
-// PHASE), 4 PWL(T1 V1 ...) with its point count, r= and td=; the table holds
+// PHASE), 3 EXP(V1 V2 TD1 TAU1 TD2 TAU2), 4 PWL(T1 V1 ...) with its point
+// count, r= and td=; the table holds
 // PWL_CAP points (vsource.va's holds 64; a longer deck table is refused).
 constexpr int PWL_CAP = 16;
 enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_SVO, VSRC_SVA,
        VSRC_SFREQ, VSRC_STD, VSRC_STHETA, VSRC_SPHASE, VSRC_PWL_LEN, VSRC_PWL_R, VSRC_PWL_TD, VSRC_PWL_T,
-       VSRC_PWL_V = VSRC_PWL_T + PWL_CAP, VSRC_K = VSRC_PWL_V + PWL_CAP };
+       VSRC_PWL_V = VSRC_PWL_T + PWL_CAP, VSRC_EV1 = VSRC_PWL_V + PWL_CAP, VSRC_EV2, VSRC_ETD1, VSRC_ETAU1, VSRC_ETD2,
+       VSRC_ETAU2, VSRC_K };
 static_assert(VSRC_PWL_LEN == 15 && VSRC_PWL_R == 16 && VSRC_PWL_TD == 17 && VSRC_PWL_T == 18 && VSRC_PWL_V == 34,
               "the deck reader writes PWL tables into these slots");
 // Prepared source constants: the user slots with vsource.va's defaults
 // (models/vsource.va:58-66: 0, and -1 sentinels for TR, TF, PW, PER).
 enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
        SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_PWL_LEN, SRCP_PWL_R, SRCP_PWL_TD, SRCP_PWL_T,
-       SRCP_PWL_V = SRCP_PWL_T + PWL_CAP, SRCP_K = SRCP_PWL_V + PWL_CAP };
+       SRCP_PWL_V = SRCP_PWL_T + PWL_CAP, SRCP_EV1 = SRCP_PWL_V + PWL_CAP, SRCP_EV2, SRCP_ETD1, SRCP_ETAU1, SRCP_ETD2,
+       SRCP_ETAU2, SRCP_K };
 constexpr double VA_PI = 3.14159265358979323846, VA_TWO_PI = 6.28318530717958647652;  // constants.vams
 enum { ISRC_DC = 0, ISRC_K = 1 };
 enum { RES_R = 0, RES_K = 1 };
@@ -290,10 +293,12 @@ inline void prep_kind(int kind, const double *u, double *p)
     case K_MOS1: prep_mos1(u, p); break;
     case K_DIODE: prep_diode(u, p); break;
     case K_VSRC: {
-        // 0 everywhere but the -1 sentinels of TR, TF, PW, PER and PWL's r=
+        // 0 everywhere but the -1 sentinels of TR, TF, PW, PER and PWL's r=,
+        // and EXP's 1 ns time constants (models/vsource.va:58-96)
         for (int k = 0; k < SRCP_K; k++) {
             const bool sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
-            p[k] = u[k] == u[k] ? u[k] : sentinel ? -1.0 : 0.0;
+            const double dflt = sentinel ? -1.0 : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9 : 0.0;
+            p[k] = u[k] == u[k] ? u[k] : dflt;
         }
         // an integer point count within the table
         const double len = p[SRCP_PWL_LEN];
@@ -303,6 +308,7 @@ inline void prep_kind(int kind, const double *u, double *p)
         if (u[VSRC_DC] != u[VSRC_DC]) {
             if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
             else if (p[SRCP_WAVE] == 2.0) p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * e_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+            else if (p[SRCP_WAVE] == 3.0) p[SRCP_DC] = p[SRCP_EV1];
             else if (p[SRCP_WAVE] == 4.0) p[SRCP_DC] = p[SRCP_PWL_V];
         }
         break;
@@ -467,6 +473,15 @@ ESP_HD double src_value(const P &p, double t)
                                  e_exp(va_min(-p[SRCP_STHETA] * t_sin, 80.0));
     }
     if (p[SRCP_WAVE] == 4.0) return pwl_value(p, t);
+    if (p[SRCP_WAVE] == 3.0) {
+        // EXP (models/vsource.va:252-258): two one-sided exponentials on V1
+        const double t1 = t - p[SRCP_ETD1], t2 = t - p[SRCP_ETD2];
+        const double v1 = p[SRCP_EV1], v2 = p[SRCP_EV2];
+        double out = v1;
+        if (t1 > 0.0) out = out + (v2 - v1) * (1.0 - e_exp(va_min(-t1 / src_max(p[SRCP_ETAU1], 1.0e-15), 80.0)));
+        if (t2 > 0.0) out = out + (v1 - v2) * (1.0 - e_exp(va_min(-t2 / src_max(p[SRCP_ETAU2], 1.0e-15), 80.0)));
+        return out;
+    }
     if (p[SRCP_WAVE] != 1.0) return p[SRCP_DC];
     double tr, tf, pw, per;
     pulse_widths(p, tr, tf, pw, per);

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     for (uint32_t i = tid; i < 4 * n; i += nt) ring[i] = 0.0;
     for (uint32_t j = tid; j < 4 * n_tape; j += nt) tring[j] = 0.0;
     __syncthreads();
-    // the .tran card's PULSE and SIN defaults (frontend/builder.zig:845-877)
+    // the .tran card's PULSE, SIN and EXP defaults (frontend/builder.zig:845-877)
     for (uint32_t e = tid; e < W.n_inst; e += nt) {
         const InstEntry ie = W.inst[e];
         bpc[2 * e] = inf_();
@@ -225,6 +225,12 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         if (ie.kind != K_VSRC) continue;
         double *p = prm_lds + ie.prm;
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / to.t_stop;
+        if (p[SRCP_WAVE] == 3.0) {
+            if (p[SRCP_ETD1] == 0) p[SRCP_ETD1] = to.dt_init;
+            if (p[SRCP_ETAU1] == 0) p[SRCP_ETAU1] = to.dt_init;
+            if (p[SRCP_ETD2] == 0) p[SRCP_ETD2] = p[SRCP_ETD1] + to.dt_init;
+            if (p[SRCP_ETAU2] == 0) p[SRCP_ETAU2] = to.dt_init;
+        }
         if (p[SRCP_WAVE] != 1.0) {
             p[SRCP_TD] = 1e30;
         } else {

@@ -333,7 +333,8 @@ void or_lanes_prepare(or_lanes *L)
                      * PER and PWL's r=; the point count an integer in the table */
                     for (int k = 0; k < SRCP_K; k++) {
                         const int sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
-                        p[k] = u[k] == u[k] ? u[k] : sentinel ? -1.0 : 0.0;
+                        const double dflt = sentinel ? -1.0 : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9 : 0.0;
+                        p[k] = u[k] == u[k] ? u[k] : dflt;
                     }
                     {
                         const double len = p[SRCP_PWL_LEN];
@@ -345,6 +346,8 @@ void or_lanes_prepare(or_lanes *L)
                         if (p[SRCP_WAVE] == 1.0) p[SRCP_DC] = p[SRCP_V1];
                         else if (p[SRCP_WAVE] == 2.0)
                             p[SRCP_DC] = p[SRCP_SVO] + p[SRCP_SVA] * om_sin(2.0 * VA_PI * p[SRCP_SPHASE] / 360.0);
+                        else if (p[SRCP_WAVE] == 3.0)
+                            p[SRCP_DC] = p[SRCP_EV1];
                         else if (p[SRCP_WAVE] == 4.0)
                             p[SRCP_DC] = p[SRCP_PWL_V];
                     }

@@ -632,6 +632,16 @@ static double pwl_at(const double *p, double t)
 double or_source_value(const double *p, double t)
 {
     if (p[SRCP_WAVE] == 4.0) return pwl_at(p, t);
+    if (p[SRCP_WAVE] == 3.0) {
+        /* EXP, models/vsource.va:252-258 */
+        double out = p[SRCP_EV1];
+        const double e1 = t - p[SRCP_ETD1], e2 = t - p[SRCP_ETD2];
+        if (e1 > 0.0)
+            out = out + (p[SRCP_EV2] - p[SRCP_EV1]) * (1.0 - om_exp(om_min(-e1 / vmax(p[SRCP_ETAU1], 1.0e-15), 80.0)));
+        if (e2 > 0.0)
+            out = out + (p[SRCP_EV1] - p[SRCP_EV2]) * (1.0 - om_exp(om_min(-e2 / vmax(p[SRCP_ETAU2], 1.0e-15), 80.0)));
+        return out;
+    }
     if (p[SRCP_WAVE] == 2.0) {
         /* SIN, models/vsource.va:233-244 */
         const double t_sin = t - p[SRCP_STD];

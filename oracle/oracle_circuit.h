@@ -14,7 +14,9 @@ enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 =
 /* User parameter slots per kind (NaN = not given, the default applies). */
 enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC_TR, OR_VSRC_TF, OR_VSRC_PW, OR_VSRC_PER,
        OR_VSRC_SVO, OR_VSRC_SVA, OR_VSRC_SFREQ, OR_VSRC_STD, OR_VSRC_STHETA, OR_VSRC_SPHASE, OR_VSRC_PWL_LEN,
-       OR_VSRC_PWL_R, OR_VSRC_PWL_TD, OR_VSRC_PWL_T, OR_VSRC_PWL_V = OR_VSRC_PWL_T + 16, OR_VSRC_K = OR_VSRC_PWL_V + 16 };
+       OR_VSRC_PWL_R, OR_VSRC_PWL_TD, OR_VSRC_PWL_T, OR_VSRC_PWL_V = OR_VSRC_PWL_T + 16,
+       OR_VSRC_EXP_V1 = OR_VSRC_PWL_V + 16, OR_VSRC_EXP_V2, OR_VSRC_EXP_TD1, OR_VSRC_EXP_TAU1, OR_VSRC_EXP_TD2,
+       OR_VSRC_EXP_TAU2, OR_VSRC_K };
 enum { OR_ISRC_DC = 0, OR_ISRC_K };
 enum { OR_RES_R = 0, OR_RES_K };
 enum { OR_CAP_C = 0, OR_CAP_K };

@@ -270,6 +270,12 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
         double *p = bps.prm[i];
         or_load_p(L, OR_VSRC, i, l, p);
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / opt->t_stop;
+        if (p[SRCP_WAVE] == 3.0) { /* builder.zig:862-867 */
+            if (p[SRCP_ETD1] == 0) p[SRCP_ETD1] = opt->dt_init;
+            if (p[SRCP_ETAU1] == 0) p[SRCP_ETAU1] = opt->dt_init;
+            if (p[SRCP_ETD2] == 0) p[SRCP_ETD2] = p[SRCP_ETD1] + opt->dt_init;
+            if (p[SRCP_ETAU2] == 0) p[SRCP_ETAU2] = opt->dt_init;
+        }
         if (p[SRCP_WAVE] != 1.0) {
             p[SRCP_TD] = 1e30;
         } else {

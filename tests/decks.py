@@ -79,6 +79,23 @@ DIODE_RECTIFIER_GOLDEN = {"vpeak": (9.228482e+00, 6), "vpeak.at": (5.173860e-03,
                           "vmin.at": (4.317386e-02, 6)}
 DIODE_RECTIFIER_POINTS = 610
 
+SOURCES_SP = """Independent source waveforms
+* Each source drives its own 1k load, so every node shows its waveform.
+VP p 0 PULSE(0 1 1m 0.1m 0.1m 2m 5m)
+VS s 0 SIN(0.5 0.5 1k)
+VW w 0 PWL(0 0 1m 1 3m 1 4m 0)
+VE e 0 EXP(0 1 1m 0.5m 3m 0.5m)
+IC 0 c DC 1m
+RP p 0 1k
+RS s 0 1k
+RW w 0 1k
+RE e 0 1k
+RC c 0 1k
+.tran 10u 5m
+.end
+"""
+SOURCES_POINTS = 526  # docs/learn/sources.md:55-58: 10 devices, 10 variables
+
 MONTE_AGAUSS_SP = """Divider spread with agauss and a Monte Carlo DC
 * agauss(nominal, variation, sigmas): 30 ohms is the 3-sigma spread.
 .param r1v=agauss(1k, 30, 3) r2v=agauss(1k, 30, 3)

@@ -318,6 +318,32 @@ def test_pwl_lanes_bitwise(oracle, engine):
     assert (res["status"] == 0).all()
 
 
+def test_exp_lanes_bitwise(oracle, engine):
+    """An EXP source (models/vsource.va:252-258) into measure.sp's RC, its
+    delays and time constants varied per lane, some left at 0 so the
+    .tran step fills them (frontend/builder.zig:862-867); every row bitwise
+    the oracle's."""
+    N = 16
+    say(f"case: EXP RC, {N} lanes (td1 tau1 td2 tau2 varied, zeros filled from the step), tran 10n 4u")
+
+    def deck(add):
+        add(E.VSOURCE, ["in", "0"], {0: np.nan, 1: 3.0, 50: 0.0, 51: 1.0, 52: 2e-7, 53: 1e-7, 54: 2e-6, 55: 3e-7})
+        add(E.CAPACITOR, ["out", "0"], {0: 1e-9})
+        add(E.RESISTOR, ["in", "out"], {0: 1e3})
+
+    t, c = build_both(oracle, deck)
+    rng = np.random.default_rng(23)
+    zero = lambda k: np.where(np.arange(N) % k == 0, 0.0, 1.0)
+    sets = {(E.VSOURCE, 0, 52): zero(4) * rng.uniform(0.0, 1e-6, N),
+            (E.VSOURCE, 0, 53): zero(5) * rng.uniform(2e-8, 5e-7, N),
+            (E.VSOURCE, 0, 54): zero(3) * rng.uniform(1e-6, 3e-6, N),
+            (E.VSOURCE, 0, 55): zero(7) * rng.uniform(2e-8, 5e-7, N),
+            (E.VSOURCE, 0, 51): rng.uniform(-1.0, 2.0, N)}
+    passes = [E.meas_cross(t.node("out"), 0.25, rise=1), E.meas_extremum(t.node("out"), 1)]
+    res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=4e-6, dt_init=1e-8), passes)
+    assert (res["status"] == 0).all()
+
+
 def test_diode_rs_topology_is_per_template(engine):
     say("case: a lane may not move a diode's RS across zero")
     t = E.Template()

@@ -279,3 +279,61 @@ def test_pwl_waveform_and_corners(oracle):
         assert np.allclose(x[:, c.node("in")], want, rtol=0, atol=1e-12), (rep, td)
         for tk in times[1:]:
             assert np.min(np.abs(t - (tk + (td or 0.0)))) < 1e-18, (rep, td, tk)
+
+
+def sources_run(oracle):
+    """docs/examples/sources.sp on the oracle: one PULSE, SIN, PWL and EXP
+    source and a DC current source, each into its own 1k load. Slots: PULSE
+    2-8, SIN 9-14, PWL count 15, times 18.., values 34.., EXP 50-55."""
+    from egspice_amd import spice_number as sn
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["p", 0], {0: np.nan, 1: 1.0, 2: 0.0, 3: 1.0, 4: sn("1m"), 5: sn("0.1m"), 6: sn("0.1m"),
+                             7: sn("2m"), 8: sn("5m")})
+    c.add(O.VSRC, ["s", 0], {0: np.nan, 1: 2.0, 9: 0.5, 10: 0.5, 11: sn("1k")})
+    pwl = {0: np.nan, 1: 4.0, 15: 4.0}
+    for k, (tk, vk) in enumerate([(0.0, 0.0), (sn("1m"), 1.0), (sn("3m"), 1.0), (sn("4m"), 0.0)]):
+        pwl[18 + k], pwl[34 + k] = tk, vk
+    c.add(O.VSRC, ["w", 0], pwl)
+    c.add(O.VSRC, ["e", 0], {0: np.nan, 1: 3.0, 50: 0.0, 51: 1.0, 52: sn("1m"), 53: sn("0.5m"), 54: sn("3m"),
+                             55: sn("0.5m")})
+    c.add(O.ISRC, [0, "c"], {0: sn("1m")})
+    for nd in "pswec":
+        c.add(O.RES, [nd, 0], {0: sn("1k")})
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(sn("5m"), sn("10u")))
+    return c, t, x, st
+
+
+def test_sources_golden(oracle):
+    """docs/learn/sources.md:51-58: `.tran 10u 5m` over the five waveforms
+    gives 526 points; the nodes are 5 and the V branches 4, so with time the
+    raw file holds 10 variables."""
+    import decks
+    c, t, x, st = sources_run(oracle)
+    assert st.status == 0 and st.points == decks.SOURCES_POINTS == len(t)
+    assert c.n == 1 + 5 + 4
+
+
+def test_sources_waveforms(oracle):
+    """Every node against its waveform's closed form at every accepted time
+    (models/vsource.va:217-299): PULSE and PWL are piecewise linear through
+    their corners, SIN is 0.5 + 0.5 sin(2 pi 1k t), EXP rises from 1 ms with
+    tau 0.5 ms and falls from 3 ms, and the 1 mA source holds its node at
+    1 V. Every PULSE and PWL corner is an accepted time point, and EXP's
+    delays are not (the model declares no timer for them)."""
+    from egspice_amd import spice_number as sn
+    c, t, x, _ = sources_run(oracle)
+    m = sn("1m")
+    pulse = np.interp(t, [0, m, 1.1 * m, 3.1 * m, 3.2 * m, 5 * m], [0, 0, 1, 1, 0, 0])
+    assert np.allclose(x[:, c.node("p")], pulse, rtol=0, atol=1e-12)
+    assert np.allclose(x[:, c.node("s")], 0.5 + 0.5 * np.sin(2 * np.pi * 1e3 * t), rtol=0, atol=1e-12)
+    assert np.allclose(x[:, c.node("w")], np.interp(t, [0, m, 3 * m, 4 * m, 5 * m], [0, 1, 1, 0, 0]), rtol=0,
+                       atol=1e-12)
+    e1 = np.where(t > m, 1 - np.exp(-(t - m) / (0.5 * m)), 0.0)
+    e2 = np.where(t > 3 * m, -(1 - np.exp(-(t - 3 * m) / (0.5 * m))), 0.0)
+    assert np.allclose(x[:, c.node("e")], e1 + e2, rtol=0, atol=1e-12)
+    assert np.allclose(x[:, c.node("c")], 1.0, rtol=0, atol=1e-12)
+    for corner in (m, 1.1 * m, 3 * m, 3.1 * m, 3.2 * m, 4 * m):
+        assert np.min(np.abs(t - corner)) <= 1e-15 * corner, corner
