@@ -32,7 +32,7 @@ EXPORTS = (
     "espice_lanes_newton_op", "espice_lanes_tran", "espice_lanes_tab_build", "espice_lanes_tab_refactor_solve",
     "espice_lanes_mc_draw", "espice_mlp_create", "espice_mlp_destroy", "espice_mlp_param_count",
     "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
-    "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start",
+    "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
 )
 
 
@@ -137,6 +137,7 @@ def library():
         "espice_mlp_gradients": [P, f64p, f64p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "espice_mlp_predict": [P, f64p, C.c_uint32, f64p],
         "espice_lanes_warm_start": [P, P, f64p, f64p],
+        "espice_lanes_newton_learned": [P, P, f64p, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p, f64p, f64p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -307,12 +308,24 @@ class Lanes:
     def newton(self, **kw):
         """Static Newton from the lanes' current x (no upload)."""
         o = newton_opts(**kw)
-        out = (LaneResult * self.N)()
+        res = np.zeros((self.N, 2), dtype=np.uint32)  # espice_lane_result: status, iterations
         ms = C.c_double()
-        _check(self.lib.espice_lanes_newton_op(self.h, C.byref(o), out, C.byref(ms)))
-        status = np.array([out[i].status for i in range(self.N)], dtype=np.uint32)
-        iters = np.array([out[i].iterations for i in range(self.N)], dtype=np.uint32)
-        return self.download_x(), status, iters, ms.value
+        _check(self.lib.espice_lanes_newton_op(self.h, C.byref(o), res.ctypes.data_as(C.POINTER(LaneResult)),
+                                               C.byref(ms)))
+        return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ms.value
+
+    def newton_learned(self, mlp, params, **kw):
+        """The network's start for params[N][n_in], then static Newton from
+        it, in one call. Returns (x, status, iterations, warm_ms, newton_ms)."""
+        a, ap = _f64(np.asarray(params, dtype=np.float64).reshape(self.N, -1))
+        o = newton_opts(**kw)
+        res = np.zeros((self.N, 2), dtype=np.uint32)
+        x, xp = _f64(np.zeros(self.t.n * self.N))
+        wm, nm = C.c_double(), C.c_double()
+        _check(self.lib.espice_lanes_newton_learned(self.h, mlp.h, ap, C.byref(o),
+                                                    res.ctypes.data_as(C.POINTER(LaneResult)), xp, C.byref(wm),
+                                                    C.byref(nm)))
+        return x.reshape(self.t.n, self.N), res[:, 0].copy(), res[:, 1].copy(), wm.value, nm.value
 
 
     def tran(self, opts, passes=(), record_capacity=None):

@@ -70,6 +70,23 @@ struct DevBuf {
     }
 };
 
+// Page-locked host staging, so the copies back at the end of a solve run
+// asynchronously on the lanes' stream and one synchronisation covers them.
+template <typename T>
+struct HostBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        n = count ? count : 1;
+        return hipHostMalloc(reinterpret_cast<void **>(&p), n * sizeof(T), hipHostMallocDefault);
+    }
+    ~HostBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 }  // namespace
 
 void esp::set_error(const char *name) { g_error = name; }
@@ -509,7 +526,7 @@ struct espice_lanes {
     const Template *t = nullptr;
     uint32_t N = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_w0 = nullptr, ev_w1 = nullptr;
     std::vector<double> uprm[K_NKIND];  // host, lane-major
     bool prepared = false;
     // device pattern
@@ -518,6 +535,8 @@ struct espice_lanes {
     // device lane planes
     DevBuf<double> prm[K_NKIND], lim[K_NKIND], x, x_old, dx, g, rhs;
     DevBuf<uint32_t> status, iters;
+    HostBuf<uint32_t> h_status, h_iters;
+    HostBuf<double> h_x;
     // lane solver
     DevBuf<uint8_t> st, void_col, scaled_pivot;
     DevBuf<uint32_t> cnt, pinv, lp, up, li, ui, prow, void_slots, flag, topo, stack, pstack, want, want_of, prev;
@@ -536,6 +555,8 @@ struct espice_lanes {
     {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_w0) (void)hipEventDestroy(ev_w0);
+        if (ev_w1) (void)hipEventDestroy(ev_w1);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -679,6 +700,8 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     HIP_TRY(hipStreamCreate(&L->stream));
     HIP_TRY(hipEventCreate(&L->ev0));
     HIP_TRY(hipEventCreate(&L->ev1));
+    HIP_TRY(hipEventCreate(&L->ev_w0));
+    HIP_TRY(hipEventCreate(&L->ev_w1));
     for (int k = 0; k < K_NKIND; k++) {
         L->uprm[k].resize((size_t)T.count[k] * KIND_UK[k] * N);
         for (size_t i = 0; i < T.count[k]; i++)
@@ -698,6 +721,9 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         HIP_TRY(zeroed(L->lim[k], (size_t)T.count[k] * KIND_NU[k] * N));
     }
     HIP_TRY(zeroed(L->x, n * N));
+    HIP_TRY(L->h_status.alloc(N));
+    HIP_TRY(L->h_iters.alloc(N));
+    HIP_TRY(L->h_x.alloc(n * N));
     HIP_TRY(zeroed(L->x_old, n * N));
     HIP_TRY(zeroed(L->dx, n * N));
     HIP_TRY(zeroed(L->g, (nnz + 1) * N));
@@ -754,6 +780,9 @@ espice_status espice_lanes_destroy(espice_lanes *l)
 espice_status espice_lanes_reset(espice_lanes *L)
 {
     if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    // the workgroup-per-lane kernels build every lane's solver state in LDS
+    // at launch; only the thread-per-lane solver keeps it across launches
+    if (L->use_wave) return ESPICE_OK;
     const uint32_t blocks = (L->N + 63) / 64;
     hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, L->N,
                        (uint8_t)(L->t->tridiagonal ? 1 : 0));
@@ -841,9 +870,10 @@ espice_status espice_lanes_download_x(espice_lanes *L, double *x)
     return ESPICE_OK;
 }
 
-espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *opts, espice_lane_result *out, double *kernel_ms)
+// The static Newton kernel of every lane on the lanes' stream, bracketed by
+// ev0/ev1.
+static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *opts)
 {
-    if (!L || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
@@ -862,16 +892,52 @@ espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
-    std::vector<uint32_t> st(L->N), it(L->N);
-    HIP_TRY(hipMemcpyAsync(st.data(), L->status.p, 4 * (size_t)L->N, hipMemcpyDeviceToHost, L->stream));
-    HIP_TRY(hipMemcpyAsync(it.data(), L->iters.p, 4 * (size_t)L->N, hipMemcpyDeviceToHost, L->stream));
+    return ESPICE_OK;
+}
+
+// Status, iterations and (when x is given) the solution back through the
+// pinned staging, then the one synchronisation of the solve.
+static espice_status finish_newton(espice_lanes *L, espice_lane_result *out, double *x, double *newton_ms)
+{
+    const size_t N = L->N, nx = (size_t)L->t->n * N;
+    HIP_TRY(hipMemcpyAsync(L->h_status.p, L->status.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipMemcpyAsync(L->h_iters.p, L->iters.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
+    if (x) HIP_TRY(hipMemcpyAsync(L->h_x.p, L->x.p, 8 * nx, hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
-    if (kernel_ms) {
+    if (newton_ms) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, L->ev0, L->ev1));
-        *kernel_ms = ms;
+        *newton_ms = ms;
     }
-    for (uint32_t l = 0; l < L->N; l++) out[l] = espice_lane_result{st[l], it[l]};
+    for (size_t l = 0; l < N; l++) out[l] = espice_lane_result{L->h_status.p[l], L->h_iters.p[l]};
+    if (x) std::memcpy(x, L->h_x.p, 8 * nx);
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *opts, espice_lane_result *out, double *kernel_ms)
+{
+    if (!L || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (espice_status e = launch_newton(L, opts)) return e;
+    return finish_newton(L, out, nullptr, kernel_ms);
+}
+
+espice_status espice_lanes_newton_learned(espice_lanes *L, espice_mlp *m, const double *params,
+                                          const espice_newton_opts *opts, espice_lane_result *out, double *x,
+                                          double *warm_ms, double *newton_ms)
+{
+    if (!L || !m || !params || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    hipEvent_t w0 = L->ev_w0, w1 = L->ev_w1;
+    HIP_TRY(hipEventRecord(w0, L->stream));
+    if (espice_status e = mlp_warm_start(m, params, L->N, L->x.p, L->t->n, L->N, L->stream)) return e;
+    HIP_TRY(hipEventRecord(w1, L->stream));
+    if (espice_status e = launch_newton(L, opts)) return e;
+    if (espice_status e = finish_newton(L, out, x, newton_ms)) return e;
+    if (warm_ms) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, w0, w1));
+        *warm_ms = ms;
+    }
     return ESPICE_OK;
 }
 

@@ -65,8 +65,7 @@ def learned_op(lanes, mlp, params, nominal_x=None, timing=None, **newton_kw):
     of the Newton kernels ("newton_ms") and the fallback lane counts."""
     kw = dict(newton_kw)
     kw["init_fix"] = False
-    w_ms = lanes.warm_start(mlp, params)
-    x, st, it, k_ms = lanes.newton(**kw)
+    x, st, it, w_ms, k_ms = lanes.newton_learned(mlp, params, **kw)
     ms = w_ms + k_ms
     n_ms = k_ms
     fell = {}

@@ -66,6 +66,16 @@ espice_status espice_mlp_predict(espice_mlp *m, const double *x, uint32_t count,
  * the next espice_lanes_newton_op with init_fix 0 starts there.
  * kernel_ms (optional): GPU time of the inference and the write. */
 espice_status espice_lanes_warm_start(espice_lanes *l, espice_mlp *m, const double *params, double *kernel_ms);
+/* espice_lanes_warm_start, then espice_lanes_newton_op's kernel from that
+ * start (opts->init_fix as given: 0 for a learned start), in one call with
+ * one host synchronisation: out[n_lanes] as espice_lanes_newton_op gives it
+ * and, when x is not NULL, the solution x[n * n_lanes] as
+ * espice_lanes_download_x gives it. warm_ms and newton_ms (optional): GPU
+ * time of the inference and of the Newton kernel. No counterpart in the
+ * reference, which starts its lanes cold or from the nominal point
+ * (src/analysis/dc/op.zig:588,671). */
+espice_status espice_lanes_newton_learned(espice_lanes *l, espice_mlp *m, const double *params,
+    const espice_newton_opts *opts, espice_lane_result *out, double *x, double *warm_ms, double *newton_ms);
 
 #ifdef __cplusplus
 }

@@ -177,3 +177,27 @@ def test_learned_warm_start_inverter_chain(engine):
     rows = [t.node(nm) for nm in ["in", "vdd"] + [f"n{k}" for k in range(1, 6)]]
     say(f"  max |learned - cold| on node rows: {diff[rows].max():.3e} V")
     assert np.all(diff[rows] <= bound[rows])
+
+
+@pytest.mark.gpu
+def test_newton_learned_equals_its_two_calls(engine):
+    """espice_lanes_newton_learned is espice_lanes_warm_start followed by
+    espice_lanes_newton_op with init_fix 0: the same lanes bit for bit."""
+    say("case: newton_learned vs warm_start + newton_op, 64-lane inverter chain")
+    t = chain(64)
+    rng = np.random.default_rng(5)
+    net, _ = warm.train_warm_start(t, set_vin, rng.uniform(0.0, 1.8, (512, 1)), batch=256,
+                                   schedule=((200, 1e-3),))
+    sweep = W.dc_sweep_points(64).reshape(-1, 1)
+    a = E.Lanes(t, 64)
+    set_vin(a, sweep)
+    a.prepare()
+    a.warm_start(net, sweep)
+    xa, sa, ia, _ = a.newton(init_fix=False)
+    b = E.Lanes(t, 64)
+    set_vin(b, sweep)
+    b.prepare()
+    xb, sb, ib, wms, nms = b.newton_learned(net, sweep, init_fix=False)
+    assert wms > 0 and nms > 0
+    assert np.array_equal(sa, sb) and np.array_equal(ia, ib)
+    assert np.array_equal(xa.view(np.uint64), xb.view(np.uint64))
