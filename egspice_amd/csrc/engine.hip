@@ -111,7 +111,7 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 template <bool DENSE>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
-                                                         uint32_t *status, uint32_t *iters)
+                                                         uint32_t *status, uint32_t *iters, const uint32_t *pred)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
-    const DenseLU D = dense_lu(block, lay, n);
+    const DenseLU D = dense_lu(block, lay, n, pred);
     double *prm_lds = block + lay.prm;
     for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
     for (uint32_t i = tid; i < n; i += nt) {
@@ -190,6 +190,75 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     }
 }
 
+// The template's predicted first factor: lane 0's matrix at a cold start
+// (x = 0, the limiters' seeds) factored by the serial full factor, its
+// symbolic tape written to `pred` (PredTape) for every lane's first
+// factor to try; PRED_OK stays 0 when that factor fails or voids a
+// column. One workgroup; setup work, not part of a solve.
+__global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
+                                                          const double *prm_lanes, Newton o, uint32_t *pred)
+{
+    extern __shared__ double lds[];
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t n = B.n, nnz = B.nnz;
+    uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
+    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + B.words / 2;
+    const WaveTemplate W = wave_template(B, tl);
+    WaveLane s = wave_lane(W, lay, block);
+    const DenseLU D = dense_lu(block, lay, n);
+    double *prm_lds = block + lay.prm;
+    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[i];
+    for (uint32_t i = tid; i < n; i += nt) {
+        s.x[i] = 0.0;
+        s.x_old[i] = 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) wave_fresh_solver(W, s);
+    __syncthreads();
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+    const ReduceCache rc = wave_reduce_cache(W, tid);
+    __syncthreads();
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true);
+    __syncthreads();
+    wave_reduce(W, s, rc, tid, nt);
+    __syncthreads();
+    if (tid == 0) {
+        double limit = 1e9;
+        wave_assemble(W, s, o, 0, limit);
+        bool ok = lu_factor(W.P, s.lu, s.g) == 0 && s.lu.cnt[C_NVOID] == 0;
+        for (uint32_t i = 0; i < n; i++) ok = ok && s.lu.void_col[i] == 0;
+        s.flags[2] = ok ? 1u : 0u;
+        pred[PRED_OK] = 0;
+    }
+    __syncthreads();
+    if (!s.flags[2]) return;
+    dense_build(W.P, s.lu, D);
+    const PredTape T = pred_tape(n, nnz);
+    const uint32_t nl = s.lu.cnt[C_NL], nu = s.lu.cnt[C_NU];
+    for (uint32_t i = tid; i < n; i += nt) {
+        pred[T.pinv + i] = s.lu.pinv[i];
+        pred[T.porig + i] = D.porig[i];
+        pred[T.lmask + 2 * i] = (uint32_t)D.lmask[i];
+        pred[T.lmask + 2 * i + 1] = (uint32_t)(D.lmask[i] >> 32);
+        pred[T.umask + 2 * i] = (uint32_t)D.umask[i];
+        pred[T.umask + 2 * i + 1] = (uint32_t)(D.umask[i] >> 32);
+    }
+    for (uint32_t i = tid; i <= n; i += nt) {
+        pred[T.lp + i] = s.lu.lp[i];
+        pred[T.up + i] = s.lu.up[i];
+    }
+    for (uint32_t i = tid; i < nl; i += nt) pred[T.li + i] = s.lu.li[i];
+    for (uint32_t i = tid; i < nu; i += nt) pred[T.ui + i] = s.lu.ui[i];
+    for (uint32_t i = tid; i < nnz; i += nt) pred[T.prow + i] = s.lu.prow[i];
+    for (uint32_t i = tid; i < n * n; i += nt) pred[T.amap + i] = D.amap[i];
+    if (tid == 0) {
+        pred[PRED_NL] = nl;
+        pred[PRED_NU] = nu;
+        pred[PRED_OK] = 1;
+    }
+}
+
 // One lane's transient per workgroup (wave_tran.h): the operating point at
 // t = 0, then steps until t_stop, each accepted row fed to the .meas
 // passes and, up to `to.record_cap` rows, recorded. The lane's final
@@ -198,7 +267,8 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
                                                        const double *prm_lanes, double *x_glob, uint32_t N,
                                                        Newton op_opts, Newton step_opts, TranOpts to,
                                                        const MeasPass *passes, uint32_t n_pass, TranOut *outs,
-                                                       MeasOut *meas_out, double *rec_t, double *rec_x)
+                                                       MeasOut *meas_out, double *rec_t, double *rec_x,
+                                                       const uint32_t *pred)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -209,7 +279,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
-    const DenseLU D = dense_lu(block, lay, n);
+    const DenseLU D = dense_lu(block, lay, n, pred);
     const uint32_t n_tape = W.n_tape;
     double *const a = s.cc + (nnz + 1);
     double *const ring = s.tape + n_tape;
@@ -546,6 +616,8 @@ struct espice_lanes {
     WaveHost wh;
     DevBuf<uint32_t> w_blob;
     DevBuf<double> prm_lanes;
+    DevBuf<uint32_t> pred;  // the template's predicted first factor (PredTape)
+    bool has_pred = false;
     WaveLayout lay{};
     size_t lds_bytes = 0;
     bool use_wave = false;
@@ -837,6 +909,19 @@ espice_status espice_lanes_prepare(espice_lanes *L)
     }
     if (!L->prm_host.empty())
         HIP_TRY(hipMemcpy(L->prm_lanes.p, L->prm_host.data(), L->prm_host.size() * 8, hipMemcpyHostToDevice));
+    // the symbolic analysis every lane's first factor tries (wave_lu.h
+    // dense_first_factor), from lane 0's cold start
+    L->has_pred = false;
+    if (L->use_wave && L->lay.dense != ESP_NONE && !L->t->tridiagonal) {
+        const PredTape T = pred_tape(L->t->n, L->t->nnz);
+        if (L->pred.n < T.words) HIP_TRY(L->pred.alloc(T.words));
+        const Newton o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
+        hipLaunchKernelGGL(analyze_wave_kernel, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at, L->w_blob.p,
+                           L->lay, L->prm_lanes.p, o, L->pred.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(L->stream));
+        L->has_pred = true;
+    }
     L->prepared = true;
     return ESPICE_OK;
 }
@@ -881,10 +966,12 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
     if (L->use_wave) {
         if (L->lay.dense != ESP_NONE)
             hipLaunchKernelGGL(newton_wave_kernel<true>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
+                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
+                               L->has_pred ? L->pred.p : nullptr);
         else
             hipLaunchKernelGGL(newton_wave_kernel<false>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p);
+                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
+                               nullptr);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -1022,7 +1109,8 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     step.init_fix = 0;
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     hipLaunchKernelGGL(tran_wave_kernel, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay, L->prm_lanes.p,
-                       L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p);
+                       L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
+                       L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));

@@ -33,11 +33,37 @@ struct DenseLU {
     uint64_t *umask;  // n: bit k of umask[r] when U(r,k) is structural
     uint32_t *amap;   // n*n: pattern slot of A at (pivot row r, step k), or ESP_NONE
     uint32_t *porig;  // n: original row of pivot row t
+    const uint32_t *pred;  // the template's predicted first factor (PredTape, global memory), or null
 };
 
-ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n)
+// A first factor's symbolic tape, as the template's analysis writes it to
+// global memory for every lane to try (dense_first_factor): header words
+// then the arrays at the offsets below, all for n <= 64.
+enum { PRED_OK = 0, PRED_NL = 1, PRED_NU = 2, PRED_HEAD = 8 };
+struct PredTape {
+    uint32_t pinv, lp, up, li, ui, prow, lmask, umask, amap, porig, words;
+};
+ESP_HD PredTape pred_tape(uint32_t n, uint32_t nnz)
+{
+    PredTape t;
+    t.pinv = PRED_HEAD;
+    t.lp = t.pinv + n;
+    t.up = t.lp + n + 1;
+    t.li = t.up + n + 1;
+    t.ui = t.li + n * n;
+    t.prow = t.ui + n * n;
+    t.lmask = t.prow + nnz;
+    t.umask = t.lmask + 2 * n;
+    t.amap = t.umask + 2 * n;
+    t.porig = t.amap + n * n;
+    t.words = t.porig + n;
+    return t;
+}
+
+ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n, const uint32_t *pred = nullptr)
 {
     DenseLU d;
+    d.pred = pred;
     double *f = block + L.dense;
     d.F = f;
     f += (size_t)n * n;
@@ -368,6 +394,194 @@ __device__ bool dense_probe_holds_wave(const Pattern &P, const LaneLU &s, const 
     return true;
 }
 
+// ------------------------------------------- the predicted first factor
+
+// The lane's first full factor (lane_solver.h lu_factor on a fresh solver:
+// factor_pass with keep_prev off) computed on the wavefront from the
+// template's predicted tape. factor_pass's DFS, reach and operation order
+// depend only on the pattern and on the pivot chosen at each earlier
+// step, so as long as every step's choice equals the prediction, the
+// prediction's tape is the one factor_pass would build and this replay's
+// values are its values, bit for bit (dense_refactor's argument). At each
+// step the candidates (the pivot row and the column's L rows) are judged
+// by factor_pass's rule: the largest magnitude among them (first in
+// topological order on a tie), unless the wanted row want[c] passes the
+// threshold test, plain or row-scaled (which marks the step scaled); the
+// wanted rows follow factor_pass's swaps. Returns true with the solver in
+// exactly the state lu_factor leaves; false (nothing the serial factor
+// reads changed) when the prediction is absent or any step would choose
+// otherwise, and on a tie, a zero or non-finite candidate, a void column,
+// or a fill over the cap, all of which the serial factor then decides.
+// All threads; uniform result.
+template <class A>
+__device__ bool dense_first_factor(const Pattern &P, LaneLU &s, const DenseLU &D, const A &vals)
+{
+    const uint32_t n = P.n, nnz = P.nnz, r = threadIdx.x, nt = blockDim.x;
+    const uint32_t *g = D.pred;
+    if (g == nullptr || g[PRED_OK] != 1) return false;
+    const uint32_t nl = g[PRED_NL], nu = g[PRED_NU];
+    uint64_t cap = P.cap;
+    if (P.diag_fill > 0) {
+        const double c = 3.0 * (double)P.diag_fill;
+        const uint64_t cc = (uint64_t)(c < 1e18 ? c : 1e18);
+        if (cc < cap) cap = cc;
+    }
+    if ((uint64_t)nl + nu > cap) return false;
+    const bool own = r < n;
+    const uint32_t rr = own ? r : n - 1;
+    // factor_pass's opening checks: every value finite, then the row
+    // scales, each row's largest magnitude (a bit-pattern maximum, exact
+    // for finite non-negative values) gathered column by column
+    uint64_t mb = 0;
+    for (uint32_t p = r; p < nnz; p += nt) {
+        const uint64_t b = bits(fabs_(vals[p]));
+        mb = b > mb ? b : mb;
+    }
+    if (!(from_bits(wave_bitmax(mb)) <= 1.7976931348623157e308)) return false;
+    unsigned long long *const rbits = reinterpret_cast<unsigned long long *>(&s.rscale[0]);
+    if (own) rbits[r] = 0;
+    __syncthreads();
+    if (own)
+        for (uint32_t p = P.col_ptr[r]; p < P.col_ptr[r + 1]; p++) atomicMax(&rbits[P.row_idx[p]], bits(fabs_(vals[p])));
+    __syncthreads();
+    // the predicted tape into the lane's solver and dense arrays
+    const PredTape T = pred_tape(n, nnz);
+    if (own) {
+        const double rmax = from_bits(rbits[r]);
+        s.rscale[r] = (rmax > 0 && isfinite_(rmax)) ? 1 / rmax : 1;
+        s.pinv[r] = g[T.pinv + r];
+        D.porig[r] = g[T.porig + r];
+        D.lmask[r] = (uint64_t)g[T.lmask + 2 * r + 1] << 32 | g[T.lmask + 2 * r];
+        D.umask[r] = (uint64_t)g[T.umask + 2 * r + 1] << 32 | g[T.umask + 2 * r];
+        s.void_col[r] = 0;
+        s.scaled_pivot[r] = 0;
+    }
+    for (uint32_t i = r; i <= n; i += nt) {
+        s.lp[i] = g[T.lp + i];
+        s.up[i] = g[T.up + i];
+    }
+    for (uint32_t i = r; i < nl; i += nt) s.li[i] = g[T.li + i];
+    for (uint32_t i = r; i < nu; i += nt) s.ui[i] = g[T.ui + i];
+    for (uint32_t i = r; i < nnz; i += nt) s.prow[i] = g[T.prow + i];
+    for (uint32_t i = r; i < n * n; i += nt) D.amap[i] = g[T.amap + i];
+    __syncthreads();
+    // the original row of every pivot row, and the pivot row of every
+    // original row, one per thread
+    const uint32_t orig = own ? D.porig[r] : 0, pos_of = own ? s.pinv[r] : ESP_NONE;
+    const double rs_orig = own ? s.rscale[orig] : 1.0;  // rscale of pivot row r's original row
+    uint32_t want = r, want_of = r;                      // factor_pass's want / want_of, thread i holding entry i
+    const uint64_t lm = own ? D.lmask[r] : 0, um = own ? D.umask[r] : 0;
+    const uint32_t ub = own ? s.up[r] : 0, ue = own ? s.up[r + 1] : 0;
+    bool bad = false;
+    uint64_t scaled_steps = 0;
+    const uint32_t a0 = own ? D.amap[r] : ESP_NONE;
+    uint32_t a_next = own && n > 1 ? D.amap[n + r] : ESP_NONE;
+    double wv = a0 != ESP_NONE ? vals[a0] : 0.0;
+    uint32_t u0 = lane_read_u32(ub, 0), u1 = lane_read_u32(ue, 0);
+    uint32_t uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+    for (uint32_t k = 0; k < n && !bad; k++) {
+        double w = wv;
+        const uint32_t cnt = u1 - u0, col_idx = uidx;
+        if (k + 1 < n) {
+            wv = a_next != ESP_NONE ? vals[a_next] : 0.0;
+            a_next = own && k + 2 < n ? D.amap[(k + 2) * n + r] : ESP_NONE;
+            u0 = lane_read_u32(ub, k + 1);
+            u1 = lane_read_u32(ue, k + 1);
+            uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+        }
+        for (uint32_t j0 = 0; j0 < cnt; j0 += CHUNK) {
+            uint32_t iv[CHUNK];
+            double fv[CHUNK];
+#pragma unroll
+            for (uint32_t c = 0; c < CHUNK; c++) {
+                const uint32_t j = j0 + c < cnt ? j0 + c : cnt - 1;
+                iv[c] = lane_read_u32(col_idx, j);
+                fv[c] = D.F[iv[c] * n + rr];
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < CHUNK; c++) {
+                if (j0 + c < cnt) {
+                    const double uki = lane_read(w, iv[c]);
+                    const double upd = w - fv[c] * uki;
+                    w = (lm >> iv[c] & 1) ? upd : w;
+                }
+            }
+        }
+        const bool is_u = um >> k & 1, is_l = lm >> k & 1, cand = own && (r == k || is_l);
+        if (is_u) D.F[k * n + r] = w;
+        // every reach row's value finite (factor_pass's nonfinite), then
+        // the choice among the candidates
+        const double a = fabs_(w);
+        const bool nonfinite = (is_u || cand) && !(a < inf_());
+        const uint64_t abits = cand ? bits(a) : 0;
+        const uint64_t amax_b = wave_bitmax(abits);
+        const uint64_t at_max = wave_ballot(cand && abits == amax_b);
+        const double amax = from_bits(amax_b);
+        const uint32_t c = P.q[k];
+        const uint32_t pr = lane_read_u32(want, c);            // want[c], an original row
+        const uint32_t prk = lane_read_u32(pos_of, pr);        // its predicted pivot row
+        const uint64_t cands = wave_ballot(cand);
+        uint32_t chosen = (uint32_t)__builtin_ctzll(at_max | (1ull << 63));
+        bool scaled_k = false, by_want = false;
+        if (prk >= k && prk < n) {
+            const double dmag = (cands >> prk & 1) ? fabs_(lane_read(w, prk)) : 0.0;
+            const double rs_pr = lane_read(rs_orig, prk);
+            if (dmag > 0 && dmag >= PIVOT_TOL * amax) {
+                by_want = true;
+            } else if (dmag > 0) {
+                const double smax = from_bits(wave_bitmax(cand ? bits(a * rs_orig) : 0));
+                if (dmag * rs_pr >= PIVOT_TOL * smax) {
+                    by_want = true;
+                    scaled_k = true;
+                }
+            }
+        }
+        if (by_want) chosen = prk;
+        // a tie for the largest is settled by topological order, which the
+        // dense rows do not keep: the serial factor decides it
+        if (wave_ballot(nonfinite) != 0 || !(amax > 0) || chosen != k ||
+            (!by_want && __builtin_popcountll(at_max) != 1)) {
+            bad = true;
+            break;
+        }
+        if (scaled_k) scaled_steps |= 1ull << k;
+        // factor_pass's swap of the wanted rows when the pivot is not want[c]
+        const uint32_t piv = lane_read_u32(orig, k);
+        if (piv != pr) {
+            const uint32_t c2 = lane_read_u32(want_of, piv);
+            if (r == c2) want = pr;
+            if (r == pr) want_of = c2;
+        }
+        const double d = lane_read(w, k);
+        if (r == k) D.F[k * n + k] = d;
+        if (is_l) {
+            const double l = w / d;
+            D.F[k * n + r] = l;
+            bad = bad || !(fabs_(l) < inf_());
+        }
+        bad = wave_ballot(bad) != 0;
+    }
+    if (bad) return false;
+    // the counters and flags lu_factor leaves; the factor's values live in
+    // F (the serial path's sparse lx/ux/udiag only feed dense_build, and a
+    // later full factor rewrites them before reading)
+    if (own) {
+        s.scaled_pivot[r] = (scaled_steps >> r & 1) ? 1 : 0;
+        if (P.diag_fill > 0) s.prev[r] = r;
+    }
+    for (uint32_t p = r; p < nnz; p += nt) s.vcopy[p] = vals[p];
+    if (r == 0) {
+        s.cnt[C_NL] = nl;
+        s.cnt[C_NU] = nu;
+        s.cnt[C_NVOID] = 0;
+        s.cnt[C_BASE] = nl + nu;
+        s.state[ST_LU_FACTORED] = 1;
+        s.state[ST_FACTORED] = 1;
+    }
+    __syncthreads();
+    return true;
+}
+
 // --------------------------------------------------- direct.Solver.factor
 
 // lane_solver.h direct_factor for one lane on its wavefront: 0 factored,
@@ -407,6 +621,9 @@ __device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D,
         __syncthreads();
         if (tid == 0) s.state[ST_FACTORED] = 0;
     }
+    __syncthreads();
+    if (!s.state[ST_FACTORED] && !s.state[ST_LU_FACTORED] && dense_first_factor(P, s, D, vals)) return 0;
+    __syncthreads();
     if (tid == 0) {
         uint32_t r = (uint32_t)lu_factor(P, s, vals);
         if (r == 0) s.state[ST_FACTORED] = 1;
