@@ -70,8 +70,9 @@ struct DevBuf {
     }
 };
 
-// Page-locked host staging, so the copies back at the end of a solve run
-// asynchronously on the lanes' stream and one synchronisation covers them.
+// Page-locked host memory mapped for the GPU: the workgroup-per-lane
+// kernels write their per-lane results straight into it, so a solve ends
+// with one synchronisation and no copies.
 template <typename T>
 struct HostBuf {
     T *p = nullptr;
@@ -79,7 +80,7 @@ struct HostBuf {
     hipError_t alloc(size_t count)
     {
         n = count ? count : 1;
-        return hipHostMalloc(reinterpret_cast<void **>(&p), n * sizeof(T), hipHostMallocDefault);
+        return hipHostMalloc(reinterpret_cast<void **>(&p), n * sizeof(T), hipHostMallocMapped);
     }
     ~HostBuf()
     {
@@ -111,7 +112,8 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 template <bool DENSE>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
-                                                         uint32_t *status, uint32_t *iters, const uint32_t *pred)
+                                                         uint32_t *status, uint32_t *iters, const uint32_t *pred,
+                                                         uint32_t *h_status, uint32_t *h_iters, double *h_x)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -184,9 +186,13 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     }
     __syncthreads();
     for (uint32_t i = tid; i < n; i += nt) x_glob[(size_t)i * N + l] = s.x[i];
+    if (h_x)
+        for (uint32_t i = tid; i < n; i += nt) h_x[(size_t)i * N + l] = s.x[i];
     if (tid == 0) {
         status[l] = st;
         iters[l] = it;
+        h_status[l] = st;
+        h_iters[l] = it;
     }
 }
 
@@ -607,6 +613,8 @@ struct espice_lanes {
     DevBuf<uint32_t> status, iters;
     HostBuf<uint32_t> h_status, h_iters;
     HostBuf<double> h_x;
+    uint32_t *d_status = nullptr, *d_iters = nullptr;  // their device addresses
+    double *d_x = nullptr;
     // lane solver
     DevBuf<uint8_t> st, void_col, scaled_pivot;
     DevBuf<uint32_t> cnt, pinv, lp, up, li, ui, prow, void_slots, flag, topo, stack, pstack, want, want_of, prev;
@@ -796,6 +804,9 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     HIP_TRY(L->h_status.alloc(N));
     HIP_TRY(L->h_iters.alloc(N));
     HIP_TRY(L->h_x.alloc(n * N));
+    HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&L->d_status), L->h_status.p, 0));
+    HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&L->d_iters), L->h_iters.p, 0));
+    HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&L->d_x), L->h_x.p, 0));
     HIP_TRY(zeroed(L->x_old, n * N));
     HIP_TRY(zeroed(L->dx, n * N));
     HIP_TRY(zeroed(L->g, (nnz + 1) * N));
@@ -956,8 +967,9 @@ espice_status espice_lanes_download_x(espice_lanes *L, double *x)
 }
 
 // The static Newton kernel of every lane on the lanes' stream, bracketed by
-// ev0/ev1.
-static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *opts)
+// ev0/ev1; with want_x the workgroup-per-lane kernel also writes x to the
+// mapped host copy.
+static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *opts, bool want_x)
 {
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
@@ -967,11 +979,11 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
         if (L->lay.dense != ESP_NONE)
             hipLaunchKernelGGL(newton_wave_kernel<true>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               L->has_pred ? L->pred.p : nullptr);
+                               L->has_pred ? L->pred.p : nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
         else
             hipLaunchKernelGGL(newton_wave_kernel<false>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               nullptr);
+                               nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -987,9 +999,11 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
 static espice_status finish_newton(espice_lanes *L, espice_lane_result *out, double *x, double *newton_ms)
 {
     const size_t N = L->N, nx = (size_t)L->t->n * N;
-    HIP_TRY(hipMemcpyAsync(L->h_status.p, L->status.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
-    HIP_TRY(hipMemcpyAsync(L->h_iters.p, L->iters.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
-    if (x) HIP_TRY(hipMemcpyAsync(L->h_x.p, L->x.p, 8 * nx, hipMemcpyDeviceToHost, L->stream));
+    if (!L->use_wave) {
+        HIP_TRY(hipMemcpyAsync(L->h_status.p, L->status.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
+        HIP_TRY(hipMemcpyAsync(L->h_iters.p, L->iters.p, 4 * N, hipMemcpyDeviceToHost, L->stream));
+        if (x) HIP_TRY(hipMemcpyAsync(L->h_x.p, L->x.p, 8 * nx, hipMemcpyDeviceToHost, L->stream));
+    }
     HIP_TRY(hipStreamSynchronize(L->stream));
     if (newton_ms) {
         float ms = 0;
@@ -1004,7 +1018,7 @@ static espice_status finish_newton(espice_lanes *L, espice_lane_result *out, dou
 espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *opts, espice_lane_result *out, double *kernel_ms)
 {
     if (!L || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
-    if (espice_status e = launch_newton(L, opts)) return e;
+    if (espice_status e = launch_newton(L, opts, false)) return e;
     return finish_newton(L, out, nullptr, kernel_ms);
 }
 
@@ -1018,7 +1032,7 @@ espice_status espice_lanes_newton_learned(espice_lanes *L, espice_mlp *m, const 
     HIP_TRY(hipEventRecord(w0, L->stream));
     if (espice_status e = mlp_warm_start(m, params, L->N, L->x.p, L->t->n, L->N, L->stream)) return e;
     HIP_TRY(hipEventRecord(w1, L->stream));
-    if (espice_status e = launch_newton(L, opts)) return e;
+    if (espice_status e = launch_newton(L, opts, x != nullptr)) return e;
     if (espice_status e = finish_newton(L, out, x, newton_ms)) return e;
     if (warm_ms) {
         float ms = 0;

@@ -26,6 +26,17 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr uint32_t TILE = 32;
+
+// The hidden layers' activation everywhere the network evaluates it
+// (forward, its derivative, the warm start): tanh z = 1 - 2 / (e^{2z} + 1)
+// from the hardware exponential and reciprocal. Its error (a few FP32
+// ulps, at most ~1e-7 absolute) is far below the BF16 rounding every
+// activation then takes; saturates to +-1 for large |z|.
+__device__ __forceinline__ float act_tanh(float z)
+{
+    const float e = __expf(2.0f * z);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
 constexpr uint32_t MAX_GRAD_BATCH = 4096;
 
 uint32_t pad32(uint32_t v) { return (v + TILE - 1) / TILE * TILE; }
@@ -118,7 +129,7 @@ __global__ __launch_bounds__(64) void mlp_gemm_kernel(const bf16 *__restrict__ A
         float z = acc[v];
         if constexpr (EPI != EPI_STORE) z += bv;
         C[(size_t)row * ldc + col] = z;
-        if constexpr (EPI == EPI_BIAS_TANH) H[(size_t)row * ldc + col] = (bf16)tanhf(z);
+        if constexpr (EPI == EPI_BIAS_TANH) H[(size_t)row * ldc + col] = (bf16)act_tanh(z);
     }
 }
 
@@ -201,7 +212,7 @@ __global__ void mlp_tanh_back_kernel(const float *__restrict__ dH, const float *
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= count) return;
-    const float t = tanhf(Z[e]);
+    const float t = act_tanh(Z[e]);
     const float g = dH[e] * (1.0f - t * t);
     dZ[e] = g;
     dZb[e] = (bf16)g;
@@ -269,6 +280,139 @@ __global__ void mlp_write_x_kernel(const float *__restrict__ Y, uint32_t ld, uin
     x[(size_t)i * N + l] = s != 0.0 ? mean[i] + s * (double)Y[(size_t)l * ld + i] : mean[i];
 }
 
+// The whole network for the warm start in one launch: workgroup b owns rows
+// 32b..32b+31, standardizes them into LDS, runs every layer with its
+// activations in LDS (two buffers of 32 rows, ld bf16 apart) and each of
+// its four wavefronts taking every fourth 32-column tile, and writes the
+// de-standardized outputs straight into the lanes' x. Each tile is the
+// same MFMA sequence as mlp_gemm_kernel's (k ascending) with the same
+// bias, tanh and BF16 rounding, so the result is bit for bit the
+// layer-by-layer path's.
+constexpr uint32_t INFER_MAX_LAYERS = 17;
+struct MlpNet {
+    const bf16 *w[INFER_MAX_LAYERS];
+    const float *b[INFER_MAX_LAYERS];
+    uint32_t pin[INFER_MAX_LAYERS], pout[INFER_MAX_LAYERS];
+    uint32_t layers, n_in, n_out;
+    const double *in_mean, *in_sd, *out_mean, *out_sd;
+};
+
+constexpr uint32_t INFER_THREADS = 512;
+// inputs of up to this many values travel in the kernel's arguments
+constexpr uint32_t INFER_INLINE = 128;
+struct InlineInputs {
+    double v[INFER_INLINE];
+};
+
+__global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(MlpNet net, const double *__restrict__ xin,
+                                                                  InlineInputs inl, uint32_t count,
+                                                                  double *__restrict__ x, uint32_t N, uint32_t ld)
+{
+    extern __shared__ bf16 act[];
+    const uint32_t t = threadIdx.x, wave = t >> 6, l = t & 63, i = l & 31, kb = (l >> 5) * 8;
+    const uint32_t m0 = blockIdx.x * TILE;
+    const uint32_t p0 = net.pin[0];
+    const bool inline_in = xin == nullptr;
+    // the output scaling of this lane's column of the last layer's first
+    // tile, requested before anything waits on memory
+    const uint32_t Lz = net.layers - 1, zcol = wave * TILE + i;
+    const bool zlive = zcol < net.n_out;
+    const double z_sd = zlive ? net.out_sd[zcol] : 0.0, z_mean = zlive ? net.out_mean[zcol] : 0.0;
+    for (uint32_t e = t; e < TILE * p0; e += blockDim.x) {
+        const uint32_t r = e / p0, k = e % p0, m = m0 + r;
+        float v = 0.0f;
+        if (m < count && k < net.n_in) {
+            const size_t at = (size_t)m * net.n_in + k;
+            const double xv = inline_in ? inl.v[at] : xin[at];
+            const double mean = net.in_mean[k], sd = net.in_sd[k];
+            if (sd != 0.0) v = (float)((xv - mean) / sd);
+        }
+        act[r * ld + k] = (bf16)v;
+    }
+    __syncthreads();
+    // each wave's first tile of the next layer: its weight fragments are
+    // requested while the current layer runs (weights do not depend on the
+    // activations), for layers of K <= 256
+    constexpr uint32_t PF = 16;
+    const uint32_t nw = blockDim.x / 64;
+    auto fetch = [&](uint32_t L, bf16x8 *b, float &bias) {
+        const uint32_t K = net.pin[L];
+        const bool live = wave * TILE < net.pout[L] && K <= PF * 16;
+        const bf16 *pb = net.w[L] + (size_t)(wave * TILE + i) * K + kb;
+        if (live) bias = net.b[L][wave * TILE + i];
+#pragma unroll
+        for (uint32_t j = 0; j < PF; j++)
+            if (live && 16 * j < K) b[j] = *reinterpret_cast<const bf16x8 *>(pb + 16 * j);
+    };
+    bf16x8 bn[PF];
+    float bias_n = 0.0f;
+    fetch(0, bn, bias_n);
+    for (uint32_t L = 0; L < net.layers; L++) {
+        const bf16 *in = act + (L & 1) * TILE * ld;
+        bf16 *out = act + ((L + 1) & 1) * TILE * ld;
+        const uint32_t K = net.pin[L], tiles = net.pout[L] / TILE;
+        const bool last = L + 1 == net.layers;
+        bf16x8 bc[PF];
+#pragma unroll
+        for (uint32_t j = 0; j < PF; j++) bc[j] = bn[j];
+        const float bias_c = bias_n;
+        if (!last) fetch(L + 1, bn, bias_n);
+        for (uint32_t tile = wave; tile < tiles; tile += nw) {
+            const uint32_t n0 = tile * TILE;
+            const bf16 *pa = in + i * ld + kb;
+            const bf16 *pb = net.w[L] + (size_t)(n0 + i) * K + kb;
+            floatx16 acc = {};
+            if (tile == wave && K <= PF * 16) {
+                // activations from LDS four k-steps at a time
+#pragma unroll
+                for (uint32_t j0 = 0; j0 < PF; j0 += 4) {
+                    bf16x8 a[4];
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (16 * (j0 + j) < K) a[j] = *reinterpret_cast<const bf16x8 *>(pa + 16 * (j0 + j));
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (16 * (j0 + j) < K)
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], bc[j0 + j], acc, 0, 0, 0);
+                }
+            } else {
+                // four k-steps' fragments in flight before their MFMAs,
+                // which then run in k order
+                for (uint32_t k0 = 0; k0 < K; k0 += 4 * 16) {
+                    bf16x8 a[4], b[4];
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (k0 + 16 * j < K) {
+                            a[j] = *reinterpret_cast<const bf16x8 *>(pa + k0 + 16 * j);
+                            b[j] = *reinterpret_cast<const bf16x8 *>(pb + k0 + 16 * j);
+                        }
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (k0 + 16 * j < K) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[j], acc, 0, 0, 0);
+                }
+            }
+            const uint32_t col = n0 + i;
+            const bool first = tile == wave && K <= PF * 16;
+            const float bv = first ? bias_c : net.b[L][col];
+            const bool zc = first && L == Lz;
+            const double sd = zc ? z_sd : (last && col < net.n_out ? net.out_sd[col] : 0.0);
+            const double mean = zc ? z_mean : (last && col < net.n_out ? net.out_mean[col] : 0.0);
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint32_t row = 8 * (v / 4) + 4 * (l >> 5) + (v & 3);
+                const float z = acc[v] + bv;
+                if (!last) {
+                    out[row * ld + col] = (bf16)act_tanh(z);
+                } else {
+                    const uint32_t m = m0 + row;
+                    if (m < count && col < net.n_out) x[(size_t)col * N + m] = sd != 0.0 ? mean + sd * (double)z : mean;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Y rows (de-standardized) to a row-major host-bound buffer.
 __global__ void mlp_unscale_kernel(const float *__restrict__ Y, uint32_t ld, uint32_t count, uint32_t n,
                                    const double *__restrict__ mean, const double *__restrict__ sd, double *__restrict__ y)
@@ -285,6 +429,26 @@ uint32_t blocks(size_t count, uint32_t per) { return (uint32_t)((count + per - 1
 }  // namespace
 
 // ----------------------------------------------------------------- host
+
+// Mapped, page-locked host memory that kernels read in place.
+template <typename T>
+struct Pinned {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        const hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocMapped);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    ~Pinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
 
 struct espice_mlp {
     espice_mlp_shape shape{};
@@ -307,6 +471,7 @@ struct espice_mlp {
     Dev<bf16> dzb;
     Dev<float> loss;
     Dev<double> xin;               // uploaded inputs (rows x n_in)
+    Pinned<double> h_in;           // the warm start's inputs, mapped for the GPU to read
     Dev<double> yout;
     uint64_t steps = 0;
 };
@@ -685,6 +850,43 @@ espice_status esp::mlp_warm_start(espice_mlp *m, const double *params, uint32_t 
     // the network's own stream may still be training or predicting
     HIP_TRY(hipStreamSynchronize(m->stream));
     const uint32_t rp = pad32(count);
+    uint32_t widest_in = 0;
+    for (uint32_t l = 0; l < m->layers; l++) widest_in = std::max(widest_in, std::max(m->pin[l], m->pout[l]));
+    const uint32_t ld = widest_in + 8;  // 16 bytes of padding between rows spreads them over the LDS banks
+    const size_t lds = 2 * (size_t)TILE * ld * sizeof(bf16);
+    if (m->layers <= INFER_MAX_LAYERS && lds <= 64 * 1024) {
+        MlpNet net{};
+        for (uint32_t l = 0; l < m->layers; l++) {
+            net.w[l] = m->wb[l].p;
+            net.b[l] = m->w.p + m->boff[l];
+            net.pin[l] = m->pin[l];
+            net.pout[l] = m->pout[l];
+        }
+        net.layers = m->layers;
+        net.n_in = m->shape.n_in;
+        net.n_out = m->shape.n_out;
+        net.in_mean = m->in_mean.p;
+        net.in_sd = m->in_sd.p;
+        net.out_mean = m->out_mean.p;
+        net.out_sd = m->out_sd.p;
+        // the inputs in the kernel's arguments when they are few, else
+        // staged in mapped pinned memory, which the kernel reads in place
+        // (no copy on the stream); every caller synchronizes before
+        // returning, so the next call may reuse the staging
+        const size_t in_n = (size_t)count * m->shape.n_in;
+        InlineInputs inl{};
+        double *d_in = nullptr;
+        if (in_n <= INFER_INLINE) {
+            std::memcpy(inl.v, params, sizeof(double) * in_n);
+        } else {
+            if (m->h_in.n < in_n) HIP_TRY(m->h_in.alloc(in_n));
+            std::memcpy(m->h_in.p, params, sizeof(double) * in_n);
+            HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&d_in), m->h_in.p, 0));
+        }
+        mlp_infer_kernel<<<rp / TILE, INFER_THREADS, lds, s>>>(net, d_in, inl, count, x, N, ld);
+        HIP_TRY(hipGetLastError());
+        return ESPICE_OK;
+    }
     if (espice_status e = upload_inputs(*m, params, count, s)) return e;
     if (espice_status e = forward(*m, rp, s)) return e;
     mlp_write_x_kernel<<<blocks((size_t)count * n, 256), 256, 0, s>>>(m->z[m->layers - 1].p, m->pout[m->layers - 1],

@@ -201,3 +201,25 @@ def test_newton_learned_equals_its_two_calls(engine):
     assert wms > 0 and nms > 0
     assert np.array_equal(sa, sb) and np.array_equal(ia, ib)
     assert np.array_equal(xa.view(np.uint64), xb.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_fused_warm_start_equals_layer_by_layer(engine):
+    """The warm start's one-launch network (mlp_infer_kernel) writes into
+    the lanes exactly what espice_mlp_predict's layer-by-layer kernels
+    return: the same MFMA sequences, bias, tanh and roundings, bit for bit,
+    over 100 rows (a partial last tile of 32)."""
+    say("case: fused warm-start inference vs layer-by-layer predict, 100 rows, 1->3x256->10")
+    t = chain(64)
+    rng = np.random.default_rng(8)
+    net = E.Mlp(1, t.n, hidden=256, depth=3, seed=9)
+    xs = rng.uniform(0.0, 1.8, (300, 1))
+    net.fit_scaling(xs, rng.normal(size=(300, t.n)))
+    params = rng.uniform(0.0, 1.8, (100, 1))
+    lanes = E.Lanes(t, 100)
+    set_vin(lanes, params)
+    lanes.prepare()
+    lanes.warm_start(net, params)
+    got = lanes.download_x()
+    want = net.predict(params).T
+    assert np.array_equal(got.view(np.uint64), np.ascontiguousarray(want).view(np.uint64))
