@@ -33,6 +33,7 @@ EXPORTS = (
     "espice_lanes_mc_draw", "espice_mlp_create", "espice_mlp_destroy", "espice_mlp_param_count",
     "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
     "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
+    "espice_lanes_first_factors",
 )
 
 
@@ -138,6 +139,7 @@ def library():
         "espice_mlp_predict": [P, f64p, C.c_uint32, f64p],
         "espice_lanes_warm_start": [P, P, f64p, f64p],
         "espice_lanes_newton_learned": [P, P, f64p, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p, f64p, f64p],
+        "espice_lanes_first_factors": [P, u32p, u32p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -313,6 +315,13 @@ class Lanes:
         _check(self.lib.espice_lanes_newton_op(self.h, C.byref(o), res.ctypes.data_as(C.POINTER(LaneResult)),
                                                C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ms.value
+
+    def first_factors(self):
+        """(predicted, serial): how the last Newton launch factored each
+        lane's first matrix."""
+        p, q = C.c_uint32(), C.c_uint32()
+        _check(self.lib.espice_lanes_first_factors(self.h, C.byref(p), C.byref(q)))
+        return p.value, q.value
 
     def newton_learned(self, mlp, params, **kw):
         """The network's start for params[N][n_in], then static Newton from

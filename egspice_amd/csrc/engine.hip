@@ -113,7 +113,8 @@ template <bool DENSE>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
-                                                         uint32_t *h_status, uint32_t *h_iters, double *h_x)
+                                                         uint32_t *pred_hit, uint32_t *h_status, uint32_t *h_iters,
+                                                         double *h_x)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
-    const DenseLU D = dense_lu(block, lay, n, pred);
+    const DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
     double *prm_lds = block + lay.prm;
     for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
     for (uint32_t i = tid; i < n; i += nt) {
@@ -194,6 +195,12 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
         h_status[l] = st;
         h_iters[l] = it;
     }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // The template's predicted first factor: lane 0's matrix at a cold start
@@ -624,7 +631,8 @@ struct espice_lanes {
     WaveHost wh;
     DevBuf<uint32_t> w_blob;
     DevBuf<double> prm_lanes;
-    DevBuf<uint32_t> pred;  // the template's predicted first factor (PredTape)
+    DevBuf<uint32_t> pred;      // the template's predicted first factor (PredTape)
+    DevBuf<uint32_t> pred_hit;  // per lane, the last Newton launch's first factor: 0 none, 1 predicted, 2 serial
     bool has_pred = false;
     WaveLayout lay{};
     size_t lds_bytes = 0;
@@ -801,6 +809,7 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         HIP_TRY(zeroed(L->lim[k], (size_t)T.count[k] * KIND_NU[k] * N));
     }
     HIP_TRY(zeroed(L->x, n * N));
+    HIP_TRY(zeroed(L->pred_hit, N));
     HIP_TRY(L->h_status.alloc(N));
     HIP_TRY(L->h_iters.alloc(N));
     HIP_TRY(L->h_x.alloc(n * N));
@@ -974,16 +983,21 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
+    if (L->use_wave) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((L->N + 255) / 256), dim3(256), 0, L->stream, L->pred_hit.p, L->N, 0u);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
         if (L->lay.dense != ESP_NONE)
             hipLaunchKernelGGL(newton_wave_kernel<true>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               L->has_pred ? L->pred.p : nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
+                               L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
+                               want_x ? L->d_x : nullptr);
         else
             hipLaunchKernelGGL(newton_wave_kernel<false>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
+                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -1038,6 +1052,20 @@ espice_status espice_lanes_newton_learned(espice_lanes *L, espice_mlp *m, const 
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, w0, w1));
         *warm_ms = ms;
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_first_factors(espice_lanes *L, uint32_t *predicted, uint32_t *serial)
+{
+    if (!L || !predicted || !serial) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    std::vector<uint32_t> h(L->N);
+    HIP_TRY(hipMemcpy(h.data(), L->pred_hit.p, 4 * (size_t)L->N, hipMemcpyDeviceToHost));
+    *predicted = 0;
+    *serial = 0;
+    for (uint32_t v : h) {
+        *predicted += v == 1 ? 1 : 0;
+        *serial += v == 2 ? 1 : 0;
     }
     return ESPICE_OK;
 }

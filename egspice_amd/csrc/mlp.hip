@@ -318,21 +318,10 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(MlpNet net, co
     const uint32_t Lz = net.layers - 1, zcol = wave * TILE + i;
     const bool zlive = zcol < net.n_out;
     const double z_sd = zlive ? net.out_sd[zcol] : 0.0, z_mean = zlive ? net.out_mean[zcol] : 0.0;
-    for (uint32_t e = t; e < TILE * p0; e += blockDim.x) {
-        const uint32_t r = e / p0, k = e % p0, m = m0 + r;
-        float v = 0.0f;
-        if (m < count && k < net.n_in) {
-            const size_t at = (size_t)m * net.n_in + k;
-            const double xv = inline_in ? inl.v[at] : xin[at];
-            const double mean = net.in_mean[k], sd = net.in_sd[k];
-            if (sd != 0.0) v = (float)((xv - mean) / sd);
-        }
-        act[r * ld + k] = (bf16)v;
-    }
-    __syncthreads();
-    // each wave's first tile of the next layer: its weight fragments are
-    // requested while the current layer runs (weights do not depend on the
-    // activations), for layers of K <= 256
+    // each wave's first tile of the next layer: its weight fragments and
+    // bias are requested while the current layer runs (weights do not
+    // depend on the activations; the first layer's while the inputs are
+    // standardized), for layers of K <= 256
     constexpr uint32_t PF = 16;
     const uint32_t nw = blockDim.x / 64;
     auto fetch = [&](uint32_t L, bf16x8 *b, float &bias) {
@@ -347,6 +336,18 @@ __global__ __launch_bounds__(INFER_THREADS) void mlp_infer_kernel(MlpNet net, co
     bf16x8 bn[PF];
     float bias_n = 0.0f;
     fetch(0, bn, bias_n);
+    for (uint32_t e = t; e < TILE * p0; e += blockDim.x) {
+        const uint32_t r = e / p0, k = e % p0, m = m0 + r;
+        float v = 0.0f;
+        if (m < count && k < net.n_in) {
+            const size_t at = (size_t)m * net.n_in + k;
+            const double xv = inline_in ? inl.v[at] : xin[at];
+            const double mean = net.in_mean[k], sd = net.in_sd[k];
+            if (sd != 0.0) v = (float)((xv - mean) / sd);
+        }
+        act[r * ld + k] = (bf16)v;
+    }
+    __syncthreads();
     for (uint32_t L = 0; L < net.layers; L++) {
         const bf16 *in = act + (L & 1) * TILE * ld;
         bf16 *out = act + ((L + 1) & 1) * TILE * ld;

@@ -34,6 +34,7 @@ struct DenseLU {
     uint32_t *amap;   // n*n: pattern slot of A at (pivot row r, step k), or ESP_NONE
     uint32_t *porig;  // n: original row of pivot row t
     const uint32_t *pred;  // the template's predicted first factor (PredTape, global memory), or null
+    uint32_t *pred_hit;    // the lane's record of its first factor: 1 predicted, 2 serial (global), or null
 };
 
 // A first factor's symbolic tape, as the template's analysis writes it to
@@ -60,10 +61,12 @@ ESP_HD PredTape pred_tape(uint32_t n, uint32_t nnz)
     return t;
 }
 
-ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n, const uint32_t *pred = nullptr)
+ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n, const uint32_t *pred = nullptr,
+                        uint32_t *pred_hit = nullptr)
 {
     DenseLU d;
     d.pred = pred;
+    d.pred_hit = pred_hit;
     double *f = block + L.dense;
     d.F = f;
     f += (size_t)n * n;
@@ -622,7 +625,11 @@ __device__ int wave_direct_factor(const Pattern &P, LaneLU &s, const DenseLU &D,
         if (tid == 0) s.state[ST_FACTORED] = 0;
     }
     __syncthreads();
-    if (!s.state[ST_FACTORED] && !s.state[ST_LU_FACTORED] && dense_first_factor(P, s, D, vals)) return 0;
+    if (!s.state[ST_FACTORED] && !s.state[ST_LU_FACTORED] && D.pred != nullptr) {
+        const bool hit = dense_first_factor(P, s, D, vals);
+        if (tid == 0 && D.pred_hit != nullptr && *D.pred_hit == 0) *D.pred_hit = hit ? 1u : 2u;
+        if (hit) return 0;
+    }
     __syncthreads();
     if (tid == 0) {
         uint32_t r = (uint32_t)lu_factor(P, s, vals);

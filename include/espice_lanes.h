@@ -127,6 +127,14 @@ espice_status espice_lanes_download_x(espice_lanes *l, double *x);
  * out[n_lanes]; kernel_ms (optional) receives the kernel's GPU time. */
 espice_status espice_lanes_newton_op(espice_lanes *l, const espice_newton_opts *opts, espice_lane_result *out,
     double *kernel_ms);
+/* How the last espice_lanes_newton_op (or espice_lanes_newton_learned)
+ * launch factored each lane's first matrix on the workgroup-per-lane path:
+ * *predicted lanes took the template's predicted pivot sequence (checked
+ * against the full factor's pivot rule at every step, so bitwise the same
+ * factor), *serial lanes ran the serial full factor. No counterpart in the
+ * reference, whose lanes always run the full factor first
+ * (src/solver/sparse_lu.zig:274-311); diagnostics only. */
+espice_status espice_lanes_first_factors(espice_lanes *l, uint32_t *predicted, uint32_t *serial);
 
 /* ---- the lead-tape lane LU (lane_device.zig refactor :101, solveNeg :170)
  * Builds a LaneTab from one factor of (col_ptr, row_idx, vals) on the host

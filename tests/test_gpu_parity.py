@@ -255,3 +255,58 @@ def test_sampler_drawn_lanes_bitwise(oracle, engine):
                                                          nom_kp, 0.02, 1)))
     so = run_pair(oracle, W.inverter_chain, N, params)
     assert (so & 1).mean() > 0.95
+
+
+def divider3(add):
+    """V1 a 0; R1 a b; R2 b 0; R3 a 0: one node column holds the source's
+    branch entry, so its pivot depends on the conductances' scale."""
+    add(E.VSOURCE, ["a", "0"], {0: 1.0})
+    add(E.RESISTOR, ["a", "b"], {0: 1.0})
+    add(E.RESISTOR, ["b", "0"], {0: 1.0})
+    add(E.RESISTOR, ["a", "0"], {0: 1.0})
+
+
+def first_factor_pair(oracle, builder, N, params, args=()):
+    t, c = build_both(oracle, builder, *args)
+    gl, ol = E.Lanes(t, N), O.Lanes(c, N)
+    for L in (gl, ol):
+        for kind, inst, k, vals in params:
+            L.set_param(kind, inst, k, vals)
+        L.prepare()
+    xg, sg, ig, _ = gl.newton_op()
+    xo, so, io = ol.newton_op(mode=1)
+    assert (sg == so).all() and (ig == io).all() and bits_equal(xg, xo)
+    return gl.first_factors(), so
+
+
+def test_predicted_first_factor_hits_on_the_sweep(oracle, engine):
+    """Every lane of the 64-point inverter-chain sweep takes the template's
+    predicted first factor, and the lanes stay bitwise the oracle's (whose
+    full factor is the serial one)."""
+    say("case: predicted first factor, inverter chain, 64 lanes")
+    (pred, serial), so = first_factor_pair(oracle, W.inverter_chain, 64,
+                                           [(E.VSOURCE, 1, E.VSOURCE_DC, W.dc_sweep_points(64))])
+    say(f"  predicted {pred}, serial {serial}")
+    assert (pred, serial) == (64, 0)
+    assert (so & 1).all()
+
+
+def test_predicted_first_factor_falls_back(oracle, engine):
+    """Conductances over nine decades per lane, and on every fourth lane R1
+    and R2 open (infinite), which leaves node b's column without a nonzero
+    candidate: those lanes' first factors cannot follow the prediction
+    (taken from lane 0, all finite) and run the serial factor, which decides
+    the void column; the others take the prediction. Every lane stays
+    bitwise the oracle's."""
+    N = 64
+    rng = np.random.default_rng(31)
+    vals = [np.concatenate([[1.0], 10 ** rng.uniform(0, 9, N - 1)]) for _ in range(3)]
+    open_b = np.arange(N) % 4 == 1
+    vals[0][open_b] = np.inf
+    vals[1][open_b] = np.inf
+    say(f"case: predicted first factor with fallbacks, 3-resistor divider, {N} lanes, {int(open_b.sum())} with b open")
+    (pred, serial), so = first_factor_pair(oracle, divider3, N,
+                                           [(E.RESISTOR, k, 0, vals[k]) for k in range(3)])
+    say(f"  predicted {pred}, serial {serial}, converged {int((so & 1).sum())}")
+    assert serial == int(open_b.sum()) and pred == N - serial
+    assert (so[~open_b] & 1).all()
