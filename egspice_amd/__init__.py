@@ -325,16 +325,30 @@ class Lanes:
 
     def newton_learned(self, mlp, params, **kw):
         """The network's start for params[N][n_in], then static Newton from
-        it, in one call. Returns (x, status, iterations, warm_ms, newton_ms)."""
-        a, ap = _f64(np.asarray(params, dtype=np.float64).reshape(self.N, -1))
-        o = newton_opts(**kw)
-        res = np.zeros((self.N, 2), dtype=np.uint32)
-        x, xp = _f64(np.zeros(self.t.n * self.N))
-        wm, nm = C.c_double(), C.c_double()
-        _check(self.lib.espice_lanes_newton_learned(self.h, mlp.h, ap, C.byref(o),
-                                                    res.ctypes.data_as(C.POINTER(LaneResult)), xp, C.byref(wm),
-                                                    C.byref(nm)))
-        return x.reshape(self.t.n, self.N), res[:, 0].copy(), res[:, 1].copy(), wm.value, nm.value
+        it, in one call. Returns (x, status, iterations, warm_ms, newton_ms).
+        The argument buffers and their ctypes pointers are kept per lanes
+        object, so a repeated solve marshals nothing new."""
+        p = np.asarray(params, dtype=np.float64).reshape(self.N, -1)
+        key = (p.shape, tuple(sorted(kw.items())))
+        buf = getattr(self, "_learned", None)
+        if buf is None or buf["key"] != key:
+            o = newton_opts(**kw)
+            pa = np.zeros(p.shape)
+            res = np.zeros((self.N, 2), dtype=np.uint32)
+            x = np.zeros(self.t.n * self.N)
+            wm, nm = C.c_double(), C.c_double()
+            buf = self._learned = {
+                "key": key, "p": pa, "res": res, "x": x, "wm": wm, "nm": nm,
+                "args": (pa.ctypes.data_as(C.POINTER(C.c_double)), C.byref(o),
+                         res.ctypes.data_as(C.POINTER(LaneResult)), x.ctypes.data_as(C.POINTER(C.c_double)),
+                         C.byref(wm), C.byref(nm)),
+                "opts": o,
+            }
+        np.copyto(buf["p"], p)
+        _check(self.lib.espice_lanes_newton_learned(self.h, mlp.h, *buf["args"]))
+        res = buf["res"]
+        return (buf["x"].reshape(self.t.n, self.N).copy(), res[:, 0].copy(), res[:, 1].copy(), buf["wm"].value,
+                buf["nm"].value)
 
 
     def tran(self, opts, passes=(), record_capacity=None):

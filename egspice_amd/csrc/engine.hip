@@ -126,7 +126,14 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
-    const DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
+    DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
+    if (tid == 0 && pred_hit) pred_hit[l] = 0;  // this launch's record starts empty
+    // the lane's first factor will try the prediction: its tape's loads go
+    // out with the template's
+    if (DENSE && pred != nullptr) {
+        pred_stage(W.P, s.lu, D);
+        D.staged = true;
+    }
     double *prm_lds = block + lay.prm;
     for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
     for (uint32_t i = tid; i < n; i += nt) {
@@ -195,12 +202,6 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
         h_status[l] = st;
         h_iters[l] = it;
     }
-}
-
-__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 // The template's predicted first factor: lane 0's matrix at a cold start
@@ -983,10 +984,6 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
-    if (L->use_wave) {
-        hipLaunchKernelGGL(fill_u32_kernel, dim3((L->N + 255) / 256), dim3(256), 0, L->stream, L->pred_hit.p, L->N, 0u);
-        HIP_TRY(hipGetLastError());
-    }
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
         if (L->lay.dense != ESP_NONE)

@@ -35,6 +35,7 @@ struct DenseLU {
     uint32_t *porig;  // n: original row of pivot row t
     const uint32_t *pred;  // the template's predicted first factor (PredTape, global memory), or null
     uint32_t *pred_hit;    // the lane's record of its first factor: 1 predicted, 2 serial (global), or null
+    bool staged;           // pred_stage already copied the tape into the lane's arrays
 };
 
 // A first factor's symbolic tape, as the template's analysis writes it to
@@ -67,6 +68,7 @@ ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n, const ui
     DenseLU d;
     d.pred = pred;
     d.pred_hit = pred_hit;
+    d.staged = false;
     double *f = block + L.dense;
     d.F = f;
     f += (size_t)n * n;
@@ -399,6 +401,36 @@ __device__ bool dense_probe_holds_wave(const Pattern &P, const LaneLU &s, const 
 
 // ------------------------------------------- the predicted first factor
 
+// The predicted tape's symbolic arrays into the lane's solver (pinv, lp,
+// up, li, ui, prow) and dense arrays (porig, masks, amap). It reads no
+// matrix value, so a kernel may run it in its prologue, its loads
+// overlapping the template's; the index lists are copied to the solver's
+// capacity (only a tape within it is ever used). All threads; no barrier.
+__device__ inline void pred_stage(const Pattern &P, LaneLU &s, const DenseLU &D)
+{
+    const uint32_t *g = D.pred;
+    if (g == nullptr) return;
+    const uint32_t n = P.n, nnz = P.nnz, r = threadIdx.x, nt = blockDim.x;
+    const PredTape T = pred_tape(n, nnz);
+    const uint32_t lists = n * n < P.cap ? n * n : (uint32_t)P.cap;
+    if (r < n) {
+        s.pinv[r] = g[T.pinv + r];
+        D.porig[r] = g[T.porig + r];
+        D.lmask[r] = (uint64_t)g[T.lmask + 2 * r + 1] << 32 | g[T.lmask + 2 * r];
+        D.umask[r] = (uint64_t)g[T.umask + 2 * r + 1] << 32 | g[T.umask + 2 * r];
+    }
+    for (uint32_t i = r; i <= n; i += nt) {
+        s.lp[i] = g[T.lp + i];
+        s.up[i] = g[T.up + i];
+    }
+    for (uint32_t i = r; i < lists; i += nt) {
+        s.li[i] = g[T.li + i];
+        s.ui[i] = g[T.ui + i];
+    }
+    for (uint32_t i = r; i < nnz; i += nt) s.prow[i] = g[T.prow + i];
+    for (uint32_t i = r; i < n * n; i += nt) D.amap[i] = g[T.amap + i];
+}
+
 // The lane's first full factor (lane_solver.h lu_factor on a fresh solver:
 // factor_pass with keep_prev off) computed on the wavefront from the
 // template's predicted tape. factor_pass's DFS, reach and operation order
@@ -448,25 +480,13 @@ __device__ bool dense_first_factor(const Pattern &P, LaneLU &s, const DenseLU &D
         for (uint32_t p = P.col_ptr[r]; p < P.col_ptr[r + 1]; p++) atomicMax(&rbits[P.row_idx[p]], bits(fabs_(vals[p])));
     __syncthreads();
     // the predicted tape into the lane's solver and dense arrays
-    const PredTape T = pred_tape(n, nnz);
     if (own) {
         const double rmax = from_bits(rbits[r]);
         s.rscale[r] = (rmax > 0 && isfinite_(rmax)) ? 1 / rmax : 1;
-        s.pinv[r] = g[T.pinv + r];
-        D.porig[r] = g[T.porig + r];
-        D.lmask[r] = (uint64_t)g[T.lmask + 2 * r + 1] << 32 | g[T.lmask + 2 * r];
-        D.umask[r] = (uint64_t)g[T.umask + 2 * r + 1] << 32 | g[T.umask + 2 * r];
         s.void_col[r] = 0;
         s.scaled_pivot[r] = 0;
     }
-    for (uint32_t i = r; i <= n; i += nt) {
-        s.lp[i] = g[T.lp + i];
-        s.up[i] = g[T.up + i];
-    }
-    for (uint32_t i = r; i < nl; i += nt) s.li[i] = g[T.li + i];
-    for (uint32_t i = r; i < nu; i += nt) s.ui[i] = g[T.ui + i];
-    for (uint32_t i = r; i < nnz; i += nt) s.prow[i] = g[T.prow + i];
-    for (uint32_t i = r; i < n * n; i += nt) D.amap[i] = g[T.amap + i];
+    if (!D.staged) pred_stage(P, s, D);
     __syncthreads();
     // the original row of every pivot row, and the pivot row of every
     // original row, one per thread
