@@ -635,7 +635,7 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             c.i++;
             double x;
             for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
-        } else if ((w == "pulse" || w == "sin" || w == "exp") && e.letter == 'v' && e.wave == 0) {
+        } else if ((w == "pulse" || w == "sin" || w == "exp") && (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // positional arguments fill the waveform's slots in order; the
             // ones not written keep the model defaults
             // (frontend/builder.zig:2841-2871, 2985-2990)
@@ -662,7 +662,7 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             }
             c.i++;
             e.wave = w == "pulse" ? 1 : w == "sin" ? 2 : 3;
-        } else if (w == "pwl" && e.letter == 'v' && e.wave == 0) {
+        } else if (w == "pwl" && (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // (time, value) pairs (frontend/builder.zig:2848-2859), then
             // `r=` and `td=` on the card
             c.i++;
@@ -999,8 +999,11 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         int kind;
         std::vector<std::pair<uint32_t, Value>> prm;
         switch (e.letter) {
-        case 'v': {
-            kind = ESPICE_LANES_VSOURCE;
+        case 'v':
+        case 'i': {
+            // a current source carries the voltage source's slots
+            // (models/isource.va:63-130)
+            kind = e.letter == 'v' ? ESPICE_LANES_VSOURCE : ESPICE_LANES_ISOURCE;
             if (e.wave == 4) {
                 // PWL: point count (slot 15), r= (16), td= (17), times from
                 // slot 18 and values from slot 34
@@ -1035,10 +1038,6 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
             }
             break;
         }
-        case 'i':
-            kind = ESPICE_LANES_ISOURCE;
-            prm.emplace_back(0, e.value.given ? e.value : lit(0.0));
-            break;
         case 'r':
             kind = ESPICE_LANES_RESISTOR;
             prm.emplace_back(0, e.value);

@@ -60,7 +60,7 @@ struct Element {
     std::vector<std::string> nodes;   // port nets
     std::string model;                // D and M cards
     Value value;                      // R, C, I value; V DC value
-    int wave = 0;                     // V waveform: 0 none, 1 PULSE(v1 v2 td tr tf pw per),
+    int wave = 0;                     // V or I waveform: 0 none, 1 PULSE(v1 v2 td tr tf pw per),
                                       // 2 SIN(vo va freq td theta phase),
                                       // 3 EXP(v1 v2 td1 tau1 td2 tau2), 4 PWL(t1 v1 ...)
     Value wave_args[7];

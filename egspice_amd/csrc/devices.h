@@ -40,7 +40,9 @@ enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP
        SRCP_PWL_V = SRCP_PWL_T + PWL_CAP, SRCP_EV1 = SRCP_PWL_V + PWL_CAP, SRCP_EV2, SRCP_ETD1, SRCP_ETAU1, SRCP_ETD2,
        SRCP_ETAU2, SRCP_K };
 constexpr double VA_PI = 3.14159265358979323846, VA_TWO_PI = 6.28318530717958647652;  // constants.vams
-enum { ISRC_DC = 0, ISRC_K = 1 };
+// the current source's parameters are the voltage source's, slot for slot
+// (models/isource.va:63-130 keeps the waveform names' `v` spelling)
+enum { ISRC_DC = 0, ISRC_K = VSRC_K };
 enum { RES_R = 0, RES_K = 1 };
 enum { CAP_C = 0, CAP_K = 1 };
 enum { DIO_IS = 0, DIO_N, DIO_CJO, DIO_VJ, DIO_M, DIO_FC, DIO_TT, DIO_GMIN, DIO_AREA, DIO_TNOM, DIO_TEMP, DIO_DTEMP,
@@ -64,7 +66,7 @@ enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT,
 constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4};
 constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
 constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
-constexpr int KIND_PK[K_NKIND] = {SRCP_K, 1, 2, 1, DIOP_K, M1P_K};
+constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
 // Residual rows a kind writes in a static evaluation.
 constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
 // Unknowns a kind's limiter writes (its limited image).
@@ -292,7 +294,8 @@ inline void prep_kind(int kind, const double *u, double *p)
     switch (kind) {
     case K_MOS1: prep_mos1(u, p); break;
     case K_DIODE: prep_diode(u, p); break;
-    case K_VSRC: {
+    case K_VSRC:
+    case K_ISRC: {
         // 0 everywhere but the -1 sentinels of TR, TF, PW, PER and PWL's r=,
         // and EXP's 1 ns time constants (models/vsource.va:58-96)
         for (int k = 0; k < SRCP_K; k++) {
@@ -313,7 +316,6 @@ inline void prep_kind(int kind, const double *u, double *p)
         }
         break;
     }
-    case K_ISRC: p[0] = u[ISRC_DC] == u[ISRC_DC] ? u[ISRC_DC] : 0.0; break;
     case K_RES:
         // resistor.va: rt = r at the nominal temperature (no tempcos), and
         // I(p, n) <+ V(p, n) / rt, whose slope is the reciprocal
@@ -839,10 +841,14 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tr
         o.jac[2][0] = 1.0;
         o.jac[2][1] = -1.0;
         break;
-    case K_ISRC:
-        o.res[0] = p[0];
-        o.res[1] = -p[0];
+    case K_ISRC: {
+        // I(p, n) <+ out: dc at an operating point, the waveform in a
+        // transient (models/isource.va:216-330)
+        const double out = tran ? src_value(p, t) : p[SRCP_DC];
+        o.res[0] = out;
+        o.res[1] = -out;
         break;
+    }
     case K_RES: {
         // V(p, n) / rt: a quotient by a constant, its lanes scaled by the
         // reciprocal (models/resistor.va, eval.zig's Dual division)

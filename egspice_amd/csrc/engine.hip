@@ -323,7 +323,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         const InstEntry ie = W.inst[e];
         bpc[2 * e] = inf_();
         bpc[2 * e + 1] = inf_();
-        if (ie.kind != K_VSRC) continue;
+        if (ie.kind != K_VSRC && ie.kind != K_ISRC) continue;
         double *p = prm_lds + ie.prm;
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / to.t_stop;
         if (p[SRCP_WAVE] == 3.0) {

@@ -370,7 +370,7 @@ ESP_HD double tran_next_bp(const WaveTemplate &W, const double *prm, double *bpc
     double best = inf_();
     for (uint32_t e = 0; e < W.n_inst; e++) {
         const InstEntry ie = W.inst[e];
-        if (ie.kind != K_VSRC) continue;
+        if (ie.kind != K_VSRC && ie.kind != K_ISRC) continue;
         double *c = bpc + 2 * e;
         if (tq < c[0] || tq >= c[1]) {
             c[0] = tq;

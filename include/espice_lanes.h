@@ -50,7 +50,7 @@ enum {
  * 40-115); with rs > 0 the diode takes a fresh internal row at add, and every
  * lane must then keep rs > 0 (and rs = 0 lanes stay at 0). */
 enum {
-    ESPICE_LANES_VSOURCE_PARAMS = 56, ESPICE_LANES_ISOURCE_PARAMS = 1, ESPICE_LANES_RESISTOR_PARAMS = 1,
+    ESPICE_LANES_VSOURCE_PARAMS = 56, ESPICE_LANES_ISOURCE_PARAMS = 56, ESPICE_LANES_RESISTOR_PARAMS = 1,
     ESPICE_LANES_CAPACITOR_PARAMS = 1, ESPICE_LANES_DIODE_PARAMS = 19, ESPICE_LANES_MOS1_PARAMS = 36
 };
 /* Diode parameter slots that decide its topology. */

@@ -44,7 +44,7 @@ int or_solver_is_tri(const or_solver *s);
 static const int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4};
 static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
 static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
-static const int PK[OR_NKIND] = {SRCP_K, 1, 2, 1, DIOP_K, M1P_K};
+static const int PK[OR_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
 
 /* Structural pattern of each kind (row-major bit masks over local
  * unknowns): G-plane entries and C-plane entries. */
@@ -327,7 +327,8 @@ void or_lanes_prepare(or_lanes *L)
                 switch (kind) {
                 case OR_MOS1: or_prep_mos1(u, p); break;
                 case OR_DIODE: or_prep_diode(u, p); break;
-                case OR_VSRC: {
+                case OR_VSRC:
+                case OR_ISRC: {
                     /* vsource.va parameter defaults; NaN = not given */
                     /* vsource.va's defaults: 0, the -1 sentinels of TR, TF, PW,
                      * PER and PWL's r=; the point count an integer in the table */
@@ -353,7 +354,6 @@ void or_lanes_prepare(or_lanes *L)
                     }
                     break;
                 }
-                case OR_ISRC: p[0] = u[OR_ISRC_DC] == u[OR_ISRC_DC] ? u[OR_ISRC_DC] : 0.0; break;
                 case OR_RES: /* resistor.va: rt = r, I = V / rt, slope 1 / rt */
                     p[0] = u[OR_RES_R];
                     p[1] = 1.0 / u[OR_RES_R];
@@ -415,10 +415,12 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
                 o.jac[2][0] = 1.0;
                 o.jac[2][1] = -1.0;
                 break;
-            case OR_ISRC:
-                o.res[0] = p[0];
-                o.res[1] = -p[0];
+            case OR_ISRC: { /* isource.va:216-330: dc at an OP, the waveform in a transient */
+                const double out = ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC];
+                o.res[0] = out;
+                o.res[1] = -out;
                 break;
+            }
             case OR_RES: {
                 double i = (lx[0] - lx[1]) / p[0];
                 o.res[0] = i;

@@ -17,7 +17,7 @@ enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC
        OR_VSRC_PWL_R, OR_VSRC_PWL_TD, OR_VSRC_PWL_T, OR_VSRC_PWL_V = OR_VSRC_PWL_T + 16,
        OR_VSRC_EXP_V1 = OR_VSRC_PWL_V + 16, OR_VSRC_EXP_V2, OR_VSRC_EXP_TD1, OR_VSRC_EXP_TAU1, OR_VSRC_EXP_TD2,
        OR_VSRC_EXP_TAU2, OR_VSRC_K };
-enum { OR_ISRC_DC = 0, OR_ISRC_K };
+enum { OR_ISRC_DC = 0, OR_ISRC_K = OR_VSRC_K }; /* the voltage source's slots (isource.va:63-130) */
 enum { OR_RES_R = 0, OR_RES_K };
 enum { OR_CAP_C = 0, OR_CAP_K };
 enum { OR_DIO_IS = 0, OR_DIO_N, OR_DIO_CJO, OR_DIO_VJ, OR_DIO_M, OR_DIO_FC, OR_DIO_TT, OR_DIO_GMIN, OR_DIO_AREA,

@@ -257,7 +257,9 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
 {
     const or_circuit *c = L->c;
     const uint32_t n = or_ckt_n(c), nnz = or_ckt_nnz(c), nq = n + 1, n_qt = or_tape_len(c);
-    const uint32_t n_src = or_ckt_count(c, OR_VSRC);
+    /* the independent sources, voltage then current: both carry the
+     * waveform block and its breakpoints (vsource.va, isource.va) */
+    const uint32_t n_vsrc = or_ckt_count(c, OR_VSRC), n_src = n_vsrc + or_ckt_count(c, OR_ISRC);
     const int has_charge = n_qt > 0;
     memset(st, 0, sizeof *st);
     *times = NULL;
@@ -268,7 +270,7 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     tr_bps bps = {L, l, n_src, calloc(n_src + 1, sizeof(double[SRCP_K])), calloc(n_src + 1, sizeof(src_cache)), INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n_src; i++) {
         double *p = bps.prm[i];
-        or_load_p(L, OR_VSRC, i, l, p);
+        or_load_p(L, i < n_vsrc ? OR_VSRC : OR_ISRC, i < n_vsrc ? i : i - n_vsrc, l, p);
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / opt->t_stop;
         if (p[SRCP_WAVE] == 3.0) { /* builder.zig:862-867 */
             if (p[SRCP_ETD1] == 0) p[SRCP_ETD1] = opt->dt_init;
@@ -291,7 +293,8 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     double *saved = malloc(8 * ((size_t)n_src * SRCP_K + 1));
     for (uint32_t i = 0; i < n_src; i++)
         for (int k = 0; k < SRCP_K; k++) {
-            double *slot = &L->pprm[OR_VSRC][((size_t)i * SRCP_K + k) * L->N + l];
+            const uint32_t id = i < n_vsrc ? i : i - n_vsrc;
+            double *slot = &L->pprm[i < n_vsrc ? OR_VSRC : OR_ISRC][((size_t)id * SRCP_K + k) * L->N + l];
             saved[i * SRCP_K + k] = *slot;
             *slot = bps.prm[i][k];
         }
@@ -481,7 +484,10 @@ done:
     *rows = rec.x;
     rc = (int)rec.len;
     for (uint32_t i = 0; i < n_src; i++)
-        for (int k = 0; k < SRCP_K; k++) L->pprm[OR_VSRC][((size_t)i * SRCP_K + k) * L->N + l] = saved[i * SRCP_K + k];
+        for (int k = 0; k < SRCP_K; k++) {
+            const uint32_t id = i < n_vsrc ? i : i - n_vsrc;
+            L->pprm[i < n_vsrc ? OR_VSRC : OR_ISRC][((size_t)id * SRCP_K + k) * L->N + l] = saved[i * SRCP_K + k];
+        }
     free(saved);
     or_solver_free(w.solver);
     free(w.g); free(w.a); free(w.rhs); free(w.q); free(w.cc); free(w.dx); free(w.x_old);

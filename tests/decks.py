@@ -94,7 +94,20 @@ RC c 0 1k
 .tran 10u 5m
 .end
 """
-SOURCES_POINTS = 526  # docs/learn/sources.md:55-58: 10 devices, 10 variables
+SOURCES_POINTS = 526
+# the same four waveforms as current sources into the same loads
+ISOURCES_SP = """Independent current source waveforms
+IP 0 p PULSE(0 1m 1m 0.1m 0.1m 2m 5m)
+IS 0 s SIN(0.5m 0.5m 1k)
+IW 0 w PWL(0 0 1m 1m 3m 1m 4m 0)
+IE 0 e EXP(0 1m 1m 0.5m 3m 0.5m)
+RP p 0 1k
+RS s 0 1k
+RW w 0 1k
+RE e 0 1k
+.tran 10u 5m
+.end
+"""  # docs/learn/sources.md:55-58: 10 devices, 10 variables
 
 MONTE_AGAUSS_SP = """Divider spread with agauss and a Monte Carlo DC
 * agauss(nominal, variation, sigmas): 30 ohms is the 3-sigma spread.

@@ -46,6 +46,15 @@ def test_documented_decks_read_and_build(engine, deck):
     assert e.value.status == 6 and e.value.name == "GpuUnavailable"
 
 
+@pytest.mark.skipif(not NO_GPU, reason="a GPU is visible: the deck runs in the gpu tests")
+def test_isource_waveforms_read_and_build(engine):
+    """I cards take the V cards' waveforms (models/isource.va): the deck
+    reads and builds; only the GPU is missing."""
+    with pytest.raises(E.PointsError) as e:
+        E.Problem(decks.ISOURCES_SP)
+    assert e.value.status == 6 and e.value.name == "GpuUnavailable"
+
+
 @pytest.mark.parametrize("card,name", [
     ("L1 in out 1u", "UnsupportedElement"),
     ("X1 in out sub", "UnsupportedElement"),

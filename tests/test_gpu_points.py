@@ -231,6 +231,19 @@ def test_cli_sources(tmp_path, engine):
         assert want in lines, (want, r.stdout)
 
 
+def test_cli_isources(tmp_path, engine):
+    say("case: egspice_amd/espice --backend=hip with I-source PULSE, SIN, PWL, EXP")
+    deck = tmp_path / "isources.sp"
+    deck.write_text(decks.ISOURCES_SP)
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    r = subprocess.run([exe, "--backend=hip", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    for want in ("Independent current source waveforms: 8 devices",
+                 f"  Transient Analysis: {decks.SOURCES_POINTS} points, 5 variables"):
+        assert want in lines, (want, r.stdout)
+
+
 def test_diode_rectifier_points(oracle, engine):
     N = 12
     say(f"case: diode_rectifier.sp with live rs, cjo: {N} points, bitwise vs the oracle")

@@ -344,6 +344,31 @@ def test_exp_lanes_bitwise(oracle, engine):
     assert (res["status"] == 0).all()
 
 
+def test_isource_waveform_lanes_bitwise(oracle, engine):
+    """Current sources with PULSE and EXP waveforms (models/isource.va: the
+    voltage source's slots, expressions and timers) into an RC, their
+    levels and times varied per lane; every row bitwise the oracle's."""
+    N = 16
+    say(f"case: I-source PULSE + EXP into RC, {N} lanes, tran 10n 4u")
+
+    def deck(add):
+        add(E.ISOURCE, ["0", "in"], {0: np.nan, 1: 1.0, 2: 0.0, 3: 1e-3, 4: 3e-7, 5: 1e-7, 6: 2e-7, 7: 1e-6,
+                                     8: 2.5e-6})
+        add(E.ISOURCE, ["0", "in"], {0: np.nan, 1: 3.0, 50: 0.0, 51: 5e-4, 52: 2e-7, 53: 1e-7, 54: 2e-6})
+        add(E.CAPACITOR, ["out", "0"], {0: 1e-9})
+        add(E.RESISTOR, ["in", "out"], {0: 1e3})
+        add(E.RESISTOR, ["in", "0"], {0: 1e3})
+
+    t, c = build_both(oracle, deck)
+    rng = np.random.default_rng(29)
+    sets = {(E.ISOURCE, 0, 3): rng.uniform(2e-4, 2e-3, N), (E.ISOURCE, 0, 4): rng.uniform(0.0, 1e-6, N),
+            (E.ISOURCE, 1, 53): np.where(np.arange(N) % 3 == 0, 0.0, rng.uniform(5e-8, 5e-7, N)),
+            (E.RESISTOR, 0, 0): rng.uniform(200.0, 2000.0, N)}
+    passes = [E.meas_cross(t.node("out"), 0.25, rise=1), E.meas_extremum(t.node("out"), 1)]
+    res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=4e-6, dt_init=1e-8), passes)
+    assert (res["status"] == 0).all()
+
+
 def test_diode_rs_topology_is_per_template(engine):
     say("case: a lane may not move a diode's RS across zero")
     t = E.Template()

@@ -337,3 +337,46 @@ def test_sources_waveforms(oracle):
     assert np.allclose(x[:, c.node("c")], 1.0, rtol=0, atol=1e-12)
     for corner in (m, 1.1 * m, 3 * m, 3.1 * m, 3.2 * m, 4 * m):
         assert np.min(np.abs(t - corner)) <= 1e-15 * corner, corner
+
+
+def isources_run(oracle):
+    """docs/examples/sources.sp's four waveforms as current sources
+    (models/isource.va: the voltage source's slots and expressions), each
+    driving 1 mA-scale current into its own 1k load."""
+    from egspice_amd import spice_number as sn
+    m = sn("1m")
+    c = O.Circuit(oracle)
+    c.add(O.ISRC, [0, "p"], {0: np.nan, 1: 1.0, 2: 0.0, 3: m, 4: m, 5: 0.1 * m, 6: 0.1 * m, 7: 2 * m, 8: 5 * m})
+    c.add(O.ISRC, [0, "s"], {0: np.nan, 1: 2.0, 9: 0.5 * m, 10: 0.5 * m, 11: sn("1k")})
+    pwl = {0: np.nan, 1: 4.0, 15: 4.0}
+    for k, (tk, vk) in enumerate([(0.0, 0.0), (m, m), (3 * m, m), (4 * m, 0.0)]):
+        pwl[18 + k], pwl[34 + k] = tk, vk
+    c.add(O.ISRC, [0, "w"], pwl)
+    c.add(O.ISRC, [0, "e"], {0: np.nan, 1: 3.0, 50: 0.0, 51: m, 52: m, 53: 0.5 * m, 54: 3 * m, 55: 0.5 * m})
+    for nd in "pswe":
+        c.add(O.RES, [nd, 0], {0: sn("1k")})
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(sn("5m"), sn("10u")))
+    return c, t, x, st
+
+
+def test_isource_waveforms(oracle):
+    """Each node is 1k times its source's waveform at every accepted time,
+    and the current sources' corners are breakpoints exactly as the voltage
+    sources' are (isource.va declares the same timers): the same 526 points
+    as docs/examples/sources.sp."""
+    import decks
+    from egspice_amd import spice_number as sn
+    c, t, x, st = isources_run(oracle)
+    assert st.status == 0 and st.points == decks.SOURCES_POINTS == len(t)
+    m = sn("1m")
+    pulse = np.interp(t, [0, m, 1.1 * m, 3.1 * m, 3.2 * m, 5 * m], [0, 0, 1, 1, 0, 0])
+    assert np.allclose(x[:, c.node("p")], pulse, rtol=0, atol=1e-12)
+    assert np.allclose(x[:, c.node("s")], 0.5 + 0.5 * np.sin(2 * np.pi * 1e3 * t), rtol=0, atol=1e-12)
+    assert np.allclose(x[:, c.node("w")], np.interp(t, [0, m, 3 * m, 4 * m, 5 * m], [0, 1, 1, 0, 0]), rtol=0,
+                       atol=1e-12)
+    e1 = np.where(t > m, 1 - np.exp(-(t - m) / (0.5 * m)), 0.0)
+    e2 = np.where(t > 3 * m, -(1 - np.exp(-(t - 3 * m) / (0.5 * m))), 0.0)
+    assert np.allclose(x[:, c.node("e")], e1 + e2, rtol=0, atol=1e-12)
