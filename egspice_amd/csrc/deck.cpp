@@ -635,12 +635,14 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             c.i++;
             double x;
             for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
-        } else if ((w == "pulse" || w == "sin" || w == "exp") && (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
+        } else if ((w == "pulse" || w == "sin" || w == "exp" || w == "am" || (w == "sffm" && e.letter == 'v')) &&
+                   (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // positional arguments fill the waveform's slots in order; the
             // ones not written keep the model defaults
             // (frontend/builder.zig:2841-2871, 2985-2990)
-            const int most = w == "pulse" ? 7 : 6;
-            const std::string bad = (w == "pulse" ? "BadPulse " : w == "sin" ? "BadSin " : "BadExp ") + e.name;
+            const int most = w == "pulse" ? 7 : w == "sffm" ? 8 : w == "am" ? 7 : 6;
+            const std::string bad = (w == "pulse" ? "BadPulse " : w == "sin" ? "BadSin " : w == "exp" ? "BadExp "
+                                     : w == "sffm" ? "BadSffm " : "BadAm ") + e.name;
             c.i++;
             if (!c.next_is("(")) {
                 err = bad;
@@ -661,7 +663,7 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
                 return false;
             }
             c.i++;
-            e.wave = w == "pulse" ? 1 : w == "sin" ? 2 : 3;
+            e.wave = w == "pulse" ? 1 : w == "sin" ? 2 : w == "exp" ? 3 : w == "sffm" ? 5 : 6;
         } else if (w == "pwl" && (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // (time, value) pairs (frontend/builder.zig:2848-2859), then
             // `r=` and `td=` on the card
@@ -1019,18 +1021,18 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
                 if (e.value.given) prm.emplace_back(0, e.value);
             } else if (e.wave) {
                 // PULSE args go to slots 2..8, SIN args to 9..14, EXP args
-                // to 50..55; a dc left unwritten is filled from the
+                // to 50..55, SFFM's to 56..63, AM's to 64..70; a dc left unwritten is filled from the
                 // waveform's t = 0 value when the lane constants are
                 // prepared. EXP's time constants start at 0 on a card, so
                 // the .tran step fills the omitted ones
                 // (frontend/builder.zig:2860-2866)
-                const uint32_t base = e.wave == 1 ? 2 : e.wave == 2 ? 9 : 50;
+                const uint32_t base = e.wave == 1 ? 2 : e.wave == 2 ? 9 : e.wave == 3 ? 50 : e.wave == 5 ? 56 : 64;
                 prm.emplace_back(1, lit(double(e.wave)));
                 if (e.wave == 3) {
                     if (!e.wave_args[3].given) prm.emplace_back(53, lit(0.0));
                     if (!e.wave_args[5].given) prm.emplace_back(55, lit(0.0));
                 }
-                for (int a = 0; a < 7; a++)
+                for (int a = 0; a < 8; a++)
                     if (e.wave_args[a].given) prm.emplace_back(base + a, e.wave_args[a]);
                 if (e.value.given) prm.emplace_back(0, e.value);
             } else {
@@ -1084,17 +1086,21 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
         default: err = "UnsupportedElement " + e.name; return false;
         }
         // template constants: the literals now; expressions are point values
-        double pv[64];
         const int npk[] = {ESPICE_LANES_VSOURCE_PARAMS,   ESPICE_LANES_ISOURCE_PARAMS, ESPICE_LANES_RESISTOR_PARAMS,
                            ESPICE_LANES_CAPACITOR_PARAMS, ESPICE_LANES_DIODE_PARAMS,   ESPICE_LANES_MOS1_PARAMS};
+        std::vector<double> pv(npk[kind]);
         for (int a = 0; a < npk[kind]; a++) pv[a] = NAN;
         for (auto &[idx, v] : prm) {
+            if (idx >= pv.size()) {
+                err = "InvalidParam " + e.name;
+                return false;
+            }
             std::string xerr;
             pv[idx] = v.expr ? eval_expr(*v.expr, nominal, xerr) : v.num;
             if (!xerr.empty()) pv[idx] = NAN;
         }
         uint32_t inst = 0;
-        if (espice_lanes_template_add(t, (uint32_t)kind, rows, pv, (size_t)npk[kind], &inst) != ESPICE_OK) {
+        if (espice_lanes_template_add(t, (uint32_t)kind, rows, pv.data(), (size_t)npk[kind], &inst) != ESPICE_OK) {
             size_t req;
             char buf[128];
             espice_lanes_error_message(buf, sizeof buf, &req);

@@ -62,8 +62,10 @@ struct Element {
     Value value;                      // R, C, I value; V DC value
     int wave = 0;                     // V or I waveform: 0 none, 1 PULSE(v1 v2 td tr tf pw per),
                                       // 2 SIN(vo va freq td theta phase),
-                                      // 3 EXP(v1 v2 td1 tau1 td2 tau2), 4 PWL(t1 v1 ...)
-    Value wave_args[7];
+                                      // 3 EXP(v1 v2 td1 tau1 td2 tau2), 4 PWL(t1 v1 ...),
+                                      // 5 SFFM(vo va fm mdi fc td phasem phasec), V only,
+                                      // 6 AM(va vo mf fc td phasec phases)
+    Value wave_args[8];
     std::vector<Value> pwl;           // PWL times and values, interleaved
     Value pwl_r, pwl_td;              // PWL r= and td=
     std::map<std::string, Value> kv;  // W=, L=, AD=, M=, area

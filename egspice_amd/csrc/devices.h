@@ -30,7 +30,10 @@ constexpr int PWL_CAP = 16;
 enum { VSRC_DC = 0, VSRC_WAVE, VSRC_V1, VSRC_V2, VSRC_TD, VSRC_TR, VSRC_TF, VSRC_PW, VSRC_PER, VSRC_SVO, VSRC_SVA,
        VSRC_SFREQ, VSRC_STD, VSRC_STHETA, VSRC_SPHASE, VSRC_PWL_LEN, VSRC_PWL_R, VSRC_PWL_TD, VSRC_PWL_T,
        VSRC_PWL_V = VSRC_PWL_T + PWL_CAP, VSRC_EV1 = VSRC_PWL_V + PWL_CAP, VSRC_EV2, VSRC_ETD1, VSRC_ETAU1, VSRC_ETD2,
-       VSRC_ETAU2, VSRC_K };
+       VSRC_ETAU2, VSRC_FVO, VSRC_FVA, VSRC_FFM, VSRC_FMDI, VSRC_FFC, VSRC_FTD, VSRC_FPHM, VSRC_FPHC, VSRC_AVA, VSRC_AVO,
+       VSRC_AMF, VSRC_AFC, VSRC_ATD, VSRC_APHC, VSRC_APHS, VSRC_K };
+static_assert(VSRC_EV1 == 50 && VSRC_FVO == 56 && VSRC_AVA == 64 && VSRC_K == 71,
+              "the deck reader writes EXP, SFFM and AM arguments into these slots");
 static_assert(VSRC_PWL_LEN == 15 && VSRC_PWL_R == 16 && VSRC_PWL_TD == 17 && VSRC_PWL_T == 18 && VSRC_PWL_V == 34,
               "the deck reader writes PWL tables into these slots");
 // Prepared source constants: the user slots with vsource.va's defaults
@@ -38,7 +41,8 @@ static_assert(VSRC_PWL_LEN == 15 && VSRC_PWL_R == 16 && VSRC_PWL_TD == 17 && VSR
 enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
        SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_PWL_LEN, SRCP_PWL_R, SRCP_PWL_TD, SRCP_PWL_T,
        SRCP_PWL_V = SRCP_PWL_T + PWL_CAP, SRCP_EV1 = SRCP_PWL_V + PWL_CAP, SRCP_EV2, SRCP_ETD1, SRCP_ETAU1, SRCP_ETD2,
-       SRCP_ETAU2, SRCP_K };
+       SRCP_ETAU2, SRCP_FVO, SRCP_FVA, SRCP_FFM, SRCP_FMDI, SRCP_FFC, SRCP_FTD, SRCP_FPHM, SRCP_FPHC, SRCP_AVA, SRCP_AVO,
+       SRCP_AMF, SRCP_AFC, SRCP_ATD, SRCP_APHC, SRCP_APHS, SRCP_K };
 constexpr double VA_PI = 3.14159265358979323846, VA_TWO_PI = 6.28318530717958647652;  // constants.vams
 // the current source's parameters are the voltage source's, slot for slot
 // (models/isource.va:63-130 keeps the waveform names' `v` spelling)
@@ -67,6 +71,11 @@ constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4};
 constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
 constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
 constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
+// room for one instance's user or prepared parameters on the host
+constexpr int PARAM_MAX = 128;
+static_assert(VSRC_K <= PARAM_MAX && SRCP_K <= PARAM_MAX && DIO_K <= PARAM_MAX && DIOP_K <= PARAM_MAX &&
+                  M1_K <= PARAM_MAX && M1P_K <= PARAM_MAX,
+              "every kind's parameters fit PARAM_MAX");
 // Residual rows a kind writes in a static evaluation.
 constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
 // Unknowns a kind's limiter writes (its limited image).
@@ -299,8 +308,9 @@ inline void prep_kind(int kind, const double *u, double *p)
         // 0 everywhere but the -1 sentinels of TR, TF, PW, PER and PWL's r=,
         // and EXP's 1 ns time constants (models/vsource.va:58-96)
         for (int k = 0; k < SRCP_K; k++) {
-            const bool sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
-            const double dflt = sentinel ? -1.0 : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9 : 0.0;
+            const bool sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R || k == SRCP_FFM;
+            const double dflt = sentinel ? -1.0 : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9
+                                                : k == SRCP_FMDI ? 90.0 : 0.0;
             p[k] = u[k] == u[k] ? u[k] : dflt;
         }
         // an integer point count within the table
@@ -475,6 +485,24 @@ ESP_HD double src_value(const P &p, double t)
                                  e_exp(va_min(-p[SRCP_STHETA] * t_sin, 80.0));
     }
     if (p[SRCP_WAVE] == 4.0) return pwl_value(p, t);
+    if (p[SRCP_WAVE] == 5.0) {
+        // SFFM (models/vsource.va:304-317): zero until TD, then the carrier
+        // phase-modulated by the clamped index
+        double mdi = p[SRCP_FMDI];
+        if (mdi > p[SRCP_FFC] / p[SRCP_FFM]) mdi = p[SRCP_FFC] / p[SRCP_FFM];
+        else if (mdi < 0.0) mdi = 0.0;
+        const double t_fm = t - p[SRCP_FTD];
+        if (t_fm <= 0.0) return 0.0;
+        return p[SRCP_FVO] + p[SRCP_FVA] * e_sin((VA_TWO_PI * p[SRCP_FFC] * t_fm + p[SRCP_FPHC] * VA_PI / 180.0) +
+                                                 mdi * e_sin(VA_TWO_PI * p[SRCP_FFM] * t_fm + p[SRCP_FPHM] * VA_PI / 180.0));
+    }
+    if (p[SRCP_WAVE] == 6.0) {
+        // AM (vsource.va and isource.va alike): 0 before TD
+        if (!(t >= p[SRCP_ATD])) return 0.0;
+        const double env = p[SRCP_AVO] + e_sin(VA_TWO_PI * p[SRCP_AMF] * t + p[SRCP_APHS] * VA_PI / 180.0);
+        const double car = e_sin(VA_TWO_PI * p[SRCP_AFC] * t + p[SRCP_APHC] * VA_PI / 180.0);
+        return p[SRCP_AVA] * env * car;
+    }
     if (p[SRCP_WAVE] == 3.0) {
         // EXP (models/vsource.va:252-258): two one-sided exponentials on V1
         const double t1 = t - p[SRCP_ETD1], t2 = t - p[SRCP_ETD2];

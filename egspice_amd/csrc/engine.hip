@@ -325,6 +325,10 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         bpc[2 * e + 1] = inf_();
         if (ie.kind != K_VSRC && ie.kind != K_ISRC) continue;
         double *p = prm_lds + ie.prm;
+        if (p[SRCP_WAVE] == 5.0) {
+            if (p[SRCP_FFM] == -1.0) p[SRCP_FFM] = 5.0 / to.t_stop;
+            if (p[SRCP_FFC] == 0.0) p[SRCP_FFC] = 500.0 / to.t_stop;
+        }
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / to.t_stop;
         if (p[SRCP_WAVE] == 3.0) {
             if (p[SRCP_ETD1] == 0) p[SRCP_ETD1] = to.dt_init;
@@ -902,12 +906,16 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         const uint32_t cnt = L->t->count[kind];
         const int uk = KIND_UK[kind], pk = KIND_PK[kind];
         std::vector<double> host((size_t)cnt * pk * N);
-        double u[64], p[64];
+        double u[PARAM_MAX], p[PARAM_MAX];
         for (uint32_t i = 0; i < cnt; i++)
             for (size_t l = 0; l < N; l++) {
                 for (int k = 0; k < uk; k++) u[k] = L->uprm[kind][((size_t)i * uk + k) * N + l];
                 // a diode's ai row is the template's topology: RS may move
                 // per lane but not across zero (diode.va:364-365)
+                // SFFM on a current source follows isrcload.c's own slots and
+                // a dc fill from the .tran defaults (isource.va, builder.zig
+                // dcFromWaveform); not restated here
+                if (kind == K_ISRC && u[VSRC_WAVE] == 5.0) return fail(ESPICE_INVALID_ARGUMENT, "IsourceSffmUnsupported");
                 if (kind == K_DIODE) {
                     const uint32_t *nd = &L->t->nodes[K_DIODE][(size_t)i * 3];
                     if ((u[DIO_RS] > 0.0) != (nd[2] != nd[0])) return fail(ESPICE_INVALID_ARGUMENT, "DiodeRsTopology");

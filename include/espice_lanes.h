@@ -42,15 +42,17 @@ enum {
     ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5
 };
 /* User parameters per kind (NaN = not given). VSOURCE: dc, waveform (0 DC,
- * 1 PULSE, 2 SIN, 3 EXP, 4 PWL), PULSE v1 v2 td tr tf pw per, SIN vo va freq td
+ * 1 PULSE, 2 SIN, 3 EXP, 4 PWL, 5 SFFM, 6 AM), PULSE v1 v2 td tr tf pw per, SIN vo va freq td
  * theta phase, PWL point count, r=, td=, 16 times, 16 values (its table
- * holds 64), EXP v1 v2 td1 tau1 td2 tau2 (models/vsource.va:58-96); a dc not given is the waveform's t = 0
+ * holds 64), EXP v1 v2 td1 tau1 td2 tau2, SFFM vo va fm mdi fc td phasem
+ * phasec, AM va vo mf fc td phasec phases (models/vsource.va:58-130); the
+ * current source has the same slots (models/isource.va:63-130), SFFM excepted; a dc not given is the waveform's t = 0
  * value (frontend/builder.zig:1599-1605, 2934-2966). DIODE: is n cjo vj m fc
  * tt gmin area tnom temp dtemp eg xti rs trs trs2 tm1 tm2 (models/diode.va:
  * 40-115); with rs > 0 the diode takes a fresh internal row at add, and every
  * lane must then keep rs > 0 (and rs = 0 lanes stay at 0). */
 enum {
-    ESPICE_LANES_VSOURCE_PARAMS = 56, ESPICE_LANES_ISOURCE_PARAMS = 56, ESPICE_LANES_RESISTOR_PARAMS = 1,
+    ESPICE_LANES_VSOURCE_PARAMS = 71, ESPICE_LANES_ISOURCE_PARAMS = 71, ESPICE_LANES_RESISTOR_PARAMS = 1,
     ESPICE_LANES_CAPACITOR_PARAMS = 1, ESPICE_LANES_DIODE_PARAMS = 19, ESPICE_LANES_MOS1_PARAMS = 36
 };
 /* Diode parameter slots that decide its topology. */

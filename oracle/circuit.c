@@ -45,6 +45,11 @@ static const int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4};
 static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
 static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
 static const int PK[OR_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
+/* room for one instance's user or prepared parameters */
+#define OR_PARAM_MAX 128
+_Static_assert(OR_VSRC_K <= OR_PARAM_MAX && SRCP_K <= OR_PARAM_MAX && OR_DIO_K <= OR_PARAM_MAX && DIOP_K <= OR_PARAM_MAX &&
+                   OR_M1_K <= OR_PARAM_MAX && M1P_K <= OR_PARAM_MAX,
+               "every kind's parameters fit OR_PARAM_MAX");
 
 /* Structural pattern of each kind (row-major bit masks over local
  * unknowns): G-plane entries and C-plane entries. */
@@ -320,7 +325,7 @@ void or_lanes_prepare(or_lanes *L)
     const uint32_t N = L->N;
     for (int kind = 0; kind < OR_NKIND; kind++) {
         uint32_t cnt = L->c->count[kind];
-        double u[64], p[64];
+        double u[OR_PARAM_MAX], p[OR_PARAM_MAX];
         for (uint32_t i = 0; i < cnt; i++)
             for (uint32_t l = 0; l < N; l++) {
                 for (int k = 0; k < UK[kind]; k++) u[k] = L->uprm[kind][((size_t)i * UK[kind] + k) * N + l];
@@ -334,7 +339,9 @@ void or_lanes_prepare(or_lanes *L)
                      * PER and PWL's r=; the point count an integer in the table */
                     for (int k = 0; k < SRCP_K; k++) {
                         const int sentinel = (k >= SRCP_TR && k <= SRCP_PER) || k == SRCP_PWL_R;
-                        const double dflt = sentinel ? -1.0 : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9 : 0.0;
+                        const double dflt = (sentinel || k == SRCP_FFM) ? -1.0
+                                            : (k == SRCP_ETAU1 || k == SRCP_ETAU2) ? 1.0e-9
+                                            : k == SRCP_FMDI ? 90.0 : 0.0;
                         p[k] = u[k] == u[k] ? u[k] : dflt;
                     }
                     {
@@ -393,7 +400,7 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             const uint32_t *sl = c->slots[kind] + (size_t)id * c->kslots[kind];
-            double p[64], lx[4], xl[4];
+            double p[OR_PARAM_MAX], lx[4], xl[4];
             or_devout o;
             memset(&o, 0, sizeof o);
             or_load_p(L, kind, id, l, p);
@@ -501,7 +508,7 @@ int or_limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old,
         unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
-            double p[64], cur[4], old[4], lim[4];
+            double p[OR_PARAM_MAX], cur[4], old[4], lim[4];
             double *lm = lim_at(L, kind, id, l);
             or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) {
@@ -527,7 +534,7 @@ void or_seed_pass(or_lanes *L, uint32_t l, const double *x)
         unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
-            double p[64], cur[4], lim[4];
+            double p[OR_PARAM_MAX], cur[4], lim[4];
             double *lm = lim_at(L, kind, id, l);
             or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) cur[u] = x[nd[u]];

@@ -632,6 +632,23 @@ static double pwl_at(const double *p, double t)
 double or_source_value(const double *p, double t)
 {
     if (p[SRCP_WAVE] == 4.0) return pwl_at(p, t);
+    if (p[SRCP_WAVE] == 5.0) {
+        /* SFFM, models/vsource.va:304-317 */
+        double mdi = p[SRCP_FMDI];
+        if (mdi > p[SRCP_FFC] / p[SRCP_FFM]) mdi = p[SRCP_FFC] / p[SRCP_FFM];
+        else if (mdi < 0.0) mdi = 0.0;
+        const double tf = t - p[SRCP_FTD];
+        if (tf <= 0.0) return 0.0;
+        return p[SRCP_FVO] + p[SRCP_FVA] * om_sin((VA_TWO_PI * p[SRCP_FFC] * tf + p[SRCP_FPHC] * VA_PI / 180.0) +
+                                                  mdi * om_sin(VA_TWO_PI * p[SRCP_FFM] * tf + p[SRCP_FPHM] * VA_PI / 180.0));
+    }
+    if (p[SRCP_WAVE] == 6.0) {
+        /* AM, vsource.va and isource.va alike */
+        if (!(t >= p[SRCP_ATD])) return 0.0;
+        const double env = p[SRCP_AVO] + om_sin(VA_TWO_PI * p[SRCP_AMF] * t + p[SRCP_APHS] * VA_PI / 180.0);
+        const double car = om_sin(VA_TWO_PI * p[SRCP_AFC] * t + p[SRCP_APHC] * VA_PI / 180.0);
+        return p[SRCP_AVA] * env * car;
+    }
     if (p[SRCP_WAVE] == 3.0) {
         /* EXP, models/vsource.va:252-258 */
         double out = p[SRCP_EV1];

@@ -16,7 +16,9 @@ enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC
        OR_VSRC_SVO, OR_VSRC_SVA, OR_VSRC_SFREQ, OR_VSRC_STD, OR_VSRC_STHETA, OR_VSRC_SPHASE, OR_VSRC_PWL_LEN,
        OR_VSRC_PWL_R, OR_VSRC_PWL_TD, OR_VSRC_PWL_T, OR_VSRC_PWL_V = OR_VSRC_PWL_T + 16,
        OR_VSRC_EXP_V1 = OR_VSRC_PWL_V + 16, OR_VSRC_EXP_V2, OR_VSRC_EXP_TD1, OR_VSRC_EXP_TAU1, OR_VSRC_EXP_TD2,
-       OR_VSRC_EXP_TAU2, OR_VSRC_K };
+       OR_VSRC_EXP_TAU2, OR_VSRC_SFFM_VO, OR_VSRC_SFFM_VA, OR_VSRC_SFFM_FM, OR_VSRC_SFFM_MDI, OR_VSRC_SFFM_FC,
+       OR_VSRC_SFFM_TD, OR_VSRC_SFFM_PHASEM, OR_VSRC_SFFM_PHASEC, OR_VSRC_AM_VA, OR_VSRC_AM_VO, OR_VSRC_AM_MF,
+       OR_VSRC_AM_FC, OR_VSRC_AM_TD, OR_VSRC_AM_PHASEC, OR_VSRC_AM_PHASES, OR_VSRC_K };
 enum { OR_ISRC_DC = 0, OR_ISRC_K = OR_VSRC_K }; /* the voltage source's slots (isource.va:63-130) */
 enum { OR_RES_R = 0, OR_RES_K };
 enum { OR_CAP_C = 0, OR_CAP_K };

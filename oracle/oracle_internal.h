@@ -43,7 +43,8 @@ void or_seed_diode(const double *p, const double *cur, double *lim);
 enum { SRCP_DC = 0, SRCP_WAVE, SRCP_V1, SRCP_V2, SRCP_TD, SRCP_TR, SRCP_TF, SRCP_PW, SRCP_PER, SRCP_SVO, SRCP_SVA,
        SRCP_SFREQ, SRCP_STD, SRCP_STHETA, SRCP_SPHASE, SRCP_PWL_LEN, SRCP_PWL_R, SRCP_PWL_TD, SRCP_PWL_T,
        SRCP_PWL_V = SRCP_PWL_T + OR_PWL_CAP, SRCP_EV1 = SRCP_PWL_V + OR_PWL_CAP, SRCP_EV2, SRCP_ETD1, SRCP_ETAU1,
-       SRCP_ETD2, SRCP_ETAU2, SRCP_K };
+       SRCP_ETD2, SRCP_ETAU2, SRCP_FVO, SRCP_FVA, SRCP_FFM, SRCP_FMDI, SRCP_FFC, SRCP_FTD, SRCP_FPHM, SRCP_FPHC,
+       SRCP_AVA, SRCP_AVO, SRCP_AMF, SRCP_AFC, SRCP_ATD, SRCP_APHC, SRCP_APHS, SRCP_K };
 /* The source's value at transient time t (models/vsource.va:233-247). */
 double or_source_value(const double *p, double t);
 #define VA_PI 3.14159265358979323846     /* constants.vams M_PI */

@@ -271,6 +271,10 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     for (uint32_t i = 0; i < n_src; i++) {
         double *p = bps.prm[i];
         or_load_p(L, i < n_vsrc ? OR_VSRC : OR_ISRC, i < n_vsrc ? i : i - n_vsrc, l, p);
+        if (p[SRCP_WAVE] == 5.0) { /* builder.zig:854-857 */
+            if (p[SRCP_FFM] == -1.0) p[SRCP_FFM] = 5.0 / opt->t_stop;
+            if (p[SRCP_FFC] == 0.0) p[SRCP_FFC] = 500.0 / opt->t_stop;
+        }
         if (p[SRCP_WAVE] == 2.0 && p[SRCP_SFREQ] == 0) p[SRCP_SFREQ] = 1.0 / opt->t_stop;
         if (p[SRCP_WAVE] == 3.0) { /* builder.zig:862-867 */
             if (p[SRCP_ETD1] == 0) p[SRCP_ETD1] = opt->dt_init;

@@ -106,7 +106,7 @@ void hl_prepare(void *hv)
     HL *h = (HL *)hv;
     const size_t N = h->N;
     for (int kind = 0; kind < K_NKIND; kind++) {
-        double u[64], p[64];
+        double u[PARAM_MAX], p[PARAM_MAX];
         for (uint32_t i = 0; i < h->t->count[kind]; i++)
             for (size_t l = 0; l < N; l++) {
                 for (int k = 0; k < KIND_UK[kind]; k++) u[k] = h->uprm[kind][((size_t)i * KIND_UK[kind] + k) * N + l];

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 VSRC, ISRC, RES, CAP, DIODE, MOS1 = range(6)
-UK = {VSRC: 56, ISRC: 56, RES: 1, CAP: 1, DIODE: 19, MOS1: 36}
+UK = {VSRC: 71, ISRC: 71, RES: 1, CAP: 1, DIODE: 19, MOS1: 36}
 
 # user parameter slots (oracle/oracle_circuit.h)
 VSRC_DC = 0

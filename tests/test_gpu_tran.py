@@ -369,6 +369,41 @@ def test_isource_waveform_lanes_bitwise(oracle, engine):
     assert (res["status"] == 0).all()
 
 
+def test_sffm_am_lanes_bitwise(oracle, engine):
+    """SFFM (FM and FC from the stop time when omitted) and AM voltage
+    sources into an RC (models/vsource.va:304-330), their frequencies,
+    indices and delays varied per lane; every row bitwise the oracle's."""
+    N = 16
+    say(f"case: SFFM + AM sources into RC, {N} lanes, tran 10n 4u")
+
+    def deck(add):
+        add(E.VSOURCE, ["f", "0"], {0: np.nan, 1: 5.0, 56: 0.1, 57: 1.0, 59: 2.0, 61: 2e-7})
+        add(E.VSOURCE, ["a", "0"], {0: np.nan, 1: 6.0, 64: 1.5, 65: 0.5, 66: 1e6, 67: 5e6, 68: 5e-7})
+        add(E.CAPACITOR, ["out", "0"], {0: 1e-10})
+        add(E.RESISTOR, ["f", "out"], {0: 1e3})
+        add(E.RESISTOR, ["a", "out"], {0: 2e3})
+
+    t, c = build_both(oracle, deck)
+    rng = np.random.default_rng(37)
+    sets = {(E.VSOURCE, 0, 58): np.where(np.arange(N) % 2 == 0, -1.0, rng.uniform(2e5, 2e6, N)),
+            (E.VSOURCE, 0, 59): rng.uniform(0.0, 8.0, N),
+            (E.VSOURCE, 1, 66): rng.uniform(5e5, 3e6, N), (E.VSOURCE, 1, 68): rng.uniform(0.0, 1e-6, N)}
+    passes = [E.meas_extremum(t.node("out"), 1), E.meas_integ(t.node("out"), rms=True)]
+    res, _, _, _ = run_pair(oracle, t, c, sets, dict(t_stop=4e-6, dt_init=1e-8), passes)
+    assert (res["status"] == 0).all()
+
+
+def test_isource_sffm_is_refused(engine):
+    say("case: SFFM on a current source is refused at prepare")
+    t = E.Template()
+    t.add(E.ISOURCE, ["0", "a"], {0: np.nan, 1: 5.0, 56: 0.0, 57: 1e-3})
+    t.add(E.RESISTOR, ["a", "0"], {0: 1e3})
+    t.freeze()
+    gl = E.Lanes(t, 2)
+    with pytest.raises(E.EngineError, match="IsourceSffmUnsupported"):
+        gl.prepare()
+
+
 def test_diode_rs_topology_is_per_template(engine):
     say("case: a lane may not move a diode's RS across zero")
     t = E.Template()

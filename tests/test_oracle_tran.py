@@ -380,3 +380,34 @@ def test_isource_waveforms(oracle):
     e1 = np.where(t > m, 1 - np.exp(-(t - m) / (0.5 * m)), 0.0)
     e2 = np.where(t > 3 * m, -(1 - np.exp(-(t - 3 * m) / (0.5 * m))), 0.0)
     assert np.allclose(x[:, c.node("e")], e1 + e2, rtol=0, atol=1e-12)
+
+
+def test_sffm_am_waveforms(oracle):
+    """SFFM and AM voltage sources (models/vsource.va:304-330) into 1k
+    loads, against the models' expressions in numpy at every accepted time:
+    SFFM is 0 until TD, its index clamped to FC/FM, FM and FC filled from
+    the .tran stop time when omitted (frontend/builder.zig:854-857, here
+    FC = 500/TSTOP = 100 kHz); AM is 0 before TD. `.tran 1u 5m`."""
+    from egspice_amd import spice_number as sn
+    m = sn("1m")
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["f", 0], {0: np.nan, 1: 5.0, 56: 0.2, 57: 1.0, 58: sn("2k"), 59: 3.0, 61: 0.5 * m, 62: 30.0,
+                             63: 10.0})
+    c.add(O.VSRC, ["a", 0], {0: np.nan, 1: 6.0, 64: 2.0, 65: 0.5, 66: sn("1k"), 67: sn("10k"), 68: m, 69: 20.0,
+                             70: 45.0})
+    for nd in "fa":
+        c.add(O.RES, [nd, 0], {0: sn("1k")})
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(5 * m, sn("1u")))
+    assert st.status == 0 and len(t) > 5000
+    fc, fm, mdi = 500.0 / (5 * m), 2e3, 3.0
+    tf = t - 0.5 * m
+    f = np.where(tf <= 0, 0.0, 0.2 + np.sin((2 * np.pi * fc * tf + 10.0 * np.pi / 180)
+                                            + mdi * np.sin(2 * np.pi * fm * tf + 30.0 * np.pi / 180)))
+    assert np.allclose(x[:, c.node("f")], f, rtol=0, atol=1e-9)
+    a = np.where(t >= m, 2.0 * (0.5 + np.sin(2 * np.pi * 1e3 * t + 45.0 * np.pi / 180))
+                 * np.sin(2 * np.pi * 1e4 * t + 20.0 * np.pi / 180), 0.0)
+    assert np.allclose(x[:, c.node("a")], a, rtol=0, atol=1e-9)
+    assert np.all(x[t < 0.5 * m, c.node("f")] == 0.0)
