@@ -697,6 +697,19 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
                 if (!to_value(t[c.i + 2], dst, err)) return false;
                 c.i += 3;
             }
+            // a numeric r= must be one of the table's times before its last
+            // (frontend/builder.zig:1580-1594, ngspice vsrcpar.c:160-176)
+            if (e.pwl_r.given && !e.pwl_r.expr) {
+                bool on_point = false;
+                for (size_t k = 0; k + 1 < e.pwl.size(); k += 2)
+                    on_point = on_point || (!e.pwl[k].expr && e.pwl[k].num == e.pwl_r.num);
+                const Value &last = e.pwl[e.pwl.size() - 2];
+                const double t_last = last.expr ? e.pwl_r.num : last.num;
+                if (!on_point || e.pwl_r.num >= t_last) {
+                    err = "InvalidParameterValue " + e.name + " PWL r= is not a time point before the last";
+                    return false;
+                }
+            }
         } else if (w == "sin" || w == "pwl" || w == "exp" || w == "sffm" || w == "am" || w == "pulse") {
             err = "UnsupportedWaveform " + w;
             return false;

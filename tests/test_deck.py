@@ -61,6 +61,8 @@ def test_isource_waveforms_read_and_build(engine):
     ("I2 a 0 SFFM(0 1 1k 2 10k)", "UnsupportedWaveform"),
     ("V2 a 0 SFFM(0 1 1k 2 10k 0 0 0 1)", "BadSffm"),
     ("V2 a 0 AM(1 0 1k 10k 0 0 0 1)", "BadAm"),
+    ("V2 a 0 PWL(0 0 1u 1 2u 0) r=1.5u", "InvalidParameterValue"),
+    ("V2 a 0 PWL(0 0 1u 1 2u 0) r=2u", "InvalidParameterValue"),
     ("V2 a 0 EXP(0 1 1n 1n 2n 1n 7)", "BadExp"),
     ("V2 a 0 SIN(0 1 1k 0 0 0 9)", "BadSin"),
     (".subckt sub a b", "UnsupportedCard"),
