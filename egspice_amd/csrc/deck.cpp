@@ -10,6 +10,11 @@
 
 #include "../../include/espice_lanes.h"
 
+namespace {
+// The engine's PWL table (devices.h PWL_CAP; models/vsource.va holds 64).
+constexpr size_t PWL_POINTS = 16;
+}  // namespace
+
 namespace esp {
 
 // ------------------------------------------------------------- numbers
@@ -640,7 +645,10 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
             // positional arguments fill the waveform's slots in order; the
             // ones not written keep the model defaults
             // (frontend/builder.zig:2841-2871, 2985-2990)
-            const int most = w == "pulse" ? 7 : w == "sffm" ? 8 : w == "am" ? 7 : 6;
+            // PULSE's eighth argument is ngspice's pulse count NP and AM
+            // reads ngspice's eight (VO VMO VMA FM FC TD PHASEM PHASEC); the
+            // build maps both (frontend/builder.zig:1478-1562)
+            const int most = w == "pulse" ? 8 : w == "sffm" ? 8 : w == "am" ? 8 : 6;
             const std::string bad = (w == "pulse" ? "BadPulse " : w == "sin" ? "BadSin " : w == "exp" ? "BadExp "
                                      : w == "sffm" ? "BadSffm " : "BadAm ") + e.name;
             c.i++;
@@ -658,12 +666,15 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
                 k++;
                 c.i++;
             }
-            if (!c.next_is(")") || k < 2) {
+            // trailing arguments may be left off (their slots keep the
+            // model defaults); AM needs two to be read in ngspice's order
+            if (!c.next_is(")") || k < (w == "am" ? 2 : 1)) {
                 err = bad;
                 return false;
             }
             c.i++;
             e.wave = w == "pulse" ? 1 : w == "sin" ? 2 : w == "exp" ? 3 : w == "sffm" ? 5 : 6;
+            e.n_wave_args = k;
         } else if (w == "pwl" && (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // (time, value) pairs (frontend/builder.zig:2848-2859), then
             // `r=` and `td=` on the card
@@ -686,7 +697,7 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
                 err = "BadPwl " + e.name;
                 return false;
             }
-            if (e.pwl.size() / 2 > 16) {
+            if (e.pwl.size() / 2 > PWL_POINTS) {
                 err = "PwlTooLong " + e.name;
                 return false;
             }
@@ -983,6 +994,64 @@ Value lit(double x)
 
 }  // namespace
 
+// PULSE with ngspice's pulse count NP (the eighth argument,
+// frontend/builder.zig:1505-1545 pulseTrain): false, and the card keeps its
+// first seven arguments, when the count never ends the train before the
+// .tran stop; else `prm` gets a PWL of NP pulses' corners and then V1 (true),
+// or err is set (a table past the engine's PWL capacity, arguments that are
+// not numbers).
+bool pulse_train(const Deck &d, const Element &e, std::vector<std::pair<uint32_t, Value>> &prm, std::string &err)
+{
+    double a[8];
+    for (int q = 0; q < 8; q++) {
+        if (e.wave_args[q].expr || !d.tran || d.tstop.expr || d.tstep.expr) {
+            err = "UnresolvedParameter " + e.name + " a PULSE count needs numbers and a numeric .tran";
+            return false;
+        }
+        a[q] = e.wave_args[q].num;
+    }
+    const double tstep = d.tstep.num, tstop = d.tstop.num;
+    const double v1 = a[0], v2 = a[1], td = a[2];
+    const double tr = a[3] != 0 ? a[3] : tstep, tf = a[4] != 0 ? a[4] : tstep;
+    const double pw = a[5] != 0 ? a[5] : tstop;
+    const double per = std::max(a[6] != 0 ? a[6] : tstop, tr + pw + tf);
+    const double end = td + a[7] * per;
+    if (!(a[7] > 0) || end >= tstop) return false;
+    const double pulses = std::ceil(a[7]);
+    std::vector<double> pts{td, v1};
+    double t_prev = td, v_prev = v1;
+    bool cut = false;
+    for (double k = 0; k < pulses && !cut && pts.size() <= 2 * 64; k += 1) {
+        const double base = td + k * per;
+        const double corners[4][2] = {{base + tr, v2}, {base + tr + pw, v2}, {base + tr + pw + tf, v1}, {base + per, v1}};
+        for (auto &c : corners) {
+            if (c[0] == t_prev) continue;
+            if (c[0] > end) {
+                const double v_end = v_prev + (c[1] - v_prev) * (end - t_prev) / (c[0] - t_prev);
+                pts.insert(pts.end(), {end, v_end, end, v1});
+                cut = true;
+                break;
+            }
+            pts.insert(pts.end(), {c[0], c[1]});
+            t_prev = c[0];
+            v_prev = c[1];
+        }
+    }
+    const size_t np = pts.size() / 2;
+    if (np > PWL_POINTS) {
+        err = "PwlTooLong " + e.name + " a PULSE count past the engine's PWL table";
+        return false;
+    }
+    prm.emplace_back(1, lit(4.0));
+    prm.emplace_back(15, lit((double)np));
+    for (size_t k = 0; k < np; k++) {
+        prm.emplace_back((uint32_t)(18 + k), lit(pts[2 * k]));
+        prm.emplace_back((uint32_t)(34 + k), lit(pts[2 * k + 1]));
+    }
+    if (e.value.given) prm.emplace_back(0, e.value);
+    return true;
+}
+
 bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
 {
     Deck &d = bd.deck;
@@ -1032,7 +1101,45 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
                     prm.emplace_back((uint32_t)(34 + k), e.pwl[2 * k + 1]);
                 }
                 if (e.value.given) prm.emplace_back(0, e.value);
+            } else if (e.wave == 6) {
+                // ngspice's AM(VO VMO VMA FM FC TD PHASEM PHASEC), value
+                // VO + (VMO + VMA sin(2 pi FM t' + PHASEM)) sin(2 pi FC t' +
+                // PHASEC) on t' = t - TD, onto the models' AM(VA VO MF FC TD
+                // PHASEC PHASES) (frontend/builder.zig:1547-1562 amArgs): the
+                // delay moves into the phases; VMA = 0 holds the envelope at
+                // sin(90 deg); FM and FC default to 5/TSTOP and 500/TSTOP; a
+                // nonzero VO has no slot and is refused
+                double a[8] = {0, 0, 1, 0, 0, 0, 0, 0};
+                const bool tstop_num = d.tran && d.tstop.given && !d.tstop.expr;
+                const double tstop = tstop_num ? d.tstop.num : 0.0;
+                a[3] = 5.0 / tstop;
+                a[4] = 500.0 / tstop;
+                for (int q = 0; q < 8 && q < e.n_wave_args; q++) {
+                    if (!e.wave_args[q].given) continue;
+                    if (e.wave_args[q].expr) {
+                        err = "UnresolvedParameter " + e.name + " AM arguments must be numbers";
+                        return false;
+                    }
+                    a[q] = e.wave_args[q].num;
+                }
+                if (!tstop_num && (e.n_wave_args < 5)) {
+                    err = "UnresolvedParameter " + e.name + " AM's FM and FC defaults need a numeric .tran stop";
+                    return false;
+                }
+                const double vo = a[0], vmo = a[1], vma = a[2], fm = a[3], fc = a[4], td = a[5], phm = a[6], phc = a[7];
+                if (vo != 0.0) {
+                    err = "UnsupportedCard " + e.name + " AM with a nonzero VO offset";
+                    return false;
+                }
+                const double ours[7] = {vma != 0.0 ? vma : vmo, vma != 0.0 ? vmo / vma : 0.0, vma != 0.0 ? fm : 0.0, fc,
+                                        td, phc - 360.0 * fc * td, vma != 0.0 ? phm - 360.0 * fm * td : 90.0};
+                prm.emplace_back(1, lit(6.0));
+                for (int q = 0; q < 7; q++) prm.emplace_back((uint32_t)(64 + q), lit(ours[q]));
+                if (e.value.given) prm.emplace_back(0, e.value);
+            } else if (e.wave == 1 && e.n_wave_args == 8 && pulse_train(d, e, prm, err)) {
+                // NP pulses within the run: a PWL of their corners
             } else if (e.wave) {
+                if (!err.empty()) return false;
                 // PULSE args go to slots 2..8, SIN args to 9..14, EXP args
                 // to 50..55, SFFM's to 56..63, AM's to 64..70; a dc left unwritten is filled from the
                 // waveform's t = 0 value when the lane constants are
@@ -1045,7 +1152,7 @@ bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)
                     if (!e.wave_args[3].given) prm.emplace_back(53, lit(0.0));
                     if (!e.wave_args[5].given) prm.emplace_back(55, lit(0.0));
                 }
-                for (int a = 0; a < 8; a++)
+                for (int a = 0; a < (e.wave == 1 ? 7 : 8); a++)
                     if (e.wave_args[a].given) prm.emplace_back(base + a, e.wave_args[a]);
                 if (e.value.given) prm.emplace_back(0, e.value);
             } else {

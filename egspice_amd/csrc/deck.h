@@ -66,6 +66,7 @@ struct Element {
                                       // 5 SFFM(vo va fm mdi fc td phasem phasec), V only,
                                       // 6 AM(va vo mf fc td phasec phases)
     Value wave_args[8];
+    int n_wave_args = 0;              // arguments the card gave
     std::vector<Value> pwl;           // PWL times and values, interleaved
     Value pwl_r, pwl_td;              // PWL r= and td=
     std::map<std::string, Value> kv;  // W=, L=, AD=, M=, area

@@ -1047,11 +1047,21 @@ espice_status espice_lanes_newton_learned(espice_lanes *L, espice_mlp *m, const 
 {
     if (!L || !m || !params || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    // every check launch_newton makes comes before the inference is queued:
+    // the inference reads the network's mapped input buffer in place, so no
+    // error may return while it could still run
+    if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
     hipEvent_t w0 = L->ev_w0, w1 = L->ev_w1;
     HIP_TRY(hipEventRecord(w0, L->stream));
-    if (espice_status e = mlp_warm_start(m, params, L->N, L->x.p, L->t->n, L->N, L->stream)) return e;
+    if (espice_status e = mlp_warm_start(m, params, L->N, L->x.p, L->t->n, L->N, L->stream)) {
+        (void)hipStreamSynchronize(L->stream);
+        return e;
+    }
     HIP_TRY(hipEventRecord(w1, L->stream));
-    if (espice_status e = launch_newton(L, opts, x != nullptr)) return e;
+    if (espice_status e = launch_newton(L, opts, x != nullptr)) {
+        (void)hipStreamSynchronize(L->stream);
+        return e;
+    }
     if (espice_status e = finish_newton(L, out, x, newton_ms)) return e;
     if (warm_ms) {
         float ms = 0;

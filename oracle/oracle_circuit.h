@@ -96,6 +96,7 @@ typedef struct {
 
 /* Records every accepted row: times[k] and rows[k*n + i] (malloc'd, the
  * caller frees them). Returns the row count. */
+void or_tran_coeffs(int method, double dt, double dt_prev, double xmu, double *out);
 int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *o, double **times, double **rows, or_tran_stats *st);
 
 /* .meas passes over a recorded waveform (meas.c). A clause's counts use

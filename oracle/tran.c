@@ -55,6 +55,16 @@ static tr_coeffs coeffs(int method, double dt, double dt_prev, double xmu)
     return c;
 }
 
+/* The coefficients, exported for the known-value pins of
+ * src/analysis/tests/transient.zig:605-636 (ag0, ag1, ag2 into out). */
+void or_tran_coeffs(int method, double dt, double dt_prev, double xmu, double *out)
+{
+    const tr_coeffs c = coeffs(method, dt, dt_prev, xmu);
+    out[0] = c.ag0;
+    out[1] = c.ag1;
+    out[2] = c.ag2;
+}
+
 /* tran.zig methodAt :1298: order 1 is BE, else the top method. */
 static int method_at(int top, int k) { return k == 1 ? 0 : top; }
 

@@ -119,6 +119,7 @@ def load():
         "or_eval_lane": (None, [P, C.c_uint32, f64p, f64p, f64p]),
         "or_dc_sweep": (C.c_int, [P, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.POINTER(NewtonOpts), f64p, u32p]),
         "or_fetlim": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "or_tran_coeffs": (None, [C.c_int, C.c_double, C.c_double, C.c_double, f64p]),
         "or_tran_run": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), C.POINTER(f64p), C.POINTER(f64p),
                                   C.POINTER(TranStats)]),
         "or_meas_cross": (C.c_double, [f64p, f64p, C.c_uint32, C.POINTER(MeasClause)]),

@@ -60,7 +60,11 @@ def test_isource_waveforms_read_and_build(engine):
     ("X1 in out sub", "UnsupportedElement"),
     ("I2 a 0 SFFM(0 1 1k 2 10k)", "UnsupportedWaveform"),
     ("V2 a 0 SFFM(0 1 1k 2 10k 0 0 0 1)", "BadSffm"),
-    ("V2 a 0 AM(1 0 1k 10k 0 0 0 1)", "BadAm"),
+    ("V2 a 0 AM(1 0 1k 10k 0 0 0 1)", "UnsupportedCard"),
+    ("V2 a 0 AM(0 1 0.5 1k 10k 0 0 0 1)", "BadAm"),
+    ("V2 a 0 AM(0 {r} 0.5 1k 10k)", "UnresolvedParameter"),
+    ("V2 a 0 PULSE(0 1 1u 1n 1n 0.5u 1.5u 5)", "PwlTooLong"),
+    ("V2 a 0 PULSE(0 1 1u 1n 1n 1u 2u 3 1)", "BadPulse"),
     ("V2 a 0 PWL(0 0 1u 1 2u 0) r=1.5u", "InvalidParameterValue"),
     ("V2 a 0 PWL(0 0 1u 1 2u 0) r=2u", "InvalidParameterValue"),
     ("V2 a 0 EXP(0 1 1n 1n 2n 1n 7)", "BadExp"),

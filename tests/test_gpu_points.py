@@ -350,3 +350,47 @@ def test_monte_tran_statistics(engine):
     # the nominal point: R C ln 2 within the step control's resolution
     m0, r0 = pb.measures(0, 1, ["tc"])
     assert r0.tolist() == [0] and abs(m0[0, 0] - 1e3 * 1e-9 * math.log(2)) < 1e-9
+
+
+AM_SFFM_SP = """AM and SFFM in ngspice's argument order
+VA a 0 AM(0 1 0.5 1k 10k 0.1m 30 45)
+VZ z 0 AM(0 0.8 0 1k 10k 0.2m 0 60)
+VF f 0 SFFM(0.2 1 2k 3 10k 0.5m 30 10)
+RA a 0 1k
+RZ z 0 1k
+RF f 0 1k
+.tran 0.2u 2m
+{meas}
+.end
+"""
+
+
+def test_am_sffm_cards_follow_ngspice(engine):
+    """AM(VO VMO VMA FM FC TD PHASEM PHASEC) read in ngspice's order has
+    ngspice's value VO + (VMO + VMA sin(2 pi FM t' + PHASEM)) sin(2 pi FC t'
+    + PHASEC), t' = t - TD (vsrcload.c as restated by
+    frontend/builder.zig:1547-1562 amArgs); VMA = 0 holds the envelope at
+    VMO; SFFM(VO VA FM MDI FC TD PHASEM PHASEC) is VO + VA sin(2 pi FC t' +
+    PHASEC + MDI sin(2 pi FM t' + PHASEM)) after TD (models/vsource.va:
+    304-317). FIND AT interpolates between accepted points (dt <= 0.2u), so
+    the tolerance is the interpolation's, far below any slot mix-up."""
+    say("case: AM and SFFM deck cards in ngspice order, FIND AT against the textbook forms")
+    at = [0.35e-3, 0.61e-3, 0.9e-3, 1.27e-3, 1.73e-3]
+    meas = "\n".join(f".meas tran {nd}{k} FIND v({nd}) AT={t!r}" for nd in "azf" for k, t in enumerate(at))
+    pb = E.Problem(AM_SFFM_SP.format(meas=meas))
+    names = [f"{nd}{k}" for nd in "azf" for k in range(len(at))]
+    pb.run_all()
+    m, reasons = pb.measures(0, 1, names)
+    assert reasons.tolist() == [0]
+    d2r = math.pi / 180
+    for k, t in enumerate(at):
+        tp = t - 0.1e-3
+        a = (1 + 0.5 * math.sin(2 * math.pi * 1e3 * tp + 30 * d2r)) * math.sin(2 * math.pi * 1e4 * tp + 45 * d2r)
+        tz = t - 0.2e-3
+        z = 0.8 * math.sin(2 * math.pi * 1e4 * tz + 60 * d2r)
+        tf = t - 0.5e-3
+        f = 0.0 if tf <= 0 else 0.2 + math.sin(2 * math.pi * 1e4 * tf + 10 * d2r
+                                               + 3 * math.sin(2 * math.pi * 2e3 * tf + 30 * d2r))
+        for nd, want in (("a", a), ("z", z), ("f", f)):
+            got = m[0, names.index(f"{nd}{k}")]
+            assert abs(got - want) < 2e-3, (nd, t, got, want)

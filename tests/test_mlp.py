@@ -97,7 +97,9 @@ def test_mlp_forward_matches_oracle(engine):
     scale = y.std(0)
     err = np.abs(got - want) / scale
     say(f"  max |gpu - oracle| / std(y) = {err.max():.2e}")
-    assert err.max() < 2e-2
+    # FP32 MFMA sums against the restatement's FP64 sums of the same BF16
+    # operands: measured 4.4e-4 of std(y) at most
+    assert err.max() < 6e-4
     assert np.all(np.isfinite(got))
 
 
@@ -117,7 +119,8 @@ def test_mlp_gradients_match_oracle(engine):
             a, b = g[at:at + size], want[at:at + size]
             rel = np.linalg.norm(a - b) / np.linalg.norm(b)
             say(f"  layer {k} {part}: relative error {rel:.2e}")
-            assert rel < 2e-2, (k, part, rel)
+            # measured 1.2e-3 at most (FP32 against FP64 accumulation)
+            assert rel < 2.5e-3, (k, part, rel)
             at += size
     assert at == g.size
 

@@ -103,10 +103,46 @@ def test_rc_lowpass_points(oracle):
     assert st.rej_lte >= 1  # the run goes through an LTE rejection
 
 
-@pytest.mark.parametrize("method", [0, 1])
+def coeffs(oracle, method, dt, dt_prev, xmu=0.5):
+    out = np.zeros(3)
+    oracle.or_tran_coeffs(method, dt, dt_prev, xmu, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def test_integrator_coeffs_known_values(oracle):
+    """The reference's known values of the integrator coefficients
+    (src/analysis/tests/transient.zig:605-636): BE 1/dt; the trapezoid at
+    xmu = 0.5 is 2/dt with a unit history weight, at xmu = 0 backward Euler;
+    gear-2 at r = 1 is the uniform BDF2 triple (1.5/dt, 0.5/dt), and on a
+    non-uniform grid (r in 0.25..4) it is exact on constants, on the ramp
+    q(t) = t (dq/dt = 1) and on t^2 (derivative 0 at t_n)."""
+    dt = 1e-9
+    assert abs(coeffs(oracle, 0, dt, dt)[0] - 1e9) <= 1e-12 * 1e9
+    for h in (1e-9, 3.7e-13, 0.1):
+        c = coeffs(oracle, 1, h, h)
+        assert c[0] == 2.0 / h and c[1] == 1.0
+    assert coeffs(oracle, 1, dt, dt, 0.0)[1] == 0.0
+    u = coeffs(oracle, 2, dt, dt)
+    assert abs(u[0] - 1.5e9) <= 1e-12 * 1.5e9 and abs(u[2] - 0.5e9) <= 1e-12 * 0.5e9
+    for r in (0.25, 0.5, 1.0, 2.0, 4.0):
+        dt1 = dt / r
+        c = coeffs(oracle, 2, dt, dt1)
+        ag1 = -(c[0] + c[2])
+        dqdt = c[0] * 0.0 + ag1 * (-dt) + c[2] * (-(dt + dt1))
+        assert abs(dqdt - 1.0) <= 1e-12
+        sq = c[0] * 0.0 + ag1 * (dt * dt) + c[2] * ((dt + dt1) * (dt + dt1))
+        assert abs(sq) <= 1e-22
+    # a plausible slip, the ratio inverted, fails the ramp: pin that the
+    # check can see it
+    r = 2.0
+    wrong_ag0 = (1.0 + 2.0 / r) / ((1.0 + 1.0 / r) * dt)
+    assert abs(wrong_ag0 - coeffs(oracle, 2, dt, dt / r)[0]) > 1e6
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
 def test_rc_step_closed_form(oracle, method):
-    """After the edge the output follows 1 - exp(-(t - t0)/RC); BE and the
-    trapezoid stay within a few reltol of it."""
+    """After the edge the output follows 1 - exp(-(t - t0)/RC); BE, the
+    trapezoid and gear-2 stay within a few reltol of it."""
     c = rc_deck(oracle, (0, 1, 1e-6, 1e-9, 1e-9, 5e-6, 20e-6), 1e3, 1e-9)
     L = O.Lanes(c, 1)
     L.prepare()
@@ -116,6 +152,10 @@ def test_rc_step_closed_form(oracle, method):
     sel = (t > 1.2e-6) & (t < 5.9e-6)
     exact = 1 - np.exp(-(t[sel] - 1.0005e-6) / 1e-6)
     assert np.max(np.abs(vout[sel] - exact)) < (2e-3 if method == 0 else 5e-4)
+    if method == 2:
+        # gear-2 is second order: its error sits with the trapezoid's, well
+        # under backward Euler's
+        assert np.max(np.abs(vout[sel] - exact)) < 5e-4
     assert np.all(x[:, c.node("in")][t <= 1e-6] == 0)
 
 
@@ -411,3 +451,43 @@ def test_sffm_am_waveforms(oracle):
                  * np.sin(2 * np.pi * 1e4 * t + 20.0 * np.pi / 180), 0.0)
     assert np.allclose(x[:, c.node("a")], a, rtol=0, atol=1e-9)
     assert np.all(x[t < 0.5 * m, c.node("f")] == 0.0)
+
+
+def test_sffm_am_reduce_to_sin(oracle):
+    """SFFM and AM pinned through the cases where they reduce to the SIN
+    waveform (itself pinned by the reference's printed rectifier and
+    sources runs): SFFM with MDI = 0 is VO + VA sin(2 pi FC t' + PHASEC)
+    after TD; AM with MF = 0 and PHASES = 90 has the constant envelope VO + 1,
+    so it is the sine of amplitude VA (VO + 1) at FC; and SFFM's index is
+    clamped to FC/FM, so MDI above the clamp equals MDI at it. Every pair
+    drives its own 1k load; the nodes agree at every accepted time."""
+    from egspice_amd import spice_number as sn
+    m = sn("1m")
+    c = O.Circuit(oracle)
+    # SFFM(VO VA FM MDI FC TD PHASEM PHASEC) slots 56..63; SIN(VO VA FREQ TD
+    # THETA PHASE) slots 9..14; AM(VA VO MF FC TD PHASEC PHASES) slots 64..70
+    c.add(O.VSRC, ["f0", 0], {0: 0.2, 1: 5.0, 56: 0.2, 57: 1.5, 58: sn("2k"), 59: 0.0, 60: sn("10k"), 61: 0.5 * m,
+                              62: 30.0, 63: 10.0})
+    c.add(O.VSRC, ["s0", 0], {0: 0.2, 1: 2.0, 9: 0.2, 10: 1.5, 11: sn("10k"), 12: 0.5 * m, 13: 0.0, 14: 10.0})
+    c.add(O.VSRC, ["a0", 0], {0: 0.0, 1: 6.0, 64: 2.0, 65: 0.5, 66: 0.0, 67: sn("10k"), 68: 0.0, 69: 20.0, 70: 90.0})
+    c.add(O.VSRC, ["s1", 0], {0: 0.0, 1: 2.0, 9: 0.0, 10: 2.0 * (0.5 + 1.0), 11: sn("10k"), 12: 0.0, 13: 0.0,
+                              14: 20.0})
+    c.add(O.VSRC, ["fa", 0], {0: 0.0, 1: 5.0, 56: 0.0, 57: 1.0, 58: sn("2k"), 59: 50.0, 60: sn("10k"), 61: 0.0,
+                              62: 30.0, 63: 10.0})
+    c.add(O.VSRC, ["fb", 0], {0: 0.0, 1: 5.0, 56: 0.0, 57: 1.0, 58: sn("2k"), 59: 5.0, 60: sn("10k"), 61: 0.0,
+                              62: 30.0, 63: 10.0})
+    for nd in ("f0", "s0", "a0", "s1", "fa", "fb"):
+        c.add(O.RES, [nd, 0], {0: sn("1k")})
+    c.freeze()
+    L = O.Lanes(c, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(2 * m, sn("1u")))
+    assert st.status == 0 and len(t) > 1000
+    after = t > 0.5 * m
+    v = lambda nd: x[:, c.node(nd)]
+    assert np.allclose(v("f0")[after], v("s0")[after], rtol=0, atol=1e-12)
+    assert np.all(v("f0")[t <= 0.5 * m] == 0.0)
+    assert np.allclose(v("a0"), v("s1"), rtol=0, atol=1e-12)
+    assert np.allclose(v("fa"), v("fb"), rtol=0, atol=1e-12)
+    # and none of the reductions is vacuous
+    assert np.ptp(v("f0")) > 2.5 and np.ptp(v("a0")) > 5.0 and np.ptp(v("fa")) > 1.5
