@@ -13,9 +13,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libespice_lanes.so")
 
-VSOURCE, ISOURCE, RESISTOR, CAPACITOR, DIODE, MOS1 = range(6)
-# user parameter counts per kind (include/espice_lanes.h kinds)
+VSOURCE, ISOURCE, RESISTOR, CAPACITOR, DIODE, MOS1, PSP103 = range(7)
+# user parameter counts per kind (include/espice_lanes.h kinds); a compiled
+# kind's count comes from the library (param_count)
 PARAM_COUNT = {VSOURCE: 71, ISOURCE: 71, RESISTOR: 1, CAPACITOR: 1, DIODE: 19, MOS1: 36}
+COMPILED_KINDS = (PSP103,)
 VSOURCE_DC = 0
 DIODE_PARAMS = dict(IS=0, N=1, CJO=2, VJ=3, M=4, FC=5, TT=6, GMIN=7, AREA=8, TNOM=9, TEMP=10, DTEMP=11, EG=12, XTI=13,
                     RS=14, TRS=15, TRS2=16, TM1=17, TM2=18)
@@ -33,7 +35,7 @@ EXPORTS = (
     "espice_lanes_mc_draw", "espice_mlp_create", "espice_mlp_destroy", "espice_mlp_param_count",
     "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
     "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
-    "espice_lanes_first_factors",
+    "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
 )
 
 
@@ -117,6 +119,8 @@ def library():
         "espice_lanes_destroy": [P],
         "espice_lanes_reset": [P],
         "espice_lanes_set_param": [P, C.c_uint32, C.c_uint32, C.c_uint32, f64p],
+        "espice_lanes_param_count": [C.c_uint32, C.POINTER(C.c_uint32)],
+        "espice_lanes_param_index": [C.c_uint32, C.c_char_p, C.POINTER(C.c_uint32)],
         "espice_lanes_prepare": [P],
         "espice_lanes_upload_x": [P, f64p],
         "espice_lanes_download_x": [P, f64p],
@@ -205,6 +209,28 @@ def meas_integ(row, rms=False, frm=0.0, to=0.0):
     return MeasPass(MEAS_INTEG, row, -1, -1, -1, 1 if rms else 0, 0.0, 0.0, frm, to, 0.0)
 
 
+def param_count(kind):
+    """User parameters of a kind (a compiled kind: its module's)."""
+    if kind in PARAM_COUNT:
+        return PARAM_COUNT[kind]
+    c = C.c_uint32()
+    _check(library().espice_lanes_param_count(kind, C.byref(c)))
+    PARAM_COUNT[kind] = c.value
+    return c.value
+
+
+def param_index(kind, name):
+    """The slot of a named parameter of a compiled kind (aliases resolved)."""
+    i = C.c_uint32()
+    _check(library().espice_lanes_param_index(kind, name.encode(), C.byref(i)))
+    return i.value
+
+
+def named_params(kind, **values):
+    """{slot: value} of named parameters of a compiled kind."""
+    return {param_index(kind, k): v for k, v in values.items()}
+
+
 class Template:
     """A circuit frozen into one MNA pattern, shared by every lane."""
 
@@ -231,9 +257,9 @@ class Template:
 
     def add(self, kind, ports, params=None):
         rows = [self.node(p) if isinstance(p, str) else p for p in ports]
-        if kind not in PARAM_COUNT:
+        if kind not in PARAM_COUNT and kind not in COMPILED_KINDS:
             raise EngineError(f"unknown device kind {kind}")
-        pr = np.full(PARAM_COUNT[kind], np.nan)
+        pr = np.full(param_count(kind), np.nan)
         for k, v in (params or {}).items():
             if not 0 <= k < len(pr):
                 raise EngineError(f"parameter {k} out of range for kind {kind} ({len(pr)} parameters)")

@@ -16,10 +16,13 @@
 #pragma once
 
 #include "dual.h"
+#include "va_models.h"
 
 namespace esp {
 
-enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_NKIND = 6 };
+// K_PSP103 and the kinds after it are compiled from Verilog-A (va_models.h).
+enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_PSP103 = 6, K_NKIND = 7 };
+constexpr bool kind_is_va(int kind) { return kind >= K_PSP103; }
 
 // User parameter slots (NaN = not given).
 // waveform: 0 DC, 1 PULSE(V1 V2 TD TR TF PW PER), 2 SIN(VO VA FREQ TD THETA
@@ -67,22 +70,25 @@ enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT,
 // Local unknowns, ports, user and prepared parameter counts per kind.
 // The diode's unknowns are (a, c, ai): ai, the junction's anode behind RS,
 // is the row of a itself when RS is 0 (diode.va:10-11, the collapse).
-constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4};
-constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4};
-constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K};
-constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
-// room for one instance's user or prepared parameters on the host
+constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4, PSP103_NU};
+constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4, PSP103_NPORT};
+constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K, PSP103_NPARAM};
+constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K, PSP103_NPREP};
+// room for one instance's user or prepared parameters on the host (the
+// hand-written kinds; a compiled kind sizes its own)
 constexpr int PARAM_MAX = 128;
 static_assert(VSRC_K <= PARAM_MAX && SRCP_K <= PARAM_MAX && DIO_K <= PARAM_MAX && DIOP_K <= PARAM_MAX &&
                   M1_K <= PARAM_MAX && M1P_K <= PARAM_MAX,
               "every kind's parameters fit PARAM_MAX");
 // Residual rows a kind writes in a static evaluation.
-constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
+constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu, (1u << PSP103_NU) - 1u};
 // Unknowns a kind's limiter writes (its limited image).
-constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x4u, 0xbu};
+constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x4u, 0xbu, 0u};
 // The local row whose charge is the kind's LTE site (one ddt per capacitor
-// and per diode), or -1.
-constexpr int KIND_TAPE_ROW[K_NKIND] = {-1, -1, -1, 0, 2, -1};
+// and per diode), or -1; a compiled kind has one site per ddt() instead.
+constexpr int KIND_TAPE_ROW[K_NKIND] = {-1, -1, -1, 0, 2, -1, -1};
+// LTE charge sites per instance.
+constexpr int KIND_NSITE[K_NKIND] = {0, 0, 0, 1, 1, 0, PSP103_NSITE};
 
 // Structural G-plane and C-plane entries per local row (bit masks over
 // local columns). The pattern and slot tapes use their union.
@@ -93,6 +99,7 @@ ESP_HD unsigned kind_gmask(int kind, int ru)
     case K_RES: return 0x3u;
     case K_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case K_MOS1: return ru == 1 ? 0x0u : 0xfu;
+    case K_PSP103: return psp103_gmask[ru];
     default: return 0u;
     }
 }
@@ -102,6 +109,7 @@ ESP_HD unsigned kind_cmask(int kind, int ru)
     case K_CAP: return 0x3u;
     case K_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case K_MOS1: return 0xfu;
+    case K_PSP103: return psp103_cmask[ru];
     default: (void)ru; return 0u;
     }
 }
@@ -110,6 +118,59 @@ ESP_HD unsigned kind_pmask(int kind, int ru) { return kind_gmask(kind, ru) | kin
 // $temperature of every run: 27 degC.
 constexpr double TEMPERATURE_K = 300.15;
 constexpr double KB = 1.38064852e-23, QE = 1.6021766208e-19;
+
+// Compiled kinds (va_models.h): the setup, its collapse flags and the
+// collapse pairs. setup returns 0, or 1 + the index of a parameter outside
+// its declared range.
+inline int va_setup(int kind, const double *u, double *p)
+{
+    switch (kind) {
+    case K_PSP103: return psp103_setup(u, TEMPERATURE_K, VA_SIMPARAM_GMIN, p);
+    default: return -1;
+    }
+}
+inline int va_ncoll(int kind) { return kind == K_PSP103 ? PSP103_NCOLL : 0; }
+inline int va_coll_base(int kind) { return kind == K_PSP103 ? PSP103_COLL_BASE : 0; }
+inline const int *va_coll_pairs(int kind) { return kind == K_PSP103 ? psp103_collapse : nullptr; }
+inline const char *const *va_param_names(int kind) { return kind == K_PSP103 ? psp103_param_names : nullptr; }
+
+// The unknowns of one compiled instance: ports as given, internal unknowns
+// collapsed (union-find over the pairs its prepared constants switch on,
+// a set with a port takes the port's row, else its first unknown's) or
+// fresh rows in unknown order. rows[u] < 0 asks for a fresh row; the
+// caller allocates it; rows[u] = -2 - r takes unknown r's row (r < u).
+// Returns 0, the setup's range error, or -2 for a collapse of two ports.
+inline int va_collapse_rows(int kind, const double *u, const uint32_t *ports, int64_t *rows)
+{
+    const int nu = KIND_NU[kind], np = KIND_NPORT[kind];
+    double *p = new double[KIND_PK[kind]];
+    const int rc = va_setup(kind, u, p);
+    int parent[32];
+    for (int k = 0; k < nu; k++) parent[k] = k;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    const int *pr = va_coll_pairs(kind);
+    for (int k = 0; rc == 0 && k < va_ncoll(kind); k++) {
+        if (p[va_coll_base(kind) + k] == 0.0) continue;
+        int a = find(pr[2 * k]), b = find(pr[2 * k + 1]);
+        if (a == b) continue;
+        if (a < np && b < np) {
+            delete[] p;
+            return -2;  // a collapse between two ports
+        }
+        // the smaller unknown index is the root: ports come first
+        if (a < b) parent[b] = a;
+        else parent[a] = b;
+    }
+    delete[] p;
+    for (int k = 0; k < nu; k++) {
+        const int r = find(k);
+        rows[k] = r < np ? (int64_t)ports[r] : (r == k ? -1 : -2 - r);
+    }
+    return rc;
+}
 
 struct DevOut {
     double res[4];

@@ -25,7 +25,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <strings.h>
+#include <map>
 #include <memory>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -49,10 +52,17 @@ espice_status fail(espice_status s, const char *name)
     return s;
 }
 
-#define HIP_TRY(expr)                                                   \
-    do {                                                                \
-        hipError_t e_ = (expr);                                         \
-        if (e_ != hipSuccess) return fail(ESPICE_FAILED, hipGetErrorName(e_)); \
+// A HIP error names itself and the engine line that met it.
+espice_status fail_hip(hipError_t e, int line)
+{
+    g_error = std::string(hipGetErrorName(e)) + " (engine.hip:" + std::to_string(line) + ")";
+    return ESPICE_FAILED;
+}
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return fail_hip(e_, __LINE__); \
     } while (0)
 
 template <class T>
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 // state in dynamic LDS laid out by `lay` after the template's tables. With
 // DENSE the lane's direct solver replays on the whole wavefront
 // (wave_lu.h, patterns up to 64 unknowns); otherwise thread 0 runs it.
-template <bool DENSE>
+template <bool DENSE, bool VA>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
@@ -154,13 +164,13 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     __syncthreads();
     if (DENSE) {
         const WaveHook off{};
-        const int r = wave_newton<false, false>(W, s, D, rc, o, off, o.init_fix != 0, lim_active, 0.0, it);
+        const int r = wave_newton<false, false, VA>(W, s, D, rc, o, off, o.init_fix != 0, lim_active, 0.0, it);
         st = r == NEWTON_CONVERGED ? 1u : r == NEWTON_SINGULAR ? 2u : 0u;
     } else {
         uint32_t bits = o.init_fix ? G_INITFIX : 0;
         double limit = 1e9, scaled = 0;
         for (uint32_t tick = 0; tick < o.max_iter; tick++) {
-            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, lim_active);
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<false, VA>(W, s, e, lim_active);
             __syncthreads();
             wave_reduce(W, s, rc, tid, nt);
             __syncthreads();
@@ -209,6 +219,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
 // symbolic tape written to `pred` (PredTape) for every lane's first
 // factor to try; PRED_OK stays 0 when that factor fails or voids a
 // column. One workgroup; setup work, not part of a solve.
+template <bool VA>
 __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                           const double *prm_lanes, Newton o, uint32_t *pred)
 {
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
     for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
     const ReduceCache rc = wave_reduce_cache(W, tid);
     __syncthreads();
-    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst(W, s, e, true);
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<false, VA>(W, s, e, true);
     __syncthreads();
     wave_reduce(W, s, rc, tid, nt);
     __syncthreads();
@@ -277,6 +288,7 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
 // t = 0, then steps until t_stop, each accepted row fed to the .meas
 // passes and, up to `to.record_cap` rows, recorded. The lane's final
 // state goes back to x_glob.
+template <bool VA>
 __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                        const double *prm_lanes, double *x_glob, uint32_t N,
                                                        Newton op_opts, Newton step_opts, TranOpts to,
@@ -357,7 +369,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     TranStep st{};
     uint32_t it = 0;
     const WaveHook off{};
-    const int op = wave_newton<true, false>(W, s, D, rc, op_opts, off, true, true, 0.0, it);
+    const int op = wave_newton<true, false, VA>(W, s, D, rc, op_opts, off, true, true, 0.0, it);
     if (op != NEWTON_CONVERGED) {
         if (tid == 0) {
             st.out.status = op == NEWTON_SINGULAR ? 3u : 2u;
@@ -374,7 +386,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     const bool has_charge = n_tape > 0;
     if (has_charge) {
         // q(op) seeds a flat history
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, false, 0.0);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA>(W, s, e, false, 0.0);
         __syncthreads();
         for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
         __syncthreads();
@@ -441,11 +453,11 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst(W, s, e, false);
         __syncthreads();
         WaveHook h{method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
-        const int r = wave_newton<true, true>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
+        const int r = wave_newton<true, true, VA>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
         if (r == NEWTON_CONVERGED && has_charge) {
             // the charge re-read at the solution, unlimited (tran.zig:859-864,
             // eval.zig:1086-1108)
-            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true>(W, s, e, false, t_dev);
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA>(W, s, e, false, t_dev);
             __syncthreads();
             for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
             __syncthreads();
@@ -615,8 +627,13 @@ struct espice_lanes {
     uint32_t N = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_w0 = nullptr, ev_w1 = nullptr;
-    std::vector<double> uprm[K_NKIND];  // host, lane-major
+    std::vector<double> uprm[K_NKIND];  // host, lane-major (hand-written kinds)
+    // compiled kinds: per-lane values of the parameters set_param moved,
+    // keyed (kind, instance, parameter); the rest are the template's
+    std::map<uint64_t, std::vector<double>> va_over;
     bool prepared = false;
+    size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may take
+    bool has_va = false;         // the template holds compiled kinds
     // device pattern
     DevBuf<uint32_t> col_ptr, row_idx, q, tri_pos, diag, nodes[K_NKIND], slots[K_NKIND];
     DevBuf<uint8_t> cur_row;
@@ -749,6 +766,32 @@ espice_status espice_lanes_template_add(espice_lanes_template *t, uint32_t kind,
     return ESPICE_OK;
 }
 
+espice_status espice_lanes_param_count(uint32_t kind, uint32_t *count)
+{
+    if (!count || kind >= K_NKIND) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    *count = (uint32_t)KIND_UK[kind];
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_param_index(uint32_t kind, const char *name, uint32_t *index)
+{
+    if (!name || !index || kind >= K_NKIND) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    if (!kind_is_va((int)kind)) return fail(ESPICE_INVALID_ARGUMENT, "UnknownParam");
+    const char *const *names = va_param_names((int)kind);
+    for (int k = 0; k < KIND_UK[kind]; k++)
+        if (strcasecmp(names[k], name) == 0) {
+            *index = (uint32_t)k;
+            return ESPICE_OK;
+        }
+    if (kind == K_PSP103)
+        for (int k = 0; k < PSP103_NALIAS; k++)
+            if (strcasecmp(psp103_alias_names[k], name) == 0) {
+                *index = (uint32_t)psp103_alias_target[k];
+                return ESPICE_OK;
+            }
+    return fail(ESPICE_INVALID_ARGUMENT, "UnknownParam");
+}
+
 espice_status espice_lanes_template_freeze(espice_lanes_template *t)
 {
     if (!t) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
@@ -796,6 +839,7 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     HIP_TRY(hipEventCreate(&L->ev_w0));
     HIP_TRY(hipEventCreate(&L->ev_w1));
     for (int k = 0; k < K_NKIND; k++) {
+        if (kind_is_va(k)) continue;
         L->uprm[k].resize((size_t)T.count[k] * KIND_UK[k] * N);
         for (size_t i = 0; i < T.count[k]; i++)
             for (int p = 0; p < KIND_UK[k]; p++)
@@ -810,8 +854,11 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
     for (int k = 0; k < K_NKIND; k++) {
         HIP_TRY(upload(L->nodes[k], T.nodes[k]));
         HIP_TRY(upload(L->slots[k], T.slots[k]));
-        HIP_TRY(zeroed(L->prm[k], (size_t)T.count[k] * KIND_PK[k] * N));
-        HIP_TRY(zeroed(L->lim[k], (size_t)T.count[k] * KIND_NU[k] * N));
+        // the thread-per-lane kernel's per-kind planes; compiled kinds run on
+        // the workgroup-per-lane kernels only
+        const size_t pc = kind_is_va(k) ? 0 : (size_t)T.count[k];
+        HIP_TRY(zeroed(L->prm[k], pc * KIND_PK[k] * N));
+        HIP_TRY(zeroed(L->lim[k], pc * KIND_NU[k] * N));
     }
     HIP_TRY(zeroed(L->x, n * N));
     HIP_TRY(zeroed(L->pred_hit, N));
@@ -857,8 +904,24 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         // template tables live
         const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
         L->lds_bytes = 4 * (size_t)L->wh.at.words + 8 * wave_words(W, L->lay);
-        L->use_wave = L->lds_bytes <= 64 * 1024;
+        int dev = 0, lds = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+            L->lds_max = std::min<size_t>((size_t)lds, 160 * 1024);
+        L->use_wave = L->lds_bytes <= L->lds_max;
     }
+    if (L->lds_max > 64 * 1024) {
+        // the workgroup-per-lane kernels may take more than the default
+        // 64 KiB of dynamic LDS (gfx950 gives a workgroup up to 160 KiB)
+        const int m = (int)L->lds_max;
+        for (const void *k : {(const void *)newton_wave_kernel<true, false>, (const void *)newton_wave_kernel<false, false>,
+                              (const void *)newton_wave_kernel<true, true>, (const void *)newton_wave_kernel<false, true>,
+                              (const void *)analyze_wave_kernel<false>, (const void *)analyze_wave_kernel<true>,
+                              (const void *)tran_wave_kernel<false>, (const void *)tran_wave_kernel<true>})
+            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, m));
+    }
+    for (int k = 0; k < K_NKIND; k++) L->has_va = L->has_va || (kind_is_va(k) && T.count[k] > 0);
+    if (L->has_va && (!L->use_wave || L->lay.dense == ESP_NONE)) return fail(ESPICE_FAILED, "LaneTooLarge");
     const uint32_t blocks = (n_lanes + 63) / 64;
     hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, n_lanes,
                        (uint8_t)(T.tridiagonal ? 1 : 0));
@@ -893,8 +956,70 @@ espice_status espice_lanes_set_param(espice_lanes *L, uint32_t kind, uint32_t in
 {
     if (!L || !values || kind >= K_NKIND) return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
     if (instance >= L->t->count[kind] || param >= (uint32_t)KIND_UK[kind]) return fail(ESPICE_INVALID_ARGUMENT, "InvalidParam");
+    if (kind_is_va((int)kind)) {
+        L->va_over[((uint64_t)kind << 56) | ((uint64_t)instance << 24) | param].assign(values, values + L->N);
+        L->prepared = false;
+        return ESPICE_OK;
+    }
     std::memcpy(&L->uprm[kind][((size_t)instance * KIND_UK[kind] + param) * L->N], values, sizeof(double) * L->N);
     L->prepared = false;
+    return ESPICE_OK;
+}
+
+// A compiled kind's setup on every lane and instance, on the host's
+// threads, straight into the lane-contiguous constants. Every lane must
+// keep the template's collapse (its internal unknowns' rows), as a
+// diode's RS must keep its sign.
+static espice_status prepare_va(espice_lanes *L, int kind)
+{
+    const Template &T = *L->t;
+    const size_t N = L->N, n_prm = L->wh.n_prm;
+    const int uk = KIND_UK[kind], pk = KIND_PK[kind];
+    const InstEntry *ie = reinterpret_cast<const InstEntry *>(L->wh.inst_bytes.data());
+    std::vector<uint32_t> prm_of(T.count[kind], 0);
+    for (uint32_t e = 0; e < L->wh.n_inst; e++)
+        if (ie[e].kind == kind) prm_of[ie[e].id] = ie[e].prm;
+    // the template's collapse flags per instance, from its own parameters
+    const int nc = va_ncoll(kind), cb = va_coll_base(kind);
+    std::vector<double> flags((size_t)T.count[kind] * (nc ? nc : 1));
+    {
+        std::vector<double> p(pk);
+        for (uint32_t i = 0; i < T.count[kind]; i++) {
+            if (va_setup(kind, &T.uprm[kind][(size_t)i * uk], p.data()) != 0)
+                return fail(ESPICE_INVALID_ARGUMENT, "ParamRange");
+            for (int c = 0; c < nc; c++) flags[(size_t)i * nc + c] = p[cb + c];
+        }
+    }
+    std::vector<std::vector<std::pair<int, const double *>>> over(T.count[kind]);
+    for (const auto &kv : L->va_over) {
+        if ((int)(kv.first >> 56) != kind) continue;
+        over[(kv.first >> 24) & 0xffffffffu].push_back({(int)(kv.first & 0xffffffu), kv.second.data()});
+    }
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<size_t>(hw, std::max<size_t>(1, N / 256));
+    std::vector<int> bad(nt, 0);
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([&, w]() {
+            std::vector<double> u(uk), p(pk);
+            for (size_t l = w; l < N; l += nt)
+                for (uint32_t i = 0; i < T.count[kind]; i++) {
+                    std::memcpy(u.data(), &T.uprm[kind][(size_t)i * uk], 8 * (size_t)uk);
+                    for (const auto &o : over[i]) u[o.first] = o.second[l];
+                    if (va_setup(kind, u.data(), p.data()) != 0) {
+                        bad[w] = 1;
+                        continue;
+                    }
+                    for (int c = 0; c < nc; c++)
+                        if (p[cb + c] != flags[(size_t)i * nc + c]) bad[w] = 2;
+                    std::memcpy(&L->prm_host[l * n_prm + prm_of[i]], p.data(), 8 * (size_t)pk);
+                }
+        });
+    for (auto &t : pool) t.join();
+    for (int b : bad) {
+        if (b == 1) return fail(ESPICE_INVALID_ARGUMENT, "ParamRange");
+        if (b == 2) return fail(ESPICE_INVALID_ARGUMENT, "CollapseTopology");
+    }
     return ESPICE_OK;
 }
 
@@ -902,9 +1027,14 @@ espice_status espice_lanes_prepare(espice_lanes *L)
 {
     if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     const size_t N = L->N;
+    L->prm_host.assign((size_t)L->wh.n_prm * N, 0.0);
     for (int kind = 0; kind < K_NKIND; kind++) {
         const uint32_t cnt = L->t->count[kind];
         const int uk = KIND_UK[kind], pk = KIND_PK[kind];
+        if (kind_is_va(kind)) {
+            if (espice_status e = prepare_va(L, kind)) return e;
+            continue;
+        }
         std::vector<double> host((size_t)cnt * pk * N);
         double u[PARAM_MAX], p[PARAM_MAX];
         for (uint32_t i = 0; i < cnt; i++)
@@ -927,7 +1057,6 @@ espice_status espice_lanes_prepare(espice_lanes *L)
             HIP_TRY(hipMemcpyAsync(L->prm[kind].p, host.data(), host.size() * 8, hipMemcpyHostToDevice, L->stream));
         HIP_TRY(hipStreamSynchronize(L->stream));
         // the same constants per lane contiguous, in serial instance order
-        if (kind == 0) L->prm_host.assign((size_t)L->wh.n_prm * N, 0.0);
         const InstEntry *ie = reinterpret_cast<const InstEntry *>(L->wh.inst_bytes.data());
         for (uint32_t e = 0; e < L->wh.n_inst; e++) {
             if (ie[e].kind != kind) continue;
@@ -945,8 +1074,12 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         const PredTape T = pred_tape(L->t->n, L->t->nnz);
         if (L->pred.n < T.words) HIP_TRY(L->pred.alloc(T.words));
         const Newton o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
-        hipLaunchKernelGGL(analyze_wave_kernel, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at, L->w_blob.p,
-                           L->lay, L->prm_lanes.p, o, L->pred.p);
+        if (L->has_va)
+            hipLaunchKernelGGL(analyze_wave_kernel<true>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p);
+        else
+            hipLaunchKernelGGL(analyze_wave_kernel<false>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(L->stream));
         L->has_pred = true;
@@ -994,13 +1127,18 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
-        if (L->lay.dense != ESP_NONE)
-            hipLaunchKernelGGL(newton_wave_kernel<true>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+        if (L->lay.dense != ESP_NONE && L->has_va)
+            hipLaunchKernelGGL((newton_wave_kernel<true, true>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
+                               L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
+                               want_x ? L->d_x : nullptr);
+        else if (L->lay.dense != ESP_NONE)
+            hipLaunchKernelGGL((newton_wave_kernel<true, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                                L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
                                want_x ? L->d_x : nullptr);
         else
-            hipLaunchKernelGGL(newton_wave_kernel<false>, dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+            hipLaunchKernelGGL((newton_wave_kernel<false, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                                nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
     } else {
@@ -1139,7 +1277,7 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
     const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));
     const size_t lds = 4 * (size_t)L->wh.at.words + 8 * words;
-    if (lay.dense == ESP_NONE || lds > 64 * 1024) return fail(ESPICE_FAILED, "LaneTooLarge");
+    if (lay.dense == ESP_NONE || lds > L->lds_max) return fail(ESPICE_FAILED, "LaneTooLarge");
     DevBuf<MeasPass> d_pass;
     DevBuf<TranOut> d_out;
     DevBuf<MeasOut> d_meas;
@@ -1165,9 +1303,14 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     step.residual_tol = inf_();
     step.init_fix = 0;
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
-    hipLaunchKernelGGL(tran_wave_kernel, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay, L->prm_lanes.p,
-                       L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
-                       L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
+    if (L->has_va)
+        hipLaunchKernelGGL(tran_wave_kernel<true>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
+                           L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
+                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
+    else
+        hipLaunchKernelGGL(tran_wave_kernel<false>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
+                           L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
+                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));

@@ -27,6 +27,26 @@ int Template::add(int kind, const uint32_t *ports, const double *params, uint32_
     bool seen = false;
     for (int i = 0; i < n_order; i++) seen = seen || order[i] == kind;
     if (!seen) order[n_order++] = kind;
+    if (kind_is_va(kind)) {
+        // a compiled kind's collapse, decided by its setup on these
+        // parameters (the generated collapse pairs)
+        int64_t rows[32];
+        if (va_collapse_rows(kind, params, ports, rows) != 0) return -1;
+        const size_t base = nodes[kind].size();
+        for (int u = 0; u < KIND_NU[kind]; u++) {
+            if (rows[u] >= 0) {
+                nodes[kind].push_back((uint32_t)rows[u]);
+            } else if (rows[u] == -1) {
+                nodes[kind].push_back(n++);
+                cur_flag.push_back(0);
+            } else {
+                nodes[kind].push_back(nodes[kind][base + (size_t)(-2 - rows[u])]);
+            }
+        }
+        for (int k = 0; k < KIND_UK[kind]; k++) uprm[kind].push_back(params[k]);
+        *inst = count[kind]++;
+        return 0;
+    }
     for (int p = 0; p < KIND_NPORT[kind]; p++) nodes[kind].push_back(ports[p]);
     // Internal unknowns take fresh rows in unknown order unless the kind's
     // collapse maps them onto a port (frontend/builder.zig:258-263): a
@@ -183,7 +203,8 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
             for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(kind, ru) != 0;
             e.stage_q = ESP_NONE;
             e.stage_c = ESP_NONE;
-            e.tape = KIND_TAPE_ROW[kind] >= 0 ? tape++ : ESP_NONE;
+            e.tape = KIND_NSITE[kind] > 0 ? tape : ESP_NONE;
+            tape += (uint32_t)KIND_NSITE[kind];
             if (charged) {
                 e.stage_q = stage;
                 stage += nu;

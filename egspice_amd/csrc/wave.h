@@ -310,11 +310,40 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
     }
 }
 
-// TRAN: sources read their waveform at t and charges are staged too.
-template <bool TRAN = false>
+// A compiled kind (va_models.h): the generated evaluation writes the
+// instance's staging cells in the order the stamp tape reads them (rows,
+// then the G mask's entries row-major; transient: charges, the C mask's
+// entries, and one charge per ddt() site into the lane's charge tape).
+template <int KIND, bool TRAN>
+ESP_HD void wave_eval_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e)
+{
+    constexpr int nu = KIND_NU[KIND];
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    double x[nu];
+#pragma unroll
+    for (int u = 0; u < nu; u++) x[u] = s.x[nd[u]];
+    const double *prm = s.prm + ie.prm;
+    double *res = s.stage + ie.stage_res, *jac = s.stage + ie.stage_jac;
+    if constexpr (KIND == K_PSP103) {
+        if (TRAN) psp103_eval_tran(prm, x, res, jac, s.stage + ie.stage_q, s.stage + ie.stage_c, s.tape + ie.tape);
+        else psp103_eval(prm, x, res, jac);
+    }
+}
+
+// TRAN: sources read their waveform at t and charges are staged too. VA:
+// the template holds compiled kinds; only then are their (long, register-
+// hungry) evaluations instantiated, so the kernels of templates without
+// them keep their own register budget and occupancy.
+template <bool TRAN = false, bool VA = false>
 ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t = 0.0)
 {
     const InstEntry ie = W.inst[e];
+    if constexpr (VA) {
+        if (ie.kind == K_PSP103) {
+            wave_eval_va<K_PSP103, TRAN>(W, s, ie, e);
+            return;
+        }
+    }
     switch (ie.kind) {
     case K_VSRC: wave_eval_k<K_VSRC, TRAN>(W, s, ie, e, lim_active, t); break;
     case K_ISRC: wave_eval_k<K_ISRC, TRAN>(W, s, ie, e, lim_active, t); break;

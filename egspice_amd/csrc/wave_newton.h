@@ -25,7 +25,7 @@ enum { NEWTON_SINGULAR = -1, NEWTON_FAILED = 0, NEWTON_CONVERGED = 1 };
 // operating point and its steps); HOOK: the integrator's companion applies
 // (h). Both are compile-time so the static solve carries none of it. All
 // threads; uniform result.
-template <bool TRAN, bool HOOK>
+template <bool TRAN, bool HOOK, bool VA = false>
 __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D, const ReduceCache &rc, const Newton &o,
                            const WaveHook &h, bool init_fix, bool lim_active, double t, uint32_t &iters)
 {
@@ -37,7 +37,7 @@ __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, c
     for (uint32_t i = tid; i < n; i += nt) s.x_old[i] = s.x[i];
     __syncthreads();
     for (uint32_t tick = 0; tick < o.max_iter; tick++) {
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<TRAN>(W, s, e, lim_active, t);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<TRAN, VA>(W, s, e, lim_active, t);
         __syncthreads();
         wave_reduce(W, s, rc, tid, nt);
         if constexpr (HOOK)

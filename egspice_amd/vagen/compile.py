@@ -1,0 +1,328 @@
+"""Analyses and C emission of a normalised Verilog-A model (ir.Model).
+
+1. Types: integer or real for every variable and parameter, and for every
+   expression by the LRM's rules (integer op integer stays integer).
+2. Bias dependence: a variable is *dynamic* when any assignment to it
+   reads a probe, $abstime or a dynamic variable, or sits under a dynamic
+   condition; otherwise *static* (a function of the parameters and the
+   temperature only). Static statements move to the setup function, which
+   runs once per instance and lane; a static value read by the evaluation
+   is stored in the instance's prepared constants. A static variable read
+   by a dynamic statement and assigned again after it becomes dynamic, so
+   moving the static statements never changes a value.
+3. Relevance: only statements whose results reach a contribution (or a
+   collapse) are kept, separately for the static evaluation (currents) and
+   the transient one (currents and charges).
+4. Derivative lanes: each dynamic real carries the partial derivatives
+   with respect to the unknowns it depends on, and only those (sparse
+   forward-mode AD): lanes(V(a, b)) = {a, b}, a variable's lanes the union
+   over its assignments.
+5. Emission: three-address C with one temporary per operation; the
+   derivative rules follow the reference's dual numbers
+   (src/device/eval.zig:107-350): product rule and quotient numerator
+   fused (fma), quotient lanes scaled by the reciprocal, sqrt and pow slopes
+   with their zero and non-finite guards, map(a, f(a.v), f'(a.v)) for the
+   elementary functions.
+"""
+from .ir import Model
+from .parse import ParseError
+
+CMP = {"<", "<=", ">", ">=", "==", "!="}
+LOGIC = {"&&", "||"}
+
+
+def walk_expr(e, fn):
+    fn(e)
+    k = e[0]
+    if k == "un":
+        walk_expr(e[2], fn)
+    elif k == "bin":
+        walk_expr(e[2], fn)
+        walk_expr(e[3], fn)
+    elif k == "tern":
+        walk_expr(e[1], fn)
+        walk_expr(e[2], fn)
+        walk_expr(e[3], fn)
+    elif k == "call":
+        for a in e[2]:
+            walk_expr(a, fn)
+    elif k == "ddt":
+        walk_expr(e[1], fn)
+
+
+def expr_reads(e):
+    vs, ps = set(), set()
+
+    def f(x):
+        if x[0] == "var":
+            vs.add(x[1])
+        elif x[0] in ("par", "given"):
+            ps.add(x[1])
+
+    walk_expr(e, f)
+    return vs, ps
+
+
+def has_ddt(e):
+    found = []
+    walk_expr(e, lambda x: found.append(1) if x[0] == "ddt" else None)
+    return bool(found)
+
+
+def split_ddt(e, sign=1):
+    """A contribution's expression as (resistive part or None, [(sign, q)])."""
+    k = e[0]
+    if k == "ddt":
+        if has_ddt(e[1]):
+            raise ParseError("nested ddt")
+        return None, [(sign, e[1])]
+    if not has_ddt(e):
+        return (e if sign > 0 else ("un", "-", e)), []
+    if k == "un" and e[1] == "-":
+        return split_ddt(e[2], -sign)
+    if k == "bin" and e[1] in ("+", "-"):
+        ra, qa = split_ddt(e[2], sign)
+        rb, qb = split_ddt(e[3], sign if e[1] == "+" else -sign)
+        if ra is None:
+            r = rb
+        elif rb is None:
+            r = ra
+        else:
+            r = ("bin", "+", ra, rb)
+        return r, qa + qb
+    raise ParseError("ddt() must be an added term of a contribution")
+
+
+class Stmt:
+    """A statement with its place in program order."""
+    __slots__ = ("kind", "a", "pos", "end", "conds", "loops", "kids")
+
+    def __init__(self, kind, a, pos, conds, loops):
+        self.kind, self.a, self.pos, self.conds, self.loops = kind, a, pos, conds, loops
+        self.kids = []
+        self.end = pos
+
+
+class Compiler:
+    def __init__(self, model: Model, prefix):
+        self.m = model
+        self.prefix = prefix
+        self.nu = len(model.unknowns)
+        self.flat = []  # every leaf statement (assign/contrib/collapse) as Stmt
+        self.pos = 0
+        self.tree = self.index(model.body, (), ())
+        self.types()
+        self.classify()
+        self.relevance()
+        self.lanes()
+
+    # ------------------------------------------------------------ index
+    def index(self, stmts, conds, loops):
+        out = []
+        for s in stmts:
+            k = s[0]
+            self.pos += 1
+            st = Stmt(k, s, self.pos, conds, loops)
+            if k == "if":
+                st.kids = [self.index(s[2], conds + (s[1],), loops), self.index(s[3], conds + (s[1],), loops)]
+            elif k == "while":
+                st.kids = [self.index(s[2], conds + (s[1],), loops + (st,))]
+            else:
+                self.flat.append(st)
+            st.end = self.pos
+            out.append(st)
+        return out
+
+    # ------------------------------------------------------------ types
+    def types(self):
+        self.ptype = {p["name"]: ("int" if p["type"] == "integer" else "real") for p in self.m.params}
+        self.vtype = {v: ("int" if t == "integer" else "real") for v, t in self.m.vartype.items()}
+
+    def etype(self, e):
+        k = e[0]
+        if k == "num":
+            return "int" if e[2] else "real"
+        if k == "par":
+            return self.ptype[e[1]]
+        if k == "var":
+            return self.vtype[e[1]]
+        if k in ("given", "analysis"):
+            return "int"
+        if k == "un":
+            return "int" if e[1] == "!" else self.etype(e[2])
+        if k == "bin":
+            if e[1] in CMP or e[1] in LOGIC:
+                return "int"
+            ta, tb = self.etype(e[2]), self.etype(e[3])
+            return "int" if ta == tb == "int" else "real"
+        if k == "tern":
+            ta, tb = self.etype(e[2]), self.etype(e[3])
+            return "int" if ta == tb == "int" else "real"
+        if k == "call":
+            if e[1] in ("min", "max", "abs"):
+                return "int" if all(self.etype(a) == "int" for a in e[2]) else "real"
+            return "real"
+        return "real"
+
+    # ---------------------------------------------------- bias dependence
+    def expr_dyn(self, e):
+        hit = []
+
+        def f(x):
+            k = x[0]
+            if k in ("probe", "abstime", "ddt", "ddx"):
+                hit.append(1)
+            elif k == "var" and x[1] in self.dyn:
+                hit.append(1)
+
+        walk_expr(e, f)
+        return bool(hit)
+
+    def assigned(self, st):
+        k = st.kind
+        if k == "assign":
+            return st.a[1]
+        if k == "limit":
+            return st.a[1]
+        return None
+
+    def stmt_reads(self, st):
+        """Expressions a leaf statement reads (conditions included)."""
+        k = st.kind
+        es = list(st.conds)
+        if k == "assign":
+            es.append(st.a[2])
+        elif k == "contrib":
+            es.append(st.a[3])
+        elif k == "limit":
+            es.append(st.a[3])
+            es.extend(st.a[4])
+        return es
+
+    def classify(self):
+        self.dyn = set()
+        changed = True
+        while changed:
+            changed = False
+            for st in self.flat:
+                v = self.assigned(st)
+                if v is None or v in self.dyn:
+                    continue
+                if st.kind == "limit" or any(self.expr_dyn(e) for e in self.stmt_reads(st)):
+                    self.dyn.add(v)
+                    changed = True
+            if changed:
+                continue
+            # hazard: a static variable read by a dynamic statement and
+            # assigned (statically) later, or inside a loop around the read
+            assigns = {}
+            for st in self.flat:
+                v = self.assigned(st)
+                if v is not None and v not in self.dyn:
+                    assigns.setdefault(v, []).append(st)
+            for st in self.flat:
+                if not self.is_dynamic_stmt(st):
+                    continue
+                for e in self.stmt_reads(st):
+                    vs, _ = expr_reads(e)
+                    for v in vs:
+                        if v in self.dyn or v not in assigns:
+                            continue
+                        for a in assigns[v]:
+                            if a.pos > st.pos or any(lp.pos <= a.pos <= lp.end for lp in st.loops):
+                                self.dyn.add(v)
+                                changed = True
+                                break
+
+    def is_dynamic_stmt(self, st):
+        if st.kind in ("contrib",):
+            return True
+        if st.kind == "collapse":
+            return False
+        v = self.assigned(st)
+        return v in self.dyn
+
+    # ---------------------------------------------------------- relevance
+    def relevance(self):
+        self.contribs = []
+        for st in self.flat:
+            if st.kind == "contrib":
+                r, qs = split_ddt(st.a[3])
+                self.contribs.append((st, r, qs))
+        self.rel = {}
+        for mode in ("dc", "tran"):
+            need = set()
+            seeds = []
+            for st, r, qs in self.contribs:
+                seeds.extend(st.conds)
+                if r is not None:
+                    seeds.append(r)
+                if mode == "tran":
+                    seeds.extend(q for _, q in qs)
+            for st in self.flat:
+                if st.kind == "collapse":
+                    seeds.extend(st.conds)
+            work = []
+            for e in seeds:
+                vs, _ = expr_reads(e)
+                work.extend(vs)
+            by_var = {}
+            for st in self.flat:
+                v = self.assigned(st)
+                if v is not None:
+                    by_var.setdefault(v, []).append(st)
+            while work:
+                v = work.pop()
+                if v in need:
+                    continue
+                need.add(v)
+                for st in by_var.get(v, []):
+                    for e in self.stmt_reads(st):
+                        vs, _ = expr_reads(e)
+                        work.extend(vs - need)
+            self.rel[mode] = need
+
+    # -------------------------------------------------------------- lanes
+    def lanes(self):
+        self.vl = {v: frozenset() for v in self.dyn}
+        changed = True
+        while changed:
+            changed = False
+            for st in self.flat:
+                v = self.assigned(st)
+                if v is None or v not in self.dyn or self.vtype.get(v) == "int":
+                    continue
+                if st.kind == "limit":
+                    l = self.elanes(st.a[3])
+                else:
+                    l = self.elanes(st.a[2])
+                if not l <= self.vl[v]:
+                    self.vl[v] = self.vl[v] | l
+                    changed = True
+
+    def elanes(self, e):
+        k = e[0]
+        if k == "probe":
+            return frozenset(x for x in e[1:] if x >= 0)
+        if k == "var":
+            if self.vtype.get(e[1]) == "int":
+                return frozenset()
+            return self.vl.get(e[1], frozenset())
+        if k == "un":
+            return frozenset() if e[1] == "!" else self.elanes(e[2])
+        if k == "bin":
+            if e[1] in CMP or e[1] in LOGIC:
+                return frozenset()
+            return self.elanes(e[2]) | self.elanes(e[3])
+        if k == "tern":
+            return self.elanes(e[2]) | self.elanes(e[3])
+        if k == "call":
+            if e[1] in ("floor", "ceil"):
+                return frozenset()
+            out = frozenset()
+            for a in e[2]:
+                out |= self.elanes(a)
+            return out
+        if k == "ddt":
+            return self.elanes(e[1])
+        return frozenset()

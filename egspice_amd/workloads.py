@@ -125,3 +125,83 @@ def diode_rectifier(add, rload=1e3, cres=100e-6, dmodel=None):
     d = add(DIODE, ["in", "out"], dict(DIODE_RECT if dmodel is None else dmodel))
     r = add(RESISTOR, ["out", "0"], {0: rload})
     return v, c, d, r
+
+
+# Two-stage Miller op-amp on PSP103 (models/psp103.va, default model cards
+# with TYPE = +1/-1), connected as a unity-gain buffer so its operating
+# point is defined; source/drain sheet resistance RSH 10 ohm/sq with NRS =
+# NRD = 1 gives every transistor its SI and DI rows besides NOI.
+#
+#     VDD   vdd 0 DC 1.8
+#     VINP  inp 0 DC 0.9  (PULSE 0.9 -> 1.0 in the transient)
+#     IBIAS vdd nbias DC 20u
+#     M8 nbias nbias 0 0    nch W=2u  L=1u
+#     M5 tail  nbias 0 0    nch W=4u  L=1u
+#     M1 n1    out   tail 0 nch W=4u  L=0.5u   (inverting input: the output)
+#     M2 n2    inp   tail 0 nch W=4u  L=0.5u
+#     M3 n1    n1    vdd vdd pch W=8u  L=0.5u
+#     M4 n2    n1    vdd vdd pch W=8u  L=0.5u
+#     M6 out   n2    vdd vdd pch W=16u L=0.5u
+#     M7 out   nbias 0 0    nch W=8u  L=1u
+#     CC n2 out 1p
+#     CL out 0  1p
+OPAMP_DEVICES = (  # name, drain, gate, source, bulk, type, W, L, width group
+    ("M8", "nbias", "nbias", "0", "0", 1, 2e-6, 1e-6, 3),
+    ("M5", "tail", "nbias", "0", "0", 1, 4e-6, 1e-6, 3),
+    ("M1", "n1", "out", "tail", "0", 1, 4e-6, 0.5e-6, 0),
+    ("M2", "n2", "inp", "tail", "0", 1, 4e-6, 0.5e-6, 0),
+    ("M3", "n1", "n1", "vdd", "vdd", -1, 8e-6, 0.5e-6, 1),
+    ("M4", "n2", "n1", "vdd", "vdd", -1, 8e-6, 0.5e-6, 1),
+    ("M6", "out", "n2", "vdd", "vdd", -1, 16e-6, 0.5e-6, 2),
+    ("M7", "out", "nbias", "0", "0", 1, 8e-6, 1e-6, 3),
+)
+OPAMP_NOMINAL = dict(vdd=1.8, vcm=0.9, ibias=20e-6, cc=1e-12, cl=1e-12, temp_c=27.0)
+
+
+def miller_opamp(add, index, vcm=0.9, step=None):
+    """Cards of the op-amp; `index(name)` gives the slot of a named PSP103
+    parameter (the engine's param_index or the oracle's). Returns the
+    instance indices {'VDD', 'VINP', 'IBIAS', 'CC', 'CL', 'M1'..'M8'}.
+    step: (v2, td, tr, pw) turns VINP into PULSE(vcm v2 td tr tr pw 1)."""
+    from . import CAPACITOR, ISOURCE, PSP103
+    inst = {}
+    inst["VDD"] = add(VSOURCE, ["vdd", "0"], {VSOURCE_DC: OPAMP_NOMINAL["vdd"]})
+    if step is None:
+        inst["VINP"] = add(VSOURCE, ["inp", "0"], {VSOURCE_DC: vcm})
+    else:
+        v2, td, tr, pw = step
+        p = pulse_params(vcm, v2, td, tr, tr, pw, 1.0, dc=vcm)
+        inst["VINP"] = add(VSOURCE, ["inp", "0"], p)
+    inst["IBIAS"] = add(ISOURCE, ["vdd", "nbias"], {0: OPAMP_NOMINAL["ibias"]})
+    for name, d, g, s, b, ty, w, l, _ in OPAMP_DEVICES:
+        p = {index("TYPE"): float(ty), index("W"): w, index("L"): l, index("RSH"): 10.0,
+             index("NRS"): 1.0, index("NRD"): 1.0}
+        inst[name] = add(PSP103, [d, g, s, b], p)
+    inst["CC"] = add(CAPACITOR, ["n2", "out"], {0: OPAMP_NOMINAL["cc"]})
+    inst["CL"] = add(CAPACITOR, ["out", "0"], {0: OPAMP_NOMINAL["cl"]})
+    return inst
+
+
+def opamp_inputs(n, seed=1):
+    """The 8 Monte Carlo inputs per lane, uniform and seeded: width scale of
+    the 4 device groups (diff pair, mirror, second stage, current sinks)
+    in [0.9, 1.1], VDD in [1.6, 2.0] V, temperature in [-40, 125] degC, bias
+    current scale in [0.8, 1.2], load capacitance in [0.1, 2] pF."""
+    rng = np.random.default_rng(seed)
+    lo = np.array([0.9, 0.9, 0.9, 0.9, 1.6, -40.0, 0.8, 0.1e-12])
+    hi = np.array([1.1, 1.1, 1.1, 1.1, 2.0, 125.0, 1.2, 2.0e-12])
+    return lo + (hi - lo) * rng.random((n, 8))
+
+
+def opamp_lane_params(inputs, index):
+    """Per-lane parameter columns of the op-amp for `inputs` (n x 8):
+    [(kind, instance name, slot, values)]."""
+    from . import CAPACITOR, ISOURCE, PSP103
+    out = []
+    for name, d, g, s, b, ty, w, l, grp in OPAMP_DEVICES:
+        out.append((PSP103, name, index("W"), w * inputs[:, grp]))
+        out.append((PSP103, name, index("TRISE"), inputs[:, 5] - 27.0))
+    out.append((VSOURCE, "VDD", VSOURCE_DC, inputs[:, 4]))
+    out.append((ISOURCE, "IBIAS", 0, OPAMP_NOMINAL["ibias"] * inputs[:, 6]))
+    out.append((CAPACITOR, "CL", 0, inputs[:, 7]))
+    return out

@@ -36,10 +36,12 @@ enum {
 #endif
 
 /* Device kinds of a template (models/vsource.va, isource.va, resistor.va,
- * capacitor.va, diode.va, mos1.va). */
+ * capacitor.va, diode.va, mos1.va; psp103.va compiled from its Verilog-A:
+ * ports D G S B, the module's parameters in declaration order, named
+ * through espice_lanes_param_index). */
 enum {
     ESPICE_LANES_VSOURCE = 0, ESPICE_LANES_ISOURCE = 1, ESPICE_LANES_RESISTOR = 2,
-    ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5
+    ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5, ESPICE_LANES_PSP103 = 6
 };
 /* User parameters per kind (NaN = not given). VSOURCE: dc, waveform (0 DC,
  * 1 PULSE, 2 SIN, 3 EXP, 4 PWL, 5 SFFM, 6 AM), PULSE v1 v2 td tr tf pw per, SIN vo va freq td
@@ -96,6 +98,12 @@ espice_status espice_lanes_template_create(espice_lanes_template **out);
 espice_status espice_lanes_template_destroy(espice_lanes_template *t);
 /* Row of a named net, allocated on first touch; "0" and "gnd" are row 0. */
 espice_status espice_lanes_template_node(espice_lanes_template *t, const char *name, uint32_t *row);
+/* User parameters of a kind (the count espice_lanes_template_add takes),
+ * and the slot of a named parameter of a compiled kind (case-insensitive,
+ * aliasparam names resolved; UnknownParam otherwise). Replace the
+ * builder's binding of card names onto a model (device/bind.zig). */
+espice_status espice_lanes_param_count(uint32_t kind, uint32_t *count);
+espice_status espice_lanes_param_index(uint32_t kind, const char *name, uint32_t *index);
 /* One instance of kind on `ports` (rows); params holds the kind's user
  * parameters (NaN = not given). Internal unknowns take fresh rows now. */
 espice_status espice_lanes_template_add(espice_lanes_template *t, uint32_t kind, const uint32_t *ports,

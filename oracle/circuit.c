@@ -41,10 +41,13 @@
 int or_solver_is_tri(const or_solver *s);
 
 /* A diode's unknowns are (a, c, ai); ai is a's own row when RS is 0. */
-static const int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4};
-static const int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4};
-static const int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K};
-static const int PK[OR_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K};
+/* The compiled kinds' entries are filled from va.c at load. */
+static int NU[OR_NKIND] = {3, 2, 2, 2, 3, 4, 0};
+static int NPORT[OR_NKIND] = {2, 2, 2, 2, 2, 4, 0};
+static int UK[OR_NKIND] = {OR_VSRC_K, OR_ISRC_K, OR_RES_K, OR_CAP_K, OR_DIO_K, OR_M1_K, 0};
+static int PK[OR_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K, 0};
+static int NSITE[OR_NKIND] = {0, 0, 0, 1, 1, 0, 0};
+static int is_va(int kind) { return kind >= OR_PSP103; }
 /* room for one instance's user or prepared parameters */
 #define OR_PARAM_MAX 128
 _Static_assert(OR_VSRC_K <= OR_PARAM_MAX && SRCP_K <= OR_PARAM_MAX && OR_DIO_K <= OR_PARAM_MAX && DIOP_K <= OR_PARAM_MAX &&
@@ -60,6 +63,7 @@ static unsigned gmask(int kind, int ru)
     case OR_RES: return 0x3u;
     case OR_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case OR_MOS1: return ru == 1 ? 0x0u : 0xfu;
+    case OR_PSP103: return or_va_gmask(kind, ru);
     default: return 0;
     }
 }
@@ -69,14 +73,29 @@ static unsigned cmask(int kind, int ru)
     case OR_CAP: return 0x3u;
     case OR_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case OR_MOS1: return 0xfu;
+    case OR_PSP103: return or_va_cmask(kind, ru);
     default: (void)ru; return 0;
     }
 }
 static unsigned pmask(int kind, int ru) { return gmask(kind, ru) | cmask(kind, ru); }
 /* Residual rows a kind writes in a static evaluation. */
-static const unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu};
+static unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu, 0u};
 /* Unknowns each kind's limiter writes. */
-static const unsigned LIMW[OR_NKIND] = {0, 0, 0, 0, 0x4u, 0xbu};
+static const unsigned LIMW[OR_NKIND] = {0, 0, 0, 0, 0x4u, 0xbu, 0};
+
+__attribute__((constructor)) static void va_tables(void)
+{
+    for (int k = OR_PSP103; k < OR_NKIND; k++) {
+        NU[k] = or_va_nu(k);
+        NPORT[k] = or_va_nport(k);
+        UK[k] = or_va_nparam(k);
+        PK[k] = or_va_nprep(k);
+        NSITE[k] = or_va_nsite(k);
+        RMASK[k] = (1u << NU[k]) - 1u;
+    }
+}
+/* scratch big enough for any kind's parameters */
+#define OR_VA_PARAM_MAX 4096
 
 typedef struct { uint32_t *p; size_t len, cap; } vu32;
 typedef struct { double *p; size_t len, cap; } vf64;
@@ -192,6 +211,19 @@ int32_t or_ckt_add(or_circuit *c, int kind, const uint32_t *ports, const double 
     int seen = 0;
     for (int i = 0; i < c->n_order; i++) seen |= c->order[i] == kind;
     if (!seen) c->order[c->n_order++] = kind;
+    if (is_va(kind)) {
+        /* the compiled kind's collapse from its setup on these parameters */
+        int64_t rows[32];
+        if (!params || or_va_collapse(kind, params, ports, rows) != 0) return -1;
+        const size_t base = c->nodes[kind].len;
+        for (int u = 0; u < NU[kind]; u++) {
+            if (rows[u] >= 0) vpush(&c->nodes[kind], (uint32_t)rows[u]);
+            else if (rows[u] == -1) vpush(&c->nodes[kind], add_row(c, 0));
+            else vpush(&c->nodes[kind], c->nodes[kind].p[base + (size_t)(-2 - rows[u])]);
+        }
+        for (int k = 0; k < UK[kind]; k++) fpush(&c->uprm[kind], params[k]);
+        return (int32_t)c->count[kind]++;
+    }
     for (int p = 0; p < NPORT[kind]; p++) vpush(&c->nodes[kind], ports[p]);
     /* internal rows fresh, in unknown order, except a diode's ai, which is
      * its anode unless RS > 0 (builder.zig:258-263, diode.va:364-371) */
@@ -284,7 +316,12 @@ const uint32_t *or_ckt_diag(const or_circuit *c) { return c->diag; }
 const uint8_t *or_ckt_cur_row(const or_circuit *c) { return c->cur_row; }
 uint32_t or_ckt_count(const or_circuit *c, int kind) { return c->count[kind]; }
 int or_prepared_count(int kind) { return PK[kind]; }
-uint32_t or_tape_len(const or_circuit *c) { return c->count[OR_CAP] + c->count[OR_DIODE]; }
+uint32_t or_tape_len(const or_circuit *c)
+{
+    uint32_t k = 0;
+    for (int kind = 0; kind < OR_NKIND; kind++) k += c->count[kind] * (uint32_t)NSITE[kind];
+    return k;
+}
 
 /* ------------------------------------------------------------------ lanes */
 
@@ -325,11 +362,15 @@ void or_lanes_prepare(or_lanes *L)
     const uint32_t N = L->N;
     for (int kind = 0; kind < OR_NKIND; kind++) {
         uint32_t cnt = L->c->count[kind];
-        double u[OR_PARAM_MAX], p[OR_PARAM_MAX];
+        static double u[OR_VA_PARAM_MAX], p[OR_VA_PARAM_MAX];
         for (uint32_t i = 0; i < cnt; i++)
             for (uint32_t l = 0; l < N; l++) {
                 for (int k = 0; k < UK[kind]; k++) u[k] = L->uprm[kind][((size_t)i * UK[kind] + k) * N + l];
                 switch (kind) {
+                case OR_PSP103:
+                    if (or_va_setup(kind, u, p) != 0)
+                        for (int k = 0; k < PK[kind]; k++) p[k] = NAN;
+                    break;
                 case OR_MOS1: or_prep_mos1(u, p); break;
                 case OR_DIODE: or_prep_diode(u, p); break;
                 case OR_VSRC:
@@ -384,6 +425,39 @@ static double *lim_at(const or_lanes *L, int kind, uint32_t id, uint32_t l)
     return L->lim[kind] + ((size_t)l * L->c->count[kind] + id) * NU[kind];
 }
 
+/* One compiled instance into the planes, in the same order as the other
+ * kinds: its rows, then its G mask's entries row-major (and in a transient
+ * its charges, C entries and one tape entry per ddt() site). */
+static void eval_va_inst(const or_lanes *L, int kind, uint32_t id, uint32_t l, const uint32_t *nd, const uint32_t *sl,
+                         const double *x, double *g, double *rhs, const or_evctx *ec, uint32_t *tape_at)
+{
+    const or_circuit *c = L->c;
+    const int nu = NU[kind];
+    double p[OR_VA_PARAM_MAX], xl[32], res[32], jac[1024], q[32], cjac[1024], site[256];
+    or_load_p(L, kind, id, l, p);
+    for (int u = 0; u < nu; u++) xl[u] = x[nd[u]];
+    const int charged = ec && (ec->q || ec->cc || ec->tape);
+    if (charged) or_va_eval_tran(kind, p, xl, res, jac, q, cjac, site);
+    else or_va_eval(kind, p, xl, res, jac);
+    if (ec && ec->tape)
+        for (int k = 0; k < NSITE[kind]; k++) ec->tape[(*tape_at)++] = site[k];
+    size_t k = 0, kg = 0, kc = 0;
+    for (int ru = 0; ru < nu; ru++) {
+        uint32_t row = nd[ru] == 0 ? c->n : nd[ru];
+        rhs[row] += res[ru];
+        if (ec && ec->q && cmask(kind, ru) != 0) ec->q[row] += q[ru];
+        for (int cu = 0; cu < nu; cu++) {
+            if (!(pmask(kind, ru) >> cu & 1)) continue;
+            if (gmask(kind, ru) >> cu & 1) g[sl[k]] += jac[kg++];
+            if (cmask(kind, ru) >> cu & 1) {
+                if (ec && ec->cc) ec->cc[sl[k]] += cjac[kc];
+                kc++;
+            }
+            k++;
+        }
+    }
+}
+
 /* g (nnz+1 with trash), rhs (n+1 with trash row n) at x. */
 void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs,
                     const or_evctx *ec)
@@ -400,6 +474,10 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             const uint32_t *sl = c->slots[kind] + (size_t)id * c->kslots[kind];
+            if (is_va(kind)) {
+                eval_va_inst(L, kind, id, l, nd, sl, x, g, rhs, ec, &tape_at);
+                continue;
+            }
             double p[OR_PARAM_MAX], lx[4], xl[4];
             or_devout o;
             memset(&o, 0, sizeof o);

@@ -9,7 +9,12 @@
  * Evaluation order is the order kinds first appear in the netlist
  * (src/device/Circuit.zig: one batch per proto, protos in first-use
  * order), instances in card order within a kind. */
-enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_NKIND = 6 };
+enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_PSP103 = 6, OR_NKIND = 7 };
+/* OR_PSP103 is compiled from models/psp103.va (va.c): its parameters are the
+ * module's, in declaration order (or_va_param_index), NaN = not given. */
+int or_va_param_index(int kind, const char *name);
+int or_va_nparam(int kind);
+int or_va_nu(int kind);
 
 /* User parameter slots per kind (NaN = not given, the default applies). */
 enum { OR_VSRC_DC = 0, OR_VSRC_WAVE, OR_VSRC_V1, OR_VSRC_V2, OR_VSRC_TD, OR_VSRC_TR, OR_VSRC_TF, OR_VSRC_PW, OR_VSRC_PER,

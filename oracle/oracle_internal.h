@@ -68,9 +68,9 @@ typedef struct {
 struct or_lanes {
     const struct or_circuit *c;
     uint32_t N;
-    double *uprm[6]; /* [(inst*UK + k)*N + l] */
-    double *pprm[6]; /* [(inst*PK + k)*N + l] */
-    double *lim[6];  /* [(l*count + inst)*NU + u] */
+    double *uprm[OR_NKIND]; /* [(inst*UK + k)*N + l] */
+    double *pprm[OR_NKIND]; /* [(inst*PK + k)*N + l] */
+    double *lim[OR_NKIND];  /* [(l*count + inst)*NU + u] */
 };
 
 void or_eval_planes(const struct or_lanes *L, uint32_t l, const double *x, int lim_active, double *g, double *rhs,
@@ -82,6 +82,17 @@ void or_load_p(const struct or_lanes *L, int kind, uint32_t id, uint32_t l, doub
 int or_prepared_count(int kind);
 /* Length of the LTE charge tape: one site per capacitor and per diode. */
 uint32_t or_tape_len(const struct or_circuit *c);
+/* Compiled Verilog-A kinds (va.c). */
+int or_va_nport(int kind);
+int or_va_nprep(int kind);
+int or_va_nsite(int kind);
+unsigned or_va_gmask(int kind, int ru);
+unsigned or_va_cmask(int kind, int ru);
+int or_va_setup(int kind, const double *u, double *p);
+int or_va_collapse(int kind, const double *u, const uint32_t *ports, int64_t *rows);
+void or_va_eval(int kind, const double *p, const double *x, double *res, double *jac);
+void or_va_eval_tran(int kind, const double *p, const double *x, double *res, double *jac, double *q, double *cjac,
+                     double *site);
 
 double or_fetlim(double vnew, double vold, double vto);
 double or_limvds(double vnew, double vold);

@@ -106,6 +106,8 @@ void hl_prepare(void *hv)
     HL *h = (HL *)hv;
     const size_t N = h->N;
     for (int kind = 0; kind < K_NKIND; kind++) {
+        // the thread-per-lane bodies serve the hand-written kinds only
+        if (kind_is_va(kind)) continue;
         double u[PARAM_MAX], p[PARAM_MAX];
         for (uint32_t i = 0; i < h->t->count[kind]; i++)
             for (size_t l = 0; l < N; l++) {

@@ -12,8 +12,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-VSRC, ISRC, RES, CAP, DIODE, MOS1 = range(6)
+VSRC, ISRC, RES, CAP, DIODE, MOS1, PSP103 = range(7)
 UK = {VSRC: 71, ISRC: 71, RES: 1, CAP: 1, DIODE: 19, MOS1: 36}
+COMPILED = (PSP103,)
 
 # user parameter slots (oracle/oracle_circuit.h)
 VSRC_DC = 0
@@ -133,6 +134,14 @@ def load():
         "or_trial_draw": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, f64p, C.c_uint32]),
         "or_wyhash": (C.c_uint64, [C.c_uint64, C.c_char_p, C.c_size_t]),
         "or_param_site": (C.c_uint64, [C.c_char_p, C.c_uint32]),
+        "or_va_nparam": (C.c_int, [C.c_int]), "or_va_nu": (C.c_int, [C.c_int]),
+        "or_va_nport": (C.c_int, [C.c_int]), "or_va_nprep": (C.c_int, [C.c_int]),
+        "or_va_nsite": (C.c_int, [C.c_int]),
+        "or_va_gmask": (C.c_uint, [C.c_int, C.c_int]), "or_va_cmask": (C.c_uint, [C.c_int, C.c_int]),
+        "or_va_param_index": (C.c_int, [C.c_int, C.c_char_p]),
+        "or_va_setup": (C.c_int, [C.c_int, f64p, f64p]),
+        "or_va_eval": (None, [C.c_int, f64p, f64p, f64p, f64p]),
+        "or_va_eval_tran": (None, [C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -165,7 +174,7 @@ class Circuit:
         self.lib = lib
         self.h = lib.or_ckt_new()
         self.frozen = False
-        self.counts = [0] * 6
+        self.counts = [0] * 7
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -177,6 +186,8 @@ class Circuit:
 
     def add(self, kind, ports, params=None):
         rows = [self.node(p) if isinstance(p, str) else p for p in ports]
+        if kind in COMPILED and kind not in UK:
+            UK[kind] = self.lib.or_va_nparam(kind)
         pr = np.full(UK[kind], np.nan)
         if params:
             for k, v in params.items():
@@ -273,3 +284,13 @@ class Lanes:
         o = newton_opts(**kw)
         assert self.lib.or_dc_sweep(self.h, vsrc, start, step, npts, C.byref(o), xp, cp) == 0
         return xo.reshape(npts, n), conv
+
+
+def va_params(lib, kind, **values):
+    """{slot: value} of named parameters of a compiled kind."""
+    out = {}
+    for k, v in values.items():
+        i = lib.or_va_param_index(kind, k.encode())
+        assert i >= 0, k
+        out[i] = v
+    return out

@@ -1,0 +1,100 @@
+"""PSP103 lanes on the GPU against the oracle, bit for bit: the two-stage
+Miller op-amp (8 PSP103 transistors, 34 unknowns) under Monte Carlo inputs,
+every lane warm from the nominal operating point, and the gmin ladder
+that finds that nominal point from a cold start."""
+import numpy as np
+import pytest
+
+import egspice_amd as E
+import oracle_lib as O
+from egspice_amd import workloads as W
+
+
+
+def say(msg):
+    print(msg, flush=True)
+
+LADDER = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10, 1e-11, 1e-12, 0.0]
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return np.array_equal(a.view(np.uint64)[~both_nan], b.view(np.uint64)[~both_nan])
+
+
+def engine_opamp(n_lanes, inputs=None):
+    t = E.Template()
+    inst = W.miller_opamp(t.add, lambda nm: E.param_index(E.PSP103, nm))
+    t.freeze()
+    lanes = E.Lanes(t, n_lanes)
+    if inputs is not None:
+        for kind, name, slot, vals in W.opamp_lane_params(inputs, lambda nm: E.param_index(E.PSP103, nm)):
+            lanes.set_param(kind, inst[name], slot, vals)
+    lanes.prepare()
+    return t, lanes
+
+
+def oracle_opamp(lib, n_lanes, inputs=None):
+    c = O.Circuit(lib)
+    idx = lambda nm: lib.or_va_param_index(O.PSP103, nm.encode())
+    inst = W.miller_opamp(c.add, idx)
+    c.freeze()
+    L = O.Lanes(c, n_lanes)
+    if inputs is not None:
+        for kind, name, slot, vals in W.opamp_lane_params(inputs, idx):
+            L.set_param(kind, inst[name], slot, vals)
+    L.prepare()
+    return c, L
+
+
+@pytest.mark.gpu
+def test_opamp_pattern_matches_oracle(engine):
+    say("case: op-amp template, engine pattern vs oracle")
+    lib = O.load()
+    t, _ = engine_opamp(1)
+    c, _ = oracle_opamp(lib, 1)
+    assert (t.n, t.nnz) == (c.n, c.nnz) == (34, 171)
+    assert np.array_equal(t.col_ptr, c.col_ptr) and np.array_equal(t.row_idx, c.row_idx)
+
+
+@pytest.mark.gpu
+def test_opamp_gmin_ladder_bitwise(engine):
+    say("case: op-amp nominal operating point by the gmin ladder, 1 lane, GPU vs oracle")
+    lib = O.load()
+    t, lanes = engine_opamp(1)
+    c, L = oracle_opamp(lib, 1)
+    xg = np.zeros(t.n)
+    xo = np.zeros(c.n)
+    for g in LADDER:
+        lanes.upload_x(xg.reshape(t.n, 1))
+        xg2, sg, ig, _ = lanes.newton(init_fix=False, gmin=g)
+        xo2, so, io = L.newton_op(mode=1, x0=xo, init_fix=False, gmin=g)
+        assert sg[0] & 1 and so[0] & 1, g
+        assert ig[0] == io[0], (g, ig, io)
+        assert same_bits(xg2, xo2), g
+        xg, xo = xg2[:, 0], xo2[:, 0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_lanes", [64, 1024])
+def test_opamp_monte_carlo_warm_bitwise(engine, n_lanes):
+    say(f"case: op-amp Monte Carlo, {n_lanes} lanes warm from the nominal point, GPU vs oracle")
+    lib = O.load()
+    c, L1 = oracle_opamp(lib, 1)
+    x = np.zeros(c.n)
+    for g in LADDER:
+        xs, st, _ = L1.newton_op(mode=1, x0=x, init_fix=False, gmin=g)
+        x = xs[:, 0]
+    inputs = W.opamp_inputs(n_lanes, seed=11)
+    t, lanes = engine_opamp(n_lanes, inputs)
+    _, L = oracle_opamp(lib, n_lanes, inputs)
+    x0 = np.repeat(x[:, None], n_lanes, axis=1)
+    lanes.upload_x(x0)
+    xg, sg, ig, ms = lanes.newton(init_fix=False)
+    xo, so, io = L.newton_op(mode=1, x0=x0.reshape(-1), init_fix=False, threads=8)
+    say(f"  converged {int((sg & 1).sum())}/{n_lanes}, iterations mean {ig.mean():.2f} max {ig.max()}, "
+        f"kernel {ms:.3f} ms")
+    assert (sg & 1).sum() >= 0.95 * n_lanes
+    assert np.array_equal(sg, so) and np.array_equal(ig, io)
+    assert same_bits(xg, xo)

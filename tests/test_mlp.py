@@ -98,8 +98,10 @@ def test_mlp_forward_matches_oracle(engine):
     err = np.abs(got - want) / scale
     say(f"  max |gpu - oracle| / std(y) = {err.max():.2e}")
     # FP32 MFMA sums against the restatement's FP64 sums of the same BF16
-    # operands: measured 4.4e-4 of std(y) at most
-    assert err.max() < 6e-4
+    # operands: where the two sums straddle a BF16 rounding boundary an
+    # activation lands one BF16 step (2^-8 relative) away; measured 1.0e-3
+    # of std(y) at most over 300 rows and 12 outputs
+    assert err.max() < 2e-3
     assert np.all(np.isfinite(got))
 
 
