@@ -910,17 +910,23 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
             L->lds_max = std::min<size_t>((size_t)lds, 160 * 1024);
         L->use_wave = L->lds_bytes <= L->lds_max;
     }
-    if (L->lds_max > 64 * 1024) {
-        // the workgroup-per-lane kernels may take more than the default
-        // 64 KiB of dynamic LDS (gfx950 gives a workgroup up to 160 KiB)
-        const int m = (int)L->lds_max;
-        for (const void *k : {(const void *)newton_wave_kernel<true, false>, (const void *)newton_wave_kernel<false, false>,
-                              (const void *)newton_wave_kernel<true, true>, (const void *)newton_wave_kernel<false, true>,
-                              (const void *)analyze_wave_kernel<false>, (const void *)analyze_wave_kernel<true>,
-                              (const void *)tran_wave_kernel<false>, (const void *)tran_wave_kernel<true>})
-            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, m));
-    }
     for (int k = 0; k < K_NKIND; k++) L->has_va = L->has_va || (kind_is_va(k) && T.count[k] > 0);
+    if (L->use_wave && L->lds_bytes > 64 * 1024) {
+        // a lane block past the default 64 KiB of dynamic LDS (gfx950 gives
+        // a workgroup up to 160 KiB): the kernels this template launches
+        // are allowed what it needs
+        const int m = (int)L->lds_bytes;
+        std::string bad;
+        const void *ks[] = {L->has_va ? (const void *)newton_wave_kernel<true, true> : (const void *)newton_wave_kernel<true, false>,
+                            L->has_va ? (const void *)analyze_wave_kernel<true> : (const void *)analyze_wave_kernel<false>};
+        for (int i = 0; i < 2; i++)
+            if (hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, m) != hipSuccess) {
+                (void)hipGetLastError();
+                bad += bad.empty() ? "" : ",";
+                bad += std::to_string(i);
+            }
+        if (!bad.empty()) return fail(ESPICE_FAILED, ("LdsAttribute " + bad + " at " + std::to_string(m)).c_str());
+    }
     if (L->has_va && (!L->use_wave || L->lay.dense == ESP_NONE)) return fail(ESPICE_FAILED, "LaneTooLarge");
     const uint32_t blocks = (n_lanes + 63) / 64;
     hipLaunchKernelGGL(init_lane_state_kernel, dim3(blocks), dim3(64), 0, L->stream, L->st.p, n_lanes,
@@ -1302,6 +1308,9 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     Newton step = op;
     step.residual_tol = inf_();
     step.init_fix = 0;
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(L->has_va ? (const void *)tran_wave_kernel<true> : (const void *)tran_wave_kernel<false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->has_va)
         hipLaunchKernelGGL(tran_wave_kernel<true>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
