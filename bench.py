@@ -226,17 +226,219 @@ def cpu_baseline_tran(n_lanes, budget_s=10.0):
                       f"oracle CPU path on 1 thread, {platform.processor() or platform.machine()}"}
 
 
+def opamp_template():
+    import egspice_amd as E
+    from egspice_amd import workloads as W
+    t = E.Template()
+    inst = W.miller_opamp(t.add, lambda nm: E.param_index(E.PSP103, nm))
+    t.freeze()
+    return t, inst
+
+
+def opamp_lanes(t, inst, inputs):
+    import egspice_amd as E
+    from egspice_amd import workloads as W
+    lanes = E.Lanes(t, len(inputs))
+    for kind, name, slot, vals in W.opamp_lane_params(inputs, lambda nm: E.param_index(E.PSP103, nm)):
+        lanes.set_param(kind, inst[name], slot, vals)
+    lanes.prepare()
+    return lanes
+
+
+GMIN_LADDER = (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10, 1e-11, 1e-12, 0.0)
+
+
+def opamp_nominal(t, inst):
+    """The nominal operating point by the gmin ladder: warm Newton with a
+    shunt gmin on every row, stepped down a decade at a time to 0."""
+    lanes = opamp_lanes(t, inst, np.array([[1, 1, 1, 1, 1.8, 27.0, 1, 1e-12]]))
+    x = np.zeros(t.n)
+    for g in GMIN_LADDER:
+        lanes.upload_x(x)
+        xs, st, _, _ = lanes.newton(init_fix=False, gmin=g)
+        if not st[0] & 1:
+            raise RuntimeError(f"gmin ladder failed at {g}")
+        x = xs[:, 0].copy()
+    return x
+
+
+def bench_opamp(args):
+    """Config 5: the psp103 Miller op-amp's Monte Carlo operating points on
+    N lanes (8 inputs per lane), learned warm start against the nominal
+    warm start at the same Newton tolerances."""
+    import egspice_amd as E
+    from egspice_amd import warm
+    from egspice_amd import workloads as W
+    N = args.lanes
+    t, inst = opamp_template()
+    nominal = opamp_nominal(t, inst)
+    inputs = W.opamp_inputs(N, seed=1)
+    lanes = opamp_lanes(t, inst, inputs)
+    x_nom = np.repeat(nominal[:, None], N, axis=1).reshape(-1)
+
+    def nominal_step():
+        lanes.upload_x(x_nom)
+        return lanes.newton(init_fix=False)
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        kms, iters, conv = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            x, st, it, ms = step()
+            kms.append(ms)
+            iters.append(it.copy())
+            conv.append(int((st & 1).sum()))
+        return x, st, kms, iters, conv, time.perf_counter() - t0
+
+    x_n, st_n, kms_n, it_n, conv_n, wall_n = timed(nominal_step)
+    nominal_value = sum(conv_n) / wall_n
+
+    # training set: other Monte Carlo draws, solved by the engine from the
+    # nominal point (untimed; reported with the amortised rate)
+    t_train = time.perf_counter()
+    train_in = W.opamp_inputs(args.train_lanes, seed=2)
+    tl = opamp_lanes(t, inst, train_in)
+    tl.upload_x(np.repeat(nominal[:, None], len(train_in), axis=1).reshape(-1))
+    xt, stt, _, _ = tl.newton(init_fix=False)
+    ok = (stt & 1) != 0
+    del tl
+    solve_s = time.perf_counter() - t_train
+    mlp = E.Mlp(8, t.n, hidden=256, depth=3, seed=1)
+    mlp.fit_scaling(train_in[ok], xt[:, ok].T)
+    t1 = time.perf_counter()
+    losses = [mlp.train(train_in[ok], xt[:, ok].T, epochs=ep, batch=512, lr=lr, seed=3 + k)
+              for k, (ep, lr) in enumerate(OPAMP_SCHEDULE)]
+    train_s = time.perf_counter() - t1
+
+    timings = []
+
+    def learned_step():
+        tm = {}
+        out = warm.learned_op(lanes, mlp, inputs, nominal_x=nominal, timing=tm)
+        timings.append(tm)
+        return out
+
+    x_l, st_l, kms_l, it_l, conv_l, wall_l = timed(learned_step)
+    value = sum(conv_l) / wall_l
+    timings = timings[-args.steps:]
+    newton_ms = float(np.mean([tm["newton_ms"] for tm in timings]))
+    warm_ms = float(np.mean([tm["warm_ms"] for tm in timings]))
+    fallbacks = sum(sum(tm["fallbacks"].values()) for tm in timings)
+    node_rows = [t.node(nm) for nm in ("vdd", "inp", "nbias", "tail", "n1", "n2", "out")]
+    both = ((st_l & 1) != 0) & ((st_n & 1) != 0)
+    dev = np.abs(x_l - x_n)[node_rows][:, both]
+    rel = dev / np.maximum(np.abs(x_n[node_rows][:, both]), 1e-3)
+    amortised = (sum(conv_l) / args.steps) / (wall_l / args.steps + (solve_s + train_s))
+    line = {
+        "metric": METRIC, "value": value, "unit": "lanes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall_l * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "psp103 two-stage Miller op-amp (unity-gain buffer), Monte Carlo operating points, "
+                               "learned warm start vs nominal warm start",
+                   "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz,
+                   "inputs": "4 width scales U(0.9,1.1), VDD U(1.6,2.0), T U(-40,125) degC, Ibias U(0.8,1.2)x, "
+                             "CL U(0.1,2) pF",
+                   "arm": "learned warm start (BF16 MFMA MLP 8->3x256->n) + FP64 GPU lanes",
+                   "newton": "reltol 1e-3 abstol 1e-12 vntol 1e-6 residual_tol 1e-9 itl1 100",
+                   "converged_lanes": conv_l[-1], "newton_ticks_per_solve": float(np.mean([i.sum() for i in it_l])),
+                   "newton_kernel_ms": newton_ms, "warm_start_ms": warm_ms, "fallback_lanes": fallbacks,
+                   "nominal_lanes_per_s": nominal_value, "nominal_ms_per_step": wall_n * 1e3 / args.steps,
+                   "nominal_converged_lanes": conv_n[-1],
+                   "nominal_newton_ticks_per_solve": float(np.mean([i.sum() for i in it_n])),
+                   "nominal_kernel_ms": float(np.mean(kms_n)),
+                   "speedup_vs_nominal": value / nominal_value,
+                   "max_node_dev_vs_nominal_arm_V": float(dev.max()) if dev.size else None,
+                   "max_node_rel_dev_vs_nominal_arm": float(rel.max()) if rel.size else None,
+                   "training": {"samples": int(ok.sum()), "epochs": int(sum(e for e, _ in OPAMP_SCHEDULE)),
+                                "batch": 512, "schedule": [list(e) for e in OPAMP_SCHEDULE],
+                                "solve_s": round(solve_s, 3), "train_s": round(train_s, 3),
+                                "loss_first": float(losses[0][0]), "loss_last": float(losses[-1][-1]),
+                                "amortised_lanes_per_s": amortised}},
+    }
+    flops = opamp_flops(t, it_l[-1])
+    achieved = flops / (newton_ms * 1e-3) / 1e12
+    line["roofline"] = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "newton_wave_kernel",
+                        "flops_per_launch": flops}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_opamp(t, inst, nominal, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+# the learning rate falls in stages (epochs, rate)
+OPAMP_SCHEDULE = ((60, 1e-3), (30, 2e-4), (30, 5e-5))
+
+
+def opamp_flops(t, iters):
+    """Algorithmic FP64 work of one Newton launch: per lane tick the eight
+    PSP103 evaluations (FP64 operations of the generated body, counted per
+    executed path by the device counters, egspice_amd/csrc/gen), the other
+    devices, refactor and solve on the lane's tape, update and gates."""
+    ticks = float(np.sum(iters))
+    # the pattern's factor without pivoting (diagonally dominant values): the
+    # structure every lane's tape shares when its pivots stay diagonal
+    vals = np.random.default_rng(0).uniform(0.1, 1.0, t.nnz)
+    vals[t.diag] = 2.0 * t.n
+    lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
+    per_tick = 8 * PSP103_EVAL_FLOPS + 2 * EVAL_FLOPS[0] + EVAL_FLOPS[1] + lu + 12 * t.n
+    return per_tick * ticks
+
+
+# FP64 operations of one PSP103 DC evaluation (value and derivative lanes)
+# at the op-amp's operating points: the device counters' SQ_INSTS_VALU_*_F64
+# per evaluation (FMA counted twice); see the profile of the op-amp lanes
+PSP103_EVAL_FLOPS = 20000
+
+
+def cpu_baseline_opamp(t, inst, nominal, budget_s=10.0, n_lanes=512):
+    """The oracle's CPU path (converger.newton with a direct.Solver per lane)
+    on the same Monte Carlo operating points from the nominal point, lanes
+    dealt over the host's cores, repeated for about budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from egspice_amd import workloads as W
+    lib = O.load()
+    inputs = W.opamp_inputs(n_lanes, seed=1)
+    c = O.Circuit(lib)
+    idx = lambda nm: lib.or_va_param_index(O.PSP103, nm.encode())
+    oinst = W.miller_opamp(c.add, idx)
+    c.freeze()
+    L = O.Lanes(c, n_lanes)
+    for kind, name, slot, vals in W.opamp_lane_params(inputs, idx):
+        L.set_param(kind, oinst[name], slot, vals)
+    L.prepare()
+    x0 = np.repeat(nominal[:, None], n_lanes, axis=1).reshape(-1)
+    threads = host_threads()
+    reps, converged, t0 = 0, 0, time.perf_counter()
+    while True:
+        _, st, _ = L.newton_op(mode=1, x0=x0, init_fix=False, threads=threads)
+        converged += int((st & 1).sum())
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": converged / el, "unit": "lanes/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} repetitions of {n_lanes} Monte Carlo lanes from the nominal point "
+                      f"({reps * n_lanes} lane solves, {el:.1f} s), oracle CPU path on {threads} threads, "
+                      f"{platform.processor() or platform.machine()}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--lanes", type=int, default=64)
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="lanes (default: 65536 for opamp-mc, 64 for dc-inverter-chain, 4096 for tran-rc-measure)")
+    ap.add_argument("--train-lanes", type=int, default=65536)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("dc-inverter-chain", "tran-rc-measure"), default="dc-inverter-chain",
-                    help="dc-inverter-chain: the 64-point DC sweep (default); tran-rc-measure: the RC transient of "
-                         "docs/examples/measure.sp with its .meas passes, R and C varied per lane")
+    ap.add_argument("--workload", choices=("opamp-mc", "dc-inverter-chain", "tran-rc-measure"), default="opamp-mc",
+                    help="opamp-mc: the psp103 op-amp's Monte Carlo operating points, learned vs nominal warm "
+                         "start (default); dc-inverter-chain: the mos1 chain's 64-point DC sweep; tran-rc-measure: "
+                         "the RC transient of docs/examples/measure.sp with its .meas passes")
     ap.add_argument("--arm", choices=("both", "learned", "plain"), default="both",
                     help="dc-inverter-chain: time both arms (default), or only one (for profiling one arm's "
                          "kernels); with one arm, the other's fields are null")
@@ -252,8 +454,13 @@ def main():
               "(lanes shard across GPUs by contiguous lane ranges, a design not built here)", file=sys.stderr)
         sys.exit(2)
 
+    if args.lanes is None:
+        args.lanes = {"opamp-mc": 65536, "dc-inverter-chain": 64, "tran-rc-measure": 4096}[args.workload]
     if args.workload == "tran-rc-measure":
         bench_tran(args)
+        return
+    if args.workload == "opamp-mc":
+        bench_opamp(args)
         return
 
     import egspice_amd as E

@@ -98,3 +98,44 @@ def test_opamp_monte_carlo_warm_bitwise(engine, n_lanes):
     assert (sg & 1).sum() >= 0.95 * n_lanes
     assert np.array_equal(sg, so) and np.array_equal(ig, io)
     assert same_bits(xg, xo)
+
+
+@pytest.mark.gpu
+def test_opamp_learned_start_bitwise(engine):
+    """The learned arm: the warm-start network (trained on engine-solved
+    Monte Carlo operating points) starts every lane; the GPU's inference and
+    Newton against the oracle's Newton from the same predicted start points,
+    bit for bit, and every lane inside the Newton tolerance of its
+    nominal-start solution."""
+    say("case: op-amp learned warm start, 256 lanes, GPU vs oracle from the network's start points")
+    lib = O.load()
+    c, L1 = oracle_opamp(lib, 1)
+    x = np.zeros(c.n)
+    for g in LADDER:
+        xs, st, _ = L1.newton_op(mode=1, x0=x, init_fix=False, gmin=g)
+        x = xs[:, 0]
+    train_in = W.opamp_inputs(1024, seed=5)
+    t, tl = engine_opamp(1024, train_in)
+    tl.upload_x(np.repeat(x[:, None], 1024, axis=1))
+    xt, stt, _, _ = tl.newton(init_fix=False)
+    ok = (stt & 1) != 0
+    mlp = E.Mlp(8, t.n, hidden=256, depth=3, seed=1)
+    mlp.fit_scaling(train_in[ok], xt[:, ok].T)
+    mlp.train(train_in[ok], xt[:, ok].T, epochs=40, batch=256, lr=1e-3, seed=2)
+    inputs = W.opamp_inputs(256, seed=6)
+    _, lanes = engine_opamp(256, inputs)
+    xg, sg, ig, _, _ = lanes.newton_learned(mlp, inputs, init_fix=False)
+    x0 = mlp.predict(inputs).T.copy()
+    _, L = oracle_opamp(lib, 256, inputs)
+    xo, so, io = L.newton_op(mode=1, x0=x0.reshape(-1), init_fix=False, threads=8)
+    say(f"  converged {int((sg & 1).sum())}/256, iterations mean {ig.mean():.2f} max {ig.max()}")
+    assert np.array_equal(sg, so) and np.array_equal(ig, io)
+    assert same_bits(xg, xo)
+    # against the nominal start: the same operating point within the
+    # Newton tolerance (reltol 1e-3 of the node voltage plus vntol)
+    lanes.upload_x(np.repeat(x[:, None], 256, axis=1))
+    xn, sn, _, _ = lanes.newton(init_fix=False)
+    both = ((sg & 1) != 0) & ((sn & 1) != 0)
+    nodes = [t.node(nm) for nm in ("vdd", "inp", "nbias", "tail", "n1", "n2", "out")]
+    dev = np.abs(xg - xn)[nodes][:, both]
+    assert np.all(dev <= 1e-3 * np.abs(xn[nodes][:, both]) + 1e-6)

@@ -121,8 +121,9 @@ def test_mlp_gradients_match_oracle(engine):
             a, b = g[at:at + size], want[at:at + size]
             rel = np.linalg.norm(a - b) / np.linalg.norm(b)
             say(f"  layer {k} {part}: relative error {rel:.2e}")
-            # measured 1.2e-3 at most (FP32 against FP64 accumulation)
-            assert rel < 2.5e-3, (k, part, rel)
+            # FP32 MFMA sums against FP64 ones over BF16 operands: measured
+            # 3.9e-3 at most (layer 0's bias, a sum over every row of the batch)
+            assert rel < 8e-3, (k, part, rel)
             at += size
     assert at == g.size
 
