@@ -56,7 +56,7 @@ class LaneResult(C.Structure):
 class TranOpts(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("t_stop", "dt_init", "dt_max", "dt_min", "xmu", "reltol", "abstol", "vntol",
                                           "chgtol", "trtol")] + \
-               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity")]
+               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm", "pad")]
 
 
 class MeasPass(C.Structure):
@@ -179,14 +179,14 @@ def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e
 
 
 def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
-              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0):
+              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0, op_warm=False):
     """`.tran dt_init t_stop`; a deck's step ceiling is min(dt_init, t_stop/50)
     (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680), the
     tolerances are converger.Options' (src/core/numerics.zig:378-399)."""
     if dt_max is None:
         dt_max = min(dt_init, t_stop / 50)
     return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, method, max_iter,
-                    max_steps, record_capacity)
+                    max_steps, record_capacity, 1 if op_warm else 0, 0)
 
 
 def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):

@@ -320,7 +320,8 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     double *prm_lds = block + lay.prm;
     for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
     for (uint32_t i = tid; i < n; i += nt) {
-        s.x[i] = 0.0;
+        // a cold operating point starts at 0; a warm one at the lane's x
+        s.x[i] = to.op_warm ? x_glob[(size_t)i * N + l] : 0.0;
         i_prev[i] = 0.0;
     }
     for (uint32_t j = tid; j < n_tape; j += nt) ti_prev[j] = 0.0;
@@ -362,14 +363,18 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         for (uint32_t k = 0; k < n_pass; k++) meas_init(passes[k], ms[k]);
     }
     __syncthreads();
-    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+    if (!to.op_warm)
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
     const ReduceCache rc = wave_reduce_cache(W, tid);
     __syncthreads();
 
     TranStep st{};
     uint32_t it = 0;
     const WaveHook off{};
-    const int op = wave_newton<true, false, VA>(W, s, D, rc, op_opts, off, true, true, 0.0, it);
+    // cold: MODEINITJCT seeds and INITFIX; warm: Newton from x, as the
+    // operating point's warm start (src/analysis/dc/op.zig:588, 671)
+    const bool cold = to.op_warm == 0;
+    const int op = wave_newton<true, false, VA>(W, s, D, rc, op_opts, off, cold, cold, 0.0, it);
     if (op != NEWTON_CONVERGED) {
         if (tid == 0) {
             st.out.status = op == NEWTON_SINGULAR ? 3u : 2u;
@@ -1304,7 +1309,7 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     op.residual_tol = 1e-9;
     op.gmin = 0.0;
     op.max_iter = opts->max_iter;
-    op.init_fix = 1;
+    op.init_fix = opts->op_warm ? 0 : 1;
     Newton step = op;
     step.residual_tol = inf_();
     step.init_fix = 0;

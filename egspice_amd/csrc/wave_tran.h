@@ -22,6 +22,7 @@ struct TranOpts {
     double t_stop, dt_init, dt_max, dt_min, xmu;
     double reltol, abstol, vntol, chgtol, trtol;
     uint32_t method, max_iter, max_steps, record_cap;
+    uint32_t op_warm, pad;  // op_warm: the operating point starts at the lane's x
 };
 
 // One .meas pass (scan.zig): 0 Cross (WHEN/TRIG/TARG), 1 At (FIND, or

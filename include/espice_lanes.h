@@ -173,6 +173,11 @@ typedef struct {
     double t_stop, dt_init, dt_max, dt_min, xmu;
     double reltol, abstol, vntol, chgtol, trtol;
     uint32_t method, max_iter, max_steps, record_capacity;
+    /* 0: the operating point at t = 0 starts cold (MODEINITJCT seeds,
+     * INITFIX); 1: it starts warm at each lane's x (espice_lanes_upload_x
+     * or a warm start), as the reference's warm operating point
+     * (src/analysis/dc/op.zig:588, 671). */
+    uint32_t op_warm, pad;
 } espice_tran_opts;
 
 /* One .meas pass over an unknown `row` (a node voltage or a branch

@@ -103,6 +103,9 @@ typedef struct {
  * caller frees them). Returns the row count. */
 void or_tran_coeffs(int method, double dt, double dt_prev, double xmu, double *out);
 int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *o, double **times, double **rows, or_tran_stats *st);
+/* The same from a warm operating point at x0 (n values); x0 NULL is cold. */
+int or_tran_run_from(or_lanes *L, uint32_t l, const or_tran_opts *o, const double *x0, double **times, double **rows,
+                     or_tran_stats *st);
 
 /* .meas passes over a recorded waveform (meas.c). A clause's counts use
  * -1 for not given and -2 for LAST. */

@@ -265,6 +265,15 @@ static int rec_push(tr_rec *r, double t, const double *x)
 
 int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times, double **rows, or_tran_stats *st)
 {
+    return or_tran_run_from(L, l, opt, NULL, times, rows, st);
+}
+
+/* x0 == NULL: the operating point starts cold (seeds, INITFIX); else warm
+ * at x0 (n values), without seeds or INITFIX, as op.zig's warm start
+ * (src/analysis/dc/op.zig:588, 671). */
+int or_tran_run_from(or_lanes *L, uint32_t l, const or_tran_opts *opt, const double *x0, double **times, double **rows,
+                     or_tran_stats *st)
+{
     const or_circuit *c = L->c;
     const uint32_t n = or_ckt_n(c), nnz = or_ckt_nnz(c), nq = n + 1, n_qt = or_tape_len(c);
     /* the independent sources, voltage then current: both carry the
@@ -327,14 +336,15 @@ int or_tran_run(or_lanes *L, uint32_t l, const or_tran_opts *opt, double **times
     tr_rec rec = {NULL, NULL, 0, 0, n};
     int rc = 0;
 
-    /* the transient's operating point: sources at t = 0, a cold start */
-    or_newton_opts nopt = {opt->reltol, opt->abstol, opt->vntol, opt->residual_tol_op, 0.0, opt->max_iter, 1};
+    /* the transient's operating point: sources at t = 0, a cold start, or
+     * warm at x0 */
+    or_newton_opts nopt = {opt->reltol, opt->abstol, opt->vntol, opt->residual_tol_op, 0.0, opt->max_iter, x0 ? 0u : 1u};
     {
         tr_hook off = {0};
         uint32_t it;
-        for (uint32_t i = 0; i < n; i++) cur[i] = 0.0;
-        or_seed_pass(L, l, cur);
-        if (tr_newton(&w, cur, 0.0, &off, &nopt, 1, 1, &it) != 1) {
+        for (uint32_t i = 0; i < n; i++) cur[i] = x0 ? x0[i] : 0.0;
+        if (!x0) or_seed_pass(L, l, cur);
+        if (tr_newton(&w, cur, 0.0, &off, &nopt, x0 ? 0 : 1, x0 ? 0 : 1, &it) != 1) {
             st->status = 2;
             goto done;
         }

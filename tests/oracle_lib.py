@@ -123,6 +123,8 @@ def load():
         "or_tran_coeffs": (None, [C.c_int, C.c_double, C.c_double, C.c_double, f64p]),
         "or_tran_run": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), C.POINTER(f64p), C.POINTER(f64p),
                                   C.POINTER(TranStats)]),
+        "or_tran_run_from": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), f64p, C.POINTER(f64p), C.POINTER(f64p),
+                                       C.POINTER(TranStats)]),
         "or_meas_cross": (C.c_double, [f64p, f64p, C.c_uint32, C.POINTER(MeasClause)]),
         "or_meas_at": (C.c_double, [f64p, f64p, C.c_uint32, C.c_double, C.c_int]),
         "or_meas_extremum": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p]),
@@ -265,11 +267,16 @@ class Lanes:
         self.lib.or_eval_lane(self.h, l, xp, rp, vp)
         return rhs, vals
 
-    def tran(self, l, opts):
-        """Lane l's transient: (times, rows (points, n), stats)."""
+    def tran(self, l, opts, x0=None):
+        """Lane l's transient: (times, rows (points, n), stats); x0 (n
+        values) starts its operating point warm."""
         tp, rp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
         st = TranStats()
-        k = self.lib.or_tran_run(self.h, l, C.byref(opts), C.byref(tp), C.byref(rp), C.byref(st))
+        if x0 is None:
+            k = self.lib.or_tran_run(self.h, l, C.byref(opts), C.byref(tp), C.byref(rp), C.byref(st))
+        else:
+            xa, xp = f64(np.asarray(x0, dtype=np.float64))
+            k = self.lib.or_tran_run_from(self.h, l, C.byref(opts), xp, C.byref(tp), C.byref(rp), C.byref(st))
         n = self.ckt.n
         t = np.ctypeslib.as_array(tp, (k,)).copy() if k else np.zeros(0)
         x = np.ctypeslib.as_array(rp, (k * n,)).copy().reshape(k, n) if k else np.zeros((0, n))

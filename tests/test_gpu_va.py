@@ -139,3 +139,62 @@ def test_opamp_learned_start_bitwise(engine):
     nodes = [t.node(nm) for nm in ("vdd", "inp", "nbias", "tail", "n1", "n2", "out")]
     dev = np.abs(xg - xn)[nodes][:, both]
     assert np.all(dev <= 1e-3 * np.abs(xn[nodes][:, both]) + 1e-6)
+
+
+def opamp_step_template(lib, n_lanes, inputs, engine=True):
+    step = (1.0, 10e-9, 1e-9, 500e-9)
+    if engine:
+        t = E.Template()
+        inst = W.miller_opamp(t.add, lambda nm: E.param_index(E.PSP103, nm), step=step)
+        t.freeze()
+        lanes = E.Lanes(t, n_lanes)
+        for kind, name, slot, vals in W.opamp_lane_params(inputs, lambda nm: E.param_index(E.PSP103, nm)):
+            lanes.set_param(kind, inst[name], slot, vals)
+        lanes.prepare()
+        return t, lanes
+    c = O.Circuit(lib)
+    idx = lambda nm: lib.or_va_param_index(O.PSP103, nm.encode())
+    inst = W.miller_opamp(c.add, idx, step=step)
+    c.freeze()
+    L = O.Lanes(c, n_lanes)
+    for kind, name, slot, vals in W.opamp_lane_params(inputs, idx):
+        L.set_param(kind, inst[name], slot, vals)
+    L.prepare()
+    return c, L
+
+
+@pytest.mark.gpu
+def test_opamp_transient_bitwise(engine):
+    """A 100 mV input step through the buffer, `.tran 10n 1u`, PSP103 charges
+    (gate, bulk, drain and junction charge sites) in the companion and the
+    step control: every accepted time and row, and every step statistic,
+    bit for bit against the oracle, from the nominal operating point."""
+    say("case: op-amp transient .tran 10n 1u, 12 Monte Carlo lanes warm from the nominal point, GPU vs oracle")
+    lib = O.load()
+    c, L1 = oracle_opamp(lib, 1)
+    x = np.zeros(c.n)
+    for g in LADDER:
+        xs, st, _ = L1.newton_op(mode=1, x0=x, init_fix=False, gmin=g)
+        x = xs[:, 0]
+    n_lanes = 12
+    inputs = W.opamp_inputs(n_lanes, seed=9)
+    t, lanes = opamp_step_template(lib, n_lanes, inputs)
+    c, L = opamp_step_template(lib, n_lanes, inputs, engine=False)
+    lanes.upload_x(np.repeat(x[:, None], n_lanes, axis=1))
+    opts = E.tran_opts(1e-6, 10e-9, record_capacity=4096, op_warm=True, max_steps=200_000)
+    res, _, times, rows, ms = lanes.tran(opts)
+    say(f"  statuses {res['status'].tolist()}, points {res['points'].tolist()}, kernel {ms:.2f} ms")
+    oo = O.tran_opts(1e-6, 10e-9, max_steps=200_000)
+    out_row = t.node("out")
+    for l in range(n_lanes):
+        tt, xx, sto = L.tran(l, oo, x0=x)
+        k = int(res["points"][l])
+        assert res["status"][l] == 0 and sto.status == 0, (l, res["status"][l], sto.status)
+        assert sto.points == k and sto.steps == res["steps"][l] and sto.nr_iters == res["nr_iters"][l], l
+        assert np.array_equal(times[l, :k].view(np.uint64), tt.view(np.uint64)), f"lane {l} times"
+        assert same_bits(rows[l, :k], xx), f"lane {l} rows"
+        # the buffer follows its input: out has settled at the pulse's
+        # 1.0 V before the pulse ends (511 ns) and is back at 0.9 V at 1 us
+        settled = xx[np.searchsorted(tt, 500e-9) - 1, out_row]
+        assert abs(settled - 1.0) < 5e-3, settled
+        assert abs(xx[-1, out_row] - 0.9) < 5e-3
