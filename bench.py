@@ -226,11 +226,14 @@ def cpu_baseline_tran(n_lanes, budget_s=10.0):
                       f"oracle CPU path on 1 thread, {platform.processor() or platform.machine()}"}
 
 
-def opamp_template():
+OPAMP_STEP = (1.0, 10e-9, 1e-9, 500e-9)  # VINP PULSE(0.9 1.0 10n 1n 1n 500n 1)
+
+
+def opamp_template(step=OPAMP_STEP):
     import egspice_amd as E
     from egspice_amd import workloads as W
     t = E.Template()
-    inst = W.miller_opamp(t.add, lambda nm: E.param_index(E.PSP103, nm))
+    inst = W.miller_opamp(t.add, lambda nm: E.param_index(E.PSP103, nm), step=step)
     t.freeze()
     return t, inst
 
@@ -262,38 +265,62 @@ def opamp_nominal(t, inst):
     return x
 
 
-def bench_opamp(args):
-    """Config 5: the psp103 Miller op-amp's Monte Carlo operating points on
-    N lanes (8 inputs per lane), learned warm start against the nominal
-    warm start at the same Newton tolerances."""
+def opamp_passes(t):
+    """The op-amp's .meas passes: .meas tran vmax MAX v(out); .meas tran
+    tpd WHEN v(out)=0.95 RISE=1 (the 50% point of the 100 mV step)."""
     import egspice_amd as E
-    from egspice_amd import warm
+    out = t.node("out")
+    return [E.meas_extremum(out, 1), E.meas_cross(out, 0.95, rise=1)]
+
+
+def bench_opamp(args):
+    """Config 5: the psp103 Miller op-amp's Monte Carlo lanes, each an
+    operating point and a 1 us transient (`.tran 10n 1u`, a 100 mV input
+    pulse, two .meas passes), learned warm start of the operating point
+    against the nominal warm start at the same Newton tolerances. One step
+    is one launch of every lane's whole analysis."""
+    import egspice_amd as E
     from egspice_amd import workloads as W
     N = args.lanes
     t, inst = opamp_template()
     nominal = opamp_nominal(t, inst)
     inputs = W.opamp_inputs(N, seed=1)
     lanes = opamp_lanes(t, inst, inputs)
+    passes = opamp_passes(t)
     x_nom = np.repeat(nominal[:, None], N, axis=1).reshape(-1)
+    topts = E.tran_opts(1e-6, 10e-9, op_warm=True, max_steps=200_000)
+
+    def lane_ok(res, meas):
+        return (res["status"] == 0) & np.all(np.isfinite(meas[:, :, 0]), axis=1)
 
     def nominal_step():
         lanes.upload_x(x_nom)
-        return lanes.newton(init_fix=False)
+        res, meas, _, _, ms = lanes.tran(topts, passes)
+        return res, meas, ms, 0.0
 
     def timed(step):
         for _ in range(args.warmup):
             step()
-        kms, iters, conv = [], [], []
+        kms, wms, ticks, conv = [], [], [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            x, st, it, ms = step()
+            res, meas, ms, wm = step()
             kms.append(ms)
-            iters.append(it.copy())
-            conv.append(int((st & 1).sum()))
-        return x, st, kms, iters, conv, time.perf_counter() - t0
+            wms.append(wm)
+            ticks.append(float(res["nr_iters"].sum()))
+            conv.append(int(lane_ok(res, meas).sum()))
+        return res, meas, kms, wms, ticks, conv, time.perf_counter() - t0
 
-    x_n, st_n, kms_n, it_n, conv_n, wall_n = timed(nominal_step)
+    res_n, meas_n, kms_n, _, ticks_n, conv_n, wall_n = timed(nominal_step)
     nominal_value = sum(conv_n) / wall_n
+
+    # operating points alone, from the nominal point (the training set's
+    # solver and the OP-only comparison)
+    def op_nominal():
+        lanes.upload_x(x_nom)
+        return lanes.newton(init_fix=False)
+
+    op_n = [op_nominal() for _ in range(2)][-1]
 
     # training set: other Monte Carlo draws, solved by the engine from the
     # nominal point (untimed; reported with the amortised rate)
@@ -312,56 +339,61 @@ def bench_opamp(args):
               for k, (ep, lr) in enumerate(OPAMP_SCHEDULE)]
     train_s = time.perf_counter() - t1
 
-    timings = []
-
     def learned_step():
-        tm = {}
-        out = warm.learned_op(lanes, mlp, inputs, nominal_x=nominal, timing=tm)
-        timings.append(tm)
-        return out
+        wm = lanes.warm_start(mlp, inputs)
+        res, meas, _, _, ms = lanes.tran(topts, passes)
+        return res, meas, ms, wm
 
-    x_l, st_l, kms_l, it_l, conv_l, wall_l = timed(learned_step)
+    res_l, meas_l, kms_l, wms_l, ticks_l, conv_l, wall_l = timed(learned_step)
     value = sum(conv_l) / wall_l
-    timings = timings[-args.steps:]
-    newton_ms = float(np.mean([tm["newton_ms"] for tm in timings]))
-    warm_ms = float(np.mean([tm["warm_ms"] for tm in timings]))
-    fallbacks = sum(sum(tm["fallbacks"].values()) for tm in timings)
+
+    # operating points alone from the network's start
+    op_l = [lanes.newton_learned(mlp, inputs, init_fix=False) for _ in range(2)][-1]
+    x_l, st_l = op_l[0], op_l[1]
+    x_n, st_n = op_n[0], op_n[1]
     node_rows = [t.node(nm) for nm in ("vdd", "inp", "nbias", "tail", "n1", "n2", "out")]
     both = ((st_l & 1) != 0) & ((st_n & 1) != 0)
     dev = np.abs(x_l - x_n)[node_rows][:, both]
-    rel = dev / np.maximum(np.abs(x_n[node_rows][:, both]), 1e-3)
+    ok_both = lane_ok(res_l, meas_l) & lane_ok(res_n, meas_n)
+    meas_dev = np.abs(meas_l[ok_both, :, 0] - meas_n[ok_both, :, 0]) / np.maximum(np.abs(meas_n[ok_both, :, 0]), 1e-30)
     amortised = (sum(conv_l) / args.steps) / (wall_l / args.steps + (solve_s + train_s))
     line = {
         "metric": METRIC, "value": value, "unit": "lanes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall_l * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "psp103 two-stage Miller op-amp (unity-gain buffer), Monte Carlo operating points, "
+        "config": {"workload": "psp103 two-stage Miller op-amp (unity-gain buffer, 34 unknowns) Monte Carlo: "
+                               "operating point + .tran 10n 1u with a 100 mV input pulse and 2 .meas passes per lane; "
                                "learned warm start vs nominal warm start",
                    "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz,
                    "inputs": "4 width scales U(0.9,1.1), VDD U(1.6,2.0), T U(-40,125) degC, Ibias U(0.8,1.2)x, "
-                             "CL U(0.1,2) pF",
-                   "arm": "learned warm start (BF16 MFMA MLP 8->3x256->n) + FP64 GPU lanes",
+                             "CL U(0.1,2) pF, seeded",
+                   "arm": "learned warm start (BF16 MFMA MLP 8->3x256->34) of the operating point + FP64 GPU "
+                          "transient lanes",
                    "newton": "reltol 1e-3 abstol 1e-12 vntol 1e-6 residual_tol 1e-9 itl1 100",
-                   "converged_lanes": conv_l[-1], "newton_ticks_per_solve": float(np.mean([i.sum() for i in it_l])),
-                   "newton_kernel_ms": newton_ms, "warm_start_ms": warm_ms, "fallback_lanes": fallbacks,
+                   "converged_lanes": conv_l[-1],
+                   "newton_ticks_per_lane": ticks_l[-1] / N, "tran_kernel_ms": float(np.mean(kms_l)),
+                   "warm_start_ms": float(np.mean(wms_l)),
+                   "points_per_lane": float(res_l["points"].mean()),
                    "nominal_lanes_per_s": nominal_value, "nominal_ms_per_step": wall_n * 1e3 / args.steps,
-                   "nominal_converged_lanes": conv_n[-1],
-                   "nominal_newton_ticks_per_solve": float(np.mean([i.sum() for i in it_n])),
-                   "nominal_kernel_ms": float(np.mean(kms_n)),
+                   "nominal_converged_lanes": conv_n[-1], "nominal_newton_ticks_per_lane": ticks_n[-1] / N,
+                   "nominal_tran_kernel_ms": float(np.mean(kms_n)),
                    "speedup_vs_nominal": value / nominal_value,
-                   "max_node_dev_vs_nominal_arm_V": float(dev.max()) if dev.size else None,
-                   "max_node_rel_dev_vs_nominal_arm": float(rel.max()) if rel.size else None,
+                   "meas_max_rel_dev_learned_vs_nominal": float(meas_dev.max()) if meas_dev.size else None,
+                   "op_only": {"learned_newton_ms": op_l[4], "learned_inference_ms": op_l[3],
+                               "learned_ticks_per_lane": float(op_l[2].sum()) / N,
+                               "nominal_newton_ms": op_n[3], "nominal_ticks_per_lane": float(op_n[2].sum()) / N,
+                               "max_node_dev_learned_vs_nominal_V": float(dev.max()) if dev.size else None},
                    "training": {"samples": int(ok.sum()), "epochs": int(sum(e for e, _ in OPAMP_SCHEDULE)),
                                 "batch": 512, "schedule": [list(e) for e in OPAMP_SCHEDULE],
                                 "solve_s": round(solve_s, 3), "train_s": round(train_s, 3),
                                 "loss_first": float(losses[0][0]), "loss_last": float(losses[-1][-1]),
                                 "amortised_lanes_per_s": amortised}},
     }
-    flops = opamp_flops(t, it_l[-1])
-    achieved = flops / (newton_ms * 1e-3) / 1e12
+    flops = opamp_tran_flops(t, res_l, len(passes))
+    achieved = flops / (float(np.mean(kms_l)) * 1e-3) / 1e12
     line["roofline"] = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "newton_wave_kernel",
-                        "flops_per_launch": flops}
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(f"tran_wave_kernel_opamp_{N}"),
+                        "kernel": "tran_wave_kernel", "flops_per_launch": flops}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_opamp(t, inst, nominal, args.cpu_budget)
     print(json.dumps(line), flush=True)
@@ -369,6 +401,21 @@ def bench_opamp(args):
 
 # the learning rate falls in stages (epochs, rate)
 OPAMP_SCHEDULE = ((60, 1e-3), (30, 2e-4), (30, 5e-5))
+
+
+def opamp_tran_flops(t, res, n_passes):
+    """Algorithmic FP64 work of one transient launch: per Newton tick the
+    eight PSP103 evaluations with charges, the other devices, the companion
+    rows and a = g + ag0 c, refactor + solve, update and gates; per accepted
+    step the charges re-read at the solution, the LTE bound over the charge
+    sites, the predictor and currents and the passes."""
+    vals = np.random.default_rng(0).uniform(0.1, 1.0, t.nnz)
+    vals[t.diag] = 2.0 * t.n
+    lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
+    sites = 8 * PSP103_SITES + 2
+    per_tick = 8 * PSP103_TRAN_EVAL_FLOPS + 2 * EVAL_FLOPS[0] + EVAL_FLOPS[1] + 5 * t.n + 2 * t.nnz + lu + 12 * t.n
+    per_step = 8 * PSP103_TRAN_EVAL_FLOPS + 30 * sites + 7 * t.n + 10 * n_passes
+    return float(res["nr_iters"].sum()) * per_tick + float(res["steps"].sum()) * per_step
 
 
 def opamp_flops(t, iters):
@@ -390,6 +437,8 @@ def opamp_flops(t, iters):
 # at the op-amp's operating points: the device counters' SQ_INSTS_VALU_*_F64
 # per evaluation (FMA counted twice); see the profile of the op-amp lanes
 PSP103_EVAL_FLOPS = 20000
+PSP103_TRAN_EVAL_FLOPS = 24000
+PSP103_SITES = 11
 
 
 def cpu_baseline_opamp(t, inst, nominal, budget_s=10.0, n_lanes=512):

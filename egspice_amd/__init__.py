@@ -378,7 +378,8 @@ class Lanes:
 
 
     def tran(self, opts, passes=(), record_capacity=None):
-        """Every lane's transient from a cold operating point.
+        """Every lane's transient (its operating point cold, or warm at the
+        lanes' x with opts.op_warm).
 
         Returns (results (N,) TranResult fields as a dict of arrays, meas
         (N, P, 3), times (N, cap), rows (N, cap, n), kernel_ms); times/rows

@@ -60,7 +60,9 @@ bool is_tridiag(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
 // `at`) that a workgroup copies into LDS before its first tick.
 struct InstEntry;
 struct WaveBlob {
-    uint32_t n, nnz, cap, n_inst, n_stage, n_prm, n_lim, tri, n_tape, _pad;
+    // n_prm_lds: the leading prepared constants a workgroup copies into LDS
+    // (every kind but the compiled ones, whose constants stay in HBM)
+    uint32_t n, nnz, cap, n_inst, n_stage, n_prm, n_lim, tri, n_tape, n_prm_lds;
     uint64_t diag_fill;
     uint32_t col_ptr, row_idx, q, tri_pos, diag, cur_row, inst, nodes, slots, node_off, slot_off, rptr, ridx, gptr,
         gidx, qptr, qidx, cptr, cidx, words;
@@ -68,7 +70,7 @@ struct WaveBlob {
 struct WaveHost {
     std::vector<uint8_t> inst_bytes;  // InstEntry array
     std::vector<uint32_t> nodes_flat, slots_flat, node_off, slot_off, rptr, ridx, gptr, gidx, qptr, qidx, cptr, cidx;
-    uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0, n_tape = 0;
+    uint32_t n_inst = 0, n_stage = 0, n_prm = 0, n_lim = 0, n_tape = 0, n_prm_lds = 0;
     WaveBlob at{};
     std::vector<uint32_t> blob;
 };

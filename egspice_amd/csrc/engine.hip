@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
                                                          uint32_t *pred_hit, uint32_t *h_status, uint32_t *h_iters,
-                                                         double *h_x)
+                                                         double *h_x, double *lu_g)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -136,6 +136,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lu_scratch_words(W.cap));
     DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
     if (tid == 0 && pred_hit) pred_hit[l] = 0;  // this launch's record starts empty
     // the lane's first factor will try the prediction: its tape's loads go
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
         D.staged = true;
     }
     double *prm_lds = block + lay.prm;
-    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    for (uint32_t i = tid; i < W.n_prm_lds; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    s.gprm = prm_lanes + (size_t)l * W.n_prm;
     for (uint32_t i = tid; i < n; i += nt) {
         double v = x_glob[(size_t)i * N + l];
         s.x[i] = v;
@@ -221,7 +223,8 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
 // column. One workgroup; setup work, not part of a solve.
 template <bool VA>
 __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
-                                                          const double *prm_lanes, Newton o, uint32_t *pred)
+                                                          const double *prm_lanes, Newton o, uint32_t *pred,
+                                                          double *lu_g)
 {
     extern __shared__ double lds[];
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -231,9 +234,11 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g);
     const DenseLU D = dense_lu(block, lay, n);
     double *prm_lds = block + lay.prm;
-    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[i];
+    for (uint32_t i = tid; i < W.n_prm_lds; i += nt) prm_lds[i] = prm_lanes[i];
+    s.gprm = prm_lanes;
     for (uint32_t i = tid; i < n; i += nt) {
         s.x[i] = 0.0;
         s.x_old[i] = 0.0;
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
                                                        Newton op_opts, Newton step_opts, TranOpts to,
                                                        const MeasPass *passes, uint32_t n_pass, TranOut *outs,
                                                        MeasOut *meas_out, double *rec_t, double *rec_x,
-                                                       const uint32_t *pred)
+                                                       const uint32_t *pred, double *lu_g)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -305,6 +310,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     double *block = lds + B.words / 2;
     const WaveTemplate W = wave_template(B, tl);
     WaveLane s = wave_lane(W, lay, block);
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lu_scratch_words(W.cap));
     const DenseLU D = dense_lu(block, lay, n, pred);
     const uint32_t n_tape = W.n_tape;
     double *const a = s.cc + (nnz + 1);
@@ -318,7 +324,8 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     double *const bpc = scal + TRAN_SCALARS;
     MeasState *const ms = reinterpret_cast<MeasState *>(bpc + 2 * (size_t)W.n_inst);
     double *prm_lds = block + lay.prm;
-    for (uint32_t i = tid; i < W.n_prm; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    for (uint32_t i = tid; i < W.n_prm_lds; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    s.gprm = prm_lanes + (size_t)l * W.n_prm;
     for (uint32_t i = tid; i < n; i += nt) {
         // a cold operating point starts at 0; a warm one at the lane's x
         s.x[i] = to.op_warm ? x_glob[(size_t)i * N + l] : 0.0;
@@ -660,6 +667,7 @@ struct espice_lanes {
     DevBuf<double> prm_lanes;
     DevBuf<uint32_t> pred;      // the template's predicted first factor (PredTape)
     DevBuf<uint32_t> pred_hit;  // per lane, the last Newton launch's first factor: 0 none, 1 predicted, 2 serial
+    DevBuf<double> lu_g;        // per lane, the serial L/U arrays of a lu_global layout (HBM)
     bool has_pred = false;
     WaveLayout lay{};
     size_t lds_bytes = 0;
@@ -908,7 +916,13 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         // sizes only: the lane block's layout does not depend on where the
         // template tables live
         const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
+        // templates with compiled kinds keep the serial factor's L/U arrays
+        // in HBM: their lane blocks then fit more workgroups per CU
+        bool va = false;
+        for (int k = 0; k < K_NKIND; k++) va = va || (kind_is_va(k) && T.count[k] > 0);
+        L->lay.lu_global = va ? 1u : 0u;
         L->lds_bytes = 4 * (size_t)L->wh.at.words + 8 * wave_words(W, L->lay);
+        HIP_TRY(L->lu_g.alloc(va ? lu_scratch_words(cap) * N : 1));
         int dev = 0, lds = 0;
         HIP_TRY(hipGetDevice(&dev));
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -1087,10 +1101,10 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         const Newton o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
         if (L->has_va)
             hipLaunchKernelGGL(analyze_wave_kernel<true>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p);
+                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p, L->lu_g.p);
         else
             hipLaunchKernelGGL(analyze_wave_kernel<false>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p);
+                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p, L->lu_g.p);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(L->stream));
         L->has_pred = true;
@@ -1142,16 +1156,16 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
             hipLaunchKernelGGL((newton_wave_kernel<true, true>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                                L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                               want_x ? L->d_x : nullptr);
+                               want_x ? L->d_x : nullptr, L->lu_g.p);
         else if (L->lay.dense != ESP_NONE)
             hipLaunchKernelGGL((newton_wave_kernel<true, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                                L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                               want_x ? L->d_x : nullptr);
+                               want_x ? L->d_x : nullptr, L->lu_g.p);
         else
             hipLaunchKernelGGL((newton_wave_kernel<false, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr);
+                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -1285,6 +1299,7 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     // given) their charges are identically zero and the lane is exact.
     if (!tran_charges_zero(*L)) return fail(ESPICE_FAILED, "TranChargesUnsupported");
     WaveLayout lay{};
+    lay.lu_global = L->lay.lu_global;
     const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
     const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));
     const size_t lds = 4 * (size_t)L->wh.at.words + 8 * words;
@@ -1320,11 +1335,11 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     if (L->has_va)
         hipLaunchKernelGGL(tran_wave_kernel<true>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
                            L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
-                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
+                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
     else
         hipLaunchKernelGGL(tran_wave_kernel<false>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
                            L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
-                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr);
+                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));

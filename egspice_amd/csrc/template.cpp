@@ -159,6 +159,13 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
 {
     std::vector<InstEntry> inst;
     uint32_t stage = 0, prm = 0, lim = 0, tape = 0;
+    // prepared constants: the hand-written kinds' first (copied into each
+    // workgroup's LDS), then the compiled kinds' (read from HBM: hundreds of
+    // constants per instance that would crowd the lane block out of LDS)
+    uint32_t prm_lds = 0;
+    for (int t = 0; t < T.n_order; t++)
+        if (!kind_is_va(T.order[t])) prm_lds += T.count[T.order[t]] * (uint32_t)KIND_PK[T.order[t]];
+    uint32_t prm_va = prm_lds;
     std::vector<std::vector<uint32_t>> rows(T.n), slots(T.nnz), qrows(T.n), cslots(T.nnz);
     H.nodes_flat.clear();
     H.slots_flat.clear();
@@ -173,8 +180,13 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
             e.stage_res = stage;
             stage += nu;
             e.stage_jac = stage;
-            e.prm = prm;
-            prm += KIND_PK[kind];
+            if (kind_is_va(kind)) {
+                e.prm = prm_va;
+                prm_va += KIND_PK[kind];
+            } else {
+                e.prm = prm;
+                prm += KIND_PK[kind];
+            }
             e.lim = lim;
             if (KIND_LIMW[kind]) lim += nu;
             const uint32_t *nd = &T.nodes[kind][(size_t)id * nu];
@@ -231,7 +243,8 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
     H.n_tape = tape;
     H.n_inst = (uint32_t)inst.size();
     H.n_stage = stage;
-    H.n_prm = prm;
+    H.n_prm = prm_va;
+    H.n_prm_lds = prm_lds;
     H.n_lim = lim;
     H.inst_bytes.resize(inst.size() * sizeof(InstEntry));
     if (!inst.empty()) std::memcpy(H.inst_bytes.data(), inst.data(), H.inst_bytes.size());
@@ -274,6 +287,7 @@ void build_wave(const Template &T, uint32_t cap, WaveHost &H)
     B.n_inst = H.n_inst;
     B.n_stage = H.n_stage;
     B.n_prm = H.n_prm;
+    B.n_prm_lds = H.n_prm_lds;
     B.n_lim = H.n_lim;
     B.tri = T.tridiagonal ? 1u : 0u;
     B.diag_fill = T.diag_fill;

@@ -56,6 +56,7 @@ struct WaveTemplate {
     uint32_t n_tape;              // LTE charge sites
     uint32_t n_stage;             // staging cells per lane
     uint32_t n_prm;               // prepared constants per lane
+    uint32_t n_prm_lds;           // the leading ones held in LDS
     uint32_t n_lim;               // limiter words per lane
     uint32_t cap;                 // L+U capacity per lane
 };
@@ -87,6 +88,7 @@ ESP_HD WaveTemplate wave_template(const WaveBlob &B, const uint32_t *base)
     W.n_tape = B.n_tape;
     W.n_stage = B.n_stage;
     W.n_prm = B.n_prm;
+    W.n_prm_lds = B.n_prm_lds;
     W.n_lim = B.n_lim;
     W.cap = B.cap;
     return W;
@@ -98,7 +100,16 @@ struct WaveLayout {
     uint32_t dense;  // wave_lu.h's dense replay arrays, or ESP_NONE when n > 64
     uint32_t tran;   // the transient's arrays (wave_tran.h), or ESP_NONE
     uint32_t words;
+    // set by the caller before wave_words: the serial factor's L/U values
+    // and index lists (lx, ux, li, ui; 3 cap words) live in the lane's HBM
+    // scratch (lu_scratch_words) rather than LDS. The dense replay keeps
+    // its own copy of the factor in LDS, so only the serial factor, the
+    // dense rebuild and the replay's column lists read them.
+    uint32_t lu_global;
 };
+
+// 8-byte words of one lane's HBM scratch for a lu_global layout.
+ESP_HD size_t lu_scratch_words(uint32_t cap) { return 3 * (size_t)cap; }
 
 // 8-byte words of a transient lane's extra arrays (wave_tran.h TranLane):
 // q (n), cc (nnz + 1), a (nnz + 1), the current tape (n_tape), the charge
@@ -124,6 +135,7 @@ ESP_HD size_t dense_words(uint32_t n)
 struct WaveLane {
     double *x, *x_old, *dx, *g, *rhs, *lim, *stage;
     const double *prm;
+    const double *gprm;  // the lane's whole constants in HBM (compiled kinds read theirs there)
     LaneLU lu;
     uint32_t *flags;  // 0 singular, 1 limited, 2 done, 3 bits, 4 ok, 5 scaled bits lo, 6 hi
     double *q, *cc, *tape;  // transient: row charges, C plane, charge per LTE site
@@ -140,13 +152,14 @@ ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L, bool transient = 
     L.rhs = (uint32_t)at; at += n + 1;
     L.lim = (uint32_t)at; at += W.n_lim;
     L.stage = (uint32_t)at; at += W.n_stage;
-    L.prm = (uint32_t)at; at += W.n_prm;
-    // doubles of the solver: lx, ux (cap), udiag, w, y, rscale (n), vcopy
-    // (nnz), tri (3n), probe (3n)
-    L.lu_f64 = (uint32_t)at; at += 2 * cap + 4 * n + nnz + 6 * n;
-    // u32 words of the solver: cnt 4, pinv n, lp n+1, up n+1, li, ui (cap),
-    // prow, void_slots (nnz), flag..prev 7n
-    size_t u32w = 4 + n + 2 * (n + 1) + 2 * cap + 2 * nnz + 7 * n;
+    L.prm = (uint32_t)at; at += W.n_prm_lds;
+    // doubles of the solver: lx, ux (cap, unless lu_global), udiag, w, y,
+    // rscale (n), vcopy (nnz), tri (3n), probe (3n)
+    const size_t lucap = L.lu_global ? 0 : cap;
+    L.lu_f64 = (uint32_t)at; at += 2 * lucap + 4 * n + nnz + 6 * n;
+    // u32 words of the solver: cnt 4, pinv n, lp n+1, up n+1, li, ui (cap,
+    // unless lu_global), prow, void_slots (nnz), flag..prev 7n
+    size_t u32w = 4 + n + 2 * (n + 1) + 2 * lucap + 2 * nnz + 7 * n;
     L.lu_u32 = (uint32_t)at; at += (u32w + 1) / 2;
     // bytes: state 3, void_col n, scaled_pivot n
     size_t u8w = 3 + 2 * n;
@@ -179,10 +192,12 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
     s.lim = block + L.lim;
     s.stage = block + L.stage;
     s.prm = block + L.prm;
+    s.gprm = s.prm;
+    const uint32_t lucap = L.lu_global ? 0 : cap;
     double *f = block + L.lu_f64;
     LaneLU &u = s.lu;
-    u.lx = LV<double>{f, 1}; f += cap;
-    u.ux = LV<double>{f, 1}; f += cap;
+    u.lx = LV<double>{f, 1}; f += lucap;
+    u.ux = LV<double>{f, 1}; f += lucap;
     u.udiag = LV<double>{f, 1}; f += n;
     u.w = LV<double>{f, 1}; f += n;
     u.y = LV<double>{f, 1}; f += n;
@@ -199,8 +214,8 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
     u.pinv = LV<uint32_t>{w, 1}; w += n;
     u.lp = LV<uint32_t>{w, 1}; w += n + 1;
     u.up = LV<uint32_t>{w, 1}; w += n + 1;
-    u.li = LV<uint32_t>{w, 1}; w += cap;
-    u.ui = LV<uint32_t>{w, 1}; w += cap;
+    u.li = LV<uint32_t>{w, 1}; w += lucap;
+    u.ui = LV<uint32_t>{w, 1}; w += lucap;
     u.prow = LV<uint32_t>{w, 1}; w += nnz;
     u.void_slots = LV<uint32_t>{w, 1}; w += nnz;
     u.flag = LV<uint32_t>{w, 1}; w += n;
@@ -215,6 +230,7 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
     u.void_col = LV<uint8_t>{b, 1}; b += n;
     u.scaled_pivot = LV<uint8_t>{b, 1};
     s.flags = reinterpret_cast<uint32_t *>(block + L.flags);
+    if (L.lu_global) u.lx = u.ux = LV<double>{nullptr, 1}, u.li = u.ui = LV<uint32_t>{nullptr, 1};
     s.q = s.cc = s.tape = nullptr;
     if (L.tran != ESP_NONE) {
         double *t = block + L.tran;
@@ -223,6 +239,16 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
         s.tape = t + n + 2 * ((size_t)nnz + 1);
     }
     return s;
+}
+
+// Points a lu_global lane's serial L/U arrays at its HBM scratch.
+ESP_HD void wave_bind_lu_scratch(WaveLane &s, const WaveTemplate &W, double *scratch)
+{
+    s.lu.lx = LV<double>{scratch, 1};
+    s.lu.ux = LV<double>{scratch + W.cap, 1};
+    uint32_t *w = reinterpret_cast<uint32_t *>(scratch + 2 * (size_t)W.cap);
+    s.lu.li = LV<uint32_t>{w, 1};
+    s.lu.ui = LV<uint32_t>{w + W.cap, 1};
 }
 
 struct LP {
@@ -322,7 +348,7 @@ ESP_HD void wave_eval_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie
     double x[nu];
 #pragma unroll
     for (int u = 0; u < nu; u++) x[u] = s.x[nd[u]];
-    const double *prm = s.prm + ie.prm;
+    const double *prm = s.gprm + ie.prm;
     double *res = s.stage + ie.stage_res, *jac = s.stage + ie.stage_jac;
     if constexpr (KIND == K_PSP103) {
         if (TRAN) psp103_eval_tran(prm, x, res, jac, s.stage + ie.stage_q, s.stage + ie.stage_c, s.tape + ie.tape);

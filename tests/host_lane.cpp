@@ -9,6 +9,7 @@
 
 #include "../egspice_amd/csrc/engine.h"
 #include "../egspice_amd/csrc/wave.h"
+#include "../egspice_amd/csrc/wave_tran.h"
 
 using namespace esp;
 
@@ -176,7 +177,7 @@ void hl_newton_wave(void *hv, const Newton *o, double *x, uint32_t *status, uint
     build_wave(*t, h->cap, wh);
     // the kernel's view: every table read from the packed blob
     const WaveTemplate W = wave_template(wh.at, wh.blob.data());
-    WaveLayout lay;
+    WaveLayout lay{};
     size_t words = wave_words(W, lay);
     std::vector<double> block(words);
     const InstEntry *ie = W.inst;
@@ -244,3 +245,25 @@ double hl_sin(double x) { return e_sin(x); }
 double hl_pow(double x, double y) { return e_pow(x, y); }
 
 }  // extern "C"
+
+extern "C" {
+// The workgroup-per-lane kernels' LDS bytes for this template: the static
+// Newton's lane block and a transient's with n_passes .meas passes.
+void hl_layout(void *tv, uint32_t n_passes, uint64_t *out)
+{
+    Template *t = (Template *)tv;
+    WaveHost H;
+    build_wave(*t, (uint32_t)std::max<size_t>((size_t)t->n * (t->n - 1), 1), H);
+    const WaveTemplate W = wave_template(H.at, H.blob.data());
+    WaveLayout L{}, LT{};
+    out[0] = 4 * (uint64_t)H.at.words + 8 * wave_words(W, L);
+    out[1] = 4 * (uint64_t)H.at.words + 8 * wave_words(W, LT, true, tran_extra_words(W.n_inst, n_passes));
+    out[2] = 4 * (uint64_t)H.at.words;
+    out[3] = H.n_prm;
+    out[4] = H.n_prm_lds;
+    out[5] = 8 * (uint64_t)(L.lu_u32 - L.lu_f64);
+    out[6] = 8 * (uint64_t)(L.flags - L.lu_u32);
+    out[7] = L.dense == ESP_NONE ? 0 : 8 * (uint64_t)dense_words(t->n);
+    out[8] = 8 * (uint64_t)(L.prm - L.stage);
+}
+}
