@@ -36,6 +36,7 @@ EXPORTS = (
     "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
     "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
     "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
+    "espice_lanes_tab_refactor_solve_device",
 )
 
 
@@ -121,6 +122,7 @@ def library():
         "espice_lanes_set_param": [P, C.c_uint32, C.c_uint32, C.c_uint32, f64p],
         "espice_lanes_param_count": [C.c_uint32, C.POINTER(C.c_uint32)],
         "espice_lanes_param_index": [C.c_uint32, C.c_char_p, C.POINTER(C.c_uint32)],
+        "espice_lanes_tab_refactor_solve_device": [C.POINTER(LaneTab), u32p, C.c_size_t, C.c_uint32, P, P, P, P, P],
         "espice_lanes_prepare": [P],
         "espice_lanes_upload_x": [P, f64p],
         "espice_lanes_download_x": [P, f64p],
@@ -596,6 +598,17 @@ def tab_build(n, col_ptr, row_idx, vals):
     idx, ip = _u32(np.zeros(ln.value))
     _check(lib.espice_lanes_tab_build(n, cpp, rip, vp, C.byref(tab), ip, ln.value, C.byref(ln)))
     return tab, idx
+
+
+def tab_refactor_solve_device(tab, idx, n_lanes, a_ptr, rhs_ptr, dx_ptr, fail_ptr, stream=None):
+    """The allocation-free replay: a, rhs, dx and fail are device addresses
+    (ints); enqueued on `stream` (a hipStream_t address or None); the caller
+    synchronizes before reading dx and fail."""
+    lib = library()
+    ii, ip = _u32(idx)
+    V = C.c_void_p
+    _check(lib.espice_lanes_tab_refactor_solve_device(C.byref(tab), ip, len(ii), n_lanes, V(a_ptr), V(rhs_ptr),
+                                                      V(dx_ptr), V(fail_ptr), V(stream)))
 
 
 def tab_refactor_solve(tab, idx, a, rhs):

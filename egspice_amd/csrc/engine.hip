@@ -28,6 +28,7 @@
 #include <strings.h>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <string>
 #include <vector>
@@ -630,6 +631,21 @@ __global__ __launch_bounds__(64) void tab_refactor_kernel(LaneTab t, const uint3
     fail[l] = bad ? 1 : 0;
     if (!bad) tab_solve_neg(t, idx, LV<double>{lx + l, N}, LV<double>{ux + l, N}, LV<double>{udiag + l, N}, rv,
                             LV<double>{y + l, N}, dv);
+}
+
+// ------------------------------------------- the allocation-free replay
+
+// Scratch of espice_lanes_tab_refactor_solve_device: the tape's index words
+// followed by every lane's lx, ux, udiag, w and y planes. One array per GPU,
+// held until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress gives per device (scratch_base).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / 8];
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double *p, size_t count)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
 }
 
 // ------------------------------------------------------------------ batch
@@ -1493,6 +1509,75 @@ espice_status espice_lanes_tab_refactor_solve(const espice_lane_tab *tab, const 
     if (kernel_ms) *kernel_ms = ms;
     HIP_TRY(hipMemcpy(dx, d_dx.p, 8 * n * N, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(fail_out, d_fail.p, N, hipMemcpyDeviceToHost));
+    return ESPICE_OK;
+}
+
+// The scratch array's device address on the current GPU, looked up once per
+// device and kept.
+static espice_status scratch_base(double **out)
+{
+    static std::mutex mu;
+    static double *base[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(ESPICE_FAILED, "DeviceIndex");
+    std::lock_guard<std::mutex> guard(mu);
+    if (!base[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        base[dev] = static_cast<double *>(p);
+    }
+    *out = base[dev];
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_tab_refactor_solve_device(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
+                                                     uint32_t n_lanes, const double *a, const double *rhs, double *dx,
+                                                     uint8_t *fail_out, void *stream)
+{
+    if (!tab || !idx || !a || !rhs || !dx || !fail_out || n_lanes == 0) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    const espice_lane_tab &T = *tab;
+    const size_t need = (size_t)T.flags + T.n;
+    if (need > idx_len || T.col_ptr + T.n + 1 > idx_len || T.prow + T.nnz > idx_len || T.li + T.nnz_l > idx_len ||
+        T.ui + T.nnz_u > idx_len || T.void_slots + T.n_void_slots > idx_len)
+        return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t k = 0; k <= T.n; k++)
+        if (idx[T.lp + k] > T.nnz_l || idx[T.up + k] > T.nnz_u || idx[T.col_ptr + k] > T.nnz)
+            return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t k = 0; k < T.n; k++)
+        if (idx[T.q + k] >= T.n || idx[T.pinv + k] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz; p++)
+        if (idx[T.prow + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz_l; p++)
+        if (idx[T.li + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t p = 0; p < T.nnz_u; p++)
+        if (idx[T.ui + p] >= T.n) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    for (uint32_t v = 0; v < T.n_void_slots; v++)
+        if (idx[T.void_slots + v] >= T.nnz) return fail(ESPICE_INVALID_ARGUMENT, "TabOutOfRange");
+    const size_t N = n_lanes, n = T.n;
+    // words of the scratch: the index words (two per double), then lx
+    // (nnz_l), ux (nnz_u), udiag, w, y (n each) per lane
+    const size_t idx_words = (idx_len + 1) / 2;
+    const size_t nl = std::max<size_t>(T.nnz_l, 1), nu = std::max<size_t>(T.nnz_u, 1);
+    const size_t words = idx_words + N * (nl + nu + 3 * n);
+    if (words * 8 > NOALLOC_SCRATCH_BYTES) return fail(ESPICE_INVALID_ARGUMENT, "ScratchTooSmall");
+    double *base = nullptr;
+    if (espice_status e = scratch_base(&base)) return e;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t *d_idx = reinterpret_cast<uint32_t *>(base);
+    double *d_lx = base + idx_words, *d_ux = d_lx + N * nl, *d_udiag = d_ux + N * nu;
+    double *d_w = d_udiag + N * n, *d_y = d_w + N * n;
+    HIP_TRY(hipMemcpyAsync(d_idx, idx, 4 * idx_len, hipMemcpyHostToDevice, st));
+    // w must start at zero (tab_refactor's scatter vector); a fill kernel,
+    // not a memset, so the call can sit in a captured graph
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((N * n + 255) / 256, 1024)), dim3(256), 0, st,
+                       d_w, N * n);
+    HIP_TRY(hipGetLastError());
+    LaneTab t{T.n, T.nnz, T.nnz_l, T.nnz_u, T.growth, T.wide, T.col_ptr, T.prow, T.q, T.pinv, T.lp, T.li, T.up, T.ui,
+              T.flags, T.void_slots, T.n_void_slots, 0};
+    hipLaunchKernelGGL(tab_refactor_kernel, dim3((n_lanes + 63) / 64), dim3(64), 0, st, t, d_idx, n_lanes, a, rhs, dx,
+                       fail_out, d_lx, d_ux, d_udiag, d_w, d_y);
+    HIP_TRY(hipGetLastError());
     return ESPICE_OK;
 }
 

@@ -161,6 +161,21 @@ espice_status espice_lanes_tab_build(uint32_t n, const uint32_t *col_ptr, const 
 espice_status espice_lanes_tab_refactor_solve(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
     uint32_t n_lanes, const double *a, const double *rhs, double *dx, uint8_t *fail, double *kernel_ms);
 
+/* The same replay allocating nothing: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. a, rhs,
+ * dx and fail are DEVICE pointers with espice_lanes_tab_refactor_solve's
+ * layouts; idx (idx_len words) is on the host and is checked there, then
+ * copied into the library's fixed scratch array together with every lane's
+ * factor planes: ceil(idx_len / 2) + n_lanes * (max(nnz_l, 1) + max(nnz_u, 1)
+ * + 3 n) doubles must fit its 64 MiB, else INVALID_ARGUMENT
+ * "ScratchTooSmall". The work is enqueued on `stream` (a hipStream_t; NULL
+ * the default stream) and the call returns without waiting: the caller
+ * synchronizes before reading dx and fail. One scratch array per GPU,
+ * shared by every call on it: calls that overlap on one GPU race. */
+espice_status espice_lanes_tab_refactor_solve_device(const espice_lane_tab *tab, const uint32_t *idx, size_t idx_len,
+    uint32_t n_lanes, const double *a, const double *rhs, double *dx, uint8_t *fail, void *stream);
+
 /* ---- transient lanes (the device step control of tran.runGpuLanes,
  * src/analysis/tran/tran.zig:1736-1912, with src/solver/kernels/lane.zig's
  * step machine :353-700 and the .meas passes of src/output/scan.zig) ----
