@@ -14,9 +14,13 @@
    collapse) are kept, separately for the static evaluation (currents) and
    the transient one (currents and charges).
 4. Derivative lanes: each dynamic real carries the partial derivatives
-   with respect to the unknowns it depends on, and only those (sparse
-   forward-mode AD): lanes(V(a, b)) = {a, b}, a variable's lanes the union
-   over its assignments.
+   with respect to the branch voltages it depends on, and only those
+   (sparse forward-mode AD): every distinct probed pair of nodes {a, b} is
+   one lane (V(a, b) and V(b, a) are the same lane with opposite signs), a
+   variable's lanes the union over its assignments; a contribution's
+   Jacobian on node columns follows by the chain rule at the stamp
+   (d/dx_a = +d_p, d/dx_b = -d_p). A compact model's core reads three or
+   four branch voltages where it would carry four or more node lanes.
 5. Emission: three-address C with one temporary per operation; the
    derivative rules follow the reference's dual numbers
    (src/device/eval.zig:107-350): product rule and quotient numerator
@@ -112,6 +116,7 @@ class Compiler:
         self.pos = 0
         self.tree = self.index(model.body, (), ())
         self.types()
+        self.probes()
         self.classify()
         self.relevance()
         self.lanes()
@@ -242,6 +247,29 @@ class Compiler:
         v = self.assigned(st)
         return v in self.dyn
 
+    # ------------------------------------------------------------ probes
+    def probes(self):
+        """One derivative lane per distinct probed node pair, canonical
+        orientation (smaller index first; ground is -1)."""
+        self.probe_id = {}
+        self.probe_pair = []
+
+        def f(e):
+            if e[0] == "probe" and not (e[1] < 0 and e[2] < 0):
+                key = (min(e[1], e[2]), max(e[1], e[2]))
+                if key not in self.probe_id:
+                    self.probe_id[key] = len(self.probe_pair)
+                    self.probe_pair.append(key)
+
+        for st in self.flat:
+            for e in self.stmt_reads(st):
+                walk_expr(e, f)
+
+    def probe_lane(self, a, b):
+        """(lane, sign) of V(a, b)."""
+        key = (min(a, b), max(a, b))
+        return self.probe_id[key], (1.0 if (a, b) == key else -1.0)
+
     # ---------------------------------------------------------- relevance
     def relevance(self):
         self.contribs = []
@@ -303,7 +331,9 @@ class Compiler:
     def elanes(self, e):
         k = e[0]
         if k == "probe":
-            return frozenset(x for x in e[1:] if x >= 0)
+            if e[1] < 0 and e[2] < 0:
+                return frozenset()
+            return frozenset([self.probe_lane(e[1], e[2])[0]])
         if k == "var":
             if self.vtype.get(e[1]) == "int":
                 return frozenset()

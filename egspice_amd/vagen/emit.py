@@ -100,15 +100,12 @@ class Emitter:
                 return Val("0.0")
             if self.mode == "setup":
                 raise ParseError("a probe in static code")
-            d = {}
+            lane, sign = self.c.probe_lane(a, b)
+            d = {lane: "1.0" if sign > 0 else "-1.0"}
             if b < 0:
-                d[a] = "1.0"
                 return Val(f"x[{a}]", d)
             if a < 0:
-                d[b] = "-1.0"
                 return Val(self.const(f"-(x[{b}])"), d)
-            d[a] = "1.0"
-            d[b] = "-1.0"
             return Val(self.const(f"x[{a}] - x[{b}]"), d)
         if k == "zero":
             return Val("0.0")
@@ -420,12 +417,20 @@ class Emitter:
             self.out("}")
 
     def stamp(self, a, b, val, rv, jv, jidx):
-        for node, sgn in ((a, "+"), (b, "-")):
+        """Rows a (+) and b (-); each lane's partial onto its probe's node
+        columns by the chain rule (+ on the pair's first node, - on its
+        second)."""
+        for node, sgn in ((a, 1), (b, -1)):
             if node < 0:
                 continue
-            self.out(f"{rv}[{node}] {sgn}= {val.v};")
+            self.out(f"{rv}[{node}] {'+' if sgn > 0 else '-'}= {val.v};")
             for l in sorted(val.d):
-                self.out(f"{jv}[{jidx[(node, l)]}] {sgn}= {val.d[l]};")
+                pa, pb = self.c.probe_pair[l]
+                for col, cs in ((pa, 1), (pb, -1)):
+                    if col < 0:
+                        continue
+                    op = "+" if sgn * cs > 0 else "-"
+                    self.out(f"{jv}[{jidx[(node, col)]}] {op}= {val.d[l]};")
 
 
 def structure(c):
@@ -436,15 +441,21 @@ def structure(c):
     for st, r, qs in c.contribs:
         a, b = st.a[1], st.a[2]
         rows = [x for x in (a, b) if x >= 0]
+        def cols(lanes):
+            out = set()
+            for l in lanes:
+                out |= {x for x in c.probe_pair[l] if x >= 0}
+            return out
+
         if r is not None:
             for row in rows:
-                for l in c.elanes(r):
-                    g.add((row, l))
+                for col in cols(c.elanes(r)):
+                    g.add((row, col))
         for sign, q in qs:
             sites.append((id(st), id(q)))
             for row in rows:
-                for l in c.elanes(q):
-                    cc.add((row, l))
+                for col in cols(c.elanes(q)):
+                    cc.add((row, col))
     return sorted(g), sorted(cc), sites
 
 
