@@ -8,8 +8,9 @@
  * (include/espice.h), restricted to the calls below; a caller written for
  * the reference's points API links against this library unchanged.
  *
- * Deck subset (ngspice dialect): V (DC, PULSE), I (DC), R, C, D, M
- * (LEVEL=1) elements; .model NMOS/PMOS/D; .param with {expr} and 'expr';
+ * Deck subset (ngspice dialect): V and I (DC, PULSE, SIN, EXP, SFFM, AM,
+ * PWL; I's SFFM refused), R, C, D, M (LEVEL=1) elements; .model NMOS/PMOS/D;
+ * .param with {expr} and 'expr';
  * .op; .tran tstep tstop [tstart [tmax]]; .meas tran TRIG/TARG, WHEN,
  * FIND/DERIV ... AT=, MIN, MAX, AVG, RMS, INTEG [FROM= TO=], PARAM=; .meas op
  * FIND. Any other card that would change an analysis result fails create
@@ -77,9 +78,8 @@ typedef enum {
 
 /* include/espice.h:286-308: the same 48-byte layout. abort_after must be
  * NaN (off), measures_only and early_stop 0, tangent_step 0: the lane
- * engine records whole transients (INVALID_ARGUMENT "Unsupported" else).
- * warm_start 1 starts each point's operating point from the template's
- * nominal one. */
+ * engine records whole transients, and warm_start 0 (INVALID_ARGUMENT
+ * "Unsupported" else). */
 typedef struct {
     size_t struct_size;
     espice_bytes abort_measure;

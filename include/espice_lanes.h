@@ -220,7 +220,8 @@ typedef struct {
     uint32_t status, points, steps, attempts, rej_newton, rej_lte, bp_landings, nr_iters;
 } espice_tran_result;
 
-/* Every lane's transient in one launch from a cold operating point:
+/* Every lane's transient in one launch, its operating point solved first
+ * (cold, or from download/upload_x's state with opts->op_warm):
  * out[n_lanes]; meas[n_lanes * n_passes] lane-major by pass; when
  * record_capacity > 0, times[n_lanes * record_capacity] and
  * rows[n_lanes * record_capacity * n] receive each lane's first
@@ -228,7 +229,8 @@ typedef struct {
  * each lane's state at its last accepted point. FAILED "LaneTooLarge" for a
  * template whose transient lane does not fit one workgroup's LDS or has
  * more than 64 unknowns; FAILED "TranChargesUnsupported" for a template
- * with diodes or mos1 devices, whose charge models are not staged. */
+ * whose mos1 devices carry charge (TOX, overlap or junction capacitance):
+ * the mos1 charge model is not staged. */
 espice_status espice_lanes_tran(espice_lanes *l, const espice_tran_opts *opts, const espice_meas_pass *passes,
     uint32_t n_passes, espice_tran_result *out, espice_meas_result *meas, double *times, double *rows,
     double *kernel_ms);
