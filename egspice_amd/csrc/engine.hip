@@ -399,9 +399,9 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     const bool has_charge = n_tape > 0;
     if (has_charge) {
         // q(op) seeds a flat history
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA>(W, s, e, false, 0.0);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA, true>(W, s, e, false, 0.0);
         __syncthreads();
-        for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
+        for (uint32_t e = tid; e < n; e += nt) wave_reduce_charge(W, s, e);
         __syncthreads();
         for (uint32_t k = 1; k < 4; k++) {
             for (uint32_t i = tid; i < n; i += nt) ring[k * (size_t)n + i] = s.q[i];
@@ -447,6 +447,9 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
                 scal[5] = c.ag1;
                 scal[6] = c.ag2;
                 for (uint32_t k = 0; k < 4; k++) scal[8 + k] = (double)st.perm[k];
+                scal[12] = st.dt;
+                scal[13] = st.dt_old[0];
+                scal[14] = st.dt_old[1];
             }
         }
         __syncthreads();
@@ -470,14 +473,26 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         if (r == NEWTON_CONVERGED && has_charge) {
             // the charge re-read at the solution, unlimited (tran.zig:859-864,
             // eval.zig:1086-1108)
-            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA>(W, s, e, false, t_dev);
+            for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA, true>(W, s, e, false, t_dev);
             __syncthreads();
-            for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
+            for (uint32_t e = tid; e < n; e += nt) wave_reduce_charge(W, s, e);
             __syncthreads();
             for (uint32_t i = tid; i < n; i += nt) ring[p0 * (size_t)n + i] = s.q[i];
             for (uint32_t j = tid; j < n_tape; j += nt) tring[p0 * (size_t)n_tape + j] = s.tape[j];
         }
         __syncthreads();
+        // the step's two LTE bounds (its own method and, for a promotion,
+        // the run's), over the charge sites across the wavefront
+        double del_w = 0.0, trial_w = 0.0;
+        if (r == NEWTON_CONVERGED && has_charge) {
+            const TranCoeffs c{method, scal[4], scal[5], scal[6]};
+            const LteIn e{scal[12], scal[13], scal[14], to.reltol, to.abstol, to.chgtol, to.trtol};
+            const uint32_t q3 = (uint32_t)scal[11];
+            const double *t0p = tring + p0 * (size_t)n_tape, *t1p = tring + p1 * (size_t)n_tape;
+            const double *t2p = tring + p2 * (size_t)n_tape, *t3p = tring + q3 * (size_t)n_tape;
+            del_w = wave_step_bound(method, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+            trial_w = wave_step_bound((int)to.method, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+        }
         // Sim.end up to the commit
         if (tid == 0) {
             const TranCoeffs c{method, scal[4], scal[5], scal[6]};
@@ -492,13 +507,9 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
                 tran_floor_retry(to, st);
             } else {
                 double dt_next = st.steps == 0 ? dt : zmin(dt * 2.0, to.dt_max);
-                const LteIn e{dt, st.dt_old[0], st.dt_old[1], to.reltol, to.abstol, to.chgtol, to.trtol};
-                const uint32_t q3 = st.perm[3];
-                const double *t0p = tring + p0 * (size_t)n_tape, *t1p = tring + p1 * (size_t)n_tape;
-                const double *t2p = tring + p2 * (size_t)n_tape, *t3p = tring + q3 * (size_t)n_tape;
                 accept = true;
                 if (has_charge && st.steps > 0) {
-                    const double del = step_bound(method, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+                    const double del = del_w;
                     if (!(del >= 0.9 * dt)) {
                         st.out.rej_lte++;
                         if (del != del) {
@@ -517,8 +528,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
                     st.order_run = st.order == st.run_order ? st.order_run + 1 : 1;
                     st.run_order = st.order;
                     if (st.steps > 0 && st.order == 1) {
-                        const int promote = (int)to.method;
-                        const double trial_del = step_bound(promote, method, t0p, t1p, t2p, t3p, ti_prev, n_tape, c, e);
+                        const double trial_del = trial_w;
                         if (trial_del == trial_del) {
                             const double nd2 = zmin(2.0 * dt, trial_del);
                             if (nd2 > 1.05 * dt) st.order = 2;

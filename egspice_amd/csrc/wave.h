@@ -356,17 +356,36 @@ ESP_HD void wave_eval_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie
     }
 }
 
+// A compiled kind's charge rows and charge sites alone, values only (the
+// charge re-read at a transient solution): the value path of the transient
+// evaluation, so the same bits, without its derivative lanes or currents.
+template <int KIND>
+ESP_HD void wave_charge_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e)
+{
+    constexpr int nu = KIND_NU[KIND];
+    const uint32_t *nd = W.nodes_flat + W.node_off[e];
+    double x[nu];
+#pragma unroll
+    for (int u = 0; u < nu; u++) x[u] = s.x[nd[u]];
+    const double *prm = s.gprm + ie.prm;
+    if constexpr (KIND == K_PSP103) psp103_eval_charge(prm, x, s.stage + ie.stage_q, s.tape + ie.tape);
+}
+
 // TRAN: sources read their waveform at t and charges are staged too. VA:
 // the template holds compiled kinds; only then are their (long, register-
 // hungry) evaluations instantiated, so the kernels of templates without
-// them keep their own register budget and occupancy.
-template <bool TRAN = false, bool VA = false>
+// them keep their own register budget and occupancy. QONLY (transient):
+// only the charge rows and charge sites are read afterwards, so compiled
+// kinds stage just those (the other kinds evaluate as usual).
+template <bool TRAN = false, bool VA = false, bool QONLY = false>
 ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t = 0.0)
 {
+    static_assert(TRAN || !QONLY, "charges are staged in a transient");
     const InstEntry ie = W.inst[e];
     if constexpr (VA) {
         if (ie.kind == K_PSP103) {
-            wave_eval_va<K_PSP103, TRAN>(W, s, ie, e);
+            if constexpr (QONLY) wave_charge_va<K_PSP103>(W, s, ie, e);
+            else wave_eval_va<K_PSP103, TRAN>(W, s, ie, e);
             return;
         }
     }

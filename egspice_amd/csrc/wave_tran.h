@@ -93,38 +93,75 @@ struct LteIn {
     double dt, dt1, dt2, reltol, abstol, chgtol, trtol;
 };
 
-// integrator.zig stepBound / minDel over the charge sites (thread 0).
-ESP_HD double step_bound(int method, int cur_method, const double *q0, const double *q1, const double *q2,
-                         const double *q3, const double *ip, uint32_t len, const TranCoeffs &c, const LteIn &e)
+// integrator.zig stepBound / minDel: one charge site's bound; `bad` is set
+// when one of its inputs or the bound is NaN (the reference poisons the
+// whole bound then).
+ESP_HD double step_site(int method, int cur_method, const double *q0, const double *q1, const double *q2,
+                        const double *q3, const double *ip, uint32_t j, const TranCoeffs &c, const LteIn &e,
+                        bool &bad)
 {
     const double coeff = method == 0 ? 0.5 : method == 1 ? 0.08333333333 : 0.2222222222;
     const double sum01 = e.dt + e.dt1, sum12 = e.dt1 + e.dt2, sum012 = e.dt + (e.dt1 + e.dt2);
-    double vmin = inf_(), poison = 0;
+    const double qc = q0[j], qp = q1[j], qp2 = q2[j], i_p = ip[j];
+    double i_new;
+    if (cur_method == 1) i_new = c.ag0 * (qc - qp) - c.ag1 * i_p;
+    else if (cur_method == 2) i_new = c.ag0 * (qc - qp) - c.ag2 * (qp - qp2);
+    else i_new = c.ag0 * (qc - qp);
+    const double volttol = e.abstol + e.reltol * sel_max(fabs_(i_new), fabs_(i_p));
+    const double chargetol = e.reltol * sel_max(sel_max(fabs_(qc), fabs_(qp)), e.chgtol) / e.dt;
+    const double tol = sel_max(volttol, chargetol);
+    // (x - x) is 0, or NaN for a NaN or infinite x
+    double poison = (qc - qc) + (qp - qp) + (qp2 - qp2) + (i_p - i_p);
+    const double f12 = (qp - qp2) / e.dt1;
+    double dd = ((qc - qp) / e.dt - f12) / sum01;
+    if (method != 0) {
+        const double qp3 = q3[j];
+        poison += qp3 - qp3;
+        const double f123 = (f12 - (qp2 - qp3) / e.dt2) / sum12;
+        dd = (dd - f123) / sum012;
+    }
+    const double del = e.trtol * tol / sel_max(e.abstol, coeff * fabs_(dd));
+    bad = bad || poison != poison || del != del;
+    return del;
+}
+
+// The bound over every site: their minimum, NaN when any site is poisoned,
+// the square root of it for the second-order methods.
+ESP_HD double step_bound(int method, int cur_method, const double *q0, const double *q1, const double *q2,
+                         const double *q3, const double *ip, uint32_t len, const TranCoeffs &c, const LteIn &e)
+{
+    double vmin = inf_();
+    bool bad = false;
     for (uint32_t j = 0; j < len; j++) {
-        const double qc = q0[j], qp = q1[j], qp2 = q2[j], i_p = ip[j];
-        double i_new;
-        if (cur_method == 1) i_new = c.ag0 * (qc - qp) - c.ag1 * i_p;
-        else if (cur_method == 2) i_new = c.ag0 * (qc - qp) - c.ag2 * (qp - qp2);
-        else i_new = c.ag0 * (qc - qp);
-        const double volttol = e.abstol + e.reltol * sel_max(fabs_(i_new), fabs_(i_p));
-        const double chargetol = e.reltol * sel_max(sel_max(fabs_(qc), fabs_(qp)), e.chgtol) / e.dt;
-        const double tol = sel_max(volttol, chargetol);
-        poison += (qc - qc) + (qp - qp) + (qp2 - qp2) + (i_p - i_p);
-        const double f12 = (qp - qp2) / e.dt1;
-        double dd = ((qc - qp) / e.dt - f12) / sum01;
-        if (method != 0) {
-            const double qp3 = q3[j];
-            poison += qp3 - qp3;
-            const double f123 = (f12 - (qp2 - qp3) / e.dt2) / sum12;
-            dd = (dd - f123) / sum012;
-        }
-        const double del = e.trtol * tol / sel_max(e.abstol, coeff * fabs_(dd));
-        poison += del == del ? 0 : del;
+        const double del = step_site(method, cur_method, q0, q1, q2, q3, ip, j, c, e, bad);
         vmin = del < vmin ? del : vmin;
     }
-    if (poison != poison) return __builtin_nan("");
+    if (bad) return __builtin_nan("");
     return method == 0 ? vmin : __builtin_sqrt(vmin);
 }
+
+#if defined(__HIPCC__)
+// step_bound across the wavefront (all threads, one wave): thread t takes
+// sites t, t + 64, ...; a minimum and a flag do not depend on the order the
+// sites are visited in, so every thread returns step_bound's bits.
+__device__ inline double wave_step_bound(int method, int cur_method, const double *q0, const double *q1,
+                                         const double *q2, const double *q3, const double *ip, uint32_t len,
+                                         const TranCoeffs &c, const LteIn &e)
+{
+    double vmin = inf_();
+    bool bad = false;
+    for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) {
+        const double del = step_site(method, cur_method, q0, q1, q2, q3, ip, j, c, e, bad);
+        vmin = del < vmin ? del : vmin;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(vmin, off);
+        vmin = o < vmin ? o : vmin;
+    }
+    if (__ballot(bad ? 1 : 0) != 0) return __builtin_nan("");
+    return method == 0 ? vmin : __builtin_sqrt(vmin);
+}
+#endif
 
 // --------------------------------------------------------- .meas passes
 

@@ -11,8 +11,8 @@
    by a dynamic statement and assigned again after it becomes dynamic, so
    moving the static statements never changes a value.
 3. Relevance: only statements whose results reach a contribution (or a
-   collapse) are kept, separately for the static evaluation (currents) and
-   the transient one (currents and charges).
+   collapse) are kept, separately for the static evaluation (currents), the
+   transient one (currents and charges) and the charge values alone.
 4. Derivative lanes: each dynamic real carries the partial derivatives
    with respect to the branch voltages it depends on, and only those
    (sparse forward-mode AD): every distinct probed pair of nodes {a, b} is
@@ -278,14 +278,16 @@ class Compiler:
                 r, qs = split_ddt(st.a[3])
                 self.contribs.append((st, r, qs))
         self.rel = {}
-        for mode in ("dc", "tran"):
+        for mode in ("dc", "tran", "charge"):
             need = set()
             seeds = []
             for st, r, qs in self.contribs:
+                if mode == "charge" and not qs:
+                    continue
                 seeds.extend(st.conds)
-                if r is not None:
+                if r is not None and mode != "charge":
                     seeds.append(r)
-                if mode == "tran":
+                if mode != "dc":
                     seeds.extend(q for _, q in qs)
             for st in self.flat:
                 if st.kind == "collapse":

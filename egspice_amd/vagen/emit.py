@@ -45,6 +45,7 @@ class Emitter:
         self.lines = []
         self.ind = 1
         self.tmp = 0
+        self.noad = False  # values only: no derivative lanes
 
     # ------------------------------------------------------------ output
     def out(self, s):
@@ -66,8 +67,8 @@ class Emitter:
         if self.mode == "setup":
             return Val(f"v_{name}", ty=ty)
         if name in c.dyn:
-            return Val(f"v_{name}", {l: f"v_{name}__{l}" for l in sorted(c.vl.get(name, ()))} if ty == "real" else {},
-                       ty)
+            lanes = () if self.noad or ty != "real" else sorted(c.vl.get(name, ()))
+            return Val(f"v_{name}", {l: f"v_{name}__{l}" for l in lanes}, ty)
         self.used_static.add(("v", name))
         return Val(f"@v:{name}@", ty=ty)
 
@@ -101,7 +102,7 @@ class Emitter:
             if self.mode == "setup":
                 raise ParseError("a probe in static code")
             lane, sign = self.c.probe_lane(a, b)
-            d = {lane: "1.0" if sign > 0 else "-1.0"}
+            d = {} if self.noad else {lane: "1.0" if sign > 0 else "-1.0"}
             if b < 0:
                 return Val(f"x[{a}]", d)
             if a < 0:
@@ -207,7 +208,7 @@ class Emitter:
 
     def tern(self, e):
         cnd = self.ex(e[1])
-        lanes = sorted(self.c.elanes(e[2]) | self.c.elanes(e[3])) if self.mode != "setup" else []
+        lanes = sorted(self.c.elanes(e[2]) | self.c.elanes(e[3])) if self.mode != "setup" and not self.noad else []
         ty = "int" if self.c.etype(e[2]) == self.c.etype(e[3]) == "int" else "real"
         r = self.t()
         ctype = "double" if ty == "real" else "int"
@@ -323,6 +324,8 @@ class Emitter:
             return False
         if k == "contrib":
             r, qs = split_ddt(st.a[3])
+            if self.mode == "charge":
+                return bool(qs)
             return r is not None or (self.mode == "tran" and qs)
         if k in ("assign", "limit"):
             v = st.a[1]
@@ -390,7 +393,7 @@ class Emitter:
             else:
                 val = self.as_real(val)
                 line = f"v_{v} = {val.v};"
-                if self.mode != "setup":
+                if self.mode != "setup" and not self.noad:
                     for l in sorted(c.vl.get(v, ())):
                         line += f" v_{v}__{l} = {val.d.get(l, '0.0')};"
                 self.out(line)
@@ -403,10 +406,10 @@ class Emitter:
             r, qs = split_ddt(st.a[3])
             self.out("{")
             self.ind += 1
-            if r is not None:
+            if r is not None and self.mode != "charge":
                 val = self.as_real(self.ex(r))
                 self.stamp(a, b, val, "res", "jac", self.jidx)
-            if self.mode == "tran":
+            if self.mode in ("tran", "charge"):
                 for sign, q in qs:
                     val = self.as_real(self.ex(q))
                     if sign < 0:
@@ -424,6 +427,8 @@ class Emitter:
             if node < 0:
                 continue
             self.out(f"{rv}[{node}] {'+' if sgn > 0 else '-'}= {val.v};")
+            if self.noad:
+                continue
             for l in sorted(val.d):
                 pa, pb = self.c.probe_pair[l]
                 for col, cs in ((pa, 1), (pb, -1)):
@@ -476,9 +481,10 @@ def generate(c):
     # ---------------------------------------------------------- eval
     evals = {}
     used_static = set()
-    for mode in ("dc", "tran"):
+    for mode in ("dc", "tran", "charge"):
         e = Emitter(c)
         e.mode = mode
+        e.noad = mode == "charge"
         e.used_static = set()
         e.jidx, e.cidx, e.site_index = jidx, cidx, site_index
         e.coll_base, e.coll_index = 0, {}
@@ -598,19 +604,27 @@ def generate(c):
     H.append("    return 0;")
     H.append("}")
 
-    # eval functions
-    for mode in ("dc", "tran"):
+    # eval functions: the static evaluation, the transient one, and the
+    # transient's charge rows and sites alone, values only (the same
+    # operations on the value path, so the same bits as eval_tran's)
+    for mode in ("dc", "tran", "charge"):
         lines, _ = evals[mode]
-        fname = f"{P}_eval" if mode == "dc" else f"{P}_eval_tran"
-        args = "const double *VA_RESTRICT prep, const double *VA_RESTRICT x, double *VA_RESTRICT res_out, double *VA_RESTRICT jac_out"
-        outs = [("res", nu), ("jac", len(gpat))]
+        fname = {"dc": f"{P}_eval", "tran": f"{P}_eval_tran", "charge": f"{P}_eval_charge"}[mode]
+        args = "const double *VA_RESTRICT prep, const double *VA_RESTRICT x"
+        outs = []
+        if mode != "charge":
+            args += ", double *VA_RESTRICT res_out, double *VA_RESTRICT jac_out"
+            outs += [("res", nu), ("jac", len(gpat))]
         if mode == "tran":
             args += ", double *VA_RESTRICT q_out, double *VA_RESTRICT cjac_out, double *VA_RESTRICT site_out"
             outs += [("q", nu), ("cjac", len(cpat)), ("site", len(sites))]
+        if mode == "charge":
+            args += ", double *VA_RESTRICT q_out, double *VA_RESTRICT site_out"
+            outs += [("q", nu), ("site", len(sites))]
         H.append("")
         H.append(f"VA_FN void {fname}({args})")
         H.append("{")
-        H.append(f"#define VA_TRAN {1 if mode == 'tran' else 0}")
+        H.append(f"#define VA_TRAN {0 if mode == 'dc' else 1}")
         for nm, cnt in outs:
             H.append(f"    double {nm}[{max(1, cnt)}];")
             H.append(f"    for (int k = 0; k < {max(1, cnt)}; k++) {nm}[k] = 0.0;")
@@ -631,7 +645,7 @@ def generate(c):
             if ty == "int":
                 H.append(f"    int v_{v} = 0;")
             else:
-                ls = sorted(c.vl.get(v, ()))
+                ls = sorted(c.vl.get(v, ())) if mode != "charge" else []
                 H.append(f"    double v_{v} = 0.0" + "".join(f", v_{v}__{l} = 0.0" for l in ls) + ";")
         H.extend(lines)
         for nm, cnt in outs:

@@ -93,6 +93,7 @@ int or_va_collapse(int kind, const double *u, const uint32_t *ports, int64_t *ro
 void or_va_eval(int kind, const double *p, const double *x, double *res, double *jac);
 void or_va_eval_tran(int kind, const double *p, const double *x, double *res, double *jac, double *q, double *cjac,
                      double *site);
+void or_va_eval_charge(int kind, const double *p, const double *x, double *q, double *site);
 
 double or_fetlim(double vnew, double vold, double vto);
 double or_limvds(double vnew, double vold);

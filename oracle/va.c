@@ -104,3 +104,11 @@ void or_va_eval_tran(int kind, const double *p, const double *x, double *res, do
 {
     if (kind == OR_PSP103) psp103_eval_tran(p, x, res, jac, q, cjac, site);
 }
+
+/* The transient's charge rows and sites alone, values only (what the
+ * lane engine evaluates when it re-reads the charges at a solution); the
+ * same value operations as or_va_eval_tran, so the same bits. */
+void or_va_eval_charge(int kind, const double *p, const double *x, double *q, double *site)
+{
+    if (kind == OR_PSP103) psp103_eval_charge(p, x, q, site);
+}

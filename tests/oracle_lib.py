@@ -144,6 +144,7 @@ def load():
         "or_va_setup": (C.c_int, [C.c_int, f64p, f64p]),
         "or_va_eval": (None, [C.c_int, f64p, f64p, f64p, f64p]),
         "or_va_eval_tran": (None, [C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]),
+        "or_va_eval_charge": (None, [C.c_int, f64p, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -141,6 +141,25 @@ def test_source_drain_symmetry(lib):
         assert ids2 == pytest.approx(ids1, rel=1e-12, abs=1e-20)
 
 
+def test_charge_only_evaluation_is_bitwise_the_transient_one(lib):
+    """eval_charge (charge rows and sites, no derivative lanes: the lane
+    engine's re-read of the charges at a solution) gives the same bits as
+    eval_tran's charge rows and sites, both transistor types, biases across
+    every region including reversed drain and source."""
+    nu = lib.or_va_nu(O.PSP103)
+    rng = np.random.default_rng(11)
+    for ty, w, l in ((1, 2e-6, 0.5e-6), (-1, 8e-6, 1e-6), (1, 20e-6, 0.2e-6)):
+        rc, p = setup(lib, TYPE=ty, W=w, L=l, RSH=10.0, NRS=1.0, NRD=1.0)
+        assert rc == 0
+        for x in biases(rng, 40) * ty:
+            _, _, q, _, site = evaluate(lib, p, x)
+            q2, s2 = np.zeros(nu), np.zeros(max(1, lib.or_va_nsite(O.PSP103)))
+            xa = np.ascontiguousarray(x)
+            lib.or_va_eval_charge(O.PSP103, O.f64(p)[1], O.f64(xa)[1], O.f64(q2)[1], O.f64(s2)[1])
+            assert q.tobytes() == q2.tobytes()
+            assert site.tobytes() == s2.tobytes()
+
+
 @pytest.mark.skipif(not HAVE_SRC, reason="the Verilog-A source is not in this tree")
 def test_emitted_code_matches_direct_evaluation(lib):
     """The generated setup + evaluation against a statement-by-statement
