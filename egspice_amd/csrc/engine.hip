@@ -131,13 +131,15 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     const uint32_t l = blockIdx.x;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t n = B.n;
-    // the template's tables first (B.words is even), then the lane block
+    // the template's tables first (B.words is even; the lean layout reads
+    // them in global memory), then the lane block
     uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
-    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
-    double *block = lds + B.words / 2;
-    const WaveTemplate W = wave_template(B, tl);
+    if (!lay.lu_global)
+        for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + table_lds_words(B.words, lay) / 2;
+    const WaveTemplate W = wave_template(B, lay.lu_global ? blob : tl);
     WaveLane s = wave_lane(W, lay, block);
-    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lu_scratch_words(W.cap));
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lane_scratch_words(W.cap, W.n_tape));
     DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
     if (tid == 0 && pred_hit) pred_hit[l] = 0;  // this launch's record starts empty
     // the lane's first factor will try the prediction: its tape's loads go
@@ -231,9 +233,10 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t n = B.n, nnz = B.nnz;
     uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
-    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
-    double *block = lds + B.words / 2;
-    const WaveTemplate W = wave_template(B, tl);
+    if (!lay.lu_global)
+        for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + table_lds_words(B.words, lay) / 2;
+    const WaveTemplate W = wave_template(B, lay.lu_global ? blob : tl);
     WaveLane s = wave_lane(W, lay, block);
     if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g);
     const DenseLU D = dense_lu(block, lay, n);
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
     for (uint32_t i = tid; i < nl; i += nt) pred[T.li + i] = s.lu.li[i];
     for (uint32_t i = tid; i < nu; i += nt) pred[T.ui + i] = s.lu.ui[i];
     for (uint32_t i = tid; i < nnz; i += nt) pred[T.prow + i] = s.lu.prow[i];
-    for (uint32_t i = tid; i < n * n; i += nt) pred[T.amap + i] = D.amap[i];
+    for (uint32_t i = tid; i < n * n; i += nt) pred[T.amap + i] = amap_slot(D, i);
     if (tid == 0) {
         pred[PRED_NL] = nl;
         pred[PRED_NU] = nu;
@@ -307,19 +310,23 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t n = B.n, nnz = B.nnz;
     uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
-    for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
-    double *block = lds + B.words / 2;
-    const WaveTemplate W = wave_template(B, tl);
+    if (!lay.lu_global)
+        for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + table_lds_words(B.words, lay) / 2;
+    const WaveTemplate W = wave_template(B, lay.lu_global ? blob : tl);
     WaveLane s = wave_lane(W, lay, block);
-    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lu_scratch_words(W.cap));
+    double *const lane_scr = lu_g + (size_t)l * lane_scratch_words(W.cap, W.n_tape);
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lane_scr);
     const DenseLU D = dense_lu(block, lay, n, pred);
     const uint32_t n_tape = W.n_tape;
     double *const a = s.cc + (nnz + 1);
     double *const ring = s.tape + n_tape;
     double *const i_prev = ring + 4 * (size_t)n;
-    double *const tring = i_prev + n;
+    // the charge sites' history: in the lane's HBM scratch in the lean
+    // layout (read and written once per step), else in the block
+    double *const tring = lay.lu_global ? lane_scr + 3 * (size_t)W.cap : i_prev + n;
     double *const ti_prev = tring + 4 * (size_t)n_tape;
-    double *const cur = ti_prev + n_tape;
+    double *const cur = lay.lu_global ? i_prev + n : ti_prev + n_tape;
     double *const prev = cur + n;
     double *const scal = prev + n;
     double *const bpc = scal + TRAN_SCALARS;
@@ -947,8 +954,8 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         bool va = false;
         for (int k = 0; k < K_NKIND; k++) va = va || (kind_is_va(k) && T.count[k] > 0);
         L->lay.lu_global = va ? 1u : 0u;
-        L->lds_bytes = 4 * (size_t)L->wh.at.words + 8 * wave_words(W, L->lay);
-        HIP_TRY(L->lu_g.alloc(va ? lu_scratch_words(cap) * N : 1));
+        L->lds_bytes = 4 * table_lds_words(L->wh.at.words, L->lay) + 8 * wave_words(W, L->lay);
+        HIP_TRY(L->lu_g.alloc(va ? lane_scratch_words(cap, W.n_tape) * N : 1));
         int dev = 0, lds = 0;
         HIP_TRY(hipGetDevice(&dev));
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -1328,7 +1335,7 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     lay.lu_global = L->lay.lu_global;
     const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
     const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));
-    const size_t lds = 4 * (size_t)L->wh.at.words + 8 * words;
+    const size_t lds = 4 * table_lds_words(L->wh.at.words, lay) + 8 * words;
     if (lay.dense == ESP_NONE || lds > L->lds_max) return fail(ESPICE_FAILED, "LaneTooLarge");
     DevBuf<MeasPass> d_pass;
     DevBuf<TranOut> d_out;

@@ -100,35 +100,49 @@ struct WaveLayout {
     uint32_t dense;  // wave_lu.h's dense replay arrays, or ESP_NONE when n > 64
     uint32_t tran;   // the transient's arrays (wave_tran.h), or ESP_NONE
     uint32_t words;
-    // set by the caller before wave_words: the serial factor's L/U values
-    // and index lists (lx, ux, li, ui; 3 cap words) live in the lane's HBM
-    // scratch (lu_scratch_words) rather than LDS. The dense replay keeps
-    // its own copy of the factor in LDS, so only the serial factor, the
-    // dense rebuild and the replay's column lists read them.
+    // set by the caller before wave_words: the lean layout of templates
+    // with compiled kinds, whose lanes are long-running and LDS-bound (the
+    // workgroups a CU holds). The serial factor's L/U values and index
+    // lists (lx, ux, li, ui; 3 cap words) and a transient's charge-site
+    // history (the tape ring and its currents, 5 n_tape words) live in the
+    // lane's HBM scratch (lane_scratch_words) rather than LDS, and the
+    // kernels read the template's tables from global memory instead of an
+    // LDS copy. The dense replay keeps its own copy of the factor and of
+    // the U column lists in LDS, so only the serial factor, the dense
+    // rebuild and the step's end touch the HBM arrays.
     uint32_t lu_global;
 };
 
-// 8-byte words of one lane's HBM scratch for a lu_global layout.
-ESP_HD size_t lu_scratch_words(uint32_t cap) { return 3 * (size_t)cap; }
+// 8-byte words of one lane's HBM scratch for a lu_global layout: the L/U
+// arrays, then the tape ring (4 n_tape) and its currents (n_tape).
+ESP_HD size_t lane_scratch_words(uint32_t cap, uint32_t n_tape) { return 3 * (size_t)cap + 5 * (size_t)n_tape; }
 
 // 8-byte words of a transient lane's extra arrays (wave_tran.h TranLane):
 // q (n), cc (nnz + 1), a (nnz + 1), the current tape (n_tape), the charge
 // ring (4 n) and its dynamic current (n), the tape ring (4 n_tape) and its
-// current (n_tape), x_prev and x_trial (n each), and 32 step scalars.
-ESP_HD size_t tran_words(uint32_t n, uint32_t nnz, uint32_t n_tape)
+// current (n_tape) unless they live in HBM (lu_global), x_prev and x_trial
+// (n each), and 32 step scalars.
+ESP_HD size_t tran_words(uint32_t n, uint32_t nnz, uint32_t n_tape, bool lu_global = false)
 {
-    return (size_t)n + 2 * ((size_t)nnz + 1) + n_tape + 5 * (size_t)n + 5 * (size_t)n_tape + 2 * (size_t)n + 32;
+    const size_t tape_hist = lu_global ? 0 : 5 * (size_t)n_tape;
+    return (size_t)n + 2 * ((size_t)nnz + 1) + n_tape + 5 * (size_t)n + tape_hist + 2 * (size_t)n + 32;
 }
+
+// 4-byte words of LDS the template's tables take in front of a lane block:
+// none in the lean layout, which reads them from global memory.
+ESP_HD size_t table_lds_words(uint32_t blob_words, const WaveLayout &L) { return L.lu_global ? 0 : blob_words; }
 
 // Largest pattern the dense wave replay takes: one pivot row per thread of
 // the lane's 64-wide wavefront.
 constexpr uint32_t DENSE_MAX_N = 64;
 
 // 8-byte words of the dense replay arrays: F (n*n f64), the two row masks
-// (2n u64), amap (n*n u32) and porig (n u32).
-ESP_HD size_t dense_words(uint32_t n)
+// (2n u64), amap (n*n u16) and porig (n u32); with the solver's lists in
+// HBM, also an LDS copy of the U column lists (at most n(n-1)/2 u32).
+ESP_HD size_t dense_words(uint32_t n, bool lu_global = false)
 {
-    return (size_t)n * n + 2 * (size_t)n + ((size_t)n * n + 1) / 2 + ((size_t)n + 1) / 2;
+    const size_t ucols = lu_global ? ((size_t)n * (n - 1) / 2 + 1) / 2 : 0;
+    return (size_t)n * n + 2 * (size_t)n + ((size_t)n * n + 3) / 4 + ((size_t)n + 1) / 2 + ucols;
 }
 
 // The lane's block carved into typed views (stride 1).
@@ -167,14 +181,14 @@ ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L, bool transient = 
     L.flags = (uint32_t)at; at += 4;  // 8 u32 flags
     if (n <= DENSE_MAX_N) {
         L.dense = (uint32_t)at;
-        at += dense_words((uint32_t)n);
+        at += dense_words((uint32_t)n, L.lu_global != 0);
     } else {
         L.dense = ESP_NONE;
     }
     L.tran = ESP_NONE;
     if (transient) {
         L.tran = (uint32_t)at;
-        at += tran_words((uint32_t)n, (uint32_t)nnz, W.n_tape) + tran_extra;
+        at += tran_words((uint32_t)n, (uint32_t)nnz, W.n_tape, L.lu_global != 0) + tran_extra;
     }
     L.words = (uint32_t)at;
     return at;

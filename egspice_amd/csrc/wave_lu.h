@@ -27,12 +27,18 @@
 
 namespace esp {
 
+constexpr uint16_t AMAP_NONE = 0xffffu;
+
 struct DenseLU {
     double *F;        // n*n, column k at F + k*n, rows in pivot order
     uint64_t *lmask;  // n: bit k of lmask[r] when L(r,k) is structural
     uint64_t *umask;  // n: bit k of umask[r] when U(r,k) is structural
-    uint32_t *amap;   // n*n: pattern slot of A at (pivot row r, step k), or ESP_NONE
+    uint16_t *amap;   // n*n: pattern slot of A at (pivot row r, step k), or AMAP_NONE
     uint32_t *porig;  // n: original row of pivot row t
+    // the U column lists (the solver's ui, up[n] entries in its order) in
+    // LDS when the solver keeps its own lists in HBM (WaveLayout.lu_global),
+    // else null and the replay reads the solver's LDS lists
+    uint32_t *ucols;
     const uint32_t *pred;  // the template's predicted first factor (PredTape, global memory), or null
     uint32_t *pred_hit;    // the lane's record of its first factor: 1 predicted, 2 serial (global), or null
     bool staged;           // pred_stage already copied the tape into the lane's arrays
@@ -76,10 +82,26 @@ ESP_HD DenseLU dense_lu(double *block, const WaveLayout &L, uint32_t n, const ui
     f += n;
     d.umask = reinterpret_cast<uint64_t *>(f);
     f += n;
-    d.amap = reinterpret_cast<uint32_t *>(f);
-    f += ((size_t)n * n + 1) / 2;
+    d.amap = reinterpret_cast<uint16_t *>(f);
+    f += ((size_t)n * n + 3) / 4;
     d.porig = reinterpret_cast<uint32_t *>(f);
+    f += ((size_t)n + 1) / 2;
+    d.ucols = L.lu_global ? reinterpret_cast<uint32_t *>(f) : nullptr;
     return d;
+}
+
+// amap entry i as a pattern slot, or ESP_NONE (slots fit 16 bits: nnz <= n*n
+// <= 4096 for the dense replay's n <= 64).
+__device__ inline uint32_t amap_slot(const DenseLU &D, uint32_t i)
+{
+    const uint32_t a = D.amap[i];
+    return a == AMAP_NONE ? ESP_NONE : a;
+}
+
+// The U column lists the replay reads: the LDS copy, or the solver's own.
+__device__ inline const uint32_t *dense_ucols(const DenseLU &D, const LaneLU &s)
+{
+    return D.ucols != nullptr ? D.ucols : s.ui.p;
 }
 
 // ----------------------------------------------------------- wave helpers
@@ -107,7 +129,7 @@ __device__ inline void dense_build(const Pattern &P, const LaneLU &s, const Dens
         D.umask[i] = 0;
         D.porig[s.pinv[i]] = i;
     }
-    for (uint32_t i = tid; i < n * n; i += nt) D.amap[i] = ESP_NONE;
+    for (uint32_t i = tid; i < n * n; i += nt) D.amap[i] = AMAP_NONE;
     __syncthreads();
     for (uint32_t k = 0; k < n; k++) {
         const uint32_t c = P.q[k];
@@ -122,8 +144,10 @@ __device__ inline void dense_build(const Pattern &P, const LaneLU &s, const Dens
             atomicOr(reinterpret_cast<unsigned long long *>(&D.umask[r]), 1ull << k);
         }
         if (tid == 0) D.F[k * n + k] = s.udiag[k];
-        for (uint32_t p = P.col_ptr[c] + tid; p < P.col_ptr[c + 1]; p += nt) D.amap[k * n + s.prow[p]] = p;
+        for (uint32_t p = P.col_ptr[c] + tid; p < P.col_ptr[c + 1]; p += nt) D.amap[k * n + s.prow[p]] = (uint16_t)p;
     }
+    if (D.ucols != nullptr)
+        for (uint32_t p = tid; p < s.up[n]; p += nt) D.ucols[p] = s.ui[p];
     __syncthreads();
 }
 
@@ -156,6 +180,7 @@ constexpr uint32_t CHUNK = 8;
 template <class A>
 __device__ int dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &D, const A &vals)
 {
+    const uint32_t *const ucol = dense_ucols(D, s);
     const uint32_t n = P.n, r = threadIdx.x, nt = blockDim.x;
     const bool own = r < n;
     const uint32_t rr = own ? r : n - 1;  // in-bounds row for unconditional loads
@@ -170,20 +195,20 @@ __device__ int dense_refactor(const Pattern &P, const LaneLU &s, const DenseLU &
     const uint32_t ub = own ? s.up[r] : 0, ue = own ? s.up[r + 1] : 0;
     uint64_t nonfinite = 0;
     bool growth_fail = false, pivot_fail = false, wide_all = false, wide_kept = false, nan_l = false;
-    const uint32_t a0 = own ? D.amap[r] : ESP_NONE;
-    uint32_t a_next = own && n > 1 ? D.amap[n + r] : ESP_NONE;
+    const uint32_t a0 = own ? amap_slot(D, r) : ESP_NONE;
+    uint32_t a_next = own && n > 1 ? amap_slot(D, n + r) : ESP_NONE;
     double wv = a0 != ESP_NONE ? vals[a0] : 0.0;
     uint32_t u0 = lane_read_u32(ub, 0), u1 = lane_read_u32(ue, 0);
-    uint32_t uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+    uint32_t uidx = r < u1 - u0 ? ucol[u0 + r] : 0;
     for (uint32_t k = 0; k < n; k++) {
         double w = wv;
         const uint32_t cnt = u1 - u0, col_idx = uidx;
         if (k + 1 < n) {
             wv = a_next != ESP_NONE ? vals[a_next] : 0.0;
-            a_next = own && k + 2 < n ? D.amap[(k + 2) * n + r] : ESP_NONE;
+            a_next = own && k + 2 < n ? amap_slot(D, (k + 2) * n + r) : ESP_NONE;
             u0 = lane_read_u32(ub, k + 1);
             u1 = lane_read_u32(ue, k + 1);
-            uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+            uidx = r < u1 - u0 ? ucol[u0 + r] : 0;
         }
         for (uint32_t j0 = 0; j0 < cnt; j0 += CHUNK) {
             uint32_t iv[CHUNK];
@@ -309,12 +334,13 @@ __device__ void dense_solve_neg(const Pattern &P, const DenseLU &D, const R &rhs
 // subtracts them; the products are formed on their own rows in parallel.
 __device__ inline double dense_sweeps_t(uint32_t n, const LaneLU &s, const DenseLU &D, double y)
 {
+    const uint32_t *const ucol = dense_ucols(D, s);
     const uint32_t r = threadIdx.x;
     const uint32_t rr = r < n ? r : n - 1;
     for (uint32_t k = 0; k < n; k++) {
         const double prod = D.F[k * n + rr] * y;
         double v = lane_read(y, k);
-        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) v = v - lane_read(prod, s.ui[p]);
+        for (uint32_t p = s.up[k]; p < s.up[k + 1]; p++) v = v - lane_read(prod, ucol[p]);
         const double yk = v / D.F[k * n + k];
         y = r == k ? yk : y;
     }
@@ -427,8 +453,12 @@ __device__ inline void pred_stage(const Pattern &P, LaneLU &s, const DenseLU &D)
         s.li[i] = g[T.li + i];
         s.ui[i] = g[T.ui + i];
     }
+    if (D.ucols != nullptr) {
+        const uint32_t nu_ = g[T.up + n];
+        for (uint32_t i = r; i < nu_ && i < lists; i += nt) D.ucols[i] = g[T.ui + i];
+    }
     for (uint32_t i = r; i < nnz; i += nt) s.prow[i] = g[T.prow + i];
-    for (uint32_t i = r; i < n * n; i += nt) D.amap[i] = g[T.amap + i];
+    for (uint32_t i = r; i < n * n; i += nt) D.amap[i] = g[T.amap + i] == ESP_NONE ? AMAP_NONE : (uint16_t)g[T.amap + i];
 }
 
 // The lane's first full factor (lane_solver.h lu_factor on a fresh solver:
@@ -451,6 +481,7 @@ __device__ inline void pred_stage(const Pattern &P, LaneLU &s, const DenseLU &D)
 template <class A>
 __device__ bool dense_first_factor(const Pattern &P, LaneLU &s, const DenseLU &D, const A &vals)
 {
+    const uint32_t *const ucol = dense_ucols(D, s);
     const uint32_t n = P.n, nnz = P.nnz, r = threadIdx.x, nt = blockDim.x;
     const uint32_t *g = D.pred;
     if (g == nullptr || g[PRED_OK] != 1) return false;
@@ -497,20 +528,20 @@ __device__ bool dense_first_factor(const Pattern &P, LaneLU &s, const DenseLU &D
     const uint32_t ub = own ? s.up[r] : 0, ue = own ? s.up[r + 1] : 0;
     bool bad = false;
     uint64_t scaled_steps = 0;
-    const uint32_t a0 = own ? D.amap[r] : ESP_NONE;
-    uint32_t a_next = own && n > 1 ? D.amap[n + r] : ESP_NONE;
+    const uint32_t a0 = own ? amap_slot(D, r) : ESP_NONE;
+    uint32_t a_next = own && n > 1 ? amap_slot(D, n + r) : ESP_NONE;
     double wv = a0 != ESP_NONE ? vals[a0] : 0.0;
     uint32_t u0 = lane_read_u32(ub, 0), u1 = lane_read_u32(ue, 0);
-    uint32_t uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+    uint32_t uidx = r < u1 - u0 ? ucol[u0 + r] : 0;
     for (uint32_t k = 0; k < n && !bad; k++) {
         double w = wv;
         const uint32_t cnt = u1 - u0, col_idx = uidx;
         if (k + 1 < n) {
             wv = a_next != ESP_NONE ? vals[a_next] : 0.0;
-            a_next = own && k + 2 < n ? D.amap[(k + 2) * n + r] : ESP_NONE;
+            a_next = own && k + 2 < n ? amap_slot(D, (k + 2) * n + r) : ESP_NONE;
             u0 = lane_read_u32(ub, k + 1);
             u1 = lane_read_u32(ue, k + 1);
-            uidx = r < u1 - u0 ? s.ui[u0 + r] : 0;
+            uidx = r < u1 - u0 ? ucol[u0 + r] : 0;
         }
         for (uint32_t j0 = 0; j0 < cnt; j0 += CHUNK) {
             uint32_t iv[CHUNK];

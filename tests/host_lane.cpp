@@ -256,14 +256,18 @@ void hl_layout(void *tv, uint32_t n_passes, uint64_t *out)
     build_wave(*t, (uint32_t)std::max<size_t>((size_t)t->n * (t->n - 1), 1), H);
     const WaveTemplate W = wave_template(H.at, H.blob.data());
     WaveLayout L{}, LT{};
-    out[0] = 4 * (uint64_t)H.at.words + 8 * wave_words(W, L);
-    out[1] = 4 * (uint64_t)H.at.words + 8 * wave_words(W, LT, true, tran_extra_words(W.n_inst, n_passes));
+    // the lean layout of templates with compiled kinds, as the engine picks it
+    bool va = false;
+    for (int k = 0; k < K_NKIND; k++) va = va || (kind_is_va(k) && t->count[k] > 0);
+    L.lu_global = LT.lu_global = va ? 1u : 0u;
+    out[0] = 4 * (uint64_t)table_lds_words(H.at.words, L) + 8 * wave_words(W, L);
+    out[1] = 4 * (uint64_t)table_lds_words(H.at.words, LT) + 8 * wave_words(W, LT, true, tran_extra_words(W.n_inst, n_passes));
     out[2] = 4 * (uint64_t)H.at.words;
     out[3] = H.n_prm;
     out[4] = H.n_prm_lds;
     out[5] = 8 * (uint64_t)(L.lu_u32 - L.lu_f64);
     out[6] = 8 * (uint64_t)(L.flags - L.lu_u32);
-    out[7] = L.dense == ESP_NONE ? 0 : 8 * (uint64_t)dense_words(t->n);
+    out[7] = L.dense == ESP_NONE ? 0 : 8 * (uint64_t)dense_words(t->n, va);
     out[8] = 8 * (uint64_t)(L.prm - L.stage);
 }
 }
