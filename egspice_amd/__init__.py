@@ -13,11 +13,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libespice_lanes.so")
 
-VSOURCE, ISOURCE, RESISTOR, CAPACITOR, DIODE, MOS1, PSP103 = range(7)
+VSOURCE, ISOURCE, RESISTOR, CAPACITOR, DIODE, MOS1, PSP103, BSIM4 = range(8)
 # user parameter counts per kind (include/espice_lanes.h kinds); a compiled
 # kind's count comes from the library (param_count)
 PARAM_COUNT = {VSOURCE: 71, ISOURCE: 71, RESISTOR: 1, CAPACITOR: 1, DIODE: 19, MOS1: 36}
-COMPILED_KINDS = (PSP103,)
+COMPILED_KINDS = (PSP103, BSIM4)
 VSOURCE_DC = 0
 DIODE_PARAMS = dict(IS=0, N=1, CJO=2, VJ=3, M=4, FC=5, TT=6, GMIN=7, AREA=8, TNOM=9, TEMP=10, DTEMP=11, EG=12, XTI=13,
                     RS=14, TRS=15, TRS2=16, TM1=17, TM2=18)

@@ -21,7 +21,9 @@
 namespace esp {
 
 // K_PSP103 and the kinds after it are compiled from Verilog-A (va_models.h).
-enum Kind : int { K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_PSP103 = 6, K_NKIND = 7 };
+enum Kind : int {
+    K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_PSP103 = 6, K_BSIM4 = 7, K_NKIND = 8
+};
 constexpr bool kind_is_va(int kind) { return kind >= K_PSP103; }
 
 // User parameter slots (NaN = not given).
@@ -70,10 +72,10 @@ enum { DIOP_IS = 0, DIOP_VTE, DIOP_GMIN, DIOP_VCRIT, DIOP_CJO, DIOP_VJ, DIOP_MT,
 // Local unknowns, ports, user and prepared parameter counts per kind.
 // The diode's unknowns are (a, c, ai): ai, the junction's anode behind RS,
 // is the row of a itself when RS is 0 (diode.va:10-11, the collapse).
-constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4, PSP103_NU};
-constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4, PSP103_NPORT};
-constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K, PSP103_NPARAM};
-constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K, PSP103_NPREP};
+constexpr int KIND_NU[K_NKIND] = {3, 2, 2, 2, 3, 4, PSP103_NU, BSIM4VA_NU};
+constexpr int KIND_NPORT[K_NKIND] = {2, 2, 2, 2, 2, 4, PSP103_NPORT, BSIM4VA_NPORT};
+constexpr int KIND_UK[K_NKIND] = {VSRC_K, ISRC_K, RES_K, CAP_K, DIO_K, M1_K, PSP103_NPARAM, BSIM4VA_NPARAM};
+constexpr int KIND_PK[K_NKIND] = {SRCP_K, SRCP_K, 2, 1, DIOP_K, M1P_K, PSP103_NPREP, BSIM4VA_NPREP};
 // room for one instance's user or prepared parameters on the host (the
 // hand-written kinds; a compiled kind sizes its own)
 constexpr int PARAM_MAX = 128;
@@ -81,14 +83,15 @@ static_assert(VSRC_K <= PARAM_MAX && SRCP_K <= PARAM_MAX && DIO_K <= PARAM_MAX &
                   M1_K <= PARAM_MAX && M1P_K <= PARAM_MAX,
               "every kind's parameters fit PARAM_MAX");
 // Residual rows a kind writes in a static evaluation.
-constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu, (1u << PSP103_NU) - 1u};
+constexpr unsigned KIND_RMASK[K_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu, (1u << PSP103_NU) - 1u,
+                                          (1u << BSIM4VA_NU) - 1u};
 // Unknowns a kind's limiter writes (its limited image).
-constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x4u, 0xbu, 0u};
+constexpr unsigned KIND_LIMW[K_NKIND] = {0u, 0u, 0u, 0u, 0x4u, 0xbu, PSP103_LIMW, BSIM4VA_LIMW};
 // The local row whose charge is the kind's LTE site (one ddt per capacitor
 // and per diode), or -1; a compiled kind has one site per ddt() instead.
-constexpr int KIND_TAPE_ROW[K_NKIND] = {-1, -1, -1, 0, 2, -1, -1};
+constexpr int KIND_TAPE_ROW[K_NKIND] = {-1, -1, -1, 0, 2, -1, -1, -1};
 // LTE charge sites per instance.
-constexpr int KIND_NSITE[K_NKIND] = {0, 0, 0, 1, 1, 0, PSP103_NSITE};
+constexpr int KIND_NSITE[K_NKIND] = {0, 0, 0, 1, 1, 0, PSP103_NSITE, BSIM4VA_NSITE};
 
 // Structural G-plane and C-plane entries per local row (bit masks over
 // local columns). The pattern and slot tapes use their union.
@@ -100,6 +103,7 @@ ESP_HD unsigned kind_gmask(int kind, int ru)
     case K_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case K_MOS1: return ru == 1 ? 0x0u : 0xfu;
     case K_PSP103: return psp103_gmask[ru];
+    case K_BSIM4: return bsim4va_gmask[ru];
     default: return 0u;
     }
 }
@@ -110,6 +114,7 @@ ESP_HD unsigned kind_cmask(int kind, int ru)
     case K_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case K_MOS1: return 0xfu;
     case K_PSP103: return psp103_cmask[ru];
+    case K_BSIM4: return bsim4va_cmask[ru];
     default: (void)ru; return 0u;
     }
 }
@@ -126,13 +131,33 @@ inline int va_setup(int kind, const double *u, double *p)
 {
     switch (kind) {
     case K_PSP103: return psp103_setup(u, TEMPERATURE_K, VA_SIMPARAM_GMIN, p);
+    case K_BSIM4: return bsim4va_setup(u, TEMPERATURE_K, VA_SIMPARAM_GMIN, p);
     default: return -1;
     }
 }
-inline int va_ncoll(int kind) { return kind == K_PSP103 ? PSP103_NCOLL : 0; }
-inline int va_coll_base(int kind) { return kind == K_PSP103 ? PSP103_COLL_BASE : 0; }
-inline const int *va_coll_pairs(int kind) { return kind == K_PSP103 ? psp103_collapse : nullptr; }
-inline const char *const *va_param_names(int kind) { return kind == K_PSP103 ? psp103_param_names : nullptr; }
+inline int va_ncoll(int kind) { return kind == K_PSP103 ? PSP103_NCOLL : kind == K_BSIM4 ? BSIM4VA_NCOLL : 0; }
+inline int va_coll_base(int kind)
+{
+    return kind == K_PSP103 ? PSP103_COLL_BASE : kind == K_BSIM4 ? BSIM4VA_COLL_BASE : 0;
+}
+inline const int *va_coll_pairs(int kind)
+{
+    return kind == K_PSP103 ? psp103_collapse : kind == K_BSIM4 ? bsim4va_collapse : nullptr;
+}
+inline const char *const *va_param_names(int kind)
+{
+    return kind == K_PSP103 ? psp103_param_names : kind == K_BSIM4 ? bsim4va_param_names : nullptr;
+}
+// aliasparam names of a compiled kind and the parameter each stands for
+inline int va_nalias(int kind) { return kind == K_PSP103 ? PSP103_NALIAS : kind == K_BSIM4 ? BSIM4VA_NALIAS : 0; }
+inline const char *const *va_alias_names(int kind)
+{
+    return kind == K_PSP103 ? psp103_alias_names : kind == K_BSIM4 ? bsim4va_alias_names : nullptr;
+}
+inline const int *va_alias_target(int kind)
+{
+    return kind == K_PSP103 ? psp103_alias_target : kind == K_BSIM4 ? bsim4va_alias_target : nullptr;
+}
 
 // The unknowns of one compiled instance: ports as given, internal unknowns
 // collapsed (union-find over the pairs its prepared constants switch on,
@@ -399,92 +424,8 @@ inline void prep_kind(int kind, const double *u, double *p)
 }
 
 // ------------------------------------------------------------- limiting
-
-ESP_HD double fetlim(double vnew, double vold, double vto)
-{
-    double vtsthi = fabs_(2 * (vold - vto)) + 2;
-    double vtstlo = vtsthi / 2 + 2;
-    double vtox = vto + 3.5;
-    double delv = vnew - vold;
-    if (vold >= vto) {
-        if (vold >= vtox) {
-            if (delv <= 0) {
-                if (vnew >= vtox) {
-                    if (-delv > vtstlo) vnew = vold - vtstlo;
-                } else {
-                    vnew = va_max(vnew, vto + 2);
-                }
-            } else {
-                if (delv >= vtsthi) vnew = vold + vtsthi;
-            }
-        } else {
-            if (delv <= 0) vnew = va_max(vnew, vto - 0.5);
-            else vnew = va_min(vnew, vto + 4);
-        }
-    } else {
-        if (delv <= 0) {
-            if (-delv > vtsthi) vnew = vold - vtsthi;
-        } else {
-            double vtemp = vto + 0.5;
-            if (vnew <= vtemp) {
-                if (delv > vtstlo) vnew = vold + vtstlo;
-            } else {
-                vnew = vtemp;
-            }
-        }
-    }
-    return vnew;
-}
-
-ESP_HD double limvds(double vnew, double vold)
-{
-    if (vold >= 3.5) {
-        if (vnew > vold) vnew = va_min(vnew, 3 * vold + 2);
-        else if (vnew < 3.5) vnew = va_max(vnew, 2);
-    } else {
-        if (vnew > vold) vnew = va_min(vnew, 4);
-        else vnew = va_max(vnew, -0.5);
-    }
-    return vnew;
-}
-
-ESP_HD double pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck)
-{
-    if (vnew > vcrit && fabs_(vnew - vold) > vt + vt) {
-        if (vold > 0) {
-            double arg = 1 + (vnew - vold) / vt;
-            if (arg > 0) vnew = vold + vt * e_log(arg);
-            else vnew = vcrit;
-        } else {
-            vnew = vt * e_log(vnew / vt);
-        }
-        *icheck = 1;
-    } else {
-        if (vnew < 0) {
-            double arg = vold > 0 ? -1 * vold - 1 : 2 * vold - 1;
-            if (vnew < arg) {
-                vnew = arg;
-                *icheck = 1;
-            } else {
-                *icheck = 0;
-            }
-        } else {
-            *icheck = 0;
-        }
-    }
-    return vnew;
-}
-
-ESP_HD void snap_limited(int nu, unsigned reads, unsigned writes, double *lx, const double *cur)
-{
-    double scale = 0;
-    for (int u = 0; u < nu; u++)
-        if (reads >> u & 1) scale = fmax_(scale, fabs_(cur[u]));
-    double tol = 16 * 2.220446049250313e-16 * scale;
-    for (int u = 0; u < nu; u++)
-        if ((writes & reads) >> u & 1)
-            if (fabs_(lx[u] - cur[u]) <= tol) lx[u] = cur[u];
-}
+// fetlim, limvds, pnjlim and snap_limited: limiting.h (included through
+// va_models.h, whose compiled kinds limit with them too).
 
 // -------------------------------------------------------------- sources
 

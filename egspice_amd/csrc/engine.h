@@ -122,6 +122,7 @@ ESP_HD void eval_planes(const Batch &B, uint32_t l, bool lim_active)
     for (uint32_t i = 0; i <= n; i++) rhs[i] = 0.0;
     for (int t = 0; t < B.n_order; t++) {
         const int kind = B.order[t];
+        if (kind_is_va(kind)) continue;  // the workgroup-per-lane kernels only
         const int nu = KIND_NU[kind], pk = KIND_PK[kind];
         const unsigned limw = KIND_LIMW[kind];
         for (uint32_t id = 0; id < B.count[kind]; id++) {
@@ -180,7 +181,8 @@ ESP_HD bool limit_pass(const Batch &B, uint32_t l, bool lim_active)
     for (int t = 0; t < B.n_order; t++) {
         const int kind = B.order[t];
         const unsigned limw = KIND_LIMW[kind];
-        if (limw == 0) continue;
+        // compiled kinds run in the workgroup-per-lane kernels only
+        if (limw == 0 || kind_is_va(kind)) continue;
         const int nu = KIND_NU[kind], pk = KIND_PK[kind];
         for (uint32_t id = 0; id < B.count[kind]; id++) {
             const uint32_t *nd = B.nodes[kind] + (size_t)id * nu;
@@ -209,7 +211,8 @@ ESP_HD void seed_pass(const Batch &B, uint32_t l)
     for (int t = 0; t < B.n_order; t++) {
         const int kind = B.order[t];
         const unsigned limw = KIND_LIMW[kind];
-        if (limw == 0) continue;
+        // compiled kinds run in the workgroup-per-lane kernels only
+        if (limw == 0 || kind_is_va(kind)) continue;
         const int nu = KIND_NU[kind], pk = KIND_PK[kind];
         for (uint32_t id = 0; id < B.count[kind]; id++) {
             const uint32_t *nd = B.nodes[kind] + (size_t)id * nu;

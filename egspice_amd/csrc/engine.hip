@@ -161,7 +161,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     __syncthreads();
     bool lim_active = false;
     if (o.init_fix) {
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst<VA>(W, s, e);
         lim_active = true;
     }
     const ReduceCache rc = wave_reduce_cache(W, tid);
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
                 break;
             }
             int lim_any = 0;
-            for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
+            for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst<VA>(W, s, e, lim_active) ? 1 : 0;
             lim_any = __syncthreads_or(lim_any);
             lim_active = true;
             if (tid == 0) {
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
     __syncthreads();
     if (tid == 0) wave_fresh_solver(W, s);
     __syncthreads();
-    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst<VA>(W, s, e);
     const ReduceCache rc = wave_reduce_cache(W, tid);
     __syncthreads();
     for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<false, VA>(W, s, e, true);
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     }
     __syncthreads();
     if (!to.op_warm)
-        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst(W, s, e);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst<VA>(W, s, e);
     const ReduceCache rc = wave_reduce_cache(W, tid);
     __syncthreads();
 
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         // accepted point, the limiters cleared by the previous solve
         // (tran.zig:1185-1189, eval.zig:2430-2434); Newton then starts with
         // limiting active
-        for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst(W, s, e, false);
+        for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst<VA>(W, s, e, false);
         __syncthreads();
         WaveHook h{method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
         const int r = wave_newton<true, true, VA>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
@@ -829,12 +829,11 @@ espice_status espice_lanes_param_index(uint32_t kind, const char *name, uint32_t
             *index = (uint32_t)k;
             return ESPICE_OK;
         }
-    if (kind == K_PSP103)
-        for (int k = 0; k < PSP103_NALIAS; k++)
-            if (strcasecmp(psp103_alias_names[k], name) == 0) {
-                *index = (uint32_t)psp103_alias_target[k];
-                return ESPICE_OK;
-            }
+    for (int k = 0; k < va_nalias((int)kind); k++)
+        if (strcasecmp(va_alias_names((int)kind)[k], name) == 0) {
+            *index = (uint32_t)va_alias_target((int)kind)[k];
+            return ESPICE_OK;
+        }
     return fail(ESPICE_INVALID_ARGUMENT, "UnknownParam");
 }
 

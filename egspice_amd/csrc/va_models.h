@@ -12,9 +12,13 @@
 //
 //   psp103   models/psp103.va (module PSP103VA; D G S B, internal NOI GP SI
 //            DI BP BI BS BD; PSP 103.8.2 with JUNCAP 200.6.2)
+//   bsim4va  models/bsim4va.va (BSIM 4.8; D G S B, internal DI GI SI BI GM
+//            SBULK DBULK, most collapsed by rgateMod/rbodyMod/rdsMod); its
+//            $limit group limits the gate, drain and bulk legs
 #pragma once
 
 #include "emath.h"
+#include "limiting.h"
 
 namespace esp {
 
@@ -31,13 +35,22 @@ namespace esp {
 #define VA_FLOOR __builtin_floor
 #define VA_CEIL __builtin_ceil
 #define VA_FMA __builtin_fma
+// parameter arithmetic only (a model's setup, which the lane engine runs on
+// the host): the C library's
+#define VA_ATAN atan
 #define VA_INF __builtin_huge_val()
 #define VA_IDIV(a, b) ((b) ? (a) / (b) : 0)
 #define VA_IMOD(a, b) ((b) ? (a) % (b) : 0)
 // real to integer: nearest, ties away from zero (LRM 4.2.1.1)
 #define VA_RTOI(x) ((int)__builtin_round(x))
 
+#define VA_FETLIM fetlim
+#define VA_LIMVDS limvds
+#define VA_PNJLIM pnjlim
+#define VA_SNAP snap_limited
+
 #include "gen/psp103.h"
+#include "gen/bsim4va.h"
 
 #undef VA_FN
 #undef VA_TABLE
@@ -52,10 +65,15 @@ namespace esp {
 #undef VA_FLOOR
 #undef VA_CEIL
 #undef VA_FMA
+#undef VA_ATAN
 #undef VA_INF
 #undef VA_IDIV
 #undef VA_IMOD
 #undef VA_RTOI
+#undef VA_FETLIM
+#undef VA_LIMVDS
+#undef VA_PNJLIM
+#undef VA_SNAP
 
 // $simparam("gmin") the models read: the deck's gmin option, 1e-12 S by
 // default (src/core/numerics.zig:386, src/device/abi.zig:408).

@@ -355,19 +355,74 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
 // then the G mask's entries row-major; transient: charges, the C mask's
 // entries, and one charge per ddt() site into the lane's charge tape).
 template <int KIND, bool TRAN>
-ESP_HD void wave_eval_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e)
+ESP_HD void va_eval_kind(const double *prm, const double *x, double *res, double *jac, double *q, double *cj,
+                         double *site)
+{
+    if constexpr (KIND == K_PSP103) {
+        if (TRAN) psp103_eval_tran(prm, x, res, jac, q, cj, site);
+        else psp103_eval(prm, x, res, jac);
+    } else if constexpr (KIND == K_BSIM4) {
+        if (TRAN) bsim4va_eval_tran(prm, x, res, jac, q, cj, site);
+        else bsim4va_eval(prm, x, res, jac);
+    }
+}
+
+// A compiled kind's evaluation into its staging cells. With limiting active
+// it runs at the limited image of the unknowns its limiter writes and, as
+// for the hand-written kinds, adds x minus the image dotted with each row's
+// gradient to the residual and (transient) the charge rows; the charge
+// sites stay uncorrected (only the re-read at a solution, unlimited, feeds
+// the step control).
+template <int KIND, bool TRAN>
+ESP_HD void wave_eval_va(const WaveTemplate &W, WaveLane &s, const InstEntry &ie, uint32_t e, bool lim_active)
 {
     constexpr int nu = KIND_NU[KIND];
+    constexpr unsigned limw = KIND_LIMW[KIND];
     const uint32_t *nd = W.nodes_flat + W.node_off[e];
     double x[nu];
 #pragma unroll
     for (int u = 0; u < nu; u++) x[u] = s.x[nd[u]];
     const double *prm = s.gprm + ie.prm;
     double *res = s.stage + ie.stage_res, *jac = s.stage + ie.stage_jac;
-    if constexpr (KIND == K_PSP103) {
-        if (TRAN) psp103_eval_tran(prm, x, res, jac, s.stage + ie.stage_q, s.stage + ie.stage_c, s.tape + ie.tape);
-        else psp103_eval(prm, x, res, jac);
+    double *q = s.stage + ie.stage_q, *cj = s.stage + ie.stage_c;
+    if constexpr (limw != 0) {
+        if (lim_active) {
+            double corr[nu];
+            bool live = false;
+#pragma unroll
+            for (int u = 0; u < nu; u++) {
+                const double xi = x[u];
+                if (limw >> u & 1) x[u] = s.lim[ie.lim + u];
+                corr[u] = xi - x[u];
+                live = live || corr[u] != 0;
+            }
+            va_eval_kind<KIND, TRAN>(prm, x, res, jac, q, cj, s.tape + ie.tape);
+            if (live) {
+                uint32_t k = 0;
+#pragma unroll
+                for (int ru = 0; ru < nu; ru++) {
+                    const unsigned gm = kind_gmask(KIND, ru);
+                    double acc = 0;
+                    for (int cu = 0; cu < nu; cu++)
+                        if (gm >> cu & 1) acc += jac[k++] * corr[cu];
+                    res[ru] += acc;
+                }
+                if (TRAN) {
+                    uint32_t kc = 0;
+#pragma unroll
+                    for (int ru = 0; ru < nu; ru++) {
+                        const unsigned cm = kind_cmask(KIND, ru);
+                        double acc = 0;
+                        for (int cu = 0; cu < nu; cu++)
+                            if (cm >> cu & 1) acc += cj[kc++] * corr[cu];
+                        q[ru] += acc;
+                    }
+                }
+            }
+            return;
+        }
     }
+    va_eval_kind<KIND, TRAN>(prm, x, res, jac, q, cj, s.tape + ie.tape);
 }
 
 // A compiled kind's charge rows and charge sites alone, values only (the
@@ -383,6 +438,7 @@ ESP_HD void wave_charge_va(const WaveTemplate &W, WaveLane &s, const InstEntry &
     for (int u = 0; u < nu; u++) x[u] = s.x[nd[u]];
     const double *prm = s.gprm + ie.prm;
     if constexpr (KIND == K_PSP103) psp103_eval_charge(prm, x, s.stage + ie.stage_q, s.tape + ie.tape);
+    else if constexpr (KIND == K_BSIM4) bsim4va_eval_charge(prm, x, s.stage + ie.stage_q, s.tape + ie.tape);
 }
 
 // TRAN: sources read their waveform at t and charges are staged too. VA:
@@ -399,7 +455,12 @@ ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool 
     if constexpr (VA) {
         if (ie.kind == K_PSP103) {
             if constexpr (QONLY) wave_charge_va<K_PSP103>(W, s, ie, e);
-            else wave_eval_va<K_PSP103, TRAN>(W, s, ie, e);
+            else wave_eval_va<K_PSP103, TRAN>(W, s, ie, e, lim_active);
+            return;
+        }
+        if (ie.kind == K_BSIM4) {
+            if constexpr (QONLY) wave_charge_va<K_BSIM4>(W, s, ie, e);
+            else wave_eval_va<K_BSIM4, TRAN>(W, s, ie, e, lim_active);
             return;
         }
     }
@@ -517,7 +578,6 @@ ESP_HD bool wave_limit_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie
     constexpr int nu = KIND_NU[KIND];
     constexpr unsigned limw = KIND_LIMW[KIND];
     const uint32_t *nd = W.nodes_flat + W.node_off[e];
-    LP p{s.prm + ie.prm};
     double cur[nu], old[nu], lim[nu];
 #pragma unroll
     for (int u = 0; u < nu; u++) {
@@ -525,20 +585,31 @@ ESP_HD bool wave_limit_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie
         old[u] = ((limw >> u & 1) && lim_active) ? s.lim[ie.lim + u] : s.x_old[nd[u]];
     }
     int ch;
-    if constexpr (KIND == K_MOS1) ch = limit_mos1(p, cur, old, lim);
-    else ch = limit_diode(p, cur, old, lim);
+    if constexpr (KIND == K_BSIM4) {
+        ch = bsim4va_limit(s.gprm + ie.prm, cur, old, lim);
+    } else {
+        LP p{s.prm + ie.prm};
+        if constexpr (KIND == K_MOS1) ch = limit_mos1(p, cur, old, lim);
+        else ch = limit_diode(p, cur, old, lim);
+    }
 #pragma unroll
     for (int u = 0; u < nu; u++)
         if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
     return ch != 0;
 }
 
+// VA: compiled kinds may be present (their limiters are instantiated only
+// then, as their evaluations are).
+template <bool VA = false>
 ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
 {
     const InstEntry ie = W.inst[e];
     switch (ie.kind) {
     case K_DIODE: return wave_limit_k<K_DIODE>(W, s, ie, e, lim_active);
     case K_MOS1: return wave_limit_k<K_MOS1>(W, s, ie, e, lim_active);
+    case K_BSIM4:
+        if constexpr (VA) return wave_limit_k<K_BSIM4>(W, s, ie, e, lim_active);
+        return false;
     default: return false;
     }
 }
@@ -549,23 +620,31 @@ ESP_HD void wave_seed_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
     constexpr int nu = KIND_NU[KIND];
     constexpr unsigned limw = KIND_LIMW[KIND];
     const uint32_t *nd = W.nodes_flat + W.node_off[e];
-    LP p{s.prm + ie.prm};
     double cur[nu], lim[nu];
 #pragma unroll
     for (int u = 0; u < nu; u++) cur[u] = s.x[nd[u]];
-    if constexpr (KIND == K_MOS1) seed_mos1(p, cur, lim);
-    else seed_diode(p, cur, lim);
+    if constexpr (KIND == K_BSIM4) {
+        bsim4va_seed(s.gprm + ie.prm, cur, lim);
+    } else {
+        LP p{s.prm + ie.prm};
+        if constexpr (KIND == K_MOS1) seed_mos1(p, cur, lim);
+        else seed_diode(p, cur, lim);
+    }
 #pragma unroll
     for (int u = 0; u < nu; u++)
         if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
 }
 
+template <bool VA = false>
 ESP_HD void wave_seed_inst(const WaveTemplate &W, WaveLane &s, uint32_t e)
 {
     const InstEntry ie = W.inst[e];
     switch (ie.kind) {
     case K_DIODE: wave_seed_k<K_DIODE>(W, s, ie, e); break;
     case K_MOS1: wave_seed_k<K_MOS1>(W, s, ie, e); break;
+    case K_BSIM4:
+        if constexpr (VA) wave_seed_k<K_BSIM4>(W, s, ie, e);
+        break;
     default: break;
     }
 }

@@ -84,7 +84,7 @@ __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, c
             return NEWTON_SINGULAR;
         }
         int lim_any = 0;
-        for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst(W, s, e, lim_active) ? 1 : 0;
+        for (uint32_t e = tid; e < W.n_inst; e += nt) lim_any |= wave_limit_inst<VA>(W, s, e, lim_active) ? 1 : 0;
         lim_any = __syncthreads_or(lim_any);
         lim_active = true;
         if (tid == 0) {

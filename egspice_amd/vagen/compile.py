@@ -50,7 +50,7 @@ def walk_expr(e, fn):
     elif k == "call":
         for a in e[2]:
             walk_expr(a, fn)
-    elif k == "ddt":
+    elif k in ("ddt", "ddx"):
         walk_expr(e[1], fn)
 
 
@@ -74,12 +74,14 @@ def has_ddt(e):
 
 
 def split_ddt(e, sign=1):
-    """A contribution's expression as (resistive part or None, [(sign, q)])."""
+    """A contribution's expression as (resistive part or None, [(sign, q,
+    factor)]): q is the charge of one ddt() term (times its constant factor,
+    or factor None)."""
     k = e[0]
     if k == "ddt":
         if has_ddt(e[1]):
             raise ParseError("nested ddt")
-        return None, [(sign, e[1])]
+        return None, [(sign, e[1], None)]
     if not has_ddt(e):
         return (e if sign > 0 else ("un", "-", e)), []
     if k == "un" and e[1] == "-":
@@ -94,7 +96,170 @@ def split_ddt(e, sign=1):
         else:
             r = ("bin", "+", ra, rb)
         return r, qa + qb
+    if k == "bin" and e[1] == "*" and (e[2][0] == "ddt") != (e[3][0] == "ddt"):
+        # k * ddt(q) = ddt(k * q) for a time-invariant k (checked by the
+        # compiler: no probe, no state)
+        kf, d = (e[3], e[2]) if e[2][0] == "ddt" else (e[2], e[3])
+        if has_ddt(kf) or has_ddt(d[1]):
+            raise ParseError("nested ddt")
+        return None, [(sign, ("bin", "*", kf, d[1]), kf)]
     raise ParseError("ddt() must be an added term of a contribution")
+
+
+def map_expr(e, fn):
+    """e with every ('var', name) node replaced by fn(name)."""
+    k = e[0]
+    if k == "var":
+        return fn(e[1])
+    if k == "un":
+        return (k, e[1], map_expr(e[2], fn))
+    if k == "bin":
+        return (k, e[1], map_expr(e[2], fn), map_expr(e[3], fn))
+    if k == "tern":
+        return (k, map_expr(e[1], fn), map_expr(e[2], fn), map_expr(e[3], fn))
+    if k == "call":
+        return (k, e[1], [map_expr(a, fn) for a in e[2]])
+    if k == "ddt":
+        return (k, map_expr(e[1], fn))
+    if k == "ddx":
+        return (k, map_expr(e[1], fn), e[2])
+    return e
+
+
+def split_webs(body, vartype):
+    """Renames every variable apart into its webs (the definitions that
+    reach a common use, closed under sharing a use), so that a temporary
+    reused for parameter arithmetic and later for bias-dependent values is
+    two variables, and the bias-dependence split sees each on its own.
+    Reaching definitions over the structured body (if: both branches
+    merge; while: to a fixed point over the back edge); a variable read
+    before any assignment reaches from its initial value. Returns the new
+    body; vartype gains the new names."""
+    parent = {}
+
+    def find(d):
+        while parent[d] != d:
+            parent[d] = parent[parent[d]]
+            d = parent[d]
+        return d
+
+    def union(a, b):
+        a, b = find(a), find(b)
+        if a != b:
+            parent[max(a, b)] = min(a, b)
+
+    defs = {}  # id(statement) -> def id
+    counter = [0]
+
+    def new_def(key):
+        if key not in defs:
+            defs[key] = counter[0]
+            parent[counter[0]] = counter[0]
+            counter[0] += 1
+        return defs[key]
+
+    def init_def(v):
+        return new_def(("init", v))
+
+    def reads(e):
+        vs, _ = expr_reads(e)
+        return vs
+
+    def use(v, reach):
+        ds = reach.get(v)
+        if ds is None:
+            ds = {init_def(v)}
+        it = iter(ds)
+        first = next(it)
+        for d in it:
+            union(first, d)
+        return first
+
+    def merge(a, b):
+        out = dict(a)
+        for v, ds in b.items():
+            out[v] = out.get(v, set()) | ds if v in out else set(ds)
+        for v in a:
+            if v not in b:
+                out[v] = set(a[v]) | {init_def(v)}
+        for v in b:
+            if v not in a:
+                out[v] = out[v] | {init_def(v)}
+        return out
+
+    def walk(stmts, reach, rename):
+        """One pass; with rename, returns the rewritten statements too."""
+        out = []
+        for s in stmts:
+            k = s[0]
+
+            def rx(e):
+                for v in reads(e):
+                    use(v, reach)
+                if rename:
+                    return map_expr(e, lambda v: ("var", name_of(v, find(use(v, reach)))))
+                return e
+
+            if k == "assign":
+                e = rx(s[2])
+                d = new_def(("d", id(s)))
+                reach[s[1]] = {d}
+                out.append(("assign", name_of(s[1], find(d)) if rename else s[1], e))
+            elif k == "limit":
+                pr = rx(s[3])
+                args = [rx(a) for a in s[4]]
+                d = new_def(("d", id(s)))
+                reach[s[1]] = {d}
+                out.append(("limit", name_of(s[1], find(d)) if rename else s[1], s[2], pr, args))
+            elif k == "contrib":
+                out.append(("contrib", s[1], s[2], rx(s[3])))
+            elif k == "collapse":
+                out.append(s)
+            elif k == "if":
+                c = rx(s[1])
+                r1 = dict((v, set(ds)) for v, ds in reach.items())
+                r2 = dict((v, set(ds)) for v, ds in reach.items())
+                t = walk(s[2], r1, rename)
+                f = walk(s[3], r2, rename)
+                reach.clear()
+                reach.update(merge(r1, r2))
+                out.append(("if", c, t, f))
+            elif k == "while":
+                head = dict((v, set(ds)) for v, ds in reach.items())
+                while True:
+                    body_in = dict((v, set(ds)) for v, ds in head.items())
+                    for v in reads(s[1]):
+                        use(v, body_in)
+                    walk(s[2], body_in, False)
+                    nxt = merge(head, body_in)
+                    if all(nxt.get(v) == head.get(v) for v in set(nxt) | set(head)):
+                        break
+                    head = nxt
+                reach.clear()
+                reach.update(head)
+                if rename:
+                    c = map_expr(s[1], lambda v: ("var", name_of(v, find(use(v, dict(head))))))
+                    inner = dict((v, set(ds)) for v, ds in head.items())
+                    b = walk(s[2], inner, True)
+                    out.append(("while", c, b))
+                else:
+                    out.append(s)
+            else:
+                raise ParseError(f"statement {k}")
+        return out
+
+    names = {}
+
+    def name_of(v, web):
+        key = (v, web)
+        if key not in names:
+            taken = sum(1 for (w, _) in names if w == v)
+            names[key] = v if taken == 0 else f"{v}__w{taken}"
+            vartype[names[key]] = vartype[v]
+        return names[key]
+
+    walk(body, {}, False)
+    return walk(body, {}, True)
 
 
 class Stmt:
@@ -114,7 +279,8 @@ class Compiler:
         self.nu = len(model.unknowns)
         self.flat = []  # every leaf statement (assign/contrib/collapse) as Stmt
         self.pos = 0
-        self.tree = self.index(model.body, (), ())
+        self.body = split_webs(model.body, model.vartype)
+        self.tree = self.index(self.body, (), ())
         self.types()
         self.probes()
         self.classify()
@@ -201,7 +367,6 @@ class Compiler:
             es.append(st.a[3])
         elif k == "limit":
             es.append(st.a[3])
-            es.extend(st.a[4])
         return es
 
     def classify(self):
@@ -228,7 +393,7 @@ class Compiler:
             for st in self.flat:
                 if not self.is_dynamic_stmt(st):
                     continue
-                for e in self.stmt_reads(st):
+                for e in self.stmt_reads(st) + (list(st.a[4]) if st.kind == "limit" else []):
                     vs, _ = expr_reads(e)
                     for v in vs:
                         if v in self.dyn or v not in assigns:
@@ -276,21 +441,31 @@ class Compiler:
         for st in self.flat:
             if st.kind == "contrib":
                 r, qs = split_ddt(st.a[3])
+                for _, _, kf in qs:
+                    if kf is not None and self.expr_dyn(kf):
+                        raise ParseError("a bias-dependent factor of ddt()")
                 self.contribs.append((st, r, qs))
         self.rel = {}
-        for mode in ("dc", "tran", "charge"):
+        self.limits = [st for st in self.flat if st.kind == "limit"]
+        for mode in ("dc", "tran", "charge", "limitargs"):
             need = set()
             seeds = []
+            if mode == "limitargs":
+                # the limiter's arguments (evaluated at the limiter's own
+                # iterate, values only)
+                for st in self.limits:
+                    seeds.extend(st.conds)
+                    seeds.extend(st.a[4])
             for st, r, qs in self.contribs:
-                if mode == "charge" and not qs:
+                if mode == "limitargs" or (mode == "charge" and not qs):
                     continue
                 seeds.extend(st.conds)
                 if r is not None and mode != "charge":
                     seeds.append(r)
                 if mode != "dc":
-                    seeds.extend(q for _, q in qs)
+                    seeds.extend(q for _, q, _ in qs)
             for st in self.flat:
-                if st.kind == "collapse":
+                if st.kind == "collapse" and mode != "limitargs":
                     seeds.extend(st.conds)
             work = []
             for e in seeds:
@@ -311,6 +486,20 @@ class Compiler:
                         vs, _ = expr_reads(e)
                         work.extend(vs - need)
             self.rel[mode] = need
+        # does a ddx() sit on a statement a mode keeps?
+        self.ddx_in = {}
+        for mode in self.rel:
+            hit = False
+            for st in self.flat:
+                v = self.assigned(st)
+                if v is None or v not in self.rel[mode]:
+                    continue
+                for e in self.stmt_reads(st):
+                    walk_expr(e, lambda x: None)
+                    found = []
+                    walk_expr(e, lambda x: found.append(1) if x[0] == "ddx" else None)
+                    hit = hit or bool(found)
+            self.ddx_in[mode] = hit
 
     # -------------------------------------------------------------- lanes
     def lanes(self):
@@ -357,4 +546,5 @@ class Compiler:
             return out
         if k == "ddt":
             return self.elanes(e[1])
+        # ddx() is a value: the derivative lanes stop there
         return frozenset()

@@ -138,7 +138,26 @@ class Emitter:
         if k == "ddt":
             raise ParseError("ddt() outside a contribution")
         if k == "ddx":
-            raise ParseError("ddx() on a relevant path")
+            # the node partial from the branch-voltage lanes: a lane's pair
+            # (a, b) moves with node a (+) and against node b (-); a value
+            # without derivative lanes of its own
+            if self.mode == "setup":
+                raise ParseError("ddx() in static code")
+            x = self.as_real(self.ex(e[1]))
+            node = e[2]
+            terms = []
+            for l in sorted(x.d):
+                pa, pb = self.c.probe_pair[l]
+                if pa == node:
+                    terms.append(x.d[l])
+                elif pb == node:
+                    terms.append(f"-({x.d[l]})")
+            if not terms:
+                return Val("0.0")
+            acc = self.const(terms[0])
+            for t in terms[1:]:
+                acc = self.const(f"{acc} + {t}")
+            return Val(acc)
         if k == "un":
             a = self.ex(e[2])
             if e[1] == "-":
@@ -320,7 +339,14 @@ class Emitter:
                 return True
             if k == "assign":
                 v = st.a[1]
-                return v not in c.dyn and (v in c.rel["dc"] or v in c.rel["tran"])
+                return v not in c.dyn and any(v in c.rel[m] for m in ("dc", "tran", "limitargs"))
+            return False
+        if self.mode == "limitargs":
+            if k == "limit":
+                return True
+            if k == "assign":
+                v = st.a[1]
+                return v in c.dyn and v in c.rel["limitargs"]
             return False
         if k == "contrib":
             r, qs = split_ddt(st.a[3])
@@ -379,6 +405,22 @@ class Emitter:
     def leaf(self, st):
         k = st.kind
         c = self.c
+        if k == "limit" and self.mode == "limitargs":
+            # the limiter's arguments where the call stands in program order
+            base = self.limit_base[id(st)]
+            self.out("{")
+            self.ind += 1
+            for i, a in enumerate(st.a[4]):
+                val = self.as_real(self.ex(a))
+                self.out(f"la[{base + i}] = {val.v};")
+            self.ind -= 1
+            self.out("}")
+            return
+        if k == "limit":
+            # the evaluation runs at the limited image of the unknowns, so
+            # $limit(V(a, b), ...) reads V(a, b) there
+            st = type(st)("assign", ("assign", st.a[1], st.a[3]), st.pos, st.conds, st.loops)
+            k = "assign"
         if k == "assign":
             v = st.a[1]
             self.out("{")
@@ -410,12 +452,12 @@ class Emitter:
                 val = self.as_real(self.ex(r))
                 self.stamp(a, b, val, "res", "jac", self.jidx)
             if self.mode in ("tran", "charge"):
-                for sign, q in qs:
+                for i, (sign, q, _) in enumerate(qs):
                     val = self.as_real(self.ex(q))
                     if sign < 0:
                         val = Val(self.const(f"-({val.v})"), {l: self.const(f"-({x})") for l, x in val.d.items()})
                     self.stamp(a, b, val, "q", "cjac", self.cidx)
-                    self.out(f"site[{self.site_index[(id(st), id(q))]}] = {val.v};")
+                    self.out(f"site[{self.site_index[(id(st), i)]}] = {val.v};")
             self.ind -= 1
             self.out("}")
 
@@ -427,7 +469,7 @@ class Emitter:
             if node < 0:
                 continue
             self.out(f"{rv}[{node}] {'+' if sgn > 0 else '-'}= {val.v};")
-            if self.noad:
+            if self.noad or self.mode == "charge":
                 continue
             for l in sorted(val.d):
                 pa, pb = self.c.probe_pair[l]
@@ -456,8 +498,8 @@ def structure(c):
             for row in rows:
                 for col in cols(c.elanes(r)):
                     g.add((row, col))
-        for sign, q in qs:
-            sites.append((id(st), id(q)))
+        for i, (sign, q, _) in enumerate(qs):
+            sites.append((id(st), i))
             for row in rows:
                 for col in cols(c.elanes(q)):
                     cc.add((row, col))
@@ -478,13 +520,22 @@ def generate(c):
     pindex = {p["name"]: i for i, p in enumerate(params)}
 
     body = []
+    # the limiter's arguments, in call order
+    limit_base, nla = {}, 0
+    for st in c.limits:
+        limit_base[id(st)] = nla
+        nla += len(st.a[4])
+    modes = ("dc", "tran", "charge") + (("limitargs",) if c.limits else ())
     # ---------------------------------------------------------- eval
     evals = {}
     used_static = set()
-    for mode in ("dc", "tran", "charge"):
+    for mode in modes:
         e = Emitter(c)
         e.mode = mode
-        e.noad = mode == "charge"
+        e.limit_base = limit_base
+        # the charge values alone carry no derivative lanes, unless ddx()
+        # reads lanes on their path; the limiter's arguments are values
+        e.noad = (mode == "charge" and not c.ddx_in["charge"]) or (mode == "limitargs" and not c.ddx_in["limitargs"])
         e.used_static = set()
         e.jidx, e.cidx, e.site_index = jidx, cidx, site_index
         e.coll_base, e.coll_index = 0, {}
@@ -607,12 +658,15 @@ def generate(c):
     # eval functions: the static evaluation, the transient one, and the
     # transient's charge rows and sites alone, values only (the same
     # operations on the value path, so the same bits as eval_tran's)
-    for mode in ("dc", "tran", "charge"):
+    for mode in modes:
         lines, _ = evals[mode]
-        fname = {"dc": f"{P}_eval", "tran": f"{P}_eval_tran", "charge": f"{P}_eval_charge"}[mode]
+        fname = {"dc": f"{P}_eval", "tran": f"{P}_eval_tran", "charge": f"{P}_eval_charge",
+                 "limitargs": f"{P}_limit_args"}[mode]
         args = "const double *VA_RESTRICT prep, const double *VA_RESTRICT x"
         outs = []
-        if mode != "charge":
+        if mode == "limitargs":
+            args += ", double *VA_RESTRICT la"
+        if mode in ("dc", "tran"):
             args += ", double *VA_RESTRICT res_out, double *VA_RESTRICT jac_out"
             outs += [("res", nu), ("jac", len(gpat))]
         if mode == "tran":
@@ -624,7 +678,9 @@ def generate(c):
         H.append("")
         H.append(f"VA_FN void {fname}({args})")
         H.append("{")
-        H.append(f"#define VA_TRAN {0 if mode == 'dc' else 1}")
+        H.append(f"#define VA_TRAN {0 if mode in ('dc', 'limitargs') else 1}")
+        if mode == "limitargs":
+            H.append(f"    for (int k = 0; k < {nla}; k++) la[k] = 0.0;")
         for nm, cnt in outs:
             H.append(f"    double {nm}[{max(1, cnt)}];")
             H.append(f"    for (int k = 0; k < {max(1, cnt)}; k++) {nm}[k] = 0.0;")
@@ -645,7 +701,8 @@ def generate(c):
             if ty == "int":
                 H.append(f"    int v_{v} = 0;")
             else:
-                ls = sorted(c.vl.get(v, ())) if mode != "charge" else []
+                noad = mode in ("charge", "limitargs") and not c.ddx_in[mode]
+                ls = sorted(c.vl.get(v, ())) if not noad else []
                 H.append(f"    double v_{v} = 0.0" + "".join(f", v_{v}__{l} = 0.0" for l in ls) + ";")
         H.extend(lines)
         for nm, cnt in outs:
@@ -653,6 +710,7 @@ def generate(c):
                 H.append(f"    for (int k = 0; k < {cnt}; k++) {nm}_out[k] = {nm}[k];")
         H.append("#undef VA_TRAN")
         H.append("}")
+    H.extend(limit_functions(c, P, nu, limit_base, nla, [(st.a[1], st.a[2]) for st in collapses], coll_base))
     return "\n".join(H) + "\n", dict(nprep=nprep, nj=len(gpat), ncj=len(cpat), nsite=len(sites),
                                      ncoll=len(collapses), nparam=len(params), gpat=gpat, cpat=cpat)
 
@@ -660,3 +718,139 @@ def generate(c):
 def m_expr(m, e):
     """A parameter default or range bound normalised in the module scope."""
     return m.expr(e, [{}], [])
+
+
+def limit_functions(c, P, nu, limit_base, nla, pairs, coll_base):
+    """The limiter of a model's $limit calls, as the reference's VerA
+    groups them for a MOSFET (models/mos1.va:286-308): "fetlimds" on both
+    gate legs and "limvds" on the channel limit the leg that controls the
+    channel in the sign of the old vds, then derive the other; "pnjlimds"
+    on both bulk legs limits one junction, picked by the limited vds, and
+    moves only the bulk. The argument after the frame sign `type` is the
+    MODEINITJCT seed. P_limit writes the limited image of the unknowns
+    (node space: the source is kept, the gate, drain and bulk move) and
+    returns pnjlim's check; P_seed writes the seeds' image. An unknown
+    collapsed onto a limited one (the same node of the instance) takes its
+    image, so the evaluation reads one potential per node: P_LIMW names
+    the limited unknowns and every unknown a collapse may join to one. The
+    includer defines VA_FETLIM, VA_LIMVDS, VA_PNJLIM and VA_SNAP (the 16-ulp
+    snap of the image back onto the unknowns)."""
+    U = P.upper()
+    if not c.limits:
+        return [f"#define {U}_LIMW 0u"]
+    legs = {}
+    for st in c.limits:
+        fn = st.a[2]
+        a, b = st.a[3][1], st.a[3][2]
+        legs.setdefault(fn, []).append((a, b, limit_base[id(st)], len(st.a[4])))
+    if sorted(legs) != ["fetlimds", "limvds", "pnjlimds"] or len(legs["fetlimds"]) != 2 or \
+            len(legs["limvds"]) != 1 or len(legs["pnjlimds"]) != 2:
+        raise ParseError("unsupported $limit group: " + ", ".join(f"{k} x{len(v)}" for k, v in sorted(legs.items())))
+    (g1, s_, f1, n1), (g2, d_, f2, n2) = legs["fetlimds"]
+    (dv, sv, lv, nl), = legs["limvds"]
+    (b1, sb, p1, np1), (b2, db, p2, np2) = legs["pnjlimds"]
+    if not (g1 == g2 and dv == d_ and sv == s_ and sb == s_ and db == d_ and b1 == b2) or min(g1, s_, d_, b1) < 0:
+        raise ParseError("a $limit group whose legs do not share gate, drain, source and bulk")
+    if n1 < 3 or n2 < 2 or nl < 2 or np1 < 4 or np2 < 3:
+        raise ParseError("a $limit leg without its frame sign or seed")
+    g, d, src, b = g1, d_, s_, b1
+    writes = (1 << g) | (1 << d) | (1 << b)
+    reads = writes | (1 << src)
+    # unknowns some collapse can join to a limited one
+    comp = list(range(nu))
+
+    def root(u):
+        while comp[u] != u:
+            u = comp[u]
+        return u
+
+    for a_, b_ in pairs:
+        if a_ >= 0 and b_ >= 0:
+            ra, rb = root(a_), root(b_)
+            if ra != rb:
+                comp[max(ra, rb)] = min(ra, rb)
+    partners = [u for u in range(nu) if not (writes >> u & 1) and any(root(u) == root(w) for w in (g, d, b))]
+    if any(root(src) == root(w) for w in (g, d, b)):
+        raise ParseError("a $limit group whose source may collapse onto a limited node")
+    ext = writes | sum(1 << u for u in partners)
+
+    def partner_lines():
+        out = []
+        if not partners:
+            return out
+        out.append("    {")
+        out.append(f"        int rep[{nu}];")
+        out.append(f"        for (int u = 0; u < {nu}; u++) rep[u] = u;")
+        for k, (a_, b_) in enumerate(pairs):
+            if a_ < 0 or b_ < 0:
+                continue
+            out.append(f"        if (prep[{coll_base + k}] != 0.0) {{")
+            out.append(f"            int ra = {a_}, rb = {b_};")
+            out.append("            while (rep[ra] != ra) ra = rep[ra];")
+            out.append("            while (rep[rb] != rb) rb = rep[rb];")
+            out.append("            if (ra < rb) rep[rb] = ra; else if (rb < ra) rep[ra] = rb;")
+            out.append("        }")
+        out.append(f"        for (int u = 0; u < {nu}; u++)")
+        out.append("            while (rep[rep[u]] != rep[u]) rep[u] = rep[rep[u]];")
+        for u in partners:
+            for w in (g, d, b):
+                if root(u) == root(w):
+                    out.append(f"        if (rep[{u}] == rep[{w}]) lim[{u}] = lim[{w}];")
+        out.append("    }")
+        return out
+
+    L = []
+    L.append(f"#define {U}_LIMW {hex(ext)}u")
+    L.append("")
+    L.append(f"VA_FN int {P}_limit(const double *VA_RESTRICT prep, const double *VA_RESTRICT cur, const double *VA_RESTRICT old,")
+    L.append("                    double *VA_RESTRICT lim)")
+    L.append("{")
+    L.append(f"    double la[{max(1, nla)}];")
+    L.append(f"    {P}_limit_args(prep, old, la);")
+    L.append(f"    const double type = la[{f1 + 1}];")
+    L.append(f"    const double vgs_o = type * (old[{g}] - old[{src}]), vds_o = type * (old[{d}] - old[{src}]);")
+    L.append(f"    const double vbs_o = type * (old[{b}] - old[{src}]), vbd_o = type * (old[{b}] - old[{d}]);")
+    L.append("    const double vgd_o = vgs_o - vds_o;")
+    L.append(f"    double vgs = type * (cur[{g}] - cur[{src}]), vds = type * (cur[{d}] - cur[{src}]);")
+    L.append(f"    double vbs = type * (cur[{b}] - cur[{src}]), vbd = type * (cur[{b}] - cur[{d}]);")
+    L.append("    double vgd = vgs - vds;")
+    L.append("    if (vds_o >= 0.0) {")
+    L.append(f"        vgs = VA_FETLIM(vgs, vgs_o, la[{f1}]);")
+    L.append("        vds = vgs - vgd;")
+    L.append("        vds = VA_LIMVDS(vds, vds_o);")
+    L.append("    } else {")
+    L.append(f"        vgd = VA_FETLIM(vgd, vgd_o, la[{f2}]);")
+    L.append("        vds = vgs - vgd;")
+    L.append("        vds = -VA_LIMVDS(-vds, -vds_o);")
+    L.append("        vgs = vgd + vds;")
+    L.append("    }")
+    L.append("    int check = 0;")
+    L.append("    if (vds >= 0.0) {")
+    L.append(f"        vbs = VA_PNJLIM(vbs, vbs_o, la[{p1}], la[{p1 + 1}], &check);")
+    L.append("    } else {")
+    L.append(f"        vbd = VA_PNJLIM(vbd, vbd_o, la[{p2}], la[{p2 + 1}], &check);")
+    L.append("        vbs = vbd + vds;")
+    L.append("    }")
+    L.append(f"    for (int u = 0; u < {nu}; u++) lim[u] = cur[u];")
+    L.append(f"    const double s0 = cur[{src}];")
+    L.append(f"    lim[{d}] = s0 + type * vds;")
+    L.append(f"    lim[{g}] = s0 + type * vgs;")
+    L.append(f"    lim[{b}] = s0 + type * vbs;")
+    L.append(f"    VA_SNAP({nu}, {hex(reads)}u, {hex(writes)}u, lim, cur);")
+    L.extend(partner_lines())
+    L.append("    return check;")
+    L.append("}")
+    L.append("")
+    L.append(f"VA_FN void {P}_seed(const double *VA_RESTRICT prep, const double *VA_RESTRICT cur, double *VA_RESTRICT lim)")
+    L.append("{")
+    L.append(f"    double la[{max(1, nla)}];")
+    L.append(f"    {P}_limit_args(prep, cur, la);")
+    L.append(f"    const double type = la[{f1 + 1}];")
+    L.append(f"    for (int u = 0; u < {nu}; u++) lim[u] = cur[u];")
+    L.append(f"    const double s0 = cur[{src}];")
+    L.append(f"    lim[{d}] = s0 + type * la[{lv + 1}];")
+    L.append(f"    lim[{g}] = s0 + type * la[{f1 + 2}];")
+    L.append(f"    lim[{b}] = s0 + type * la[{p1 + 3}];")
+    L.extend(partner_lines())
+    L.append("}")
+    return L

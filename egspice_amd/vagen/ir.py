@@ -15,6 +15,7 @@ to are turned into IR nodes:
                ('analysis', tran?)    analysis("...") in the large-signal run
                ('call', fn, [args]) ('un', op, e) ('bin', op, a, b)
                ('tern', c, a, b) ('ddt', e)
+               ('ddx', e, n)          the partial of e in node n's potential
   statements   ('assign', var, e) ('if', c, [then], [else])
                ('while', c, [body]) ('contrib', a, b, e) ('collapse', a, b)
                ('limit', var, kind, e_probe, [args])   (bsim4's $limit)
@@ -161,7 +162,11 @@ class Model:
             if fn == "ddt":
                 return ("ddt", self.expr(args[0], scope, pre))
             if fn == "ddx":
-                return ("ddx",)
+                # ddx(x, V(n)): the partial of x in the potential of node n
+                if len(args) != 2 or args[1][0] != "call" or args[1][1] != "V" or len(args[1][2]) != 1:
+                    raise ParseError("ddx() takes an expression and V(node)")
+                n, _ = self.node_pair(args[1][2], "ddx")
+                return ("ddx", self.expr(args[0], scope, pre), n)
             if fn == "$temperature":
                 return ("temp",)
             if fn == "$vt":
@@ -171,6 +176,10 @@ class Model:
                 what = args[0][1] if args and args[0][0] == "str" else None
                 if what == "gmin":
                     return ("gmin",)
+                if what == "tnom":
+                    # the circuit's nominal temperature, degC: 27 unless a
+                    # deck's .options tnom sets it (frontend/analyses.zig:397)
+                    return ("num", 27.0, False)
                 if len(args) > 1:
                     return self.expr(args[1], scope, pre)
                 raise ParseError(f"$simparam({what}) without a default")
@@ -195,7 +204,13 @@ class Model:
             if fn in self.mod.functions:
                 return self.inline(self.mod.functions[fn], args, scope, pre)
             if fn == "$limit":
-                raise ParseError("$limit outside an assignment")
+                # lifted into a limit statement on a fresh variable, read here
+                a, b = self.node_pair(args[0][2], "$limit") if args[0][0] == "call" else (None, None)
+                lf = args[1][1] if len(args) > 1 and args[1][0] == "str" else None
+                extra = [self.expr(x, scope, pre) for x in args[2:]]
+                tmp = self.fresh("limit", "real")
+                pre.append(("limit", tmp, lf, ("probe", a, b), extra))
+                return ("var", tmp)
             raise ParseError(f"unsupported function {fn}")
         raise ParseError(f"unsupported expression {k}")
 

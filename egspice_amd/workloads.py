@@ -205,3 +205,51 @@ def opamp_lane_params(inputs, index):
     out.append((ISOURCE, "IBIAS", 0, OPAMP_NOMINAL["ibias"] * inputs[:, 6]))
     out.append((CAPACITOR, "CL", 0, inputs[:, 7]))
     return out
+
+
+# A 31-stage ring oscillator on BSIM4 (bsim4va): NMOS W/L 1u/0.1u, PMOS
+# 2u/0.1u, VTH0 +-0.3 V, TOXE 3 nm, 1 fF load per stage, VDD 1.2 V; a 20 ps
+# current pulse into n1 starts it from its (metastable) operating point.
+RING_NOMINAL = dict(vdd=1.2, cload=1e-15, vth0=0.3, toxe=3e-9, wn=1e-6, wp=2e-6, l=0.1e-6,
+                    kick=(50e-6, 0.1e-9, 5e-12, 20e-12))
+
+
+def ring_oscillator(add, index, stages=31):
+    """Cards of the ring (`index(name)`: the slot of a named BSIM4 parameter):
+
+        VDD vdd 0 DC 1.2
+        IKICK 0 n1 PULSE(0 50u 0.1n 5p 5p 20p 1)
+        MN<k> n<k+1> n<k> 0 0 nch W=1u L=0.1u VTH0=0.3 TOXE=3n
+        MP<k> n<k+1> n<k> vdd vdd pch W=2u L=0.1u VTH0=-0.3 TOXE=3n
+        C<k> n<k+1> 0 1f                      (n<stages+1> is n1)
+
+    Returns {'VDD', 'IKICK', 'MN1'.., 'MP1'.., 'C1'..} instance indices."""
+    from . import BSIM4, ISOURCE
+    R = RING_NOMINAL
+    inst = {"VDD": add(VSOURCE, ["vdd", "0"], {VSOURCE_DC: R["vdd"]})}
+    i, td, tr, pw = R["kick"]
+    inst["IKICK"] = add(ISOURCE, ["0", "n1"], pulse_params(0.0, i, td, tr, tr, pw, 1.0, dc=0.0))
+    for k in range(1, stages + 1):
+        a, y = f"n{k}", f"n{k % stages + 1}"
+        for name, ty, w, b in (("MN", 1.0, R["wn"], "0"), ("MP", -1.0, R["wp"], "vdd")):
+            p = {index("TYPE"): ty, index("W"): w, index("L"): R["l"], index("VTH0"): ty * R["vth0"],
+                 index("TOXE"): R["toxe"]}
+            inst[f"{name}{k}"] = add(BSIM4, [y, a, b, b], p)
+    for k in range(1, stages + 1):
+        inst[f"C{k}"] = add(CAPACITOR, [f"n{k % stages + 1}", "0"], {0: R["cload"]})
+    return inst
+
+
+def ring_lane_params(n, index, stages=31, seed=1):
+    """Per-lane Monte Carlo columns: every transistor's VTH0 ~ N(+-0.3 V,
+    15 mV) and TOXE ~ N(3 nm, 2%), seeded: [(kind, instance name, slot,
+    values)]."""
+    from . import BSIM4
+    R = RING_NOMINAL
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(1, stages + 1):
+        for name, ty in (("MN", 1.0), ("MP", -1.0)):
+            out.append((BSIM4, f"{name}{k}", index("VTH0"), ty * R["vth0"] + rng.normal(0.0, 0.015, n)))
+            out.append((BSIM4, f"{name}{k}", index("TOXE"), R["toxe"] * (1.0 + rng.normal(0.0, 0.02, n))))
+    return out

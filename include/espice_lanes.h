@@ -36,12 +36,13 @@ enum {
 #endif
 
 /* Device kinds of a template (models/vsource.va, isource.va, resistor.va,
- * capacitor.va, diode.va, mos1.va; psp103.va compiled from its Verilog-A:
- * ports D G S B, the module's parameters in declaration order, named
- * through espice_lanes_param_index). */
+ * capacitor.va, diode.va, mos1.va; psp103.va and bsim4va.va compiled from
+ * their Verilog-A: ports D G S B, the module's parameters in declaration
+ * order, named through espice_lanes_param_index). */
 enum {
     ESPICE_LANES_VSOURCE = 0, ESPICE_LANES_ISOURCE = 1, ESPICE_LANES_RESISTOR = 2,
-    ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5, ESPICE_LANES_PSP103 = 6
+    ESPICE_LANES_CAPACITOR = 3, ESPICE_LANES_DIODE = 4, ESPICE_LANES_MOS1 = 5, ESPICE_LANES_PSP103 = 6,
+    ESPICE_LANES_BSIM4 = 7
 };
 /* User parameters per kind (NaN = not given). VSOURCE: dc, waveform (0 DC,
  * 1 PULSE, 2 SIN, 3 EXP, 4 PWL, 5 SFFM, 6 AM), PULSE v1 v2 td tr tf pw per, SIN vo va freq td

@@ -63,7 +63,8 @@ static unsigned gmask(int kind, int ru)
     case OR_RES: return 0x3u;
     case OR_DIODE: return ru == 0 ? 0x5u : ru == 1 ? 0x6u : 0x7u;
     case OR_MOS1: return ru == 1 ? 0x0u : 0xfu;
-    case OR_PSP103: return or_va_gmask(kind, ru);
+    case OR_PSP103:
+    case OR_BSIM4: return or_va_gmask(kind, ru);
     default: return 0;
     }
 }
@@ -73,7 +74,8 @@ static unsigned cmask(int kind, int ru)
     case OR_CAP: return 0x3u;
     case OR_DIODE: return ru == 0 ? 0x0u : 0x6u;
     case OR_MOS1: return 0xfu;
-    case OR_PSP103: return or_va_cmask(kind, ru);
+    case OR_PSP103:
+    case OR_BSIM4: return or_va_cmask(kind, ru);
     default: (void)ru; return 0;
     }
 }
@@ -81,7 +83,7 @@ static unsigned pmask(int kind, int ru) { return gmask(kind, ru) | cmask(kind, r
 /* Residual rows a kind writes in a static evaluation. */
 static unsigned RMASK[OR_NKIND] = {0x7u, 0x3u, 0x3u, 0x0u, 0x7u, 0xdu, 0u};
 /* Unknowns each kind's limiter writes. */
-static const unsigned LIMW[OR_NKIND] = {0, 0, 0, 0, 0x4u, 0xbu, 0};
+static unsigned LIMW[OR_NKIND] = {0, 0, 0, 0, 0x4u, 0xbu, 0, 0};
 
 __attribute__((constructor)) static void va_tables(void)
 {
@@ -92,6 +94,7 @@ __attribute__((constructor)) static void va_tables(void)
         PK[k] = or_va_nprep(k);
         NSITE[k] = or_va_nsite(k);
         RMASK[k] = (1u << NU[k]) - 1u;
+        LIMW[k] = or_va_limw(k);
     }
 }
 /* scratch big enough for any kind's parameters */
@@ -368,6 +371,7 @@ void or_lanes_prepare(or_lanes *L)
                 for (int k = 0; k < UK[kind]; k++) u[k] = L->uprm[kind][((size_t)i * UK[kind] + k) * N + l];
                 switch (kind) {
                 case OR_PSP103:
+                case OR_BSIM4:
                     if (or_va_setup(kind, u, p) != 0)
                         for (int k = 0; k < PK[kind]; k++) p[k] = NAN;
                     break;
@@ -427,18 +431,54 @@ static double *lim_at(const or_lanes *L, int kind, uint32_t id, uint32_t l)
 
 /* One compiled instance into the planes, in the same order as the other
  * kinds: its rows, then its G mask's entries row-major (and in a transient
- * its charges, C entries and one tape entry per ddt() site). */
+ * its charges, C entries and one tape entry per ddt() site). With limiting
+ * active it is evaluated at the limited image of the unknowns its limiter
+ * writes, and x minus the image, dotted with each row's structural
+ * gradient, is added to the residual and the charge rows (the charge
+ * sites are not corrected: only the unlimited re-read at a solution feeds
+ * the step control). */
 static void eval_va_inst(const or_lanes *L, int kind, uint32_t id, uint32_t l, const uint32_t *nd, const uint32_t *sl,
-                         const double *x, double *g, double *rhs, const or_evctx *ec, uint32_t *tape_at)
+                         const double *x, int lim_active, double *g, double *rhs, const or_evctx *ec,
+                         uint32_t *tape_at)
 {
     const or_circuit *c = L->c;
     const int nu = NU[kind];
-    double p[OR_VA_PARAM_MAX], xl[32], res[32], jac[1024], q[32], cjac[1024], site[256];
+    double p[OR_VA_PARAM_MAX], xl[32], res[32], jac[1024], q[32], cjac[1024], site[256], corr[32];
     or_load_p(L, kind, id, l, p);
-    for (int u = 0; u < nu; u++) xl[u] = x[nd[u]];
+    int live = 0;
+    const unsigned writes = LIMW[kind];
+    for (int u = 0; u < nu; u++) {
+        xl[u] = x[nd[u]];
+        corr[u] = 0.0;
+    }
+    if (writes && lim_active) {
+        const double *lm = lim_at(L, kind, id, l);
+        for (int u = 0; u < nu; u++) {
+            const double xi = xl[u];
+            if (writes >> u & 1) xl[u] = lm[u];
+            corr[u] = xi - xl[u];
+            live |= corr[u] != 0;
+        }
+    }
     const int charged = ec && (ec->q || ec->cc || ec->tape);
     if (charged) or_va_eval_tran(kind, p, xl, res, jac, q, cjac, site);
     else or_va_eval(kind, p, xl, res, jac);
+    if (live) {
+        int kg = 0, kc = 0;
+        for (int ru = 0; ru < nu; ru++) {
+            double acc = 0;
+            for (int cu = 0; cu < nu; cu++)
+                if (gmask(kind, ru) >> cu & 1) acc += jac[kg++] * corr[cu];
+            res[ru] += acc;
+        }
+        if (charged)
+            for (int ru = 0; ru < nu; ru++) {
+                double acc = 0;
+                for (int cu = 0; cu < nu; cu++)
+                    if (cmask(kind, ru) >> cu & 1) acc += cjac[kc++] * corr[cu];
+                q[ru] += acc;
+            }
+    }
     if (ec && ec->tape)
         for (int k = 0; k < NSITE[kind]; k++) ec->tape[(*tape_at)++] = site[k];
     size_t k = 0, kg = 0, kc = 0;
@@ -475,7 +515,7 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
             const uint32_t *sl = c->slots[kind] + (size_t)id * c->kslots[kind];
             if (is_va(kind)) {
-                eval_va_inst(L, kind, id, l, nd, sl, x, g, rhs, ec, &tape_at);
+                eval_va_inst(L, kind, id, l, nd, sl, x, lim_active, g, rhs, ec, &tape_at);
                 continue;
             }
             double p[OR_PARAM_MAX], lx[4], xl[4];
@@ -581,19 +621,22 @@ int or_limit_pass(or_lanes *L, uint32_t l, const double *x, const double *x_old,
     int flag = 0;
     for (int t = 0; t < c->n_order; t++) {
         int kind = c->order[t];
-        if (kind != OR_MOS1 && kind != OR_DIODE) continue;
+        const unsigned writes = LIMW[kind];
+        if (writes == 0) continue;
         const int nu = NU[kind];
-        unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
-            double p[OR_PARAM_MAX], cur[4], old[4], lim[4];
+            double p[OR_VA_PARAM_MAX];  /* per call: lanes run on several threads */
+            double cur[32], old[32], lim[32];
             double *lm = lim_at(L, kind, id, l);
             or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) {
                 cur[u] = x[nd[u]];
                 old[u] = ((writes >> u & 1) && lim_active) ? lm[u] : x_old[nd[u]];
             }
-            int ch = kind == OR_MOS1 ? or_limit_mos1(p, cur, old, lim) : or_limit_diode(p, cur, old, lim);
+            int ch = is_va(kind)         ? or_va_limit(kind, p, cur, old, lim)
+                     : kind == OR_MOS1 ? or_limit_mos1(p, cur, old, lim)
+                                       : or_limit_diode(p, cur, old, lim);
             if (ch) flag = 1;
             for (int u = 0; u < nu; u++)
                 if (writes >> u & 1) lm[u] = lim[u];
@@ -607,16 +650,18 @@ void or_seed_pass(or_lanes *L, uint32_t l, const double *x)
     const or_circuit *c = L->c;
     for (int t = 0; t < c->n_order; t++) {
         int kind = c->order[t];
-        if (kind != OR_MOS1 && kind != OR_DIODE) continue;
+        const unsigned writes = LIMW[kind];
+        if (writes == 0) continue;
         const int nu = NU[kind];
-        unsigned writes = LIMW[kind];
         for (uint32_t id = 0; id < c->count[kind]; id++) {
             const uint32_t *nd = c->nodes[kind].p + (size_t)id * nu;
-            double p[OR_PARAM_MAX], cur[4], lim[4];
+            double p[OR_VA_PARAM_MAX];  /* per call: lanes run on several threads */
+            double cur[32], lim[32];
             double *lm = lim_at(L, kind, id, l);
             or_load_p(L, kind, id, l, p);
             for (int u = 0; u < nu; u++) cur[u] = x[nd[u]];
-            if (kind == OR_MOS1) or_seed_mos1(p, cur, lim);
+            if (is_va(kind)) or_va_seed(kind, p, cur, lim);
+            else if (kind == OR_MOS1) or_seed_mos1(p, cur, lim);
             else or_seed_diode(p, cur, lim);
             for (int u = 0; u < nu; u++)
                 if (writes >> u & 1) lm[u] = lim[u];

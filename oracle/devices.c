@@ -186,7 +186,7 @@ double or_pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck)
 }
 
 /* eval.zig:1116-1125 snapLimited over the unknowns in `reads`/`writes`. */
-static void snap(int nu, unsigned reads, unsigned writes, double *lx, const double *cur)
+void or_snap_limited(int nu, unsigned reads, unsigned writes, double *lx, const double *cur)
 {
     double scale = 0;
     for (int u = 0; u < nu; u++)
@@ -494,7 +494,7 @@ int or_limit_mos1(const double *p, const double *cur, const double *old, double 
     lim[1] = s + type * vgs;
     lim[2] = s;
     lim[3] = s + type * vbs;
-    snap(4, 0xf, 0xb, lim, cur);
+    or_snap_limited(4, 0xf, 0xb, lim, cur);
     return check;
 }
 
@@ -577,7 +577,7 @@ int or_limit_diode(const double *p, const double *cur, const double *old, double
     lim[0] = cur[0];
     lim[1] = cur[1];
     lim[2] = cur[1] + vd;
-    snap(3, 0x6, 0x4, lim, cur);
+    or_snap_limited(3, 0x6, 0x4, lim, cur);
     return check;
 }
 

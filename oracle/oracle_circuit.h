@@ -9,7 +9,7 @@
  * Evaluation order is the order kinds first appear in the netlist
  * (src/device/Circuit.zig: one batch per proto, protos in first-use
  * order), instances in card order within a kind. */
-enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_PSP103 = 6, OR_NKIND = 7 };
+enum { OR_VSRC = 0, OR_ISRC = 1, OR_RES = 2, OR_CAP = 3, OR_DIODE = 4, OR_MOS1 = 5, OR_PSP103 = 6, OR_BSIM4 = 7, OR_NKIND = 8 };
 /* OR_PSP103 is compiled from models/psp103.va (va.c): its parameters are the
  * module's, in declaration order (or_va_param_index), NaN = not given. */
 int or_va_param_index(int kind, const char *name);

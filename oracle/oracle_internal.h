@@ -94,9 +94,13 @@ void or_va_eval(int kind, const double *p, const double *x, double *res, double 
 void or_va_eval_tran(int kind, const double *p, const double *x, double *res, double *jac, double *q, double *cjac,
                      double *site);
 void or_va_eval_charge(int kind, const double *p, const double *x, double *q, double *site);
+unsigned or_va_limw(int kind);
+int or_va_limit(int kind, const double *p, const double *cur, const double *old, double *lim);
+void or_va_seed(int kind, const double *p, const double *cur, double *lim);
 
 double or_fetlim(double vnew, double vold, double vto);
 double or_limvds(double vnew, double vold);
 double or_pnjlim(double vnew, double vold, double vt, double vcrit, int *icheck);
+void or_snap_limited(int nu, unsigned reads, unsigned writes, double *lx, const double *cur);
 
 #endif

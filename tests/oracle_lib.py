@@ -12,9 +12,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-VSRC, ISRC, RES, CAP, DIODE, MOS1, PSP103 = range(7)
+VSRC, ISRC, RES, CAP, DIODE, MOS1, PSP103, BSIM4 = range(8)
 UK = {VSRC: 71, ISRC: 71, RES: 1, CAP: 1, DIODE: 19, MOS1: 36}
-COMPILED = (PSP103,)
+COMPILED = (PSP103, BSIM4)
 
 # user parameter slots (oracle/oracle_circuit.h)
 VSRC_DC = 0
@@ -145,6 +145,11 @@ def load():
         "or_va_eval": (None, [C.c_int, f64p, f64p, f64p, f64p]),
         "or_va_eval_tran": (None, [C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]),
         "or_va_eval_charge": (None, [C.c_int, f64p, f64p, f64p, f64p]),
+        "or_va_limw": (C.c_uint, [C.c_int]),
+        "or_va_limit": (C.c_int, [C.c_int, f64p, f64p, f64p, f64p]),
+        "or_va_seed": (None, [C.c_int, f64p, f64p, f64p]),
+        "or_va_limit_args": (None, [C.c_int, f64p, f64p, f64p]),
+        "or_limit_mos1": (C.c_int, [f64p, f64p, f64p, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -177,7 +182,7 @@ class Circuit:
         self.lib = lib
         self.h = lib.or_ckt_new()
         self.frozen = False
-        self.counts = [0] * 7
+        self.counts = [0] * 8
 
     def __del__(self):
         if getattr(self, "h", None):

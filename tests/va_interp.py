@@ -25,10 +25,13 @@ class Interp:
         self.given = set()
         self.p = {}
         ptype = {p["name"]: p["type"] for p in model.params}
+        # parameter names are case-insensitive, as instance and model cards
+        # read them
+        user = {k.lower(): v for k, v in user.items()}
         for p in model.params:
             nm = p["name"]
-            if nm in user:
-                v = user[nm]
+            if nm.lower() in user:
+                v = user[nm.lower()]
                 self.given.add(nm)
             else:
                 v = self.expr(model.expr(p["default"], [{}], []), None)
@@ -111,6 +114,12 @@ class Interp:
                     q = abs(a) // abs(b)
                     return q if (a >= 0) == (b >= 0) else -q
                 return a / b if b != 0 else (math.copysign(math.inf, a) * math.copysign(1, b) if a else math.nan)
+            if op == "%":
+                # integer remainder, truncated as C's (0 for a zero divisor)
+                if b == 0:
+                    return 0
+                r = abs(a) % abs(b)
+                return r if a >= 0 else -r
             if op == "<":
                 return int(a < b)
             if op == "<=":
@@ -189,12 +198,15 @@ class Interp:
                     if b >= 0:
                         self.res[b] -= val
                 if self.tran:
-                    for sign, q in qs:
+                    for sign, q, _ in qs:
                         val = sign * self.expr(q, self.x)
                         if a >= 0:
                             self.q[a] += val
                         if b >= 0:
                             self.q[b] -= val
+            elif k == "limit":
+                # evaluated at x itself (a limited image is an x too)
+                self.v[s[1]] = self.expr(s[3], self.x)
             elif k == "collapse":
                 pass
             else:
