@@ -25,6 +25,11 @@ enum Kind : int {
     K_VSRC = 0, K_ISRC = 1, K_RES = 2, K_CAP = 3, K_DIODE = 4, K_MOS1 = 5, K_PSP103 = 6, K_BSIM4 = 7, K_NKIND = 8
 };
 constexpr bool kind_is_va(int kind) { return kind >= K_PSP103; }
+// The kernels are instantiated per set of compiled kinds a template holds
+// (a bit per kind), so that a template's kernel carries only its own
+// models' evaluations and register budget.
+enum { VA_PSP = 1, VA_B4 = 2, VA_SETS = 4 };
+constexpr int va_bit(int kind) { return kind == K_PSP103 ? VA_PSP : kind == K_BSIM4 ? VA_B4 : 0; }
 
 // User parameter slots (NaN = not given).
 // waveform: 0 DC, 1 PULSE(V1 V2 TD TR TF PW PER), 2 SIN(VO VA FREQ TD THETA

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 // state in dynamic LDS laid out by `lay` after the template's tables. With
 // DENSE the lane's direct solver replays on the whole wavefront
 // (wave_lu.h, patterns up to 64 unknowns); otherwise thread 0 runs it.
-template <bool DENSE, bool VA>
+template <bool DENSE, int VA>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
 // symbolic tape written to `pred` (PredTape) for every lane's first
 // factor to try; PRED_OK stays 0 when that factor fails or voids a
 // column. One workgroup; setup work, not part of a solve.
-template <bool VA>
+template <int VA>
 __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                           const double *prm_lanes, Newton o, uint32_t *pred,
                                                           double *lu_g)
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
 // t = 0, then steps until t_stop, each accepted row fed to the .meas
 // passes and, up to `to.record_cap` rows, recorded. The lane's final
 // state goes back to x_glob.
-template <bool VA>
+template <int VA>
 __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                        const double *prm_lanes, double *x_glob, uint32_t N,
                                                        Newton op_opts, Newton step_opts, TranOpts to,
@@ -679,6 +679,7 @@ struct espice_lanes {
     bool prepared = false;
     size_t lds_max = 64 * 1024;  // dynamic LDS a workgroup may take
     bool has_va = false;         // the template holds compiled kinds
+    int va_mask = 0;             // which (va_bit), the kernels' instantiation
     // device pattern
     DevBuf<uint32_t> col_ptr, row_idx, q, tri_pos, diag, nodes[K_NKIND], slots[K_NKIND];
     DevBuf<uint8_t> cur_row;
@@ -747,6 +748,52 @@ struct espice_lanes {
         return B;
     }
 };
+
+// The workgroup-per-lane kernels of each set of compiled kinds (va_bit):
+// their launches, and their addresses for hipFuncSetAttribute.
+template <int V>
+static void launch_analyze(espice_lanes *L, const Newton &o)
+{
+    hipLaunchKernelGGL(analyze_wave_kernel<V>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at, L->w_blob.p,
+                       L->lay, L->prm_lanes.p, o, L->pred.p, L->lu_g.p);
+}
+
+template <int V>
+static void launch_newton_dense(espice_lanes *L, const Newton &o, bool want_x)
+{
+    hipLaunchKernelGGL((newton_wave_kernel<true, V>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+                       L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
+                       L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
+                       want_x ? L->d_x : nullptr, L->lu_g.p);
+}
+
+template <int V>
+static void launch_tran(espice_lanes *L, uint32_t N, const WaveLayout &lay, size_t lds, const Newton &op,
+                        const Newton &step, const TranOpts &to, const MeasPass *passes, uint32_t n_passes,
+                        TranOut *outs, MeasOut *meas, double *rec_t, double *rec_x)
+{
+    hipLaunchKernelGGL(tran_wave_kernel<V>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
+                       L->prm_lanes.p, L->x.p, N, op, step, to, passes, n_passes, outs, meas, rec_t, rec_x,
+                       L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
+}
+
+static void (*const LAUNCH_ANALYZE[VA_SETS])(espice_lanes *, const Newton &) = {
+    launch_analyze<0>, launch_analyze<1>, launch_analyze<2>, launch_analyze<3>};
+static void (*const LAUNCH_NEWTON[VA_SETS])(espice_lanes *, const Newton &, bool) = {
+    launch_newton_dense<0>, launch_newton_dense<1>, launch_newton_dense<2>, launch_newton_dense<3>};
+static void (*const TRAN_LAUNCH[VA_SETS])(espice_lanes *, uint32_t, const WaveLayout &, size_t, const Newton &,
+                                          const Newton &, const TranOpts &, const MeasPass *, uint32_t, TranOut *,
+                                          MeasOut *, double *, double *) = {launch_tran<0>, launch_tran<1>,
+                                                                            launch_tran<2>, launch_tran<3>};
+static const void *const ANALYZE_KERNEL[VA_SETS] = {
+    (const void *)analyze_wave_kernel<0>, (const void *)analyze_wave_kernel<1>, (const void *)analyze_wave_kernel<2>,
+    (const void *)analyze_wave_kernel<3>};
+static const void *const NEWTON_KERNEL[VA_SETS] = {
+    (const void *)newton_wave_kernel<true, 0>, (const void *)newton_wave_kernel<true, 1>,
+    (const void *)newton_wave_kernel<true, 2>, (const void *)newton_wave_kernel<true, 3>};
+static const void *const TRAN_KERNEL[VA_SETS] = {
+    (const void *)tran_wave_kernel<0>, (const void *)tran_wave_kernel<1>, (const void *)tran_wave_kernel<2>,
+    (const void *)tran_wave_kernel<3>};
 
 namespace {
 
@@ -962,14 +1009,14 @@ espice_status espice_lanes_create(const espice_lanes_template *tt, uint32_t n_la
         L->use_wave = L->lds_bytes <= L->lds_max;
     }
     for (int k = 0; k < K_NKIND; k++) L->has_va = L->has_va || (kind_is_va(k) && T.count[k] > 0);
+    for (int k = 0; k < K_NKIND; k++) L->va_mask |= T.count[k] > 0 ? va_bit(k) : 0;
     if (L->use_wave && L->lds_bytes > 64 * 1024) {
         // a lane block past the default 64 KiB of dynamic LDS (gfx950 gives
         // a workgroup up to 160 KiB): the kernels this template launches
         // are allowed what it needs
         const int m = (int)L->lds_bytes;
         std::string bad;
-        const void *ks[] = {L->has_va ? (const void *)newton_wave_kernel<true, true> : (const void *)newton_wave_kernel<true, false>,
-                            L->has_va ? (const void *)analyze_wave_kernel<true> : (const void *)analyze_wave_kernel<false>};
+        const void *ks[] = {NEWTON_KERNEL[L->va_mask], ANALYZE_KERNEL[L->va_mask]};
         for (int i = 0; i < 2; i++)
             if (hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, m) != hipSuccess) {
                 (void)hipGetLastError();
@@ -1131,12 +1178,7 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         const PredTape T = pred_tape(L->t->n, L->t->nnz);
         if (L->pred.n < T.words) HIP_TRY(L->pred.alloc(T.words));
         const Newton o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1};
-        if (L->has_va)
-            hipLaunchKernelGGL(analyze_wave_kernel<true>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p, L->lu_g.p);
-        else
-            hipLaunchKernelGGL(analyze_wave_kernel<false>, dim3(1), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, o, L->pred.p, L->lu_g.p);
+        LAUNCH_ANALYZE[L->va_mask](L, o);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(L->stream));
         L->has_pred = true;
@@ -1184,18 +1226,10 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
-        if (L->lay.dense != ESP_NONE && L->has_va)
-            hipLaunchKernelGGL((newton_wave_kernel<true, true>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                               want_x ? L->d_x : nullptr, L->lu_g.p);
-        else if (L->lay.dense != ESP_NONE)
-            hipLaunchKernelGGL((newton_wave_kernel<true, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
-                               L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                               want_x ? L->d_x : nullptr, L->lu_g.p);
+        if (L->lay.dense != ESP_NONE)
+            LAUNCH_NEWTON[L->va_mask](L, o, want_x);
         else
-            hipLaunchKernelGGL((newton_wave_kernel<false, false>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
+            hipLaunchKernelGGL((newton_wave_kernel<false, 0>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                                nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p);
     } else {
@@ -1361,17 +1395,10 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     step.residual_tol = inf_();
     step.init_fix = 0;
     if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(L->has_va ? (const void *)tran_wave_kernel<true> : (const void *)tran_wave_kernel<false>,
+        HIP_TRY(hipFuncSetAttribute(TRAN_KERNEL[L->va_mask],
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
-    if (L->has_va)
-        hipLaunchKernelGGL(tran_wave_kernel<true>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
-                           L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
-                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
-    else
-        hipLaunchKernelGGL(tran_wave_kernel<false>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
-                           L->prm_lanes.p, L->x.p, N, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p,
-                           L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
+    TRAN_LAUNCH[L->va_mask](L, N, lay, lds, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));

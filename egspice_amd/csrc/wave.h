@@ -447,17 +447,19 @@ ESP_HD void wave_charge_va(const WaveTemplate &W, WaveLane &s, const InstEntry &
 // them keep their own register budget and occupancy. QONLY (transient):
 // only the charge rows and charge sites are read afterwards, so compiled
 // kinds stage just those (the other kinds evaluate as usual).
-template <bool TRAN = false, bool VA = false, bool QONLY = false>
-ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t = 0.0)
+template <bool TRAN, int VA, bool QONLY>
+ESP_HD void wave_eval_inst_body(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t)
 {
     static_assert(TRAN || !QONLY, "charges are staged in a transient");
     const InstEntry ie = W.inst[e];
-    if constexpr (VA) {
+    if constexpr ((VA & VA_PSP) != 0) {
         if (ie.kind == K_PSP103) {
             if constexpr (QONLY) wave_charge_va<K_PSP103>(W, s, ie, e);
             else wave_eval_va<K_PSP103, TRAN>(W, s, ie, e, lim_active);
             return;
         }
+    }
+    if constexpr ((VA & VA_B4) != 0) {
         if (ie.kind == K_BSIM4) {
             if constexpr (QONLY) wave_charge_va<K_BSIM4>(W, s, ie, e);
             else wave_eval_va<K_BSIM4, TRAN>(W, s, ie, e, lim_active);
@@ -473,6 +475,23 @@ ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool 
     case K_MOS1: wave_eval_k<K_MOS1, TRAN>(W, s, ie, e, lim_active, t); break;
     default: break;
     }
+}
+
+// With compiled kinds the per-instance evaluation is one outlined call: the
+// kernel around it (the solve, the step control) keeps its registers, and
+// the call's register budget is the evaluations' alone.
+template <bool TRAN, int VA, bool QONLY>
+__host__ __device__ __attribute__((noinline)) void wave_eval_inst_call(const WaveTemplate &W, WaveLane &s, uint32_t e,
+                                                                       bool lim_active, double t)
+{
+    wave_eval_inst_body<TRAN, VA, QONLY>(W, s, e, lim_active, t);
+}
+
+template <bool TRAN = false, int VA = 0, bool QONLY = false>
+ESP_HD void wave_eval_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active, double t = 0.0)
+{
+    if constexpr (VA != 0) wave_eval_inst_call<TRAN, VA, QONLY>(W, s, e, lim_active, t);
+    else wave_eval_inst_body<TRAN, VA, QONLY>(W, s, e, lim_active, t);
 }
 
 // Phase (transient): charge row r (e < n) or C slot e - n over its staging
@@ -598,9 +617,9 @@ ESP_HD bool wave_limit_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie
     return ch != 0;
 }
 
-// VA: compiled kinds may be present (their limiters are instantiated only
+// VA: the compiled kinds present (their limiters are instantiated only
 // then, as their evaluations are).
-template <bool VA = false>
+template <int VA = 0>
 ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool lim_active)
 {
     const InstEntry ie = W.inst[e];
@@ -608,7 +627,7 @@ ESP_HD bool wave_limit_inst(const WaveTemplate &W, WaveLane &s, uint32_t e, bool
     case K_DIODE: return wave_limit_k<K_DIODE>(W, s, ie, e, lim_active);
     case K_MOS1: return wave_limit_k<K_MOS1>(W, s, ie, e, lim_active);
     case K_BSIM4:
-        if constexpr (VA) return wave_limit_k<K_BSIM4>(W, s, ie, e, lim_active);
+        if constexpr ((VA & VA_B4) != 0) return wave_limit_k<K_BSIM4>(W, s, ie, e, lim_active);
         return false;
     default: return false;
     }
@@ -635,7 +654,7 @@ ESP_HD void wave_seed_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
         if (limw >> u & 1) s.lim[ie.lim + u] = lim[u];
 }
 
-template <bool VA = false>
+template <int VA = 0>
 ESP_HD void wave_seed_inst(const WaveTemplate &W, WaveLane &s, uint32_t e)
 {
     const InstEntry ie = W.inst[e];
@@ -643,7 +662,7 @@ ESP_HD void wave_seed_inst(const WaveTemplate &W, WaveLane &s, uint32_t e)
     case K_DIODE: wave_seed_k<K_DIODE>(W, s, ie, e); break;
     case K_MOS1: wave_seed_k<K_MOS1>(W, s, ie, e); break;
     case K_BSIM4:
-        if constexpr (VA) wave_seed_k<K_BSIM4>(W, s, ie, e);
+        if constexpr ((VA & VA_B4) != 0) wave_seed_k<K_BSIM4>(W, s, ie, e);
         break;
     default: break;
     }

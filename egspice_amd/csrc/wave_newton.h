@@ -25,7 +25,7 @@ enum { NEWTON_SINGULAR = -1, NEWTON_FAILED = 0, NEWTON_CONVERGED = 1 };
 // operating point and its steps); HOOK: the integrator's companion applies
 // (h). Both are compile-time so the static solve carries none of it. All
 // threads; uniform result.
-template <bool TRAN, bool HOOK, bool VA = false>
+template <bool TRAN, bool HOOK, int VA = 0>
 __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D, const ReduceCache &rc, const Newton &o,
                            const WaveHook &h, bool init_fix, bool lim_active, double t, uint32_t &iters)
 {
