@@ -419,17 +419,152 @@ def opamp_tran_flops(t, res, n_passes):
     lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
     sites = 8 * PSP103_SITES + 2
     per_tick = 8 * PSP103_TRAN_EVAL_FLOPS + 2 * EVAL_FLOPS[0] + EVAL_FLOPS[1] + 5 * t.n + 2 * t.nnz + lu + 12 * t.n
-    per_step = 8 * PSP103_TRAN_EVAL_FLOPS + 30 * sites + 7 * t.n + 10 * n_passes
+    per_step = 8 * PSP103_CHARGE_FLOPS + 30 * sites + 7 * t.n + 10 * n_passes
     return float(res["nr_iters"].sum()) * per_tick + float(res["steps"].sum()) * per_step
 
 
 # FP64 operations of one PSP103 evaluation (value and derivative lanes, the
-# stamps) along the paths the op-amp's transistors take at its nominal
-# operating point, mean of the eight: `python -m egspice_amd.vagen.flops
-# egspice_amd/csrc/gen/psp103.h psp103` (FMA 2, exp and log 30, pow 62)
-PSP103_EVAL_FLOPS = 5825
-PSP103_TRAN_EVAL_FLOPS = 6611
+# stamps; the charges alone: values only) along the paths the op-amp's
+# transistors take at its nominal operating point, mean of the eight:
+# `python -m egspice_amd.vagen.flops egspice_amd/csrc/gen/psp103.h psp103`
+# (FMA 2, exp and log 30, pow 62)
+PSP103_EVAL_FLOPS = 5724
+PSP103_TRAN_EVAL_FLOPS = 6529
+PSP103_CHARGE_FLOPS = 1616
 PSP103_SITES = 11
+# the same for BSIM4 at an inverter's transistors over its input swing:
+# `python -m egspice_amd.vagen.flops egspice_amd/csrc/gen/bsim4va.h bsim4va`
+BSIM4_EVAL_FLOPS = 2315
+BSIM4_TRAN_EVAL_FLOPS = 3633
+BSIM4_CHARGE_FLOPS = 1630
+BSIM4_SITES = 5
+
+
+def ring_tran_flops(t, res, n_passes, stages=31):
+    """Algorithmic FP64 work of one ring transient launch: per Newton tick
+    the 62 BSIM4 evaluations with charges and the limiter's passes, the
+    loads, the source, the companion rows, a = g + ag0 c, refactor + solve,
+    update and gates; per step the charges re-read at the solution, the LTE
+    bound, the predictor and the passes."""
+    vals = np.random.default_rng(0).uniform(0.1, 1.0, t.nnz)
+    vals[t.diag] = 2.0 * t.n
+    lu, _ = lu_flops(t.col_ptr, t.row_idx, vals, t.n)
+    sites = 2 * stages * BSIM4_SITES + stages
+    per_tick = (2 * stages * (BSIM4_TRAN_EVAL_FLOPS + 60) + stages * 8 + EVAL_FLOPS[0] + 5 * t.n + 2 * t.nnz + lu
+                + 12 * t.n)
+    per_step = 2 * stages * BSIM4_CHARGE_FLOPS + 30 * sites + 7 * t.n + 10 * n_passes
+    return float(res["nr_iters"].sum()) * per_tick + float(res["steps"].sum()) * per_step
+
+
+def bench_ring(args):
+    """Config 2: the 31-stage BSIM4 ring oscillator's Monte Carlo lanes
+    (every transistor's VTH0 ~ N(+-0.3 V, 15 mV), TOXE ~ N(3 nm, 2%)), each
+    a cold operating point (MODEINITJCT seeds, limited Newton) and a 5 ns
+    trapezoidal transient (`.tran 10p 5n`, a 20 ps kick into n1) with the
+    period's two crossings as .meas passes (v(n1) rising through VDD/2,
+    the first two after 0.2 ns; frequency = 1/period on the host). A lane counts when its transient reached t_stop and both
+    crossings resolved. One step is one launch of every lane's analysis."""
+    import egspice_amd as E
+    from egspice_amd import workloads as W
+    N = args.lanes
+    t = E.Template()
+    idx = lambda nm: E.param_index(E.BSIM4, nm)
+    inst = W.ring_oscillator(t.add, idx)
+    t.freeze()
+    lanes = E.Lanes(t, N)
+    for kind, name, slot, vals in W.ring_lane_params(N, idx, seed=1):
+        lanes.set_param(kind, inst[name], slot, vals)
+    lanes.prepare()
+    n1 = t.node("n1")
+    m = W.RING_MEAS
+    passes = [E.meas_cross(n1, m["val"], rise=1, td=m["td"]), E.meas_cross(n1, m["val"], rise=2, td=m["td"])]
+    topts = E.tran_opts(5e-9, 10e-12, max_steps=200_000)
+
+    def step():
+        res, meas, _, _, ms = lanes.tran(topts, passes)
+        ok = (res["status"] == 0) & np.all(np.isfinite(meas[:, :, 0]), axis=1)
+        return res, meas, ms, int(ok.sum())
+
+    for _ in range(min(args.warmup, 1)):
+        step()
+    kms, conv = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, meas, ms, c = step()
+        kms.append(ms)
+        conv.append(c)
+    wall = time.perf_counter() - t0
+    value = sum(conv) / wall
+    ok = (res["status"] == 0) & np.all(np.isfinite(meas[:, :, 0]), axis=1)
+    period = meas[ok, 1, 0] - meas[ok, 0, 0]
+    line = {
+        "metric": "converged Monte Carlo lanes/sec (ring oscillator transient)", "value": value, "unit": "lanes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "31-stage BSIM4 (bsim4va) ring oscillator Monte Carlo, 1 fF per stage: cold operating "
+                               "point + .tran 10p 5n trapezoidal, period by 2 crossing passes",
+                   "lanes": N, "unknowns": t.n, "pattern_nnz": t.nnz,
+                   "inputs": "per transistor VTH0 N(+-0.3 V, 15 mV), TOXE N(3 nm, 2%), seeded",
+                   "completed_lanes": int((res["status"] == 0).sum()), "lanes_with_period": conv[-1],
+                   "period_ns_mean": float(period.mean() * 1e9) if period.size else None,
+                   "period_ns_std": float(period.std() * 1e9) if period.size else None,
+                   "frequency_ghz_mean": float(np.mean(1.0 / period) * 1e-9) if period.size else None,
+                   "newton_ticks_per_lane": float(res["nr_iters"].mean()), "steps_per_lane": float(res["steps"].mean()),
+                   "tran_kernel_ms": float(np.mean(kms))},
+    }
+    flops = ring_tran_flops(t, res, len(passes))
+    achieved = flops / (float(np.mean(kms)) * 1e-3) / 1e12
+    line["roofline"] = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(f"tran_wave_kernel_ring_{N}"),
+                        "kernel": "tran_wave_kernel", "flops_per_launch": flops}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_ring(args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_ring(budget_s=10.0, n_lanes=256):
+    """The oracle's CPU path on the same ring lanes (cold operating point,
+    `.tran 10p 5n`, both crossing passes), dealt over the host's cores for
+    about budget_s seconds."""
+    import concurrent.futures as cf
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from egspice_amd import workloads as W
+    lib = O.load()
+    c = O.Circuit(lib)
+    idx = lambda nm: lib.or_va_param_index(O.BSIM4, nm.encode())
+    oinst = W.ring_oscillator(c.add, idx)
+    c.freeze()
+    L = O.Lanes(c, n_lanes)
+    for kind, name, slot, vals in W.ring_lane_params(n_lanes, idx, seed=1):
+        L.set_param(kind, oinst[name], slot, vals)
+    L.prepare()
+    opts = O.tran_opts(5e-9, 10e-12, max_steps=200_000)
+    n1 = c.node("n1")
+    m = W.RING_MEAS
+    clauses = [O.clause(val=m["val"], rise=1, td=m["td"]), O.clause(val=m["val"], rise=2, td=m["td"])]
+
+    def lane(l):
+        tt, xx, st = L.tran(l, opts)
+        if st.status != 0:
+            return 0
+        y = np.ascontiguousarray(xx[:, n1])
+        got = [lib.or_meas_cross(O.f64(tt)[1], O.f64(y)[1], len(tt), C.byref(cl)) for cl in clauses]
+        return 1 if all(np.isfinite(g) for g in got) else 0
+
+    threads = host_threads()
+    done, t0, l = 0, time.perf_counter(), 0
+    with cf.ThreadPoolExecutor(threads) as ex:
+        while True:
+            done += sum(ex.map(lane, [(l + k) % n_lanes for k in range(threads * 2)]))
+            l += threads * 2
+            el = time.perf_counter() - t0
+            if el >= budget_s:
+                break
+    return {"value": done / el, "unit": "lanes/s", "cores": threads, "kind": "port",
+            "sample": f"{l} Monte Carlo ring lanes (cold operating point, .tran 10p 5n, 2 crossing passes) in "
+                      f"{el:.1f} s, oracle CPU path on {threads} threads, {platform.processor() or platform.machine()}"}
 
 
 def cpu_baseline_opamp(t, inst, nominal, budget_s=10.0, n_lanes=512):
@@ -487,14 +622,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--lanes", type=int, default=None,
-                    help="lanes (default: 65536 for opamp-mc, 64 for dc-inverter-chain, 4096 for tran-rc-measure)")
+                    help="lanes (default: 65536 for opamp-mc, 4096 for ring-bsim4 and tran-rc-measure, 64 for "
+                         "dc-inverter-chain)")
     ap.add_argument("--train-lanes", type=int, default=65536)
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("opamp-mc", "dc-inverter-chain", "tran-rc-measure"), default="opamp-mc",
-                    help="opamp-mc: the psp103 op-amp's Monte Carlo operating points, learned vs nominal warm "
-                         "start (default); dc-inverter-chain: the mos1 chain's 64-point DC sweep; tran-rc-measure: "
-                         "the RC transient of docs/examples/measure.sp with its .meas passes")
+    ap.add_argument("--workload", choices=("opamp-mc", "ring-bsim4", "dc-inverter-chain", "tran-rc-measure"),
+                    default="opamp-mc",
+                    help="opamp-mc: the psp103 op-amp's Monte Carlo lanes, learned vs nominal warm start "
+                         "(default); ring-bsim4: the 31-stage BSIM4 ring oscillator's Monte Carlo transient; "
+                         "dc-inverter-chain: the mos1 chain's 64-point DC sweep; tran-rc-measure: the RC transient "
+                         "of docs/examples/measure.sp with its .meas passes")
     ap.add_argument("--arm", choices=("both", "learned", "plain"), default="both",
                     help="dc-inverter-chain: time both arms (default), or only one (for profiling one arm's "
                          "kernels); with one arm, the other's fields are null")
@@ -511,7 +649,11 @@ def main():
         sys.exit(2)
 
     if args.lanes is None:
-        args.lanes = {"opamp-mc": 65536, "dc-inverter-chain": 64, "tran-rc-measure": 4096}[args.workload]
+        args.lanes = {"opamp-mc": 65536, "ring-bsim4": 4096, "dc-inverter-chain": 64,
+                      "tran-rc-measure": 4096}[args.workload]
+    if args.workload == "ring-bsim4":
+        bench_ring(args)
+        return
     if args.workload == "tran-rc-measure":
         bench_tran(args)
         return

@@ -209,16 +209,20 @@ def opamp_lane_params(inputs, index):
 
 # A 31-stage ring oscillator on BSIM4 (bsim4va): NMOS W/L 1u/0.1u, PMOS
 # 2u/0.1u, VTH0 +-0.3 V, TOXE 3 nm, 1 fF load per stage, VDD 1.2 V; a 20 ps
-# current pulse into n1 starts it from its (metastable) operating point.
+# 200 uA current pulse into n1 at 50 ps starts it from its (metastable)
+# operating point, so that every lane oscillates from the first period.
 RING_NOMINAL = dict(vdd=1.2, cload=1e-15, vth0=0.3, toxe=3e-9, wn=1e-6, wp=2e-6, l=0.1e-6,
-                    kick=(50e-6, 0.1e-9, 5e-12, 20e-12))
+                    kick=(200e-6, 0.05e-9, 5e-12, 20e-12))
+# the period's two crossings: v(n1) rising through VDD/2, the first and the
+# second after 0.2 ns (past the kick)
+RING_MEAS = dict(val=0.6, td=0.2e-9)
 
 
 def ring_oscillator(add, index, stages=31):
     """Cards of the ring (`index(name)`: the slot of a named BSIM4 parameter):
 
         VDD vdd 0 DC 1.2
-        IKICK 0 n1 PULSE(0 50u 0.1n 5p 5p 20p 1)
+        IKICK 0 n1 PULSE(0 200u 50p 5p 5p 20p 1)
         MN<k> n<k+1> n<k> 0 0 nch W=1u L=0.1u VTH0=0.3 TOXE=3n
         MP<k> n<k+1> n<k> vdd vdd pch W=2u L=0.1u VTH0=-0.3 TOXE=3n
         C<k> n<k+1> 0 1f                      (n<stages+1> is n1)
