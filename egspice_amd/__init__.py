@@ -75,7 +75,7 @@ class TranResult(C.Structure):
                                           "bp_landings", "nr_iters")]
 
 
-MEAS_CROSS, MEAS_AT, MEAS_EXTREMUM, MEAS_INTEG = range(4)
+MEAS_CROSS, MEAS_AT, MEAS_EXTREMUM, MEAS_INTEG, MEAS_FIND = range(5)
 LAST = -2
 
 
@@ -199,6 +199,12 @@ def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):
 def meas_at(row, at, deriv=False):
     """FIND / DERIV ... AT= (src/output/scan.zig At)."""
     return MeasPass(MEAS_AT, row, -1, -1, -1, 1 if deriv else 0, 0.0, 0.0, 0.0, 0.0, at)
+
+
+def meas_find(row, deriv=False):
+    """FIND / DERIV ... WHEN on unknown `row`: must follow its WHEN's
+    meas_cross pass (src/output/scan.zig Find)."""
+    return MeasPass(MEAS_FIND, row, -1, -1, -1, 1 if deriv else 0, 0.0, 0.0, 0.0, 0.0, 0.0)
 
 
 def meas_extremum(row, op, frm=0.0, to=0.0):

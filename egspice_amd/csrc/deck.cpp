@@ -480,7 +480,7 @@ bool side_conditions(Cursor &c, Meas::Side &s, std::string &err, const char *sto
     while (c.more() && !(stop && c.peek() == stop)) {
         const std::string key = c.peek();
         if (key == "when") {
-            err = "UnsupportedMeasure FIND WHEN";
+            err = "BadMeasure when";
             return false;
         }
         if (!(c.i + 2 < c.t.size() && c.t[c.i + 1] == "=")) {
@@ -552,7 +552,28 @@ bool read_meas(const std::vector<std::string> &t, Meas &m, std::string &err)
     }
     if (f == "find" || f == "deriv" || f == "derivative") {
         m.kind = f == "find" ? "find" : "deriv";
-        if (!vector_at(c, m.a.vec, err) || !side_conditions(c, m.a, err, nullptr)) return false;
+        if (!vector_at(c, m.a.vec, err) || !side_conditions(c, m.a, err, "when")) return false;
+        if (c.next_is("when")) {
+            // FIND / DERIV sig WHEN sig2=val [RISE= FALL= CROSS= ...]: the
+            // WHEN's Cross, then the value of sig at its event
+            c.i++;
+            m.kind += "_when";
+            if (!vector_at(c, m.b.vec, err)) return false;
+            if (!(c.next_is("=") && c.i + 1 < t.size())) {
+                err = "BadWhen";
+                return false;
+            }
+            if (!to_value(t[c.i + 1], m.b.val, err)) return false;
+            c.i += 2;
+            if (!side_conditions(c, m.b, err, nullptr)) return false;
+            if (m.analysis != "tran" || m.b.td.given || m.b.from.given || m.b.rise == -2 || m.b.fall == -2 ||
+                m.b.cross == -2) {
+                // measured as it goes only without TD=, FROM= or a LAST count
+                err = "UnsupportedMeasure " + f + " WHEN";
+                return false;
+            }
+            return true;
+        }
         if (m.analysis == "tran" && !m.a.at.given) {
             err = "UnsupportedMeasure " + f;
             return false;

@@ -422,7 +422,10 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     __syncthreads();
     if (tid == 0) {
         if (cap > 0) rec_t[(size_t)l * cap] = 0.0;
-        for (uint32_t k = 0; k < n_pass; k++) meas_feed(passes[k], ms[k], 0.0, cur[passes[k].row]);
+        for (uint32_t k = 0; k < n_pass; k++) {
+            if (passes[k].kind == 4) meas_feed_find(passes[k], ms[k], ms[k - 1], 0.0, cur[passes[k].row]);
+            else meas_feed(passes[k], ms[k], 0.0, cur[passes[k].row]);
+        }
         st.delmin = 1e-11 * to.dt_max;
         st.min_break = 10.0 * st.delmin;
         st.bp_lo = inf_();
@@ -602,7 +605,10 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
             }
             if (st.out.points < cap) rec_t[(size_t)l * cap + st.out.points] = t_new;
             st.out.points++;
-            for (uint32_t k = 0; k < n_pass; k++) meas_feed(passes[k], ms[k], t_new, cur[passes[k].row]);
+            for (uint32_t k = 0; k < n_pass; k++) {
+                if (passes[k].kind == 4) meas_feed_find(passes[k], ms[k], ms[k - 1], t_new, cur[passes[k].row]);
+                else meas_feed(passes[k], ms[k], t_new, cur[passes[k].row]);
+            }
             st.bp_next = tran_next_bp(W, prm_lds, bpc, st, t_new, t_new + st.min_break);
             st.bp_clamped = false;
             if (st.bp_next != inf_()) {
@@ -1358,7 +1364,16 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     if (!(opts->t_stop > 0) || !(opts->dt_init > 0) || !(opts->dt_max > 0) || opts->method > 2)
         return fail(ESPICE_INVALID_ARGUMENT, "InvalidTran");
     for (uint32_t k = 0; k < n_passes; k++)
-        if (passes[k].kind > 3 || passes[k].row >= n) return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
+        if (passes[k].kind > 4 || passes[k].row >= n) return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
+    // a Find reads the Cross before it as its WHEN; the device measures one
+    // as it goes only when that WHEN has no TD=, FROM= or LAST count (the
+    // cards the reference measures as it goes, docs/using/cli.md:186-197)
+    for (uint32_t k = 0; k < n_passes; k++)
+        if (passes[k].kind == 4) {
+            const espice_meas_pass &w = passes[k - (k > 0 ? 1 : 0)];
+            if (k == 0 || w.kind != 0 || w.td != 0 || w.from != 0 || w.rise == -2 || w.fall == -2 || w.cross == -2)
+                return fail(ESPICE_INVALID_ARGUMENT, "InvalidMeasure");
+        }
     // The Meyer and junction charges of mos1.va are not staged yet: a lane
     // whose MOSFETs would carry charge is refused
     // rather than run as static; with every such parameter zero (or not

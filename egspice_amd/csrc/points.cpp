@@ -212,6 +212,10 @@ bool plan_passes(Problem &P, std::string &err)
             if ((c.first = side_pass(m.a, 0, 0)) < 0) return false;
         } else if (m.kind == "find" || m.kind == "deriv") {
             if ((c.first = side_pass(m.a, 1, m.kind == "deriv" ? 1 : 0)) < 0) return false;
+        } else if (m.kind == "find_when" || m.kind == "deriv_when") {
+            // the WHEN's Cross, then the Find reading it; the card's value is the Find's
+            if (side_pass(m.b, 0, 0) < 0 || (c.first = side_pass(m.a, 4, m.kind == "deriv_when" ? 1 : 0)) < 0)
+                return false;
         } else if (m.kind == "min" || m.kind == "max" || m.kind == "avg") {
             if ((c.first = side_pass(m.a, 2, m.kind == "min" ? 0 : m.kind == "max" ? 1 : 2)) < 0) return false;
         } else if (m.kind == "rms" || m.kind == "integ") {

@@ -27,7 +27,8 @@ struct TranOpts {
 
 // One .meas pass (scan.zig): 0 Cross (WHEN/TRIG/TARG), 1 At (FIND, or
 // DERIV with op 1), 2 Extremum (op 0 MIN, 1 MAX, 2 AVG), 3 Integ (op 0
-// INTEG, 1 RMS). `row` is the unknown it reads.
+// INTEG, 1 RMS), 4 Find (FIND, or DERIV with op 1, ... WHEN: the pass before
+// it is its WHEN's Cross). `row` is the unknown it reads.
 struct MeasPass {
     uint32_t kind, row;
     int32_t rise, fall, cross;
@@ -179,7 +180,7 @@ ESP_HD void meas_init(const MeasPass &m, MeasState &s)
     if (m.kind == 0) {
         s.d[2] = __builtin_nan("");  // last
         s.d[3] = __builtin_nan("");  // the event
-    } else if (m.kind == 1) {
+    } else if (m.kind == 1 || m.kind == 4) {
         s.d[2] = __builtin_nan("");  // value
     } else if (m.kind == 3) {
         s.d[4] = __builtin_nan("");  // first_x
@@ -359,12 +360,65 @@ ESP_HD void meas_feed(const MeasPass &m, MeasState &s, double xv, double v)
     }
 }
 
+// Find (scan.zig Find, :339-380) after its WHEN's Cross has read the same
+// sample: until the WHEN ends it keeps the FIND vector's last three rows;
+// at the WHEN's event it becomes an At there, fed those rows first, then
+// the rows after. d0 px, d1 pv, d2 value, d4 the event, d8..d13 the rows
+// (x, v) oldest first; i0 the At's rows, i1 rows kept, i2 state (0
+// waiting, 1 armed, 2 no event), i3 rows fed.
+ESP_HD bool find_at_feed(const MeasPass &m, MeasState &s, double xv, double v)
+{
+    if (s.i[0] > 0 && s.d[0] <= s.d[4] && xv >= s.d[4]) {
+        s.d[2] = m.op ? (v - s.d[1]) / (xv - s.d[0]) : s.d[1] + (s.d[4] - s.d[0]) * (v - s.d[1]) / (xv - s.d[0]);
+        return true;
+    }
+    s.i[0] += 1;
+    s.d[0] = xv;
+    s.d[1] = v;
+    return false;
+}
+
+ESP_HD void meas_feed_find(const MeasPass &m, MeasState &s, const MeasState &when, double xv, double v)
+{
+    if (s.i[7]) return;
+    if (s.i[2] == 1) {
+        if (find_at_feed(m, s, xv, v) || xv > s.d[4]) s.i[7] = 1;
+        return;
+    }
+    if (s.i[1] == 3)
+        for (int k = 0; k < 4; k++) s.d[8 + k] = s.d[10 + k];
+    s.i[1] = s.i[1] < 3 ? s.i[1] + 1 : 3;
+    s.d[8 + 2 * (s.i[1] - 1)] = xv;
+    s.d[9 + 2 * (s.i[1] - 1)] = v;
+    s.i[3] += 1;
+    if (!when.i[7]) return;
+    const double time = when.i[6] ? when.d[3] : when.d[2];
+    if (time != time) {
+        s.i[2] = 2;
+        s.i[7] = 1;
+        return;
+    }
+    s.i[2] = 1;
+    s.d[4] = time;
+    if (s.i[3] > s.i[1] && time <= s.d[8]) {
+        s.i[7] = 1;
+        return;
+    }
+    for (int k = 0; k < s.i[1]; k++) {
+        const double rx = s.d[8 + 2 * k], rv = s.d[9 + 2 * k];
+        if (find_at_feed(m, s, rx, rv) || rx > time) {
+            s.i[7] = 1;
+            return;
+        }
+    }
+}
+
 ESP_HD MeasOut meas_result(const MeasPass &m, MeasState &s)
 {
     MeasOut r{__builtin_nan(""), __builtin_nan(""), __builtin_nan("")};
     if (m.kind == 0) {
         r.value = s.i[6] ? s.d[3] : s.d[2];
-    } else if (m.kind == 1) {
+    } else if (m.kind == 1 || m.kind == 4) {
         r.value = s.d[2];
     } else if (m.kind == 2) {
         if (m.op == 2) {

@@ -200,7 +200,10 @@ typedef struct {
  * current), scan.zig's kinds: 0 Cross (WHEN/TRIG/TARG: val, td, from, to,
  * rise/fall/cross counts, -1 not given, -2 LAST), 1 At (FIND, op 1 DERIV:
  * at), 2 Extremum (op 0 MIN, 1 MAX, 2 AVG: from, to), 3 Integ (op 0 INTEG,
- * 1 RMS: from, to). */
+ * 1 RMS: from, to), 4 Find (FIND, op 1 DERIV, ... WHEN; src/output/
+ * scan.zig:339-380): the value of its row where the pass just before it, a
+ * Cross without td, from or a LAST count (its WHEN), finds its event;
+ * INVALID_ARGUMENT "InvalidMeasure" otherwise. */
 typedef struct {
     uint32_t kind, row;
     int32_t rise, fall, cross;
@@ -208,7 +211,7 @@ typedef struct {
     double val, td, from, to, at;
 } espice_meas_pass;
 
-/* Cross: value = the event time; At: value; Extremum: value, aux0 = where
+/* Cross: value = the event time; At, Find: value; Extremum: value, aux0 = where
  * (AVG: the last time read); Integ: value, aux0 = from, aux1 = to. NaN when
  * the pass found nothing. */
 typedef struct {

@@ -247,3 +247,15 @@ void or_meas_integ(const double *x, const double *y, uint32_t len, double from, 
     *rfrom = f;
     *rto = t;
 }
+
+/* FIND (slope 0) or DERIV (1) of y2 WHEN y crosses as `cl` says: the value
+ * at the WHEN's event (measure.zig's Wave.valueAt / slopeAt there). The
+ * device's Find (scan.zig:339-380) measures this as it goes, with the same
+ * bits, for a WHEN without TD=, FROM= or a LAST count. */
+double or_meas_find_when(const double *x, const double *y, const double *y2, uint32_t len, const or_meas_clause *cl,
+                         int slope)
+{
+    const double at = or_meas_cross(x, y, len, cl);
+    if (isnan(at)) return NAN;
+    return or_meas_at(x, y2, len, at, slope);
+}

@@ -118,6 +118,8 @@ typedef struct {
 double or_meas_cross(const double *x, const double *y, uint32_t len, const or_meas_clause *c);
 /* y at `at` (slope 0) or the slope of the segment holding it (slope 1). */
 double or_meas_at(const double *x, const double *y, uint32_t len, double at, int slope);
+double or_meas_find_when(const double *x, const double *y, const double *y2, uint32_t len, const or_meas_clause *cl,
+                         int slope);
 /* op 0 MIN, 1 MAX (value, where), 2 AVG (value, last scale read). */
 void or_meas_extremum(const double *x, const double *y, uint32_t len, double from, double to, int op, double *value,
                       double *at);

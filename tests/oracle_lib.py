@@ -127,6 +127,7 @@ def load():
                                        C.POINTER(TranStats)]),
         "or_meas_cross": (C.c_double, [f64p, f64p, C.c_uint32, C.POINTER(MeasClause)]),
         "or_meas_at": (C.c_double, [f64p, f64p, C.c_uint32, C.c_double, C.c_int]),
+        "or_meas_find_when": (C.c_double, [f64p, f64p, f64p, C.c_uint32, C.POINTER(MeasClause), C.c_int]),
         "or_meas_extremum": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p]),
         "or_meas_integ": (None, [f64p, f64p, C.c_uint32, C.c_double, C.c_double, C.c_int, f64p, f64p, f64p]),
         "or_limvds": (C.c_double, [C.c_double, C.c_double]),

@@ -73,7 +73,8 @@ def test_isource_waveforms_read_and_build(engine):
     (".options reltol=1e-4", "UnsupportedCard"),
     (".ic v(out)=1", "UnsupportedCard"),
     (".include other.sp", "UnsupportedCard"),
-    (".meas tran bad FIND v(out) WHEN v(in)=0.5", "UnsupportedMeasure"),
+    (".meas tran bad FIND v(out) WHEN v(in)=0.5 TD=1u", "UnsupportedMeasure"),
+    (".meas tran bad FIND v(out) WHEN v(in)=0.5 RISE=LAST", "UnsupportedMeasure"),
     (".meas ac gain MAX v(out)", "UnsupportedMeasureAnalysis"),
     (".model q1 NPN(BF=100)", "UnsupportedModel"),
     (".model m3 NMOS(LEVEL=3)", "UnsupportedModelLevel"),

@@ -95,6 +95,33 @@ def test_measure_sp_nominal_point(oracle, engine):
     assert "Transient Analysis: 1029 points, 4 variables" in pb.report()
 
 
+def test_find_when_cards(oracle, engine):
+    """`.meas tran ... FIND/DERIV ... WHEN` cards through the deck reader and
+    the points API: the WHEN's Cross and the Find as device passes, bitwise
+    the oracle's value at the WHEN's event on the same waveform."""
+    say("case: measure.sp with FIND ... WHEN and DERIV ... WHEN cards, point 0")
+    text = decks.MEASURE_SP.replace(".end", ".meas tran vin_at FIND v(in) WHEN v(out)=0.5 FALL=1\n"
+                                           ".meas tran sl_at DERIV v(out) WHEN v(out)=0.3 RISE=1\n"
+                                           ".meas tran never FIND v(out) WHEN v(out)=0.3 RISE=2\n.end")
+    pb = E.Problem(text)
+    pb.run_all()
+    m, reasons = pb.measures(0, 1, ["vin_at", "sl_at", "never"])
+    pulse = tuple(sn(x) for x in ("0", "1", "1u", "1n", "1n", "5u", "20u"))
+    ck = O.Circuit(oracle)
+    W.rc_pulse(ck.add, r=sn("1k"), c=sn("1n"), pulse=pulse)
+    ck.freeze()
+    L = O.Lanes(ck, 1)
+    L.prepare()
+    t, x, st = L.tran(0, O.tran_opts(sn("10u"), sn("10n"), max_steps=200_000))
+    n = len(t)
+    vin, vout = (np.ascontiguousarray(x[:, k]) for k in (ck.node("in"), ck.node("out")))
+    fw = lambda y, y2, slope, **k: oracle.or_meas_find_when(P(t), P(y), P(y2), n, C.byref(O.clause(**k)), slope)
+    want = [fw(vout, vin, 0, val=sn("0.5"), fall=1), fw(vout, vout, 1, val=sn("0.3"), rise=1),
+            fw(vout, vout, 0, val=sn("0.3"), rise=2)]
+    assert bits_equal(m[0], want)
+    assert m[0, 0] == 0.0 and np.isfinite(m[0, 1]) and np.isnan(m[0, 2])
+
+
 def test_measure_sp_live_points(oracle, engine):
     N = 48
     say(f"case: measure.sp with live r, c: {N} points in one lane batch, bitwise vs the oracle")

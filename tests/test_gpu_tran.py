@@ -88,6 +88,11 @@ def oracle_meas(oracle, passes, tt, xx):
             out[k, 0] = oracle.or_meas_cross(P(tt), P(y), N, C.byref(cl))
         elif p.kind == E.MEAS_AT:
             out[k, 0] = oracle.or_meas_at(P(tt), P(y), N, p.at, p.op)
+        elif p.kind == E.MEAS_FIND:
+            w = passes[k - 1]
+            yw = np.ascontiguousarray(xx[:, w.row])
+            cl = O.clause(val=w.val, rise=w.rise, fall=w.fall, cross=w.cross, td=w.td, frm=w.frm, to=w.to)
+            out[k, 0] = oracle.or_meas_find_when(P(tt), P(yw), P(y), N, C.byref(cl), p.op)
         elif p.kind == E.MEAS_EXTREMUM:
             v, a = C.c_double(), C.c_double()
             oracle.or_meas_extremum(P(tt), P(y), N, p.frm, p.to, p.op, C.byref(v), C.byref(a))
@@ -160,6 +165,34 @@ def test_measure_sp_lanes_bitwise(oracle, engine):
     printed(m[9], 3.685512e+05, 6)
     printed(m[10], -9.93246e-10, 5)
     assert tuple(meas[0, 10, 1:]) == (1e-6, 6e-6)
+
+
+def test_find_when_lanes_bitwise(oracle, engine):
+    """FIND / DERIV ... WHEN as device passes (the WHEN's Cross, then the
+    Find reading it) on measure.sp's lanes, bit for bit against the oracle's
+    value at the WHEN's event; an event that never comes gives NaN on both
+    sides, and a Find whose WHEN has TD= is refused."""
+    N = 32
+    say(f"case: measure.sp RC, {N} lanes, FIND/DERIV ... WHEN passes")
+    t, c = build_both(oracle, W.rc_pulse)
+    rng = np.random.default_rng(12)
+    r = rng.uniform(500.0, 2000.0, N)
+    cap = rng.uniform(0.5e-9, 2e-9, N)
+    out, vin = t.node("out"), t.node("in")
+    passes = [E.meas_cross(out, 0.5, fall=1), E.meas_find(vin),
+              E.meas_cross(vin, 0.5, rise=1), E.meas_find(out, deriv=True),
+              E.meas_cross(out, 0.3, rise=2), E.meas_find(out),
+              E.meas_cross(out, 0.7, cross=1, to=8e-6), E.meas_find(out, deriv=True)]
+    res, meas, _, _ = run_pair(oracle, t, c, {(E.RESISTOR, 0, 0): r, (E.CAPACITOR, 0, 0): cap},
+                               dict(t_stop=10e-6, dt_init=10e-9), passes)
+    assert (res["status"] == 0).all()
+    assert np.all(meas[:, 1, 0] == 0.0)                     # v(in) is back at 0 when out falls
+    assert np.all(np.isfinite(meas[:, 3, 0]))
+    assert np.all(np.isnan(meas[:, 5, 0]))                  # a second rise never comes
+    lanes = E.Lanes(t, 1)
+    lanes.prepare()
+    with pytest.raises(E.EngineError):
+        lanes.tran(E.tran_opts(10e-6, 10e-9), [E.meas_cross(out, 0.5, rise=1, td=1e-6), E.meas_find(vin)])
 
 
 def test_rc_lowpass_lanes_bitwise(oracle, engine):

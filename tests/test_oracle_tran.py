@@ -94,6 +94,27 @@ def test_measure_sp_values(oracle, measure_run):
     assert (a.value, b.value) == (1e-6, 6e-6)
 
 
+def test_find_when_values(oracle, measure_run):
+    """FIND / DERIV ... WHEN on measure.sp's waveform: FIND of a vector at
+    its own crossing gives the crossing level, DERIV there the RC slope
+    (1 - v)/RC of the charging step; v(in) is back at 0 when v(out) falls
+    through 0.5 after 6 us; an event that never comes gives NaN. A swapped
+    vector, a value for a slope or a missed event fails one of these."""
+    c, t, x, _ = measure_run
+    tt = np.ascontiguousarray(t)
+    N = len(tt)
+    vout = np.ascontiguousarray(x[:, c.node("out")])
+    vin = np.ascontiguousarray(x[:, c.node("in")])
+    fw = lambda y, y2, slope, **k: oracle.or_meas_find_when(P(tt), P(y), P(y2), N, C.byref(O.clause(**k)), slope)
+    assert fw(vout, vout, 0, val=0.5, rise=1) == pytest.approx(0.5, abs=1e-15)
+    # dv/dt = (1 - v) / RC at v = 0.5 while charging, RC = 1 us (within the
+    # step's LTE: the local slope of the piecewise-linear waveform)
+    assert fw(vout, vout, 1, val=0.5, rise=1) == pytest.approx(0.5e6, rel=5e-3)
+    assert fw(vout, vin, 0, val=0.5, fall=1) == 0.0
+    assert fw(vout, vin, 0, val=0.5, rise=1) == 1.0
+    assert math.isnan(fw(vout, vout, 0, val=0.5, rise=2))
+
+
 def test_rc_lowpass_points(oracle):
     c = rc_deck(oracle, (0, 1, 0, 1e-6, 1e-6, 5e-3, 10e-3), 1e3, 159e-9, dc=0.0)
     L = O.Lanes(c, 1)
