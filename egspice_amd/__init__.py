@@ -57,7 +57,9 @@ class LaneResult(C.Structure):
 class TranOpts(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("t_stop", "dt_init", "dt_max", "dt_min", "xmu", "reltol", "abstol", "vntol",
                                           "chgtol", "trtol")] + \
-               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm", "pad")]
+               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm", "pad",
+                                          "early_stop", "pad2")] + \
+               [("abort_after", C.c_double), ("abort_mask", C.c_uint64)]
 
 
 class MeasPass(C.Structure):
@@ -181,14 +183,19 @@ def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e
 
 
 def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
-              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0, op_warm=False):
+              chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0, op_warm=False,
+              early_stop=False, abort_after=float("nan"), abort_mask=(1 << 64) - 1):
     """`.tran dt_init t_stop`; a deck's step ceiling is min(dt_init, t_stop/50)
     (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680), the
-    tolerances are converger.Options' (src/core/numerics.zig:378-399)."""
+    tolerances are converger.Options' (src/core/numerics.zig:378-399).
+    early_stop / abort_after / abort_mask: the points options' measure
+    handling (a lane ends once its passes stop; past abort_after, a running
+    pass of the mask aborts it with status 4)."""
     if dt_max is None:
         dt_max = min(dt_init, t_stop / 50)
     return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, method, max_iter,
-                    max_steps, record_capacity, 1 if op_warm else 0, 0)
+                    max_steps, record_capacity, 1 if op_warm else 0, 0, 1 if early_stop else 0, 0, abort_after,
+                    abort_mask)
 
 
 def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):
@@ -441,6 +448,15 @@ class PointsOptions(C.Structure):
     _fields_ = [("struct_size", C.c_size_t), ("abort_measure", EspiceBytes), ("abort_after", C.c_double),
                 ("measures_only", C.c_uint8), ("early_stop", C.c_uint8), ("warm_start", C.c_uint8),
                 ("tangent_step", C.c_double)]
+
+
+def points_options(abort_measure=None, abort_after=float("nan"), measures_only=False, early_stop=False,
+                   warm_start=False, tangent_step=0.0):
+    """espice_points_options (include/espice.h) for Problem.append_points."""
+    o = PointsOptions(C.sizeof(PointsOptions), _bytes(abort_measure or ""), abort_after, int(measures_only),
+                      int(early_stop), int(warm_start), tangent_step)
+    o._keep = abort_measure
+    return o
 
 
 def _points_lib():

@@ -605,9 +605,21 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
             }
             if (st.out.points < cap) rec_t[(size_t)l * cap + st.out.points] = t_new;
             st.out.points++;
+            bool open = false, bounded_open = false;
             for (uint32_t k = 0; k < n_pass; k++) {
                 if (passes[k].kind == 4) meas_feed_find(passes[k], ms[k], ms[k - 1], t_new, cur[passes[k].row]);
                 else meas_feed(passes[k], ms[k], t_new, cur[passes[k].row]);
+                open = open || !ms[k].i[7];
+                bounded_open = bounded_open || (!ms[k].i[7] && k < 64 && (to.abort_mask >> k & 1));
+            }
+            // the point's measure bound and early stop (tran.zig:1059-1064):
+            // past abort_after with a bounded measure unresolved the lane
+            // fails; with every measure resolved an early-stopping lane ends
+            if (to.abort_after == to.abort_after && t_new > to.abort_after && bounded_open) {
+                st.out.status = 4;
+                st.stopped = true;
+            } else if (to.early_stop && n_pass > 0 && !open) {
+                st.stopped = true;
             }
             st.bp_next = tran_next_bp(W, prm_lds, bpc, st, t_new, t_new + st.min_break);
             st.bp_clamped = false;

@@ -44,6 +44,18 @@ struct Point {
     uint32_t tran_points = 0;
     uint32_t trial = 0;        // Monte Carlo trial (HSPICE numbering from 1), 0: nominal draws
     std::vector<double> x;     // the operating point (n rows), when one was solved
+    // espice_points_options of the append that made the point
+    // (include/espice.h:286-301): points run in batches of equal options
+    struct Opts {
+        bool early_stop = false, warm_start = false;
+        double abort_after = NAN;
+        int abort_card = -1;  // the card abort_measure names, -1 every card
+        bool operator==(const Opts &o) const
+        {
+            return early_stop == o.early_stop && warm_start == o.warm_start && abort_card == o.abort_card &&
+                   ((abort_after != abort_after && o.abort_after != o.abort_after) || abort_after == o.abort_after);
+        }
+    } opts;
 };
 
 // One .meas card as device passes.
@@ -62,6 +74,7 @@ struct Problem {
     std::vector<espice_meas_pass> passes;
     std::vector<CardPasses> cards;
     std::vector<double> x_nominal;  // point 0's operating point: the start of `.dc monte` trials
+    std::vector<double> x_tran_nominal;  // the template's nominal operating point (warm_start), NaN: none
     uint32_t seed_trial = 0;        // the trial every non-Monte-Carlo point draws
     std::string error;
     ~Problem()
@@ -323,6 +336,44 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
         o.max_iter = 100;
         o.max_steps = 1000000000u;
         o.record_capacity = need_op && !have_op ? 1 : 0;
+        // the batch's points options (one set per batch)
+        const Point::Opts &po = P.points[b].opts;
+        o.early_stop = po.early_stop ? 1u : 0u;
+        o.abort_after = po.abort_after;
+        o.abort_mask = 0;
+        for (size_t k = 0; k < d.meas.size(); k++) {
+            if (po.abort_card >= 0 && (int)k != po.abort_card) continue;
+            const CardPasses &c = P.cards[k];
+            for (int q : {c.first, c.second})
+                if (q >= 0 && q < 64) o.abort_mask |= 1ull << q;
+            // a FIND ... WHEN card's WHEN is the pass before its Find
+            if ((d.meas[k].kind == "find_when" || d.meas[k].kind == "deriv_when") && c.first > 0 && c.first - 1 < 64)
+                o.abort_mask |= 1ull << (c.first - 1);
+        }
+        // warm_start: every point's operating point from the template's
+        // nominal one (src/analysis/dc/op.zig:588, 671), solved cold once
+        if (po.warm_start) {
+            if (P.x_tran_nominal.empty()) {
+                espice_lanes *L0 = nullptr;
+                if (espice_lanes_create(P.tpl, 1, &L0) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "GpuUnavailable");
+                std::vector<double> z(n, 0.0), x1(n);
+                espice_newton_opts no{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1u};
+                espice_lane_result r1{};
+                double ms0;
+                bool ok = espice_lanes_prepare(L0) == ESPICE_OK && espice_lanes_upload_x(L0, z.data()) == ESPICE_OK &&
+                          espice_lanes_newton_op(L0, &no, &r1, &ms0) == ESPICE_OK && r1.status == 1 &&
+                          espice_lanes_download_x(L0, x1.data()) == ESPICE_OK;
+                espice_lanes_destroy(L0);
+                P.x_tran_nominal = ok ? x1 : std::vector<double>(n, NAN);
+            }
+            if (std::isfinite(P.x_tran_nominal[0])) {
+                std::vector<double> xw((size_t)n * N);
+                for (uint32_t l = 0; l < N; l++)
+                    for (uint32_t i = 0; i < n; i++) xw[(size_t)i * N + l] = P.x_tran_nominal[i];
+                if (espice_lanes_upload_x(L, xw.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Upload");
+                o.op_warm = 1;
+            }
+        }
         std::vector<espice_tran_result> tr(N);
         std::vector<espice_meas_result> mr(P.passes.size() * N + 1);
         std::vector<double> t0(o.record_capacity ? N : 1), x0(o.record_capacity ? (size_t)N * n : 1);
@@ -349,9 +400,10 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             pt.tran_points = tr[l].points;
             if (s != 0) {
                 pt.failed = true;
-                pt.reason = s == 1 ? ESPICE_REASON_TIMESTEP_TOO_SMALL
-                          : s == 3 ? ESPICE_REASON_SINGULAR_MATRIX
-                                   : ESPICE_REASON_NO_CONVERGENCE;
+                pt.reason = s == 1   ? ESPICE_REASON_TIMESTEP_TOO_SMALL
+                            : s == 3 ? ESPICE_REASON_SINGULAR_MATRIX
+                            : s == 4 ? ESPICE_REASON_ABORTED
+                                     : ESPICE_REASON_NO_CONVERGENCE;
                 continue;
             }
             const espice_meas_result *m = &mr[(size_t)l * P.passes.size()];
@@ -552,7 +604,7 @@ espice_status espice_run_all(espice_problem *problem)
         size_t e = b;
         const bool trial = P->points[b].trial != 0 && P->bd.deck.monte_dc;
         while (e < n && !P->points[e].ran && !P->points[e].released && e - b < LANE_CAP &&
-               (P->points[e].trial != 0 && P->bd.deck.monte_dc) == trial)
+               (P->points[e].trial != 0 && P->bd.deck.monte_dc) == trial && P->points[e].opts == P->points[b].opts)
             e++;
         const espice_status s = run_batch(*P, b, e);
         if (s != ESPICE_OK) return s;
@@ -571,6 +623,7 @@ espice_status espice_append_points(espice_problem *problem, const double *values
         return fail(P, ESPICE_ABI_MISMATCH, "AbiMismatch");
     if (count == 0 || !values || !first_id) return fail(P, ESPICE_INVALID_ARGUMENT, "InvalidArgument");
     if (P->live.empty()) return fail(P, ESPICE_INVALID_ARGUMENT, "NoLiveParameters");
+    Point::Opts po;
     if (opts) {
         if (opts->measures_only > 1 || opts->early_stop > 1 || opts->warm_start > 1)
             return fail(P, ESPICE_INVALID_ARGUMENT, "InvalidArgument");
@@ -579,19 +632,27 @@ espice_status espice_append_points(espice_problem *problem, const double *values
         if (opts->abort_measure.len) {
             const std::string name = fold(opts->abort_measure.data, opts->abort_measure.len);
             bool found = false;
-            for (auto &m : P->bd.deck.meas) found = found || m.name == name;
+            for (size_t k = 0; k < P->bd.deck.meas.size(); k++)
+                if (P->bd.deck.meas[k].name == name) {
+                    found = true;
+                    po.abort_card = (int)k;
+                }
             if (!found) return fail(P, ESPICE_INVALID_ARGUMENT, "UnknownMeasure");
         }
-        if (opts->measures_only || opts->early_stop || !std::isnan(opts->abort_after) || opts->warm_start)
-            return fail(P, ESPICE_INVALID_ARGUMENT, "Unsupported");
         if (opts->struct_size >= sizeof(espice_points_options) && opts->tangent_step > 0)
             return fail(P, ESPICE_INVALID_ARGUMENT, "TangentsUnavailable");
+        // measures_only: the lanes measure as the .tran runs and keep no
+        // waveform in any case, so it changes nothing here
+        po.early_stop = opts->early_stop != 0;
+        po.warm_start = opts->warm_start != 0;
+        po.abort_after = opts->abort_after;
     }
     const size_t k = P->live.size();
     *first_id = P->queries;
     for (size_t i = 0; i < count; i++) {
         Point pt;
         pt.live.assign(values + i * k, values + (i + 1) * k);
+        pt.opts = po;
         pt.trial = P->seed_trial;
         pt.first_id = P->queries;
         P->queries += P->per_point;

@@ -23,6 +23,11 @@ struct TranOpts {
     double reltol, abstol, vntol, chgtol, trtol;
     uint32_t method, max_iter, max_steps, record_cap;
     uint32_t op_warm, pad;  // op_warm: the operating point starts at the lane's x
+    // early_stop: the lane ends once every pass has stopped; abort_after
+    // (NaN off): past it, a pass of abort_mask still running aborts the lane
+    uint32_t early_stop, pad2;
+    double abort_after;
+    uint64_t abort_mask;
 };
 
 // One .meas pass (scan.zig): 0 Cross (WHEN/TRIG/TARG), 1 At (FIND, or
@@ -41,7 +46,8 @@ struct MeasOut {
 };
 
 struct TranOut {
-    uint32_t status;  // 0 reached t_stop, 1 dt underflow, 2 no operating point, 3 singular operating point
+    uint32_t status;  // 0 reached t_stop (or stopped early), 1 dt underflow, 2 no operating point, 3 singular
+                      // operating point, 4 aborted by the measure bound
     uint32_t points, steps, attempts, rej_newton, rej_lte, bp_landings, nr_iters;
 };
 

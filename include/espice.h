@@ -76,10 +76,18 @@ typedef enum {
     ESPICE_REASON_OTHER = 255
 } espice_reason;
 
-/* include/espice.h:286-308: the same 48-byte layout. abort_after must be
- * NaN (off), measures_only and early_stop 0, tangent_step 0: the lane
- * engine records whole transients, and warm_start 0 (INVALID_ARGUMENT
- * "Unsupported" else). */
+/* include/espice.h:286-308: the same 48-byte layout and meanings. The
+ * lanes measure every .tran card as they run and keep no waveform, so
+ * measures_only changes nothing; early_stop ends a point's transient once
+ * its measures resolve (the values are those of the whole run);
+ * abort_after (NaN off) fails a point with ESPICE_REASON_ABORTED once t
+ * passes it while abort_measure's card (or, unnamed, any card) is still
+ * unresolved (src/analysis/tran/tran.zig:1059-1064); warm_start starts
+ * every point's operating point from the template's nominal one
+ * (src/analysis/dc/op.zig:588, 671). tangent_step must be 0
+ * (INVALID_ARGUMENT "TangentsUnavailable"). Points of one append share
+ * their options; points of appends with different options run in
+ * separate lane batches. */
 typedef struct {
     size_t struct_size;
     espice_bytes abort_measure;

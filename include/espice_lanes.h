@@ -194,6 +194,15 @@ typedef struct {
      * or a warm start), as the reference's warm operating point
      * (src/analysis/dc/op.zig:588, 671). */
     uint32_t op_warm, pad;
+    /* The points options' measure handling (include/espice.h:286-301,
+     * src/analysis/tran/tran.zig:1059-1064), checked at every accepted
+     * point after the passes read it: early_stop 1 ends a lane once every
+     * pass has stopped (its values are final); abort_after (NaN off) ends a
+     * lane with status 4 once t passes it while a pass whose bit is set in
+     * abort_mask (pass k: bit k) is still running. */
+    uint32_t early_stop, pad2;
+    double abort_after;
+    uint64_t abort_mask;
 } espice_tran_opts;
 
 /* One .meas pass over an unknown `row` (a node voltage or a branch

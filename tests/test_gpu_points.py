@@ -202,6 +202,92 @@ def test_cmos_inverter_points(oracle, engine):
         assert bits_equal(m[l, 0], targ - trig), l
 
 
+# src/tests/espice.zig:1789-1798, the reference's measure-bound deck
+ABORT_SP = """abort bound
+.param rv=1k
+v1 in 0 pulse(0 1 0.1n 0.1n 0.1n 20n 40n)
+r1 in out {rv}
+c1 out 0 0.1p
+.tran 1p 10n
+.meas tran tpd trig v(in) val=0.5 rise=1 targ v(out) val=0.5 rise=1
+.end
+"""
+
+
+def test_points_abort_after_and_early_stop(engine):
+    """The reference's own points test (src/tests/espice.zig:1820-1861):
+    rv = 1k, 100k, 2k appended twice, plain and with measures_only,
+    early_stop and abort_after = 1 ns: the bounded run aborts only the
+    100k point (its delay is past the bound), and the other two keep the
+    plain run's values bit for bit."""
+    say("case: abort.sp points, plain vs measures_only + early_stop + abort_after=1n")
+    pb = E.Problem(ABORT_SP, live=("rv",))
+    rows = [[1e3], [100e3], [2e3]]
+    plain = pb.append_points(rows)
+    bounded = pb.append_points(rows, E.points_options(measures_only=True, early_stop=True, abort_after=1e-9))
+    pb.run_all()
+    want, want_why = pb.measures(plain, 3, ["tpd"])
+    got, why = pb.measures(bounded, 3, ["tpd"])
+    assert want_why.tolist() == [0, 0, 0]
+    assert why.tolist() == [0, 6, 0]  # ok, ESPICE_REASON_ABORTED, ok
+    assert np.isnan(got[1, 0])
+    assert bits_equal(got[[0, 2], 0], want[[0, 2], 0])
+    # a bound on another card than the unresolved one aborts nothing; an
+    # unknown card name is refused
+    with pytest.raises(E.PointsError, match="UnknownMeasure"):
+        pb.append_points(rows, E.points_options(abort_measure="nope", abort_after=1e-9))
+
+
+def test_points_warm_start(oracle, engine):
+    """warm_start: every point's operating point from the template's nominal
+    one (op.zig:588, 671): bitwise the oracle's transient started from its
+    own nominal operating point, and inside the Newton tolerance of the
+    cold start's measures."""
+    N = 8
+    say(f"case: cmos_inverter.sp, live vdd and wn, {N} points warm from the nominal operating point")
+    pb = E.Problem(decks.CMOS_INVERTER_SP, live=("vdd", "wn"))
+    rng = np.random.default_rng(19)
+    rows = np.stack([rng.uniform(1.6, 2.0, N), rng.uniform(0.8e-6, 1.5e-6, N)], axis=1)
+    cold = pb.append_points(rows)
+    warm = pb.append_points(rows, E.points_options(warm_start=True))
+    pb.run_all()
+    mc, rc = pb.measures(cold, N, ["tphl"])
+    mw, rw = pb.measures(warm, N, ["tphl"])
+    assert (rc == 0).all() and (rw == 0).all()
+    assert np.allclose(mw, mc, rtol=1e-3, atol=1e-13)
+    M = E.MOS1_PARAMS
+    model = lambda typ, vto, kp: {M["TYPE"]: typ, M["VTO"]: sn(vto), M["KP"]: sn(kp), M["LAMBDA"]: sn("0.05"),
+                                  M["GAMMA"]: sn("0.4"), M["PHI"]: sn("0.8"), M["L"]: sn("180n")}
+
+    def inverter(vdd, wn):
+        ck = O.Circuit(oracle)
+        ck.add(O.VSRC, ["vdd", "0"], {0: vdd})
+        ck.add(O.VSRC, ["in", "0"], W.pulse_params(sn("0"), vdd, sn("1n"), sn("100p"), sn("100p"), sn("5n"),
+                                                   sn("10n"), dc=sn("0")))
+        ck.add(O.CAP, ["out", "0"], {0: sn("10f")})
+        ck.add(O.MOS1, ["out", "in", "0", "0"], {**model(1.0, "0.45", "200u"), M["W"]: wn})
+        ck.add(O.MOS1, ["out", "in", "vdd", "vdd"], {**model(-1.0, "-0.45", "80u"), M["W"]: 2 * wn})
+        ck.freeze()
+        L = O.Lanes(ck, 1)
+        L.prepare()
+        return ck, L
+
+    # the template's nominal operating point: the deck's own values, cold
+    ck0, L0 = inverter(sn("1.8"), sn("1u"))
+    x0, s0, _ = L0.newton_op(mode=1, x0=np.zeros(ck0.n), init_fix=True)
+    assert s0[0] & 1
+    for l in range(0, N, 3):
+        vdd, wn = rows[l]
+        ck, L = inverter(vdd, wn)
+        t, x, st = L.tran(0, O.tran_opts(sn("20n"), sn("10p"), max_steps=200_000), x0=x0[:, 0])
+        assert st.status == 0
+        n = len(t)
+        vin, vout = (np.ascontiguousarray(x[:, ck.node(k)]) for k in ("in", "out"))
+        trig = oracle.or_meas_cross(P(t), P(vin), n, C.byref(O.clause(val=sn("0.9"), rise=1)))
+        targ = oracle.or_meas_cross(P(t), P(vout), n, C.byref(O.clause(val=sn("0.9"), fall=1)))
+        assert bits_equal(mw[l, 0], targ - trig), l
+
+
 def test_cli_prints_the_reference_lines(tmp_path, engine):
     say("case: egspice_amd/espice --backend=hip measure.sp")
     deck = tmp_path / "measure.sp"
