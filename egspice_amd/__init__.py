@@ -36,7 +36,8 @@ EXPORTS = (
     "espice_mlp_set_scaling", "espice_mlp_get_params", "espice_mlp_set_params", "espice_mlp_train",
     "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
     "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
-    "espice_lanes_tab_refactor_solve_device",
+    "espice_lanes_tab_refactor_solve_device", "espice_lanes_template_gmin_stamps", "espice_lanes_shape",
+    "espice_lanes_newton_homotopy", "espice_lanes_op_ladder",
 )
 
 
@@ -52,6 +53,18 @@ class NewtonOpts(C.Structure):
 
 class LaneResult(C.Structure):
     _fields_ = [("status", C.c_uint32), ("iterations", C.c_uint32)]
+
+
+class LaneHomotopy(C.Structure):
+    _fields_ = [("gshunt", C.c_double), ("gmodel", C.c_double), ("srcfact", C.c_double),
+                ("flags", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class LadderOpts(C.Structure):
+    _fields_ = [("gmin_start", C.c_double), ("max_iter", C.c_uint32), ("pad", C.c_uint32)]
+
+
+HOM_ACTIVE, HOM_COLD = 1, 2
 
 
 class TranOpts(C.Structure):
@@ -148,6 +161,12 @@ def library():
         "espice_lanes_warm_start": [P, P, f64p, f64p],
         "espice_lanes_newton_learned": [P, P, f64p, C.POINTER(NewtonOpts), C.POINTER(LaneResult), f64p, f64p, f64p],
         "espice_lanes_first_factors": [P, u32p, u32p],
+        "espice_lanes_template_gmin_stamps": [P, u32p, u32p],
+        "espice_lanes_shape": [P, u32p, u32p],
+        "espice_lanes_newton_homotopy": [P, C.POINTER(NewtonOpts), C.POINTER(LaneHomotopy), C.POINTER(LaneResult),
+                                         f64p],
+        "espice_lanes_op_ladder": [P, C.POINTER(NewtonOpts), C.POINTER(LadderOpts), C.POINTER(LaneResult), u32p,
+                                   f64p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -299,6 +318,14 @@ class Template:
         self.col_ptr, self.row_idx, self.diag, self.cur_row, self.q = cp, ri, dg, cr, q
         return self
 
+    def gmin_stamps(self):
+        """(col, slot) per row: where the operating point's stepping rungs
+        load their diagonal gmin (NONE where a row takes none)."""
+        col, colp = _u32(np.zeros(self.n))
+        slot, slotp = _u32(np.zeros(self.n))
+        _check(self.lib.espice_lanes_template_gmin_stamps(self.h, colp, slotp))
+        return col, slot
+
 
 class Lanes:
     """N lanes of one template, resident on the GPU."""
@@ -356,6 +383,34 @@ class Lanes:
         _check(self.lib.espice_lanes_newton_op(self.h, C.byref(o), res.ctypes.data_as(C.POINTER(LaneResult)),
                                                C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ms.value
+
+    def newton_homotopy(self, gshunt, gmodel, srcfact, flags, **kw):
+        """Static Newton of the active lanes under per-lane homotopy controls
+        (arrays or scalars) from the lanes' current x; returns
+        (x, status, iterations, kernel_ms)."""
+        h = np.zeros(self.N, dtype=[("gshunt", "f8"), ("gmodel", "f8"), ("srcfact", "f8"), ("flags", "u4"),
+                                    ("pad", "u4")])
+        h["gshunt"], h["gmodel"], h["srcfact"], h["flags"] = gshunt, gmodel, srcfact, flags
+        o = newton_opts(**kw)
+        res = np.zeros((self.N, 2), dtype=np.uint32)
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_newton_homotopy(self.h, C.byref(o), h.ctypes.data_as(C.POINTER(LaneHomotopy)),
+                                                     res.ctypes.data_as(C.POINTER(LaneResult)), C.byref(ms)))
+        return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ms.value
+
+    def op_ladder(self, x0=None, gmin_start=1e-2, ladder_max_iter=100, **kw):
+        """Every lane's operating point by the plain rung and, where it
+        fails, the stepping rungs; returns (x, status, iterations, rung,
+        kernel_ms)."""
+        self.upload_x(np.zeros(self.t.n * self.N) if x0 is None else x0)
+        o = newton_opts(**kw)
+        lo = LadderOpts(gmin_start, ladder_max_iter, 0)
+        res = np.zeros((self.N, 2), dtype=np.uint32)
+        rung, rp = _u32(np.zeros(self.N))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_op_ladder(self.h, C.byref(o), C.byref(lo),
+                                               res.ctypes.data_as(C.POINTER(LaneResult)), rp, C.byref(ms)))
+        return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), rung, ms.value
 
     def first_factors(self):
         """(predicted, serial): how the last Newton launch factored each

@@ -132,11 +132,11 @@ constexpr double KB = 1.38064852e-23, QE = 1.6021766208e-19;
 // Compiled kinds (va_models.h): the setup, its collapse flags and the
 // collapse pairs. setup returns 0, or 1 + the index of a parameter outside
 // its declared range.
-inline int va_setup(int kind, const double *u, double *p)
+inline int va_setup(int kind, const double *u, double *p, double gmin = VA_SIMPARAM_GMIN)
 {
     switch (kind) {
-    case K_PSP103: return psp103_setup(u, TEMPERATURE_K, VA_SIMPARAM_GMIN, p);
-    case K_BSIM4: return bsim4va_setup(u, TEMPERATURE_K, VA_SIMPARAM_GMIN, p);
+    case K_PSP103: return psp103_setup(u, TEMPERATURE_K, gmin, p);
+    case K_BSIM4: return bsim4va_setup(u, TEMPERATURE_K, gmin, p);
     default: return -1;
     }
 }
@@ -855,8 +855,12 @@ ESP_HD void seed_diode(const P &p, const double *cur, double *lim)
 // Evaluation of one instance of `kind` at local voltages lx. In a
 // transient (`tran`) sources read their waveform at t and the charges and
 // dQ/du of capacitors are filled in (capacitor.va: charge c*V(p,n)).
+// Independent sources multiply their value by `srcfact`, the operating
+// point's source-stepping factor $simparam("sourceScaleFactor")
+// (vsource.va:333, isource.va:332); 1 leaves the value's bits.
 template <class P>
-ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tran = false, double t = 0.0)
+ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tran = false, double t = 0.0,
+                      double srcfact = 1.0)
 {
     for (int r = 0; r < 4; r++) {
         o.res[r] = 0.0;
@@ -870,7 +874,7 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tr
     case K_VSRC:
         o.res[0] = lx[2];
         o.res[1] = -lx[2];
-        o.res[2] = lx[0] - lx[1] - (tran ? src_value(p, t) : p[SRCP_DC]);
+        o.res[2] = lx[0] - lx[1] - (tran ? src_value(p, t) : p[SRCP_DC]) * srcfact;
         o.jac[0][2] = 1.0;
         o.jac[1][2] = -1.0;
         o.jac[2][0] = 1.0;
@@ -879,7 +883,7 @@ ESP_HD void eval_kind(int kind, const P &p, const double *lx, DevOut &o, bool tr
     case K_ISRC: {
         // I(p, n) <+ out: dc at an operating point, the waveform in a
         // transient (models/isource.va:216-330)
-        const double out = tran ? src_value(p, t) : p[SRCP_DC];
+        const double out = (tran ? src_value(p, t) : p[SRCP_DC]) * srcfact;
         o.res[0] = out;
         o.res[1] = -out;
         break;

@@ -47,6 +47,9 @@ struct Template {
     int add(int kind, const uint32_t *ports, const double *params, uint32_t *inst);
     int freeze();
     uint32_t find_slot(uint32_t row, uint32_t col) const;
+    // the operating point's gmin stamps (2n words: row i's column or
+    // ESP_NONE, then its slot), frozen templates only
+    void gmin_stamps(std::vector<uint32_t> &out) const;
 };
 
 bool is_tridiag(uint32_t n, const uint32_t *col_ptr, const uint32_t *row_idx);
@@ -83,6 +86,30 @@ constexpr uint32_t G_INITFIX = 2u;
 struct Newton {
     double reltol, abstol, vntol, residual_tol, gmin;
     uint32_t max_iter, init_fix;
+};
+
+// One lane's homotopy for one static Newton of the operating point's
+// stepping rungs (src/analysis/dc/op.zig:185-330): the diagonal gmin
+// shunt, the gmin the compiled kinds read as $simparam("gmin") (the
+// reference's setHomotopy, Circuit.zig:1394-1405), the independent
+// sources' factor ("sourceScaleFactor") and whether the lane runs, cold
+// (INITFIX) or warm.
+struct LaneHom {
+    double gshunt, gmodel, srcfact;
+    uint32_t flags, pad;
+};
+constexpr uint32_t HOM_ACTIVE = 1u, HOM_INITFIX = 2u;
+static_assert(sizeof(LaneHom) == 32, "LaneHom is four 8-byte words");
+
+// A homotopy launch: per-lane controls (null: none, every lane runs as
+// the Newton options say), the gmin stamps (2n words: row i's column or
+// ESP_NONE, then its slot; null: every diagonal) and the words of a lane's
+// prepared constants that hold the compiled kinds' $simparam("gmin").
+struct Homotopy {
+    const LaneHom *lane;
+    const uint32_t *stamp;
+    const uint32_t *gslot;
+    uint32_t n_gslot;
 };
 
 // Everything a lane reads or writes, as lane-major planes.

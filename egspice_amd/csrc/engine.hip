@@ -120,17 +120,33 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 // state in dynamic LDS laid out by `lay` after the template's tables. With
 // DENSE the lane's direct solver replays on the whole wavefront
 // (wave_lu.h, patterns up to 64 unknowns); otherwise thread 0 runs it.
+// With a homotopy (hom.lane), a lane whose control is inactive returns at
+// once; the others take their gmin shunt, cold or warm start and source
+// factor from it and first write their model gmin into their own
+// prepared constants.
 template <bool DENSE, int VA>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
-                                                         const double *prm_lanes, double *x_glob, uint32_t N, Newton o,
+                                                         double *prm_lanes, double *x_glob, uint32_t N, Newton o_in,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
                                                          uint32_t *pred_hit, uint32_t *h_status, uint32_t *h_iters,
-                                                         double *h_x, double *lu_g)
+                                                         double *h_x, double *lu_g, Homotopy hom)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t n = B.n;
+    Newton o = o_in;
+    double srcfact = 1.0;
+    if (hom.lane != nullptr) {
+        const LaneHom h = hom.lane[l];
+        if (!(h.flags & HOM_ACTIVE)) return;
+        o.gmin = h.gshunt;
+        o.init_fix = (h.flags & HOM_INITFIX) ? 1u : 0u;
+        srcfact = h.srcfact;
+        for (uint32_t i = tid; i < hom.n_gslot; i += nt) prm_lanes[(size_t)l * B.n_prm + hom.gslot[i]] = h.gmodel;
+        __threadfence_block();
+        __syncthreads();
+    }
     // the template's tables first (B.words is even; the lean layout reads
     // them in global memory), then the lane block
     uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
@@ -139,6 +155,10 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     double *block = lds + table_lds_words(B.words, lay) / 2;
     const WaveTemplate W = wave_template(B, lay.lu_global ? blob : tl);
     WaveLane s = wave_lane(W, lay, block);
+    if (hom.lane != nullptr) {
+        s.gstamp = hom.stamp;
+        s.srcfact = srcfact;
+    }
     if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lane_scratch_words(W.cap, W.n_tape));
     DenseLU D = dense_lu(block, lay, n, pred, pred_hit ? pred_hit + l : nullptr);
     if (tid == 0 && pred_hit) pred_hit[l] = 0;  // this launch's record starts empty
@@ -725,6 +745,13 @@ struct espice_lanes {
     size_t lds_bytes = 0;
     bool use_wave = false;
     std::vector<double> prm_host;
+    // the operating point's homotopy launches: per-lane controls, the gmin
+    // stamps, and the prepared-constant words holding a compiled kind's
+    // $simparam("gmin") (found once per prepare)
+    DevBuf<LaneHom> hom_lane;
+    DevBuf<uint32_t> hom_stamp, hom_gslot;
+    uint32_t n_gslot = 0;
+    bool hom_ready = false;
 
     ~espice_lanes()
     {
@@ -777,12 +804,12 @@ static void launch_analyze(espice_lanes *L, const Newton &o)
 }
 
 template <int V>
-static void launch_newton_dense(espice_lanes *L, const Newton &o, bool want_x)
+static void launch_newton_dense(espice_lanes *L, const Newton &o, bool want_x, const Homotopy &hom)
 {
     hipLaunchKernelGGL((newton_wave_kernel<true, V>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                        L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                        L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                       want_x ? L->d_x : nullptr, L->lu_g.p);
+                       want_x ? L->d_x : nullptr, L->lu_g.p, hom);
 }
 
 template <int V>
@@ -797,7 +824,7 @@ static void launch_tran(espice_lanes *L, uint32_t N, const WaveLayout &lay, size
 
 static void (*const LAUNCH_ANALYZE[VA_SETS])(espice_lanes *, const Newton &) = {
     launch_analyze<0>, launch_analyze<1>, launch_analyze<2>, launch_analyze<3>};
-static void (*const LAUNCH_NEWTON[VA_SETS])(espice_lanes *, const Newton &, bool) = {
+static void (*const LAUNCH_NEWTON[VA_SETS])(espice_lanes *, const Newton &, bool, const Homotopy &) = {
     launch_newton_dense<0>, launch_newton_dense<1>, launch_newton_dense<2>, launch_newton_dense<3>};
 static void (*const TRAN_LAUNCH[VA_SETS])(espice_lanes *, uint32_t, const WaveLayout &, size_t, const Newton &,
                                           const Newton &, const TranOpts &, const MeasPass *, uint32_t, TranOut *,
@@ -906,6 +933,18 @@ espice_status espice_lanes_template_freeze(espice_lanes_template *t)
 {
     if (!t) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     if (t->t.freeze() != 0) return fail(ESPICE_FAILED, "OutOfWorkspace");
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_gmin_stamps(const espice_lanes_template *t, uint32_t *col, uint32_t *slot)
+{
+    if (!t || !col || !slot) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!t->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "NotFrozen");
+    std::vector<uint32_t> st;
+    t->t.gmin_stamps(st);
+    const uint32_t n = t->t.n;
+    std::memcpy(col, st.data(), 4 * (size_t)n);
+    std::memcpy(slot, st.data() + n, 4 * (size_t)n);
     return ESPICE_OK;
 }
 
@@ -1202,6 +1241,7 @@ espice_status espice_lanes_prepare(espice_lanes *L)
         L->has_pred = true;
     }
     L->prepared = true;
+    L->hom_ready = false;
     return ESPICE_OK;
 }
 
@@ -1237,19 +1277,21 @@ espice_status espice_lanes_download_x(espice_lanes *L, double *x)
 // The static Newton kernel of every lane on the lanes' stream, bracketed by
 // ev0/ev1; with want_x the workgroup-per-lane kernel also writes x to the
 // mapped host copy.
-static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *opts, bool want_x)
+static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *opts, bool want_x,
+                                   const Homotopy &hom = Homotopy{})
 {
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
     if (opts->max_iter == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
+    if (hom.lane != nullptr && !L->use_wave) return fail(ESPICE_INVALID_ARGUMENT, "HomotopyLaneTooLarge");
     Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
     if (L->use_wave) {
         if (L->lay.dense != ESP_NONE)
-            LAUNCH_NEWTON[L->va_mask](L, o, want_x);
+            LAUNCH_NEWTON[L->va_mask](L, o, want_x, hom);
         else
             hipLaunchKernelGGL((newton_wave_kernel<false, 0>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p);
+                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p, hom);
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -1278,6 +1320,72 @@ static espice_status finish_newton(espice_lanes *L, espice_lane_result *out, dou
     }
     for (size_t l = 0; l < N; l++) out[l] = espice_lane_result{L->h_status.p[l], L->h_iters.p[l]};
     if (x) std::memcpy(x, L->h_x.p, 8 * nx);
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_shape(const espice_lanes *L, uint32_t *n, uint32_t *n_lanes)
+{
+    if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (n) *n = L->t->n;
+    if (n_lanes) *n_lanes = L->N;
+    return ESPICE_OK;
+}
+
+// The homotopy tables of a prepared batch: the template's gmin stamps and,
+// per compiled instance, the prepared constants that carry the model's
+// gmin. Those are found on lane 0 by a second setup at another gmin: a
+// constant that moves must equal the gmin in both setups (a copy of
+// $simparam("gmin"), as psp103's and bsim4va's are), so writing the
+// homotopy's gmin there is the reference's setup rerun.
+static espice_status homotopy_tables(espice_lanes *L)
+{
+    if (L->hom_ready) return ESPICE_OK;
+    const Template &T = *L->t;
+    std::vector<uint32_t> st;
+    T.gmin_stamps(st);
+    HIP_TRY(upload(L->hom_stamp, st));
+    std::vector<uint32_t> gslot;
+    const InstEntry *ie = reinterpret_cast<const InstEntry *>(L->wh.inst_bytes.data());
+    constexpr double PROBE = 0x1p-15;
+    for (uint32_t e = 0; e < L->wh.n_inst; e++) {
+        const int kind = ie[e].kind;
+        if (!kind_is_va(kind)) continue;
+        const int uk = KIND_UK[kind], pk = KIND_PK[kind];
+        std::vector<double> u(&T.uprm[kind][(size_t)ie[e].id * uk], &T.uprm[kind][(size_t)ie[e].id * uk] + uk);
+        for (const auto &kv : L->va_over)
+            if ((int)(kv.first >> 56) == kind && ((kv.first >> 24) & 0xffffffffu) == ie[e].id)
+                u[kv.first & 0xffffffu] = kv.second[0];
+        std::vector<double> p0(pk), p1(pk);
+        if (va_setup(kind, u.data(), p0.data()) != 0 || va_setup(kind, u.data(), p1.data(), PROBE) != 0)
+            return fail(ESPICE_INVALID_ARGUMENT, "ParamRange");
+        for (int k = 0; k < pk; k++) {
+            if (std::memcmp(&p0[k], &p1[k], 8) == 0) continue;
+            if (p0[k] != VA_SIMPARAM_GMIN || p1[k] != PROBE) return fail(ESPICE_INVALID_ARGUMENT, "HomotopyGminDerived");
+            gslot.push_back(ie[e].prm + (uint32_t)k);
+        }
+    }
+    L->n_gslot = (uint32_t)gslot.size();
+    if (gslot.empty()) gslot.push_back(0);  // a valid buffer; n_gslot says none
+    HIP_TRY(upload(L->hom_gslot, gslot));
+    HIP_TRY(L->hom_lane.alloc(L->N));
+    L->hom_ready = true;
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_newton_homotopy(espice_lanes *L, const espice_newton_opts *opts,
+                                           const espice_lane_homotopy *lanes, espice_lane_result *out,
+                                           double *kernel_ms)
+{
+    static_assert(sizeof(espice_lane_homotopy) == sizeof(LaneHom), "the C-ABI control is LaneHom");
+    if (!L || !opts || !lanes || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    if (espice_status e = homotopy_tables(L)) return e;
+    HIP_TRY(hipMemcpyAsync(L->hom_lane.p, lanes, sizeof(LaneHom) * L->N, hipMemcpyHostToDevice, L->stream));
+    const Homotopy hom{L->hom_lane.p, L->hom_stamp.p, L->hom_gslot.p, L->n_gslot};
+    if (espice_status e = launch_newton(L, opts, false, hom)) return e;
+    if (espice_status e = finish_newton(L, out, nullptr, kernel_ms)) return e;
+    for (size_t l = 0; l < L->N; l++)
+        if (!(lanes[l].flags & ESPICE_HOM_ACTIVE)) out[l] = espice_lane_result{0, 0};
     return ESPICE_OK;
 }
 

@@ -150,6 +150,89 @@ int Template::freeze()
     return 0;
 }
 
+// ngspice's placement of the stepping rungs' diagonal gmin
+// (src/analysis/dc/op.zig:711-790 gminStamps, after spMNA_Preorder of
+// sputils.c and LoadGmin of spsmp.c): every column with no diagonal
+// element (a voltage source's branch, whose diagonal nothing stamps) is
+// swapped with a column holding a symmetric pair of +-1 entries, lone
+// pairs first, then the first column with several; each row's gmin then
+// lands on the entry its column's position holds. A grounded source's
+// gmin so goes onto its two incidence entries rather than its node's
+// diagonal. The reference reads |value| == 1 off an assembled iterate; in
+// the lane kinds a branch column and its row hold nothing but the
+// source's own +-1 incidence (no other kind stamps a branch), so the test
+// is structural here: `unit` marks those entries.
+void Template::gmin_stamps(std::vector<uint32_t> &out) const
+{
+    std::vector<uint8_t> unit(nnz, 0);
+    for (uint32_t id = 0; id < count[K_VSRC]; id++) {
+        const uint32_t *nd = &nodes[K_VSRC][(size_t)id * 3];
+        if (nd[0] == nd[1]) continue;  // +1 and -1 on one entry sum to 0
+        for (int t = 0; t < 2; t++) {
+            if (nd[t] == 0) continue;
+            unit[find_slot(nd[t], nd[2])] = 1;
+            unit[find_slot(nd[2], nd[t])] = 1;
+        }
+    }
+    std::vector<uint32_t> perm(n), dcol(n);
+    for (uint32_t i = 0; i < n; i++) {
+        perm[i] = i;
+        dcol[i] = cur_row[i] ? ESP_NONE : i;
+    }
+    // the first +-1 entry of position j's column whose transpose is +-1 too,
+    // and how many such (counting stops at 2)
+    auto twins = [&](uint32_t j, uint32_t &row) {
+        uint32_t cnt = 0;
+        const uint32_t col = perm[j];
+        for (uint32_t p = col_ptr[col]; p < col_ptr[col + 1]; p++) {
+            const uint32_t r = row_idx[p];
+            if (r == 0 || !unit[p]) continue;
+            const uint32_t q = find_slot(j, perm[r]);
+            if (q == ESP_NONE || !unit[q]) continue;
+            if (++cnt == 1) row = r;
+            else break;
+        }
+        return cnt;
+    };
+    auto swap = [&](uint32_t j, uint32_t r) {
+        dcol[j] = perm[r];
+        dcol[r] = perm[j];
+        std::swap(perm[j], perm[r]);
+    };
+    uint32_t start = 1;
+    for (;;) {
+        bool swapped = false, again = false;
+        for (uint32_t j = start; j < n; j++) {
+            if (dcol[j] != ESP_NONE) continue;
+            uint32_t r = 0;
+            const uint32_t c = twins(j, r);
+            if (c == 1) {
+                swap(j, r);
+                swapped = true;
+            } else if (c > 1 && !again) {
+                again = true;
+                start = j;
+            }
+        }
+        if (!again) break;
+        for (uint32_t j = start; !swapped && j < n; j++) {
+            if (dcol[j] != ESP_NONE) continue;
+            uint32_t r = 0;
+            if (twins(j, r) > 0) {
+                swap(j, r);
+                swapped = true;
+            }
+        }
+        if (!swapped) break;
+    }
+    out.assign(2 * (size_t)n, ESP_NONE);
+    for (uint32_t i = 1; i < n; i++) {
+        if (dcol[i] == ESP_NONE) continue;
+        out[i] = dcol[i];
+        out[n + i] = find_slot(i, dcol[i]);
+    }
+}
+
 }  // namespace esp
 
 

@@ -153,6 +153,12 @@ struct WaveLane {
     LaneLU lu;
     uint32_t *flags;  // 0 singular, 1 limited, 2 done, 3 bits, 4 ok, 5 scaled bits lo, 6 hi
     double *q, *cc, *tape;  // transient: row charges, C plane, charge per LTE site
+    // operating-point homotopy (the ladder's stepping rungs): where the
+    // diagonal gmin lands, row i at column gstamp[i] (ESP_NONE: nowhere)
+    // and slot gstamp[n + i], or every diagonal when null; the independent
+    // sources' scale factor
+    const uint32_t *gstamp;
+    double srcfact;
 };
 
 ESP_HD size_t wave_words(const WaveTemplate &W, WaveLayout &L, bool transient = false, size_t tran_extra = 0)
@@ -207,6 +213,8 @@ ESP_HD WaveLane wave_lane(const WaveTemplate &W, const WaveLayout &L, double *bl
     s.stage = block + L.stage;
     s.prm = block + L.prm;
     s.gprm = s.prm;
+    s.gstamp = nullptr;
+    s.srcfact = 1.0;
     const uint32_t lucap = L.lu_global ? 0 : cap;
     double *f = block + L.lu_f64;
     LaneLU &u = s.lu;
@@ -293,7 +301,7 @@ ESP_HD void wave_eval_k(const WaveTemplate &W, WaveLane &s, const InstEntry &ie,
             if (limw >> u & 1) lx[u] = s.lim[ie.lim + u];
     }
     DevOut o;
-    eval_kind(KIND, p, lx, o, TRAN, t);
+    eval_kind(KIND, p, lx, o, TRAN, t, s.srcfact);
     bool charged = false;
 #pragma unroll
     for (int ru = 0; ru < nu; ru++) charged = charged || kind_cmask(KIND, ru) != 0;
@@ -680,7 +688,9 @@ ESP_HD void wave_fresh_solver(const WaveTemplate &W, WaveLane &s)
 }
 
 // Thread 0's finish of the planes: ground stamp, gmin, and at tick 0 the
-// runaway bound (converger.zig loadExtras/plausibleLimit).
+// runaway bound (converger.zig loadExtras/plausibleLimit). The gmin lands
+// on every diagonal, or at the lane's stamps (converger.zig:700-718
+// loadGmin: vals[slot] += gmin, rhs[row] += gmin * x[col]).
 ESP_HD void wave_assemble(const WaveTemplate &W, WaveLane &s, const Newton &o, uint32_t tick, double &limit)
 {
     const uint32_t n = W.P.n;
@@ -688,8 +698,15 @@ ESP_HD void wave_assemble(const WaveTemplate &W, WaveLane &s, const Newton &o, u
     g[W.diag[0]] += 1.0;
     rhs[0] += x[0];
     if (o.gmin > 0) {
-        for (uint32_t i = 0; i < n; i++) g[W.diag[i]] += o.gmin;
-        for (uint32_t i = 0; i < n; i++) rhs[i] += o.gmin * x[i];
+        if (s.gstamp != nullptr) {
+            for (uint32_t i = 0; i < n; i++)
+                if (s.gstamp[i] != ESP_NONE) g[s.gstamp[n + i]] += o.gmin;
+            for (uint32_t i = 0; i < n; i++)
+                if (s.gstamp[i] != ESP_NONE) rhs[i] += o.gmin * x[s.gstamp[i]];
+        } else {
+            for (uint32_t i = 0; i < n; i++) g[W.diag[i]] += o.gmin;
+            for (uint32_t i = 0; i < n; i++) rhs[i] += o.gmin * x[i];
+        }
     }
     if (tick == 0) limit = plausible_limit(n, x, rhs, g, W.diag);
 }

@@ -117,6 +117,13 @@ espice_status espice_lanes_template_info(const espice_lanes_template *t, uint32_
 /* Copies the frozen pattern; any pointer may be NULL. */
 espice_status espice_lanes_template_pattern(const espice_lanes_template *t, uint32_t *col_ptr, uint32_t *row_idx,
     uint32_t *diag, uint8_t *current_row, uint32_t *q);
+/* Where the operating point's stepping rungs load their diagonal gmin
+ * (src/analysis/dc/op.zig:711-790 gminStamps: ngspice's spMNA_Preorder
+ * swaps each voltage-source branch column with a symmetric +-1 pair, then
+ * LoadGmin adds gmin at each row's (row, col[row])). col[n], slot[n]:
+ * row i's column and pattern slot, or 0xffffffff where row i takes none
+ * (row 0, ground, never does). */
+espice_status espice_lanes_template_gmin_stamps(const espice_lanes_template *t, uint32_t *col, uint32_t *slot);
 
 /* ---- a batch of lanes on the GPU (GpuLanes.initStatic, lanes.zig:1080) - */
 espice_status espice_lanes_create(const espice_lanes_template *t, uint32_t n_lanes, espice_lanes **out);
@@ -146,6 +153,57 @@ espice_status espice_lanes_newton_op(espice_lanes *l, const espice_newton_opts *
  * reference, whose lanes always run the full factor first
  * (src/solver/sparse_lu.zig:274-311); diagnostics only. */
 espice_status espice_lanes_first_factors(espice_lanes *l, uint32_t *predicted, uint32_t *serial);
+/* Unknowns per lane and lanes of a batch. */
+espice_status espice_lanes_shape(const espice_lanes *l, uint32_t *n, uint32_t *n_lanes);
+
+/* One lane's homotopy for espice_lanes_newton_homotopy: the operating
+ * point's stepping rungs (src/analysis/dc/op.zig:185-330) solve with a
+ * diagonal gmin shunt gshunt (S) at the gmin stamps, the compiled kinds'
+ * $simparam("gmin") at gmodel (the reference reruns their setup,
+ * Circuit.zig:1394-1405 setHomotopy, eval.zig:2596-2612) and every
+ * independent source's value times srcfact ($simparam("sourceScaleFactor"),
+ * models/vsource.va:333, isource.va:332). flags: bit 0 the lane runs,
+ * bit 1 it starts cold (the limiters' MODEINITJCT seeds, then INITFIX)
+ * rather than warm. */
+typedef struct {
+    double gshunt, gmodel, srcfact;
+    uint32_t flags, pad;
+} espice_lane_homotopy;
+enum { ESPICE_HOM_ACTIVE = 1, ESPICE_HOM_COLD = 2 };
+/* The static Newton of the lanes whose control is active, each under its
+ * homotopy, from its x (espice_lanes_upload_x); the others are untouched
+ * and their out entries read {0, 0}. opts gives the tolerances and
+ * max_iter; its gmin and init_fix are the controls'. A compiled kind's
+ * gmodel lands in the prepared constants that hold its $simparam("gmin")
+ * (found by rerunning its setup at a second gmin: HomotopyGminDerived when
+ * a constant derives from it otherwise) and stays there: a lane's last
+ * homotopy solve leaves its model gmin. HomotopyLaneTooLarge when the
+ * lanes do not run workgroup-per-lane. */
+espice_status espice_lanes_newton_homotopy(espice_lanes *l, const espice_newton_opts *opts,
+    const espice_lane_homotopy *lanes, espice_lane_result *out, double *kernel_ms);
+
+/* The operating-point ladder's options: gmin_start (1e-2 S, the gmin
+ * descent's first good gmin, src/core/numerics.zig:390) and the Newton
+ * budget of each stepping solve (100: optionsFromTolerances' floor of 100
+ * over itl2 = 50, converger.zig:86-88). */
+typedef struct {
+    double gmin_start;
+    uint32_t max_iter, pad;
+} espice_ladder_opts;
+/* Every lane's operating point by the reference's ladder
+ * (src/analysis/dc/op.zig:140-330 solveLadder/afterPlain, the host half of
+ * solveOnDevice :660-688): the plain rung on every lane (opts as
+ * espice_lanes_newton_op), then, for each lane it leaves unconverged or
+ * singular, dynamic gmin stepping from gmin_start down to the deck's gmin
+ * (1e-12 S) and a clean solve without the shunt, else source stepping
+ * from zero (with the sources-off gmin rescue) and a final Newton. The
+ * lanes that need a rung run together, one homotopy launch per solve.
+ * out[l]: status 1 converged, 0 not; iterations summed over every solve,
+ * saturating at 65535. rung[l] (optional): 1 plain, 2 gmin stepping, 3
+ * source stepping, 0 none converged (the reference's JFNK and OPtran rungs
+ * are not built). Each lane's x is left on the device. */
+espice_status espice_lanes_op_ladder(espice_lanes *l, const espice_newton_opts *opts, const espice_ladder_opts *lo,
+    espice_lane_result *out, uint32_t *rung, double *kernel_ms);
 
 /* ---- the lead-tape lane LU (lane_device.zig refactor :101, solveNeg :170)
  * Builds a LaneTab from one factor of (col_ptr, row_idx, vals) on the host

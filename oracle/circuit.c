@@ -255,6 +255,8 @@ static uint32_t find_slot(const or_circuit *c, uint32_t row, uint32_t col)
     return (lo < c->col_ptr[col + 1] && c->row_idx[lo] == row) ? lo : NONE;
 }
 
+uint32_t or_find_slot(const or_circuit *c, uint32_t row, uint32_t col) { return find_slot(c, row, col); }
+
 int or_ckt_freeze(or_circuit *c)
 {
     const uint32_t n = c->n;
@@ -417,6 +419,24 @@ void or_lanes_prepare(or_lanes *L)
     }
 }
 
+/* Lane l's compiled instances set up again with $simparam("gmin") at
+ * `gmin` (Circuit.zig:1394-1405 setHomotopy: the models that read it rerun
+ * their setup). */
+void or_lanes_set_model_gmin(or_lanes *L, uint32_t l, double gmin)
+{
+    const uint32_t N = L->N;
+    double u[OR_VA_PARAM_MAX], p[OR_VA_PARAM_MAX];
+    for (int kind = 0; kind < OR_NKIND; kind++) {
+        if (!is_va(kind)) continue;
+        for (uint32_t i = 0; i < L->c->count[kind]; i++) {
+            for (int k = 0; k < UK[kind]; k++) u[k] = L->uprm[kind][((size_t)i * UK[kind] + k) * N + l];
+            if (or_va_setup_g(kind, u, gmin, p) != 0)
+                for (int k = 0; k < PK[kind]; k++) p[k] = NAN;
+            for (int k = 0; k < PK[kind]; k++) L->pprm[kind][((size_t)i * PK[kind] + k) * N + l] = p[k];
+        }
+    }
+}
+
 /* ------------------------------------------------------------ evaluation */
 
 void or_load_p(const or_lanes *L, int kind, uint32_t id, uint32_t l, double *p)
@@ -507,6 +527,7 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
     memset(rhs, 0, 8 * (c->n + 1));
     if (ec && ec->q) memset(ec->q, 0, 8 * (c->n + 1));
     if (ec && ec->cc) memset(ec->cc, 0, 8 * (c->nnz + 1));
+    const double sf = ec && ec->srcfact ? *ec->srcfact : 1.0;
     uint32_t tape_at = 0;
     for (int t = 0; t < c->n_order; t++) {
         int kind = c->order[t];
@@ -534,14 +555,14 @@ void or_eval_planes(const or_lanes *L, uint32_t l, const double *x, int lim_acti
             case OR_VSRC:
                 o.res[0] = lx[2];
                 o.res[1] = -lx[2];
-                o.res[2] = lx[0] - lx[1] - (ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC]);
+                o.res[2] = lx[0] - lx[1] - (ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC]) * sf;
                 o.jac[0][2] = 1.0;
                 o.jac[1][2] = -1.0;
                 o.jac[2][0] = 1.0;
                 o.jac[2][1] = -1.0;
                 break;
             case OR_ISRC: { /* isource.va:216-330: dc at an OP, the waveform in a transient */
-                const double out = ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC];
+                const double out = (ec && ec->tran ? or_source_value(p, ec->t) : p[SRCP_DC]) * sf;
                 o.res[0] = out;
                 o.res[1] = -out;
                 break;
@@ -692,7 +713,12 @@ void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double 
  * dx on success, nonzero for a singular lane. */
 typedef int (*lin_fn)(void *ctx, uint32_t l, const double *a, const double *rhs, double *dx);
 
-static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double *x, or_outcome *out, lin_fn lin, void *lctx, int lim_active)
+/* `stamp` (2n words, or NULL for every diagonal): where gmin lands, row
+ * i at column stamp[i] (OR_NONE: nowhere) and slot stamp[n + i]
+ * (converger.zig:700-718 loadGmin). `srcfact`: the sources' factor, NULL
+ * for 1. */
+static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double *x, or_outcome *out, lin_fn lin, void *lctx, int lim_active,
+                       const uint32_t *stamp, const double *srcfact)
 {
     const or_circuit *c = L->c;
     const uint32_t n = c->n, nnz = c->nnz;
@@ -700,12 +726,18 @@ static void newton_one(or_lanes *L, uint32_t l, const or_newton_opts *o, double 
     double *dx = malloc(8 * n), *x_old = malloc(8 * n);
     uint32_t bits = o->init_fix ? G_INITFIX : 0;
     double limit = 1e9;
+    or_evctx sec = {0, 0.0, NULL, NULL, NULL, srcfact};
     memcpy(x_old, x, 8 * n);
     out->status = 0;
     out->iterations = o->max_iter;
     for (uint32_t tick = 0; tick < o->max_iter; tick++) {
-        or_eval_planes(L, l, x, lim_active, g, rhs, NULL);
-        if (o->gmin > 0) {
+        or_eval_planes(L, l, x, lim_active, g, rhs, srcfact ? &sec : NULL);
+        if (o->gmin > 0 && stamp) {
+            for (uint32_t i = 0; i < n; i++)
+                if (stamp[i] != OR_NONE) g[stamp[n + i]] += o->gmin;
+            for (uint32_t i = 0; i < n; i++)
+                if (stamp[i] != OR_NONE) rhs[i] += o->gmin * x[stamp[i]];
+        } else if (o->gmin > 0) {
             for (uint32_t i = 0; i < n; i++) {
                 g[c->diag[i]] += o->gmin;
             }
@@ -827,7 +859,7 @@ static void *op_worker(void *arg)
             t.udiag = calloc(n, 8);
             t.w = calloc(n, 8);
             t.y = calloc(n, 8);
-            newton_one(L, l, o, xl, &j->out[l], lin_tape, &t, o->init_fix);
+            newton_one(L, l, o, xl, &j->out[l], lin_tape, &t, o->init_fix, NULL, NULL);
             if (t.peels) j->out[l].status |= 4u;
             or_solver_free(t.own);
             free(t.lx); free(t.ux); free(t.udiag); free(t.w); free(t.y);
@@ -837,7 +869,7 @@ static void *op_worker(void *arg)
                 j->rc = -1;
                 break;
             }
-            newton_one(L, l, o, xl, &j->out[l], lin_direct, s, o->init_fix);
+            newton_one(L, l, o, xl, &j->out[l], lin_direct, s, o->init_fix, NULL, NULL);
             or_solver_free(s);
         }
         for (uint32_t i = 0; i < n; i++) j->x[(size_t)i * N + l] = xl[i];
@@ -895,6 +927,29 @@ int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_o
     return or_newton_op_mt(L, mode, o, x, out, 1);
 }
 
+/* One lane's static Newton under a homotopy (the ladder's stepping
+ * solves): gmin o->gmin at `stamp`, sources times srcfact, cold (the
+ * limiters' seeds, INITFIX) when o->init_fix, on the lane's own ordered
+ * direct solver, as or_newton_op's mode 1. */
+int or_newton_hom(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out, const uint32_t *stamp,
+                  double srcfact)
+{
+    const or_circuit *c = L->c;
+    const uint32_t n = c->n;
+    uint32_t *q = malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (or_order(n, c->col_ptr, c->row_idx, q) != 0) {
+        free(q);
+        return -1;
+    }
+    or_solver *s = or_solver_new_ordered(n, c->col_ptr, c->row_idx, q, or_diag_fill(n, c->col_ptr, c->row_idx, q));
+    free(q);
+    if (!s) return -1;
+    if (o->init_fix) or_seed_pass(L, l, xl);
+    newton_one(L, l, o, xl, out, lin_direct, s, o->init_fix, stamp, &srcfact);
+    or_solver_free(s);
+    return 0;
+}
+
 /* ------------------------------------------------------------- dc sweep */
 
 int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t npts, const or_newton_opts *o, double *x_out, uint32_t *converged)
@@ -919,7 +974,7 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
         if (pt != 0) {
             /* prepareWarm: limit the predictor against the last point */
             or_limit_pass(L, 0, sol, prev, 0);
-            newton_one(L, 0, &warm, sol, &oc, lin_direct, s, 1);
+            newton_one(L, 0, &warm, sol, &oc, lin_direct, s, 1, NULL, NULL);
         }
         if (!(oc.status & 1)) {
             /* cold point: the plain operating-point rung from zero */
@@ -927,7 +982,7 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
             or_newton_opts cold = *o;
             cold.init_fix = 1;
             or_seed_pass(L, 0, sol);
-            newton_one(L, 0, &cold, sol, &oc, lin_direct, s, 1);
+            newton_one(L, 0, &cold, sol, &oc, lin_direct, s, 1, NULL, NULL);
         }
         converged[pt] = oc.status & 1;
         memcpy(x_out + (size_t)pt * n, sol, 8 * n);

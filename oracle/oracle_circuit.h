@@ -147,4 +147,25 @@ void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double 
  * the last point, warm Newton per point). x_out: npts*n, point-major. */
 int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t npts, const or_newton_opts *o, double *x_out, uint32_t *converged);
 
+/* The operating point's ladder after the plain rung (ladder.c,
+ * src/analysis/dc/op.zig:140-415). */
+/* One lane's static Newton under a homotopy: gmin o->gmin at `stamp`
+ * (2n words; NULL: every diagonal), sources times srcfact, cold when
+ * o->init_fix; the lane's ordered direct solver. */
+int or_newton_hom(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out, const uint32_t *stamp,
+                  double srcfact);
+/* Lane l's compiled instances set up at $simparam("gmin") = gmin. */
+void or_lanes_set_model_gmin(or_lanes *L, uint32_t l, double gmin);
+/* gminStamps from lane l's planes at x = 0 (2n words). */
+void or_gmin_stamps(or_lanes *L, uint32_t l, uint32_t *stamp);
+double or_gmin_factor(double factor, uint32_t iterations, uint32_t budget);
+int or_gmin_stalled(const double *gmin, const int *conv, uint32_t count, double start, double next);
+uint32_t or_gmin_replay(const int *conv, const uint32_t *it, uint32_t count, double gmin_start, uint32_t budget,
+                        double *next, int *action);
+/* Plain rung then rungs 2-3 and the final Newton per failing lane; x
+ * (n * N node-major) in and out; out[l] status 1 / 0, iterations summed
+ * (saturating at 65535); rung[l] 1 plain, 2 gmin, 3 source, 0 none. */
+int or_op_ladder(or_lanes *L, const or_newton_opts *plain, double gmin_start, uint32_t budget, double *x,
+                 or_outcome *out, uint32_t *rung);
+
 #endif

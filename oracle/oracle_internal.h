@@ -63,6 +63,9 @@ typedef struct {
     double *q;    /* n + 1 row charges (ground and trash row included), or NULL */
     double *cc;   /* nnz + 1 C-plane values, or NULL */
     double *tape; /* one charge per LTE site, or NULL */
+    /* the independent sources' factor, $simparam("sourceScaleFactor")
+     * (vsource.va:333, isource.va:332); NULL for 1 */
+    const double *srcfact;
 } or_evctx;
 
 struct or_lanes {
@@ -89,6 +92,12 @@ int or_va_nsite(int kind);
 unsigned or_va_gmask(int kind, int ru);
 unsigned or_va_cmask(int kind, int ru);
 int or_va_setup(int kind, const double *u, double *p);
+int or_va_setup_g(int kind, const double *u, double gmin, double *p);
+/* The pattern slot of (row, col), OR_NONE when absent. */
+uint32_t or_find_slot(const struct or_circuit *c, uint32_t row, uint32_t col);
+/* $simparam("gmin") of the compiled kinds outside the stepping rungs: the
+ * deck's gmin (src/core/numerics.zig:386) */
+#define OR_SIMPARAM_GMIN 1e-12
 int or_va_collapse(int kind, const double *u, const uint32_t *ports, int64_t *rows);
 void or_va_eval(int kind, const double *p, const double *x, double *res, double *jac);
 void or_va_eval_tran(int kind, const double *p, const double *x, double *res, double *jac, double *q, double *cjac,

@@ -199,7 +199,7 @@ static int tr_newton(tr_ws *w, double *x, double t, const tr_hook *h, const or_n
                      int lim_active, uint32_t *iterations)
 {
     const uint32_t n = w->n, nnz = w->nnz;
-    or_evctx ec = {1, t, w->q, h->on ? w->cc : NULL, NULL};
+    or_evctx ec = {1, t, w->q, h->on ? w->cc : NULL, NULL, NULL};
     uint32_t bits = init_fix ? 2u : 0u;
     double limit = 1e9;
     memcpy(w->x_old, x, 8 * n);
@@ -352,7 +352,7 @@ int or_tran_run_from(or_lanes *L, uint32_t l, const or_tran_opts *opt, const dou
     memcpy(prev, cur, 8 * n);
     if (has_charge) {
         /* q(op) seeds a flat history; i_prev starts at 0 */
-        or_evctx ec = {1, 0.0, qh[1], NULL, th[1]};
+        or_evctx ec = {1, 0.0, qh[1], NULL, th[1], NULL};
         or_eval_planes(L, l, cur, 0, w.g, w.rhs, &ec);
         for (int k = 2; k < TR_NH; k++) {
             memcpy(qh[k], qh[1], 8 * nq);
@@ -409,7 +409,7 @@ int or_tran_run_from(or_lanes *L, uint32_t l, const or_tran_opts *opt, const dou
         if (has_charge) {
             /* the charge re-read at the solution, unlimited (HostOps.charges
              * tran.zig:1207-1213, evalQRange eval.zig:1086-1108) */
-            or_evctx ec = {1, t_dev, qh[0], NULL, th[0]};
+            or_evctx ec = {1, t_dev, qh[0], NULL, th[0], NULL};
             or_eval_planes(L, l, trial, 0, w.g, w.rhs, &ec);
         }
         if (has_charge && steps > 0) {

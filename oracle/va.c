@@ -47,7 +47,6 @@
 
 /* $simparam("gmin"): the deck's gmin, 1e-12 S by default
  * (src/core/numerics.zig:386). */
-#define OR_SIMPARAM_GMIN 1e-12
 
 /* One compiled kind's generated tables and functions. */
 typedef struct {
@@ -103,6 +102,15 @@ int or_va_setup(int kind, const double *u, double *p)
 {
     const va_kind *K = kind_of(kind);
     return K ? K->setup(u, OR_TEMPERATURE_K, OR_SIMPARAM_GMIN, p) : -1;
+}
+
+/* The setup with $simparam("gmin") at `gmin`: the reference reruns it when
+ * the operating point's stepping rungs publish theirs
+ * (src/device/eval.zig:2596-2612 setHomotopy). */
+int or_va_setup_g(int kind, const double *u, double gmin, double *p)
+{
+    const va_kind *K = kind_of(kind);
+    return K ? K->setup(u, OR_TEMPERATURE_K, gmin, p) : -1;
 }
 
 /* Rows of one instance's unknowns: ports as given; an internal unknown in a

@@ -119,6 +119,14 @@ def load():
         "or_lead_tab": (C.c_int, [P, f64p, C.c_int, C.POINTER(LaneTab), C.POINTER(u32p), C.POINTER(C.c_size_t)]),
         "or_eval_lane": (None, [P, C.c_uint32, f64p, f64p, f64p]),
         "or_dc_sweep": (C.c_int, [P, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.POINTER(NewtonOpts), f64p, u32p]),
+        "or_newton_hom": (C.c_int, [P, C.c_uint32, C.POINTER(NewtonOpts), f64p, C.POINTER(Outcome), u32p, C.c_double]),
+        "or_lanes_set_model_gmin": (None, [P, C.c_uint32, C.c_double]),
+        "or_gmin_stamps": (None, [P, C.c_uint32, u32p]),
+        "or_gmin_factor": (C.c_double, [C.c_double, C.c_uint32, C.c_uint32]),
+        "or_gmin_stalled": (C.c_int, [f64p, C.POINTER(C.c_int), C.c_uint32, C.c_double, C.c_double]),
+        "or_gmin_replay": (C.c_uint32, [C.POINTER(C.c_int), u32p, C.c_uint32, C.c_double, C.c_uint32, f64p,
+                                        C.POINTER(C.c_int)]),
+        "or_op_ladder": (C.c_int, [P, C.POINTER(NewtonOpts), C.c_double, C.c_uint32, f64p, C.POINTER(Outcome), u32p]),
         "or_fetlim": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "or_tran_coeffs": (None, [C.c_int, C.c_double, C.c_double, C.c_double, f64p]),
         "or_tran_run": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), C.POINTER(f64p), C.POINTER(f64p),
@@ -290,6 +298,42 @@ class Lanes:
         self.lib.free_(C.cast(tp, C.c_void_p))
         self.lib.free_(C.cast(rp, C.c_void_p))
         return t, x, st
+
+    def gmin_stamps(self, l=0):
+        n = self.ckt.n
+        st, sp = u32(np.zeros(2 * n))
+        self.lib.or_gmin_stamps(self.h, l, sp)
+        return st[:n].copy(), st[n:].copy()
+
+    def newton_hom(self, l, x, gshunt, srcfact, stamps=True, gmodel=None, **kw):
+        """Lane l's Newton under a homotopy from x (n values); returns
+        (x, status, iterations)."""
+        n = self.ckt.n
+        if gmodel is not None:
+            self.lib.or_lanes_set_model_gmin(self.h, l, gmodel)
+        xa, xp = f64(np.array(x, dtype=np.float64))
+        sp = None
+        if stamps:
+            st, sp = u32(np.zeros(2 * n))
+            self.lib.or_gmin_stamps(self.h, l, sp)
+        o = newton_opts(gmin=gshunt, **kw)
+        out = Outcome()
+        assert self.lib.or_newton_hom(self.h, l, C.byref(o), xp, C.byref(out), sp, srcfact) == 0
+        return xa, out.status, out.iterations
+
+    def op_ladder(self, x0=None, gmin_start=1e-2, ladder_max_iter=100, **kw):
+        """The plain rung then the stepping rungs per failing lane; returns
+        (x (n, N), status, iterations, rung)."""
+        n, N = self.ckt.n, self.N
+        x = np.zeros(n * N) if x0 is None else np.array(x0, dtype=np.float64).reshape(-1)
+        xa, xp = f64(x)
+        out = (Outcome * N)()
+        rung, rp = u32(np.zeros(N))
+        o = newton_opts(**kw)
+        assert self.lib.or_op_ladder(self.h, C.byref(o), gmin_start, ladder_max_iter, xp, out, rp) == 0
+        status = np.array([out[i].status for i in range(N)], dtype=np.uint32)
+        iters = np.array([out[i].iterations for i in range(N)], dtype=np.uint32)
+        return xa.reshape(n, N), status, iters, rung
 
     def dc_sweep(self, vsrc, start, step, npts, **kw):
         n = self.ckt.n
