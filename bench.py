@@ -248,21 +248,14 @@ def opamp_lanes(t, inst, inputs):
     return lanes
 
 
-GMIN_LADDER = (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10, 1e-11, 1e-12, 0.0)
-
-
 def opamp_nominal(t, inst):
-    """The nominal operating point by the gmin ladder: warm Newton with a
-    shunt gmin on every row, stepped down a decade at a time to 0."""
+    """The nominal operating point by the operating-point ladder from a
+    cold start (the plain rung, then dynamic gmin stepping), one lane."""
     lanes = opamp_lanes(t, inst, np.array([[1, 1, 1, 1, 1.8, 27.0, 1, 1e-12]]))
-    x = np.zeros(t.n)
-    for g in GMIN_LADDER:
-        lanes.upload_x(x)
-        xs, st, _, _ = lanes.newton(init_fix=False, gmin=g)
-        if not st[0] & 1:
-            raise RuntimeError(f"gmin ladder failed at {g}")
-        x = xs[:, 0].copy()
-    return x
+    xs, st, _, rung, _ = lanes.op_ladder()
+    if st[0] != 1:
+        raise RuntimeError("the nominal operating point did not converge")
+    return xs[:, 0].copy()
 
 
 def opamp_passes(t):

@@ -37,7 +37,7 @@ EXPORTS = (
     "espice_mlp_gradients", "espice_mlp_predict", "espice_lanes_warm_start", "espice_lanes_newton_learned",
     "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
     "espice_lanes_tab_refactor_solve_device", "espice_lanes_template_gmin_stamps", "espice_lanes_shape",
-    "espice_lanes_newton_homotopy", "espice_lanes_op_ladder",
+    "espice_lanes_newton_homotopy", "espice_lanes_op_ladder", "espice_lanes_set_source", "espice_lanes_dc_sweep",
 )
 
 
@@ -163,10 +163,13 @@ def library():
         "espice_lanes_first_factors": [P, u32p, u32p],
         "espice_lanes_template_gmin_stamps": [P, u32p, u32p],
         "espice_lanes_shape": [P, u32p, u32p],
-        "espice_lanes_newton_homotopy": [P, C.POINTER(NewtonOpts), C.POINTER(LaneHomotopy), C.POINTER(LaneResult),
-                                         f64p],
+        "espice_lanes_newton_homotopy": [P, C.POINTER(NewtonOpts), C.POINTER(LaneHomotopy), f64p,
+                                         C.POINTER(LaneResult), f64p],
         "espice_lanes_op_ladder": [P, C.POINTER(NewtonOpts), C.POINTER(LadderOpts), C.POINTER(LaneResult), u32p,
                                    f64p],
+        "espice_lanes_set_source": [P, C.c_uint32, C.c_uint32, C.c_double],
+        "espice_lanes_dc_sweep": [P, C.c_uint32, C.c_uint32, f64p, C.c_uint32, C.POINTER(NewtonOpts),
+                                  C.POINTER(LadderOpts), f64p, u32p, f64p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -395,7 +398,7 @@ class Lanes:
         res = np.zeros((self.N, 2), dtype=np.uint32)
         ms = C.c_double()
         _check(self.lib.espice_lanes_newton_homotopy(self.h, C.byref(o), h.ctypes.data_as(C.POINTER(LaneHomotopy)),
-                                                     res.ctypes.data_as(C.POINTER(LaneResult)), C.byref(ms)))
+                                                     None, res.ctypes.data_as(C.POINTER(LaneResult)), C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ms.value
 
     def op_ladder(self, x0=None, gmin_start=1e-2, ladder_max_iter=100, **kw):
@@ -411,6 +414,20 @@ class Lanes:
         _check(self.lib.espice_lanes_op_ladder(self.h, C.byref(o), C.byref(lo),
                                                res.ctypes.data_as(C.POINTER(LaneResult)), rp, C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), rung, ms.value
+
+    def dc_sweep(self, kind, inst, values, gmin_start=1e-2, ladder_max_iter=100, **kw):
+        """A `.dc` sweep of one source's dc value over every lane; returns
+        (x (points, n, N), converged (points, N), kernel_ms)."""
+        v, vp = _f64(np.asarray(values, dtype=np.float64))
+        npts = v.size
+        o = newton_opts(**kw)
+        lo = LadderOpts(gmin_start, ladder_max_iter, 0)
+        xo, xp = _f64(np.zeros(npts * self.t.n * self.N))
+        cv, cp = _u32(np.zeros(npts * self.N))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_dc_sweep(self.h, kind, inst, vp, npts, C.byref(o), C.byref(lo), xp, cp,
+                                              C.byref(ms)))
+        return xo.reshape(npts, self.t.n, self.N), cv.reshape(npts, self.N), ms.value
 
     def first_factors(self):
         """(predicted, serial): how the last Newton launch factored each

@@ -98,18 +98,22 @@ struct LaneHom {
     double gshunt, gmodel, srcfact;
     uint32_t flags, pad;
 };
-constexpr uint32_t HOM_ACTIVE = 1u, HOM_INITFIX = 2u;
+constexpr uint32_t HOM_ACTIVE = 1u, HOM_INITFIX = 2u, HOM_PREDICT = 4u;
 static_assert(sizeof(LaneHom) == 32, "LaneHom is four 8-byte words");
 
 // A homotopy launch: per-lane controls (null: none, every lane runs as
 // the Newton options say), the gmin stamps (2n words: row i's column or
-// ESP_NONE, then its slot; null: every diagonal) and the words of a lane's
-// prepared constants that hold the compiled kinds' $simparam("gmin").
+// ESP_NONE, then its slot; null: every diagonal), the words of a lane's
+// prepared constants that hold the compiled kinds' $simparam("gmin"), and
+// for HOM_PREDICT lanes the previous point (node-major like x), against
+// which a `.dc` point's predicted start is limited before its Newton
+// (src/analysis/dc/dc.zig:478-483 prepareWarm).
 struct Homotopy {
     const LaneHom *lane;
     const uint32_t *stamp;
     const uint32_t *gslot;
     uint32_t n_gslot;
+    const double *x_prev;
 };
 
 // Everything a lane reads or writes, as lane-major planes.

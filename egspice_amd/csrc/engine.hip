@@ -183,6 +183,15 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     if (o.init_fix) {
         for (uint32_t e = tid; e < W.n_inst; e += nt) wave_seed_inst<VA>(W, s, e);
         lim_active = true;
+    } else if (hom.lane != nullptr && (hom.lane[l].flags & HOM_PREDICT)) {
+        // the predicted start limited against the previous point, the
+        // Newton then starting from the limiters' images
+        for (uint32_t i = tid; i < n; i += nt) s.x_old[i] = hom.x_prev[(size_t)i * N + l];
+        __syncthreads();
+        for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst<VA>(W, s, e, false);
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += nt) s.x_old[i] = s.x[i];
+        lim_active = true;
     }
     const ReduceCache rc = wave_reduce_cache(W, tid);
     uint32_t st = 0, it = o.max_iter;
@@ -750,6 +759,7 @@ struct espice_lanes {
     // $simparam("gmin") (found once per prepare)
     DevBuf<LaneHom> hom_lane;
     DevBuf<uint32_t> hom_stamp, hom_gslot;
+    DevBuf<double> hom_xprev;
     uint32_t n_gslot = 0;
     bool hom_ready = false;
 
@@ -1127,6 +1137,40 @@ espice_status espice_lanes_set_param(espice_lanes *L, uint32_t kind, uint32_t in
     return ESPICE_OK;
 }
 
+__global__ __launch_bounds__(256) void set_strided_kernel(double *p, size_t stride, size_t offset, double v, uint32_t N)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < N) p[(size_t)l * stride + offset] = v;
+}
+
+espice_status espice_lanes_set_source(espice_lanes *L, uint32_t kind, uint32_t instance, double value)
+{
+    if (!L) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if ((kind != K_VSRC && kind != K_ISRC) || instance >= L->t->count[kind])
+        return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    if (!(value == value)) return fail(ESPICE_INVALID_ARGUMENT, "InvalidParam");
+    const size_t N = L->N, uk = KIND_UK[kind], pk = KIND_PK[kind];
+    // the user value (a later prepare keeps it) and its prepared copy, which
+    // is the value itself once given (models/vsource.va:216-221)
+    for (size_t l = 0; l < N; l++) L->uprm[kind][((size_t)instance * uk + VSRC_DC) * N + l] = value;
+    const InstEntry *ie = reinterpret_cast<const InstEntry *>(L->wh.inst_bytes.data());
+    const dim3 grid((uint32_t)((N + 255) / 256));
+    for (uint32_t e = 0; e < L->wh.n_inst; e++) {
+        if (ie[e].kind != kind || ie[e].id != instance) continue;
+        const size_t word = ie[e].prm + SRCP_DC;
+        for (size_t l = 0; l < N; l++) L->prm_host[l * L->wh.n_prm + word] = value;
+        hipLaunchKernelGGL(set_strided_kernel, grid, dim3(256), 0, L->stream, L->prm_lanes.p, (size_t)L->wh.n_prm,
+                           word, value, (uint32_t)N);
+    }
+    if (L->prm[kind].p)
+        hipLaunchKernelGGL(set_strided_kernel, grid, dim3(256), 0, L->stream, L->prm[kind].p, (size_t)1,
+                           ((size_t)instance * pk + SRCP_DC) * N, value, (uint32_t)N);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return ESPICE_OK;
+}
+
 // A compiled kind's setup on every lane and instance, on the host's
 // threads, straight into the lane-contiguous constants. Every lane must
 // keep the template's collapse (its internal unknowns' rows), as a
@@ -1373,15 +1417,24 @@ static espice_status homotopy_tables(espice_lanes *L)
 }
 
 espice_status espice_lanes_newton_homotopy(espice_lanes *L, const espice_newton_opts *opts,
-                                           const espice_lane_homotopy *lanes, espice_lane_result *out,
-                                           double *kernel_ms)
+                                           const espice_lane_homotopy *lanes, const double *x_prev,
+                                           espice_lane_result *out, double *kernel_ms)
 {
     static_assert(sizeof(espice_lane_homotopy) == sizeof(LaneHom), "the C-ABI control is LaneHom");
     if (!L || !opts || !lanes || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    bool predict = false;
+    for (size_t l = 0; l < L->N; l++) predict = predict || (lanes[l].flags & ESPICE_HOM_PREDICT);
+    if (predict && !x_prev) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
     if (espice_status e = homotopy_tables(L)) return e;
     HIP_TRY(hipMemcpyAsync(L->hom_lane.p, lanes, sizeof(LaneHom) * L->N, hipMemcpyHostToDevice, L->stream));
-    const Homotopy hom{L->hom_lane.p, L->hom_stamp.p, L->hom_gslot.p, L->n_gslot};
+    if (predict) {
+        HIP_TRY(L->hom_xprev.alloc((size_t)L->t->n * L->N));
+        HIP_TRY(hipMemcpyAsync(L->hom_xprev.p, x_prev, sizeof(double) * L->t->n * L->N, hipMemcpyHostToDevice,
+                               L->stream));
+    }
+    const Homotopy hom{L->hom_lane.p, L->hom_stamp.p, L->hom_gslot.p, L->n_gslot,
+                       predict ? L->hom_xprev.p : nullptr};
     if (espice_status e = launch_newton(L, opts, false, hom)) return e;
     if (espice_status e = finish_newton(L, out, nullptr, kernel_ms)) return e;
     for (size_t l = 0; l < L->N; l++)

@@ -279,6 +279,73 @@ private:
     std::vector<LaneLadder> lanes_;
 };
 
+
+// Rounds of the ladder until every lane the driver started is done: each
+// round uploads the host's x, runs every climbing lane's next solve (one
+// launch per Newton budget the round needs: the stepping solves' and the
+// plain one's) and hands each lane its outcome.
+espice_status climb(espice_lanes *L, const espice_newton_opts *opts, const espice_ladder_opts *lo, Driver &d,
+                    uint32_t N, double &ms_all)
+{
+    std::vector<espice_lane_homotopy> hom(N), part(N);
+    std::vector<espice_lane_result> res(N), r(N);
+    std::vector<uint32_t> budget(N);
+    bool any = false;
+    for (uint32_t l = 0; l < N; l++) any = any || d.lanes()[l].ph != DONE;
+    while (any) {
+        for (uint32_t l = 0; l < N; l++) hom[l] = d.request(l, budget[l]);
+        if (espice_status e = espice_lanes_upload_x(L, d.x().data())) return e;
+        const uint32_t budgets[2] = {lo->max_iter, opts->max_iter};
+        const int n_budgets = lo->max_iter == opts->max_iter ? 1 : 2;
+        for (int b = 0; b < n_budgets; b++) {
+            bool some = false;
+            for (uint32_t l = 0; l < N; l++) {
+                part[l] = hom[l];
+                if (n_budgets == 2 && budget[l] != budgets[b]) part[l].flags = 0;
+                some = some || (part[l].flags & ESPICE_HOM_ACTIVE);
+            }
+            if (!some) continue;
+            espice_newton_opts o = *opts;
+            o.max_iter = budgets[b];
+            o.gmin = 0.0;
+            o.init_fix = 0;
+            double ms = 0.0;
+            if (espice_status e = espice_lanes_newton_homotopy(L, &o, part.data(), nullptr, r.data(), &ms)) return e;
+            ms_all += ms;
+            for (uint32_t l = 0; l < N; l++)
+                if (part[l].flags & ESPICE_HOM_ACTIVE) res[l] = r[l];
+        }
+        // the lanes that ran leave their x on the device; the rest keep
+        // the host's copy, which upload_x wrote there unchanged
+        if (espice_status e = espice_lanes_download_x(L, d.x().data())) return e;
+        any = false;
+        for (uint32_t l = 0; l < N; l++) {
+            if (!(hom[l].flags & ESPICE_HOM_ACTIVE)) continue;
+            d.respond(l, res[l]);
+            any = any || d.lanes()[l].ph != DONE;
+        }
+    }
+    return ESPICE_OK;
+}
+
+// The plain rung's outcome of lane l: converged lanes are done (rung 1),
+// the others start the gmin descent.
+void after_plain(Driver &d, uint32_t l, const espice_lane_result &r)
+{
+    LaneLadder &s = d.lanes()[l];
+    const bool singular = (r.status & 2u) != 0;
+    s.total = singular ? 0 : std::min<uint32_t>(r.iterations, 65535u);
+    s.steps.clear();
+    if ((r.status & 1u) && !singular) {
+        s.converged = true;
+        s.rung = 1;
+        s.ph = DONE;
+        return;
+    }
+    s.converged = false;
+    d.start_gmin(l);
+}
+
 }  // namespace
 
 extern "C" espice_status espice_lanes_op_ladder(espice_lanes *L, const espice_newton_opts *opts,
@@ -295,61 +362,106 @@ extern "C" espice_status espice_lanes_op_ladder(espice_lanes *L, const espice_ne
     ms_all += ms;
     Driver d(n, N, lo->gmin_start, lo->max_iter, opts->max_iter);
     if (espice_status e = espice_lanes_download_x(L, d.x().data())) return e;
-    bool any = false;
-    for (uint32_t l = 0; l < N; l++) {
-        LaneLadder &s = d.lanes()[l];
-        const bool singular = (out[l].status & 2u) != 0;
-        s.total = singular ? 0 : std::min<uint32_t>(out[l].iterations, 65535u);
-        if ((out[l].status & 1u) && !singular) {
-            s.converged = true;
-            s.rung = 1;
-            continue;
-        }
-        d.start_gmin(l);
-        any = true;
-    }
-    std::vector<espice_lane_homotopy> hom(N);
-    std::vector<espice_lane_result> res(N);
-    std::vector<uint32_t> budget(N);
-    while (any) {
-        // one round: every climbing lane's next solve, one launch per Newton
-        // budget the round needs (the stepping rungs' and the plain one's)
-        for (uint32_t l = 0; l < N; l++) hom[l] = d.request(l, budget[l]);
-        if (espice_status e = espice_lanes_upload_x(L, d.x().data())) return e;
-        uint32_t budgets[2] = {lo->max_iter, opts->max_iter};
-        const int n_budgets = lo->max_iter == opts->max_iter ? 1 : 2;
-        for (int b = 0; b < n_budgets; b++) {
-            std::vector<espice_lane_homotopy> part(hom);
-            bool some = false;
-            for (uint32_t l = 0; l < N; l++) {
-                if (n_budgets == 2 && budget[l] != budgets[b]) part[l].flags = 0;
-                some = some || (part[l].flags & ESPICE_HOM_ACTIVE);
-            }
-            if (!some) continue;
-            espice_newton_opts o = *opts;
-            o.max_iter = budgets[b];
-            o.gmin = 0.0;
-            o.init_fix = 0;
-            std::vector<espice_lane_result> r(N);
-            if (espice_status e = espice_lanes_newton_homotopy(L, &o, part.data(), r.data(), &ms)) return e;
-            ms_all += ms;
-            for (uint32_t l = 0; l < N; l++)
-                if (part[l].flags & ESPICE_HOM_ACTIVE) res[l] = r[l];
-        }
-        // the lanes that ran leave their x on the device; the rest keep
-        // the host's copy, which upload_x wrote there unchanged
-        if (espice_status e = espice_lanes_download_x(L, d.x().data())) return e;
-        any = false;
-        for (uint32_t l = 0; l < N; l++) {
-            if (!(hom[l].flags & ESPICE_HOM_ACTIVE)) continue;
-            d.respond(l, res[l]);
-            any = any || d.lanes()[l].ph != DONE;
-        }
-    }
+    for (uint32_t l = 0; l < N; l++) after_plain(d, l, out[l]);
+    if (espice_status e = climb(L, opts, lo, d, N, ms_all)) return e;
     for (uint32_t l = 0; l < N; l++) {
         const LaneLadder &s = d.lanes()[l];
         out[l] = espice_lane_result{s.converged ? 1u : 0u, s.total};
         if (rung) rung[l] = s.converged ? s.rung : 0;
+    }
+    if (kernel_ms) *kernel_ms = ms_all;
+    return ESPICE_OK;
+}
+
+extern "C" espice_status espice_lanes_dc_sweep(espice_lanes *L, uint32_t kind, uint32_t instance,
+                                               const double *values, uint32_t n_points,
+                                               const espice_newton_opts *opts, const espice_ladder_opts *lo,
+                                               double *x_out, uint32_t *converged, double *kernel_ms)
+{
+    if (!L || !values || !opts || !lo || !x_out || !converged) return ESPICE_INVALID_ARGUMENT;
+    if (!(lo->gmin_start > 0.0) || lo->max_iter == 0 || opts->max_iter == 0) return ESPICE_INVALID_ARGUMENT;
+    uint32_t n = 0, N = 0;
+    if (espice_status e = espice_lanes_shape(L, &n, &N)) return e;
+    const size_t nx = (size_t)n * N;
+    double ms_all = 0.0;
+    Driver d(n, N, lo->gmin_start, lo->max_iter, opts->max_iter);
+    // sol (d.x) and the previous point per lane, both node-major
+    std::vector<double> &x = d.x();
+    std::vector<double> prev(nx, 0.0);
+    std::vector<uint8_t> cold(N, 1);
+    std::vector<espice_lane_homotopy> hom(N);
+    std::vector<espice_lane_result> r(N);
+    for (uint32_t pt = 0; pt < n_points; pt++) {
+        if (espice_status e = espice_lanes_set_source(L, kind, instance, values[pt])) return e;
+        // MODEINITPRED (dc.zig:463-475): the last two points extrapolated,
+        // the rotation running on cold points too
+        for (size_t k = 0; k < nx; k++) {
+            const double last = x[k];
+            x[k] = 2.0 * last - prev[k];
+            prev[k] = last;
+        }
+        // the warm lanes' Newton from the predictor limited against the last
+        // point (prepareWarm), with the `.dc` budget (warmOptions: itl2)
+        bool any_warm = false;
+        for (uint32_t l = 0; l < N; l++) {
+            hom[l] = espice_lane_homotopy{0.0, GMIN_DECK, 1.0, 0u, 0u};
+            if (!cold[l]) {
+                hom[l].flags = ESPICE_HOM_ACTIVE | ESPICE_HOM_PREDICT;
+                any_warm = true;
+            }
+        }
+        std::vector<uint8_t> done(N, 0);
+        if (any_warm) {
+            if (espice_status e = espice_lanes_upload_x(L, x.data())) return e;
+            espice_newton_opts o = *opts;
+            o.max_iter = lo->max_iter;
+            o.gmin = 0.0;
+            o.init_fix = 0;
+            double ms = 0.0;
+            if (espice_status e = espice_lanes_newton_homotopy(L, &o, hom.data(), prev.data(), r.data(), &ms))
+                return e;
+            ms_all += ms;
+            if (espice_status e = espice_lanes_download_x(L, x.data())) return e;
+            for (uint32_t l = 0; l < N; l++)
+                if (!cold[l] && (r[l].status & 1u) && !(r[l].status & 2u)) done[l] = 1;
+        }
+        // every other lane: the cold ladder from zero (endPoint's
+        // coldStart + nodesetLadder), its plain rung one homotopy launch
+        bool any_cold = false;
+        for (uint32_t l = 0; l < N; l++) {
+            hom[l] = espice_lane_homotopy{0.0, GMIN_DECK, 1.0, 0u, 0u};
+            d.lanes()[l].ph = DONE;
+            if (done[l]) continue;
+            for (uint32_t i = 0; i < n; i++) x[(size_t)i * N + l] = 0.0;
+            hom[l].flags = ESPICE_HOM_ACTIVE | ESPICE_HOM_COLD;
+            any_cold = true;
+        }
+        if (any_cold) {
+            if (espice_status e = espice_lanes_upload_x(L, x.data())) return e;
+            espice_newton_opts o = *opts;
+            o.gmin = 0.0;
+            o.init_fix = 0;
+            double ms = 0.0;
+            if (espice_status e = espice_lanes_newton_homotopy(L, &o, hom.data(), nullptr, r.data(), &ms)) return e;
+            ms_all += ms;
+            if (espice_status e = espice_lanes_download_x(L, x.data())) return e;
+            for (uint32_t l = 0; l < N; l++)
+                if (!done[l]) after_plain(d, l, r[l]);
+            if (espice_status e = climb(L, opts, lo, d, N, ms_all)) return e;
+            for (uint32_t l = 0; l < N; l++)
+                if (!done[l] && d.lanes()[l].converged) done[l] = 1;
+        }
+        // endPoint: a converged point is recorded and the next one warm;
+        // the first point's solution is also the predictor's previous one
+        // (dctrcurv.c:457-462, dc.zig:524-527)
+        for (uint32_t l = 0; l < N; l++) {
+            converged[(size_t)pt * N + l] = done[l];
+            for (uint32_t i = 0; i < n; i++)
+                x_out[((size_t)pt * n + i) * N + l] = done[l] ? x[(size_t)i * N + l] : NAN;
+            cold[l] = done[l] ? 0 : 1;
+            if (done[l] && pt == 0)
+                for (uint32_t i = 0; i < n; i++) prev[(size_t)i * N + l] = x[(size_t)i * N + l];
+        }
     }
     if (kernel_ms) *kernel_ms = ms_all;
     return ESPICE_OK;

@@ -297,10 +297,14 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             for (uint32_t l = 0; l < N; l++)
                 for (uint32_t i = 0; i < n; i++) x0[(size_t)i * N + l] = P.x_nominal[i];
         if (espice_lanes_upload_x(L, x0.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Upload");
+        // the plain rung on every lane, the stepping rungs for the lanes it
+        // leaves (op.zig:626-688 solveGpuLanes)
         espice_newton_opts o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, warm ? 0u : 1u};
+        const espice_ladder_opts lo{1e-2, 100, 0};
         std::vector<espice_lane_result> r(N);
         double ms;
-        if (espice_lanes_newton_op(L, &o, r.data(), &ms) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "NewtonOp");
+        if (espice_lanes_op_ladder(L, &o, &lo, r.data(), nullptr, &ms) != ESPICE_OK)
+            return fail(&P, ESPICE_FAILED, "NewtonOp");
         xop.resize((size_t)n * N);
         if (espice_lanes_download_x(L, xop.data()) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "Download");
         have_op = true;
@@ -309,7 +313,7 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             if (pt.failed) continue;
             if (r[l].status != 1) {
                 pt.failed = true;
-                pt.reason = r[l].status == 2 ? ESPICE_REASON_SINGULAR_MATRIX : ESPICE_REASON_NO_CONVERGENCE;
+                pt.reason = ESPICE_REASON_NO_CONVERGENCE;
                 continue;
             }
             pt.x.resize(n);
@@ -351,17 +355,19 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
                 o.abort_mask |= 1ull << (c.first - 1);
         }
         // warm_start: every point's operating point from the template's
-        // nominal one (src/analysis/dc/op.zig:588, 671), solved cold once
+        // nominal one (src/analysis/dc/op.zig:588, 671), solved cold once by
+        // the ladder
         if (po.warm_start) {
             if (P.x_tran_nominal.empty()) {
                 espice_lanes *L0 = nullptr;
                 if (espice_lanes_create(P.tpl, 1, &L0) != ESPICE_OK) return fail(&P, ESPICE_FAILED, "GpuUnavailable");
                 std::vector<double> z(n, 0.0), x1(n);
                 espice_newton_opts no{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1u};
+                const espice_ladder_opts lo{1e-2, 100, 0};
                 espice_lane_result r1{};
                 double ms0;
                 bool ok = espice_lanes_prepare(L0) == ESPICE_OK && espice_lanes_upload_x(L0, z.data()) == ESPICE_OK &&
-                          espice_lanes_newton_op(L0, &no, &r1, &ms0) == ESPICE_OK && r1.status == 1 &&
+                          espice_lanes_op_ladder(L0, &no, &lo, &r1, nullptr, &ms0) == ESPICE_OK && r1.status == 1 &&
                           espice_lanes_download_x(L0, x1.data()) == ESPICE_OK;
                 espice_lanes_destroy(L0);
                 P.x_tran_nominal = ok ? x1 : std::vector<double>(n, NAN);

@@ -137,6 +137,11 @@ espice_status espice_lanes_set_param(espice_lanes *l, uint32_t kind, uint32_t in
 /* Temperature and geometry preparation of every instance on every lane,
  * then the upload. Required after set_param and before a solve. */
 espice_status espice_lanes_prepare(espice_lanes *l);
+/* A voltage or current source's dc value on every lane of a prepared
+ * batch at once, no prepare needed: a `.dc` sweep's per-point install
+ * (ParamRef.set and recomputeType of the swept source,
+ * src/analysis/dc/dc.zig:442-457). */
+espice_status espice_lanes_set_source(espice_lanes *l, uint32_t kind, uint32_t instance, double value);
 /* x[n * n_lanes] lane-major: each lane's start point. */
 espice_status espice_lanes_upload_x(espice_lanes *l, const double *x);
 espice_status espice_lanes_download_x(espice_lanes *l, double *x);
@@ -169,18 +174,22 @@ typedef struct {
     double gshunt, gmodel, srcfact;
     uint32_t flags, pad;
 } espice_lane_homotopy;
-enum { ESPICE_HOM_ACTIVE = 1, ESPICE_HOM_COLD = 2 };
+enum { ESPICE_HOM_ACTIVE = 1, ESPICE_HOM_COLD = 2, ESPICE_HOM_PREDICT = 4 };
 /* The static Newton of the lanes whose control is active, each under its
  * homotopy, from its x (espice_lanes_upload_x); the others are untouched
  * and their out entries read {0, 0}. opts gives the tolerances and
- * max_iter; its gmin and init_fix are the controls'. A compiled kind's
+ * max_iter; its gmin and init_fix are the controls'. A warm lane with
+ * ESPICE_HOM_PREDICT first limits its start against x_prev (n * n_lanes,
+ * node-major like x; required when any lane predicts), as a `.dc` point's
+ * predictor is limited against the previous point
+ * (src/analysis/dc/dc.zig:478-483 prepareWarm). A compiled kind's
  * gmodel lands in the prepared constants that hold its $simparam("gmin")
  * (found by rerunning its setup at a second gmin: HomotopyGminDerived when
  * a constant derives from it otherwise) and stays there: a lane's last
  * homotopy solve leaves its model gmin. HomotopyLaneTooLarge when the
  * lanes do not run workgroup-per-lane. */
 espice_status espice_lanes_newton_homotopy(espice_lanes *l, const espice_newton_opts *opts,
-    const espice_lane_homotopy *lanes, espice_lane_result *out, double *kernel_ms);
+    const espice_lane_homotopy *lanes, const double *x_prev, espice_lane_result *out, double *kernel_ms);
 
 /* The operating-point ladder's options: gmin_start (1e-2 S, the gmin
  * descent's first good gmin, src/core/numerics.zig:390) and the Newton
@@ -204,6 +213,20 @@ typedef struct {
  * are not built). Each lane's x is left on the device. */
 espice_status espice_lanes_op_ladder(espice_lanes *l, const espice_newton_opts *opts, const espice_ladder_opts *lo,
     espice_lane_result *out, uint32_t *rung, double *kernel_ms);
+/* A `.dc` sweep of one source's dc value over every lane
+ * (src/analysis/dc/dc.zig:161-293 runGpuLanes with its Sweep :303-530):
+ * point by point the source takes values[pt] on every lane; each lane that
+ * converged at the last point starts from the last two points extrapolated
+ * (MODEINITPRED; the first point's solution also stands for the one before
+ * it), limited against the last point, with a Newton of lo->max_iter
+ * iterations (warmOptions' itl2); a lane that fails it, or has no converged
+ * last point, solves the point cold by the whole ladder (the plain rung at
+ * opts->max_iter, then the stepping rungs). x_out[n_points][n][n_lanes]
+ * (NaN where the point did not converge), converged[n_points][n_lanes].
+ * The source keeps the last value. */
+espice_status espice_lanes_dc_sweep(espice_lanes *l, uint32_t kind, uint32_t instance, const double *values,
+    uint32_t n_points, const espice_newton_opts *opts, const espice_ladder_opts *lo, double *x_out,
+    uint32_t *converged, double *kernel_ms);
 
 /* ---- the lead-tape lane LU (lane_device.zig refactor :101, solveNeg :170)
  * Builds a LaneTab from one factor of (col_ptr, row_idx, vals) on the host

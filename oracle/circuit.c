@@ -934,6 +934,14 @@ int or_newton_op(or_lanes *L, int mode, const or_newton_opts *o, double *x, or_o
 int or_newton_hom(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out, const uint32_t *stamp,
                   double srcfact)
 {
+    return or_newton_hom_lim(L, l, o, xl, out, stamp, srcfact, o->init_fix != 0);
+}
+
+/* The same with the limiters' images live from the start when
+ * lim_active (a `.dc` point's predictor limited against the last point). */
+int or_newton_hom_lim(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out,
+                      const uint32_t *stamp, double srcfact, int lim_active)
+{
     const or_circuit *c = L->c;
     const uint32_t n = c->n;
     uint32_t *q = malloc(sizeof(uint32_t) * (n ? n : 1));
@@ -945,7 +953,7 @@ int or_newton_hom(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, 
     free(q);
     if (!s) return -1;
     if (o->init_fix) or_seed_pass(L, l, xl);
-    newton_one(L, l, o, xl, out, lin_direct, s, o->init_fix, stamp, &srcfact);
+    newton_one(L, l, o, xl, out, lin_direct, s, lim_active, stamp, &srcfact);
     or_solver_free(s);
     return 0;
 }

@@ -352,3 +352,71 @@ int or_op_ladder(or_lanes *L, const or_newton_opts *plain, double gmin_start, ui
     free(stamp);
     return 0;
 }
+
+/* ------------------------------------------------------------- dc sweep */
+
+int or_dc_sweep_lanes(or_lanes *L, int kind, uint32_t inst, const double *values, uint32_t npts,
+                      const or_newton_opts *plain, double gmin_start, uint32_t budget, double *x_out,
+                      uint32_t *converged)
+{
+    const or_circuit *c = L->c;
+    const uint32_t n = or_ckt_n(c), N = L->N;
+    const int pk = or_prepared_count(kind);
+    double *sol = malloc(8 * (n ? n : 1)), *prev = malloc(8 * (n ? n : 1));
+    uint32_t *stamp = malloc(8 * (n ? n : 1));
+    for (uint32_t l = 0; l < N; l++) {
+        memset(sol, 0, 8 * n);
+        memset(prev, 0, 8 * n);
+        or_gmin_stamps(L, l, stamp);
+        lane_ctx k = {L, l, n, budget, plain, stamp, OR_SIMPARAM_GMIN, 0};
+        int cold = 1;
+        for (uint32_t pt = 0; pt < npts; pt++) {
+            /* the swept value, user and prepared (the dc value once given) */
+            L->uprm[kind][((size_t)inst * (kind == OR_VSRC ? OR_VSRC_K : OR_ISRC_K) + OR_VSRC_DC) * N + l] = values[pt];
+            L->pprm[kind][((size_t)inst * pk + SRCP_DC) * N + l] = values[pt];
+            /* MODEINITPRED: the last two points extrapolated */
+            for (uint32_t i = 0; i < n; i++) {
+                const double last = sol[i];
+                sol[i] = 2.0 * last - prev[i];
+                prev[i] = last;
+            }
+            int done = 0;
+            or_outcome r;
+            if (!cold) {
+                /* prepareWarm, then the warm Newton (warmOptions: itl2) */
+                or_limit_pass(L, l, sol, prev, 0);
+                or_newton_opts w = *plain;
+                w.max_iter = budget;
+                w.gmin = 0.0;
+                w.init_fix = 0;
+                or_newton_hom_lim(L, l, &w, sol, &r, NULL, 1.0, 1);
+                done = (r.status & 1u) && !(r.status & 2u);
+            }
+            if (!done) {
+                /* coldStart + the whole ladder */
+                memset(sol, 0, 8 * n);
+                or_newton_opts p0 = *plain;
+                p0.gmin = 0.0;
+                p0.init_fix = 1;
+                or_newton_hom(L, l, &p0, sol, &r, NULL, 1.0);
+                done = (r.status & 1u) && !(r.status & 2u);
+                if (!done) {
+                    done = after_plain(&k, sol, gmin_start) != 0;
+                    if (k.model_gmin != OR_SIMPARAM_GMIN) {
+                        or_lanes_set_model_gmin(L, l, OR_SIMPARAM_GMIN);
+                        k.model_gmin = OR_SIMPARAM_GMIN;
+                    }
+                }
+            }
+            converged[(size_t)pt * N + l] = (uint32_t)done;
+            for (uint32_t i = 0; i < n; i++) x_out[((size_t)pt * n + i) * N + l] = done ? sol[i] : NAN;
+            cold = !done;
+            /* the first point's solution stands for the one before it too */
+            if (done && pt == 0) memcpy(prev, sol, 8 * n);
+        }
+    }
+    free(sol);
+    free(prev);
+    free(stamp);
+    return 0;
+}

@@ -154,6 +154,15 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
  * o->init_fix; the lane's ordered direct solver. */
 int or_newton_hom(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out, const uint32_t *stamp,
                   double srcfact);
+int or_newton_hom_lim(or_lanes *L, uint32_t l, const or_newton_opts *o, double *xl, or_outcome *out,
+                      const uint32_t *stamp, double srcfact, int lim_active);
+/* A `.dc` sweep of source (kind, inst)'s dc value over every lane, one lane
+ * at a time (dc.zig:161-530): warm from the predictor limited against the
+ * last point, else the cold ladder. x_out[npts][n][N] (NaN: no
+ * convergence), converged[npts][N]. */
+int or_dc_sweep_lanes(or_lanes *L, int kind, uint32_t inst, const double *values, uint32_t npts,
+                      const or_newton_opts *plain, double gmin_start, uint32_t budget, double *x_out,
+                      uint32_t *converged);
 /* Lane l's compiled instances set up at $simparam("gmin") = gmin. */
 void or_lanes_set_model_gmin(or_lanes *L, uint32_t l, double gmin);
 /* gminStamps from lane l's planes at x = 0 (2n words). */

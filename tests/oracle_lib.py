@@ -127,6 +127,8 @@ def load():
         "or_gmin_replay": (C.c_uint32, [C.POINTER(C.c_int), u32p, C.c_uint32, C.c_double, C.c_uint32, f64p,
                                         C.POINTER(C.c_int)]),
         "or_op_ladder": (C.c_int, [P, C.POINTER(NewtonOpts), C.c_double, C.c_uint32, f64p, C.POINTER(Outcome), u32p]),
+        "or_dc_sweep_lanes": (C.c_int, [P, C.c_int, C.c_uint32, f64p, C.c_uint32, C.POINTER(NewtonOpts), C.c_double,
+                                        C.c_uint32, f64p, u32p]),
         "or_fetlim": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "or_tran_coeffs": (None, [C.c_int, C.c_double, C.c_double, C.c_double, f64p]),
         "or_tran_run": (C.c_int, [P, C.c_uint32, C.POINTER(TranOpts), C.POINTER(f64p), C.POINTER(f64p),
@@ -334,6 +336,18 @@ class Lanes:
         status = np.array([out[i].status for i in range(N)], dtype=np.uint32)
         iters = np.array([out[i].iterations for i in range(N)], dtype=np.uint32)
         return xa.reshape(n, N), status, iters, rung
+
+    def dc_sweep_lanes(self, kind, inst, values, gmin_start=1e-2, ladder_max_iter=100, **kw):
+        """The `.dc` sweep over every lane: (x (points, n, N), converged
+        (points, N))."""
+        n, N = self.ckt.n, self.N
+        v, vp = f64(np.asarray(values, dtype=np.float64))
+        xo, xp = f64(np.zeros(v.size * n * N))
+        cv, cp = u32(np.zeros(v.size * N))
+        o = newton_opts(**kw)
+        assert self.lib.or_dc_sweep_lanes(self.h, kind, inst, vp, v.size, C.byref(o), gmin_start, ladder_max_iter,
+                                          xp, cp) == 0
+        return xo.reshape(v.size, n, N), cv.reshape(v.size, N)
 
     def dc_sweep(self, vsrc, start, step, npts, **kw):
         n = self.ckt.n

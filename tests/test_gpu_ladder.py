@@ -131,3 +131,67 @@ def test_ladder_opamp_bitwise(oracle, engine):
     xg2, sg2, ig2, _ = g.newton(init_fix=False)
     xo2, so2, io2 = o.newton_op(mode=1, x0=xo.reshape(-1), init_fix=False)
     assert np.array_equal(sg2, so2) and np.array_equal(ig2, io2) and same_bits(xg2, xo2)
+
+
+def sweep_values(start, step, npts):
+    v, out = start, []
+    for k in range(npts):
+        if k:
+            v += step
+        out.append(v)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("warm_iter", [100, 2])
+def test_dc_sweep_chain_bitwise(oracle, engine, warm_iter):
+    """The `.dc` sweep over 64 Monte Carlo lanes of the inverter chain (VTO
+    spread), 0-1.8 V in 19 points: warm from the limited predictor, and
+    with a 2-iteration warm budget through the cold ladder at most points;
+    x and convergence bitwise the oracle's."""
+    N = 64
+    say(f"case: .dc sweep, inverter chain, {N} lanes, 19 points, warm budget {warm_iter}")
+    vin = sweep_values(0.0, 0.1, 19)
+    vto = np.linspace(0.6, 0.8, N)
+    t = E.Template()
+    W.inverter_chain(t.add)
+    t.freeze()
+    g = E.Lanes(t, N)
+    c = O.Circuit(oracle)
+    W.inverter_chain(c.add)
+    c.freeze()
+    o = O.Lanes(c, N)
+    for i in range(0, 10, 2):
+        g.set_param(E.MOS1, i, E.MOS1_PARAMS["VTO"], vto)
+        o.set_param(O.MOS1, i, O.M1["VTO"], vto)
+    g.prepare()
+    o.prepare()
+    xg, cg, ms = g.dc_sweep(E.VSOURCE, 1, vin, ladder_max_iter=warm_iter)
+    xo, co = o.dc_sweep_lanes(O.VSRC, 1, vin, ladder_max_iter=warm_iter)
+    say(f"  converged {int(cg.sum())}/{cg.size}, kernels {ms:.3f} ms")
+    assert co.all()
+    assert np.array_equal(cg, co)
+    assert same_bits(xg, xo)
+
+
+def test_dc_sweep_diode_golden(oracle, engine):
+    """docs/examples/diode_iv.sp on 4 lanes: the reference's printed i(v1)
+    at 10 and 20 mV (docs/learn/analyses.md:88-93) and bitwise the oracle."""
+    say("case: .dc sweep, diode_iv.sp, 4 lanes, 81 points")
+    vin = sweep_values(0.0, 0.01, 81)
+    t = E.Template()
+    t.add(E.VSOURCE, ["a", "0"], {E.VSOURCE_DC: 0.0})
+    t.add(E.DIODE, ["a", "0"], {E.DIODE_PARAMS["IS"]: 1e-14, E.DIODE_PARAMS["N"]: 1.0})
+    t.freeze()
+    g = E.Lanes(t, 4)
+    g.prepare()
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["a", "0"], {O.VSRC_DC: 0.0})
+    c.add(O.DIODE, ["a", "0"], {O.DIO["IS"]: 1e-14, O.DIO["N"]: 1.0})
+    c.freeze()
+    o = O.Lanes(c, 4)
+    o.prepare()
+    xg, cg, _ = g.dc_sweep(E.VSOURCE, 0, vin)
+    xo, co = o.dc_sweep_lanes(O.VSRC, 0, vin)
+    assert cg.all() and np.array_equal(cg, co) and same_bits(xg, xo)
+    for k, v in {1: -1.386624089968383e-14, 2: -3.166792969038269e-14}.items():
+        assert (np.abs(xg[k, 2] / v - 1) <= 4e-16).all()

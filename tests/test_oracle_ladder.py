@@ -220,3 +220,77 @@ def test_source_stepping_and_failures(oracle):
     ok = st == 1
     assert np.abs(x[:, ok] - xr[:, ok]).max() <= 1e-5
     kcl_ok(L, x, np.nonzero(ok)[0])
+
+
+def sweep_values(start, step, npts):
+    """The swept values accumulated as the reference steps them
+    (dc.zig:373-379: += step, not start + k * step)."""
+    v, out = start, []
+    for k in range(npts):
+        if k:
+            v += step
+        out.append(v)
+    return np.array(out)
+
+
+def test_dc_sweep_lanes_diode_golden(oracle):
+    """docs/examples/diode_iv.sp's printed rows (docs/learn/analyses.md:
+    88-93) on every lane of a `.dc` sweep over lanes: i(v1) at 10 and 20 mV
+    to 4e-16."""
+    c = O.Circuit(oracle)
+    c.add(O.VSRC, ["a", "0"], {O.VSRC_DC: 0.0})
+    c.add(O.DIODE, ["a", "0"], {O.DIO["IS"]: 1e-14, O.DIO["N"]: 1.0})
+    c.freeze()
+    L = O.Lanes(c, 3)
+    L.prepare()
+    x, conv = L.dc_sweep_lanes(O.VSRC, 0, sweep_values(0.0, 0.01, 81))
+    assert conv.all()
+    golden = {1: -1.386624089968383e-14, 2: -3.166792969038269e-14}
+    for k, v in golden.items():
+        assert (np.abs(x[k, 2] / v - 1) <= 4e-16).all()
+        assert (np.abs(x[k, 1] - 0.01 * k) <= 1e-17).all()
+
+
+def test_dc_sweep_lanes_chain(oracle):
+    """The inverter chain swept 0-1.8 V on 6 Monte Carlo lanes (VTO spread):
+    every point converges, each lane's curve is its own lane swept alone,
+    and each point equals that point's cold operating point within the
+    Newton tolerance (reltol 1e-3 of the node voltage, plus 1e-5);
+    with the warm budget cut to 2 iterations the warm solves fail and the
+    cold ladder still finds every point."""
+    vin = sweep_values(0.0, 0.1, 19)
+    vto = np.array([0.65, 0.68, 0.7, 0.72, 0.75, 0.7])
+
+    def lanes(n_lanes, vt):
+        c = O.Circuit(oracle)
+        W.inverter_chain(c.add)
+        c.freeze()
+        L = O.Lanes(c, n_lanes)
+        for i in range(0, 10, 2):
+            L.set_param(O.MOS1, i, O.M1["VTO"], vt)
+        L.prepare()
+        return c, L
+
+    c, L = lanes(6, vto)
+    x, conv = L.dc_sweep_lanes(O.VSRC, 1, vin)
+    assert conv.all()
+    c1, L1 = lanes(1, vto[3:4])
+    x1, conv1 = L1.dc_sweep_lanes(O.VSRC, 1, vin)
+    assert np.array_equal(x[:, :, 3].view(np.uint64), x1[:, :, 0].view(np.uint64))
+    for k in (0, 7, 9, 18):
+        cc, Lc = lanes(6, vto)
+        Lc.set_param(O.VSRC, 1, O.VSRC_DC, np.full(6, vin[k]))
+        Lc.prepare()
+        xc, sc, _ = Lc.newton_op(mode=1)
+        assert (sc & 1).all()
+        assert within_tol(x[k], xc), k
+    c, L = lanes(6, vto)
+    x2, conv2 = L.dc_sweep_lanes(O.VSRC, 1, vin, ladder_max_iter=2)
+    assert conv2.all()
+    assert within_tol(x2, x)
+
+
+def within_tol(a, b):
+    """Two Newton answers of the same equations from different starts:
+    within reltol (1e-3) of the larger magnitude, plus 1e-5."""
+    return bool(np.all(np.abs(a - b) <= 1e-3 * np.maximum(np.abs(a), np.abs(b)) + 1e-5))
