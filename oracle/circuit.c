@@ -994,6 +994,9 @@ int or_dc_sweep(or_lanes *L, uint32_t vsrc, double start, double step, uint32_t 
         }
         converged[pt] = oc.status & 1;
         memcpy(x_out + (size_t)pt * n, sol, 8 * n);
+        /* the first point's solution stands for the one before it too
+         * (dctrcurv.c:457-462, dc.zig:524-527) */
+        if (pt == 0 && (oc.status & 1)) memcpy(prev, sol, 8 * n);
     }
     or_solver_free(s);
     free(sol);
