@@ -258,6 +258,27 @@ def opamp_nominal(t, inst):
     return xs[:, 0].copy()
 
 
+def mixed_summary(op_m, op_n, node_rows):
+    """The mixed-precision operating points (FP32 factor and solves, FP64
+    refinement) against the FP64 lanes from the same start: convergence,
+    ticks, FP32 solves and FP64 fallbacks per lane, and the error per lane
+    (the largest node-voltage difference; its maximum and median)."""
+    x_m, st_m, it_m, solves, fallbacks, ms = op_m
+    x_n, st_n, it_n = op_n[0], op_n[1], op_n[2]
+    both = ((st_m & 1) != 0) & ((st_n & 1) != 0)
+    err = np.abs(x_m - x_n)[node_rows][:, both].max(axis=0) if both.any() else np.zeros(0)
+    rel = (np.abs(x_m - x_n)[node_rows][:, both] / np.maximum(np.abs(x_n[node_rows][:, both]), 1e-30)).max(axis=0) \
+        if both.any() else np.zeros(0)
+    n = len(st_m)
+    return {"newton_ms": ms, "converged_lanes": int((st_m & 1).sum()), "ticks_per_lane": float(it_m.sum()) / n,
+            "same_ticks_as_fp64_lanes": int((it_m == it_n).sum()),
+            "fp32_solves_per_lane": float(solves.sum()) / n, "fp32_solves_per_tick": float(solves.sum()) /
+            max(float(it_m.sum()), 1.0), "fp64_fallback_ticks": int(fallbacks.sum()),
+            "max_node_err_vs_fp64_V": float(err.max()) if err.size else None,
+            "median_node_err_vs_fp64_V": float(np.median(err)) if err.size else None,
+            "max_node_rel_err_vs_fp64": float(rel.max()) if rel.size else None}
+
+
 def opamp_passes(t):
     """The op-amp's .meas passes: .meas tran vmax MAX v(out); .meas tran
     tpd WHEN v(out)=0.95 RISE=1 (the 50% point of the 100 mV step)."""
@@ -318,6 +339,14 @@ def bench_opamp(args):
         return lanes.newton(init_fix=False)
 
     op_n = [op_nominal() for _ in range(2)][-1]
+
+    # the same operating points with FP32 factors and FP64 refinement: the
+    # error against the FP64 lanes (bitwise the oracle's) per lane
+    def op_mixed():
+        lanes.upload_x(x_nom)
+        return lanes.newton_mixed(init_fix=False)
+
+    op_m = [op_mixed() for _ in range(2)][-1]
 
     # training set: other Monte Carlo draws, solved by the engine from the
     # nominal point (untimed; reported with the amortised rate)
@@ -380,7 +409,8 @@ def bench_opamp(args):
                    "op_only": {"learned_newton_ms": op_l[4], "learned_inference_ms": op_l[3],
                                "learned_ticks_per_lane": float(op_l[2].sum()) / N,
                                "nominal_newton_ms": op_n[3], "nominal_ticks_per_lane": float(op_n[2].sum()) / N,
-                               "max_node_dev_learned_vs_nominal_V": float(dev.max()) if dev.size else None},
+                               "max_node_dev_learned_vs_nominal_V": float(dev.max()) if dev.size else None,
+                               "mixed": mixed_summary(op_m, op_n, node_rows)},
                    "training": {"samples": int(ok.sum()), "epochs": int(sum(e for e, _ in OPAMP_SCHEDULE)),
                                 "batch": 512, "schedule": [list(e) for e in OPAMP_SCHEDULE],
                                 "solve_s": round(solve_s, 3), "train_s": round(train_s, 3),

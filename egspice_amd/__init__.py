@@ -38,6 +38,7 @@ EXPORTS = (
     "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
     "espice_lanes_tab_refactor_solve_device", "espice_lanes_template_gmin_stamps", "espice_lanes_shape",
     "espice_lanes_newton_homotopy", "espice_lanes_op_ladder", "espice_lanes_set_source", "espice_lanes_dc_sweep",
+    "espice_lanes_newton_mixed",
 )
 
 
@@ -70,8 +71,8 @@ HOM_ACTIVE, HOM_COLD = 1, 2
 class TranOpts(C.Structure):
     _fields_ = [(f, C.c_double) for f in ("t_stop", "dt_init", "dt_max", "dt_min", "xmu", "reltol", "abstol", "vntol",
                                           "chgtol", "trtol")] + \
-               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm", "pad",
-                                          "early_stop", "pad2")] + \
+               [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm",
+                                          "mixed_refine", "early_stop", "pad2")] + \
                [("abort_after", C.c_double), ("abort_mask", C.c_uint64)]
 
 
@@ -87,7 +88,7 @@ class MeasResult(C.Structure):
 
 class TranResult(C.Structure):
     _fields_ = [(f, C.c_uint32) for f in ("status", "points", "steps", "attempts", "rej_newton", "rej_lte",
-                                          "bp_landings", "nr_iters")]
+                                          "bp_landings", "nr_iters", "fp32_solves", "fp64_fallbacks")]
 
 
 MEAS_CROSS, MEAS_AT, MEAS_EXTREMUM, MEAS_INTEG, MEAS_FIND = range(5)
@@ -168,6 +169,7 @@ def library():
         "espice_lanes_op_ladder": [P, C.POINTER(NewtonOpts), C.POINTER(LadderOpts), C.POINTER(LaneResult), u32p,
                                    f64p],
         "espice_lanes_set_source": [P, C.c_uint32, C.c_uint32, C.c_double],
+        "espice_lanes_newton_mixed": [P, C.POINTER(NewtonOpts), C.c_uint32, C.POINTER(LaneResult), u32p, f64p],
         "espice_lanes_dc_sweep": [P, C.c_uint32, C.c_uint32, f64p, C.c_uint32, C.POINTER(NewtonOpts),
                                   C.POINTER(LadderOpts), f64p, u32p, f64p],
     }
@@ -206,18 +208,20 @@ def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e
 
 def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
               chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0, op_warm=False,
-              early_stop=False, abort_after=float("nan"), abort_mask=(1 << 64) - 1):
+              early_stop=False, abort_after=float("nan"), abort_mask=(1 << 64) - 1, mixed_refine=0):
     """`.tran dt_init t_stop`; a deck's step ceiling is min(dt_init, t_stop/50)
     (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680), the
     tolerances are converger.Options' (src/core/numerics.zig:378-399).
     early_stop / abort_after / abort_mask: the points options' measure
     handling (a lane ends once its passes stop; past abort_after, a running
-    pass of the mask aborts it with status 4)."""
+    pass of the mask aborts it with status 4). mixed_refine k > 0: every
+    Newton tick's linear step in FP32 with FP64 refinement, at most k FP32
+    solves per tick."""
     if dt_max is None:
         dt_max = min(dt_init, t_stop / 50)
     return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, method, max_iter,
-                    max_steps, record_capacity, 1 if op_warm else 0, 0, 1 if early_stop else 0, 0, abort_after,
-                    abort_mask)
+                    max_steps, record_capacity, 1 if op_warm else 0, mixed_refine, 1 if early_stop else 0, 0,
+                    abort_after, abort_mask)
 
 
 def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):
@@ -414,6 +418,18 @@ class Lanes:
         _check(self.lib.espice_lanes_op_ladder(self.h, C.byref(o), C.byref(lo),
                                                res.ctypes.data_as(C.POINTER(LaneResult)), rp, C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), rung, ms.value
+
+    def newton_mixed(self, max_refine=10, **kw):
+        """Static Newton from the lanes' current x with FP32 factors and
+        FP64 refinement; returns (x, status, iterations, solves, fallbacks,
+        kernel_ms)."""
+        o = newton_opts(**kw)
+        res = np.zeros((self.N, 2), dtype=np.uint32)
+        ref, rp = _u32(np.zeros(2 * self.N))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_newton_mixed(self.h, C.byref(o), max_refine,
+                                                  res.ctypes.data_as(C.POINTER(LaneResult)), rp, C.byref(ms)))
+        return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), ref[0::2].copy(), ref[1::2].copy(), ms.value
 
     def dc_sweep(self, kind, inst, values, gmin_start=1e-2, ladder_max_iter=100, **kw):
         """A `.dc` sweep of one source's dc value over every lane; returns

@@ -108,6 +108,14 @@ static_assert(sizeof(LaneHom) == 32, "LaneHom is four 8-byte words");
 // for HOM_PREDICT lanes the previous point (node-major like x), against
 // which a `.dc` point's predicted start is limited before its Newton
 // (src/analysis/dc/dc.zig:478-483 prepareWarm).
+// A mixed-precision launch (FP32 factor and solves with FP64 refinement,
+// wave_mixed.h): the FP32 solves allowed per tick, and per lane two words
+// out: the FP32 solves taken and the ticks handed back to FP64.
+struct MixedCtl {
+    uint32_t max_refine;
+    uint32_t *out;
+};
+
 struct Homotopy {
     const LaneHom *lane;
     const uint32_t *stamp;

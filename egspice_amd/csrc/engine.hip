@@ -124,13 +124,16 @@ __global__ __launch_bounds__(64) void newton_op_kernel(Batch B, Newton o)
 // once; the others take their gmin shunt, cold or warm start and source
 // factor from it and first write their model gmin into their own
 // prepared constants.
-template <bool DENSE, int VA>
+// MIXED: every tick's linear step in FP32 with FP64 refinement, its arrays
+// after the lane block (mixed_words), tallies to mix.out.
+template <bool DENSE, int VA, bool MIXED = false>
 __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                          double *prm_lanes, double *x_glob, uint32_t N, Newton o_in,
                                                          uint32_t *status, uint32_t *iters, const uint32_t *pred,
                                                          uint32_t *pred_hit, uint32_t *h_status, uint32_t *h_iters,
-                                                         double *h_x, double *lu_g, Homotopy hom)
+                                                         double *h_x, double *lu_g, Homotopy hom, MixedCtl mix)
 {
+    static_assert(DENSE || !MIXED, "the mixed step takes a dense lane (n <= 64)");
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -198,8 +201,19 @@ __global__ __launch_bounds__(64) void newton_wave_kernel(WaveBlob B, const uint3
     __syncthreads();
     if (DENSE) {
         const WaveHook off{};
-        const int r = wave_newton<false, false, VA>(W, s, D, rc, o, off, o.init_fix != 0, lim_active, 0.0, it);
+        MixedRun mx{};
+        if constexpr (MIXED) {
+            mx.m = mixed_lu(block + lay.words, n);
+            mx.max_refine = mix.max_refine;
+            mixed_rows(W.P, mx.m);
+        }
+        const int r = wave_newton<false, false, VA, MIXED>(W, s, D, rc, o, off, o.init_fix != 0, lim_active, 0.0, it,
+                                                           &mx);
         st = r == NEWTON_CONVERGED ? 1u : r == NEWTON_SINGULAR ? 2u : 0u;
+        if (MIXED && tid == 0) {
+            mix.out[2 * (size_t)l] = mx.solves;
+            mix.out[2 * (size_t)l + 1] = mx.fallbacks;
+        }
     } else {
         uint32_t bits = o.init_fix ? G_INITFIX : 0;
         double limit = 1e9, scaled = 0;
@@ -326,13 +340,13 @@ __global__ __launch_bounds__(64) void analyze_wave_kernel(WaveBlob B, const uint
 // t = 0, then steps until t_stop, each accepted row fed to the .meas
 // passes and, up to `to.record_cap` rows, recorded. The lane's final
 // state goes back to x_glob.
-template <int VA>
+template <int VA, bool MIXED = false>
 __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
                                                        const double *prm_lanes, double *x_glob, uint32_t N,
                                                        Newton op_opts, Newton step_opts, TranOpts to,
                                                        const MeasPass *passes, uint32_t n_pass, TranOut *outs,
                                                        MeasOut *meas_out, double *rec_t, double *rec_x,
-                                                       const uint32_t *pred, double *lu_g)
+                                                       const uint32_t *pred, double *lu_g, MixedCtl mix)
 {
     extern __shared__ double lds[];
     const uint32_t l = blockIdx.x;
@@ -347,6 +361,14 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     double *const lane_scr = lu_g + (size_t)l * lane_scratch_words(W.cap, W.n_tape);
     if (lay.lu_global) wave_bind_lu_scratch(s, W, lane_scr);
     const DenseLU D = dense_lu(block, lay, n, pred);
+    // MIXED: the Newton ticks' linear steps in FP32 with FP64 refinement,
+    // their arrays after the lane block
+    MixedRun mx{};
+    if constexpr (MIXED) {
+        mx.m = mixed_lu(block + lay.words, n);
+        mx.max_refine = mix.max_refine;
+        mixed_rows(W.P, mx.m);
+    }
     const uint32_t n_tape = W.n_tape;
     double *const a = s.cc + (nnz + 1);
     double *const ring = s.tape + n_tape;
@@ -418,7 +440,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     // cold: MODEINITJCT seeds and INITFIX; warm: Newton from x, as the
     // operating point's warm start (src/analysis/dc/op.zig:588, 671)
     const bool cold = to.op_warm == 0;
-    const int op = wave_newton<true, false, VA>(W, s, D, rc, op_opts, off, cold, cold, 0.0, it);
+    const int op = wave_newton<true, false, VA, MIXED>(W, s, D, rc, op_opts, off, cold, cold, 0.0, it, &mx);
     if (op != NEWTON_CONVERGED) {
         if (tid == 0) {
             st.out.status = op == NEWTON_SINGULAR ? 3u : 2u;
@@ -508,7 +530,7 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
         for (uint32_t e = tid; e < W.n_inst; e += nt) (void)wave_limit_inst<VA>(W, s, e, false);
         __syncthreads();
         WaveHook h{method, scal[4], scal[5], scal[6], ring + p1 * (size_t)n, ring + p2 * (size_t)n, i_prev, a};
-        const int r = wave_newton<true, true, VA>(W, s, D, rc, step_opts, h, false, true, t_dev, it);
+        const int r = wave_newton<true, true, VA, MIXED>(W, s, D, rc, step_opts, h, false, true, t_dev, it, &mx);
         if (r == NEWTON_CONVERGED && has_charge) {
             // the charge re-read at the solution, unlimited (tran.zig:859-864,
             // eval.zig:1086-1108)
@@ -668,6 +690,8 @@ __global__ __launch_bounds__(64) void tran_wave_kernel(WaveBlob B, const uint32_
     for (uint32_t i = tid; i < n; i += nt) x_glob[(size_t)i * N + l] = cur[i];
     if (tid == 0) {
         st.out.steps = st.steps;
+        st.out.fp32_solves = mx.solves;
+        st.out.fp64_fallbacks = mx.fallbacks;
         outs[l] = st.out;
         for (uint32_t k = 0; k < n_pass; k++) meas_out[(size_t)l * n_pass + k] = meas_result(passes[k], ms[k]);
     }
@@ -819,27 +843,49 @@ static void launch_newton_dense(espice_lanes *L, const Newton &o, bool want_x, c
     hipLaunchKernelGGL((newton_wave_kernel<true, V>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                        L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
                        L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters,
-                       want_x ? L->d_x : nullptr, L->lu_g.p, hom);
+                       want_x ? L->d_x : nullptr, L->lu_g.p, hom, MixedCtl{});
 }
 
 template <int V>
+static void launch_newton_mixed(espice_lanes *L, const Newton &o, size_t lds, const MixedCtl &mix)
+{
+    hipLaunchKernelGGL((newton_wave_kernel<true, V, true>), dim3(L->N), dim3(64), lds, L->stream, L->wh.at,
+                       L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
+                       L->has_pred ? L->pred.p : nullptr, L->pred_hit.p, L->d_status, L->d_iters, nullptr,
+                       L->lu_g.p, Homotopy{}, mix);
+}
+
+template <int V, bool MIXED = false>
 static void launch_tran(espice_lanes *L, uint32_t N, const WaveLayout &lay, size_t lds, const Newton &op,
                         const Newton &step, const TranOpts &to, const MeasPass *passes, uint32_t n_passes,
-                        TranOut *outs, MeasOut *meas, double *rec_t, double *rec_x)
+                        TranOut *outs, MeasOut *meas, double *rec_t, double *rec_x, const MixedCtl &mix)
 {
-    hipLaunchKernelGGL(tran_wave_kernel<V>, dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
+    hipLaunchKernelGGL((tran_wave_kernel<V, MIXED>), dim3(N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
                        L->prm_lanes.p, L->x.p, N, op, step, to, passes, n_passes, outs, meas, rec_t, rec_x,
-                       L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p);
+                       L->has_pred && lay.dense != ESP_NONE ? L->pred.p : nullptr, L->lu_g.p, mix);
 }
 
 static void (*const LAUNCH_ANALYZE[VA_SETS])(espice_lanes *, const Newton &) = {
     launch_analyze<0>, launch_analyze<1>, launch_analyze<2>, launch_analyze<3>};
 static void (*const LAUNCH_NEWTON[VA_SETS])(espice_lanes *, const Newton &, bool, const Homotopy &) = {
     launch_newton_dense<0>, launch_newton_dense<1>, launch_newton_dense<2>, launch_newton_dense<3>};
-static void (*const TRAN_LAUNCH[VA_SETS])(espice_lanes *, uint32_t, const WaveLayout &, size_t, const Newton &,
-                                          const Newton &, const TranOpts &, const MeasPass *, uint32_t, TranOut *,
-                                          MeasOut *, double *, double *) = {launch_tran<0>, launch_tran<1>,
-                                                                            launch_tran<2>, launch_tran<3>};
+// the mixed-precision Newton is built for the hand-written kinds and for
+// templates holding PSP103 (the op-amp); the other sets are refused
+static void (*const LAUNCH_NEWTON_MIXED[VA_SETS])(espice_lanes *, const Newton &, size_t, const MixedCtl &) = {
+    launch_newton_mixed<0>, launch_newton_mixed<1>, nullptr, nullptr};
+static const void *const NEWTON_MIXED_KERNEL[VA_SETS] = {
+    (const void *)newton_wave_kernel<true, 0, true>, (const void *)newton_wave_kernel<true, 1, true>, nullptr,
+    nullptr};
+typedef void (*TranLaunch)(espice_lanes *, uint32_t, const WaveLayout &, size_t, const Newton &, const Newton &,
+                           const TranOpts &, const MeasPass *, uint32_t, TranOut *, MeasOut *, double *, double *,
+                           const MixedCtl &);
+static const TranLaunch TRAN_LAUNCH[VA_SETS] = {launch_tran<0>, launch_tran<1>, launch_tran<2>, launch_tran<3>};
+// the mixed-precision transient, for the hand-written kinds and templates
+// holding PSP103 (the op-amp)
+static const TranLaunch TRAN_LAUNCH_MIXED[VA_SETS] = {launch_tran<0, true>, launch_tran<1, true>, nullptr,
+                                                      nullptr};
+static const void *const TRAN_MIXED_KERNEL[VA_SETS] = {(const void *)tran_wave_kernel<0, true>,
+                                                       (const void *)tran_wave_kernel<1, true>, nullptr, nullptr};
 static const void *const ANALYZE_KERNEL[VA_SETS] = {
     (const void *)analyze_wave_kernel<0>, (const void *)analyze_wave_kernel<1>, (const void *)analyze_wave_kernel<2>,
     (const void *)analyze_wave_kernel<3>};
@@ -1335,7 +1381,8 @@ static espice_status launch_newton(espice_lanes *L, const espice_newton_opts *op
         else
             hipLaunchKernelGGL((newton_wave_kernel<false, 0>), dim3(L->N), dim3(64), L->lds_bytes, L->stream, L->wh.at,
                                L->w_blob.p, L->lay, L->prm_lanes.p, L->x.p, L->N, o, L->status.p, L->iters.p,
-                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p, hom);
+                               nullptr, nullptr, L->d_status, L->d_iters, want_x ? L->d_x : nullptr, L->lu_g.p, hom,
+                               MixedCtl{});
     } else {
         const Batch B = L->batch();
         const uint32_t blocks = (L->N + 63) / 64;
@@ -1440,6 +1487,31 @@ espice_status espice_lanes_newton_homotopy(espice_lanes *L, const espice_newton_
     for (size_t l = 0; l < L->N; l++)
         if (!(lanes[l].flags & ESPICE_HOM_ACTIVE)) out[l] = espice_lane_result{0, 0};
     return ESPICE_OK;
+}
+
+espice_status espice_lanes_newton_mixed(espice_lanes *L, const espice_newton_opts *opts, uint32_t max_refine,
+                                        espice_lane_result *out, uint32_t *refine, double *kernel_ms)
+{
+    if (!L || !opts || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    if (opts->max_iter == 0 || max_refine == 0) return fail(ESPICE_INVALID_ARGUMENT, "MaxIter");
+    if (!L->use_wave || L->lay.dense == ESP_NONE) return fail(ESPICE_FAILED, "LaneTooLarge");
+    if (!LAUNCH_NEWTON_MIXED[L->va_mask]) return fail(ESPICE_FAILED, "MixedUnsupported");
+    const size_t lds = L->lds_bytes + 8 * mixed_words(L->t->n, L->t->nnz);
+    if (lds > L->lds_max) return fail(ESPICE_FAILED, "LaneTooLarge");
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(NEWTON_MIXED_KERNEL[L->va_mask], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    DevBuf<uint32_t> d_mix;
+    HIP_TRY(d_mix.alloc(2 * (size_t)L->N));
+    Newton o{opts->reltol, opts->abstol, opts->vntol, opts->residual_tol, opts->gmin, opts->max_iter, opts->init_fix};
+    HIP_TRY(hipEventRecord(L->ev0, L->stream));
+    LAUNCH_NEWTON_MIXED[L->va_mask](L, o, lds, MixedCtl{max_refine, d_mix.p});
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(L->ev1, L->stream));
+    if (refine)
+        HIP_TRY(hipMemcpyAsync(refine, d_mix.p, 8 * (size_t)L->N, hipMemcpyDeviceToHost, L->stream));
+    return finish_newton(L, out, nullptr, kernel_ms);
 }
 
 espice_status espice_lanes_newton_op(espice_lanes *L, const espice_newton_opts *opts, espice_lane_result *out, double *kernel_ms)
@@ -1556,7 +1628,9 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     lay.lu_global = L->lay.lu_global;
     const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
     const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, n_passes));
-    const size_t lds = 4 * table_lds_words(L->wh.at.words, lay) + 8 * words;
+    const bool mixed = opts->mixed_refine != 0;
+    if (mixed && !TRAN_LAUNCH_MIXED[L->va_mask]) return fail(ESPICE_FAILED, "MixedUnsupported");
+    const size_t lds = 4 * table_lds_words(L->wh.at.words, lay) + 8 * words + (mixed ? 8 * mixed_words(n, L->t->nnz) : 0);
     if (lay.dense == ESP_NONE || lds > L->lds_max) return fail(ESPICE_FAILED, "LaneTooLarge");
     DevBuf<MeasPass> d_pass;
     DevBuf<TranOut> d_out;
@@ -1583,10 +1657,11 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     step.residual_tol = inf_();
     step.init_fix = 0;
     if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(TRAN_KERNEL[L->va_mask],
+        HIP_TRY(hipFuncSetAttribute(mixed ? TRAN_MIXED_KERNEL[L->va_mask] : TRAN_KERNEL[L->va_mask],
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIP_TRY(hipEventRecord(L->ev0, L->stream));
-    TRAN_LAUNCH[L->va_mask](L, N, lay, lds, op, step, to, d_pass.p, n_passes, d_out.p, d_meas.p, d_t.p, d_x.p);
+    (mixed ? TRAN_LAUNCH_MIXED : TRAN_LAUNCH)[L->va_mask](L, N, lay, lds, op, step, to, d_pass.p, n_passes, d_out.p,
+                                                         d_meas.p, d_t.p, d_x.p, MixedCtl{opts->mixed_refine, nullptr});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(L->ev1, L->stream));
     HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(TranOut) * N, hipMemcpyDeviceToHost, L->stream));

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "wave_lu.h"
+#include "wave_mixed.h"
 
 namespace esp {
 
@@ -20,14 +21,25 @@ struct WaveHook {
 
 enum { NEWTON_SINGULAR = -1, NEWTON_FAILED = 0, NEWTON_CONVERGED = 1 };
 
+// The mixed-precision linear step's controls and tallies (wave_mixed.h):
+// at most max_refine FP32 solves per tick; solves and the ticks handed back
+// to the FP64 factor are counted per lane.
+struct MixedRun {
+    MixedLU m;
+    uint32_t max_refine;
+    uint32_t solves, fallbacks;
+};
+
 // Solves from s.x; a converged lane returns x_k (x = x_old) and the
 // iteration count. TRAN: sources read their waveform at t (a transient's
 // operating point and its steps); HOOK: the integrator's companion applies
-// (h). Both are compile-time so the static solve carries none of it. All
-// threads; uniform result.
-template <bool TRAN, bool HOOK, int VA = 0>
+// (h). Both are compile-time so the static solve carries none of it.
+// MIXED: each tick's linear step runs in FP32 with FP64 refinement (mx).
+// All threads; uniform result.
+template <bool TRAN, bool HOOK, int VA = 0, bool MIXED = false>
 __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, const DenseLU &D, const ReduceCache &rc, const Newton &o,
-                           const WaveHook &h, bool init_fix, bool lim_active, double t, uint32_t &iters)
+                           const WaveHook &h, bool init_fix, bool lim_active, double t, uint32_t &iters,
+                           MixedRun *mx = nullptr)
 {
     static_assert(TRAN || !HOOK, "the companion only runs in a transient");
     const uint32_t tid = threadIdx.x, nt = blockDim.x, n = W.P.n, nnz = W.P.nnz;
@@ -66,8 +78,17 @@ __device__ __forceinline__ int wave_newton(const WaveTemplate &W, WaveLane &s, c
         } else {
             if (tid == 0) wave_assemble(W, s, o, tick, limit);
         }
-        const int fl = wave_direct_factor(W.P, s.lu, D, vals, &s.flags[2]);
-        if (fl == 0) {
+        bool mixed_done = false;
+        if constexpr (MIXED) {
+            uint32_t solves = 0;
+            mixed_done = mixed_step(W.P, mx->m, vals, s.rhs, s.dx, mx->max_refine, solves);
+            mx->solves += solves;
+            if (!mixed_done) mx->fallbacks++;
+        }
+        const int fl = mixed_done ? 0 : wave_direct_factor(W.P, s.lu, D, vals, &s.flags[2]);
+        if (mixed_done) {
+            scaled = wave_update_norm(n, s.x, s.dx, s.x_old, W.cur_row, o.reltol, o.abstol, o.vntol);
+        } else if (fl == 0) {
             if (s.lu.state[ST_TRI]) {
                 if (tid == 0) {
                     LV<double> rhs{s.rhs, 1}, dx{s.dx, 1};

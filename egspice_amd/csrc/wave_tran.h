@@ -22,7 +22,9 @@ struct TranOpts {
     double t_stop, dt_init, dt_max, dt_min, xmu;
     double reltol, abstol, vntol, chgtol, trtol;
     uint32_t method, max_iter, max_steps, record_cap;
-    uint32_t op_warm, pad;  // op_warm: the operating point starts at the lane's x
+    // op_warm: the operating point starts at the lane's x; mixed_refine
+    // (the host's switch): FP32 linear steps with at most this many solves
+    uint32_t op_warm, mixed_refine;
     // early_stop: the lane ends once every pass has stopped; abort_after
     // (NaN off): past it, a pass of abort_mask still running aborts the lane
     uint32_t early_stop, pad2;
@@ -49,6 +51,9 @@ struct TranOut {
     uint32_t status;  // 0 reached t_stop (or stopped early), 1 dt underflow, 2 no operating point, 3 singular
                       // operating point, 4 aborted by the measure bound
     uint32_t points, steps, attempts, rej_newton, rej_lte, bp_landings, nr_iters;
+    // the mixed-precision linear steps (zero in FP64): FP32 solves taken,
+    // Newton ticks handed back to the FP64 factor
+    uint32_t fp32_solves, fp64_fallbacks;
 };
 
 constexpr uint32_t MEAS_WORDS = 24;  // LDS words of one pass's state

@@ -158,6 +158,24 @@ espice_status espice_lanes_newton_op(espice_lanes *l, const espice_newton_opts *
  * reference, whose lanes always run the full factor first
  * (src/solver/sparse_lu.zig:274-311); diagnostics only. */
 espice_status espice_lanes_first_factors(espice_lanes *l, uint32_t *predicted, uint32_t *serial);
+/* The static Newton of every lane with each tick's linear step in mixed
+ * precision (the lane path's FP32 numeric LU and triangular solves,
+ * recovered by FP64 iterative refinement; the reference's
+ * src/solver/sparse_lu.zig:973-1322 refinement against its residual bars):
+ * the FP64 matrix rounded to FP32, factored densely with partial
+ * pivoting after an exact power-of-two equilibration, solved in FP32, and
+ * refined with FP64 residuals r = -rhs - A dx until the residual meets the
+ * FP64 solve's backward-error bar max|r| <= max|dx| ||A||inf sqrt(n) 2^-53
+ * (LAPACK dsgesv's test), at most max_refine FP32 solves per tick; a tick
+ * whose refinement overflows or runs out of solves takes the FP64 factor
+ * instead. Device evaluation, the residual,
+ * the update and every gate stay FP64, so a converged lane's operating
+ * point lies inside the same Newton tolerance as the FP64 path's.
+ * refine[2 * n_lanes] (optional): per lane the FP32 solves taken and the
+ * ticks handed back to FP64. Lanes up to 64 unknowns, templates of the
+ * hand-written kinds or with PSP103 (MixedUnsupported otherwise). */
+espice_status espice_lanes_newton_mixed(espice_lanes *l, const espice_newton_opts *opts, uint32_t max_refine,
+    espice_lane_result *out, uint32_t *refine, double *kernel_ms);
 /* Unknowns per lane and lanes of a batch. */
 espice_status espice_lanes_shape(const espice_lanes *l, uint32_t *n, uint32_t *n_lanes);
 
@@ -274,7 +292,12 @@ typedef struct {
      * INITFIX); 1: it starts warm at each lane's x (espice_lanes_upload_x
      * or a warm start), as the reference's warm operating point
      * (src/analysis/dc/op.zig:588, 671). */
-    uint32_t op_warm, pad;
+    uint32_t op_warm;
+    /* 0: every Newton tick's linear step in FP64; k > 0: in FP32 with FP64
+     * iterative refinement, at most k FP32 solves per tick
+     * (espice_lanes_newton_mixed's step; templates of the hand-written
+     * kinds or with PSP103, MixedUnsupported otherwise). */
+    uint32_t mixed_refine;
     /* The points options' measure handling (include/espice.h:286-301,
      * src/analysis/tran/tran.zig:1059-1064), checked at every accepted
      * point after the passes read it: early_stop 1 ends a lane once every
@@ -309,9 +332,12 @@ typedef struct {
 } espice_meas_result;
 
 /* status 0 reached t_stop, 1 dt underflow, 2 no operating point, 3
- * singular operating point; points = recorded rows including t = 0. */
+ * singular operating point, 4 aborted by abort_after; points = recorded
+ * rows including t = 0; with mixed_refine, the FP32 solves taken and the
+ * Newton ticks handed back to the FP64 factor (0 otherwise). */
 typedef struct {
     uint32_t status, points, steps, attempts, rej_newton, rej_lte, bp_landings, nr_iters;
+    uint32_t fp32_solves, fp64_fallbacks;
 } espice_tran_result;
 
 /* Every lane's transient in one launch, its operating point solved first
