@@ -38,7 +38,7 @@ EXPORTS = (
     "espice_lanes_first_factors", "espice_lanes_param_count", "espice_lanes_param_index",
     "espice_lanes_tab_refactor_solve_device", "espice_lanes_template_gmin_stamps", "espice_lanes_shape",
     "espice_lanes_newton_homotopy", "espice_lanes_op_ladder", "espice_lanes_set_source", "espice_lanes_dc_sweep",
-    "espice_lanes_newton_mixed",
+    "espice_lanes_newton_mixed", "espice_lanes_template_ac_drive", "espice_lanes_ac",
 )
 
 
@@ -170,6 +170,8 @@ def library():
                                    f64p],
         "espice_lanes_set_source": [P, C.c_uint32, C.c_uint32, C.c_double],
         "espice_lanes_newton_mixed": [P, C.POINTER(NewtonOpts), C.c_uint32, C.POINTER(LaneResult), u32p, f64p],
+        "espice_lanes_template_ac_drive": [P, C.c_uint32, C.c_uint32, C.c_double, C.c_double, f64p],
+        "espice_lanes_ac": [P, f64p, C.c_uint32, f64p, f64p, f64p],
         "espice_lanes_dc_sweep": [P, C.c_uint32, C.c_uint32, f64p, C.c_uint32, C.POINTER(NewtonOpts),
                                   C.POINTER(LadderOpts), f64p, u32p, f64p],
     }
@@ -325,6 +327,14 @@ class Template:
         self.col_ptr, self.row_idx, self.diag, self.cur_row, self.q = cp, ri, dg, cr, q
         return self
 
+    def ac_drive(self, sources):
+        """The AC excitation (2n: real parts, then imaginary) of sources =
+        [(kind, instance, mag, phase_deg)]."""
+        d, dp = _f64(np.zeros(2 * self.n))
+        for kind, inst, mag, phase in sources:
+            _check(self.lib.espice_lanes_template_ac_drive(self.h, kind, inst, mag, phase, dp))
+        return d
+
     def gmin_stamps(self):
         """(col, slot) per row: where the operating point's stepping rungs
         load their diagonal gmin (NONE where a row takes none)."""
@@ -418,6 +428,18 @@ class Lanes:
         _check(self.lib.espice_lanes_op_ladder(self.h, C.byref(o), C.byref(lo),
                                                res.ctypes.data_as(C.POINTER(LaneResult)), rp, C.byref(ms)))
         return self.download_x(), res[:, 0].copy(), res[:, 1].copy(), rung, ms.value
+
+    def ac(self, omega, drive):
+        """Every lane's small-signal response at its x: complex array
+        (frequencies, n, N) and the kernel's milliseconds."""
+        w, wp = _f64(np.asarray(omega, dtype=np.float64))
+        d, dp = _f64(np.asarray(drive, dtype=np.float64))
+        assert d.size == 2 * self.t.n
+        out, op = _f64(np.zeros(w.size * 2 * self.t.n * self.N))
+        ms = C.c_double()
+        _check(self.lib.espice_lanes_ac(self.h, wp, w.size, dp, op, C.byref(ms)))
+        o = out.reshape(w.size, 2, self.t.n, self.N)
+        return o[:, 0] + 1j * o[:, 1], ms.value
 
     def newton_mixed(self, max_refine=10, **kw):
         """Static Newton from the lanes' current x with FP32 factors and

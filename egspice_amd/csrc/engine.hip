@@ -721,6 +721,154 @@ __global__ __launch_bounds__(64) void tab_refactor_kernel(LaneTab t, const uint3
                             LV<double>{y + l, N}, dv);
 }
 
+// ------------------------------------------------------- AC lanes
+
+// One lane's small-signal response at its operating point
+// (src/analysis/ac/ac.zig:40-86, one lane of freq.Stream): the lane's G
+// and C planes once, from one transient-mode evaluation at x (no limiting;
+// the ground row pinned), then for each frequency (G + j w C) X = drive
+// factored densely in complex FP64 with partial pivoting (largest
+// |re| + |im|, the lowest row among equals) on the wavefront and solved,
+// thread r owning row r of the vectors. Out: [(k * 2 + part) * n + i] * N
+// + l, part 0 the real planes, 1 the imaginary; NaN where the matrix is
+// singular. The complex matrix lives after the lane block (2 n * n doubles).
+template <int VA>
+__global__ __launch_bounds__(64) void ac_wave_kernel(WaveBlob B, const uint32_t *blob, WaveLayout lay,
+                                                     const double *prm_lanes, const double *x_glob, uint32_t N,
+                                                     const double *omega, uint32_t n_freq, const double *drive,
+                                                     double *out, double *lu_g)
+{
+    extern __shared__ double lds[];
+    const uint32_t l = blockIdx.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t n = B.n, nnz = B.nnz;
+    uint32_t *tl = reinterpret_cast<uint32_t *>(lds);
+    if (!lay.lu_global)
+        for (uint32_t i = tid; i < B.words; i += nt) tl[i] = blob[i];
+    double *block = lds + table_lds_words(B.words, lay) / 2;
+    const WaveTemplate W = wave_template(B, lay.lu_global ? blob : tl);
+    WaveLane s = wave_lane(W, lay, block);
+    if (lay.lu_global) wave_bind_lu_scratch(s, W, lu_g + (size_t)l * lane_scratch_words(W.cap, W.n_tape));
+    double *prm_lds = block + lay.prm;
+    for (uint32_t i = tid; i < W.n_prm_lds; i += nt) prm_lds[i] = prm_lanes[(size_t)l * W.n_prm + i];
+    s.gprm = prm_lanes + (size_t)l * W.n_prm;
+    for (uint32_t i = tid; i < n; i += nt) s.x[i] = x_glob[(size_t)i * N + l];
+    __syncthreads();
+    const ReduceCache rc = wave_reduce_cache(W, tid);
+    for (uint32_t e = tid; e < W.n_inst; e += nt) wave_eval_inst<true, VA>(W, s, e, false, 0.0);
+    __syncthreads();
+    wave_reduce(W, s, rc, tid, nt);
+    for (uint32_t e = tid; e < n + nnz; e += nt) wave_reduce_charge(W, s, e);
+    __syncthreads();
+    if (tid == 0) s.g[W.diag[0]] += 1.0;
+    __syncthreads();
+    double *const Fr = block + lay.words;
+    double *const Fi = Fr + (size_t)n * n;
+    const double br = tid < n ? drive[tid] : 0.0, bi = tid < n ? drive[n + tid] : 0.0;
+    for (uint32_t k = 0; k < n_freq; k++) {
+        const double w = omega[k];
+        for (uint32_t i = tid; i < n * n; i += nt) {
+            Fr[i] = 0.0;
+            Fi[i] = 0.0;
+        }
+        __syncthreads();
+        if (tid < n)
+            for (uint32_t p = W.P.col_ptr[tid]; p < W.P.col_ptr[tid + 1]; p++) {
+                Fr[tid * n + W.P.row_idx[p]] = s.g[p];
+                Fi[tid * n + W.P.row_idx[p]] = w * s.cc[p];
+            }
+        uint32_t perm_k = tid;
+        bool singular = false;
+        __syncthreads();
+        for (uint32_t c = 0; c < n; c++) {
+            double best = -1.0;
+            if (tid >= c && tid < n) {
+                best = __builtin_fabs(Fr[c * n + tid]) + __builtin_fabs(Fi[c * n + tid]);
+                if (!(best == best)) best = __builtin_huge_val();
+            }
+            uint32_t at = tid;
+            for (int sh = 32; sh >= 1; sh >>= 1) {
+                const double ob = __shfl_xor(best, sh, 64);
+                const uint32_t oa = __shfl_xor(at, sh, 64);
+                if (ob > best || (ob == best && oa < at)) {
+                    best = ob;
+                    at = oa;
+                }
+            }
+            if (!(best > 0.0) || !__builtin_isfinite(best)) {
+                singular = true;
+                break;
+            }
+            if (at != c) {
+                if (tid < n) {
+                    const double ar = Fr[tid * n + c], ai = Fi[tid * n + c];
+                    Fr[tid * n + c] = Fr[tid * n + at];
+                    Fi[tid * n + c] = Fi[tid * n + at];
+                    Fr[tid * n + at] = ar;
+                    Fi[tid * n + at] = ai;
+                }
+                const uint32_t pc = __shfl(perm_k, c, 64), pa = __shfl(perm_k, at, 64);
+                if (tid == c) perm_k = pa;
+                if (tid == at) perm_k = pc;
+            }
+            __syncthreads();
+            // multipliers L(r, c) = A(r, c) / A(c, c)
+            const double pr = Fr[c * n + c], pi = Fi[c * n + c];
+            const double den = pr * pr + pi * pi;
+            if (tid > c && tid < n) {
+                const double ar = Fr[c * n + tid], ai = Fi[c * n + tid];
+                Fr[c * n + tid] = (ar * pr + ai * pi) / den;
+                Fi[c * n + tid] = (ai * pr - ar * pi) / den;
+            }
+            __syncthreads();
+            if (tid > c && tid < n) {
+                const double ur = Fr[tid * n + c], ui = Fi[tid * n + c];
+                if (ur != 0.0 || ui != 0.0)
+                    for (uint32_t r = c + 1; r < n; r++) {
+                        const double lr = Fr[c * n + r], li = Fi[c * n + r];
+                        Fr[tid * n + r] -= lr * ur - li * ui;
+                        Fi[tid * n + r] -= lr * ui + li * ur;
+                    }
+            }
+            __syncthreads();
+        }
+        double yr = __shfl(br, (int)perm_k, 64), yi = __shfl(bi, (int)perm_k, 64);
+        if (!singular) {
+            for (uint32_t c = 0; c + 1 < n; c++) {
+                const double cr = __shfl(yr, (int)c, 64), ci = __shfl(yi, (int)c, 64);
+                if (tid > c && tid < n) {
+                    const double lr = Fr[c * n + tid], li = Fi[c * n + tid];
+                    yr -= lr * cr - li * ci;
+                    yi -= lr * ci + li * cr;
+                }
+            }
+            for (uint32_t c = n; c-- > 0;) {
+                if (tid == c) {
+                    const double pr = Fr[c * n + c], pi = Fi[c * n + c];
+                    const double den = pr * pr + pi * pi;
+                    const double zr = (yr * pr + yi * pi) / den, zi = (yi * pr - yr * pi) / den;
+                    yr = zr;
+                    yi = zi;
+                }
+                const double cr = __shfl(yr, (int)c, 64), ci = __shfl(yi, (int)c, 64);
+                if (tid < c) {
+                    const double ur = Fr[c * n + tid], ui = Fi[c * n + tid];
+                    yr -= ur * cr - ui * ci;
+                    yi -= ur * ci + ui * cr;
+                }
+            }
+        } else {
+            yr = __builtin_nan("");
+            yi = yr;
+        }
+        if (tid < n) {
+            out[((size_t)(2 * k) * n + tid) * N + l] = yr;
+            out[((size_t)(2 * k + 1) * n + tid) * N + l] = yi;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------- the allocation-free replay
 
 // Scratch of espice_lanes_tab_refactor_solve_device: the tape's index words
@@ -876,6 +1024,19 @@ static void (*const LAUNCH_NEWTON_MIXED[VA_SETS])(espice_lanes *, const Newton &
 static const void *const NEWTON_MIXED_KERNEL[VA_SETS] = {
     (const void *)newton_wave_kernel<true, 0, true>, (const void *)newton_wave_kernel<true, 1, true>, nullptr,
     nullptr};
+template <int V>
+static void launch_ac(espice_lanes *L, const WaveLayout &lay, size_t lds, const double *omega, uint32_t n_freq,
+                      const double *drive, double *out)
+{
+    hipLaunchKernelGGL(ac_wave_kernel<V>, dim3(L->N), dim3(64), lds, L->stream, L->wh.at, L->w_blob.p, lay,
+                       L->prm_lanes.p, L->x.p, L->N, omega, n_freq, drive, out, L->lu_g.p);
+}
+static void (*const AC_LAUNCH[VA_SETS])(espice_lanes *, const WaveLayout &, size_t, const double *, uint32_t,
+                                        const double *, double *) = {launch_ac<0>, launch_ac<1>, launch_ac<2>,
+                                                                     launch_ac<3>};
+static const void *const AC_KERNEL[VA_SETS] = {(const void *)ac_wave_kernel<0>, (const void *)ac_wave_kernel<1>,
+                                               (const void *)ac_wave_kernel<2>, (const void *)ac_wave_kernel<3>};
+
 typedef void (*TranLaunch)(espice_lanes *, uint32_t, const WaveLayout &, size_t, const Newton &, const Newton &,
                            const TranOpts &, const MeasPass *, uint32_t, TranOut *, MeasOut *, double *, double *,
                            const MixedCtl &);
@@ -1671,6 +1832,67 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
         HIP_TRY(hipMemcpyAsync(times, d_t.p, sizeof(double) * N * cap, hipMemcpyDeviceToHost, L->stream));
         HIP_TRY(hipMemcpyAsync(rows, d_x.p, sizeof(double) * N * cap * n, hipMemcpyDeviceToHost, L->stream));
     }
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    if (kernel_ms) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+        *kernel_ms = ms;
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_template_ac_drive(const espice_lanes_template *t, uint32_t kind, uint32_t instance,
+                                            double mag, double phase_deg, double *drive)
+{
+    if (!t || !drive) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!t->t.frozen) return fail(ESPICE_INVALID_ARGUMENT, "NotFrozen");
+    if ((kind != K_VSRC && kind != K_ISRC) || instance >= t->t.count[kind])
+        return fail(ESPICE_INVALID_ARGUMENT, "InvalidDevice");
+    const uint32_t n = t->t.n;
+    const double rad = phase_deg * (3.14159265358979323846 / 180.0);
+    const double re = mag * std::cos(rad), im = mag * std::sin(rad);
+    // (pos, neg, re, im) as the reference's excitation (builder.zig:802-817,
+    // :1146, :1158): a V card's branch row is its neg
+    const uint32_t *nd = &t->t.nodes[kind][(size_t)instance * KIND_NU[kind]];
+    const uint32_t pos = kind == K_VSRC ? 0 : nd[0], neg = kind == K_VSRC ? nd[2] : nd[1];
+    if (pos != 0) {
+        drive[pos] -= re;
+        drive[n + pos] -= im;
+    }
+    if (neg != 0) {
+        drive[neg] += re;
+        drive[n + neg] += im;
+    }
+    return ESPICE_OK;
+}
+
+espice_status espice_lanes_ac(espice_lanes *L, const double *omega, uint32_t n_freq, const double *drive,
+                              double *out, double *kernel_ms)
+{
+    if (!L || !omega || !drive || !out) return fail(ESPICE_INVALID_ARGUMENT, "NullArgument");
+    if (!L->prepared) return fail(ESPICE_INVALID_ARGUMENT, "NotPrepared");
+    if (n_freq == 0) return ESPICE_OK;
+    if (!tran_charges_zero(*L)) return fail(ESPICE_FAILED, "AcChargesUnsupported");
+    const uint32_t n = L->t->n, N = L->N;
+    WaveLayout lay{};
+    lay.lu_global = L->lay.lu_global;
+    const WaveTemplate W = wave_template(L->wh.at, L->wh.blob.data());
+    const size_t words = wave_words(W, lay, true, tran_extra_words(W.n_inst, 0));
+    const size_t lds = 4 * table_lds_words(L->wh.at.words, lay) + 8 * words + 16 * (size_t)n * n;
+    if (lay.dense == ESP_NONE || lds > L->lds_max) return fail(ESPICE_FAILED, "LaneTooLarge");
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(AC_KERNEL[L->va_mask], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DevBuf<double> d_w, d_drive, d_out;
+    HIP_TRY(d_w.alloc(n_freq));
+    HIP_TRY(d_drive.alloc(2 * (size_t)n));
+    HIP_TRY(d_out.alloc((size_t)n_freq * 2 * n * N));
+    HIP_TRY(hipMemcpyAsync(d_w.p, omega, 8 * (size_t)n_freq, hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipMemcpyAsync(d_drive.p, drive, 16 * (size_t)n, hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipEventRecord(L->ev0, L->stream));
+    AC_LAUNCH[L->va_mask](L, lay, lds, d_w.p, n_freq, d_drive.p, d_out.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(L->ev1, L->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_out.p, 8 * (size_t)n_freq * 2 * n * N, hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
     if (kernel_ms) {
         float ms = 0;

@@ -246,6 +246,25 @@ espice_status espice_lanes_dc_sweep(espice_lanes *l, uint32_t kind, uint32_t ins
     uint32_t n_points, const espice_newton_opts *opts, const espice_ladder_opts *lo, double *x_out,
     uint32_t *converged, double *kernel_ms);
 
+/* ---- AC lanes (src/analysis/ac/ac.zig:40-86, freq.Stream) -------------
+ * The reference's AC excitation (frontend/builder.zig:802-817, 1146, 1158,
+ * 3124-3151): source (kind, instance)'s AC spec added into drive[2n] (real
+ * parts, then imaginary): a voltage source's mag*e^(j phase) on its branch
+ * row, a current source's on its two node rows (minus on p, plus on n);
+ * phase in degrees. */
+espice_status espice_lanes_template_ac_drive(const espice_lanes_template *t, uint32_t kind, uint32_t instance,
+    double mag, double phase_deg, double *drive);
+/* Every lane's small-signal response at its x (its operating point, left
+ * by a Newton or uploaded): the lane's G and C planes from one evaluation
+ * there, then for each omega[k] (rad/s) the complex system
+ * (G + j omega C) X = drive solved in FP64 (dense LU with partial pivoting
+ * on the lane's wavefront). out[n_freq][2][n][n_lanes]: for each
+ * frequency the real parts of every unknown, then the imaginary parts;
+ * NaN where the lane's matrix is singular. AcChargesUnsupported for
+ * templates whose mos1 devices carry charge; lanes up to 64 unknowns. */
+espice_status espice_lanes_ac(espice_lanes *l, const double *omega, uint32_t n_freq, const double *drive,
+    double *out, double *kernel_ms);
+
 /* ---- the lead-tape lane LU (lane_device.zig refactor :101, solveNeg :170)
  * Builds a LaneTab from one factor of (col_ptr, row_idx, vals) on the host
  * with the template solver's threshold pivoting (lanes.zig:590-607 ownTape,

@@ -701,6 +701,23 @@ void or_eval_lane(or_lanes *L, uint32_t l, const double *x, double *rhs, double 
     free(r);
 }
 
+/* Lane l's G and C planes at x (nnz values each, the ground row pinned in
+ * G), from one transient-mode evaluation without limiting: the small-signal
+ * linearization of the AC lanes (src/analysis/ac/ac.zig:48, FreqSolver). */
+void or_eval_lane_gc(or_lanes *L, uint32_t l, const double *x, double *g_out, double *c_out)
+{
+    const or_circuit *c = L->c;
+    double *g = malloc(8 * (c->nnz + 1)), *r = malloc(8 * (c->n + 1));
+    double *q = malloc(8 * (c->n + 1)), *cc = malloc(8 * (c->nnz + 1));
+    uint32_t nt = or_tape_len(c);
+    double *tape = malloc(8 * (nt + 1));
+    or_evctx ec = {1, 0.0, q, cc, tape, NULL};
+    or_eval_planes(L, l, x, 0, g, r, &ec);
+    memcpy(g_out, g, 8 * c->nnz);
+    memcpy(c_out, cc, 8 * c->nnz);
+    free(g); free(r); free(q); free(cc); free(tape);
+}
+
 /* ---------------------------------------------------------------- newton */
 
 /* Gate bits, kernels/lane.zig:703-716. */

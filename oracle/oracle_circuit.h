@@ -163,6 +163,8 @@ int or_newton_hom_lim(or_lanes *L, uint32_t l, const or_newton_opts *o, double *
 int or_dc_sweep_lanes(or_lanes *L, int kind, uint32_t inst, const double *values, uint32_t npts,
                       const or_newton_opts *plain, double gmin_start, uint32_t budget, double *x_out,
                       uint32_t *converged);
+/* Lane l's G and C planes at x (nnz each; the AC linearization). */
+void or_eval_lane_gc(or_lanes *L, uint32_t l, const double *x, double *g_out, double *c_out);
 /* Lane l's compiled instances set up at $simparam("gmin") = gmin. */
 void or_lanes_set_model_gmin(or_lanes *L, uint32_t l, double gmin);
 /* gminStamps from lane l's planes at x = 0 (2n words). */

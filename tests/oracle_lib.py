@@ -127,6 +127,7 @@ def load():
         "or_gmin_replay": (C.c_uint32, [C.POINTER(C.c_int), u32p, C.c_uint32, C.c_double, C.c_uint32, f64p,
                                         C.POINTER(C.c_int)]),
         "or_op_ladder": (C.c_int, [P, C.POINTER(NewtonOpts), C.c_double, C.c_uint32, f64p, C.POINTER(Outcome), u32p]),
+        "or_eval_lane_gc": (None, [P, C.c_uint32, f64p, f64p, f64p]),
         "or_dc_sweep_lanes": (C.c_int, [P, C.c_int, C.c_uint32, f64p, C.c_uint32, C.POINTER(NewtonOpts), C.c_double,
                                         C.c_uint32, f64p, u32p]),
         "or_fetlim": (C.c_double, [C.c_double, C.c_double, C.c_double]),
@@ -300,6 +301,20 @@ class Lanes:
         self.lib.free_(C.cast(tp, C.c_void_p))
         self.lib.free_(C.cast(rp, C.c_void_p))
         return t, x, st
+
+    def eval_gc(self, l, x):
+        """Lane l's G and C planes at x, as dense n x n matrices."""
+        n, nnz = self.ckt.n, self.ckt.nnz
+        xa, xp = f64(np.asarray(x, dtype=np.float64))
+        g, gp = f64(np.zeros(nnz))
+        cc, cp = f64(np.zeros(nnz))
+        self.lib.or_eval_lane_gc(self.h, l, xp, gp, cp)
+        G, Cm = np.zeros((n, n)), np.zeros((n, n))
+        for j in range(n):
+            for p in range(self.ckt.col_ptr[j], self.ckt.col_ptr[j + 1]):
+                G[self.ckt.row_idx[p], j] += g[p]
+                Cm[self.ckt.row_idx[p], j] += cc[p]
+        return G, Cm
 
     def gmin_stamps(self, l=0):
         n = self.ckt.n
