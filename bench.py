@@ -373,6 +373,25 @@ def bench_opamp(args):
     res_l, meas_l, kms_l, wms_l, ticks_l, conv_l, wall_l = timed(learned_step, args.steps)
     value = sum(conv_l) / wall_l
 
+    # both arms once more with the operating point solved to reltol 1e-9
+    # (the steps keep 1e-3): the two starts then reach nearly the same
+    # operating point; how many lanes' .meas values then agree to the
+    # north star's 1e-9
+    tight = E.tran_opts(1e-6, 10e-9, op_warm=True, max_steps=200_000, op_reltol=1e-9)
+    lanes.warm_start(mlp, inputs)
+    res_tl, meas_tl, _, _, ms_tl = lanes.tran(tight, passes)
+    lanes.upload_x(x_nom)
+    res_tn, meas_tn, _, _, ms_tn = lanes.tran(tight, passes)
+    ok_t = lane_ok(res_tl, meas_tl) & lane_ok(res_tn, meas_tn)
+    tight_dev = np.abs(meas_tl[ok_t, :, 0] - meas_tn[ok_t, :, 0]) / np.maximum(np.abs(meas_tn[ok_t, :, 0]), 1e-30)
+    tight_op = {"op_reltol": 1e-9, "lanes_both_ok": int(ok_t.sum()),
+                "meas_max_rel_dev_learned_vs_nominal": float(tight_dev.max()) if tight_dev.size else None,
+                "meas_median_rel_dev_learned_vs_nominal": float(np.median(tight_dev)) if tight_dev.size else None,
+                "lanes_meas_within_1e-9": int((tight_dev.max(axis=1) <= 1e-9).sum()) if tight_dev.size else 0,
+                "learned_tran_kernel_ms": ms_tl, "nominal_tran_kernel_ms": ms_tn,
+                "learned_newton_ticks_per_lane": float(res_tl["nr_iters"].sum()) / N,
+                "nominal_newton_ticks_per_lane": float(res_tn["nr_iters"].sum()) / N}
+
     # operating points alone from the network's start
     op_l = [lanes.newton_learned(mlp, inputs, init_fix=False) for _ in range(2)][-1]
     x_l, st_l = op_l[0], op_l[1]
@@ -406,6 +425,7 @@ def bench_opamp(args):
                    "nominal_tran_kernel_ms": float(np.mean(kms_n)),
                    "speedup_vs_nominal": value / nominal_value,
                    "meas_max_rel_dev_learned_vs_nominal": float(meas_dev.max()) if meas_dev.size else None,
+                   "tight_op": tight_op,
                    "op_only": {"learned_newton_ms": op_l[4], "learned_inference_ms": op_l[3],
                                "learned_ticks_per_lane": float(op_l[2].sum()) / N,
                                "nominal_newton_ms": op_n[3], "nominal_ticks_per_lane": float(op_n[2].sum()) / N,

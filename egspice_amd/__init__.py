@@ -73,7 +73,7 @@ class TranOpts(C.Structure):
                                           "chgtol", "trtol")] + \
                [(f, C.c_uint32) for f in ("method", "max_iter", "max_steps", "record_capacity", "op_warm",
                                           "mixed_refine", "early_stop", "pad2")] + \
-               [("abort_after", C.c_double), ("abort_mask", C.c_uint64)]
+               [("abort_after", C.c_double), ("abort_mask", C.c_uint64), ("op_reltol", C.c_double)]
 
 
 class MeasPass(C.Structure):
@@ -210,7 +210,7 @@ def newton_opts(init_fix=True, max_iter=100, reltol=1e-3, abstol=1e-12, vntol=1e
 
 def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, reltol=1e-3, abstol=1e-12, vntol=1e-6,
               chgtol=1e-14, trtol=7.0, max_iter=100, max_steps=1_000_000_000, record_capacity=0, op_warm=False,
-              early_stop=False, abort_after=float("nan"), abort_mask=(1 << 64) - 1, mixed_refine=0):
+              early_stop=False, abort_after=float("nan"), abort_mask=(1 << 64) - 1, mixed_refine=0, op_reltol=0.0):
     """`.tran dt_init t_stop`; a deck's step ceiling is min(dt_init, t_stop/50)
     (src/frontend/analyses.zig:565-566, src/core/query.zig:673-680), the
     tolerances are converger.Options' (src/core/numerics.zig:378-399).
@@ -223,7 +223,7 @@ def tran_opts(t_stop, dt_init, dt_max=None, method=1, xmu=0.5, dt_min=1e-18, rel
         dt_max = min(dt_init, t_stop / 50)
     return TranOpts(t_stop, dt_init, dt_max, dt_min, xmu, reltol, abstol, vntol, chgtol, trtol, method, max_iter,
                     max_steps, record_capacity, 1 if op_warm else 0, mixed_refine, 1 if early_stop else 0, 0,
-                    abort_after, abort_mask)
+                    abort_after, abort_mask, op_reltol)
 
 
 def meas_cross(row, val, rise=-1, fall=-1, cross=-1, td=0.0, frm=0.0, to=0.0):

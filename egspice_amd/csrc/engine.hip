@@ -1807,7 +1807,7 @@ espice_status espice_lanes_tran(espice_lanes *L, const espice_tran_opts *opts, c
     TranOpts to;
     std::memcpy(&to, opts, sizeof to);
     Newton op{};
-    op.reltol = opts->reltol;
+    op.reltol = opts->op_reltol > 0.0 ? opts->op_reltol : opts->reltol;
     op.abstol = opts->abstol;
     op.vntol = opts->vntol;
     op.residual_tol = 1e-9;

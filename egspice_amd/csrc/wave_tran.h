@@ -30,6 +30,7 @@ struct TranOpts {
     uint32_t early_stop, pad2;
     double abort_after;
     uint64_t abort_mask;
+    double op_reltol;  // the operating point's reltol when positive (host side)
 };
 
 // One .meas pass (scan.zig): 0 Cross (WHEN/TRIG/TARG), 1 At (FIND, or

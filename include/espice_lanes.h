@@ -326,6 +326,10 @@ typedef struct {
     uint32_t early_stop, pad2;
     double abort_after;
     uint64_t abort_mask;
+    /* The operating point's reltol at t = 0 when positive (else reltol):
+     * a tighter operating point for comparisons between start points, the
+     * transient's steps keep reltol. */
+    double op_reltol;
 } espice_tran_opts;
 
 /* One .meas pass over an unknown `row` (a node voltage or a branch
