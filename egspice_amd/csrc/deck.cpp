@@ -3,6 +3,7 @@
 #include "sampler.h"
 
 #include <cctype>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -660,7 +661,11 @@ bool read_element(const std::vector<std::string> &t, Element &e, std::string &er
         } else if (w == "ac") {
             c.i++;
             double x;
-            for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) c.i++;
+            e.ac = true;
+            for (int k = 0; k < 2 && c.more() && spice_number(c.peek(), x); k++) {
+                (k == 0 ? e.ac_mag : e.ac_phase) = x;
+                c.i++;
+            }
         } else if ((w == "pulse" || w == "sin" || w == "exp" || w == "am" || (w == "sffm" && e.letter == 'v')) &&
                    (e.letter == 'v' || e.letter == 'i') && e.wave == 0) {
             // positional arguments fill the waveform's slots in order; the
@@ -952,7 +957,17 @@ bool read_deck(const std::string &text, Deck &d, std::string &err)
             m.line = line_no;
             if (!read_meas(t, m, err)) return fail_at(err);
             d.meas.push_back(std::move(m));
-        } else if (w == ".ac" || w == ".dc" || w == ".noise" || w == ".print" || w == ".plot" || w == ".save" ||
+        } else if (w == ".ac") {
+            // .ac dec|oct|lin N fstart fstop
+            double np = 0, f0 = 0, f1 = 0;
+            if (t.size() != 5 || (t[1] != "dec" && t[1] != "oct" && t[1] != "lin") || !spice_number(t[2], np) ||
+                !spice_number(t[3], f0) || !spice_number(t[4], f1) || !(np >= 1) || np > 1e7)
+                return fail_at("BadAc");
+            d.ac_kind = t[1] == "dec" ? 0 : t[1] == "oct" ? 1 : 2;
+            d.ac_points = (uint32_t)std::floor(np + 0.5);
+            d.ac_fstart = f0;
+            d.ac_fstop = f1;
+        } else if (w == ".dc" || w == ".noise" || w == ".print" || w == ".plot" || w == ".save" ||
                    w == ".probe" || w == ".width" || w == ".title") {
             d.ignored.push_back(w);
         } else {
@@ -1071,6 +1086,43 @@ bool pulse_train(const Deck &d, const Element &e, std::vector<std::pair<uint32_t
     }
     if (e.value.given) prm.emplace_back(0, e.value);
     return true;
+}
+
+std::vector<double> ac_axis(const Deck &d)
+{
+    // FreqSweep.count and .at (src/core/numerics.zig:286-327) with the .ac
+    // card's land_on_stop: a dec grid spanning a decade or more stretches
+    // its ratio so the last point is fstop; the others step by the base's
+    // 1/points power from fstart, keeping every point up to fstop plus
+    // ngspice's stop tolerance; a lin grid spaces points evenly.
+    std::vector<double> f;
+    const double f0 = d.ac_fstart, f1 = d.ac_fstop;
+    const uint32_t pts = d.ac_points;
+    if (d.ac_kind < 0) return f;
+    if (d.ac_kind == 2) {
+        const uint32_t n = f1 != f0 ? std::max<uint32_t>(pts, 1) : 1;
+        for (uint32_t k = 0; k < n; k++)
+            f.push_back(n <= 1 ? f0 : f0 + ((double)k / (double)(n - 1)) * (f1 - f0));
+        return f;
+    }
+    const double base = d.ac_kind == 1 ? 2.0 : 10.0;
+    if (!(f0 > 0) || !(f1 >= f0) || pts == 0) return {f0};
+    double stretched = -1.0;
+    if (d.ac_kind == 0 && f1 / 10.0 >= f0) {
+        const double st = std::floor(std::log10(f1 / f0) * (double)pts);
+        if (st >= 1 && std::isfinite(st)) stretched = st;
+    }
+    const double ln_step = std::log(base) / (double)pts;
+    const double ratio = f1 / f0;
+    const double ln_span = std::isfinite(ratio) ? std::log(ratio) : std::log(f1) - std::log(f0);
+    const double steps = stretched >= 0 ? stretched : (ln_span + std::log1p(std::exp(ln_step) * 1e-3)) / ln_step;
+    if (!std::isfinite(steps) || steps < 0) return {f0};
+    const uint32_t n = (uint32_t)std::min(std::floor(steps), 4294967294.0) + 1;
+    for (uint32_t k = 0; k < n; k++) {
+        const double kf = (double)k;
+        f.push_back(stretched >= 0 ? f0 * std::pow(f1 / f0, kf / stretched) : f0 * std::pow(base, kf / (double)pts));
+    }
+    return f;
 }
 
 bool build_deck(BuiltDeck &bd, ::espice_lanes_template *t, std::string &err)

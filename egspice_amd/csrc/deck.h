@@ -70,6 +70,10 @@ struct Element {
     std::vector<Value> pwl;           // PWL times and values, interleaved
     Value pwl_r, pwl_td;              // PWL r= and td=
     std::map<std::string, Value> kv;  // W=, L=, AD=, M=, area
+    // V and I: the card's AC spec (frontend/builder.zig:3124-3151 sourceAc:
+    // AC [mag [phase]], 1 and 0 when left off)
+    bool ac = false;
+    double ac_mag = 1.0, ac_phase = 0.0;
     int line = 0;
 };
 
@@ -100,7 +104,13 @@ struct Deck {
     bool op = false, tran = false;
     Value tstep, tstop, tstart, tmax;
     std::vector<Meas> meas;
-    std::vector<std::string> ignored;  // cards read and not run (.ac, .dc, .options, ...)
+    std::vector<std::string> ignored;  // cards read and not run (.dc, .noise, .print, ...)
+    // `.ac dec|oct|lin N fstart fstop` (core/numerics.zig:263-372
+    // FreqSweep, the .ac card landing a dec grid on fstop): kind 0 dec,
+    // 1 oct, 2 lin; -1 none
+    int ac_kind = -1;
+    uint32_t ac_points = 0;
+    double ac_fstart = 0.0, ac_fstop = 0.0;
     // Monte Carlo: `.option seed=` (the reference's default 1), `.dc
     // monte=N` (N operating-point trials, each warm from the nominal
     // operating point) and `.tran ... sweep monte=N`
@@ -133,6 +143,9 @@ struct BuiltDeck {
     std::map<std::string, uint32_t> vsrc_branch;  // V card name -> branch row
     uint32_t n = 0;
 };
+
+// The `.ac` card's frequencies in Hz (FreqSweep.count/at).
+std::vector<double> ac_axis(const Deck &d);
 
 // Builds `bd.deck` into template `t` (empty, not frozen) and freezes it.
 bool build_deck(BuiltDeck &bd, struct ::espice_lanes_template *t, std::string &err);

@@ -75,6 +75,9 @@ struct Problem {
     std::vector<CardPasses> cards;
     std::vector<double> x_nominal;  // point 0's operating point: the start of `.dc monte` trials
     std::vector<double> x_tran_nominal;  // the template's nominal operating point (warm_start), NaN: none
+    // point 0's `.ac` response at its operating point: the frequencies and
+    // per frequency the real parts of every unknown, then the imaginary
+    std::vector<double> ac_freq, ac_x;
     uint32_t seed_trial = 0;        // the trial every non-Monte-Carlo point draws
     std::string error;
     ~Problem()
@@ -287,7 +290,7 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
     bool have_op = false;
     bool need_op = d.op;
     for (const Meas &m : d.meas) need_op = need_op || m.analysis == "op";
-    if (d.op || d.monte_dc) {
+    if (d.op || d.monte_dc || d.ac_kind >= 0) {
         // a `.dc monte` trial starts from the nominal operating point with
         // no junction seeding (src/analysis/sweep/lanes.zig:73-83,
         // mc.zig:128-131); a batch holds trials only or no trials
@@ -319,6 +322,33 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             pt.x.resize(n);
             for (uint32_t i = 0; i < n; i++) pt.x[i] = xop[(size_t)i * N + l];
             if (b + l == 0) P.x_nominal = pt.x;
+        }
+        // `.ac` at point 0's operating point (src/analysis/ac/ac.zig:40-86):
+        // the lanes' x is the ladder's answer
+        if (d.ac_kind >= 0 && b == 0 && !P.points[0].failed) {
+            std::vector<double> drive(2 * (size_t)n, 0.0);
+            for (size_t k = 0; k < d.elements.size(); k++) {
+                const esp::Element &e = d.elements[k];
+                if (!e.ac || P.bd.elem_kind[k] < 0) continue;
+                if (espice_lanes_template_ac_drive(P.tpl, (uint32_t)P.bd.elem_kind[k], P.bd.elem_inst[k], e.ac_mag,
+                                                   e.ac_phase, drive.data()) != ESPICE_OK)
+                    return fail(&P, ESPICE_FAILED, "AcDrive");
+            }
+            P.ac_freq = esp::ac_axis(d);
+            std::vector<double> omega(P.ac_freq.size()), out(P.ac_freq.size() * 2 * n * N);
+            for (size_t k = 0; k < omega.size(); k++) omega[k] = 2.0 * 3.14159265358979323846 * P.ac_freq[k];
+            double ms;
+            if (espice_lanes_ac(L, omega.data(), (uint32_t)omega.size(), drive.data(), out.data(), &ms) != ESPICE_OK) {
+                size_t req;
+                char buf[128];
+                espice_lanes_error_message(buf, sizeof buf, &req);
+                return fail(&P, ESPICE_FAILED, std::string("Ac: ") + buf);
+            }
+            P.ac_x.assign(omega.size() * 2 * n, 0.0);
+            for (size_t k = 0; k < omega.size(); k++)
+                for (uint32_t part = 0; part < 2; part++)
+                    for (uint32_t i = 0; i < n; i++)
+                        P.ac_x[(k * 2 + part) * n + i] = out[((k * 2 + part) * n + i) * N];
         }
     }
     if (d.tran) {
@@ -870,6 +900,36 @@ std::string problem_print_tables(espice_problem *problem)
     if (!P) return "";
     const esp::Deck &d = P->bd.deck;
     std::string out;
+    if (d.ac_kind >= 0 && !P->ac_freq.empty()) {
+        // the AC plot (src/analysis/ac/ac.zig:56-85): frequency, then every
+        // probe as the builder publishes them, V-source currents in card
+        // order and then every named node in row order, each a real and an
+        // imaginary column (frontend/builder.zig:740-760)
+        const uint32_t n = P->bd.n;
+        std::vector<std::pair<std::string, uint32_t>> probes;
+        for (size_t k = 0; k < d.elements.size(); k++) {
+            const esp::Element &e = d.elements[k];
+            if (e.letter != 'v') continue;
+            auto it = P->bd.vsrc_branch.find(e.name);
+            if (it != P->bd.vsrc_branch.end()) probes.emplace_back("i(" + e.name + ")", it->second);
+        }
+        std::vector<std::pair<uint32_t, std::string>> nodes;
+        for (auto &[name, row] : P->bd.rows) nodes.emplace_back(row, name);
+        std::sort(nodes.begin(), nodes.end());
+        for (auto &[row, name] : nodes) probes.emplace_back("v(" + name + ")", row);
+        out += "AC Analysis: " + d.title + "\nIndex\tfrequency\t(imag)";
+        for (auto &p : probes) out += "\t" + p.first + "\t(imag)";
+        out += "\n-----";
+        for (size_t k = 0; k < 2 * (probes.size() + 1); k++) out += "\t---------------";
+        out += "\n";
+        for (size_t k = 0; k < P->ac_freq.size(); k++) {
+            out += std::to_string(k) + "\t" + shortest_e(P->ac_freq[k]) + "\t" + shortest_e(0.0);
+            for (auto &p : probes)
+                out += "\t" + shortest_e(P->ac_x[(k * 2) * n + p.second]) + "\t" +
+                       shortest_e(P->ac_x[(k * 2 + 1) * n + p.second]);
+            out += "\n";
+        }
+    }
     if (d.monte_dc) {
         std::vector<std::pair<std::string, uint32_t>> probes;
         for (auto &[name, row] : P->bd.rows) probes.emplace_back("v(" + name + ")", row);

@@ -507,3 +507,29 @@ def test_am_sffm_cards_follow_ngspice(engine):
         for nd, want in (("a", a), ("z", z), ("f", f)):
             got = m[0, names.index(f"{nd}{k}")]
             assert abs(got - want) < 2e-3, (nd, t, got, want)
+
+
+def test_rc_lowpass_ac_plot(engine):
+    """rc_lowpass.sp's `.ac dec 10 10 100k` through the points path: the
+    AC plot the reference prints (docs/learn/analyses.md:28, 52-56): 41
+    points, the columns' names, and its first two rows (frequency, i(v1),
+    v(in), v(out), real and imaginary) to a few ulps of each unknown's
+    scale, at point 0's operating point."""
+    say("case: rc_lowpass.sp .ac dec 10 10 100k, points path, AC print plot")
+    pb = E.Problem(decks.RC_LOWPASS_SP)
+    pb.run_all()
+    text = pb.report(tables=True)
+    head, names, rows = parse_print_table(text)
+    assert head == "AC Analysis: RC low-pass filter"
+    assert names == ["Index", "frequency", "(imag)", "i(v1)", "(imag)", "v(in)", "(imag)", "v(out)", "(imag)"]
+    assert rows.shape == (41, 9)
+    assert rows[0, 1] == 10.0 and abs(rows[-1, 1] - 1e5) <= 1e-10
+    want = [(1e1, -9.979542742416352e-8, -9.989267655685869e-6, 1.0, 0.0, 9.999002045725759e-1,
+             -9.98926765568587e-3),
+            (1.2589254117941673e1, -1.5815586205860516e-7, -1.2575008897885423e-5, 1.0, 0.0,
+             9.998418441379414e-1, -1.2575008897885423e-2)]
+    for k, (f, ir, ii, vir, vii, vor, voi) in enumerate(want):
+        assert abs(rows[k, 1] - f) <= 2e-15 * f
+        assert abs(rows[k, 3] - ir) <= 2e-18 and abs(rows[k, 4] - ii) <= 2e-18
+        assert rows[k, 5] == vir and rows[k, 6] == vii
+        assert abs(rows[k, 7] - vor) <= 2e-15 and abs(rows[k, 8] - voi) <= 2e-15
