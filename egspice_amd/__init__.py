@@ -688,16 +688,19 @@ class Problem:
         self._check(self.lib.espice_lane_width(self.h, C.byref(n)))
         return n.value
 
-    def report(self, json=False, tables=False):
+    def report(self, json=False, tables=False, plot_format=None):
         """Point 0's .meas lines as the CLI prints them; json: its measures
-        as JSON; tables: the run's plots in SPICE3 print format (a `.dc
-        monte` deck's Monte Carlo table)."""
-        mode = 2 if tables else 1 if json else 0
+        as JSON; tables: the run's plots in SPICE3 print format (point 0's
+        AC response, a `.dc monte` deck's Monte Carlo table); plot_format
+        "csv", "ascii" or "binary": those plots in that format (bytes for
+        binary)."""
+        mode = {"csv": 3, "ascii": 4, "binary": 5}[plot_format] if plot_format else 2 if tables else 1 if json else 0
         req = C.c_size_t()
         self.lib.espice_cli_report(self.h, mode, None, 0, C.byref(req))
         buf = C.create_string_buffer(req.value)
         self._check(self.lib.espice_cli_report(self.h, mode, buf, req.value, C.byref(req)))
-        return buf.value.decode()
+        raw = buf.raw[:req.value - 1]
+        return raw if plot_format == "binary" else raw.decode()
 
 
 MC_AGAUSS, MC_GAUSS, MC_AUNIF, MC_UNIF, MC_LIMIT = range(5)

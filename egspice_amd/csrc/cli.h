@@ -9,9 +9,11 @@ extern "C" {
 #endif
 
 /* Point 0's .meas lines as the reference prints them (mode 0), its
- * measures as a JSON object (mode 1), or the run's plots in SPICE3 print
- * format (mode 2: a `.dc monte` deck's Monte Carlo table); buffer protocol
- * as espice_copy_*. */
+ * measures as a JSON object (mode 1), or the run's plots (point 0's AC
+ * response, a `.dc monte` deck's Monte Carlo table) as the reference's
+ * writers write them: mode 2 SPICE3 print, 3 CSV, 4 ASCII raw, 5 binary
+ * raw (binary bytes: *required - 1 of them count); buffer protocol as
+ * espice_copy_*. */
 espice_status espice_cli_report(espice_problem *problem, int mode, char *buf, size_t cap, size_t *required);
 /* ngspice's INPevaluate of one number with its scale suffix; 1 when `text`
  * is a number, else 0. */

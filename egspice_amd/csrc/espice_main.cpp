@@ -1,11 +1,14 @@
 // espice: the command line subset that runs a deck on the MI355X lane
 // engine (src/main.zig's batch mode with --backend).
 //
-//   espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] DECK
+//   espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE
+//          --format=print|csv|ascii|binary] DECK
 //
-// --rawfile with --format=print writes the run's plots in SPICE3 print
-// format (src/output/spice_print.zig): a `.dc monte` deck's Monte Carlo
-// table; no other format is written.
+// --rawfile writes the run's plots (point 0's `.ac` response, a `.dc
+// monte` deck's Monte Carlo table) in the reference's formats
+// (src/output/spice_print.zig, csv.zig, ascii_raw.zig, rawfile.zig);
+// binary raw when no format is named. The operating point and the
+// transient waveform are not written as plots.
 //
 // The deck runs through the points API (include/espice.h) as one point: its
 // .op and .tran on GPU lanes, its .meas cards printed as the reference
@@ -21,7 +24,8 @@
 
 static int usage()
 {
-    std::fprintf(stderr, "Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] FILE\n");
+    std::fprintf(stderr, "Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE "
+                         "--format=print|csv|ascii|binary] FILE\n");
     return 2;
 }
 
@@ -31,7 +35,8 @@ static std::string report(espice_problem *p, int json)
     espice_cli_report(p, json, nullptr, 0, &req);
     std::vector<char> b(req ? req : 1);
     if (espice_cli_report(p, json, b.data(), b.size(), &req) != ESPICE_OK) return "";
-    return b.data();
+    // every byte but the terminator: a binary raw file holds NULs
+    return std::string(b.data(), req ? req - 1 : 0);
 }
 
 int main(int argc, char **argv)
@@ -52,7 +57,8 @@ int main(int argc, char **argv)
             return false;
         };
         if (a == "-h" || a == "--help") {
-            std::printf("Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE --format=print] FILE\n");
+            std::printf("Usage: espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE "
+                        "--format=print|csv|ascii|binary] FILE\n");
             return 0;
         }
         if (value("--backend", backend) || value("--measures", measures) || value("--rawfile", rawfile) ||
@@ -66,9 +72,13 @@ int main(int argc, char **argv)
         deck = a;
     }
     if (deck.empty()) return usage();
-    if (!rawfile.empty() && format != "print") {
-        std::fprintf(stderr, "Error: --rawfile is written only with --format=print\n");
-        return 2;
+    int raw_mode = 0;
+    if (!rawfile.empty()) {
+        raw_mode = format == "print" ? 2 : format == "csv" ? 3 : format == "ascii" ? 4 : format == "binary" || format.empty() ? 5 : 0;
+        if (raw_mode == 0) {
+            std::fprintf(stderr, "Error: --format=%s is not written (print, csv, ascii, binary)\n", format.c_str());
+            return 2;
+        }
     }
     espice_create_options o;
     espice_default_options(&o);
@@ -102,13 +112,14 @@ int main(int argc, char **argv)
     }
     std::fputs(report(p, 0).c_str(), stdout);
     if (!rawfile.empty()) {
-        FILE *f = std::fopen(rawfile.c_str(), "w");
+        FILE *f = std::fopen(rawfile.c_str(), "wb");
         if (!f) {
             std::fprintf(stderr, "Error: cannot write %s\n", rawfile.c_str());
             espice_destroy(p);
             return 1;
         }
-        std::fputs(report(p, 2).c_str(), f);
+        const std::string r = report(p, raw_mode);
+        std::fwrite(r.data(), 1, r.size(), f);
         std::fclose(f);
     }
     if (!measures.empty()) {

@@ -890,21 +890,29 @@ static std::string shortest_e(double x)
     return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
 }
 
-// The run's plots in SPICE3 print format (src/output/spice_print.zig:
-// 10-44): for `.dc monte=N` the "Monte Carlo" plot, one row per converged
-// trial: its ordinal, the trial number as `run`, then every node voltage and
-// V-source current, names in order (src/analysis/sweep/mc.zig:148-171).
-std::string problem_print_tables(espice_problem *problem)
+// One plot of the run, as the reference's output writers take it
+// (src/output/types.zig Plot): variable names, real or complex, and the
+// points row by row (a complex variable's real then imaginary part).
+struct Plot {
+    std::string plotname;
+    std::vector<std::string> names;
+    bool complex = false;
+    size_t npoints = 0;
+    std::vector<double> data;
+};
+
+// The run's plots: point 0's AC response (src/analysis/ac/ac.zig:56-85:
+// frequency, then every probe as the builder publishes them, V-source
+// currents in card order and then every named node in row order,
+// frontend/builder.zig:740-760), and for `.dc monte=N` the "Monte Carlo"
+// plot, one row per converged trial: its ordinal, the trial number as
+// `run`, then every node voltage and V-source current, names in order
+// (src/analysis/sweep/mc.zig:148-171).
+std::vector<Plot> problem_plots(Problem *P)
 {
-    Problem *P = get(problem);
-    if (!P) return "";
+    std::vector<Plot> plots;
     const esp::Deck &d = P->bd.deck;
-    std::string out;
     if (d.ac_kind >= 0 && !P->ac_freq.empty()) {
-        // the AC plot (src/analysis/ac/ac.zig:56-85): frequency, then every
-        // probe as the builder publishes them, V-source currents in card
-        // order and then every named node in row order, each a real and an
-        // imaginary column (frontend/builder.zig:740-760)
         const uint32_t n = P->bd.n;
         std::vector<std::pair<std::string, uint32_t>> probes;
         for (size_t k = 0; k < d.elements.size(); k++) {
@@ -917,38 +925,133 @@ std::string problem_print_tables(espice_problem *problem)
         for (auto &[name, row] : P->bd.rows) nodes.emplace_back(row, name);
         std::sort(nodes.begin(), nodes.end());
         for (auto &[row, name] : nodes) probes.emplace_back("v(" + name + ")", row);
-        out += "AC Analysis: " + d.title + "\nIndex\tfrequency\t(imag)";
-        for (auto &p : probes) out += "\t" + p.first + "\t(imag)";
-        out += "\n-----";
-        for (size_t k = 0; k < 2 * (probes.size() + 1); k++) out += "\t---------------";
-        out += "\n";
-        for (size_t k = 0; k < P->ac_freq.size(); k++) {
-            out += std::to_string(k) + "\t" + shortest_e(P->ac_freq[k]) + "\t" + shortest_e(0.0);
-            for (auto &p : probes)
-                out += "\t" + shortest_e(P->ac_x[(k * 2) * n + p.second]) + "\t" +
-                       shortest_e(P->ac_x[(k * 2 + 1) * n + p.second]);
-            out += "\n";
+        Plot p;
+        p.plotname = "AC Analysis";
+        p.complex = true;
+        p.names.push_back("frequency");
+        for (auto &pr : probes) p.names.push_back(pr.first);
+        p.npoints = P->ac_freq.size();
+        for (size_t k = 0; k < p.npoints; k++) {
+            p.data.push_back(P->ac_freq[k]);
+            p.data.push_back(0.0);
+            for (auto &pr : probes) {
+                p.data.push_back(P->ac_x[(k * 2) * n + pr.second]);
+                p.data.push_back(P->ac_x[(k * 2 + 1) * n + pr.second]);
+            }
         }
+        plots.push_back(std::move(p));
     }
     if (d.monte_dc) {
         std::vector<std::pair<std::string, uint32_t>> probes;
         for (auto &[name, row] : P->bd.rows) probes.emplace_back("v(" + name + ")", row);
         for (auto &[name, row] : P->bd.vsrc_branch) probes.emplace_back("i(" + name + ")", row);
         std::sort(probes.begin(), probes.end());
-        out += "Monte Carlo: " + d.title + "\nIndex\trun";
-        for (auto &p : probes) out += "\t" + p.first;
-        out += "\n-----";
-        for (size_t k = 0; k <= probes.size(); k++) out += "\t---------------";
-        out += "\n";
-        size_t k = 0;
+        Plot p;
+        p.plotname = "Monte Carlo";
+        p.names.push_back("run");
+        for (auto &pr : probes) p.names.push_back(pr.first);
         for (const Point &pt : P->points) {
             if (pt.trial == 0 || !pt.ran || pt.failed || pt.x.empty()) continue;
-            out += std::to_string(k++) + "\t" + shortest_e((double)pt.trial);
-            for (auto &p : probes) out += "\t" + shortest_e(pt.x[p.second]);
+            p.data.push_back((double)pt.trial);
+            for (auto &pr : probes) p.data.push_back(pt.x[pr.second]);
+            p.npoints++;
+        }
+        plots.push_back(std::move(p));
+    }
+    return plots;
+}
+
+// ngspice's variable type of a name (src/output/rawfile.zig:10-16).
+static const char *var_type(const std::string &name)
+{
+    if (name == "time") return "time";
+    if (name == "frequency") return "frequency";
+    if (name.rfind("i(", 0) == 0 || name.find("#branch") != std::string::npos) return "current";
+    return "voltage";
+}
+
+// The plots written as one of the reference's formats: 0 SPICE3 print
+// (src/output/spice_print.zig:10-44), 1 CSV (csv.zig: a header row, a
+// complex variable as _re and _im columns, RFC 4180 quoting), 2 ASCII raw
+// (ascii_raw.zig), 3 binary raw (rawfile.zig: the text header with the
+// fixed date, then the samples as native-endian f64).
+std::string encode_plots(const std::vector<Plot> &plots, const std::string &title, int format)
+{
+    std::string out;
+    for (const Plot &p : plots) {
+        const size_t stride = p.names.size() * (p.complex ? 2 : 1);
+        if (format == 0) {
+            out += p.plotname + ": " + title + "\nIndex";
+            for (auto &nm : p.names) out += "\t" + nm + (p.complex ? "\t(imag)" : "");
+            out += "\n-----";
+            for (size_t k = 0; k < stride; k++) out += "\t---------------";
             out += "\n";
+            for (size_t k = 0; k < p.npoints; k++) {
+                out += std::to_string(k);
+                for (size_t v = 0; v < stride; v++) out += "\t" + shortest_e(p.data[k * stride + v]);
+                out += "\n";
+            }
+        } else if (format == 1) {
+            auto field = [&](const std::string &nm, const char *suffix) {
+                if (nm.find_first_of(",\"\r\n") == std::string::npos) {
+                    out += nm + suffix;
+                    return;
+                }
+                out += "\"";
+                for (char ch : nm) {
+                    if (ch == '"') out += "\"";
+                    out += ch;
+                }
+                out += std::string(suffix) + "\"";
+            };
+            for (size_t i = 0; i < p.names.size(); i++) {
+                if (i) out += ",";
+                if (p.complex) {
+                    field(p.names[i], "_re");
+                    out += ",";
+                    field(p.names[i], "_im");
+                } else {
+                    field(p.names[i], "");
+                }
+            }
+            out += "\n";
+            for (size_t k = 0; k < p.npoints; k++) {
+                for (size_t v = 0; v < stride; v++) out += (v ? "," : "") + shortest_e(p.data[k * stride + v]);
+                out += "\n";
+            }
+        } else {
+            out += "Title: " + title + "\nDate: Thu Jan  1 00:00:00 1970\nPlotname: " + p.plotname + "\n";
+            out += p.complex ? "Flags: complex\n" : "Flags: real\n";
+            out += "No. Variables: " + std::to_string(p.names.size()) + "\nNo. Points: " + std::to_string(p.npoints) +
+                   "\nVariables:\n";
+            for (size_t i = 0; i < p.names.size(); i++)
+                out += "\t" + std::to_string(i) + "\t" + p.names[i] + "\t" + var_type(p.names[i]) + "\n";
+            if (format == 3) {
+                out += "Binary:\n";
+                out.append(reinterpret_cast<const char *>(p.data.data()), p.data.size() * sizeof(double));
+            } else {
+                out += "Values:\n";
+                for (size_t k = 0; k < p.npoints; k++) {
+                    out += std::to_string(k);
+                    const double *row = &p.data[k * stride];
+                    if (p.complex) {
+                        for (size_t v = 0; v < stride / 2; v++)
+                            out += "\t" + shortest_e(row[2 * v]) + "," + shortest_e(row[2 * v + 1]) + "\n";
+                    } else {
+                        for (size_t v = 0; v < stride; v++) out += "\t" + shortest_e(row[v]) + "\n";
+                    }
+                }
+            }
         }
     }
     return out;
+}
+
+std::string problem_print_tables(espice_problem *problem, int format = 0)
+{
+    Problem *P = get(problem);
+    if (!P) return "";
+    return encode_plots(problem_plots(P), P->bd.deck.title, format);
 }
 
 // The measures of point 0 as JSON: {"name": value, ...} (null for NaN).
@@ -974,8 +1077,8 @@ extern "C" espice_status espice_cli_report(espice_problem *problem, int mode, ch
 {
     if (!get(problem)) return ESPICE_INVALID_HANDLE;
     const std::string s = mode == 1 ? esp::problem_measures_json(problem)
-                        : mode == 2 ? esp::problem_print_tables(problem)
-                                    : esp::problem_report(problem);
+                        : mode >= 2 && mode <= 5 ? esp::problem_print_tables(problem, mode - 2)
+                                                 : esp::problem_report(problem);
     if (required) *required = s.size() + 1;
     if (cap < s.size() + 1) return ESPICE_BUFFER_TOO_SMALL;
     std::memcpy(buf, s.c_str(), s.size() + 1);

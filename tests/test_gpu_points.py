@@ -533,3 +533,31 @@ def test_rc_lowpass_ac_plot(engine):
         assert abs(rows[k, 3] - ir) <= 2e-18 and abs(rows[k, 4] - ii) <= 2e-18
         assert rows[k, 5] == vir and rows[k, 6] == vii
         assert abs(rows[k, 7] - vor) <= 2e-15 and abs(rows[k, 8] - voi) <= 2e-15
+
+
+def test_rc_lowpass_ac_plot_formats(engine, tmp_path):
+    """The AC plot in the reference's other formats (src/output/csv.zig,
+    ascii_raw.zig, rawfile.zig): their headers exactly as the reference's
+    writers lay them out, and the same numbers as the print plot."""
+    say("case: rc_lowpass.sp AC plot as CSV, ASCII raw and binary raw")
+    pb = E.Problem(decks.RC_LOWPASS_SP)
+    pb.run_all()
+    _, _, rows = parse_print_table(pb.report(tables=True))
+    vals = rows[:, 1:]
+    csv = pb.report(plot_format="csv").splitlines()
+    assert csv[0] == ("frequency_re,frequency_im,i(v1)_re,i(v1)_im,v(in)_re,v(in)_im,v(out)_re,v(out)_im")
+    assert np.array_equal(np.array([[float(v) for v in ln.split(",")] for ln in csv[1:]]), vals)
+    head = ("Title: RC low-pass filter\nDate: Thu Jan  1 00:00:00 1970\nPlotname: AC Analysis\nFlags: complex\n"
+            "No. Variables: 4\nNo. Points: 41\nVariables:\n\t0\tfrequency\tfrequency\n\t1\ti(v1)\tcurrent\n"
+            "\t2\tv(in)\tvoltage\n\t3\tv(out)\tvoltage\n")
+    asc = pb.report(plot_format="ascii")
+    assert asc.startswith(head + "Values:\n")
+    lines = asc[len(head) + len("Values:\n"):].splitlines()
+    assert len(lines) == 41 * 4 and lines[0].startswith("0\t") and lines[4].startswith("1\t")
+    first = [complex(*map(float, ln.split("\t")[-1].split(","))) for ln in lines[:4]]
+    assert np.array_equal(np.array([c.real for c in first]), vals[0, 0::2])
+    b = pb.report(plot_format="binary")
+    hb = (head + "Binary:\n").encode()
+    assert b[:len(hb)] == hb
+    data = np.frombuffer(b[len(hb):], dtype=np.float64)
+    assert data.size == 41 * 8 and np.array_equal(data.reshape(41, 8), vals)
