@@ -4,11 +4,11 @@
 //   espice [--backend=hip|auto] [--measures=FILE] [--rawfile=FILE
 //          --format=print|csv|ascii|binary] DECK
 //
-// --rawfile writes the run's plots (point 0's `.ac` response, a `.dc
-// monte` deck's Monte Carlo table) in the reference's formats
-// (src/output/spice_print.zig, csv.zig, ascii_raw.zig, rawfile.zig);
-// binary raw when no format is named. The operating point and the
-// transient waveform are not written as plots.
+// --rawfile writes the run's plots (point 0's `.ac` response and
+// transient waveform, a `.dc monte` deck's Monte Carlo table) in the
+// reference's formats (src/output/spice_print.zig, csv.zig, ascii_raw.zig,
+// rawfile.zig); binary raw when no format is named. The operating point
+// is not written as a plot.
 //
 // The deck runs through the points API (include/espice.h) as one point: its
 // .op and .tran on GPU lanes, its .meas cards printed as the reference

@@ -27,6 +27,9 @@
 #include "sampler.h"
 
 using esp::BuiltDeck;
+
+// rows of point 0's transient kept for the transient plot
+constexpr uint32_t TRAN_PLOT_ROWS = 100000;
 using esp::Meas;
 
 namespace {
@@ -78,6 +81,9 @@ struct Problem {
     // point 0's `.ac` response at its operating point: the frequencies and
     // per frequency the real parts of every unknown, then the imaginary
     std::vector<double> ac_freq, ac_x;
+    // point 0's transient rows when it ran alone: times, and every unknown
+    // per row
+    std::vector<double> tran_t, tran_x;
     uint32_t seed_trial = 0;        // the trial every non-Monte-Carlo point draws
     std::string error;
     ~Problem()
@@ -370,6 +376,10 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
         o.max_iter = 100;
         o.max_steps = 1000000000u;
         o.record_capacity = need_op && !have_op ? 1 : 0;
+        // point 0 alone in its batch (the command line's run): its waveform
+        // is kept for the transient plot, up to TRAN_PLOT_ROWS rows
+        const bool plot = b == 0 && N == 1;
+        if (plot) o.record_capacity = TRAN_PLOT_ROWS;
         // the batch's points options (one set per batch)
         const Point::Opts &po = P.points[b].opts;
         o.early_stop = po.early_stop ? 1u : 0u;
@@ -412,7 +422,8 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
         }
         std::vector<espice_tran_result> tr(N);
         std::vector<espice_meas_result> mr(P.passes.size() * N + 1);
-        std::vector<double> t0(o.record_capacity ? N : 1), x0(o.record_capacity ? (size_t)N * n : 1);
+        const size_t cap = o.record_capacity;
+        std::vector<double> t0(cap ? N * cap : 1), x0(cap ? (size_t)N * cap * n : 1);
         double ms;
         const espice_status st = espice_lanes_tran(L, &o, P.passes.data(), (uint32_t)P.passes.size(), tr.data(),
                                                    mr.data(), t0.data(), x0.data(), &ms);
@@ -422,11 +433,16 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             espice_lanes_error_message(buf, sizeof buf, &req);
             return fail(&P, st, std::string("Tran: ") + buf);
         }
-        if (o.record_capacity) {
+        if (plot) {
+            const size_t rows = std::min<size_t>(tr[0].points, cap);
+            P.tran_t.assign(t0.begin(), t0.begin() + rows);
+            P.tran_x.assign(x0.begin(), x0.begin() + rows * n);
+        }
+        if (cap && !have_op && need_op) {
             // the transient's operating point: row t = 0, laid out like x
             xop.assign((size_t)n * N, NAN);
             for (uint32_t l = 0; l < N; l++)
-                for (uint32_t i = 0; i < n; i++) xop[(size_t)i * N + l] = x0[(size_t)l * n + i];
+                for (uint32_t i = 0; i < n; i++) xop[(size_t)i * N + l] = x0[(size_t)l * cap * n + i];
             have_op = true;
         }
         for (uint32_t l = 0; l < N; l++) {
@@ -908,23 +924,32 @@ struct Plot {
 // plot, one row per converged trial: its ordinal, the trial number as
 // `run`, then every node voltage and V-source current, names in order
 // (src/analysis/sweep/mc.zig:148-171).
+// The builder's probes (frontend/builder.zig:740-760): V-source branch
+// currents in card order, then every named node in row order.
+static std::vector<std::pair<std::string, uint32_t>> builder_probes(Problem *P)
+{
+    const esp::Deck &d = P->bd.deck;
+    std::vector<std::pair<std::string, uint32_t>> probes;
+    for (size_t k = 0; k < d.elements.size(); k++) {
+        const esp::Element &e = d.elements[k];
+        if (e.letter != 'v') continue;
+        auto it = P->bd.vsrc_branch.find(e.name);
+        if (it != P->bd.vsrc_branch.end()) probes.emplace_back("i(" + e.name + ")", it->second);
+    }
+    std::vector<std::pair<uint32_t, std::string>> nodes;
+    for (auto &[name, row] : P->bd.rows) nodes.emplace_back(row, name);
+    std::sort(nodes.begin(), nodes.end());
+    for (auto &[row, name] : nodes) probes.emplace_back("v(" + name + ")", row);
+    return probes;
+}
+
 std::vector<Plot> problem_plots(Problem *P)
 {
     std::vector<Plot> plots;
     const esp::Deck &d = P->bd.deck;
     if (d.ac_kind >= 0 && !P->ac_freq.empty()) {
         const uint32_t n = P->bd.n;
-        std::vector<std::pair<std::string, uint32_t>> probes;
-        for (size_t k = 0; k < d.elements.size(); k++) {
-            const esp::Element &e = d.elements[k];
-            if (e.letter != 'v') continue;
-            auto it = P->bd.vsrc_branch.find(e.name);
-            if (it != P->bd.vsrc_branch.end()) probes.emplace_back("i(" + e.name + ")", it->second);
-        }
-        std::vector<std::pair<uint32_t, std::string>> nodes;
-        for (auto &[name, row] : P->bd.rows) nodes.emplace_back(row, name);
-        std::sort(nodes.begin(), nodes.end());
-        for (auto &[row, name] : nodes) probes.emplace_back("v(" + name + ")", row);
+        const auto probes = builder_probes(P);
         Plot p;
         p.plotname = "AC Analysis";
         p.complex = true;
@@ -938,6 +963,20 @@ std::vector<Plot> problem_plots(Problem *P)
                 p.data.push_back(P->ac_x[(k * 2) * n + pr.second]);
                 p.data.push_back(P->ac_x[(k * 2 + 1) * n + pr.second]);
             }
+        }
+        plots.push_back(std::move(p));
+    }
+    if (d.tran && !P->tran_t.empty()) {
+        const uint32_t n = P->bd.n;
+        Plot p;
+        p.plotname = "Transient Analysis";
+        p.names.push_back("time");
+        const auto probes = builder_probes(P);
+        for (auto &pr : probes) p.names.push_back(pr.first);
+        p.npoints = P->tran_t.size();
+        for (size_t k = 0; k < p.npoints; k++) {
+            p.data.push_back(P->tran_t[k]);
+            for (auto &pr : probes) p.data.push_back(P->tran_x[k * n + pr.second]);
         }
         plots.push_back(std::move(p));
     }

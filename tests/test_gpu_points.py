@@ -406,9 +406,15 @@ def test_cli_refuses_what_the_lanes_cannot_run(tmp_path, engine):
 
 
 def parse_print_table(text):
+    """The first plot of a print-format text: its title line, column names
+    and rows (up to the next plot's title)."""
     lines = text.splitlines()
     names = lines[1].split("\t")
-    rows = [[float(v) for v in ln.split("\t")] for ln in lines[3:] if ln]
+    rows = []
+    for ln in lines[3:]:
+        if not ln or not ln[0].isdigit():
+            break
+        rows.append([float(v) for v in ln.split("\t")])
     return lines[0], names, np.array(rows)
 
 
@@ -546,18 +552,41 @@ def test_rc_lowpass_ac_plot_formats(engine, tmp_path):
     vals = rows[:, 1:]
     csv = pb.report(plot_format="csv").splitlines()
     assert csv[0] == ("frequency_re,frequency_im,i(v1)_re,i(v1)_im,v(in)_re,v(in)_im,v(out)_re,v(out)_im")
-    assert np.array_equal(np.array([[float(v) for v in ln.split(",")] for ln in csv[1:]]), vals)
+    # the AC plot's 41 rows (the transient plot follows with its own header)
+    assert csv[42] == "time,i(v1),v(in),v(out)"
+    assert np.array_equal(np.array([[float(v) for v in ln.split(",")] for ln in csv[1:42]]), vals)
     head = ("Title: RC low-pass filter\nDate: Thu Jan  1 00:00:00 1970\nPlotname: AC Analysis\nFlags: complex\n"
             "No. Variables: 4\nNo. Points: 41\nVariables:\n\t0\tfrequency\tfrequency\n\t1\ti(v1)\tcurrent\n"
             "\t2\tv(in)\tvoltage\n\t3\tv(out)\tvoltage\n")
     asc = pb.report(plot_format="ascii")
     assert asc.startswith(head + "Values:\n")
     lines = asc[len(head) + len("Values:\n"):].splitlines()
-    assert len(lines) == 41 * 4 and lines[0].startswith("0\t") and lines[4].startswith("1\t")
+    assert lines[41 * 4].startswith("Title: ") and lines[0].startswith("0\t") and lines[4].startswith("1\t")
     first = [complex(*map(float, ln.split("\t")[-1].split(","))) for ln in lines[:4]]
     assert np.array_equal(np.array([c.real for c in first]), vals[0, 0::2])
     b = pb.report(plot_format="binary")
     hb = (head + "Binary:\n").encode()
     assert b[:len(hb)] == hb
-    data = np.frombuffer(b[len(hb):], dtype=np.float64)
-    assert data.size == 41 * 8 and np.array_equal(data.reshape(41, 8), vals)
+    data = np.frombuffer(b[len(hb):len(hb) + 41 * 8 * 8], dtype=np.float64)
+    assert np.array_equal(data.reshape(41, 8), vals)
+    assert b[len(hb) + 41 * 8 * 8:].startswith(b"Title: RC low-pass filter")
+
+
+def test_rc_lowpass_tran_plot(engine):
+    """rc_lowpass.sp's `.tran 10u 5m` as the transient plot of point 0 (the
+    reference's run of it has 519 points, docs/learn/analyses.md:25-30):
+    time from 0 to 5 ms, v(in) the PULSE(0 1 0 1u 1u 5m 10m) source at
+    every accepted time, and the source's current -(v(in) - v(out)) / 1k."""
+    say("case: rc_lowpass.sp transient plot through the points path")
+    pb = E.Problem(decks.RC_LOWPASS_SP)
+    pb.run_all()
+    text = pb.report(tables=True)
+    at = text.index("Transient Analysis: RC low-pass filter")
+    head, names, rows = parse_print_table(text[at:])
+    assert names == ["Index", "time", "i(v1)", "v(in)", "v(out)"]
+    assert rows.shape == (519, 5)
+    t = rows[:, 1]
+    assert t[0] == 0.0 and abs(t[-1] - 5e-3) <= 1e-15 and np.all(np.diff(t) > 0)
+    vin = np.clip(t / 1e-6, 0.0, 1.0)
+    assert np.allclose(rows[:, 3], vin, rtol=0, atol=1e-12)
+    assert np.allclose(rows[:, 2], -(rows[:, 3] - rows[:, 4]) / 1e3, rtol=1e-9, atol=1e-15)
