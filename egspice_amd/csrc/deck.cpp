@@ -967,6 +967,17 @@ bool read_deck(const std::string &text, Deck &d, std::string &err)
             d.ac_points = (uint32_t)std::floor(np + 0.5);
             d.ac_fstart = f0;
             d.ac_fstop = f1;
+        } else if (w == ".dc" && t.size() == 5 && t[1] != "monte" && (t[1][0] == 'v' || t[1][0] == 'i')) {
+            // .dc NAME start stop step, one source
+            double a = 0, b = 0, st = 0;
+            if (!spice_number(t[2], a) || !spice_number(t[3], b) || !spice_number(t[4], st))
+                return fail_at("BadDc");
+            d.dc_src = t[1];
+            d.dc_start = a;
+            d.dc_stop = b;
+            d.dc_step = st;
+        } else if (w == ".dc" && t.size() > 5 && t[1] != "monte") {
+            return fail_at("UnsupportedCard .dc (a nested sweep)");
         } else if (w == ".dc" || w == ".noise" || w == ".print" || w == ".plot" || w == ".save" ||
                    w == ".probe" || w == ".width" || w == ".title") {
             d.ignored.push_back(w);
@@ -1086,6 +1097,26 @@ bool pulse_train(const Deck &d, const Element &e, std::vector<std::pair<uint32_t
     }
     if (e.value.given) prm.emplace_back(0, e.value);
     return true;
+}
+
+// The `.dc` card's swept values (src/analysis/dc/dc.zig:657-665 sweepCount,
+// 383-390): floor((stop - start) / step + 1e-6) + 1 points, one for a zero
+// step or one pointing away from stop, each value the previous plus step
+// (accumulated, as ngspice does).
+std::vector<double> dc_values(const Deck &d)
+{
+    std::vector<double> v;
+    if (d.dc_src.empty()) return v;
+    const double a = d.dc_start, b = d.dc_stop, st = d.dc_step;
+    const double sgn = st > 0 ? 1.0 : st < 0 ? -1.0 : 0.0;
+    size_t count = 1;
+    if (!(st == 0 || (b - a) * sgn < 0)) count = (size_t)std::floor((b - a) / st + 1e-6) + 1;
+    double x = a;
+    for (size_t k = 0; k < count; k++) {
+        if (k) x += st;
+        v.push_back(x);
+    }
+    return v;
 }
 
 std::vector<double> ac_axis(const Deck &d)

@@ -108,6 +108,10 @@ struct Deck {
     // `.ac dec|oct|lin N fstart fstop` (core/numerics.zig:263-372
     // FreqSweep, the .ac card landing a dec grid on fstop): kind 0 dec,
     // 1 oct, 2 lin; -1 none
+    // `.dc NAME start stop step` over one V or I source (dc.zig Options):
+    // dc_src empty when the deck has none
+    std::string dc_src;
+    double dc_start = 0.0, dc_stop = 0.0, dc_step = 0.0;
     int ac_kind = -1;
     uint32_t ac_points = 0;
     double ac_fstart = 0.0, ac_fstop = 0.0;
@@ -143,6 +147,10 @@ struct BuiltDeck {
     std::map<std::string, uint32_t> vsrc_branch;  // V card name -> branch row
     uint32_t n = 0;
 };
+
+// The `.dc` card's swept values (src/analysis/dc/dc.zig:657-665 sweepCount,
+// accumulated by += step as ngspice steps them).
+std::vector<double> dc_values(const Deck &d);
 
 // The `.ac` card's frequencies in Hz (FreqSweep.count/at).
 std::vector<double> ac_axis(const Deck &d);

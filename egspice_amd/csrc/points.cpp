@@ -84,6 +84,10 @@ struct Problem {
     // point 0's transient rows when it ran alone: times, and every unknown
     // per row
     std::vector<double> tran_t, tran_x;
+    // point 0's `.dc` sweep: the swept values and every unknown per value
+    // (NaN where a value did not converge)
+    std::vector<double> dc_v, dc_x;
+    bool dc_is_v = true;
     uint32_t seed_trial = 0;        // the trial every non-Monte-Carlo point draws
     std::string error;
     ~Problem()
@@ -477,6 +481,36 @@ espice_status run_batch(Problem &P, size_t b, size_t e)
             }
         }
     }
+    // `.dc` last, as it moves its source (dc.zig:161-293 runGpuLanes): every
+    // lane's sweep, point 0's kept for the DC plot
+    if (!d.dc_src.empty() && b == 0) {
+        int kind = -1;
+        uint32_t inst = 0;
+        for (size_t k = 0; k < d.elements.size(); k++)
+            if (d.elements[k].name == d.dc_src && P.bd.elem_kind[k] >= 0) {
+                kind = P.bd.elem_kind[k];
+                inst = P.bd.elem_inst[k];
+            }
+        if (kind < 0) return fail(&P, ESPICE_INVALID_ARGUMENT, "DcSweepSourceNotFound " + d.dc_src);
+        const std::vector<double> v = esp::dc_values(d);
+        espice_newton_opts o{1e-3, 1e-12, 1e-6, 1e-9, 0.0, 100, 1u};
+        const espice_ladder_opts lo{1e-2, 100, 0};
+        std::vector<double> xs(v.size() * (size_t)n * N);
+        std::vector<uint32_t> conv(v.size() * (size_t)N);
+        double ms;
+        if (espice_lanes_dc_sweep(L, (uint32_t)kind, inst, v.data(), (uint32_t)v.size(), &o, &lo, xs.data(),
+                                  conv.data(), &ms) != ESPICE_OK) {
+            size_t req;
+            char buf[128];
+            espice_lanes_error_message(buf, sizeof buf, &req);
+            return fail(&P, ESPICE_FAILED, std::string("Dc: ") + buf);
+        }
+        P.dc_v = v;
+        P.dc_x.assign(v.size() * (size_t)n, NAN);
+        for (size_t k = 0; k < v.size(); k++)
+            for (uint32_t i = 0; i < n; i++) P.dc_x[k * n + i] = xs[(k * n + i) * N];
+        P.dc_is_v = d.dc_src[0] == 'v';
+    }
     for (uint32_t l = 0; l < N; l++) {
         Point &pt = P.points[b + l];
         if (pt.failed) {
@@ -563,7 +597,7 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     std::string err;
     if (!esp::read_deck(text, P->bd.deck, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     const esp::Deck &d = P->bd.deck;
-    if (!d.op && !d.tran && !d.monte_dc) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
+    if (!d.op && !d.tran && !d.monte_dc && d.dc_src.empty()) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
     if (espice_lanes_template_create(&P->tpl) != ESPICE_OK) return say(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
     if (!esp::build_deck(P->bd, P->tpl, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     for (size_t k = 0; k < o->live_param_count; k++) {
@@ -963,6 +997,22 @@ std::vector<Plot> problem_plots(Problem *P)
                 p.data.push_back(P->ac_x[(k * 2) * n + pr.second]);
                 p.data.push_back(P->ac_x[(k * 2 + 1) * n + pr.second]);
             }
+        }
+        plots.push_back(std::move(p));
+    }
+    if (!d.dc_src.empty() && !P->dc_v.empty()) {
+        // "DC transfer characteristic": the sweep column named after the
+        // swept quantity (dc.zig:534-549 sweepColumn), then the probes
+        const uint32_t n = P->bd.n;
+        Plot p;
+        p.plotname = "DC transfer characteristic";
+        p.names.push_back(P->dc_is_v ? "v(v-sweep)" : "i(i-sweep)");
+        const auto probes = builder_probes(P);
+        for (auto &pr : probes) p.names.push_back(pr.first);
+        p.npoints = P->dc_v.size();
+        for (size_t k = 0; k < p.npoints; k++) {
+            p.data.push_back(P->dc_v[k]);
+            for (auto &pr : probes) p.data.push_back(P->dc_x[k * n + pr.second]);
         }
         plots.push_back(std::move(p));
     }

@@ -44,6 +44,19 @@ C1 out 0 159n
 """
 RC_LOWPASS_POINTS = 519
 
+# docs/examples/diode_iv.sp, and the first rows of the DC plot the
+# reference prints for it (docs/learn/analyses.md:86-93)
+DIODE_IV_SP = """Diode I-V curve with a DC sweep
+V1 a 0 DC 0
+D1 a 0 dmod
+.model dmod D(IS=1e-14 N=1)
+.dc V1 0 0.8 0.01
+.end
+"""
+DIODE_IV_ROWS = ["0\t0e0\t0e0\t0e0",
+                 "1\t1e-2\t-1.386624089968383e-14\t1e-2",
+                 "2\t2e-2\t-3.166792969038269e-14\t2e-2"]
+
 CMOS_INVERTER_SP = """CMOS inverter
 .param vdd=1.8 wn=1u wp={2*wn}
 VDD vdd 0 {vdd}

@@ -590,3 +590,50 @@ def test_rc_lowpass_tran_plot(engine):
     vin = np.clip(t / 1e-6, 0.0, 1.0)
     assert np.allclose(rows[:, 3], vin, rtol=0, atol=1e-12)
     assert np.allclose(rows[:, 2], -(rows[:, 3] - rows[:, 4]) / 1e3, rtol=1e-9, atol=1e-15)
+
+
+def test_diode_iv_dc_plot(engine):
+    """diode_iv.sp's `.dc V1 0 0.8 0.01` through the points path: the DC
+    transfer characteristic the reference prints (its title, the sweep
+    column v(v-sweep) and the probes, the first three rows), 81 points by
+    the reference's sweep count."""
+    say("case: diode_iv.sp .dc V1 0 0.8 0.01, points path, DC print plot")
+    pb = E.Problem(decks.DIODE_IV_SP)
+    pb.run_all()
+    text = pb.report(tables=True)
+    lines = text.splitlines()
+    assert lines[0] == "DC transfer characteristic: Diode I-V curve with a DC sweep"
+    assert lines[1] == "Index\tv(v-sweep)\ti(v1)\tv(a)"
+    # the sweep and node columns digit for digit; i(v1) to the last bit or
+    # two of the diode current (the same bar as the lanes' .dc test: the
+    # exponential's last bit differs between the two builds' exp)
+    for got, want in zip(lines[3:6], decks.DIODE_IV_ROWS):
+        g, w = got.split("\t"), want.split("\t")
+        assert [g[0], g[1], g[3]] == [w[0], w[1], w[3]]
+        assert abs(float(g[2]) - float(w[2])) <= 4e-16 * abs(float(w[2]))
+    _, _, rows = parse_print_table(text)
+    assert rows.shape == (81, 4)
+    assert abs(rows[-1, 1] - 0.8) <= 1e-15 and np.array_equal(rows[:, 1], rows[:, 3])
+    # the forward current grows by e per thermal voltage: monotone, negative
+    assert np.all(np.diff(rows[:, 2]) < 0)
+
+
+@pytest.mark.parametrize("card,count", [
+    (".dc V1 0.3 0.95 0.005", 131),  # the ratio lands just under 130 in f64
+    (".dc V1 0 1 0.3", 4),           # a step that does not divide the span
+    (".dc V1 5 0 -1", 6),            # descending
+    (".dc V1 0 1 -0.1", 1),          # a step pointing away from stop
+    (".dc I1 0 1m 0.1m", 11),        # a current source: i(i-sweep)
+])
+def test_dc_sweep_counts(engine, card, count):
+    """The reference's sweepCount cases (src/analysis/dc/dc.zig:667-681)
+    as decks: the DC plot's number of rows and its sweep column's name."""
+    say(f"case: sweep count of {card}")
+    src = "I1 0 a 0\n" if card.split()[1] == "I1" else "V1 a 0 0\n"
+    pb = E.Problem(f"sweep count\n{src}R1 a 0 1k\n{card}\n.end\n")
+    pb.run_all()
+    text = pb.report(tables=True)
+    lines = text.splitlines()
+    assert lines[1].split("\t")[1] == ("i(i-sweep)" if "I1" in card else "v(v-sweep)")
+    _, _, rows = parse_print_table(text)
+    assert rows.shape[0] == count
