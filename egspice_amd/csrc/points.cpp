@@ -597,7 +597,8 @@ espice_status espice_create(const espice_create_options *o, espice_problem **out
     std::string err;
     if (!esp::read_deck(text, P->bd.deck, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     const esp::Deck &d = P->bd.deck;
-    if (!d.op && !d.tran && !d.monte_dc && d.dc_src.empty()) return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
+    if (!d.op && !d.tran && !d.monte_dc && d.dc_src.empty() && d.ac_kind < 0)
+        return say(ESPICE_INVALID_ARGUMENT, "NoAnalysis");
     if (espice_lanes_template_create(&P->tpl) != ESPICE_OK) return say(ESPICE_OUT_OF_MEMORY, "OutOfMemory");
     if (!esp::build_deck(P->bd, P->tpl, err)) return say(ESPICE_INVALID_ARGUMENT, err);
     for (size_t k = 0; k < o->live_param_count; k++) {

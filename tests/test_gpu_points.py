@@ -637,3 +637,19 @@ def test_dc_sweep_counts(engine, card, count):
     assert lines[1].split("\t")[1] == ("i(i-sweep)" if "I1" in card else "v(v-sweep)")
     _, _, rows = parse_print_table(text)
     assert rows.shape[0] == count
+
+
+def test_ac_only_deck(engine):
+    """A deck whose one analysis is `.ac` (rc_lowpass.sp without its `.op`
+    and `.tran`): the AC lanes still solve point 0's operating point first,
+    and the AC plot equals the full deck's bit for bit."""
+    say("case: rc_lowpass.sp with .ac only, points path")
+    ac_only = "\n".join(ln for ln in decks.RC_LOWPASS_SP.splitlines() if not ln.startswith((".op", ".tran")))
+    full = E.Problem(decks.RC_LOWPASS_SP)
+    full.run_all()
+    pb = E.Problem(ac_only + "\n")
+    pb.run_all()
+    h0, n0, r0 = parse_print_table(full.report(tables=True))
+    h1, n1, r1 = parse_print_table(pb.report(tables=True))
+    assert (h1, n1) == (h0, n0) and r1.shape == (41, 9)
+    assert np.array_equal(r1, r0)
