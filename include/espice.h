@@ -11,10 +11,12 @@
  * Deck subset (ngspice dialect): V and I (DC, PULSE, SIN, EXP, SFFM, AM,
  * PWL; I's SFFM refused), R, C, D, M (LEVEL=1) elements; .model NMOS/PMOS/D;
  * .param with {expr} and 'expr';
- * .op; .tran tstep tstop [tstart [tmax]]; .meas tran TRIG/TARG, WHEN,
- * FIND/DERIV ... AT=, MIN, MAX, AVG, RMS, INTEG [FROM= TO=], PARAM=; .meas op
- * FIND. Any other card that would change an analysis result fails create
- * with a named error. */
+ * .op; .tran tstep tstop [tstart [tmax]]; .ac dec|oct|lin N fstart fstop
+ * with the sources' AC mag [phase]; .dc NAME start stop step over one V or I
+ * source (a nested sweep refused); .dc monte=N trials; .meas tran TRIG/TARG,
+ * WHEN, FIND/DERIV ... AT=, MIN, MAX, AVG, RMS, INTEG [FROM= TO=], PARAM=;
+ * .meas op FIND. Any other card that would change an analysis result fails
+ * create with a named error. */
 #ifndef ESPICE_H
 #define ESPICE_H
 
