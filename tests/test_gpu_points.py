@@ -653,3 +653,23 @@ def test_ac_only_deck(engine):
     h1, n1, r1 = parse_print_table(pb.report(tables=True))
     assert (h1, n1) == (h0, n0) and r1.shape == (41, 9)
     assert np.array_equal(r1, r0)
+
+
+def test_cli_dc_rawfile_csv(tmp_path, engine):
+    """`espice --rawfile=FILE --format=csv diode_iv.sp`: the DC plot as the
+    reference's CSV writer lays it out (src/output/csv.zig): the sweep
+    column first, then the probes, 81 rows."""
+    say("case: espice --rawfile=FILE --format=csv diode_iv.sp")
+    deck = tmp_path / "diode_iv.sp"
+    deck.write_text(decks.DIODE_IV_SP)
+    raw = tmp_path / "dc.csv"
+    exe = os.path.join(ROOT, "egspice_amd", "espice")
+    r = subprocess.run([exe, f"--rawfile={raw}", "--format=csv", str(deck)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = raw.read_text().splitlines()
+    assert lines[0] == "v(v-sweep),i(v1),v(a)"
+    assert len(lines) == 82
+    rows = np.array([[float(v) for v in ln.split(",")] for ln in lines[1:]])
+    assert rows[0].tolist() == [0.0, 0.0, 0.0] and rows[1, 0] == 0.01 and rows[1, 2] == 0.01
+    assert abs(rows[1, 1] / -1.386624089968383e-14 - 1) <= 4e-16
